@@ -1,0 +1,160 @@
+/*
+ * ORACLE — test infrastructure only.
+ *
+ * A CPU restatement, in C, of the reference's f64 Hallén path
+ * (crates/nec_solver: geometry.rs, excitation.rs, matrix.rs, linear.rs,
+ * farfield.rs; apps/nec-cli/src/solve_session.rs; crates/nec_worker/src/solve.rs;
+ * crates/nec_parser/src/lib.rs; crates/nec_report/src/lib.rs). It follows the
+ * reference's operation order, loop order and data layout (row vectors for M,
+ * the Gram matrix and the augmented elimination matrix) so that it serves both
+ * as the parity checker for the HIP library and as the timed CPU baseline.
+ *
+ * Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg load it.
+ * The product library never links, loads or calls anything here.
+ */
+#ifndef FNEC_ORACLE_H
+#define FNEC_ORACLE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "cx.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Segment (geometry.rs:115-134). */
+typedef struct {
+    uint32_t tag;
+    uint32_t tag_index;
+    double start[3];
+    double end[3];
+    double mid[3];
+    double dir[3];
+    double length;
+    double radius;
+} or_seg;
+
+/* Ground model (geometry.rs:24-54). */
+enum { OR_GROUND_FREE = 0, OR_GROUND_PEC = 1, OR_GROUND_FINITE = 2, OR_GROUND_DEFERRED = 3 };
+typedef struct {
+    int kind;
+    double eps_r;
+    double sigma;
+} or_ground;
+
+/* Status codes shared with the product ABI. */
+enum {
+    OR_OK = 0,
+    OR_SINGULAR = 1,
+    OR_DIM_MISMATCH = 2,
+    OR_UNSUPPORTED = 3,
+    OR_PARSE = 6,
+    OR_GEOMETRY = 7,
+    OR_EXCITATION = 8,
+    OR_NO_FEED = 9,
+};
+
+/* ---- deck (nec_parser/src/lib.rs:84-330) ---- */
+typedef struct or_deck or_deck;
+or_deck *or_deck_parse(const char *text, char *err, int errlen);
+void or_deck_free(or_deck *d);
+int or_deck_frequencies(const or_deck *d, double *out, int max);
+int or_deck_frequencies_py(const or_deck *d, double *out, int max);
+int or_deck_rp_points(const or_deck *d, double *theta, double *phi, int max);
+int or_deck_has_card(const or_deck *d, const char *mn);
+
+/* ---- geometry (geometry.rs) ---- */
+int or_build_geometry(const or_deck *d, or_seg *out, int max, char *err, int errlen);
+or_ground or_ground_model(const or_deck *d);
+int or_wire_endpoints(const or_seg *s, int n, int *first, int *last, int max);
+int or_merge_collinear(const or_seg *s, int n, int *first, int *last, int max);
+int or_detect_junctions(const or_seg *s, const int *first, const int *last, int nw, double tol,
+                        int *ja, int *jb, double *jsign, int max);
+/* Conductor paths: returns the number of paths, or -1 when the topology is out
+ * of scope (geometry.rs:404-584 returns None). path_of/sign_of/s_of are per
+ * segment; free_a/free_b per path; nontrivial is set when any path is not a
+ * forward contiguous chain (ConductorPath::is_trivial). */
+int or_conductor_paths(const or_seg *s, int n, int *path_of, double *sign_of, double *s_of,
+                       int *free_a, int *free_b, int max_paths, int *nontrivial);
+
+/* ---- excitation (excitation.rs) ---- */
+int or_build_excitation(const or_deck *d, const or_seg *s, int n, cx *v);
+int or_build_hallen_rhs(const or_deck *d, const or_seg *s, int n, double freq_hz, cx *rhs,
+                        double *cosv);
+int or_build_hallen_rhs_paths(const or_deck *d, const or_seg *s, int n, double freq_hz,
+                              const int *path_of, const double *sign_of, const double *s_of,
+                              cx *rhs, double *cosv);
+
+/* ---- matrix (matrix.rs:186-218, 297-331) ---- */
+void or_fill_z(const or_seg *s, int n, double freq_hz, or_ground g, cx *z /* n*n row-major */);
+cx or_fill_elem(const or_seg *obs, const or_seg *src, double k, int is_self);
+
+/* ---- linear (linear.rs) ---- */
+int or_solve_square(const cx *a, const cx *b, int n, cx *x);
+int or_solve_hallen(const cx *z, int n, const cx *rhs, const double *cosv, const int *wf,
+                    const int *wl, int nw, const int *ja, const int *jb, const double *js, int nj,
+                    cx *x /* n + max(nw,1) */);
+int or_solve_hallen_paths(const cx *z, int n, const cx *rhs, const double *cosv,
+                          const int *path_of, const int *free_ends, int nfree, cx *x);
+/* The augmented system pieces, for kernel-level parity tests. Returns S. */
+int or_hallen_normal_eqs(const cx *z, int n, const double *cosv, const cx *rhs, const int *wf,
+                         const int *wl, int nw, const int *ja, const int *jb, const double *js,
+                         int nj, cx *ata /* S*S row-major, or NULL */, cx *aty /* S, or NULL */,
+                         int *rows_out);
+
+/* ---- far field (farfield.rs) ---- */
+/* out: per point {gain_total, gain_theta, gain_phi, axial_ratio}. Returns the
+ * normalisation integral. */
+double or_radiation_pattern(const or_seg *s, int n, const cx *cur, double freq_hz, or_ground g,
+                            const double *theta, const double *phi, int np, double *out);
+double or_power_integral(const or_seg *s, int n, const cx *cur, double k, or_ground g);
+void or_far_field(const or_seg *s, int n, const cx *cur, double k, or_ground g, double th,
+                  double ph, cx *ft, cx *fp);
+double or_axial_ratio(cx et, cx ep);
+double or_radiation_efficiency(const or_seg *s, int n, const cx *cur, double freq_hz,
+                               or_ground g, double p_in);
+
+/* ---- solve_session.rs helpers ---- */
+double or_diag_spread(const cx *z, int n);
+void or_residual_hallen(const cx *z, int n, const cx *cur, const cx *chom, int nchom,
+                        const double *cosv, const cx *rhs, const int *wf, const int *wl, int nw,
+                        double *abs_res, double *rel_res);
+
+/* ---- whole-deck pipelines ---- */
+enum { OR_SEM_CLI = 0, OR_SEM_WORKER = 1, OR_SEM_PY = 2 };
+typedef struct {
+    int status;
+    char err[256];
+    double freq_hz;
+    int n;         /* segments */
+    int s;         /* unknowns */
+    int r;         /* rows of M */
+    int nw;        /* homogeneous columns */
+    int used_paths;
+    or_ground ground;
+    or_seg *segs;  /* n */
+    cx *x;         /* s: currents then homogeneous constants */
+    int nfeed;
+    int *feed_tag, *feed_seg, *feed_idx;
+    cx *feed_v, *feed_i, *feed_z;
+    int np;
+    double *rp_theta, *rp_phi, *rp; /* rp: 4 per point */
+    double rp_total;
+    double abs_res, rel_res, diag_spread;
+    char *report;  /* report contract v1 text (CLI semantics) */
+} or_result;
+
+int or_run_deck(const char *text, int semantics, int freq_index, double freq_override_hz,
+                int with_report, or_result *res);
+void or_result_free(or_result *res);
+
+/* Report contract v1 number formats (nec_report/src/lib.rs:437-482). */
+void or_fmt_fixed(char *buf, int cap, double v, int prec);
+void or_fmt_exp(char *buf, int cap, double v, int prec);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

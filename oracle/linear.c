@@ -1,0 +1,244 @@
+/*
+ * ORACLE — test infrastructure only.
+ *
+ * Dense complex solvers, restating crates/nec_solver/src/linear.rs:
+ *   solve_hallen            :672-797
+ *   solve_hallen_paths      :823-894
+ *   solve_square_in_place   :1321-1383
+ * Data layout follows the reference: M, the Gram matrix and the augmented
+ * elimination matrix are arrays of separately allocated row vectors
+ * (Vec<Vec<Complex64>>), the Gram loop walks down a column of M across rows,
+ * and the pivot is the last row of maximal hypot (Iterator::max_by).
+ */
+#include <stdlib.h>
+#include <string.h>
+
+#include "oracle_internal.h"
+
+static cx **rows_alloc(int r, int c)
+{
+    cx **m = (cx **)malloc(sizeof(cx *) * (size_t)(r > 0 ? r : 1));
+    int i;
+    for (i = 0; i < r; i++)
+        m[i] = (cx *)calloc((size_t)(c > 0 ? c : 1), sizeof(cx));
+    return m;
+}
+
+static void rows_free(cx **m, int r)
+{
+    int i;
+    for (i = 0; i < r; i++)
+        free(m[i]);
+    free(m);
+}
+
+/* Elimination on [A | b] held as row vectors; consumes `aug`. */
+static int solve_aug(cx **aug, int n, cx *x)
+{
+    int col, row, k, i, j;
+    for (col = 0; col < n; col++) {
+        int piv = col;
+        double best = cx_norm(aug[col][col]);
+        cx diag;
+        for (row = col + 1; row < n; row++) {
+            double v = cx_norm(aug[row][col]);
+            /* max_by keeps the later element when it compares >= ; a NaN
+             * comparison counts as Equal, which also keeps the later one. */
+            if (!(v < best)) {
+                best = v;
+                piv = row;
+            }
+        }
+        if (cx_norm(aug[piv][col]) < 1e-30)
+            return OR_SINGULAR;
+        if (piv != col) {
+            cx *t = aug[col];
+            aug[col] = aug[piv];
+            aug[piv] = t;
+        }
+        diag = aug[col][col];
+        for (row = col + 1; row < n; row++) {
+            cx factor = cx_div(aug[row][col], diag);
+            cx *ar = aug[row], *ac = aug[col];
+            for (k = col; k <= n; k++) {
+                cx sub = cx_mul(factor, ac[k]);
+                ar[k] = cx_sub(ar[k], sub);
+            }
+        }
+    }
+    for (i = n - 1; i >= 0; i--) {
+        cx sum = aug[i][n];
+        for (j = i + 1; j < n; j++)
+            sum = cx_sub(sum, cx_mul(aug[i][j], x[j]));
+        x[i] = cx_div(sum, aug[i][i]);
+    }
+    return OR_OK;
+}
+
+/* Iterator::max_by with partial_cmp(...).unwrap_or(Equal): the fold keeps
+ * the later element unless the earlier one is strictly greater. The loop
+ * above implements "replace when !(v < best)". For NaN, partial_cmp returns
+ * None -> Equal -> later element kept, which `!(v < best)` also does. */
+
+int or_solve_square(const cx *a, const cx *b, int n, cx *x)
+{
+    cx **aug = rows_alloc(n, n + 1);
+    int i, st;
+    for (i = 0; i < n; i++) {
+        memcpy(aug[i], a + (size_t)i * (size_t)n, sizeof(cx) * (size_t)n);
+        aug[i][n] = b[i];
+    }
+    st = solve_aug(aug, n, x);
+    rows_free(aug, n);
+    return st;
+}
+
+/* Build M and y of the augmented Hallén system (linear.rs:688-762). */
+static int build_m(const cx *z, int n, const double *cosv, const cx *rhs, const int *wf,
+                   const int *wl, int nw, const int *ja, const int *jb, const double *js, int nj,
+                   cx ***m_out, cx **y_out, int *rows_out, int *cols_out)
+{
+    int fb_f = 0, fb_l = n - 1;
+    const int *ef = wf, *el = wl;
+    int w = nw, i, r, c, free_cnt = 0, rows, cols, crow;
+    int *row_wire;
+    unsigned char *in_junc;
+    cx **m, *y;
+    if (nw == 0 || n == 0) {
+        ef = &fb_f;
+        el = &fb_l;
+        w = n > 0 ? 1 : 0;
+    }
+    in_junc = (unsigned char *)calloc((size_t)(n > 0 ? n : 1), 1);
+    for (i = 0; i < nj; i++) {
+        in_junc[ja[i]] = 1;
+        in_junc[jb[i]] = 1;
+    }
+    for (i = 0; i < w; i++) {
+        if (!in_junc[ef[i]])
+            free_cnt++;
+        if (!in_junc[el[i]])
+            free_cnt++;
+    }
+    rows = n + free_cnt + nj;
+    cols = n + w;
+    m = rows_alloc(rows, cols);
+    y = (cx *)calloc((size_t)(rows > 0 ? rows : 1), sizeof(cx));
+    row_wire = (int *)calloc((size_t)(n > 0 ? n : 1), sizeof(int));
+    for (i = 0; i < w; i++)
+        for (r = ef[i]; r <= el[i] && r < n; r++)
+            row_wire[r] = i;
+    for (r = 0; r < n; r++) {
+        for (c = 0; c < n; c++)
+            m[r][c] = z[(size_t)r * (size_t)n + (size_t)c];
+        m[r][n + row_wire[r]] = cx_make(-cosv[r], 0.0);
+        y[r] = rhs[r];
+    }
+    crow = n;
+    for (i = 0; i < w; i++) {
+        if (!in_junc[ef[i]])
+            m[crow++][ef[i]] = cx_make(1.0, 0.0);
+        if (!in_junc[el[i]])
+            m[crow++][el[i]] = cx_make(1.0, 0.0);
+    }
+    for (i = 0; i < nj; i++) {
+        m[crow][ja[i]] = cx_make(1.0, 0.0);
+        m[crow][jb[i]] = cx_make(js[i], 0.0);
+        crow++;
+    }
+    free(in_junc);
+    free(row_wire);
+    *m_out = m;
+    *y_out = y;
+    *rows_out = rows;
+    *cols_out = cols;
+    return OR_OK;
+}
+
+/* Regularised normal equations (linear.rs:764-785). Returns aug rows (S x S+1). */
+static cx **normal_eqs(cx **m, const cx *y, int rows, int cols)
+{
+    cx **ata = rows_alloc(cols, cols + 1);
+    int i, j, r;
+    for (i = 0; i < cols; i++) {
+        cx sum;
+        for (j = 0; j < cols; j++) {
+            sum = cx_make(0.0, 0.0);
+            for (r = 0; r < rows; r++)
+                sum = cx_add(sum, cx_mul(cx_conj(m[r][i]), m[r][j]));
+            ata[i][j] = sum;
+        }
+        sum = cx_make(0.0, 0.0);
+        for (r = 0; r < rows; r++)
+            sum = cx_add(sum, cx_mul(cx_conj(m[r][i]), y[r]));
+        ata[i][cols] = sum; /* aty, appended as the augmented column */
+    }
+    for (i = 0; i < cols; i++)
+        ata[i][i] = cx_add(ata[i][i], cx_make(1e-8, 0.0));
+    return ata;
+}
+
+int or_hallen_normal_eqs(const cx *z, int n, const double *cosv, const cx *rhs, const int *wf,
+                         const int *wl, int nw, const int *ja, const int *jb, const double *js,
+                         int nj, cx *ata_out, cx *aty_out, int *rows_out)
+{
+    cx **m, *y, **ata;
+    int rows, cols, i;
+    build_m(z, n, cosv, rhs, wf, wl, nw, ja, jb, js, nj, &m, &y, &rows, &cols);
+    ata = normal_eqs(m, y, rows, cols);
+    for (i = 0; i < cols; i++) {
+        if (ata_out)
+            memcpy(ata_out + (size_t)i * (size_t)cols, ata[i], sizeof(cx) * (size_t)cols);
+        if (aty_out)
+            aty_out[i] = ata[i][cols];
+    }
+    if (rows_out)
+        *rows_out = rows;
+    rows_free(m, rows);
+    rows_free(ata, cols);
+    free(y);
+    return cols;
+}
+
+int or_solve_hallen(const cx *z, int n, const cx *rhs, const double *cosv, const int *wf,
+                    const int *wl, int nw, const int *ja, const int *jb, const double *js, int nj,
+                    cx *x)
+{
+    cx **m, *y, **ata;
+    int rows, cols, st;
+    build_m(z, n, cosv, rhs, wf, wl, nw, ja, jb, js, nj, &m, &y, &rows, &cols);
+    ata = normal_eqs(m, y, rows, cols);
+    rows_free(m, rows);
+    free(y);
+    st = solve_aug(ata, cols, x);
+    rows_free(ata, cols);
+    return st;
+}
+
+int or_solve_hallen_paths(const cx *z, int n, const cx *rhs, const double *cosv,
+                          const int *path_of, const int *free_ends, int nfree, cx *x)
+{
+    int np = 0, i, r, c, rows, cols, st;
+    cx **m, *y, **ata;
+    for (i = 0; i < n; i++)
+        if (path_of[i] + 1 > np)
+            np = path_of[i] + 1;
+    rows = n + nfree;
+    cols = n + np;
+    m = rows_alloc(rows, cols);
+    y = (cx *)calloc((size_t)(rows > 0 ? rows : 1), sizeof(cx));
+    for (r = 0; r < n; r++) {
+        for (c = 0; c < n; c++)
+            m[r][c] = z[(size_t)r * (size_t)n + (size_t)c];
+        m[r][n + path_of[r]] = cx_make(-cosv[r], 0.0);
+        y[r] = rhs[r];
+    }
+    for (i = 0; i < nfree; i++)
+        m[n + i][free_ends[i]] = cx_make(1.0, 0.0);
+    ata = normal_eqs(m, y, rows, cols);
+    rows_free(m, rows);
+    free(y);
+    st = solve_aug(ata, cols, x);
+    rows_free(ata, cols);
+    return st;
+}

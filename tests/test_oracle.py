@@ -1,0 +1,383 @@
+"""Pins the C oracle (oracle/) before anything is compared against it.
+
+Checks, in order of strength:
+- the reference's own golden corpus (corpus/reference-results.json, copied to
+  tests/golden/corpus/) and the README report excerpt, to the printed digits;
+- the reference's unit-test closed forms (linear.rs:1413-1493,
+  excitation.rs:642-714, farfield.rs:787-974, matrix.rs:458-661);
+- mathematics the reference's tests leave open: an mpmath evaluation of the
+  same Gauss-Legendre fill formula, an mpmath solve of the same regularised
+  normal equations, an independently assembled Gram matrix, and the closed
+  form of the Hertzian-dipole power integral.
+"""
+import json
+import math
+
+import mpmath as mp
+import numpy as np
+import pytest
+
+from conftest import read_golden
+
+CORPUS = json.loads(read_golden("corpus", "reference-results.json"))["cases"]
+
+# Hallén delta-gap cases that exercise the path (no LD/TL/NT, no plane wave,
+# no pulse/sinusoidal solver selection).
+IN_SCOPE = [
+    "dipole-freesp-rp-51seg",
+    "dipole-xaxis-rp-grid-51seg",
+    "dipole-freesp-gm-inplace-shifted",
+    "dipole-ground-51seg",
+    "dipole-ground-rp-51seg",
+    "dipole-gn0-fresnel-51seg",
+    "dipole-gn2-deferred",
+    "dipole-gn2-near-ground-51seg",
+    "yagi-5elm-51seg",
+    "dipole-loaded",
+]
+# Two mirror-symmetric sources: the reference's own two printed values differ
+# by 3e-6 ohm (2e-8 relative), i.e. the deck's rounding sensitivity exceeds
+# the 6 printed decimals, so these are gated at 2e-6 ohm instead of the digit.
+SYMMETRIC_MULTI = ["multi-source", "multi-source-gr-180", "multi-source-gm-copy"]
+
+
+def deck(name):
+    return read_golden("corpus", CORPUS[name]["deck_file"])
+
+
+@pytest.mark.parametrize("case", IN_SCOPE)
+def test_corpus_feedpoint_digits(oracle, case):
+    c = CORPUS[case]
+    r = oracle.run_deck(deck(case), "cli")
+    z = r["feed_z"][0]
+    fz = c["feedpoint_impedance"]
+    assert abs(z.real - fz["real_ohm"]) <= 5.0e-7 + 1e-12
+    assert abs(z.imag - fz["imag_ohm"]) <= 5.0e-7 + 1e-12
+
+
+@pytest.mark.parametrize("case", SYMMETRIC_MULTI)
+def test_corpus_multi_source(oracle, case):
+    fz = CORPUS[case]["feedpoint_impedance"]
+    r = oracle.run_deck(deck(case), "cli")
+    assert len(r["feed_z"]) == 2
+    for z, key in zip(r["feed_z"], ["source_1", "source_2"]):
+        assert abs(z.real - fz[key]["real_ohm"]) <= 2e-6
+        assert abs(z.imag - fz[key]["imag_ohm"]) <= 2e-6
+    a, b = r["feed_z"]
+    assert abs(a - b) / abs(a) < 1e-7
+
+
+def test_corpus_frequency_sweep(oracle):
+    c = CORPUS["frequency-sweep-dipole"]
+    for i, f in enumerate(c["frequencies_mhz"]):
+        r = oracle.run_deck(deck("frequency-sweep-dipole"), "cli", freq_index=i)
+        assert abs(r["freq_hz"] - f * 1e6) < 1e-3
+        z = r["feed_z"][0]
+        ref = c["feedpoint_impedance"][str(f)]
+        assert abs(z.real - ref["real_ohm"]) <= 5.0e-7 + 1e-12
+        assert abs(z.imag - ref["imag_ohm"]) <= 5.0e-7 + 1e-12
+
+
+@pytest.mark.parametrize("case", ["dipole-freesp-51seg", "dipole-ground-51seg"])
+def test_corpus_current_samples(oracle, case):
+    c = CORPUS[case]
+    r = oracle.run_deck(deck(case), "cli")
+    segs = r["segs"]
+    for s in c["current_samples"]:
+        idx = int(np.nonzero((segs["tag"] == s["wire_id"]) & (segs["tag_index"] == s["segment_id"]))[0][0])
+        cur = r["currents"][idx]
+        # amplitude_db holds linear |I| in A (corpus_validation.rs:1682-1720)
+        assert float("%.6e" % abs(cur)) == pytest.approx(s["amplitude_db"], rel=0, abs=0)
+        assert round(math.degrees(math.atan2(cur.imag, cur.real)), 4) == pytest.approx(s["phase_deg"], abs=1e-9)
+
+
+@pytest.mark.parametrize("case", ["dipole-freesp-rp-51seg", "dipole-ground-rp-51seg", "dipole-xaxis-rp-grid-51seg"])
+def test_corpus_pattern_samples(oracle, case):
+    c = CORPUS[case]
+    r = oracle.run_deck(deck(case), "cli")
+    th, ph, rp = r["rp_theta"], r["rp_phi"], r["rp"]
+    for s in c["pattern_samples"]:
+        i = int(np.nonzero((np.abs(th - s["theta_deg"]) < 1e-9) & (np.abs(ph - s["phi_deg"]) < 1e-9))[0][0])
+        for col, key in [(0, "gain_db"), (1, "gain_v_db"), (2, "gain_h_db"), (3, "axial_ratio")]:
+            ref = s[key]
+            # samples are printed with 3 or 4 decimals
+            digits = len(repr(float(ref)).split(".")[1]) if "." in repr(float(ref)) else 0
+            assert abs(rp[i, col] - ref) <= 0.5 * 10.0 ** (-min(digits, 4)) + 1e-9, (key, rp[i, col], ref)
+
+
+def test_readme_report_excerpt(oracle):
+    r = oracle.run_deck(deck("dipole-freesp-rp-51seg"), "cli", report=True)
+    head = read_golden("corpus", "README_report_head.txt")
+    assert r["report"].startswith(head)
+
+
+def test_report_number_formats(oracle):
+    # Rust {:.6e} / {:.6} / {:.4} spellings (nec_report/src/lib.rs:437-482)
+    assert oracle.fmt_exp(0.0130130, 6) == "1.301300e-2"
+    assert oracle.fmt_exp(74.0, 6) == "7.400000e1"
+    assert oracle.fmt_exp(0.0, 6) == "0.000000e0"
+    assert oracle.fmt_exp(-2.5e-7, 6) == "-2.500000e-7"
+    assert oracle.fmt_fixed(-0.0000001, 6) == "-0.000000"
+    assert oracle.fmt_fixed(-999.99, 4) == "-999.9900"
+
+
+# ---- linear.rs closed forms (linear.rs:1413-1493) ----
+
+def test_solve_square_closed_forms(oracle):
+    st, x = oracle.solve_square(np.array([[2.0 + 0j]]), np.array([4.0 + 0j]))
+    assert st == 0 and abs(x[0] - 2.0) < 1e-12
+    st, x = oracle.solve_square(np.array([[3, 1], [1, 2]], complex), np.array([4, 3], complex))
+    assert st == 0 and np.allclose(x, [1, 1], atol=1e-12, rtol=0)
+    st, x = oracle.solve_square(np.array([[1 + 1j, 0], [0, 2]], complex), np.array([1 + 1j, 4], complex))
+    assert st == 0 and np.allclose(x, [1, 2], atol=1e-12, rtol=0)
+    st, _ = oracle.solve_square(np.array([[1, 2], [2, 4]], complex), np.array([1, 2], complex))
+    assert st == 1  # SolveError::Singular
+    z = np.array([[4 + 1j, 1, -1j], [1, 5 + 2j, 1], [-1j, 1, 3 + 1j]])
+    v = np.array([2 + 1j, 3 - 1j, 1])
+    st, x = oracle.solve_square(z, v)
+    assert st == 0 and np.max(np.abs(z @ x - v)) < 1e-10
+
+
+def test_solve_square_pivot_is_last_maximum(oracle):
+    # Rows 1 and 2 tie on |a[.,0]|; max_by keeps the later row, so the second
+    # elimination step sees a different (but exact) arithmetic path. With the
+    # chosen values the answer is exact either way; the tie rule is exercised
+    # by checking the result stays exact.
+    a = np.array([[1, 2, 3], [4, 1, 1], [-4, 2, 1]], complex)
+    b = a @ np.array([1, -1, 2], complex)
+    st, x = oracle.solve_square(a, b)
+    assert st == 0 and np.allclose(x, [1, -1, 2], atol=1e-13, rtol=0)
+
+
+# ---- excitation.rs (excitation.rs:642-714) ----
+
+def test_hallen_rhs_shape_and_scale(oracle):
+    r = oracle.run_deck(deck("dipole-freesp-rp-51seg"), "cli")
+    n = r["n"]
+    # The oracle's pipeline does not export rhs/cos; rebuild them from their
+    # definition and check against the solved system's residual instead.
+    assert r["rel_res"] < 1e-5
+    # 2*pi/eta0 scale (excitation.rs:671-690)
+    eta0 = 4.0 * math.pi * 1e-7 * 299792458.0
+    assert abs(2 * math.pi / eta0 - 0.016678204759907602) < 1e-15
+    assert n == 51
+
+
+# ---- fill: mpmath evaluation of the same quadrature formula ----
+
+GL8 = ([-0.960289856497536, -0.796666477413627, -0.525532409916329, -0.183434642495650,
+        0.183434642495650, 0.525532409916329, 0.796666477413627, 0.960289856497536],
+       [0.101228536290376, 0.222381034453374, 0.313706645877887, 0.362683783378362,
+        0.362683783378362, 0.313706645877887, 0.222381034453374, 0.101228536290376])
+GL4 = ([-0.861136311594953, -0.339981043584856, 0.339981043584856, 0.861136311594953],
+       [0.347854845137454, 0.652145154862626, 0.652145154862626, 0.347854845137454])
+
+
+def mp_elem(obs, src, k, is_self):
+    mp.mp.dps = 40
+    ca = sum(mp.mpf(float(obs["dir"][i])) * mp.mpf(float(src["dir"][i])) for i in range(3))
+    half = mp.mpf(float(src["length"])) / 2
+    a = mp.mpf(float(src["radius"]))
+    k = mp.mpf(k)
+    if is_self:
+        s = mp.mpc(0)
+        for t, w in zip(*GL4):
+            l = mp.mpf(t) * half
+            r = mp.sqrt(l * l + a * a)
+            s += mp.mpf(w) * (mp.exp(-1j * k * r) / r - 1 / r)
+        s *= half
+        re = mp.sqrt(half * half + a * a)
+        val = s + 2 * mp.log((half + re) / a)
+    else:
+        s = mp.mpc(0)
+        for t, w in zip(*GL8):
+            sc = mp.mpf(t) * half
+            p = [mp.mpf(float(obs["mid"][i])) - (mp.mpf(float(src["mid"][i])) + sc * mp.mpf(float(src["dir"][i]))) for i in range(3)]
+            r = mp.sqrt(p[0] ** 2 + p[1] ** 2 + p[2] ** 2 + a * a)
+            s += mp.mpf(w) * mp.exp(-1j * k * r) / r
+        val = s * half
+    return complex(val * ca)
+
+
+def test_fill_matches_mpmath_formula(oracle):
+    segs = oracle.make_segments([((0, 0, -1.0), (0, 0, 1.0), 5, 0.002),
+                                 ((0.3, 0.1, -0.7), (0.4, 0.9, 0.8), 4, 0.001)])
+    f = 30e6
+    k = 2.0 * math.pi * f / 299792458.0
+    z = oracle.fill_z(segs, f)
+    worst = 0.0
+    for i in range(len(segs)):
+        for j in range(len(segs)):
+            ref = mp_elem(segs[i], segs[j], k, i == j)
+            worst = max(worst, abs(z[i, j] - ref) / abs(ref))
+    assert worst < 1e-13, worst
+
+
+def test_fill_pec_image_matches_mpmath(oracle):
+    segs = oracle.make_segments([((-1.0, 0, 2.0), (1.0, 0, 2.0), 4, 0.001)])
+    f = 40e6
+    k = 2.0 * math.pi * f / 299792458.0
+    z = oracle.fill_z(segs, f, oracle.ground("pec"))
+    img = segs.copy()
+    img["mid"][:, 2] *= -1
+    img["dir"][:, 0] *= -1
+    img["dir"][:, 1] *= -1
+    for i in range(len(segs)):
+        for j in range(len(segs)):
+            ref = mp_elem(segs[i], segs[j], k, i == j) + mp_elem(segs[i], img[j], k, False)
+            assert abs(z[i, j] - ref) / abs(ref) < 1e-13
+    # horizontal wire over PEC: the image opposes (cos alpha = -1)
+    zf = oracle.fill_z(segs, f)
+    assert np.all(np.abs(z - zf) > 0)
+
+
+def test_fill_reciprocity_and_self_term(oracle):
+    segs = oracle.make_segments([((0, 0, -5.282), (0, 0, 5.282), 51, 0.001)])
+    z = oracle.fill_z(segs, 14.2e6)
+    # matrix.rs:490-520: reciprocity on equal radii to 1e-6
+    assert np.max(np.abs(z - z.T) / np.abs(z)) < 1e-6
+    d = np.diag(z)
+    assert np.all(np.isfinite(d)) and np.all(d.real > 0)
+    # identical segments give identical self terms
+    assert np.max(np.abs(d - d[0])) == 0.0
+
+
+# ---- normal equations and solve ----
+
+def build_m_numpy(z, cosv, rhs, wf, wl, ja, jb, js):
+    """Independent restatement of linear.rs:688-762 (row order matters only
+    through the sum, which numpy performs in its own order)."""
+    n = z.shape[0]
+    w = len(wf)
+    injunc = set(ja) | set(jb)
+    rows = [list(range(n))]
+    m = np.zeros((n, n + w), complex)
+    m[:, :n] = z
+    for wi in range(w):
+        for r in range(wf[wi], wl[wi] + 1):
+            m[r, n + wi] = -cosv[r]
+    extra = []
+    for wi in range(w):
+        for e in (wf[wi], wl[wi]):
+            if e not in injunc:
+                row = np.zeros(n + w, complex)
+                row[e] = 1
+                extra.append(row)
+    for a, b, s in zip(ja, jb, js):
+        row = np.zeros(n + w, complex)
+        row[a] = 1
+        row[b] = s
+        extra.append(row)
+    M = np.vstack([m] + extra) if extra else m
+    y = np.concatenate([rhs, np.zeros(len(extra), complex)])
+    return M, y
+
+
+def small_system(oracle):
+    r = oracle.run_deck(deck("dipole-freesp-rp-51seg"), "worker")
+    segs = r["segs"]
+    z = oracle.fill_z(segs, r["freq_hz"])
+    n = len(segs)
+    k = 2 * math.pi * r["freq_hz"] / 299792458.0
+    eta0 = 4.0 * math.pi * 1e-7 * 299792458.0
+    s = (segs["mid"] - segs["mid"][25]) @ segs["dir"][25]
+    rhs = np.array([complex(0.0, -(2 * math.pi / eta0) * math.sin(k * abs(v))) for v in s])
+    wm = (segs["mid"][0] + segs["mid"][-1]) / 2.0
+    cosv = np.cos(k * ((segs["mid"] - wm) @ segs["dir"][0]))
+    return r, z, rhs, cosv, n
+
+
+def test_gram_matches_independent_assembly(oracle):
+    r, z, rhs, cosv, n = small_system(oracle)
+    ata, aty, rows = oracle.hallen_normal_eqs(z, rhs, cosv, [0], [n - 1])
+    M, y = build_m_numpy(z, cosv, rhs, [0], [n - 1], [], [], [])
+    assert rows == M.shape[0] == n + 2
+    G = M.conj().T @ M + 1e-8 * np.eye(M.shape[1])
+    assert np.max(np.abs(ata - G)) / np.max(np.abs(G)) < 1e-14
+    assert np.max(np.abs(aty - M.conj().T @ y)) / np.max(np.abs(aty)) < 1e-14
+
+
+def test_solve_hallen_matches_mpmath(oracle):
+    """mpmath solve (50 digits) of the oracle's own regularised normal
+    equations: the oracle's f64 elimination error must be at the
+    kappa(G)*eps level."""
+    r, z, rhs, cosv, n = small_system(oracle)
+    ata, aty, _ = oracle.hallen_normal_eqs(z, rhs, cosv, [0], [n - 1])
+    st, x = oracle.solve_hallen(z, rhs, cosv, [0], [n - 1])
+    assert st == 0
+    mp.mp.dps = 50
+    A = mp.matrix([[mp.mpc(complex(v)) for v in row] for row in ata])
+    b = mp.matrix([mp.mpc(complex(v)) for v in aty])
+    xe = mp.lu_solve(A, b)
+    xe = np.array([complex(v) for v in xe])
+    rel = np.linalg.norm(x - xe) / np.linalg.norm(xe)
+    zin_o = 1.0 / x[25]
+    zin_e = 1.0 / xe[25]
+    assert rel < 1e-8, rel
+    assert abs(zin_o - zin_e) / abs(zin_e) < 1e-9
+
+
+def test_worker_and_cli_agree_on_single_wire(oracle):
+    a = oracle.run_deck(deck("dipole-freesp-rp-51seg"), "cli")
+    b = oracle.run_deck(deck("dipole-freesp-rp-51seg"), "worker")
+    assert np.array_equal(a["x"], b["x"])
+
+
+def test_kcl_junction_row(oracle):
+    """Wire 2 starts bit-identically at wire 1's end: detect_wire_junctions
+    emits one KCL row (sign -1). The oracle's x must equal the least-squares
+    solution of an independently assembled [M; sqrt(lambda) I] system."""
+    txt = ("GW 1 8 0 0 0 0 0 1 0.001\nGW 2 8 0 0 1 0.7 0 1.5 0.001\n"
+           "EX 0 1 4 0 1 0\nFR 0 1 0 0 100 0\nEN\n")
+    r = oracle.run_deck(txt, "worker")
+    n = r["n"]
+    assert r["r"] == n + 2 + 1  # two free ends + one junction row
+    segs = r["segs"]
+    f = r["freq_hz"]
+    k = 2 * math.pi * f / 299792458.0
+    eta0 = 4.0 * math.pi * 1e-7 * 299792458.0
+    z = oracle.fill_z(segs, f)
+    cosv = np.zeros(n)
+    rhs = np.zeros(n, complex)
+    for a, b in [(0, 7), (8, 15)]:
+        wm = (segs["mid"][a] + segs["mid"][b]) / 2.0
+        cosv[a:b + 1] = np.cos(k * ((segs["mid"][a:b + 1] - wm) @ segs["dir"][a]))
+    s = (segs["mid"][0:8] - segs["mid"][3]) @ segs["dir"][3]
+    rhs[0:8] = -1j * (2 * math.pi / eta0) * np.sin(k * np.abs(s))
+    M, y = build_m_numpy(z, cosv, rhs, [0, 8], [7, 15], [7], [8], [-1.0])
+    A = np.vstack([M, math.sqrt(1e-8) * np.eye(M.shape[1])])
+    xe = np.linalg.lstsq(A, np.concatenate([y, np.zeros(M.shape[1])]), rcond=None)[0]
+    assert np.linalg.norm(r["x"] - xe) / np.linalg.norm(xe) < 1e-7
+
+
+# ---- far field (farfield.rs:787-974) and the power integral ----
+
+def hertzian():
+    return np.array([(1, 1, (0, 0, -0.005), (0, 0, 0.005), (0, 0, 0), (0, 0, 1), 0.01, 1e-4)],
+                    dtype=__import__("oracle").SEG_DTYPE), np.array([1.0 + 0j])
+
+
+def test_hertzian_directivity_and_pec(oracle):
+    segs, cur = hertzian()
+    out, total = oracle.radiation_pattern(segs, cur, 14.2e6, [90.0, 0.0], [0.0, 0.0])
+    assert abs(out[0, 0] - 10 * math.log10(1.5)) < 1e-3
+    assert out[1, 0] < -10.0
+    assert out[0, 2] < -10.0 and out[0, 1] > 0.0
+    # closed form: integral of |F|^2 sin(theta) over the sphere = (I L)^2 8 pi / 3
+    assert abs(total - 0.01 ** 2 * 8 * math.pi / 3) / total < 1e-4
+    pec, _ = oracle.radiation_pattern(segs, cur, 14.2e6, [90.0], [0.0], oracle.ground("pec"))
+    assert abs(pec[0, 0] - out[0, 0] - 3.0) < 0.1
+    below, _ = oracle.radiation_pattern(segs, cur, 14.2e6, [120.0], [0.0], oracle.ground("pec"))
+    assert list(below[0]) == [-999.99, -999.99, -999.99, 0.0]
+
+
+def test_axial_ratio_limits(oracle):
+    def ar(a, b):
+        return oracle.axial_ratio(complex(*a), complex(*b))
+
+    assert abs(ar((1, 0), (0, 0))) < 1e-12
+    assert abs(ar((1, 0), (1, 0))) < 1e-12
+    assert abs(abs(ar((1, 0), (0, 1))) - 1.0) < 1e-9
+    assert abs(ar((1, 0), (0, 1)) + ar((1, 0), (0, -1))) < 1e-9
+    e = ar((2, 0), (0, 1))
+    assert 0 < abs(e) < 1
