@@ -1,0 +1,238 @@
+"""ctypes binding of ``libfnec_hip.so`` (the C ABI in ``include/fnec_hip.h``).
+
+This is the Python-side mirror of the reference's ``nec_accel`` entry points
+(crates/nec_accel/src/wgpu_device.rs:461,832,1294): callers pass segment
+geometry, Hallén right-hand sides and wire/junction topology as arrays and get
+currents, impedances and pattern rows back. Every function raises
+``FnecError`` on a non-zero status; ``FnecUnavailable`` when the library or a
+gfx950 device is missing (the reference's ``None`` -> CPU fallback signal).
+There is no CPU fallback inside this module.
+"""
+from __future__ import annotations
+
+import ctypes as C  # noqa: N812
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libfnec_hip.so")
+
+OK, SINGULAR, DIM_MISMATCH, UNAVAILABLE, RESOURCE_EXHAUSTED, INTERNAL, INVALID_ARG = range(7)
+PREC_FP64, PREC_MIXED = 0, 1
+GROUND = {"free": 0, "pec": 1, "finite": 2}
+
+SEG = np.dtype([("mid", np.float64, 3), ("dir", np.float64, 3), ("length", np.float64),
+                ("radius", np.float64)])
+
+EXPORTS = [
+    "fnec_hip_device_count", "fnec_hip_init", "fnec_hip_last_error", "fnec_hip_fill_z",
+    "fnec_hip_solve_hallen", "fnec_hip_solve_hallen_paths", "fnec_hip_rp",
+    "fnec_hip_solve_batch", "fnec_hip_session_create", "fnec_hip_session_submit",
+    "fnec_hip_session_poll", "fnec_hip_session_pending", "fnec_hip_session_destroy",
+    "fnec_hip_request_footprint", "fnec_hip_debug_normal_eqs", "fnec_hip_debug_mfma_probe",
+]
+
+
+class FnecError(RuntimeError):
+    def __init__(self, status: int, msg: str):
+        super().__init__(f"fnec_hip status {status}: {msg}")
+        self.status = status
+
+
+class FnecUnavailable(FnecError):
+    pass
+
+
+class Ground(C.Structure):
+    _fields_ = [("kind", C.c_int), ("eps_r", C.c_double), ("sigma", C.c_double)]
+
+
+class SolveStats(C.Structure):
+    _fields_ = [("abs_res", C.c_double), ("rel_res", C.c_double), ("diag_spread", C.c_double),
+                ("backward_error", C.c_double), ("refine_iters", C.c_int), ("n_swaps", C.c_int),
+                ("rows", C.c_int), ("unknowns", C.c_int), ("t_fill_us", C.c_double),
+                ("t_gram_us", C.c_double), ("t_lu_us", C.c_double), ("t_solve_us", C.c_double),
+                ("t_total_us", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class C64(C.Structure):
+    _fields_ = [("re", C.c_double), ("im", C.c_double)]
+
+
+class Request(C.Structure):
+    _fields_ = [("segs", C.c_void_p), ("n", C.c_size_t), ("rhs", C.c_void_p), ("cos_vec", C.c_void_p),
+                ("wire_first", C.c_void_p), ("wire_last", C.c_void_p), ("n_wires", C.c_size_t),
+                ("j_a", C.c_void_p), ("j_b", C.c_void_p), ("j_sign", C.c_void_p), ("n_junc", C.c_size_t),
+                ("freq_hz", C.c_double), ("ground", Ground), ("feed_index", C.c_size_t),
+                ("v_feed", C64), ("rp_theta", C.c_void_p), ("rp_phi", C.c_void_p),
+                ("rp_npts", C.c_size_t), ("tag", C.c_uint64)]
+
+
+class Result(C.Structure):
+    _fields_ = [("tag", C.c_uint64), ("status", C.c_int), ("z_in", C64), ("feed_current", C64),
+                ("x", C.c_void_p), ("rp", C.c_void_p), ("rp_total", C.c_double),
+                ("t_submit_us", C.c_double), ("t_done_us", C.c_double), ("stats", SolveStats)]
+
+
+def build(force: bool = True) -> str:
+    """Compile libfnec_hip.so for gfx950 with hipcc (no GPU needed)."""
+    args = ["make", "-s", "-C", HERE, "-j4"]
+    if force:
+        args.insert(3, "-B")
+    subprocess.run(args, check=True)
+    return LIB_PATH
+
+
+_lib = None
+
+
+def lib():
+    """Load the HIP library; raises FnecUnavailable if it was never built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise FnecUnavailable(UNAVAILABLE, f"{LIB_PATH} is not built (run __graft_entry__.build())")
+        L = C.CDLL(LIB_PATH)
+        vp, sz, dbl, i32 = C.c_void_p, C.c_size_t, C.c_double, C.c_int
+        L.fnec_hip_last_error.restype = C.c_char_p
+        L.fnec_hip_fill_z.argtypes = [vp, sz, dbl, C.POINTER(Ground), vp]
+        L.fnec_hip_solve_hallen.argtypes = [vp, sz, vp, vp, vp, vp, sz, vp, vp, vp, sz, dbl,
+                                            C.POINTER(Ground), i32, vp, C.POINTER(SolveStats)]
+        L.fnec_hip_solve_hallen_paths.argtypes = [vp, sz, vp, vp, vp, vp, sz, dbl, C.POINTER(Ground),
+                                                  i32, vp, C.POINTER(SolveStats)]
+        L.fnec_hip_rp.argtypes = [vp, vp, sz, dbl, C.POINTER(Ground), vp, vp, sz, vp, C.POINTER(dbl)]
+        L.fnec_hip_solve_batch.argtypes = [vp, sz, i32, vp]
+        L.fnec_hip_session_create.restype = vp
+        L.fnec_hip_session_create.argtypes = [sz, i32, i32]
+        L.fnec_hip_session_submit.argtypes = [vp, C.POINTER(Request)]
+        L.fnec_hip_session_poll.argtypes = [vp, vp, i32, i32]
+        L.fnec_hip_session_pending.argtypes = [vp]
+        L.fnec_hip_session_destroy.argtypes = [vp]
+        L.fnec_hip_request_footprint.restype = sz
+        L.fnec_hip_request_footprint.argtypes = [sz, sz, sz, sz, i32]
+        L.fnec_hip_debug_normal_eqs.argtypes = [vp, sz, vp, vp, vp, vp, sz, vp, vp, vp, sz, dbl,
+                                                C.POINTER(Ground), vp]
+        L.fnec_hip_debug_mfma_probe.argtypes = [vp, vp, vp]
+        _lib = L
+    return _lib
+
+
+def _check(st: int):
+    if st != OK:
+        msg = lib().fnec_hip_last_error()
+        msg = msg.decode(errors="replace") if msg else ""
+        if st == UNAVAILABLE:
+            raise FnecUnavailable(st, msg)
+        raise FnecError(st, msg)
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p) if a is not None and a.size else None
+
+
+def ground(kind: str = "free", eps_r: float = 0.0, sigma: float = 0.0) -> Ground:
+    return Ground(GROUND[kind], float(eps_r), float(sigma))
+
+
+def segments(mid, dirv, length, radius) -> np.ndarray:
+    """Pack per-segment arrays into the fnec_seg layout."""
+    n = len(length)
+    s = np.zeros(n, SEG)
+    s["mid"] = mid
+    s["dir"] = dirv
+    s["length"] = length
+    s["radius"] = radius
+    return s
+
+
+def _sz(a):
+    return np.ascontiguousarray(np.asarray(a, dtype=np.uint64))
+
+
+def init() -> None:
+    _check(lib().fnec_hip_init())
+
+
+def device_count() -> int:
+    return lib().fnec_hip_device_count()
+
+
+def fill_z(segs: np.ndarray, freq_hz: float, g: Ground | None = None) -> np.ndarray:
+    segs = np.ascontiguousarray(segs, SEG)
+    n = len(segs)
+    z = np.zeros((n, n), np.complex128)
+    _check(lib().fnec_hip_fill_z(_p(segs), n, float(freq_hz), C.byref(g or ground()), _p(z)))
+    return z
+
+
+def solve_hallen(segs, rhs, cosv, wire_first=(), wire_last=(), ja=(), jb=(), js=(), freq_hz=14.2e6,
+                 g: Ground | None = None, precision: int = PREC_FP64, with_stats: bool = False):
+    """fnec_hip_solve_hallen: returns x (currents then homogeneous constants)."""
+    segs = np.ascontiguousarray(segs, SEG)
+    n = len(segs)
+    wf, wl, a, b = _sz(wire_first), _sz(wire_last), _sz(ja), _sz(jb)
+    sg = np.ascontiguousarray(np.asarray(js, np.float64))
+    rhs = np.ascontiguousarray(rhs, np.complex128)
+    cosv = np.ascontiguousarray(cosv, np.float64)
+    x = np.zeros(n + max(len(wf), 1), np.complex128)
+    st = SolveStats()
+    _check(lib().fnec_hip_solve_hallen(_p(segs), n, _p(rhs), _p(cosv), _p(wf), _p(wl), len(wf),
+                                       _p(a), _p(b), _p(sg), len(a), float(freq_hz),
+                                       C.byref(g or ground()), int(precision), _p(x), C.byref(st)))
+    return (x, st.as_dict()) if with_stats else x
+
+
+def solve_hallen_paths(segs, rhs, cosv, path_of, free_ends, freq_hz, g: Ground | None = None,
+                       precision: int = PREC_FP64, with_stats: bool = False):
+    segs = np.ascontiguousarray(segs, SEG)
+    n = len(segs)
+    po, fe = _sz(path_of), _sz(free_ends)
+    npaths = int(po.max()) + 1 if n else 0
+    x = np.zeros(n + npaths, np.complex128)
+    st = SolveStats()
+    _check(lib().fnec_hip_solve_hallen_paths(_p(segs), n, _p(np.ascontiguousarray(rhs, np.complex128)),
+                                             _p(np.ascontiguousarray(cosv, np.float64)), _p(po), _p(fe),
+                                             len(fe), float(freq_hz), C.byref(g or ground()),
+                                             int(precision), _p(x), C.byref(st)))
+    return (x, st.as_dict()) if with_stats else x
+
+
+def rp(segs, currents, freq_hz, theta, phi, g: Ground | None = None):
+    """fnec_hip_rp: returns (rows (npts, 4), normalisation integral)."""
+    segs = np.ascontiguousarray(segs, SEG)
+    cur = np.ascontiguousarray(currents, np.complex128)
+    th = np.ascontiguousarray(theta, np.float64)
+    ph = np.ascontiguousarray(phi, np.float64)
+    out = np.zeros((len(th), 4))
+    total = C.c_double(0.0)
+    _check(lib().fnec_hip_rp(_p(segs), _p(cur), len(segs), float(freq_hz), C.byref(g or ground()),
+                             _p(th), _p(ph), len(th), _p(out), C.byref(total)))
+    return out, total.value
+
+
+def debug_normal_eqs(segs, rhs, cosv, wire_first=(), wire_last=(), ja=(), jb=(), js=(),
+                     freq_hz=14.2e6, g: Ground | None = None) -> np.ndarray:
+    segs = np.ascontiguousarray(segs, SEG)
+    n = len(segs)
+    wf, wl, a, b = _sz(wire_first), _sz(wire_last), _sz(ja), _sz(jb)
+    sg = np.ascontiguousarray(np.asarray(js, np.float64))
+    s = n + max(len(wf), 1)
+    aug = np.zeros((s, s + 1), np.complex128)
+    _check(lib().fnec_hip_debug_normal_eqs(_p(segs), n, _p(np.ascontiguousarray(rhs, np.complex128)),
+                                           _p(np.ascontiguousarray(cosv, np.float64)), _p(wf), _p(wl),
+                                           len(wf), _p(a), _p(b), _p(sg), len(a), float(freq_hz),
+                                           C.byref(g or ground()), _p(aug)))
+    return aug
+
+
+def debug_mfma_probe(a16x4: np.ndarray, b4x16: np.ndarray) -> np.ndarray:
+    a = np.ascontiguousarray(a16x4, np.float64)
+    b = np.ascontiguousarray(b4x16, np.float64)
+    out = np.zeros((64, 4))
+    _check(lib().fnec_hip_debug_mfma_probe(_p(a), _p(b), _p(out)))
+    return out

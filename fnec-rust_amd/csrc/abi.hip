@@ -1,0 +1,188 @@
+// C ABI of include/fnec_hip.h on top of the engine.
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <mutex>
+#include <vector>
+
+#include "../../include/fnec_hip.h"
+#include "engine.h"
+
+using namespace fnec;
+
+namespace {
+
+int check_precision(int precision)
+{
+    if (precision != FNEC_PREC_FP64 && precision != FNEC_PREC_MIXED) {
+        set_error("unknown precision");
+        return FNEC_INVALID_ARG;
+    }
+    return FNEC_OK;
+}
+
+int solve_one(const HostProblem &hp, int precision, fnec_c64 *x_out, fnec_solve_stats *stats)
+{
+    std::vector<const HostProblem *> v{&hp};
+    std::vector<fnec_c64 *> xo{x_out};
+    std::vector<fnec_solve_stats> st;
+    std::vector<int> status;
+    int rc = solve_problems(v, xo, st, status, nullptr, precision);
+    if (rc)
+        return rc;
+    if (stats)
+        *stats = st[0];
+    return status[0];
+}
+
+} // namespace
+
+extern "C" {
+
+int fnec_hip_device_count(void)
+{
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess)
+        return 0;
+    int ok = 0;
+    for (int d = 0; d < count; ++d) {
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, d) == hipSuccess &&
+            std::strncmp(prop.gcnArchName, "gfx950", 6) == 0)
+            ++ok;
+    }
+    return ok;
+}
+
+int fnec_hip_init(void)
+{
+    std::lock_guard<std::mutex> lk(global_mutex());
+    return ensure_ready();
+}
+
+const char *fnec_hip_last_error(void) { return last_error(); }
+
+int fnec_hip_fill_z(const fnec_seg *segs, size_t n, double freq_hz, const fnec_ground *ground,
+                    fnec_c64 *z_out)
+{
+    std::lock_guard<std::mutex> lk(global_mutex());
+    int st = ensure_ready();
+    if (st)
+        return st;
+    if (!z_out) {
+        set_error("null output");
+        return FNEC_INVALID_ARG;
+    }
+    std::vector<fnec_c64> rhs(n, fnec_c64{0.0, 0.0});
+    std::vector<double> cosv(n, 0.0);
+    HostProblem hp;
+    std::vector<int> row_col(n, 0);
+    st = build_problem(hp, segs, n, rhs.data(), cosv.data(), row_col, 1, {}, {}, freq_hz, ground);
+    if (st)
+        return st;
+    return fill_only(hp, z_out);
+}
+
+int fnec_hip_solve_hallen(const fnec_seg *segs, size_t n, const fnec_c64 *rhs, const double *cos_vec,
+                          const size_t *wire_first, const size_t *wire_last, size_t n_wires,
+                          const size_t *j_a, const size_t *j_b, const double *j_sign, size_t n_junc,
+                          double freq_hz, const fnec_ground *ground, int precision, fnec_c64 *x_out,
+                          fnec_solve_stats *stats)
+{
+    std::lock_guard<std::mutex> lk(global_mutex());
+    int st = ensure_ready();
+    if (st)
+        return st;
+    if ((st = check_precision(precision)))
+        return st;
+    if (!x_out) {
+        set_error("null output");
+        return FNEC_INVALID_ARG;
+    }
+    HostProblem hp;
+    st = problem_from_hallen(hp, segs, n, rhs, cos_vec, wire_first, wire_last, n_wires, j_a, j_b,
+                             j_sign, n_junc, freq_hz, ground);
+    if (st)
+        return st;
+    return solve_one(hp, precision, x_out, stats);
+}
+
+int fnec_hip_solve_hallen_paths(const fnec_seg *segs, size_t n, const fnec_c64 *rhs,
+                                const double *cos_vec, const size_t *path_of,
+                                const size_t *free_ends, size_t n_free, double freq_hz,
+                                const fnec_ground *ground, int precision, fnec_c64 *x_out,
+                                fnec_solve_stats *stats)
+{
+    std::lock_guard<std::mutex> lk(global_mutex());
+    int st = ensure_ready();
+    if (st)
+        return st;
+    if ((st = check_precision(precision)))
+        return st;
+    if (!x_out) {
+        set_error("null output");
+        return FNEC_INVALID_ARG;
+    }
+    HostProblem hp;
+    st = problem_from_paths(hp, segs, n, rhs, cos_vec, path_of, free_ends, n_free, freq_hz, ground);
+    if (st)
+        return st;
+    return solve_one(hp, precision, x_out, stats);
+}
+
+int fnec_hip_rp(const fnec_seg *segs, const fnec_c64 *currents, size_t n, double freq_hz,
+                const fnec_ground *ground, const double *theta_deg, const double *phi_deg,
+                size_t npts, fnec_rp_row *out, double *total_out)
+{
+    std::lock_guard<std::mutex> lk(global_mutex());
+    int st = ensure_ready();
+    if (st)
+        return st;
+    if (npts && !out) {
+        set_error("null output");
+        return FNEC_INVALID_ARG;
+    }
+    std::vector<RpHost> jobs(1);
+    st = rp_host_from(jobs[0], segs, currents, n, freq_hz, ground, theta_deg, phi_deg, npts);
+    if (st)
+        return st;
+    std::vector<fnec_rp_row *> outs{out};
+    std::vector<double> totals;
+    st = rp_run(jobs, outs, totals);
+    if (st)
+        return st;
+    if (total_out)
+        *total_out = totals[0];
+    return FNEC_OK;
+}
+
+int fnec_hip_debug_normal_eqs(const fnec_seg *segs, size_t n, const fnec_c64 *rhs,
+                              const double *cos_vec, const size_t *wire_first,
+                              const size_t *wire_last, size_t n_wires, const size_t *j_a,
+                              const size_t *j_b, const double *j_sign, size_t n_junc,
+                              double freq_hz, const fnec_ground *ground, fnec_c64 *aug_out)
+{
+    std::lock_guard<std::mutex> lk(global_mutex());
+    int st = ensure_ready();
+    if (st)
+        return st;
+    HostProblem hp;
+    st = problem_from_hallen(hp, segs, n, rhs, cos_vec, wire_first, wire_last, n_wires, j_a, j_b,
+                             j_sign, n_junc, freq_hz, ground);
+    if (st)
+        return st;
+    return normal_eqs_only(hp, aug_out);
+}
+
+int fnec_hip_debug_mfma_probe(const double *a16x4, const double *b4x16, double *lanes_out)
+{
+    std::lock_guard<std::mutex> lk(global_mutex());
+    int st = ensure_ready();
+    if (st)
+        return st;
+    return mfma_probe(a16x4, b4x16, lanes_out);
+}
+
+} // extern "C"

@@ -1,0 +1,325 @@
+// Batched solves and the serving session (nec_worker serving mode).
+//
+// fnec_hip_solve_batch runs many independent Hallén problems (frequency-sweep
+// points, geometries of uneven N) through one set of batched launches: every
+// kernel walks a work list of (problem, tile) items, so problems of different
+// sizes share each launch. Far-field grids of the batch run afterwards in one
+// batched RP launch that reads the currents in place on the device.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <mutex>
+#include <thread>
+#include <vector>
+
+#include "../../include/fnec_hip.h"
+#include "engine.h"
+
+using namespace fnec;
+
+namespace {
+
+double now_us()
+{
+    using namespace std::chrono;
+    return (double)duration_cast<nanoseconds>(steady_clock::now().time_since_epoch()).count() * 1e-3;
+}
+
+// Owned copy of a request (the session keeps it past submit).
+struct OwnedRequest {
+    std::vector<fnec_seg> segs;
+    std::vector<fnec_c64> rhs;
+    std::vector<double> cosv;
+    std::vector<size_t> wf, wl, ja, jb;
+    std::vector<double> js, theta, phi;
+    fnec_request req;
+    double t_submit = 0.0;
+    size_t footprint = 0;
+
+    void own(const fnec_request &r)
+    {
+        segs.assign(r.segs, r.segs + r.n);
+        rhs.assign(r.rhs, r.rhs + r.n);
+        cosv.assign(r.cos_vec, r.cos_vec + r.n);
+        if (r.n_wires) {
+            wf.assign(r.wire_first, r.wire_first + r.n_wires);
+            wl.assign(r.wire_last, r.wire_last + r.n_wires);
+        }
+        if (r.n_junc) {
+            ja.assign(r.j_a, r.j_a + r.n_junc);
+            jb.assign(r.j_b, r.j_b + r.n_junc);
+            js.assign(r.j_sign, r.j_sign + r.n_junc);
+        }
+        if (r.rp_npts && r.rp_theta && r.rp_phi) {
+            theta.assign(r.rp_theta, r.rp_theta + r.rp_npts);
+            phi.assign(r.rp_phi, r.rp_phi + r.rp_npts);
+        }
+        req = r;
+        req.segs = segs.data();
+        req.rhs = rhs.data();
+        req.cos_vec = cosv.data();
+        req.wire_first = wf.empty() ? nullptr : wf.data();
+        req.wire_last = wl.empty() ? nullptr : wl.data();
+        req.j_a = ja.empty() ? nullptr : ja.data();
+        req.j_b = jb.empty() ? nullptr : jb.data();
+        req.j_sign = js.empty() ? nullptr : js.data();
+        req.rp_theta = theta.empty() ? nullptr : theta.data();
+        req.rp_phi = phi.empty() ? nullptr : phi.data();
+        req.rp_npts = theta.size();
+    }
+};
+
+// Run one batch of requests; fills results (x / rp copied when the caller
+// gave buffers, otherwise kept internally in `xs` / `rps`).
+int run_batch(const fnec_request *const *reqs, size_t count, int precision, fnec_result *results,
+              std::vector<std::vector<fnec_c64>> *xs, std::vector<std::vector<fnec_rp_row>> *rps)
+{
+    std::vector<HostProblem> hps(count);
+    std::vector<int> pre(count, FNEC_OK);
+    for (size_t i = 0; i < count; ++i) {
+        const fnec_request &r = *reqs[i];
+        results[i].tag = r.tag;
+        results[i].status = FNEC_OK;
+        pre[i] = problem_from_hallen(hps[i], r.segs, r.n, r.rhs, r.cos_vec, r.wire_first, r.wire_last,
+                                     r.n_wires, r.j_a, r.j_b, r.j_sign, r.n_junc, r.freq_hz, &r.ground);
+        if (!pre[i] && r.feed_index >= r.n) {
+            set_error("feed index outside the segment list");
+            pre[i] = FNEC_INVALID_ARG;
+        }
+    }
+    std::vector<const HostProblem *> ok;
+    std::vector<size_t> idx;
+    for (size_t i = 0; i < count; ++i) {
+        if (pre[i])
+            results[i].status = pre[i];
+        else {
+            ok.push_back(&hps[i]);
+            idx.push_back(i);
+        }
+    }
+    if (ok.empty())
+        return FNEC_OK;
+    std::vector<std::vector<fnec_c64>> xloc(ok.size());
+    std::vector<fnec_c64 *> xo(ok.size());
+    for (size_t q = 0; q < ok.size(); ++q) {
+        xloc[q].resize(ok[q]->s);
+        xo[q] = xloc[q].data();
+    }
+    std::vector<fnec_solve_stats> stats;
+    std::vector<int> status;
+    std::vector<double *> dev_x;
+    int st = solve_problems(ok, xo, stats, status, &dev_x, precision);
+    if (st)
+        return st;
+    // far-field jobs for solved requests that asked for a grid
+    std::vector<RpHost> jobs;
+    std::vector<size_t> job_of;
+    std::vector<std::vector<fnec_rp_row>> rpbuf;
+    for (size_t q = 0; q < ok.size(); ++q) {
+        const size_t i = idx[q];
+        const fnec_request &r = *reqs[i];
+        fnec_result &res = results[i];
+        res.status = status[q];
+        res.stats = stats[q];
+        if (status[q])
+            continue;
+        const fnec_c64 cur = xloc[q][r.feed_index];
+        res.feed_current = cur;
+        // Z = V / I when |I| > 1e-60, else V (nec_worker/src/solve.rs:254-260)
+        if (std::hypot(cur.re, cur.im) > 1e-60) {
+            const double ns = cur.re * cur.re + cur.im * cur.im;
+            res.z_in.re = (r.v_feed.re * cur.re + r.v_feed.im * cur.im) / ns;
+            res.z_in.im = (r.v_feed.im * cur.re - r.v_feed.re * cur.im) / ns;
+        } else {
+            res.z_in = r.v_feed;
+        }
+        if (res.x)
+            std::memcpy(res.x, xloc[q].data(), sizeof(fnec_c64) * xloc[q].size());
+        if (xs)
+            (*xs)[i] = xloc[q];
+        if (r.rp_npts && r.rp_theta && r.rp_phi) {
+            RpHost h;
+            h.cur_dev = dev_x[q];
+            int e = rp_host_from(h, r.segs, nullptr, r.n, r.freq_hz, &r.ground, r.rp_theta, r.rp_phi,
+                                 r.rp_npts);
+            if (e) {
+                res.status = e;
+                continue;
+            }
+            jobs.push_back(std::move(h));
+            job_of.push_back(i);
+        }
+    }
+    if (!jobs.empty()) {
+        rpbuf.resize(jobs.size());
+        std::vector<fnec_rp_row *> outs(jobs.size());
+        for (size_t j = 0; j < jobs.size(); ++j) {
+            rpbuf[j].resize(jobs[j].theta.size());
+            outs[j] = rpbuf[j].data();
+        }
+        std::vector<double> totals;
+        st = rp_run(jobs, outs, totals);
+        if (st)
+            return st;
+        for (size_t j = 0; j < jobs.size(); ++j) {
+            fnec_result &res = results[job_of[j]];
+            res.rp_total = totals[j];
+            if (res.rp)
+                std::memcpy(res.rp, rpbuf[j].data(), sizeof(fnec_rp_row) * rpbuf[j].size());
+            if (rps)
+                (*rps)[job_of[j]] = std::move(rpbuf[j]);
+        }
+    }
+    return FNEC_OK;
+}
+
+} // namespace
+
+struct fnec_session {
+    size_t budget = 0;
+    int precision = FNEC_PREC_FP64;
+    int max_batch = 64;
+    std::mutex mu;
+    std::deque<OwnedRequest *> queue;
+    std::deque<fnec_result> done;
+};
+
+extern "C" {
+
+size_t fnec_hip_request_footprint(size_t n, size_t n_wires, size_t n_junc, size_t rp_npts, int precision)
+{
+    (void)precision;
+    const size_t w = n_wires ? n_wires : 1;
+    const size_t s = n + w;
+    const size_t ncons = 2 * w + n_junc;
+    size_t b = problem_bytes((int)n, (int)s, (int)ncons, (int)w + 1);
+    b += rp_npts * (8 + 8 + 32 + 32) + 8 * ((181 * 360 + 255) / 256) + 1024;
+    return b;
+}
+
+int fnec_hip_solve_batch(const fnec_request *reqs, size_t count, int precision, fnec_result *results)
+{
+    std::lock_guard<std::mutex> lk(global_mutex());
+    int st = ensure_ready();
+    if (st)
+        return st;
+    if (count && (!reqs || !results)) {
+        set_error("null request or result array");
+        return FNEC_INVALID_ARG;
+    }
+    std::vector<const fnec_request *> rp(count);
+    for (size_t i = 0; i < count; ++i)
+        rp[i] = &reqs[i];
+    return run_batch(rp.data(), count, precision, results, nullptr, nullptr);
+}
+
+fnec_session *fnec_hip_session_create(size_t budget_bytes, int precision, int max_batch)
+{
+    if (precision != FNEC_PREC_FP64 && precision != FNEC_PREC_MIXED) {
+        set_error("unknown precision");
+        return nullptr;
+    }
+    fnec_session *s = new fnec_session;
+    s->budget = budget_bytes;
+    s->precision = precision;
+    s->max_batch = max_batch > 0 ? max_batch : 64;
+    return s;
+}
+
+int fnec_hip_session_submit(fnec_session *s, const fnec_request *req)
+{
+    if (!s || !req || !req->segs || !req->rhs || !req->cos_vec || req->n == 0) {
+        set_error("invalid request");
+        return FNEC_INVALID_ARG;
+    }
+    const size_t fp = fnec_hip_request_footprint(req->n, req->n_wires, req->n_junc, req->rp_npts,
+                                                 s->precision);
+    if (fp > s->budget) {
+        set_error("request footprint exceeds the session budget");
+        return FNEC_RESOURCE_EXHAUSTED;
+    }
+    OwnedRequest *o = new OwnedRequest;
+    o->own(*req);
+    o->t_submit = now_us();
+    o->footprint = fp;
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->queue.push_back(o);
+    return FNEC_OK;
+}
+
+int fnec_hip_session_pending(fnec_session *s)
+{
+    if (!s)
+        return 0;
+    std::lock_guard<std::mutex> lk(s->mu);
+    return (int)(s->queue.size() + s->done.size());
+}
+
+int fnec_hip_session_poll(fnec_session *s, fnec_result *out, int max, int wait_us)
+{
+    (void)wait_us;
+    if (!s || (max > 0 && !out)) {
+        set_error("invalid poll");
+        return -FNEC_INVALID_ARG;
+    }
+    std::vector<OwnedRequest *> batch;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        size_t used = 0;
+        while (!s->queue.empty() && (int)batch.size() < s->max_batch) {
+            OwnedRequest *o = s->queue.front();
+            if (!batch.empty() && used + o->footprint > s->budget)
+                break;
+            used += o->footprint;
+            batch.push_back(o);
+            s->queue.pop_front();
+        }
+    }
+    if (!batch.empty()) {
+        std::vector<const fnec_request *> rp(batch.size());
+        std::vector<fnec_result> res(batch.size());
+        std::memset(res.data(), 0, sizeof(fnec_result) * res.size());
+        for (size_t i = 0; i < batch.size(); ++i)
+            rp[i] = &batch[i]->req;
+        int st;
+        {
+            std::lock_guard<std::mutex> lk(global_mutex());
+            st = ensure_ready();
+            if (!st)
+                st = run_batch(rp.data(), rp.size(), s->precision, res.data(), nullptr, nullptr);
+        }
+        const double t_done = now_us();
+        std::lock_guard<std::mutex> lk(s->mu);
+        for (size_t i = 0; i < batch.size(); ++i) {
+            if (st)
+                res[i].status = st;
+            res[i].t_submit_us = batch[i]->t_submit;
+            res[i].t_done_us = t_done;
+            s->done.push_back(res[i]);
+            delete batch[i];
+        }
+    }
+    int k = 0;
+    std::lock_guard<std::mutex> lk(s->mu);
+    while (k < max && !s->done.empty()) {
+        out[k++] = s->done.front();
+        s->done.pop_front();
+    }
+    return k;
+}
+
+void fnec_hip_session_destroy(fnec_session *s)
+{
+    if (!s)
+        return;
+    for (auto *o : s->queue)
+        delete o;
+    delete s;
+}
+
+} // extern "C"

@@ -1,0 +1,74 @@
+// Shared host/device definitions for the fnec_hip engine.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace fnec {
+
+constexpr double kPi = 3.14159265358979323846;
+constexpr double kC0 = 299792458.0;
+constexpr double kMu0 = 4.0 * kPi * 1e-7;
+constexpr double kEps0Matrix = 8.854187817e-12;   // matrix.rs:33
+constexpr double kEps0FarField = 8.8541878128e-12; // farfield.rs:293
+constexpr double kLambda = 1e-8;                   // linear.rs:782
+constexpr double kSingular = 1e-30;                // linear.rs:1354
+
+constexpr int kTile = 64;      // Gram / trailing-update output tile (complex elements)
+constexpr int kGramK = 16;     // Gram K chunk
+constexpr int kPanel = 32;     // LU panel width
+constexpr int kSegStride = 8;  // doubles per segment: mid xyz, dir xyz, length, radius
+
+// One Hallén problem inside a batch. All pointers are device pointers into
+// the batch arena. Matrices are column-major with split real / imaginary
+// planes so that MFMA operands load as contiguous doubles.
+struct Prob {
+    int n;      // segments
+    int s;      // unknowns (n + homogeneous columns)
+    int r;      // rows of the augmented system (n + constraint rows)
+    int w;      // homogeneous columns
+    int ncons;  // constraint rows (free ends + junctions)
+    int nruns;  // runs of equal homogeneous column along the rows
+    int ldz;    // leading dimension of Z
+    int lda;    // leading dimension of the augmented normal matrix
+    int ground; // 0 free, 1 PEC, 2 scalar-Gamma image
+    int pad0;
+    double k;        // fill wavenumber 2*pi*f/c0 (matrix.rs:194)
+    double gam_re;   // scalar reflection coefficient for ground == 2
+    double gam_im;
+    const double *seg;     // n * kSegStride
+    double *zr, *zi;       // Z, ldz x n
+    double *ar, *ai;       // [G | M^H y], lda x (s + 1)
+    const double *cosv;    // n
+    const double *rhs;     // n complex, interleaved
+    const int *runs;       // nruns * 3: row_begin, row_end (exclusive), column
+    const int *cons;       // ncons * 2: a, b (b < 0: single-entry row)
+    const double *cons_sign; // ncons
+    int *ipiv;             // s
+    int *info;             // [0] status (0 ok, 1 singular), [1] swaps
+    double *x;             // s complex, interleaved
+    double *stats;         // [0] abs_res, [1] rel_res, [2] diag_spread
+    double *scratch;       // residual partial sums
+};
+
+// Far-field problem (one RP grid + normalisation integral).
+struct RpProb {
+    int n;
+    int npts;
+    int ground;       // 0 free, 1 PEC, 2 Fresnel
+    int ndir;         // normalisation directions (181*360 or 91*360)
+    double k;         // 2*pi/(c0/f) (farfield.rs:570-571)
+    double eps_r, sigma;
+    const double *seg;     // n * kSegStride
+    const double *cur;     // n complex interleaved
+    const double *theta;   // npts
+    const double *phi;     // npts
+    double *field;         // npts * 4: F_theta re/im, F_phi re/im
+    double *partial;       // per-block partial sums of the integral
+    int npartial;
+    int pad;
+    double *total;         // [0]
+    double *out;           // npts * 4: gains + axial ratio
+};
+
+} // namespace fnec

@@ -1,0 +1,62 @@
+// Internal interface between the C ABI (abi.hip) and the engine (engine.hip).
+#pragma once
+
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/fnec_hip.h"
+
+namespace fnec {
+
+// Host-side description of one Hallén problem (augmented system layout).
+struct HostProblem {
+    int n = 0, s = 0, r = 0, w = 0, ncons = 0;
+    int ground = 0;
+    double k = 0.0, freq = 0.0, gam_re = 0.0, gam_im = 0.0;
+    std::vector<double> seg;   // n * 8
+    std::vector<double> cosv;  // n
+    std::vector<double> rhs;   // 2n
+    std::vector<int> runs;     // 3 per run: row begin, row end, column
+    std::vector<int> cons;     // 2 per constraint row
+    std::vector<double> cons_sign;
+};
+
+// Host-side description of one far-field job.
+struct RpHost {
+    int n = 0, ground = 0, ndir = 0;
+    double k = 0.0, eps_r = 0.0, sigma = 0.0;
+    std::vector<double> seg, cur, theta, phi;
+    const double *cur_dev = nullptr; // currents already on the device
+};
+
+void set_error(const std::string &e);
+const char *last_error();
+int ensure_ready();
+std::mutex &global_mutex();
+size_t problem_bytes(int n, int s, int ncons, int nruns);
+
+int build_problem(HostProblem &hp, const fnec_seg *segs, size_t n, const fnec_c64 *rhs,
+                  const double *cos_vec, const std::vector<int> &row_col, int ncols,
+                  const std::vector<int> &cons, const std::vector<double> &cons_sign,
+                  double freq_hz, const fnec_ground *ground);
+int problem_from_hallen(HostProblem &hp, const fnec_seg *segs, size_t n, const fnec_c64 *rhs,
+                        const double *cos_vec, const size_t *wf, const size_t *wl, size_t nw,
+                        const size_t *ja, const size_t *jb, const double *js, size_t nj,
+                        double freq_hz, const fnec_ground *ground);
+int problem_from_paths(HostProblem &hp, const fnec_seg *segs, size_t n, const fnec_c64 *rhs,
+                       const double *cos_vec, const size_t *path_of, const size_t *free_ends,
+                       size_t nfree, double freq_hz, const fnec_ground *ground);
+
+int solve_problems(const std::vector<const HostProblem *> &hps, std::vector<fnec_c64 *> &x_out,
+                   std::vector<fnec_solve_stats> &stats, std::vector<int> &status,
+                   std::vector<double *> *device_x, int precision);
+int fill_only(const HostProblem &hp, fnec_c64 *z_out);
+int normal_eqs_only(const HostProblem &hp, fnec_c64 *aug_out);
+int rp_host_from(RpHost &h, const fnec_seg *segs, const fnec_c64 *cur, size_t n, double freq_hz,
+                 const fnec_ground *ground, const double *theta, const double *phi, size_t npts);
+int rp_run(const std::vector<RpHost> &jobs, std::vector<fnec_rp_row *> &outs,
+           std::vector<double> &totals);
+int mfma_probe(const double *a, const double *b, double *out);
+
+} // namespace fnec

@@ -1,0 +1,942 @@
+// Host engine of fnec_hip: device context, batch planning (problem tables and
+// work lists), the launch sequence fill -> Gram -> LU -> back substitution ->
+// residual, the far-field launches, and the C ABI of include/fnec_hip.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/fnec_hip.h"
+#include "common.h"
+#include "engine.h"
+
+namespace fnec {
+
+// kernels.hip
+__global__ void k_fill_z(const Prob *, const int4 *);
+__global__ void k_gram(const Prob *, const int4 *);
+__global__ void k_gram_cols(const Prob *, const int2 *);
+__global__ void k_gram_fix(const Prob *, const int *);
+__global__ void k_lu_panel(const Prob *, const int *, int);
+__global__ void k_lu_trsm(const Prob *, const int2 *, int);
+__global__ void k_lu_update(const Prob *, const int4 *, int);
+__global__ void k_backsub(const Prob *, const int *);
+__global__ void k_residual_rows(const Prob *, const int2 *);
+__global__ void k_residual_finish(const Prob *, const int *);
+__global__ void k_rp_points(const RpProb *, const int2 *);
+__global__ void k_rp_norm(const RpProb *, const int2 *);
+__global__ void k_rp_finish(const RpProb *, const int *);
+__global__ void k_zero(double *, size_t);
+__global__ void k_mfma_probe(const double *, const double *, double *);
+
+// ---------------------------------------------------------------------------
+// context
+// ---------------------------------------------------------------------------
+
+static thread_local std::string g_err;
+static std::mutex g_mu;
+
+void set_error(const std::string &e) { g_err = e; }
+const char *last_error() { return g_err.c_str(); }
+
+#define HIPCHK(x)                                                                               \
+    do {                                                                                        \
+        hipError_t e_ = (x);                                                                    \
+        if (e_ != hipSuccess) {                                                                 \
+            set_error(std::string(#x) + ": " + hipGetErrorString(e_));                          \
+            return FNEC_INTERNAL;                                                               \
+        }                                                                                       \
+    } while (0)
+
+struct Context {
+    bool ready = false;
+    int status = FNEC_UNAVAILABLE;
+    int device = -1;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[8];
+    // cached device arena, grown on demand
+    char *arena = nullptr;
+    size_t arena_bytes = 0;
+};
+
+static Context g_ctx;
+
+static int ctx_init()
+{
+    if (g_ctx.ready)
+        return g_ctx.status;
+    g_ctx.ready = true;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) {
+        set_error("no HIP device");
+        g_ctx.status = FNEC_UNAVAILABLE;
+        return g_ctx.status;
+    }
+    for (int d = 0; d < count; ++d) {
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, d) != hipSuccess)
+            continue;
+        if (std::strncmp(prop.gcnArchName, "gfx950", 6) == 0) {
+            g_ctx.device = d;
+            break;
+        }
+    }
+    if (g_ctx.device < 0) {
+        set_error("no gfx950 device");
+        g_ctx.status = FNEC_UNAVAILABLE;
+        return g_ctx.status;
+    }
+    if (hipSetDevice(g_ctx.device) != hipSuccess || hipStreamCreate(&g_ctx.stream) != hipSuccess) {
+        set_error("device initialisation failed");
+        g_ctx.status = FNEC_INTERNAL;
+        return g_ctx.status;
+    }
+    for (auto &e : g_ctx.ev)
+        if (hipEventCreate(&e) != hipSuccess) {
+            g_ctx.status = FNEC_INTERNAL;
+            return g_ctx.status;
+        }
+    g_ctx.status = FNEC_OK;
+    return g_ctx.status;
+}
+
+int ensure_ready()
+{
+    int st = ctx_init();
+    if (st == FNEC_OK)
+        hipSetDevice(g_ctx.device);
+    return st;
+}
+
+hipStream_t stream() { return g_ctx.stream; }
+
+static int arena_reserve(size_t bytes, char **out)
+{
+    if (bytes > g_ctx.arena_bytes) {
+        if (g_ctx.arena) {
+            HIPCHK(hipStreamSynchronize(g_ctx.stream));
+            HIPCHK(hipFree(g_ctx.arena));
+            g_ctx.arena = nullptr;
+            g_ctx.arena_bytes = 0;
+        }
+        size_t want = bytes + bytes / 4 + (64u << 20);
+        hipError_t e = hipMalloc(&g_ctx.arena, want);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            e = hipMalloc(&g_ctx.arena, bytes);
+            if (e != hipSuccess) {
+                (void)hipGetLastError();
+                set_error("device memory exhausted");
+                return FNEC_RESOURCE_EXHAUSTED;
+            }
+            want = bytes;
+        }
+        g_ctx.arena_bytes = want;
+    }
+    *out = g_ctx.arena;
+    return FNEC_OK;
+}
+
+// ---------------------------------------------------------------------------
+// host-side problem construction
+// ---------------------------------------------------------------------------
+
+static inline int roundup(int v, int m) { return (v + m - 1) / m * m; }
+
+// num-complex 0.4 sqrt (polar form) and the scalar reflection coefficient of
+// matrix.rs:285-290.
+static void csqrt_nc(double re, double im, double &sr, double &si)
+{
+    if (im == 0.0) {
+        if (!std::signbit(re)) {
+            sr = std::sqrt(re);
+            si = im;
+        } else {
+            sr = 0.0;
+            si = std::signbit(im) ? -std::sqrt(-re) : std::sqrt(-re);
+        }
+        return;
+    }
+    if (re == 0.0) {
+        double x = std::sqrt(std::fabs(im) / 2.0);
+        sr = x;
+        si = std::signbit(im) ? -x : x;
+        return;
+    }
+    double r = std::hypot(re, im), th = std::atan2(im, re);
+    double rs = std::sqrt(r);
+    sr = rs * std::cos(th / 2.0);
+    si = rs * std::sin(th / 2.0);
+}
+
+static void fresnel_scalar(double freq, double eps_r, double sigma, double &gr, double &gi)
+{
+    double omega = 2.0 * kPi * freq;
+    double er = std::isnan(eps_r) ? 1e-6 : std::max(eps_r, 1e-6);
+    double sg = std::isnan(sigma) ? 0.0 : std::max(sigma, 0.0);
+    double sr, si;
+    csqrt_nc(er, -sg / (omega * kEps0Matrix), sr, si);
+    double ar = sr - 1.0, ai = si, br = sr + 1.0, bi = si;
+    double ns = br * br + bi * bi;
+    gr = (ar * br + ai * bi) / ns;
+    gi = (ai * br - ar * bi) / ns;
+}
+
+int build_problem(HostProblem &hp, const fnec_seg *segs, size_t n, const fnec_c64 *rhs,
+                  const double *cos_vec, const std::vector<int> &row_col, int ncols,
+                  const std::vector<int> &cons, const std::vector<double> &cons_sign,
+                  double freq_hz, const fnec_ground *ground)
+{
+    if (n == 0 || !segs || !rhs || !cos_vec) {
+        set_error("empty or null input");
+        return FNEC_INVALID_ARG;
+    }
+    if (n > (size_t)1 << 20) {
+        set_error("too many segments");
+        return FNEC_INVALID_ARG;
+    }
+    if (!(freq_hz > 0.0) || !std::isfinite(freq_hz)) {
+        set_error("frequency must be positive");
+        return FNEC_INVALID_ARG;
+    }
+    hp.n = (int)n;
+    hp.w = ncols;
+    hp.s = (int)n + ncols;
+    hp.ncons = (int)cons_sign.size();
+    hp.r = (int)n + hp.ncons;
+    hp.k = 2.0 * kPi * freq_hz / kC0;
+    hp.freq = freq_hz;
+    hp.ground = 0;
+    hp.gam_re = hp.gam_im = 0.0;
+    if (ground) {
+        if (ground->kind == FNEC_GROUND_PEC)
+            hp.ground = 1;
+        else if (ground->kind == FNEC_GROUND_FINITE) {
+            hp.ground = 2;
+            fresnel_scalar(freq_hz, ground->eps_r, ground->sigma, hp.gam_re, hp.gam_im);
+        } else if (ground->kind != FNEC_GROUND_FREE) {
+            set_error("unknown ground kind");
+            return FNEC_INVALID_ARG;
+        }
+    }
+    hp.seg.resize(n * kSegStride);
+    for (size_t i = 0; i < n; ++i) {
+        double *o = &hp.seg[i * kSegStride];
+        for (int c = 0; c < 3; ++c) {
+            o[c] = segs[i].mid[c];
+            o[3 + c] = segs[i].dir[c];
+        }
+        o[6] = segs[i].length;
+        o[7] = segs[i].radius;
+    }
+    hp.cosv.assign(cos_vec, cos_vec + n);
+    hp.rhs.resize(2 * n);
+    for (size_t i = 0; i < n; ++i) {
+        hp.rhs[2 * i] = rhs[i].re;
+        hp.rhs[2 * i + 1] = rhs[i].im;
+    }
+    hp.runs.clear();
+    for (size_t r = 0; r < n;) {
+        size_t e = r + 1;
+        while (e < n && row_col[e] == row_col[r])
+            ++e;
+        hp.runs.push_back((int)r);
+        hp.runs.push_back((int)e);
+        hp.runs.push_back(row_col[r]);
+        r = e;
+    }
+    hp.cons = cons;
+    hp.cons_sign = cons_sign;
+    return FNEC_OK;
+}
+
+// Augmented-system layout of solve_hallen (linear.rs:688-762).
+int problem_from_hallen(HostProblem &hp, const fnec_seg *segs, size_t n, const fnec_c64 *rhs,
+                        const double *cos_vec, const size_t *wf, const size_t *wl, size_t nw,
+                        const size_t *ja, const size_t *jb, const double *js, size_t nj,
+                        double freq_hz, const fnec_ground *ground)
+{
+    std::vector<size_t> ef, el;
+    if (nw == 0 || n == 0) {
+        ef.push_back(0);
+        el.push_back(n ? n - 1 : 0);
+    } else {
+        if (!wf || !wl) {
+            set_error("null wire range");
+            return FNEC_INVALID_ARG;
+        }
+        ef.assign(wf, wf + nw);
+        el.assign(wl, wl + nw);
+    }
+    if (nj && (!ja || !jb || !js)) {
+        set_error("null junction arrays");
+        return FNEC_INVALID_ARG;
+    }
+    for (size_t i = 0; i < ef.size(); ++i)
+        if (ef[i] >= n || el[i] >= n) {
+            set_error("wire range outside the segment list");
+            return FNEC_INVALID_ARG;
+        }
+    std::vector<char> injunc(n, 0);
+    for (size_t i = 0; i < nj; ++i) {
+        if (ja[i] >= n || jb[i] >= n) {
+            set_error("junction index outside the segment list");
+            return FNEC_INVALID_ARG;
+        }
+        injunc[ja[i]] = injunc[jb[i]] = 1;
+    }
+    std::vector<int> row_col(n, 0);
+    for (size_t w = 0; w < ef.size(); ++w)
+        for (size_t r = ef[w]; r <= el[w] && r < n; ++r)
+            row_col[r] = (int)w;
+    std::vector<int> cons;
+    std::vector<double> sign;
+    for (size_t w = 0; w < ef.size(); ++w) {
+        if (!injunc[ef[w]]) {
+            cons.push_back((int)ef[w]);
+            cons.push_back(-1);
+            sign.push_back(1.0);
+        }
+        if (!injunc[el[w]]) {
+            cons.push_back((int)el[w]);
+            cons.push_back(-1);
+            sign.push_back(1.0);
+        }
+    }
+    for (size_t i = 0; i < nj; ++i) {
+        cons.push_back((int)ja[i]);
+        cons.push_back((int)jb[i]);
+        sign.push_back(js[i]);
+    }
+    return build_problem(hp, segs, n, rhs, cos_vec, row_col, (int)ef.size(), cons, sign, freq_hz,
+                         ground);
+}
+
+// Conductor-path layout of solve_hallen_paths (linear.rs:839-860).
+int problem_from_paths(HostProblem &hp, const fnec_seg *segs, size_t n, const fnec_c64 *rhs,
+                       const double *cos_vec, const size_t *path_of, const size_t *free_ends,
+                       size_t nfree, double freq_hz, const fnec_ground *ground)
+{
+    if (!path_of || (nfree && !free_ends)) {
+        set_error("null path arrays");
+        return FNEC_INVALID_ARG;
+    }
+    size_t np = 0;
+    std::vector<int> row_col(n);
+    for (size_t i = 0; i < n; ++i) {
+        if (path_of[i] >= n) {
+            set_error("path index out of range");
+            return FNEC_INVALID_ARG;
+        }
+        row_col[i] = (int)path_of[i];
+        np = std::max(np, path_of[i] + 1);
+    }
+    std::vector<int> cons;
+    std::vector<double> sign;
+    for (size_t i = 0; i < nfree; ++i) {
+        if (free_ends[i] >= n) {
+            set_error("free end outside the segment list");
+            return FNEC_INVALID_ARG;
+        }
+        cons.push_back((int)free_ends[i]);
+        cons.push_back(-1);
+        sign.push_back(1.0);
+    }
+    return build_problem(hp, segs, n, rhs, cos_vec, row_col, (int)np, cons, sign, freq_hz, ground);
+}
+
+// ---------------------------------------------------------------------------
+// batch planning
+// ---------------------------------------------------------------------------
+
+struct Plan {
+    std::vector<Prob> probs;
+    // work lists, concatenated into one int buffer
+    std::vector<int> ints;
+    size_t fill_off = 0, fill_cnt = 0;
+    size_t gram_off = 0, gram_cnt = 0;
+    size_t cols_off = 0, cols_cnt = 0;
+    size_t all_off = 0, all_cnt = 0;
+    size_t res_off = 0, res_cnt = 0;
+    struct Step {
+        int p;
+        size_t panel_off, panel_cnt, trsm_off, trsm_cnt, upd_off, upd_cnt;
+    };
+    std::vector<Step> steps;
+    // arena layout
+    size_t bytes = 0;
+    size_t a_begin = 0, a_end = 0; // byte range of the A planes (zeroed per run)
+    std::vector<size_t> off_seg, off_z, off_a, off_cos, off_rhs, off_runs, off_cons, off_sign,
+        off_ipiv, off_info, off_x, off_stats, off_scr;
+    size_t off_probs = 0, off_ints = 0;
+};
+
+static size_t take(size_t &cur, size_t bytes)
+{
+    size_t o = (cur + 255) & ~(size_t)255;
+    cur = o + bytes;
+    return o;
+}
+
+size_t problem_bytes(int n, int s, int ncons, int nruns)
+{
+    size_t ldz = (size_t)roundup(n, 64), lda = (size_t)roundup(s, 64);
+    size_t b = 0;
+    b += 2 * 8 * ldz * n + 256;
+    b += 2 * 8 * lda * (s + 1) + 256;
+    b += (size_t)n * 8 * kSegStride + 8 * n + 16 * n + 12 * nruns + 8 * ncons + 8 * ncons + 4 * s;
+    b += 16 * s + 64 + 32 + 32 * ((n + 255) / 256 + 1) + 12 * 256;
+    return b;
+}
+
+static void plan_build(Plan &pl, const std::vector<const HostProblem *> &hps, bool with_solve)
+{
+    const size_t np = hps.size();
+    pl.probs.resize(np);
+    size_t cur = 0;
+    pl.off_seg.resize(np);
+    pl.off_z.resize(np);
+    pl.off_a.resize(np);
+    pl.off_cos.resize(np);
+    pl.off_rhs.resize(np);
+    pl.off_runs.resize(np);
+    pl.off_cons.resize(np);
+    pl.off_sign.resize(np);
+    pl.off_ipiv.resize(np);
+    pl.off_info.resize(np);
+    pl.off_x.resize(np);
+    pl.off_stats.resize(np);
+    pl.off_scr.resize(np);
+    // A planes first, contiguous, so one zero-fill covers them
+    pl.a_begin = 0;
+    for (size_t b = 0; b < np; ++b) {
+        const HostProblem &h = *hps[b];
+        size_t lda = (size_t)roundup(h.s, 64);
+        pl.off_a[b] = with_solve ? take(cur, 2 * 8 * lda * (h.s + 1)) : 0;
+    }
+    pl.a_end = cur;
+    for (size_t b = 0; b < np; ++b) {
+        const HostProblem &h = *hps[b];
+        size_t ldz = (size_t)roundup(h.n, 64);
+        pl.off_z[b] = take(cur, 2 * 8 * ldz * h.n);
+        pl.off_seg[b] = take(cur, (size_t)h.n * 8 * kSegStride);
+        pl.off_cos[b] = take(cur, 8 * (size_t)h.n);
+        pl.off_rhs[b] = take(cur, 16 * (size_t)h.n);
+        pl.off_runs[b] = take(cur, 4 * h.runs.size() + 4);
+        pl.off_cons[b] = take(cur, 4 * h.cons.size() + 4);
+        pl.off_sign[b] = take(cur, 8 * h.cons_sign.size() + 8);
+        pl.off_ipiv[b] = take(cur, 4 * (size_t)h.s);
+        pl.off_info[b] = take(cur, 8);
+        pl.off_x[b] = take(cur, 16 * (size_t)h.s);
+        pl.off_stats[b] = take(cur, 32);
+        pl.off_scr[b] = take(cur, 32 * ((size_t)(h.n + 255) / 256 + 1));
+    }
+    // work lists
+    auto push4 = [&](int a, int b, int c, int d) {
+        pl.ints.push_back(a);
+        pl.ints.push_back(b);
+        pl.ints.push_back(c);
+        pl.ints.push_back(d);
+    };
+    auto push2 = [&](int a, int b) {
+        pl.ints.push_back(a);
+        pl.ints.push_back(b);
+    };
+    // every list starts on a 16-byte boundary so int2 / int4 loads are aligned
+    auto align4 = [&]() {
+        while (pl.ints.size() % 4)
+            pl.ints.push_back(0);
+    };
+    pl.ints.clear();
+    align4();
+    pl.fill_off = pl.ints.size();
+    for (size_t b = 0; b < np; ++b) {
+        int rt = (hps[b]->n + 63) / 64, ct = (hps[b]->n + 15) / 16;
+        for (int y = 0; y < rt; ++y)
+            for (int z = 0; z < ct; ++z)
+                push4((int)b, y, z, 0);
+    }
+    pl.fill_cnt = (pl.ints.size() - pl.fill_off) / 4;
+    if (with_solve) {
+        align4();
+        pl.gram_off = pl.ints.size();
+        for (size_t b = 0; b < np; ++b) {
+            int nt = (hps[b]->n + kTile - 1) / kTile;
+            for (int y = 0; y < nt; ++y)
+                for (int z = 0; z <= y; ++z)
+                    push4((int)b, y, z, 0);
+        }
+        pl.gram_cnt = (pl.ints.size() - pl.gram_off) / 4;
+        align4();
+        pl.cols_off = pl.ints.size();
+        for (size_t b = 0; b < np; ++b)
+            for (int y = 0; y < (hps[b]->n + 3) / 4; ++y)
+                push2((int)b, y);
+        pl.cols_cnt = (pl.ints.size() - pl.cols_off) / 2;
+        align4();
+        pl.all_off = pl.ints.size();
+        for (size_t b = 0; b < np; ++b)
+            pl.ints.push_back((int)b);
+        pl.all_cnt = np;
+        int smax = 0;
+        for (auto *h : hps)
+            smax = std::max(smax, h->s);
+        pl.steps.clear();
+        for (int p = 0; p < smax; p += kPanel) {
+            Plan::Step st;
+            st.p = p;
+            align4();
+            st.panel_off = pl.ints.size();
+            for (size_t b = 0; b < np; ++b)
+                if (hps[b]->s > p)
+                    pl.ints.push_back((int)b);
+            st.panel_cnt = pl.ints.size() - st.panel_off;
+            align4();
+            st.trsm_off = pl.ints.size();
+            for (size_t b = 0; b < np; ++b) {
+                int s = hps[b]->s;
+                if (s <= p)
+                    continue;
+                int jb = std::min(kPanel, s - p);
+                int ncols = s + 1 - p - jb;
+                for (int y = 0; y < (ncols + 63) / 64; ++y)
+                    push2((int)b, y);
+            }
+            st.trsm_cnt = (pl.ints.size() - st.trsm_off) / 2;
+            align4();
+            st.upd_off = pl.ints.size();
+            for (size_t b = 0; b < np; ++b) {
+                int s = hps[b]->s;
+                if (s <= p)
+                    continue;
+                int jb = std::min(kPanel, s - p);
+                int nrows = s - p - jb, ncols = s + 1 - p - jb;
+                if (nrows <= 0)
+                    continue;
+                for (int y = 0; y < (nrows + kTile - 1) / kTile; ++y)
+                    for (int z = 0; z < (ncols + kTile - 1) / kTile; ++z)
+                        push4((int)b, y, z, 0);
+            }
+            st.upd_cnt = (pl.ints.size() - st.upd_off) / 4;
+            pl.steps.push_back(st);
+        }
+        align4();
+        pl.res_off = pl.ints.size();
+        for (size_t b = 0; b < np; ++b)
+            for (int y = 0; y < (hps[b]->n + 255) / 256; ++y)
+                push2((int)b, y);
+        pl.res_cnt = (pl.ints.size() - pl.res_off) / 2;
+    }
+    pl.off_probs = take(cur, sizeof(Prob) * np);
+    pl.off_ints = take(cur, 4 * pl.ints.size() + 16);
+    pl.bytes = cur + 256;
+}
+
+static int plan_upload(Plan &pl, const std::vector<const HostProblem *> &hps, char *base,
+                       bool with_solve)
+{
+    hipStream_t st = g_ctx.stream;
+    const size_t np = hps.size();
+    for (size_t b = 0; b < np; ++b) {
+        const HostProblem &h = *hps[b];
+        Prob &P = pl.probs[b];
+        std::memset(&P, 0, sizeof(P));
+        P.n = h.n;
+        P.s = h.s;
+        P.r = h.r;
+        P.w = h.w;
+        P.ncons = h.ncons;
+        P.nruns = (int)h.runs.size() / 3;
+        P.ldz = roundup(h.n, 64);
+        P.lda = roundup(h.s, 64);
+        P.ground = h.ground;
+        P.k = h.k;
+        P.gam_re = h.gam_re;
+        P.gam_im = h.gam_im;
+        P.seg = (const double *)(base + pl.off_seg[b]);
+        P.zr = (double *)(base + pl.off_z[b]);
+        P.zi = P.zr + (size_t)P.ldz * h.n;
+        if (with_solve) {
+            P.ar = (double *)(base + pl.off_a[b]);
+            P.ai = P.ar + (size_t)P.lda * (h.s + 1);
+        }
+        P.cosv = (const double *)(base + pl.off_cos[b]);
+        P.rhs = (const double *)(base + pl.off_rhs[b]);
+        P.runs = (const int *)(base + pl.off_runs[b]);
+        P.cons = (const int *)(base + pl.off_cons[b]);
+        P.cons_sign = (const double *)(base + pl.off_sign[b]);
+        P.ipiv = (int *)(base + pl.off_ipiv[b]);
+        P.info = (int *)(base + pl.off_info[b]);
+        P.x = (double *)(base + pl.off_x[b]);
+        P.stats = (double *)(base + pl.off_stats[b]);
+        P.scratch = (double *)(base + pl.off_scr[b]);
+        HIPCHK(hipMemcpyAsync(base + pl.off_seg[b], h.seg.data(), 8 * h.seg.size(), hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(base + pl.off_cos[b], h.cosv.data(), 8 * h.cosv.size(), hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(base + pl.off_rhs[b], h.rhs.data(), 8 * h.rhs.size(), hipMemcpyHostToDevice, st));
+        if (!h.runs.empty())
+            HIPCHK(hipMemcpyAsync(base + pl.off_runs[b], h.runs.data(), 4 * h.runs.size(), hipMemcpyHostToDevice, st));
+        if (!h.cons.empty()) {
+            HIPCHK(hipMemcpyAsync(base + pl.off_cons[b], h.cons.data(), 4 * h.cons.size(), hipMemcpyHostToDevice, st));
+            HIPCHK(hipMemcpyAsync(base + pl.off_sign[b], h.cons_sign.data(), 8 * h.cons_sign.size(), hipMemcpyHostToDevice, st));
+        }
+    }
+    // info words start at zero; they are written only by device code after this
+    {
+        std::vector<char> zeros(8, 0);
+        for (size_t b = 0; b < np; ++b)
+            HIPCHK(hipMemcpyAsync(base + pl.off_info[b], zeros.data(), 8, hipMemcpyHostToDevice, st));
+    }
+    HIPCHK(hipMemcpyAsync(base + pl.off_probs, pl.probs.data(), sizeof(Prob) * np, hipMemcpyHostToDevice, st));
+    if (!pl.ints.empty())
+        HIPCHK(hipMemcpyAsync(base + pl.off_ints, pl.ints.data(), 4 * pl.ints.size(), hipMemcpyHostToDevice, st));
+    return FNEC_OK;
+}
+
+static inline const int *wl_ptr(char *base, const Plan &pl, size_t off)
+{
+    return (const int *)(base + pl.off_ints) + off;
+}
+
+// Launch the whole solve on the context stream; records phase events.
+static int plan_launch(Plan &pl, char *base, bool with_solve)
+{
+    hipStream_t st = g_ctx.stream;
+    const Prob *dp = (const Prob *)(base + pl.off_probs);
+    HIPCHK(hipEventRecord(g_ctx.ev[0], st));
+    if (pl.fill_cnt)
+        hipLaunchKernelGGL(k_fill_z, dim3((unsigned)pl.fill_cnt), dim3(256), 0, st, dp,
+                           (const int4 *)wl_ptr(base, pl, pl.fill_off));
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(g_ctx.ev[1], st));
+    if (!with_solve)
+        return FNEC_OK;
+    {
+        size_t count = (pl.a_end - pl.a_begin) / 8;
+        if (count)
+            hipLaunchKernelGGL(k_zero, dim3(1024), dim3(256), 0, st, (double *)(base + pl.a_begin), count);
+    }
+    hipLaunchKernelGGL(k_gram, dim3((unsigned)pl.gram_cnt), dim3(256), 0, st, dp,
+                       (const int4 *)wl_ptr(base, pl, pl.gram_off));
+    hipLaunchKernelGGL(k_gram_cols, dim3((unsigned)pl.cols_cnt), dim3(256), 0, st, dp,
+                       (const int2 *)wl_ptr(base, pl, pl.cols_off));
+    hipLaunchKernelGGL(k_gram_fix, dim3((unsigned)pl.all_cnt), dim3(256), 0, st, dp,
+                       wl_ptr(base, pl, pl.all_off));
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(g_ctx.ev[2], st));
+    for (const auto &s : pl.steps) {
+        if (s.panel_cnt)
+            hipLaunchKernelGGL(k_lu_panel, dim3((unsigned)s.panel_cnt), dim3(512), 0, st, dp,
+                               wl_ptr(base, pl, s.panel_off), s.p);
+        if (s.trsm_cnt)
+            hipLaunchKernelGGL(k_lu_trsm, dim3((unsigned)s.trsm_cnt), dim3(64), 0, st, dp,
+                               (const int2 *)wl_ptr(base, pl, s.trsm_off), s.p);
+        if (s.upd_cnt)
+            hipLaunchKernelGGL(k_lu_update, dim3((unsigned)s.upd_cnt), dim3(256), 0, st, dp,
+                               (const int4 *)wl_ptr(base, pl, s.upd_off), s.p);
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(g_ctx.ev[3], st));
+    hipLaunchKernelGGL(k_backsub, dim3((unsigned)pl.all_cnt), dim3(256), 0, st, dp,
+                       wl_ptr(base, pl, pl.all_off));
+    hipLaunchKernelGGL(k_residual_rows, dim3((unsigned)pl.res_cnt), dim3(256), 0, st, dp,
+                       (const int2 *)wl_ptr(base, pl, pl.res_off));
+    hipLaunchKernelGGL(k_residual_finish, dim3((unsigned)pl.all_cnt), dim3(64), 0, st, dp,
+                       wl_ptr(base, pl, pl.all_off));
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(g_ctx.ev[4], st));
+    return FNEC_OK;
+}
+
+static float ev_ms(int a, int b)
+{
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, g_ctx.ev[a], g_ctx.ev[b]) != hipSuccess)
+        return 0.f;
+    return ms;
+}
+
+// Solve a batch of problems; x_out[b] receives s complex values (may be null).
+int solve_problems(const std::vector<const HostProblem *> &hps, std::vector<fnec_c64 *> &x_out,
+                   std::vector<fnec_solve_stats> &stats, std::vector<int> &status,
+                   std::vector<double *> *device_x, int precision)
+{
+    if (precision != FNEC_PREC_FP64) {
+        set_error("mixed precision is not available in this build");
+        return FNEC_INVALID_ARG;
+    }
+    Plan pl;
+    plan_build(pl, hps, true);
+    char *base = nullptr;
+    int st = arena_reserve(pl.bytes, &base);
+    if (st)
+        return st;
+    st = plan_upload(pl, hps, base, true);
+    if (st)
+        return st;
+    st = plan_launch(pl, base, true);
+    if (st)
+        return st;
+    HIPCHK(hipStreamSynchronize(g_ctx.stream));
+    const float t_fill = ev_ms(0, 1) * 1000.f, t_gram = ev_ms(1, 2) * 1000.f,
+                t_lu = ev_ms(2, 3) * 1000.f, t_sol = ev_ms(3, 4) * 1000.f;
+    const size_t np = hps.size();
+    status.assign(np, FNEC_OK);
+    stats.assign(np, fnec_solve_stats{});
+    for (size_t b = 0; b < np; ++b) {
+        int info[2];
+        double sv[4];
+        HIPCHK(hipMemcpy(info, base + pl.off_info[b], 8, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(sv, base + pl.off_stats[b], 32, hipMemcpyDeviceToHost));
+        status[b] = info[0] ? FNEC_SINGULAR : FNEC_OK;
+        fnec_solve_stats &s = stats[b];
+        s.abs_res = sv[0];
+        s.rel_res = sv[1];
+        s.diag_spread = sv[2];
+        s.n_swaps = info[1];
+        s.rows = hps[b]->r;
+        s.unknowns = hps[b]->s;
+        s.t_fill_us = t_fill;
+        s.t_gram_us = t_gram;
+        s.t_lu_us = t_lu;
+        s.t_solve_us = t_sol;
+        s.t_total_us = t_fill + t_gram + t_lu + t_sol;
+        if (x_out.size() > b && x_out[b] && !info[0])
+            HIPCHK(hipMemcpy(x_out[b], base + pl.off_x[b], 16 * (size_t)hps[b]->s, hipMemcpyDeviceToHost));
+        if (device_x)
+            device_x->push_back((double *)(base + pl.off_x[b]));
+    }
+    return FNEC_OK;
+}
+
+int fill_only(const HostProblem &hp, fnec_c64 *z_out)
+{
+    Plan pl;
+    std::vector<const HostProblem *> v{&hp};
+    plan_build(pl, v, false);
+    char *base = nullptr;
+    int st = arena_reserve(pl.bytes, &base);
+    if (st)
+        return st;
+    if ((st = plan_upload(pl, v, base, false)))
+        return st;
+    if ((st = plan_launch(pl, base, false)))
+        return st;
+    const int n = hp.n, ldz = roundup(n, 64);
+    std::vector<double> zr((size_t)ldz * n), zi((size_t)ldz * n);
+    HIPCHK(hipMemcpyAsync(zr.data(), base + pl.off_z[0], 8 * zr.size(), hipMemcpyDeviceToHost, g_ctx.stream));
+    HIPCHK(hipMemcpyAsync(zi.data(), base + pl.off_z[0] + 8 * zr.size(), 8 * zi.size(), hipMemcpyDeviceToHost, g_ctx.stream));
+    HIPCHK(hipStreamSynchronize(g_ctx.stream));
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j) {
+            z_out[(size_t)i * n + j].re = zr[(size_t)j * ldz + i];
+            z_out[(size_t)i * n + j].im = zi[(size_t)j * ldz + i];
+        }
+    return FNEC_OK;
+}
+
+int normal_eqs_only(const HostProblem &hp, fnec_c64 *aug_out)
+{
+    Plan pl;
+    std::vector<const HostProblem *> v{&hp};
+    plan_build(pl, v, true);
+    char *base = nullptr;
+    int st = arena_reserve(pl.bytes, &base);
+    if (st)
+        return st;
+    if ((st = plan_upload(pl, v, base, true)))
+        return st;
+    hipStream_t s = g_ctx.stream;
+    const Prob *dp = (const Prob *)(base + pl.off_probs);
+    hipLaunchKernelGGL(k_fill_z, dim3((unsigned)pl.fill_cnt), dim3(256), 0, s, dp,
+                       (const int4 *)wl_ptr(base, pl, pl.fill_off));
+    size_t count = (pl.a_end - pl.a_begin) / 8;
+    hipLaunchKernelGGL(k_zero, dim3(1024), dim3(256), 0, s, (double *)(base + pl.a_begin), count);
+    hipLaunchKernelGGL(k_gram, dim3((unsigned)pl.gram_cnt), dim3(256), 0, s, dp,
+                       (const int4 *)wl_ptr(base, pl, pl.gram_off));
+    hipLaunchKernelGGL(k_gram_cols, dim3((unsigned)pl.cols_cnt), dim3(256), 0, s, dp,
+                       (const int2 *)wl_ptr(base, pl, pl.cols_off));
+    hipLaunchKernelGGL(k_gram_fix, dim3((unsigned)pl.all_cnt), dim3(256), 0, s, dp,
+                       wl_ptr(base, pl, pl.all_off));
+    HIPCHK(hipGetLastError());
+    const int S = hp.s, lda = roundup(S, 64);
+    std::vector<double> ar((size_t)lda * (S + 1)), ai((size_t)lda * (S + 1));
+    HIPCHK(hipMemcpyAsync(ar.data(), base + pl.off_a[0], 8 * ar.size(), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(ai.data(), base + pl.off_a[0] + 8 * ar.size(), 8 * ai.size(), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    for (int i = 0; i < S; ++i)
+        for (int j = 0; j <= S; ++j) {
+            aug_out[(size_t)i * (S + 1) + j].re = ar[(size_t)j * lda + i];
+            aug_out[(size_t)i * (S + 1) + j].im = ai[(size_t)j * lda + i];
+        }
+    return FNEC_OK;
+}
+
+// ---------------------------------------------------------------------------
+// far field
+// ---------------------------------------------------------------------------
+
+int rp_run(const std::vector<RpHost> &jobs, std::vector<fnec_rp_row *> &outs, std::vector<double> &totals)
+{
+    const size_t nj = jobs.size();
+    size_t cur = 0;
+    std::vector<size_t> o_seg(nj), o_cur(nj), o_th(nj), o_ph(nj), o_field(nj), o_part(nj),
+        o_tot(nj), o_out(nj);
+    std::vector<int> ints;
+    size_t pts_off, norm_off, all_off;
+    for (size_t j = 0; j < nj; ++j) {
+        const RpHost &h = jobs[j];
+        o_seg[j] = take(cur, 8 * h.seg.size());
+        o_cur[j] = h.cur_dev ? 0 : take(cur, 8 * h.cur.size());
+        o_th[j] = take(cur, 8 * h.theta.size() + 8);
+        o_ph[j] = take(cur, 8 * h.phi.size() + 8);
+        o_field[j] = take(cur, 32 * h.theta.size() + 32);
+        o_part[j] = take(cur, 8 * (size_t)((h.ndir + 255) / 256) + 8);
+        o_tot[j] = take(cur, 8);
+        o_out[j] = take(cur, 32 * h.theta.size() + 32);
+    }
+    pts_off = ints.size();
+    for (size_t j = 0; j < nj; ++j)
+        for (int y = 0; y < ((int)jobs[j].theta.size() + 255) / 256; ++y) {
+            ints.push_back((int)j);
+            ints.push_back(y);
+        }
+    size_t pts_cnt = (ints.size() - pts_off) / 2;
+    norm_off = ints.size();
+    for (size_t j = 0; j < nj; ++j)
+        for (int y = 0; y < (jobs[j].ndir + 255) / 256; ++y) {
+            ints.push_back((int)j);
+            ints.push_back(y);
+        }
+    size_t norm_cnt = (ints.size() - norm_off) / 2;
+    all_off = ints.size();
+    for (size_t j = 0; j < nj; ++j)
+        ints.push_back((int)j);
+    size_t o_probs = take(cur, sizeof(RpProb) * nj);
+    size_t o_ints = take(cur, 4 * ints.size() + 16);
+    size_t total = cur + 256;
+    char *base = nullptr;
+    int st = arena_reserve(total, &base);
+    if (st)
+        return st;
+    hipStream_t s = g_ctx.stream;
+    std::vector<RpProb> probs(nj);
+    for (size_t j = 0; j < nj; ++j) {
+        const RpHost &h = jobs[j];
+        RpProb &R = probs[j];
+        std::memset(&R, 0, sizeof(R));
+        R.n = h.n;
+        R.npts = (int)h.theta.size();
+        R.ground = h.ground;
+        R.ndir = h.ndir;
+        R.k = h.k;
+        R.eps_r = h.eps_r;
+        R.sigma = h.sigma;
+        R.seg = (const double *)(base + o_seg[j]);
+        R.cur = h.cur_dev ? h.cur_dev : (const double *)(base + o_cur[j]);
+        R.theta = (const double *)(base + o_th[j]);
+        R.phi = (const double *)(base + o_ph[j]);
+        R.field = (double *)(base + o_field[j]);
+        R.partial = (double *)(base + o_part[j]);
+        R.npartial = (h.ndir + 255) / 256;
+        R.total = (double *)(base + o_tot[j]);
+        R.out = (double *)(base + o_out[j]);
+        HIPCHK(hipMemcpyAsync(base + o_seg[j], h.seg.data(), 8 * h.seg.size(), hipMemcpyHostToDevice, s));
+        if (!h.cur_dev)
+            HIPCHK(hipMemcpyAsync(base + o_cur[j], h.cur.data(), 8 * h.cur.size(), hipMemcpyHostToDevice, s));
+        if (!h.theta.empty()) {
+            HIPCHK(hipMemcpyAsync(base + o_th[j], h.theta.data(), 8 * h.theta.size(), hipMemcpyHostToDevice, s));
+            HIPCHK(hipMemcpyAsync(base + o_ph[j], h.phi.data(), 8 * h.phi.size(), hipMemcpyHostToDevice, s));
+        }
+    }
+    HIPCHK(hipMemcpyAsync(base + o_probs, probs.data(), sizeof(RpProb) * nj, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(base + o_ints, ints.data(), 4 * ints.size(), hipMemcpyHostToDevice, s));
+    const RpProb *dp = (const RpProb *)(base + o_probs);
+    const int *wi = (const int *)(base + o_ints);
+    if (pts_cnt)
+        hipLaunchKernelGGL(k_rp_points, dim3((unsigned)pts_cnt), dim3(256), 0, s, dp, (const int2 *)(wi + pts_off));
+    if (norm_cnt)
+        hipLaunchKernelGGL(k_rp_norm, dim3((unsigned)norm_cnt), dim3(256), 0, s, dp, (const int2 *)(wi + norm_off));
+    hipLaunchKernelGGL(k_rp_finish, dim3((unsigned)nj), dim3(256), 0, s, dp, wi + all_off);
+    HIPCHK(hipGetLastError());
+    totals.assign(nj, 0.0);
+    for (size_t j = 0; j < nj; ++j) {
+        if (outs[j] && !jobs[j].theta.empty())
+            HIPCHK(hipMemcpyAsync(outs[j], base + o_out[j], 32 * jobs[j].theta.size(), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipMemcpyAsync(&totals[j], base + o_tot[j], 8, hipMemcpyDeviceToHost, s));
+    }
+    HIPCHK(hipStreamSynchronize(s));
+    return FNEC_OK;
+}
+
+int rp_host_from(RpHost &h, const fnec_seg *segs, const fnec_c64 *cur, size_t n, double freq_hz,
+                 const fnec_ground *ground, const double *theta, const double *phi, size_t npts)
+{
+    if (n == 0 || !segs || (!cur && !h.cur_dev) || (npts && (!theta || !phi))) {
+        set_error("empty or null input");
+        return FNEC_INVALID_ARG;
+    }
+    if (!(freq_hz > 0.0)) {
+        set_error("frequency must be positive");
+        return FNEC_INVALID_ARG;
+    }
+    h.n = (int)n;
+    h.seg.resize(n * kSegStride);
+    for (size_t i = 0; i < n; ++i) {
+        double *o = &h.seg[i * kSegStride];
+        for (int c = 0; c < 3; ++c) {
+            o[c] = segs[i].mid[c];
+            o[3 + c] = segs[i].dir[c];
+        }
+        o[6] = segs[i].length;
+        o[7] = segs[i].radius;
+    }
+    if (cur) {
+        h.cur.resize(2 * n);
+        for (size_t i = 0; i < n; ++i) {
+            h.cur[2 * i] = cur[i].re;
+            h.cur[2 * i + 1] = cur[i].im;
+        }
+    }
+    h.theta.assign(theta, theta + npts);
+    h.phi.assign(phi, phi + npts);
+    h.ground = 0;
+    h.eps_r = h.sigma = 0.0;
+    if (ground && ground->kind == FNEC_GROUND_PEC)
+        h.ground = 1;
+    else if (ground && ground->kind == FNEC_GROUND_FINITE) {
+        h.ground = 2;
+        h.eps_r = ground->eps_r;
+        h.sigma = ground->sigma;
+    }
+    h.ndir = (h.ground ? 91 : 181) * 360;
+    const double lambda = kC0 / freq_hz;
+    h.k = 2.0 * kPi / lambda;
+    return FNEC_OK;
+}
+
+int mfma_probe(const double *a, const double *b, double *out)
+{
+    char *base = nullptr;
+    int st = arena_reserve(4096 * 4, &base);
+    if (st)
+        return st;
+    hipStream_t s = g_ctx.stream;
+    HIPCHK(hipMemcpyAsync(base, a, 64 * 8, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(base + 1024, b, 64 * 8, hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_mfma_probe, dim3(1), dim3(64), 0, s, (const double *)base,
+                       (const double *)(base + 1024), (double *)(base + 2048));
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(out, base + 2048, 256 * 8, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    return FNEC_OK;
+}
+
+std::mutex &global_mutex() { return g_mu; }
+
+} // namespace fnec

@@ -1,0 +1,1069 @@
+// HIP kernels for the Hallén path on gfx950 (CDNA4).
+//
+//   k_fill_z        Z fill (matrix.rs:186-218, 297-331): FP64 Gauss-Legendre
+//                   reduced-kernel quadrature + PEC / scalar-Gamma image.
+//   k_gram          G = Z^H Z on v_mfma_f64_16x16x4f64, lower-triangle tiles.
+//   k_gram_cols     homogeneous columns, M^H y (linear.rs:735-780).
+//   k_gram_fix      W x W block, constraint rows, lambda (linear.rs:744-785).
+//   k_lu_panel      panel factorisation with the reference pivot rule
+//                   (last row of maximal hypot, linear.rs:1345-1356).
+//   k_lu_trsm       row interchanges + unit-lower triangular solve of U12.
+//   k_lu_update     trailing update A22 -= L21 U12 on MFMA.
+//   k_backsub       U x = b' (blocked, one workgroup per problem).
+//   k_residual_*    ||Z I - C cos - rhs|| and diag spread (solve_session.rs:118-207).
+//   k_rp_*          far field + normalisation integral (farfield.rs:265-647).
+#include <hip/hip_runtime.h>
+#include <math.h>
+
+#include "common.h"
+
+namespace fnec {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ d4 mfma(double a, double b, d4 c)
+{
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+// v_mfma_f64_16x16x4f64 operand layout: lane l supplies A[l%16][l/16] and
+// B[l/16][l%16]; accumulator item v of lane l is D[(l/16) + 4v][l%16].
+__device__ __forceinline__ int acc_row(int lane, int v) { return (lane >> 4) + 4 * v; }
+
+// Gauss-Legendre nodes and weights, the 15-digit literals of matrix.rs:39-72.
+__constant__ double c_gl4n[4] = {-0.861136311594953, -0.339981043584856, 0.339981043584856,
+                                 0.861136311594953};
+__constant__ double c_gl4w[4] = {0.347854845137454, 0.652145154862626, 0.652145154862626,
+                                 0.347854845137454};
+__constant__ double c_gl8n[8] = {-0.960289856497536, -0.796666477413627, -0.525532409916329,
+                                 -0.183434642495650, 0.183434642495650,  0.525532409916329,
+                                 0.796666477413627,  0.960289856497536};
+__constant__ double c_gl8w[8] = {0.101228536290376, 0.222381034453374, 0.313706645877887,
+                                 0.362683783378362, 0.362683783378362, 0.313706645877887,
+                                 0.222381034453374, 0.101228536290376};
+
+// ---------------------------------------------------------------------------
+// Z fill
+// ---------------------------------------------------------------------------
+
+// Off-diagonal element integral  half * sum_m w_m exp(-jkR_m)/R_m  over the
+// source segment (src mid s, dir d, half length h, radius a) seen from obs.
+__device__ __forceinline__ void quad8(double ox, double oy, double oz, double sx, double sy,
+                                      double sz, double dx, double dy, double dz, double h,
+                                      double a2, double k, double &re, double &im)
+{
+    double sr = 0.0, si = 0.0;
+#pragma unroll
+    for (int m = 0; m < 8; ++m) {
+        double sc = c_gl8n[m] * h;
+        double px = ox - (sx + sc * dx);
+        double py = oy - (sy + sc * dy);
+        double pz = oz - (sz + sc * dz);
+        double r = sqrt(px * px + py * py + pz * pz + a2);
+        double s, c;
+        sincos(k * r, &s, &c);
+        double wr = c_gl8w[m] / r;
+        sr += wr * c;
+        si -= wr * s;
+    }
+    re = sr * h;
+    im = si * h;
+}
+
+// Self element: 4-point GL on G - 1/R plus 2 ln((h + R_end)/a) (matrix.rs:302-316).
+__device__ __forceinline__ void self_term(double h, double a, double k, double &re, double &im)
+{
+    double sr = 0.0, si = 0.0;
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+        double l = c_gl4n[m] * h;
+        double r = sqrt(l * l + a * a);
+        double s, c;
+        sincos(k * r, &s, &c);
+        double inv = 1.0 / r;
+        sr += c_gl4w[m] * (c * inv - inv);
+        si -= c_gl4w[m] * (s * inv);
+    }
+    double rend = sqrt(h * h + a * a);
+    re = sr * h + 2.0 * log((h + rend) / a);
+    im = si * h;
+}
+
+constexpr int kFillRows = 64;
+constexpr int kFillCols = 16;
+
+// Work item: (problem, row tile, column tile). Each thread owns one
+// observation row and four source columns; a wavefront writes 64 consecutive
+// rows of one column, so the column-major stores are fully coalesced.
+__global__ __launch_bounds__(256) void k_fill_z(const Prob *__restrict__ probs,
+                                                const int4 *__restrict__ work)
+{
+    const int4 wk = work[blockIdx.x];
+    const Prob &P = probs[wk.x];
+    const int i0 = wk.y * kFillRows, j0 = wk.z * kFillCols;
+    __shared__ double so[kFillRows][6];
+    __shared__ double ss[kFillCols][8];
+    const int t = threadIdx.x;
+    for (int e = t; e < kFillRows * 6; e += 256) {
+        int row = e / 6, c = e % 6;
+        int i = i0 + row;
+        so[row][c] = i < P.n ? P.seg[(size_t)i * kSegStride + c] : 0.0;
+    }
+    for (int e = t; e < kFillCols * 8; e += 256) {
+        int col = e / 8, c = e % 8;
+        int j = j0 + col;
+        ss[col][c] = j < P.n ? P.seg[(size_t)j * kSegStride + c] : 1.0;
+    }
+    __syncthreads();
+    const int row = t & 63;
+    const int i = i0 + row;
+    if (i >= P.n)
+        return;
+    const double ox = so[row][0], oy = so[row][1], oz = so[row][2];
+    const double odx = so[row][3], ody = so[row][4], odz = so[row][5];
+    const double k = P.k;
+    for (int q = 0; q < 4; ++q) {
+        const int col = (t >> 6) * 4 + q;
+        const int j = j0 + col;
+        if (j >= P.n)
+            break;
+        const double sx = ss[col][0], sy = ss[col][1], sz = ss[col][2];
+        const double dx = ss[col][3], dy = ss[col][4], dz = ss[col][5];
+        const double h = ss[col][6] * 0.5, a = ss[col][7];
+        double re, im;
+        if (i == j)
+            self_term(h, a, k, re, im);
+        else
+            quad8(ox, oy, oz, sx, sy, sz, dx, dy, dz, h, a * a, k, re, im);
+        const double ca = odx * dx + ody * dy + odz * dz;
+        re *= ca;
+        im *= ca;
+        if (P.ground != 0) {
+            // image: z -> -z, direction (-dx, -dy, +dz) (matrix.rs:265-277)
+            double ir, ii;
+            quad8(ox, oy, oz, sx, sy, -sz, -dx, -dy, dz, h, a * a, k, ir, ii);
+            const double ica = odx * (-dx) + ody * (-dy) + odz * dz;
+            ir *= ica;
+            ii *= ica;
+            if (P.ground == 2) {
+                const double gr = P.gam_re * ir - P.gam_im * ii;
+                const double gi = P.gam_re * ii + P.gam_im * ir;
+                ir = gr;
+                ii = gi;
+            }
+            re += ir;
+            im += ii;
+        }
+        const size_t o = (size_t)j * P.ldz + i;
+        P.zr[o] = re;
+        P.zi[o] = im;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Gram G = Z^H Z (complex HERK on FP64 MFMA)
+// ---------------------------------------------------------------------------
+
+// Work item: (problem, bi, bj) with bi >= bj, 64x64 output tile. Four waves
+// in a 2x2 arrangement each own a 32x32 block = 2x2 MFMA 16x16 tiles, with
+// separate real and imaginary accumulators:
+//   Re += Zr_i Zr_j + Zi_i Zi_j ;  Im += Zr_i Zi_j - Zi_i Zr_j
+constexpr int kLdsPad = 4;
+
+__global__ __launch_bounds__(256) void k_gram(const Prob *__restrict__ probs,
+                                              const int4 *__restrict__ work)
+{
+    const int4 wk = work[blockIdx.x];
+    const Prob &P = probs[wk.x];
+    const int i0 = wk.y * kTile, j0 = wk.z * kTile;
+    const int n = P.n;
+    __shared__ double sa_r[kGramK][kTile + kLdsPad], sa_i[kGramK][kTile + kLdsPad];
+    __shared__ double sb_r[kGramK][kTile + kLdsPad], sb_i[kGramK][kTile + kLdsPad];
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const int wr = wave >> 1, wc = wave & 1;
+    d4 accr[2][2], acci[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+            accr[a][b] = d4{0.0, 0.0, 0.0, 0.0};
+            acci[a][b] = d4{0.0, 0.0, 0.0, 0.0};
+        }
+    const int lrow = lane & 15, lk = lane >> 4;
+    for (int k0 = 0; k0 < n; k0 += kGramK) {
+        // 64 columns x 16 rows per plane and operand; 4 elements per thread.
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int e = t + 256 * q;
+            const int c = e >> 4, kk = e & 15;
+            const int kr = k0 + kk;
+            const int ci = i0 + c, cj = j0 + c;
+            double ar = 0.0, ai = 0.0, br = 0.0, bi = 0.0;
+            if (kr < n) {
+                if (ci < n) {
+                    ar = P.zr[(size_t)ci * P.ldz + kr];
+                    ai = P.zi[(size_t)ci * P.ldz + kr];
+                }
+                if (cj < n) {
+                    br = P.zr[(size_t)cj * P.ldz + kr];
+                    bi = P.zi[(size_t)cj * P.ldz + kr];
+                }
+            }
+            sa_r[kk][c] = ar;
+            sa_i[kk][c] = ai;
+            sb_r[kk][c] = br;
+            sb_i[kk][c] = bi;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < kGramK; kk += 4) {
+            double xr[2], xi[2], yr[2], yi[2];
+#pragma unroll
+            for (int a = 0; a < 2; ++a) {
+                xr[a] = sa_r[kk + lk][wr * 32 + a * 16 + lrow];
+                xi[a] = sa_i[kk + lk][wr * 32 + a * 16 + lrow];
+                yr[a] = sb_r[kk + lk][wc * 32 + a * 16 + lrow];
+                yi[a] = sb_i[kk + lk][wc * 32 + a * 16 + lrow];
+            }
+#pragma unroll
+            for (int a = 0; a < 2; ++a)
+#pragma unroll
+                for (int b = 0; b < 2; ++b) {
+                    accr[a][b] = mfma(xr[a], yr[b], accr[a][b]);
+                    accr[a][b] = mfma(xi[a], yi[b], accr[a][b]);
+                    acci[a][b] = mfma(xr[a], yi[b], acci[a][b]);
+                    acci[a][b] = mfma(-xi[a], yr[b], acci[a][b]);
+                }
+        }
+        __syncthreads();
+    }
+    const bool diag = wk.y == wk.z;
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int gi = i0 + wr * 32 + a * 16 + acc_row(lane, v);
+                const int gj = j0 + wc * 32 + b * 16 + lrow;
+                if (gi >= n || gj >= n)
+                    continue;
+                if (diag && gi < gj)
+                    continue;
+                const double re = accr[a][b][v], im = acci[a][b][v];
+                P.ar[(size_t)gj * P.lda + gi] = re;
+                P.ai[(size_t)gj * P.lda + gi] = im;
+                if (gi != gj) {
+                    P.ar[(size_t)gi * P.lda + gj] = re;
+                    P.ai[(size_t)gi * P.lda + gj] = -im;
+                }
+            }
+}
+
+__device__ __forceinline__ double wave_sum(double v)
+{
+    // Fixed butterfly order: deterministic for a given lane assignment.
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1)
+        v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// One wavefront per Z column i: G[i][n+w] = -sum_{r in w} conj(Z_ri) cos_r,
+// its Hermitian mirror, and (M^H y)_i = sum_r conj(Z_ri) rhs_r.
+__global__ __launch_bounds__(256) void k_gram_cols(const Prob *__restrict__ probs,
+                                                   const int2 *__restrict__ work)
+{
+    const int2 wk = work[blockIdx.x];
+    const Prob &P = probs[wk.x];
+    const int lane = threadIdx.x & 63;
+    const int i = wk.y * 4 + (threadIdx.x >> 6);
+    if (i >= P.n)
+        return;
+    const double *zr = P.zr + (size_t)i * P.ldz, *zi = P.zi + (size_t)i * P.ldz;
+    const int s = P.s;
+    // M^H y over the n segment rows (constraint rows carry y = 0)
+    double yr = 0.0, yi = 0.0;
+    for (int r = lane; r < P.n; r += 64) {
+        const double a = zr[r], b = zi[r];
+        const double c = P.rhs[2 * r], d = P.rhs[2 * r + 1];
+        yr += a * c + b * d;
+        yi += a * d - b * c;
+    }
+    yr = wave_sum(yr);
+    yi = wave_sum(yi);
+    if (lane == 0) {
+        P.ar[(size_t)s * P.lda + i] = yr;
+        P.ai[(size_t)s * P.lda + i] = yi;
+    }
+    // homogeneous columns, accumulated run by run in row order
+    for (int w = 0; w < P.w; ++w) {
+        double gr = 0.0, gi = 0.0;
+        for (int q = 0; q < P.nruns; ++q) {
+            const int rb = P.runs[3 * q], re = P.runs[3 * q + 1], col = P.runs[3 * q + 2];
+            if (col != w)
+                continue;
+            for (int r = rb + lane; r < re; r += 64) {
+                const double c = P.cosv[r];
+                gr -= zr[r] * c;
+                gi += zi[r] * c;
+            }
+        }
+        gr = wave_sum(gr);
+        gi = wave_sum(gi);
+        if (lane == 0) {
+            const int cw = P.n + w;
+            P.ar[(size_t)cw * P.lda + i] = gr; // G[i][n+w]
+            P.ai[(size_t)cw * P.lda + i] = gi;
+            P.ar[(size_t)i * P.lda + cw] = gr; // G[n+w][i] = conj
+            P.ai[(size_t)i * P.lda + cw] = -gi;
+        }
+    }
+}
+
+// One workgroup per problem: the W x W block, (M^H y) for the homogeneous
+// columns, the sparse constraint rows (K^H K) and lambda on the diagonal.
+__global__ __launch_bounds__(256) void k_gram_fix(const Prob *__restrict__ probs,
+                                                  const int *__restrict__ plist)
+{
+    const Prob &P = probs[plist[blockIdx.x]];
+    const int t = threadIdx.x;
+    const int n = P.n, s = P.s;
+    __shared__ double red[256];
+    for (int w = 0; w < P.w; ++w) {
+        double c2 = 0.0, yr = 0.0, yi = 0.0;
+        for (int q = 0; q < P.nruns; ++q) {
+            const int rb = P.runs[3 * q], re = P.runs[3 * q + 1], col = P.runs[3 * q + 2];
+            if (col != w)
+                continue;
+            for (int r = rb + t; r < re; r += 256) {
+                const double c = P.cosv[r];
+                c2 += c * c;
+                yr -= c * P.rhs[2 * r];
+                yi -= c * P.rhs[2 * r + 1];
+            }
+        }
+        double vals[3] = {c2, yr, yi};
+        for (int v = 0; v < 3; ++v) {
+            red[t] = vals[v];
+            __syncthreads();
+            for (int off = 128; off > 0; off >>= 1) {
+                if (t < off)
+                    red[t] += red[t + off];
+                __syncthreads();
+            }
+            vals[v] = red[0];
+            __syncthreads();
+        }
+        if (t == 0) {
+            const int cw = n + w;
+            for (int v = 0; v < P.w; ++v) {
+                P.ar[(size_t)(n + v) * P.lda + cw] = v == w ? vals[0] : 0.0;
+                P.ai[(size_t)(n + v) * P.lda + cw] = 0.0;
+            }
+            P.ar[(size_t)s * P.lda + cw] = vals[1];
+            P.ai[(size_t)s * P.lda + cw] = vals[2];
+        }
+    }
+    __syncthreads();
+    if (t == 0) {
+        for (int c = 0; c < P.ncons; ++c) {
+            const int a = P.cons[2 * c], b = P.cons[2 * c + 1];
+            const double sg = P.cons_sign[c];
+            if (b < 0 || b == a) {
+                const double v = b < 0 ? 1.0 : sg * sg;
+                P.ar[(size_t)a * P.lda + a] += v;
+            } else {
+                P.ar[(size_t)a * P.lda + a] += 1.0;
+                P.ar[(size_t)b * P.lda + a] += sg; // G[a][b]
+                P.ar[(size_t)a * P.lda + b] += sg; // G[b][a]
+                P.ar[(size_t)b * P.lda + b] += sg * sg;
+            }
+        }
+    }
+    __syncthreads();
+    for (int i = t; i < s; i += 256)
+        P.ar[(size_t)i * P.lda + i] += kLambda;
+}
+
+// ---------------------------------------------------------------------------
+// LU with partial pivoting on [G | M^H y]
+// ---------------------------------------------------------------------------
+
+constexpr int kPanelThreads = 512;
+
+// One workgroup per problem factors columns [p, p + jb) of rows [p, s).
+__global__ __launch_bounds__(kPanelThreads) void k_lu_panel(const Prob *__restrict__ probs,
+                                                            const int *__restrict__ plist, int p)
+{
+    const Prob &P = probs[plist[blockIdx.x]];
+    const int s = P.s;
+    if (P.info[0] != 0 || p >= s)
+        return;
+    const int jb = min(kPanel, s - p);
+    const int t = threadIdx.x;
+    const int lda = P.lda;
+    double *ar = P.ar, *ai = P.ai;
+    __shared__ double rv[kPanelThreads];
+    __shared__ int ri[kPanelThreads];
+    __shared__ double prow_r[kPanel], prow_i[kPanel];
+    __shared__ int s_piv;
+    __shared__ double s_best;
+    for (int j = 0; j < jb; ++j) {
+        const int col = p + j;
+        const double *cr = ar + (size_t)col * lda, *ci = ai + (size_t)col * lda;
+        double best = -1.0;
+        int bidx = col;
+        for (int r = col + t; r < s; r += kPanelThreads) {
+            const double v = hypot(cr[r], ci[r]);
+            if (!(v < best)) { // later rows win ties (Iterator::max_by)
+                best = v;
+                bidx = r;
+            }
+        }
+        rv[t] = best;
+        ri[t] = bidx;
+        __syncthreads();
+        for (int off = kPanelThreads / 2; off > 0; off >>= 1) {
+            if (t < off) {
+                const double v2 = rv[t + off];
+                const int i2 = ri[t + off];
+                if (v2 > rv[t] || (v2 == rv[t] && i2 > ri[t])) {
+                    rv[t] = v2;
+                    ri[t] = i2;
+                }
+            }
+            __syncthreads();
+        }
+        if (t == 0) {
+            s_piv = ri[0];
+            s_best = rv[0];
+        }
+        __syncthreads();
+        const int piv = s_piv;
+        if (s_best < kSingular) { // NaN passes, as in linear.rs:1354
+            if (t == 0)
+                P.info[0] = 1;
+            return;
+        }
+        if (piv != col) {
+            for (int c = p + t; c < p + jb; c += kPanelThreads) {
+                const size_t o1 = (size_t)c * lda + col, o2 = (size_t)c * lda + piv;
+                const double xr = ar[o1], xi = ai[o1];
+                ar[o1] = ar[o2];
+                ai[o1] = ai[o2];
+                ar[o2] = xr;
+                ai[o2] = xi;
+            }
+            if (t == 0)
+                P.info[1] += 1;
+        }
+        if (t == 0)
+            P.ipiv[col] = piv;
+        __syncthreads();
+        for (int c = col + 1 + t; c < p + jb; c += kPanelThreads) {
+            prow_r[c - col - 1] = ar[(size_t)c * lda + col];
+            prow_i[c - col - 1] = ai[(size_t)c * lda + col];
+        }
+        __syncthreads();
+        const double dr = cr[col], di = ci[col];
+        const double nsq = dr * dr + di * di;
+        const double ir = dr / nsq, ii = -di / nsq; // 1/d
+        const int ncols = p + jb - col - 1;
+        for (int r = col + 1 + t; r < s; r += kPanelThreads) {
+            const double xr = cr[r], xi = ci[r];
+            const double lr = xr * ir - xi * ii, li = xr * ii + xi * ir;
+            ar[(size_t)col * lda + r] = lr;
+            ai[(size_t)col * lda + r] = li;
+            for (int c = 0; c < ncols; ++c) {
+                const size_t o = (size_t)(col + 1 + c) * lda + r;
+                const double ur = prow_r[c], ui = prow_i[c];
+                ar[o] -= lr * ur - li * ui;
+                ai[o] -= lr * ui + li * ur;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// Row interchanges of the panel and U12 = L11^{-1} A12 for columns
+// [p + jb, s] (the augmented right-hand side included). One thread per column.
+__global__ __launch_bounds__(64) void k_lu_trsm(const Prob *__restrict__ probs,
+                                                const int2 *__restrict__ work, int p)
+{
+    const int2 wk = work[blockIdx.x];
+    const Prob &P = probs[wk.x];
+    const int s = P.s;
+    if (P.info[0] != 0 || p >= s)
+        return;
+    const int jb = min(kPanel, s - p);
+    const int lda = P.lda;
+    __shared__ double lr[kPanel][kPanel + 1], li[kPanel][kPanel + 1];
+    const int t = threadIdx.x;
+    for (int e = t; e < jb * jb; e += 64) {
+        const int rr = e % jb, cc = e / jb;
+        lr[rr][cc] = P.ar[(size_t)(p + cc) * lda + p + rr];
+        li[rr][cc] = P.ai[(size_t)(p + cc) * lda + p + rr];
+    }
+    __syncthreads();
+    const int c = p + jb + wk.y * 64 + t;
+    if (c > s)
+        return;
+    double *cr = P.ar + (size_t)c * lda, *ci = P.ai + (size_t)c * lda;
+    for (int j = 0; j < jb; ++j) {
+        const int piv = P.ipiv[p + j];
+        if (piv != p + j) {
+            const double xr = cr[p + j], xi = ci[p + j];
+            cr[p + j] = cr[piv];
+            ci[p + j] = ci[piv];
+            cr[piv] = xr;
+            ci[piv] = xi;
+        }
+    }
+    double ur[kPanel], ui[kPanel];
+#pragma unroll
+    for (int j = 0; j < kPanel; ++j) {
+        if (j < jb) {
+            double vr = cr[p + j], vi = ci[p + j];
+            for (int m = 0; m < j; ++m) {
+                vr -= lr[j][m] * ur[m] - li[j][m] * ui[m];
+                vi -= lr[j][m] * ui[m] + li[j][m] * ur[m];
+            }
+            ur[j] = vr;
+            ui[j] = vi;
+            cr[p + j] = vr;
+            ci[p + j] = vi;
+        }
+    }
+}
+
+// Trailing update A22 -= L21 U12 over rows [p+jb, s) x columns [p+jb, s].
+// Work item: (problem, row tile, column tile). Same wave layout as k_gram;
+// K = jb <= 32 is staged whole in LDS.
+__global__ __launch_bounds__(256) void k_lu_update(const Prob *__restrict__ probs,
+                                                   const int4 *__restrict__ work, int p)
+{
+    const int4 wk = work[blockIdx.x];
+    const Prob &P = probs[wk.x];
+    const int s = P.s;
+    if (P.info[0] != 0 || p >= s)
+        return;
+    const int jb = min(kPanel, s - p);
+    const int lda = P.lda;
+    const int r0 = p + jb + wk.y * kTile, c0 = p + jb + wk.z * kTile;
+    __shared__ double sa_r[kPanel][kTile + kLdsPad], sa_i[kPanel][kTile + kLdsPad];
+    __shared__ double sb_r[kPanel][kTile + kLdsPad], sb_i[kPanel][kTile + kLdsPad];
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const int wr = wave >> 1, wc = wave & 1;
+    // A tile: rows r0.., k columns p..p+jb (column-major: contiguous rows)
+    for (int e = t; e < kPanel * kTile; e += 256) {
+        const int kk = e / kTile, rr = e % kTile;
+        const int gr = r0 + rr;
+        double vr = 0.0, vi = 0.0;
+        if (kk < jb && gr < s) {
+            vr = P.ar[(size_t)(p + kk) * lda + gr];
+            vi = P.ai[(size_t)(p + kk) * lda + gr];
+        }
+        sa_r[kk][rr] = vr;
+        sa_i[kk][rr] = vi;
+    }
+    // B tile: k rows p..p+jb, columns c0.. (contiguous k within a column)
+    for (int e = t; e < kPanel * kTile; e += 256) {
+        const int cc = e / kPanel, kk = e % kPanel;
+        const int gc = c0 + cc;
+        double vr = 0.0, vi = 0.0;
+        if (kk < jb && gc <= s) {
+            vr = P.ar[(size_t)gc * lda + p + kk];
+            vi = P.ai[(size_t)gc * lda + p + kk];
+        }
+        sb_r[kk][cc] = vr;
+        sb_i[kk][cc] = vi;
+    }
+    __syncthreads();
+    d4 accr[2][2], acci[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+            accr[a][b] = d4{0.0, 0.0, 0.0, 0.0};
+            acci[a][b] = d4{0.0, 0.0, 0.0, 0.0};
+        }
+    const int lrow = lane & 15, lk = lane >> 4;
+#pragma unroll
+    for (int kk = 0; kk < kPanel; kk += 4) {
+        double xr[2], xi[2], yr[2], yi[2];
+#pragma unroll
+        for (int a = 0; a < 2; ++a) {
+            xr[a] = sa_r[kk + lk][wr * 32 + a * 16 + lrow];
+            xi[a] = sa_i[kk + lk][wr * 32 + a * 16 + lrow];
+            yr[a] = sb_r[kk + lk][wc * 32 + a * 16 + lrow];
+            yi[a] = sb_i[kk + lk][wc * 32 + a * 16 + lrow];
+        }
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int b = 0; b < 2; ++b) {
+                accr[a][b] = mfma(xr[a], yr[b], accr[a][b]);
+                accr[a][b] = mfma(-xi[a], yi[b], accr[a][b]);
+                acci[a][b] = mfma(xr[a], yi[b], acci[a][b]);
+                acci[a][b] = mfma(xi[a], yr[b], acci[a][b]);
+            }
+    }
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int gi = r0 + wr * 32 + a * 16 + acc_row(lane, v);
+                const int gj = c0 + wc * 32 + b * 16 + lrow;
+                if (gi >= s || gj > s)
+                    continue;
+                const size_t o = (size_t)gj * lda + gi;
+                P.ar[o] -= accr[a][b][v];
+                P.ai[o] -= acci[a][b][v];
+            }
+}
+
+// Back substitution U x = b' with b' the transformed augmented column.
+// Blocks of 64 rows from the bottom; one workgroup per problem.
+constexpr int kBsBlock = 64;
+
+__global__ __launch_bounds__(256) void k_backsub(const Prob *__restrict__ probs,
+                                                 const int *__restrict__ plist)
+{
+    const Prob &P = probs[plist[blockIdx.x]];
+    if (P.info[0] != 0)
+        return;
+    const int s = P.s, lda = P.lda, t = threadIdx.x;
+    double *x = P.x;
+    for (int i = t; i < s; i += 256) {
+        x[2 * i] = P.ar[(size_t)s * lda + i];
+        x[2 * i + 1] = P.ai[(size_t)s * lda + i];
+    }
+    __syncthreads();
+    __shared__ double xb_r[kBsBlock], xb_i[kBsBlock];
+    for (int r1 = s; r1 > 0; r1 -= kBsBlock) {
+        const int r0 = max(0, r1 - kBsBlock);
+        // triangle [r0, r1): serial in i, parallel over the rows above it
+        for (int i = r1 - 1; i >= r0; --i) {
+            if (t == 0) {
+                const double dr = P.ar[(size_t)i * lda + i], di = P.ai[(size_t)i * lda + i];
+                const double nsq = dr * dr + di * di;
+                const double br = x[2 * i], bi = x[2 * i + 1];
+                const double vr = (br * dr + bi * di) / nsq, vi = (bi * dr - br * di) / nsq;
+                x[2 * i] = vr;
+                x[2 * i + 1] = vi;
+                xb_r[i - r0] = vr;
+                xb_i[i - r0] = vi;
+            }
+            __syncthreads();
+            const double vr = xb_r[i - r0], vi = xb_i[i - r0];
+            for (int m = r0 + t; m < i; m += 256) {
+                const double ur = P.ar[(size_t)i * lda + m], ui = P.ai[(size_t)i * lda + m];
+                x[2 * m] -= ur * vr - ui * vi;
+                x[2 * m + 1] -= ur * vi + ui * vr;
+            }
+            __syncthreads();
+        }
+        // rows above the block
+        for (int m = t; m < r0; m += 256) {
+            double sr = 0.0, si = 0.0;
+            for (int i = r0; i < r1; ++i) {
+                const double ur = P.ar[(size_t)i * lda + m], ui = P.ai[(size_t)i * lda + m];
+                const double vr = xb_r[i - r0], vi = xb_i[i - r0];
+                sr += ur * vr - ui * vi;
+                si += ur * vi + ui * vr;
+            }
+            x[2 * m] -= sr;
+            x[2 * m + 1] -= si;
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Residual ||Z I - C cos - rhs||, ||rhs|| and the diagonal spread
+// ---------------------------------------------------------------------------
+
+constexpr int kResRows = 256;
+
+// Work item: (problem, row block). Thread per row; partial sums per block.
+__global__ __launch_bounds__(256) void k_residual_rows(const Prob *__restrict__ probs,
+                                                       const int2 *__restrict__ work)
+{
+    const int2 wk = work[blockIdx.x];
+    const Prob &P = probs[wk.x];
+    if (P.info[0] != 0)
+        return;
+    const int t = threadIdx.x;
+    const int row = wk.y * kResRows + t;
+    __shared__ double red[3][256];
+    double r2 = 0.0, b2 = 0.0, dmag = 0.0;
+    if (row < P.n) {
+        double sr = 0.0, si = 0.0;
+        for (int c = 0; c < P.n; ++c) {
+            const double zr = P.zr[(size_t)c * P.ldz + row], zi = P.zi[(size_t)c * P.ldz + row];
+            const double xr = P.x[2 * c], xi = P.x[2 * c + 1];
+            sr += zr * xr - zi * xi;
+            si += zr * xi + zi * xr;
+        }
+        int col = 0;
+        for (int q = 0; q < P.nruns; ++q)
+            if (row >= P.runs[3 * q] && row < P.runs[3 * q + 1])
+                col = P.runs[3 * q + 2];
+        const double cr = P.x[2 * (P.n + col)], ci = P.x[2 * (P.n + col) + 1];
+        const double cv = P.cosv[row];
+        const double er = sr - cr * cv - P.rhs[2 * row];
+        const double ei = si - ci * cv - P.rhs[2 * row + 1];
+        r2 = er * er + ei * ei;
+        b2 = P.rhs[2 * row] * P.rhs[2 * row] + P.rhs[2 * row + 1] * P.rhs[2 * row + 1];
+        dmag = hypot(P.zr[(size_t)row * P.ldz + row], P.zi[(size_t)row * P.ldz + row]);
+    }
+    red[0][t] = r2;
+    red[1][t] = b2;
+    red[2][t] = row < P.n ? dmag : 0.0;
+    __syncthreads();
+    // sums for the norms; max for the diagonal, min kept separately below
+    for (int off = 128; off > 0; off >>= 1) {
+        if (t < off) {
+            red[0][t] += red[0][t + off];
+            red[1][t] += red[1][t + off];
+            red[2][t] = fmax(red[2][t], red[2][t + off]);
+        }
+        __syncthreads();
+    }
+    __shared__ double mn[256];
+    mn[t] = row < P.n ? dmag : INFINITY;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if (t < off)
+            mn[t] = fmin(mn[t], mn[t + off]);
+        __syncthreads();
+    }
+    if (t == 0) {
+        double *o = P.scratch + 4 * (size_t)wk.y;
+        o[0] = red[0][0];
+        o[1] = red[1][0];
+        o[2] = red[2][0];
+        o[3] = mn[0];
+    }
+}
+
+__global__ __launch_bounds__(64) void k_residual_finish(const Prob *__restrict__ probs,
+                                                        const int *__restrict__ plist)
+{
+    const Prob &P = probs[plist[blockIdx.x]];
+    if (threadIdx.x != 0)
+        return;
+    const int nb = (P.n + kResRows - 1) / kResRows;
+    double r2 = 0.0, b2 = 0.0, mx = 0.0, mnv = INFINITY;
+    for (int b = 0; b < nb; ++b) {
+        const double *o = P.scratch + 4 * (size_t)b;
+        r2 += o[0];
+        b2 += o[1];
+        mx = fmax(mx, o[2]);
+        mnv = fmin(mnv, o[3]);
+    }
+    const double res = sqrt(r2), den = sqrt(b2);
+    P.stats[0] = res;
+    P.stats[1] = den > 0.0 ? res / den : res;
+    double spread;
+    if (!isfinite(mx) || !isfinite(mnv))
+        spread = NAN;
+    else if (mx == 0.0)
+        spread = 0.0;
+    else
+        spread = mx / fmax(mnv, 1e-30);
+    P.stats[2] = spread;
+}
+
+// ---------------------------------------------------------------------------
+// Radiation pattern
+// ---------------------------------------------------------------------------
+
+constexpr int kRpThreads = 256;
+
+__device__ __forceinline__ void cdiv(double ar, double ai, double br, double bi, double &cr,
+                                     double &ci)
+{
+    const double ns = br * br + bi * bi;
+    cr = (ar * br + ai * bi) / ns;
+    ci = (ai * br - ar * bi) / ns;
+}
+
+__device__ __forceinline__ void csqrt_polar(double re, double im, double &sr, double &si)
+{
+    if (im == 0.0) {
+        if (!signbit(re)) {
+            sr = sqrt(re);
+            si = im;
+        } else {
+            sr = 0.0;
+            si = signbit(im) ? -sqrt(-re) : sqrt(-re);
+        }
+        return;
+    }
+    if (re == 0.0) {
+        const double x = sqrt(fabs(im) / 2.0);
+        sr = x;
+        si = signbit(im) ? -x : x;
+        return;
+    }
+    const double r = hypot(re, im), th = atan2(im, re);
+    const double rs = sqrt(r);
+    double s, c;
+    sincos(th / 2.0, &s, &c);
+    sr = rs * c;
+    si = rs * s;
+}
+
+// Fresnel coefficients (farfield.rs:333-344).
+__device__ void fresnel(double th_deg, double eps_r, double sigma, double k, double &gvr,
+                        double &gvi, double &ghr, double &ghi)
+{
+    const double omega = k * kC0;
+    const double er = eps_r, ei = -sigma / (omega * kEps0FarField);
+    const double th = th_deg * (kPi / 180.0);
+    double st, ct;
+    sincos(th, &st, &ct);
+    const double sin2 = st * st;
+    double rr, ri;
+    csqrt_polar(er - sin2, ei, rr, ri);
+    const double ecr = er * ct, eci = ei * ct;
+    cdiv(ecr - rr, eci - ri, ecr + rr, eci + ri, gvr, gvi);
+    cdiv(ct - rr, -ri, ct + rr, ri, ghr, ghi);
+}
+
+// Segments staged in LDS as {mid x,y,z, dir x,y,z, length, I re, I im}.
+constexpr int kRpStage = 9;
+constexpr int kRpChunk = 256;
+
+struct Dir {
+    double rx, ry, rz, tx, ty, tz, px, py;
+};
+
+__device__ __forceinline__ Dir unit_vectors(double th_deg, double ph_deg)
+{
+    const double th = th_deg * (kPi / 180.0), ph = ph_deg * (kPi / 180.0);
+    double st, ct, sp, cp;
+    sincos(th, &st, &ct);
+    sincos(ph, &sp, &cp);
+    Dir d;
+    d.rx = st * cp; d.ry = st * sp; d.rz = ct;
+    d.tx = ct * cp; d.ty = ct * sp; d.tz = -st;
+    d.px = -sp; d.py = cp;
+    return d;
+}
+
+// Accumulate one LDS chunk of segments into (F_theta, F_phi) for direction u,
+// in segment index order (farfield.rs:277-288, :526-543, :368-385).
+__device__ __forceinline__ void ff_chunk(const double *__restrict__ sg, int cnt, double k,
+                                         int ground, const Dir &u, double gvr, double gvi,
+                                         double ghr, double ghi, double &ftr, double &fti,
+                                         double &fpr, double &fpi)
+{
+    for (int m = 0; m < cnt; ++m) {
+        const double *g = sg + m * kRpStage;
+        const double mx = g[0], my = g[1], mz = g[2], dx = g[3], dy = g[4], dz = g[5];
+        const double len = g[6], ir = g[7], ii = g[8];
+        double s, c;
+        sincos(k * (mx * u.rx + my * u.ry + mz * u.rz), &s, &c);
+        const double pr = len * c, pi = len * s;
+        const double wr = ir * pr - ii * pi, wi = ir * pi + ii * pr;
+        const double qt = dx * u.tx + dy * u.ty + dz * u.tz;
+        const double qp = dx * u.px + dy * u.py;
+        ftr += wr * qt;
+        fti += wi * qt;
+        fpr += wr * qp;
+        fpi += wi * qp;
+        if (ground != 0) {
+            // image: position (x, y, -z), direction (-dx, -dy, dz) (farfield.rs:64-82)
+            double s2, c2;
+            sincos(k * (mx * u.rx + my * u.ry - mz * u.rz), &s2, &c2);
+            const double p2r = len * c2, p2i = len * s2;
+            const double vr = ir * p2r - ii * p2i, vi = ir * p2i + ii * p2r;
+            const double it = -dx * u.tx - dy * u.ty + dz * u.tz;
+            const double ip = -dx * u.px - dy * u.py;
+            const double atr = vr * it, ati = vi * it, apr = vr * ip, api = vi * ip;
+            if (ground == 1) {
+                ftr += atr;
+                fti += ati;
+                fpr += apr;
+                fpi += api;
+            } else {
+                ftr += gvr * atr - gvi * ati;
+                fti += gvr * ati + gvi * atr;
+                fpr -= ghr * apr - ghi * api;
+                fpi -= ghr * api + ghi * apr;
+            }
+        }
+    }
+}
+
+__device__ __forceinline__ void stage_chunk(const RpProb &R, double *sg, int base, int t)
+{
+    const int cnt = min(kRpChunk, R.n - base);
+    for (int e = t; e < cnt * kRpStage; e += kRpThreads) {
+        const int m = e / kRpStage, c = e % kRpStage;
+        const int gm = base + m;
+        double v;
+        if (c < 7)
+            v = R.seg[(size_t)gm * kSegStride + c];
+        else
+            v = R.cur[2 * (size_t)gm + (c - 7)];
+        sg[e] = v;
+    }
+}
+
+// Work item: (rp problem, block of 256 points). Writes F_theta, F_phi.
+__global__ __launch_bounds__(kRpThreads) void k_rp_points(const RpProb *__restrict__ probs,
+                                                          const int2 *__restrict__ work)
+{
+    const int2 wk = work[blockIdx.x];
+    const RpProb &R = probs[wk.x];
+    __shared__ double sg[kRpChunk * kRpStage];
+    const int t = threadIdx.x;
+    const int pt = wk.y * kRpThreads + t;
+    const bool active = pt < R.npts && !(R.ground != 0 && R.theta[pt] > 90.0);
+    double th = 0.0, ph = 0.0;
+    if (pt < R.npts) {
+        th = R.theta[pt];
+        ph = R.phi[pt];
+    }
+    const Dir u = unit_vectors(th, ph);
+    double gvr = 1.0, gvi = 0.0, ghr = -1.0, ghi = 0.0;
+    if (active && R.ground == 2)
+        fresnel(th, R.eps_r, R.sigma, R.k, gvr, gvi, ghr, ghi);
+    double ftr = 0.0, fti = 0.0, fpr = 0.0, fpi = 0.0;
+    for (int base = 0; base < R.n; base += kRpChunk) {
+        __syncthreads();
+        stage_chunk(R, sg, base, t);
+        __syncthreads();
+        if (active)
+            ff_chunk(sg, min(kRpChunk, R.n - base), R.k, R.ground, u, gvr, gvi, ghr, ghi, ftr,
+                     fti, fpr, fpi);
+    }
+    if (pt < R.npts) {
+        double *o = R.field + 4 * (size_t)pt;
+        o[0] = ftr;
+        o[1] = fti;
+        o[2] = fpr;
+        o[3] = fpi;
+    }
+}
+
+// Work item: (rp problem, block of 256 normalisation directions). The grid is
+// theta_i = i*tmax/(nt-1) (nt = 181 free / 91 ground), phi = 0..359 degrees,
+// trapezoid half weights at both theta ends, sin(theta) == 0 skipped
+// (farfield.rs:422-464). One partial sum per block, fixed reduction order.
+__global__ __launch_bounds__(kRpThreads) void k_rp_norm(const RpProb *__restrict__ probs,
+                                                        const int2 *__restrict__ work)
+{
+    const int2 wk = work[blockIdx.x];
+    const RpProb &R = probs[wk.x];
+    __shared__ double sg[kRpChunk * kRpStage];
+    __shared__ double red[kRpThreads];
+    const int t = threadIdx.x;
+    const int d = wk.y * kRpThreads + t;
+    const bool has_ground = R.ground != 0;
+    const int nt = has_ground ? 91 : 181;
+    const double tmax = has_ground ? 90.0 : 180.0;
+    const int it = d / 360, ip = d % 360;
+    const double th_deg = (double)it * tmax / ((double)nt - 1.0);
+    const double st = sin(th_deg * (kPi / 180.0));
+    const bool active = d < R.ndir && st != 0.0;
+    const double dth = tmax * (kPi / 180.0) / ((double)nt - 1.0);
+    const double dph = 2.0 * kPi / 360.0;
+    const double wt = (it == 0 || it == nt - 1) ? 0.5 * dth : dth;
+    const Dir u = unit_vectors(th_deg, (double)ip);
+    double gvr = 1.0, gvi = 0.0, ghr = -1.0, ghi = 0.0;
+    if (active && R.ground == 2)
+        fresnel(th_deg, R.eps_r, R.sigma, R.k, gvr, gvi, ghr, ghi);
+    double ftr = 0.0, fti = 0.0, fpr = 0.0, fpi = 0.0;
+    for (int base = 0; base < R.n; base += kRpChunk) {
+        __syncthreads();
+        stage_chunk(R, sg, base, t);
+        __syncthreads();
+        if (active)
+            ff_chunk(sg, min(kRpChunk, R.n - base), R.k, R.ground, u, gvr, gvi, ghr, ghi, ftr,
+                     fti, fpr, fpi);
+    }
+    const double term =
+        active ? (ftr * ftr + fti * fti + fpr * fpr + fpi * fpi) * st * wt * dph : 0.0;
+    red[t] = term;
+    __syncthreads();
+    for (int off = kRpThreads / 2; off > 0; off >>= 1) {
+        if (t < off)
+            red[t] += red[t + off];
+        __syncthreads();
+    }
+    if (t == 0)
+        R.partial[wk.y] = red[0];
+}
+
+// One workgroup per rp problem: total = ordered sum of partials, then gains
+// (farfield.rs:594-646) and the Stokes axial ratio (:659-667).
+__global__ __launch_bounds__(kRpThreads) void k_rp_finish(const RpProb *__restrict__ probs,
+                                                          const int *__restrict__ plist)
+{
+    const RpProb &R = probs[plist[blockIdx.x]];
+    __shared__ double s_total;
+    if (threadIdx.x == 0) {
+        double tot = 0.0;
+        for (int b = 0; b < R.npartial; ++b)
+            tot += R.partial[b];
+        s_total = tot;
+        R.total[0] = tot;
+    }
+    __syncthreads();
+    const double total = s_total;
+    const double norm = total > 0.0 ? 4.0 * kPi / total : 0.0;
+    for (int pt = threadIdx.x; pt < R.npts; pt += kRpThreads) {
+        double *o = R.out + 4 * (size_t)pt;
+        if (R.ground != 0 && R.theta[pt] > 90.0) {
+            o[0] = o[1] = o[2] = -999.99;
+            o[3] = 0.0;
+            continue;
+        }
+        const double *f = R.field + 4 * (size_t)pt;
+        const double ut = f[0] * f[0] + f[1] * f[1], up = f[2] * f[2] + f[3] * f[3];
+        const double uu = ut + up;
+        o[0] = uu * norm > 1e-20 ? 10.0 * log10(uu * norm) : -999.99;
+        o[1] = ut * norm > 1e-20 ? 10.0 * log10(ut * norm) : -999.99;
+        o[2] = up * norm > 1e-20 ? 10.0 * log10(up * norm) : -999.99;
+        double ar = 0.0;
+        if (uu > 1e-30) {
+            // S3 = 2 Im(E_theta conj(E_phi))
+            const double s3 = 2.0 * (f[1] * f[2] - f[0] * f[3]);
+            double q = s3 / uu;
+            q = q < -1.0 ? -1.0 : (q > 1.0 ? 1.0 : q);
+            ar = tan(0.5 * asin(q));
+        }
+        o[3] = ar;
+    }
+}
+
+// Fill a buffer with zeros (graph-capturable replacement for a memset).
+__global__ void k_zero(double *__restrict__ p, size_t count)
+{
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < count;
+         i += (size_t)gridDim.x * blockDim.x)
+        p[i] = 0.0;
+}
+
+// MFMA operand-layout probe: D = A B for A (16x4) and B (4x16) given in
+// row-major; lane l loads A[l%16][l/16] and B[l/16][l%16]. Writes the raw
+// per-lane accumulator so the host can check the output mapping.
+__global__ __launch_bounds__(64) void k_mfma_probe(const double *__restrict__ a,
+                                                   const double *__restrict__ b,
+                                                   double *__restrict__ out)
+{
+    const int l = threadIdx.x;
+    d4 acc = d4{0.0, 0.0, 0.0, 0.0};
+    acc = mfma(a[(l % 16) * 4 + l / 16], b[(l / 16) * 16 + l % 16], acc);
+    for (int v = 0; v < 4; ++v)
+        out[l * 4 + v] = acc[v];
+}
+
+} // namespace fnec

@@ -1,0 +1,213 @@
+/*
+ * fnec_hip — MI355X (gfx950) backend for the Hallén method-of-moments path of
+ * fnec-rust: Z-matrix fill, the lambda-regularised least-squares solve
+ * (Gram matrix + blocked complex LU with partial pivoting + triangular solve),
+ * feedpoint impedance / currents, and the RP far-field pattern.
+ *
+ * Plain C ABI: caller-owned buffers, no pointer retained after a call returns,
+ * no exceptions across the boundary. Thread-safe: each call runs on its own
+ * stream from a pool; the device context is initialised once per process.
+ *
+ * Status codes map onto the reference's error enums:
+ *   FNEC_OK                  Ok(..)
+ *   FNEC_SINGULAR            SolveError::Singular            (linear.rs:23-34)
+ *   FNEC_DIM_MISMATCH        SolveError::*DimensionMismatch  (linear.rs:23-34)
+ *   FNEC_UNAVAILABLE         Option::None -> caller falls back to the CPU
+ *                            (nec_accel wgpu_device.rs:832,1294,461 return None)
+ *   FNEC_RESOURCE_EXHAUSTED  ErrorCode::ResourceExhausted   (nec_worker protocol.rs:167-174)
+ *   FNEC_INTERNAL            ErrorCode::Internal
+ *   FNEC_INVALID_ARG         out-of-range index / null pointer
+ */
+#ifndef FNEC_HIP_H
+#define FNEC_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    FNEC_OK = 0,
+    FNEC_SINGULAR = 1,
+    FNEC_DIM_MISMATCH = 2,
+    FNEC_UNAVAILABLE = 3,
+    FNEC_RESOURCE_EXHAUSTED = 4,
+    FNEC_INTERNAL = 5,
+    FNEC_INVALID_ARG = 6,
+};
+
+enum { FNEC_PREC_FP64 = 0, FNEC_PREC_MIXED = 1 };
+
+enum { FNEC_GROUND_FREE = 0, FNEC_GROUND_PEC = 1, FNEC_GROUND_FINITE = 2 };
+
+/* Complex double, layout-compatible with num_complex::Complex64. */
+typedef struct {
+    double re, im;
+} fnec_c64;
+
+/* Segment input: nec_accel::ZSegmentInput (wgpu_device.rs:811-816). */
+typedef struct {
+    double mid[3];
+    double dir[3];
+    double length;
+    double radius;
+} fnec_seg;
+
+/* Ground model (nec_solver geometry.rs:24-54). FREE also covers Deferred.
+ * FINITE is the scalar-Gamma image model of GN 0/2 (matrix.rs:285-290 for the
+ * fill, farfield.rs:333-388 for the pattern). */
+typedef struct {
+    int kind;
+    double eps_r;
+    double sigma;
+} fnec_ground;
+
+/* One RP row: FarFieldResult gains and axial ratio (farfield.rs:42-57). */
+typedef struct {
+    double gain_total_dbi;
+    double gain_theta_dbi;
+    double gain_phi_dbi;
+    double axial_ratio;
+} fnec_rp_row;
+
+/* Per-solve diagnostics: the `diag:` values (solve_session.rs:118-207) plus
+ * timings in microseconds measured with HIP events on the solve's stream. */
+typedef struct {
+    double abs_res;        /* ||Z I - C cos - rhs||_2          */
+    double rel_res;        /* abs_res / ||rhs||_2              */
+    double diag_spread;    /* max|Z_ii| / min|Z_ii|            */
+    double backward_error; /* mixed: final normwise backward error of the LS normal equations */
+    int refine_iters;      /* mixed: refinement iterations used */
+    int n_swaps;           /* row interchanges taken by partial pivoting */
+    int rows;              /* R = N + constraint rows          */
+    int unknowns;          /* S = N + homogeneous columns      */
+    double t_fill_us, t_gram_us, t_lu_us, t_solve_us, t_total_us;
+} fnec_solve_stats;
+
+/* ---- device probes ----
+ * Replaces enumerate_compute_adapters / run_noop_compute_pipeline
+ * (wgpu_device.rs:24,67): number of usable gfx950 devices (0 => callers use
+ * the CPU path), and an explicit one-time context initialisation. */
+int fnec_hip_device_count(void);
+int fnec_hip_init(void);
+const char *fnec_hip_last_error(void);
+
+/* ---- Z-matrix fill ----
+ * Replaces fill_zmatrix_wgpu (wgpu_device.rs:832), in FP64 and with the PEC /
+ * scalar-Gamma image term (matrix.rs:186-218). z_out: n*n row-major,
+ * z_out[i*n+j] = A[i][j] (observation i, source j). */
+int fnec_hip_fill_z(const fnec_seg *segs, size_t n, double freq_hz, const fnec_ground *ground,
+                    fnec_c64 *z_out);
+
+/* ---- fused fill -> Gram -> LU -> triangular solve ----
+ * Replaces solve_hallen_gpu_resident (wgpu_device.rs:1294) and reproduces
+ * nec_solver::solve_hallen (linear.rs:672-797): same augmented system (one
+ * homogeneous column per wire range, I=0 rows at free wire ends, KCL rows
+ * I[a] + sign I[b] = 0), same normal equations with lambda = 1e-8, solved by
+ * LU with partial pivoting. Z never leaves the device.
+ * n_wires == 0 means one range (0, n-1), as in the reference.
+ * x_out: n + max(n_wires,1) values (currents, then one constant per wire).
+ * precision: FNEC_PREC_FP64 (plain) or FNEC_PREC_MIXED (FP32 MFMA factorisation
+ * with FP64 refinement). stats may be NULL. */
+int fnec_hip_solve_hallen(const fnec_seg *segs, size_t n, const fnec_c64 *rhs, const double *cos_vec,
+                          const size_t *wire_first, const size_t *wire_last, size_t n_wires,
+                          const size_t *j_a, const size_t *j_b, const double *j_sign, size_t n_junc,
+                          double freq_hz, const fnec_ground *ground, int precision, fnec_c64 *x_out,
+                          fnec_solve_stats *stats);
+
+/* ---- conductor-path variant ----
+ * Reproduces nec_solver::solve_hallen_paths (linear.rs:823-894), the CLI's
+ * route for bent / reversed degree-2 junctions (solve_session.rs:1338-1378):
+ * one homogeneous column per path (path_of[m] in 0..P-1), I=0 rows at the
+ * listed free ends, no KCL rows. x_out: n + P values. */
+int fnec_hip_solve_hallen_paths(const fnec_seg *segs, size_t n, const fnec_c64 *rhs,
+                                const double *cos_vec, const size_t *path_of,
+                                const size_t *free_ends, size_t n_free, double freq_hz,
+                                const fnec_ground *ground, int precision, fnec_c64 *x_out,
+                                fnec_solve_stats *stats);
+
+/* ---- RP far field ----
+ * Replaces run_rp_farfield_batch_wgpu (wgpu_device.rs:461) together with the
+ * normalisation integral the reference computes on the CPU
+ * (integrate_power_for_ground, farfield.rs:422-464, called from
+ * compute_radiation_pattern, farfield.rs:563-647): same unit vectors, image
+ * conventions, trapezoid weights, -999.99 sentinels and Stokes axial ratio.
+ * total_out (may be NULL) receives the normalisation integral. */
+int fnec_hip_rp(const fnec_seg *segs, const fnec_c64 *currents, size_t n, double freq_hz,
+                const fnec_ground *ground, const double *theta_deg, const double *phi_deg,
+                size_t npts, fnec_rp_row *out, double *total_out);
+
+/* ---- batched solves (sweeps and many geometries in one submission) ----
+ * One request = one Hallén solve (+ optional RP grid). Arrays are borrowed for
+ * the duration of the call. */
+typedef struct {
+    const fnec_seg *segs;
+    size_t n;
+    const fnec_c64 *rhs;
+    const double *cos_vec;
+    const size_t *wire_first;
+    const size_t *wire_last;
+    size_t n_wires;
+    const size_t *j_a;
+    const size_t *j_b;
+    const double *j_sign;
+    size_t n_junc;
+    double freq_hz;
+    fnec_ground ground;
+    size_t feed_index;       /* segment whose current gives Z_in = v_feed / I */
+    fnec_c64 v_feed;         /* feed voltage (worker semantics: the EX voltage) */
+    const double *rp_theta;  /* optional RP grid (NULL: none) */
+    const double *rp_phi;
+    size_t rp_npts;
+    uint64_t tag;            /* caller's request id, echoed in the result */
+} fnec_request;
+
+typedef struct {
+    uint64_t tag;
+    int status;
+    fnec_c64 z_in;
+    fnec_c64 feed_current;
+    fnec_c64 *x;             /* caller buffer of n + max(n_wires,1), or NULL */
+    fnec_rp_row *rp;         /* caller buffer of rp_npts rows, or NULL */
+    double rp_total;
+    double t_submit_us, t_done_us; /* session clock (steady, microseconds) */
+    fnec_solve_stats stats;
+} fnec_result;
+
+/* Solve every request in one batched submission (persistent kernels over a
+ * work list, one stream). results[i] corresponds to reqs[i]. */
+int fnec_hip_solve_batch(const fnec_request *reqs, size_t count, int precision, fnec_result *results);
+
+/* ---- serving session (continuous batching, nec_worker serving mode) ----
+ * Requests are admitted by device-memory footprint against budget_bytes;
+ * admitted requests are batched as they arrive and stay device-resident (Z,
+ * LU factors, currents) until their result is polled. submit copies the
+ * request's inputs; poll returns finished results (up to max) and does not
+ * block unless wait_us > 0. */
+typedef struct fnec_session fnec_session;
+fnec_session *fnec_hip_session_create(size_t budget_bytes, int precision, int max_batch);
+int fnec_hip_session_submit(fnec_session *s, const fnec_request *req);
+int fnec_hip_session_poll(fnec_session *s, fnec_result *out, int max, int wait_us);
+int fnec_hip_session_pending(fnec_session *s);
+void fnec_hip_session_destroy(fnec_session *s);
+/* Device bytes a request needs while resident (the admission unit). */
+size_t fnec_hip_request_footprint(size_t n, size_t n_wires, size_t n_junc, size_t rp_npts, int precision);
+
+/* ---- diagnostics (no counterpart in the reference) ----
+ * Kernel-level parity hooks for the test suite: the device-assembled
+ * augmented normal matrix [G | M^H y] of a solve_hallen problem (S x (S+1)
+ * row-major), and a probe of the FP64 MFMA lane layout (A 16x4 and B 4x16
+ * row-major in, 64 lanes x 4 accumulator values out). */
+int fnec_hip_debug_normal_eqs(const fnec_seg *segs, size_t n, const fnec_c64 *rhs,
+                              const double *cos_vec, const size_t *wire_first,
+                              const size_t *wire_last, size_t n_wires, const size_t *j_a,
+                              const size_t *j_b, const double *j_sign, size_t n_junc,
+                              double freq_hz, const fnec_ground *ground, fnec_c64 *aug_out);
+int fnec_hip_debug_mfma_probe(const double *a16x4, const double *b4x16, double *lanes_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

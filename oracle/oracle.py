@@ -269,3 +269,61 @@ def make_segments(wires, tag0: int = 1) -> np.ndarray:
 
 def axial_ratio(et: complex, ep: complex) -> float:
     return lib().or_axial_ratio(CX(et.real, et.imag), CX(ep.real, ep.imag))
+
+
+def deck_inputs(text: str, freq_hz: float, semantics: str = "worker") -> dict:
+    """Geometry, Hallén rhs/cos vectors, wire ranges and junction tuples of a
+    deck, as the reference's worker (raw wires, all junctions) or CLI
+    (merged wires, cross-component junctions) builds them."""
+    L = lib()
+    err = C.create_string_buffer(256)
+    L.or_deck_parse.restype = C.c_void_p
+    L.or_deck_parse.argtypes = [C.c_char_p, C.c_char_p, C.c_int]
+    L.or_deck_free.argtypes = [C.c_void_p]
+    L.or_build_geometry.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_char_p, C.c_int]
+    L.or_build_hallen_rhs.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_double, C.c_void_p, C.c_void_p]
+    L.or_wire_endpoints.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_int]
+    L.or_merge_collinear.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_int]
+    L.or_detect_junctions.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_double,
+                                      C.c_void_p, C.c_void_p, C.c_void_p, C.c_int]
+    L.or_ground_model.restype = Ground
+    L.or_ground_model.argtypes = [C.c_void_p]
+    d = L.or_deck_parse(text.encode(), err, 256)
+    if not d:
+        raise RuntimeError(err.value.decode())
+    try:
+        maxn = 1 << 16
+        segs = np.zeros(maxn, SEG_DTYPE)
+        n = L.or_build_geometry(d, _p(segs), maxn, err, 256)
+        if n < 0:
+            raise RuntimeError(err.value.decode())
+        segs = segs[:n].copy()
+        rhs = np.zeros(n, np.complex128)
+        cosv = np.zeros(n)
+        st = L.or_build_hallen_rhs(d, _p(segs), n, float(freq_hz), _p(rhs), _p(cosv))
+        if st:
+            raise RuntimeError(f"excitation error {st}")
+        wf = np.zeros(n + 1, np.int32)
+        wl = np.zeros(n + 1, np.int32)
+        if semantics == "worker":
+            nw = L.or_wire_endpoints(_p(segs), n, _p(wf), _p(wl), n + 1)
+        else:
+            nw = L.or_merge_collinear(_p(segs), n, _p(wf), _p(wl), n + 1)
+        wf, wl = wf[:nw].copy(), wl[:nw].copy()
+        mj = 4 * n + 4
+        ja = np.zeros(mj, np.int32)
+        jb = np.zeros(mj, np.int32)
+        js = np.zeros(mj)
+        nj = L.or_detect_junctions(_p(segs), _p(wf), _p(wl), nw, 1e-6, _p(ja), _p(jb), _p(js), mj)
+        ja, jb, js = ja[:nj].copy(), jb[:nj].copy(), js[:nj].copy()
+        if semantics != "worker":
+            comp = np.zeros(n, int)
+            for i in range(nw):
+                comp[wf[i]:wl[i] + 1] = i
+            keep = comp[ja] != comp[jb]
+            ja, jb, js = ja[keep], jb[keep], js[keep]
+        g = L.or_ground_model(d)
+        return {"segs": segs, "rhs": rhs, "cos": cosv, "wf": wf, "wl": wl, "ja": ja, "jb": jb,
+                "js": js, "ground": (g.kind, g.eps_r, g.sigma), "n": n}
+    finally:
+        L.or_deck_free(d)
