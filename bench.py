@@ -1,0 +1,303 @@
+#!/usr/bin/env python3
+"""Serving benchmark of the Hallén path on one MI355X.
+
+Workload: the mixed-N request stream (20k requests in full; N log-uniform in
+[51, 4096]; random parallel / junctioned wire sets; 30% with a 2x2 degree RP
+grid; 10% over PEC ground), built by fnec-rust_amd/workloads.py. A "step" is
+one window of requests submitted to the serving session and drained;
+``--steps K`` windows are timed after ``--warmup W`` untimed ones.
+
+Printed (one JSON line): served solves/s (value), p50/p99 latency, the plain
+FP64 one-request-at-a-time GPU path on a sample of the same stream (the
+north-star denominator) with the speedup over it and the max relative
+Z_in difference between the served and the plain path, the roofline of the
+dominant kernel from HIP events recorded on its stream during the timed
+region, and a CPU baseline (the oracle, timed on this host on a bounded
+sample and extrapolated over the stream's N distribution).
+"""
+from __future__ import annotations
+
+import argparse
+import concurrent.futures as cf
+import ctypes as C  # noqa: N812
+import importlib
+import json
+import math
+import os
+import platform
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+F = importlib.import_module("fnec-rust_amd")
+W = importlib.import_module("fnec-rust_amd.workloads")
+
+METRIC = "solves/s + p99 latency, mixed-N queue, vs plain FP64 GPU at rel ΔZin≤1e-6"
+PEAK_FP64_TFLOPS = 78.6
+PEAK_FP32_TFLOPS = 157.3
+PEAK_HBM_GBS = 8000.0
+
+RP_THETA = np.repeat(np.arange(0.0, 181.0, 2.0)[None, :], 180, axis=0).ravel()  # phi outer, theta inner
+RP_PHI = np.repeat(np.arange(0.0, 360.0, 2.0), 91)
+
+
+def build_requests(stream):
+    """Host side: decks -> Hallén inputs (worker semantics) -> fnec_request."""
+    items = []
+    for r in stream:
+        d = F.Deck(r["deck"])
+        p = d.problem("worker", r["freq_hz"])
+        d.close()
+        keep = {
+            "segs": np.ascontiguousarray(p["segs"]), "rhs": np.ascontiguousarray(p["rhs"]),
+            "cos": np.ascontiguousarray(p["cos"]), "wf": np.ascontiguousarray(p["wf"]),
+            "wl": np.ascontiguousarray(p["wl"]), "ja": np.ascontiguousarray(p["ja"]),
+            "jb": np.ascontiguousarray(p["jb"]), "js": np.ascontiguousarray(p["js"]),
+        }
+        q = F.Request()
+        q.segs = F._p(keep["segs"])
+        q.n = len(keep["segs"])
+        q.rhs = F._p(keep["rhs"])
+        q.cos_vec = F._p(keep["cos"])
+        q.wire_first = F._p(keep["wf"])
+        q.wire_last = F._p(keep["wl"])
+        q.n_wires = len(keep["wf"])
+        q.j_a = F._p(keep["ja"])
+        q.j_b = F._p(keep["jb"])
+        q.j_sign = F._p(keep["js"])
+        q.n_junc = len(keep["ja"])
+        q.freq_hz = r["freq_hz"]
+        q.ground = p["ground_struct"]
+        q.feed_index = int(p["feed_index"])
+        q.v_feed = F.C64(p["v_feed"].real, p["v_feed"].imag)
+        if r["rp"]:
+            q.rp_theta = F._p(RP_THETA)
+            q.rp_phi = F._p(RP_PHI)
+            q.rp_npts = len(RP_THETA)
+        q.tag = r["id"]
+        items.append({"req": q, "keep": keep, "meta": r, "p": p})
+    return items
+
+
+def serve(items, window, steps, warmup, precision, budget, max_batch, profile):
+    L = F.lib()
+    sess = L.fnec_hip_session_create(budget, precision, max_batch)
+    if not sess:
+        raise F.FnecError(F.INVALID_ARG, "session create failed")
+    out = (F.Result * 4096)()
+    results = {}
+
+    def run_window(chunk):
+        t_sub = {}
+        for it in chunk:
+            t_sub[it["req"].tag] = time.perf_counter()
+            st = L.fnec_hip_session_submit(sess, C.byref(it["req"]))
+            if st:
+                F._check(st)
+        remaining = len(chunk)
+        lat = []
+        while remaining:
+            k = L.fnec_hip_session_poll(sess, C.cast(out, C.c_void_p), 4096, 1000)
+            if k < 0:
+                F._check(-k)
+            now = time.perf_counter()
+            for i in range(k):
+                r = out[i]
+                if r.status:
+                    raise F.FnecError(r.status, f"request {r.tag} failed")
+                results[r.tag] = complex(r.z_in.re, r.z_in.im)
+                lat.append(now - t_sub[r.tag])
+            remaining -= k
+        return lat
+
+    for w in range(warmup):
+        run_window(items[w * window:(w + 1) * window])
+    if profile:
+        F.profile(enable=True, reset=True)
+    lat = []
+    t0 = time.perf_counter()
+    for s in range(steps):
+        lat += run_window(items[(warmup + s) * window:(warmup + s + 1) * window])
+    elapsed = time.perf_counter() - t0
+    prof = F.profile(enable=False) if profile else None
+    L.fnec_hip_session_destroy(sess)
+    return elapsed, np.array(lat), results, prof
+
+
+def plain_fp64(items, sample):
+    """One request at a time through fnec_hip_solve_hallen (+ fnec_hip_rp)."""
+    zs = {}
+    t0 = time.perf_counter()
+    for it in items[:sample]:
+        k = it["keep"]
+        r = it["req"]
+        x = F.solve_hallen(k["segs"], k["rhs"], k["cos"], k["wf"], k["wl"], k["ja"], k["jb"], k["js"],
+                           r.freq_hz, r.ground, F.PREC_FP64)
+        n = len(k["segs"])
+        zs[r.tag] = complex(r.v_feed.re, r.v_feed.im) / x[r.feed_index]
+        if it["meta"]["rp"]:
+            F.rp(k["segs"], x[:n], r.freq_hz, RP_THETA, RP_PHI, r.ground)
+    return time.perf_counter() - t0, zs
+
+
+def cpu_baseline(stream, budget_s, threads):
+    """Oracle (C restatement of the reference CPU path) on a bounded sample."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O  # noqa: E402
+    O.build()
+    rng = np.random.default_rng(7)
+    small = [r for r in stream if r["n"] <= 420]
+    order = rng.permutation(len(small))
+
+    def one(r):
+        t = time.perf_counter()
+        res = O.run_deck(r["deck"], "worker", 0, r["freq_hz"])
+        if r["rp"]:
+            kind = 1 if r["pec"] else 0
+            O.radiation_pattern(res["segs"], res["currents"], r["freq_hz"], RP_THETA, RP_PHI,
+                                O.Ground(kind, 0.0, 0.0))
+        return r["n"], r["rp"], time.perf_counter() - t
+
+    samples = []
+    t0 = time.perf_counter()
+    with cf.ThreadPoolExecutor(max_workers=threads) as ex:
+        futs = []
+        for i in order:
+            futs.append(ex.submit(one, small[i]))
+            if len(futs) >= 4 * threads:
+                break
+        for f in cf.as_completed(futs):
+            samples.append(f.result())
+            if time.perf_counter() - t0 > budget_s:
+                break
+        for f in futs:
+            f.cancel()
+    wall = time.perf_counter() - t0
+    ns = np.array([s[0] for s in samples], float)
+    rps = np.array([s[1] for s in samples], float)
+    ts = np.array([s[2] for s in samples])
+    # per-request single-thread cost model: t = a N^3 + b N * rp (RP terms ~ N)
+    A = np.stack([ns ** 3, ns * rps], axis=1)
+    coef, *_ = np.linalg.lstsq(A, ts, rcond=None)
+    coef = np.maximum(coef, 0.0)
+    N = np.array([r["n"] for r in stream], float)
+    R = np.array([r["rp"] for r in stream], float)
+    mean_t = float(np.mean(coef[0] * N ** 3 + coef[1] * N * R))
+    value = threads / mean_t
+    return {
+        "value": value,
+        "unit": "solves/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": (f"{len(samples)} requests with N<=420 from the same stream solved by the oracle "
+                   f"(worker semantics + RP) on {threads} threads in {wall:.1f} s; per-request cost "
+                   f"fitted as a*N^3 + b*N*[rp] and averaged over all {len(stream)} requests of the "
+                   f"stream (extrapolated)"),
+        "cpu": platform.processor() or platform.machine(),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=12)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--window", type=int, default=256, help="requests per step")
+    ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
+    ap.add_argument("--plain-sample", type=int, default=256)
+    ap.add_argument("--max-batch", type=int, default=256)
+    ap.add_argument("--budget-gib", type=float, default=96.0)
+    ap.add_argument("--cpu-budget-s", type=float, default=20.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build measures one GPU; multi-GPU runs are not part of it "
+              "(requests shard across GPUs with no device communication)", file=sys.stderr)
+        sys.exit(2)
+    F.init()
+    precision = F.PREC_MIXED if args.precision == "mixed" else F.PREC_FP64
+    total = (args.warmup + args.steps) * args.window
+    t_gen = time.perf_counter()
+    stream = W.config5_stream(count=total)
+    items = build_requests(stream)
+    t_gen = time.perf_counter() - t_gen
+    print(f"[bench] built {len(items)} requests in {t_gen:.1f}s; serving {args.steps} x {args.window}",
+          file=sys.stderr, flush=True)
+
+    elapsed, lat, zs_served, prof = serve(items, args.window, args.steps, args.warmup, precision,
+                                          int(args.budget_gib * 2 ** 30), args.max_batch, True)
+    timed = args.steps * args.window
+    value = timed / elapsed
+    print(f"[bench] served {timed} requests in {elapsed:.2f}s", file=sys.stderr, flush=True)
+
+    sample_items = items[args.warmup * args.window:args.warmup * args.window + args.plain_sample]
+    t_plain, zs_plain = plain_fp64(sample_items, args.plain_sample)
+    plain_rate = len(sample_items) / t_plain
+    served_sample_time = sum(1.0 for _ in sample_items) / value
+    dz = max(abs(zs_served[k] - zs_plain[k]) / abs(zs_plain[k]) for k in zs_plain)
+    print(f"[bench] plain FP64 one-at-a-time: {plain_rate:.1f} solves/s on {len(sample_items)}",
+          file=sys.stderr, flush=True)
+
+    # dominant kernel by time during the timed region
+    dom = max((k for k in prof if prof[k][0] > 0), key=lambda k: prof[k][1])
+    launches, tot_us, flops, _ = prof[dom]
+    avg_us = tot_us / launches
+    achieved = flops / launches / (avg_us * 1e-6) / 1e12
+    peak = PEAK_FP32_TFLOPS if precision == F.PREC_MIXED and dom in ("gram", "lu_update") else PEAK_FP64_TFLOPS
+    roofline = {
+        "kernel": dom,
+        "bound": "mfma" if dom in ("gram", "lu_update") else "valu",
+        "achieved": round(achieved, 3),
+        "peak": peak,
+        "unit": "TFLOP/s",
+        "frac": round(achieved / peak, 4),
+        "traffic": None,
+        "avg_launch_us": round(avg_us, 2),
+        "launches": int(launches),
+        "kernels": {k: {"launches": int(v[0]), "total_ms": round(v[1] / 1e3, 3),
+                        "tflops": round(v[2] / (v[1] * 1e-6) / 1e12, 3) if v[1] else None}
+                    for k, v in prof.items()},
+    }
+    cpu = None
+    if not args.no_cpu_baseline:
+        threads = min(16, os.cpu_count() or 1)
+        cpu = cpu_baseline(stream, args.cpu_budget_s, threads)
+
+    line = {
+        "metric": METRIC,
+        "value": round(value, 3),
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1000.0 * elapsed / args.steps, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "c128" if precision == F.PREC_FP64 else "c64+c128",
+        "data": "synthetic",
+        "config": {
+            "workload": "mixed-N serving stream (N log-U[51,4096], 30% 2x2 deg RP, 10% PEC), "
+                        f"{timed} timed requests in windows of {args.window}",
+            "precision": args.precision,
+            "max_batch": args.max_batch,
+        },
+        "latency_ms": {"p50": round(float(np.percentile(lat, 50)) * 1e3, 3),
+                       "p99": round(float(np.percentile(lat, 99)) * 1e3, 3),
+                       "arrival": f"windows of {args.window} submitted at once"},
+        "plain_fp64_gpu": {"value": round(plain_rate, 3), "unit": "solves/s",
+                           "sample": len(sample_items)},
+        "speedup_vs_plain_fp64": round(value / plain_rate, 3),
+        "max_rel_dzin_vs_plain_fp64": dz,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

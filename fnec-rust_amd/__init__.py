@@ -236,3 +236,140 @@ def debug_mfma_probe(a16x4: np.ndarray, b4x16: np.ndarray) -> np.ndarray:
     out = np.zeros((64, 4))
     _check(lib().fnec_hip_debug_mfma_probe(_p(a), _p(b), _p(out)))
     return out
+
+
+# ---------------------------------------------------------------------------
+# Host side (include/fnec_host.h): decks -> Hallén inputs, CLI report
+# ---------------------------------------------------------------------------
+
+SEM = {"cli": 0, "worker": 1, "py": 2}
+
+
+class DeckProblem(C.Structure):
+    _fields_ = [("route", C.c_int), ("req", Request), ("path_of", C.c_void_p), ("free_ends", C.c_void_p),
+                ("n_free", C.c_size_t), ("n_feeds", C.c_size_t), ("feed_idx", C.c_void_p),
+                ("feed_v", C.c_void_p), ("feed_tag", C.c_void_p), ("feed_seg", C.c_void_p),
+                ("seg_tag", C.c_void_p), ("seg_index", C.c_void_p)]
+
+
+def _host_sigs(L):
+    if getattr(L, "_host_ready", False):
+        return L
+    L.fnec_deck_parse.restype = C.c_void_p
+    L.fnec_deck_parse.argtypes = [C.c_char_p, C.c_char_p, C.c_size_t]
+    L.fnec_deck_free.argtypes = [C.c_void_p]
+    L.fnec_deck_frequencies.restype = C.c_size_t
+    L.fnec_deck_frequencies.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_size_t]
+    L.fnec_deck_segment_count.restype = C.c_size_t
+    L.fnec_deck_segment_count.argtypes = [C.c_void_p]
+    L.fnec_deck_build.argtypes = [C.c_void_p, C.c_int, C.c_double, C.POINTER(DeckProblem)]
+    L.fnec_deck_report.argtypes = [C.c_void_p, C.c_size_t, C.c_int, C.c_char_p, C.c_size_t,
+                                   C.POINTER(C.c_size_t)]
+    L._host_ready = True
+    return L
+
+
+def _arr(ptr, n, dtype):
+    if not ptr or n == 0:
+        return np.zeros(0, dtype)
+    nbytes = n * np.dtype(dtype).itemsize
+    return np.frombuffer((C.c_char * nbytes).from_address(ptr), dtype=dtype).copy()
+
+
+class Deck:
+    """A parsed NEC deck (nec_parser::parse) with the reference callers'
+    builders: frequencies, geometry, Hallén inputs, CLI report."""
+
+    def __init__(self, text: str):
+        L = _host_sigs(lib())
+        err = C.create_string_buffer(512)
+        h = L.fnec_deck_parse(text.encode(), err, 512)
+        if not h:
+            raise FnecError(INVALID_ARG, err.value.decode(errors="replace"))
+        self._h = h
+
+    def close(self):
+        if self._h:
+            lib().fnec_deck_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def frequencies(self, semantics: str = "cli") -> np.ndarray:
+        L = lib()
+        n = L.fnec_deck_frequencies(self._h, SEM[semantics], None, 0)
+        out = np.zeros(n)
+        L.fnec_deck_frequencies(self._h, SEM[semantics], _p(out), n)
+        return out
+
+    def problem(self, semantics: str, freq_hz: float) -> dict:
+        """Hallén inputs as numpy arrays (copies)."""
+        L = lib()
+        pr = DeckProblem()
+        _check(L.fnec_deck_build(self._h, SEM[semantics], float(freq_hz), C.byref(pr)))
+        r = pr.req
+        n = r.n
+        out = {
+            "route": pr.route,
+            "segs": _arr(r.segs, n, SEG),
+            "rhs": _arr(r.rhs, n, np.complex128),
+            "cos": _arr(r.cos_vec, n, np.float64),
+            "wf": _arr(r.wire_first, r.n_wires, np.uint64),
+            "wl": _arr(r.wire_last, r.n_wires, np.uint64),
+            "ja": _arr(r.j_a, r.n_junc, np.uint64),
+            "jb": _arr(r.j_b, r.n_junc, np.uint64),
+            "js": _arr(r.j_sign, r.n_junc, np.float64),
+            "freq_hz": r.freq_hz,
+            "ground": (r.ground.kind, r.ground.eps_r, r.ground.sigma),
+            "ground_struct": Ground(r.ground.kind, r.ground.eps_r, r.ground.sigma),
+            "feed_index": r.feed_index,
+            "v_feed": complex(r.v_feed.re, r.v_feed.im),
+            "rp_theta": _arr(r.rp_theta, r.rp_npts, np.float64),
+            "rp_phi": _arr(r.rp_phi, r.rp_npts, np.float64),
+            "path_of": _arr(pr.path_of, n if pr.route else 0, np.uint64),
+            "free_ends": _arr(pr.free_ends, pr.n_free, np.uint64),
+            "feed_idx": _arr(pr.feed_idx, pr.n_feeds, np.uint64),
+            "feed_v": _arr(pr.feed_v, pr.n_feeds, np.complex128),
+            "feed_tag": _arr(pr.feed_tag, pr.n_feeds, np.uint32),
+            "feed_seg": _arr(pr.feed_seg, pr.n_feeds, np.uint32),
+            "seg_tag": _arr(pr.seg_tag, n, np.uint32),
+            "seg_index": _arr(pr.seg_index, n, np.uint32),
+        }
+        return out
+
+    def report(self, freq_index: int = 0, precision: int = PREC_FP64) -> str:
+        L = lib()
+        need = C.c_size_t(0)
+        _check(L.fnec_deck_report(self._h, freq_index, precision, None, 0, C.byref(need)))
+        buf = C.create_string_buffer(need.value)
+        _check(L.fnec_deck_report(self._h, freq_index, precision, buf, need.value, C.byref(need)))
+        return buf.value.decode()
+
+
+def solve_problem(p: dict, precision: int = PREC_FP64, with_stats: bool = False):
+    """Solve the dict returned by Deck.problem on the GPU (either route)."""
+    if p["route"] == 1:
+        return solve_hallen_paths(p["segs"], p["rhs"], p["cos"], p["path_of"], p["free_ends"],
+                                  p["freq_hz"], p["ground_struct"], precision, with_stats)
+    return solve_hallen(p["segs"], p["rhs"], p["cos"], p["wf"], p["wl"], p["ja"], p["jb"], p["js"],
+                        p["freq_hz"], p["ground_struct"], precision, with_stats)
+
+
+def profile(enable: bool | None = None, reset: bool = False) -> dict:
+    """Per-kernel event timings: {class: (launches, total_us, flops, bytes)}."""
+    L = lib()
+    L.fnec_hip_profile_read.argtypes = [C.c_char_p, C.c_void_p]
+    if reset:
+        L.fnec_hip_profile_reset()
+    if enable is not None:
+        L.fnec_hip_profile_enable(int(enable))
+    out = {}
+    for k in ["fill", "gram", "lu_panel", "lu_trsm", "lu_update", "backsub", "rp"]:
+        buf = np.zeros(4)
+        _check(L.fnec_hip_profile_read(k.encode(), _p(buf)))
+        out[k] = tuple(float(v) for v in buf)
+    return out

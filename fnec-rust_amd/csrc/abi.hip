@@ -185,4 +185,26 @@ int fnec_hip_debug_mfma_probe(const double *a16x4, const double *b4x16, double *
     return mfma_probe(a16x4, b4x16, lanes_out);
 }
 
+void fnec_hip_profile_enable(int on)
+{
+    std::lock_guard<std::mutex> lk(global_mutex());
+    profile_enable(on);
+}
+
+void fnec_hip_profile_reset(void)
+{
+    std::lock_guard<std::mutex> lk(global_mutex());
+    profile_reset();
+}
+
+int fnec_hip_profile_read(const char *kernel, double *out4)
+{
+    std::lock_guard<std::mutex> lk(global_mutex());
+    if (!kernel || !out4) {
+        set_error("null argument");
+        return FNEC_INVALID_ARG;
+    }
+    return profile_read(kernel, out4);
+}
+
 } // extern "C"

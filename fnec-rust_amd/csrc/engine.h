@@ -58,5 +58,8 @@ int rp_host_from(RpHost &h, const fnec_seg *segs, const fnec_c64 *cur, size_t n,
 int rp_run(const std::vector<RpHost> &jobs, std::vector<fnec_rp_row *> &outs,
            std::vector<double> &totals);
 int mfma_probe(const double *a, const double *b, double *out);
+void profile_enable(int on);
+void profile_reset();
+int profile_read(const char *name, double *out4);
 
 } // namespace fnec

@@ -67,6 +67,91 @@ struct Context {
 
 static Context g_ctx;
 
+// Optional per-kernel timing: HIP events around the launches of each tracked
+// kernel class on the solve stream, with the algorithmic work of each launch.
+struct ProfClass {
+    double launches = 0, total_us = 0, flops = 0, bytes = 0;
+};
+struct ProfPending {
+    int cls;
+    hipEvent_t a, b;
+    double flops, bytes;
+};
+enum { PC_FILL, PC_GRAM, PC_LU_PANEL, PC_LU_TRSM, PC_LU_UPDATE, PC_BACKSUB, PC_RP, PC_N };
+static const char *kProfNames[PC_N] = {"fill", "gram", "lu_panel", "lu_trsm", "lu_update", "backsub", "rp"};
+static bool g_prof = false;
+static ProfClass g_prof_acc[PC_N];
+static std::vector<ProfPending> g_prof_pending;
+static std::vector<hipEvent_t> g_prof_pool;
+
+static hipEvent_t prof_event()
+{
+    if (!g_prof_pool.empty()) {
+        hipEvent_t e = g_prof_pool.back();
+        g_prof_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    (void)hipEventCreate(&e);
+    return e;
+}
+
+static hipEvent_t prof_begin()
+{
+    if (!g_prof)
+        return nullptr;
+    hipEvent_t e = prof_event();
+    (void)hipEventRecord(e, g_ctx.stream);
+    return e;
+}
+
+static void prof_end(int cls, hipEvent_t a, double flops, double bytes)
+{
+    if (!g_prof || !a)
+        return;
+    hipEvent_t b = prof_event();
+    (void)hipEventRecord(b, g_ctx.stream);
+    g_prof_pending.push_back({cls, a, b, flops, bytes});
+}
+
+// Called after a stream synchronisation: fold pending event pairs.
+static void prof_collect()
+{
+    for (auto &p : g_prof_pending) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
+            ProfClass &c = g_prof_acc[p.cls];
+            c.launches += 1;
+            c.total_us += ms * 1000.0;
+            c.flops += p.flops;
+            c.bytes += p.bytes;
+        }
+        g_prof_pool.push_back(p.a);
+        g_prof_pool.push_back(p.b);
+    }
+    g_prof_pending.clear();
+}
+
+void profile_enable(int on) { g_prof = on != 0; }
+void profile_reset()
+{
+    for (auto &c : g_prof_acc)
+        c = ProfClass{};
+}
+int profile_read(const char *name, double *out4)
+{
+    for (int i = 0; i < PC_N; ++i)
+        if (!std::strcmp(name, kProfNames[i])) {
+            out4[0] = g_prof_acc[i].launches;
+            out4[1] = g_prof_acc[i].total_us;
+            out4[2] = g_prof_acc[i].flops;
+            out4[3] = g_prof_acc[i].bytes;
+            return FNEC_OK;
+        }
+    set_error("unknown kernel class");
+    return FNEC_INVALID_ARG;
+}
+
 static int ctx_init()
 {
     if (g_ctx.ready)
@@ -110,7 +195,7 @@ int ensure_ready()
 {
     int st = ctx_init();
     if (st == FNEC_OK)
-        hipSetDevice(g_ctx.device);
+        (void)hipSetDevice(g_ctx.device);
     return st;
 }
 
@@ -369,6 +454,9 @@ struct Plan {
         size_t panel_off, panel_cnt, trsm_off, trsm_cnt, upd_off, upd_cnt;
     };
     std::vector<Step> steps;
+    // algorithmic work per launch (profiling)
+    double fill_flops = 0, gram_flops = 0, backsub_flops = 0;
+    std::vector<double> step_upd_flops, step_panel_flops, step_trsm_flops;
     // arena layout
     size_t bytes = 0;
     size_t a_begin = 0, a_end = 0; // byte range of the A planes (zeroed per run)
@@ -463,6 +551,15 @@ static void plan_build(Plan &pl, const std::vector<const HostProblem *> &hps, bo
                 push4((int)b, y, z, 0);
     }
     pl.fill_cnt = (pl.ints.size() - pl.fill_off) / 4;
+    pl.fill_flops = pl.gram_flops = pl.backsub_flops = 0;
+    for (size_t b = 0; b < np; ++b) {
+        const double n = hps[b]->n, s = hps[b]->s;
+        // fill unit: one Gauss-Legendre point = 22 flops + 5 special ops,
+        // 8 points per element and image (counted as 27 operations)
+        pl.fill_flops += n * n * 8.0 * (hps[b]->ground ? 2.0 : 1.0) * 27.0;
+        pl.gram_flops += 4.0 * n * n * n;      // Hermitian-minimal Z^H Z
+        pl.backsub_flops += 4.0 * s * s;       // 8 S^2 / 2 for the upper triangle
+    }
     if (with_solve) {
         align4();
         pl.gram_off = pl.ints.size();
@@ -524,6 +621,20 @@ static void plan_build(Plan &pl, const std::vector<const HostProblem *> &hps, bo
                         push4((int)b, y, z, 0);
             }
             st.upd_cnt = (pl.ints.size() - st.upd_off) / 4;
+            double fu = 0, fp = 0, ft = 0;
+            for (size_t b = 0; b < np; ++b) {
+                const int s = hps[b]->s;
+                if (s <= p)
+                    continue;
+                const int jb = std::min(kPanel, s - p);
+                const double nrows = s - p - jb, ncols = s + 1 - p - jb, m = s - p;
+                fu += 8.0 * nrows * ncols * jb;
+                fp += 8.0 * m * jb * jb / 2.0;
+                ft += 8.0 * ncols * jb * jb / 2.0;
+            }
+            pl.step_upd_flops.push_back(fu);
+            pl.step_panel_flops.push_back(fp);
+            pl.step_trsm_flops.push_back(ft);
             pl.steps.push_back(st);
         }
         align4();
@@ -609,9 +720,12 @@ static int plan_launch(Plan &pl, char *base, bool with_solve)
     hipStream_t st = g_ctx.stream;
     const Prob *dp = (const Prob *)(base + pl.off_probs);
     HIPCHK(hipEventRecord(g_ctx.ev[0], st));
-    if (pl.fill_cnt)
+    if (pl.fill_cnt) {
+        hipEvent_t e = prof_begin();
         hipLaunchKernelGGL(k_fill_z, dim3((unsigned)pl.fill_cnt), dim3(256), 0, st, dp,
                            (const int4 *)wl_ptr(base, pl, pl.fill_off));
+        prof_end(PC_FILL, e, pl.fill_flops, 0);
+    }
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(g_ctx.ev[1], st));
     if (!with_solve)
@@ -621,29 +735,47 @@ static int plan_launch(Plan &pl, char *base, bool with_solve)
         if (count)
             hipLaunchKernelGGL(k_zero, dim3(1024), dim3(256), 0, st, (double *)(base + pl.a_begin), count);
     }
-    hipLaunchKernelGGL(k_gram, dim3((unsigned)pl.gram_cnt), dim3(256), 0, st, dp,
-                       (const int4 *)wl_ptr(base, pl, pl.gram_off));
+    {
+        hipEvent_t e = prof_begin();
+        hipLaunchKernelGGL(k_gram, dim3((unsigned)pl.gram_cnt), dim3(256), 0, st, dp,
+                           (const int4 *)wl_ptr(base, pl, pl.gram_off));
+        prof_end(PC_GRAM, e, pl.gram_flops, 0);
+    }
     hipLaunchKernelGGL(k_gram_cols, dim3((unsigned)pl.cols_cnt), dim3(256), 0, st, dp,
                        (const int2 *)wl_ptr(base, pl, pl.cols_off));
     hipLaunchKernelGGL(k_gram_fix, dim3((unsigned)pl.all_cnt), dim3(256), 0, st, dp,
                        wl_ptr(base, pl, pl.all_off));
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(g_ctx.ev[2], st));
-    for (const auto &s : pl.steps) {
-        if (s.panel_cnt)
+    for (size_t si = 0; si < pl.steps.size(); ++si) {
+        const auto &s = pl.steps[si];
+        if (s.panel_cnt) {
+            hipEvent_t e = prof_begin();
             hipLaunchKernelGGL(k_lu_panel, dim3((unsigned)s.panel_cnt), dim3(512), 0, st, dp,
                                wl_ptr(base, pl, s.panel_off), s.p);
-        if (s.trsm_cnt)
+            prof_end(PC_LU_PANEL, e, pl.step_panel_flops[si], 0);
+        }
+        if (s.trsm_cnt) {
+            hipEvent_t e = prof_begin();
             hipLaunchKernelGGL(k_lu_trsm, dim3((unsigned)s.trsm_cnt), dim3(64), 0, st, dp,
                                (const int2 *)wl_ptr(base, pl, s.trsm_off), s.p);
-        if (s.upd_cnt)
+            prof_end(PC_LU_TRSM, e, pl.step_trsm_flops[si], 0);
+        }
+        if (s.upd_cnt) {
+            hipEvent_t e = prof_begin();
             hipLaunchKernelGGL(k_lu_update, dim3((unsigned)s.upd_cnt), dim3(256), 0, st, dp,
                                (const int4 *)wl_ptr(base, pl, s.upd_off), s.p);
+            prof_end(PC_LU_UPDATE, e, pl.step_upd_flops[si], 0);
+        }
     }
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(g_ctx.ev[3], st));
-    hipLaunchKernelGGL(k_backsub, dim3((unsigned)pl.all_cnt), dim3(256), 0, st, dp,
-                       wl_ptr(base, pl, pl.all_off));
+    {
+        hipEvent_t e = prof_begin();
+        hipLaunchKernelGGL(k_backsub, dim3((unsigned)pl.all_cnt), dim3(256), 0, st, dp,
+                           wl_ptr(base, pl, pl.all_off));
+        prof_end(PC_BACKSUB, e, pl.backsub_flops, 0);
+    }
     hipLaunchKernelGGL(k_residual_rows, dim3((unsigned)pl.res_cnt), dim3(256), 0, st, dp,
                        (const int2 *)wl_ptr(base, pl, pl.res_off));
     hipLaunchKernelGGL(k_residual_finish, dim3((unsigned)pl.all_cnt), dim3(64), 0, st, dp,
@@ -683,6 +815,7 @@ int solve_problems(const std::vector<const HostProblem *> &hps, std::vector<fnec
     if (st)
         return st;
     HIPCHK(hipStreamSynchronize(g_ctx.stream));
+    prof_collect();
     const float t_fill = ev_ms(0, 1) * 1000.f, t_gram = ev_ms(1, 2) * 1000.f,
                 t_lu = ev_ms(2, 3) * 1000.f, t_sol = ev_ms(3, 4) * 1000.f;
     const size_t np = hps.size();
@@ -858,10 +991,16 @@ int rp_run(const std::vector<RpHost> &jobs, std::vector<fnec_rp_row *> &outs, st
     HIPCHK(hipMemcpyAsync(base + o_ints, ints.data(), 4 * ints.size(), hipMemcpyHostToDevice, s));
     const RpProb *dp = (const RpProb *)(base + o_probs);
     const int *wi = (const int *)(base + o_ints);
+    double rp_terms = 0;
+    for (const auto &h : jobs)
+        rp_terms += (double)h.n * (h.ground ? 2.0 : 1.0) * ((double)h.theta.size() + h.ndir);
+    hipEvent_t e_rp = prof_begin();
     if (pts_cnt)
         hipLaunchKernelGGL(k_rp_points, dim3((unsigned)pts_cnt), dim3(256), 0, s, dp, (const int2 *)(wi + pts_off));
     if (norm_cnt)
         hipLaunchKernelGGL(k_rp_norm, dim3((unsigned)norm_cnt), dim3(256), 0, s, dp, (const int2 *)(wi + norm_off));
+    // RP unit: one (direction, segment[, image]) term = 32 flops + sin + cos
+    prof_end(PC_RP, e_rp, rp_terms * 34.0, 0);
     hipLaunchKernelGGL(k_rp_finish, dim3((unsigned)nj), dim3(256), 0, s, dp, wi + all_off);
     HIPCHK(hipGetLastError());
     totals.assign(nj, 0.0);
@@ -871,6 +1010,7 @@ int rp_run(const std::vector<RpHost> &jobs, std::vector<fnec_rp_row *> &outs, st
         HIPCHK(hipMemcpyAsync(&totals[j], base + o_tot[j], 8, hipMemcpyDeviceToHost, s));
     }
     HIPCHK(hipStreamSynchronize(s));
+    prof_collect();
     return FNEC_OK;
 }
 

@@ -1,0 +1,1102 @@
+// Host side above the fnec_hip ABI (include/fnec_host.h): NEC deck parsing,
+// geometry, wire topology, Hallén excitation vectors, the CLI pipeline and
+// report contract v1. Plain C++; the numerics of each builder follow the
+// reference's operation order so that segment data and right-hand sides are
+// the same doubles the reference's callers pass to nec_accel.
+#include <algorithm>
+#include <cctype>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/fnec_hip.h"
+#include "../../include/fnec_host.h"
+#include "engine.h"
+
+namespace {
+
+constexpr double kPi = 3.14159265358979323846;
+constexpr double kC0 = 299792458.0;
+constexpr double kMu0 = 4.0 * kPi * 1e-7;
+constexpr double kEta0 = kMu0 * kC0;
+
+inline double deg2rad(double d) { return d * (kPi / 180.0); }
+inline double rad2deg(double r) { return r * (180.0 / kPi); }
+
+struct Card {
+    std::string mn;
+    int nf = 0;
+    double f[16] = {0};
+};
+
+uint32_t as_u32(double v)
+{
+    if (std::isnan(v) || v <= 0.0)
+        return 0;
+    if (v >= 4294967295.0)
+        return 4294967295u;
+    return (uint32_t)v;
+}
+
+int32_t as_i32(double v)
+{
+    if (std::isnan(v))
+        return 0;
+    if (v <= -2147483648.0)
+        return INT32_MIN;
+    if (v >= 2147483647.0)
+        return INT32_MAX;
+    return (int32_t)v;
+}
+
+// Rust str::parse::<f64> accepts [+-]digits[.digits][e[+-]digits] and inf/nan.
+bool parse_f64(const std::string &s, double &out)
+{
+    size_t p = 0;
+    if (p < s.size() && (s[p] == '+' || s[p] == '-'))
+        ++p;
+    std::string rest = s.substr(p);
+    std::string low = rest;
+    for (auto &c : low)
+        c = (char)std::tolower((unsigned char)c);
+    if (low == "inf" || low == "infinity" || low == "nan") {
+        out = std::strtod(s.c_str(), nullptr);
+        return true;
+    }
+    size_t q = p, digits = 0;
+    while (q < s.size() && std::isdigit((unsigned char)s[q])) {
+        ++q;
+        ++digits;
+    }
+    if (q < s.size() && s[q] == '.') {
+        ++q;
+        while (q < s.size() && std::isdigit((unsigned char)s[q])) {
+            ++q;
+            ++digits;
+        }
+    }
+    if (!digits)
+        return false;
+    if (q < s.size() && (s[q] == 'e' || s[q] == 'E')) {
+        ++q;
+        if (q < s.size() && (s[q] == '+' || s[q] == '-'))
+            ++q;
+        if (q >= s.size() || !std::isdigit((unsigned char)s[q]))
+            return false;
+        while (q < s.size() && std::isdigit((unsigned char)s[q]))
+            ++q;
+    }
+    if (q != s.size())
+        return false;
+    out = std::strtod(s.c_str(), nullptr);
+    return true;
+}
+
+struct Seg {
+    uint32_t tag, idx;
+    double start[3], end[3], mid[3], dir[3];
+    double length, radius;
+};
+
+struct Ground {
+    int kind = 0; // 0 free, 1 PEC, 2 finite, 3 deferred (treated as free)
+    double eps_r = 0.0, sigma = 0.0;
+};
+
+} // namespace
+
+struct fnec_deck {
+    std::vector<Card> cards;
+    // built geometry (lazily)
+    bool geo_built = false;
+    int geo_status = FNEC_OK;
+    std::string geo_err;
+    std::vector<Seg> segs;
+    // storage for the last fnec_deck_problem
+    std::vector<fnec_seg> fsegs;
+    std::vector<fnec_c64> rhs, feed_v;
+    std::vector<double> cosv, js, theta, phi;
+    std::vector<size_t> wf, wl, ja, jb, path_of, free_ends, feed_idx;
+    std::vector<uint32_t> feed_tag, feed_seg, seg_tag, seg_index;
+};
+
+namespace {
+
+bool card_is(const Card &c, const char *m) { return c.mn == m; }
+int ex_type(const Card &c) { return (int)as_u32(c.f[0]); }
+uint32_t ex_tag(const Card &c) { return as_u32(c.f[1]); }
+uint32_t ex_seg(const Card &c) { return as_u32(c.f[2]); }
+double ex_vr(const Card &c) { return c.nf > 4 ? c.f[4] : 0.0; }
+double ex_vi(const Card &c) { return c.nf > 5 ? c.f[5] : 0.0; }
+bool ex_voltage(const Card &c) { int t = ex_type(c); return t == 0 || t == 5; }
+bool ex_plane(const Card &c) { int t = ex_type(c); return t >= 1 && t <= 3; }
+bool ex_current(const Card &c) { return ex_type(c) == 4; }
+
+int find_seg(const std::vector<Seg> &s, uint32_t tag, uint32_t idx)
+{
+    for (size_t i = 0; i < s.size(); ++i)
+        if (s[i].tag == tag && s[i].idx == idx)
+            return (int)i;
+    return -1;
+}
+
+// ---- geometry (geometry.rs:676-879) ----
+
+void rot_x(double p[3], double c, double s) { double y = c * p[1] - s * p[2], z = s * p[1] + c * p[2]; p[1] = y; p[2] = z; }
+void rot_y(double p[3], double c, double s) { double x = c * p[0] + s * p[2], z = -s * p[0] + c * p[2]; p[0] = x; p[2] = z; }
+void rot_z(double p[3], double c, double s) { double x = c * p[0] - s * p[1], y = s * p[0] + c * p[1]; p[0] = x; p[1] = y; }
+
+void recompute_dir(Seg &g)
+{
+    double dx = g.end[0] - g.start[0], dy = g.end[1] - g.start[1], dz = g.end[2] - g.start[2];
+    double len = std::sqrt(dx * dx + dy * dy + dz * dz);
+    if (len == 0.0) {
+        g.dir[0] = 0.0; g.dir[1] = 0.0; g.dir[2] = 1.0;
+    } else {
+        g.dir[0] = dx / len; g.dir[1] = dy / len; g.dir[2] = dz / len;
+    }
+}
+
+void gm_point(double p[3], const Card &c)
+{
+    rot_x(p, std::cos(deg2rad(c.f[2])), std::sin(deg2rad(c.f[2])));
+    rot_y(p, std::cos(deg2rad(c.f[3])), std::sin(deg2rad(c.f[3])));
+    rot_z(p, std::cos(deg2rad(c.f[4])), std::sin(deg2rad(c.f[4])));
+    p[0] = p[0] + c.f[5];
+    p[1] = p[1] + c.f[6];
+    p[2] = p[2] + c.f[7];
+}
+
+bool tag_in_range(uint32_t t, uint32_t first, uint32_t last)
+{
+    return (first == 0 || t >= first) && (last == 0 || t <= last);
+}
+
+int build_geometry(const std::vector<Card> &cards, std::vector<Seg> &out, std::string &err)
+{
+    out.clear();
+    for (const Card &c : cards) {
+        if (card_is(c, "GW")) {
+            uint32_t tag = as_u32(c.f[0]), ns = as_u32(c.f[1]);
+            double x1 = c.f[2], y1 = c.f[3], z1 = c.f[4];
+            double wdx = c.f[5] - x1, wdy = c.f[6] - y1, wdz = c.f[7] - z1;
+            if (ns == 0) {
+                err = "GW tag " + std::to_string(tag) + ": segment count must be >= 1";
+                return FNEC_INVALID_ARG;
+            }
+            double wl = std::sqrt(wdx * wdx + wdy * wdy + wdz * wdz);
+            if (wl == 0.0) {
+                err = "GW tag " + std::to_string(tag) + ": wire has zero length (start == end)";
+                return FNEC_INVALID_ARG;
+            }
+            if (out.size() + ns > ((size_t)1 << 20)) {
+                err = "too many segments";
+                return FNEC_INVALID_ARG;
+            }
+            double dir[3] = {wdx / wl, wdy / wl, wdz / wl};
+            double nn = (double)ns, seglen = wl / nn;
+            for (uint32_t i = 0; i < ns; ++i) {
+                Seg g;
+                double t0 = (double)i / nn, t1 = ((double)i + 1.0) / nn, tm = (t0 + t1) * 0.5;
+                g.tag = tag;
+                g.idx = i + 1;
+                g.start[0] = x1 + wdx * t0; g.start[1] = y1 + wdy * t0; g.start[2] = z1 + wdz * t0;
+                g.end[0] = x1 + wdx * t1;   g.end[1] = y1 + wdy * t1;   g.end[2] = z1 + wdz * t1;
+                g.mid[0] = x1 + wdx * tm;   g.mid[1] = y1 + wdy * tm;   g.mid[2] = z1 + wdz * tm;
+                std::memcpy(g.dir, dir, sizeof(dir));
+                g.length = seglen;
+                g.radius = c.f[8];
+                out.push_back(g);
+            }
+        } else if (card_is(c, "GM")) {
+            uint32_t inc = as_u32(c.f[0]), last = as_u32(c.f[1]), first = as_u32(c.f[8]);
+            size_t n0 = out.size();
+            if (inc == 0) {
+                for (size_t i = 0; i < n0; ++i) {
+                    Seg &g = out[i];
+                    if (!tag_in_range(g.tag, first, last))
+                        continue;
+                    gm_point(g.start, c);
+                    gm_point(g.end, c);
+                    gm_point(g.mid, c);
+                    recompute_dir(g);
+                }
+            } else {
+                for (size_t i = 0; i < n0; ++i) {
+                    Seg g = out[i];
+                    if (!tag_in_range(g.tag, first, last))
+                        continue;
+                    g.tag += inc;
+                    gm_point(g.start, c);
+                    gm_point(g.end, c);
+                    gm_point(g.mid, c);
+                    recompute_dir(g);
+                    out.push_back(g);
+                }
+            }
+        } else if (card_is(c, "GR")) {
+            uint32_t inc = as_u32(c.f[0]), count = as_u32(c.f[1]);
+            std::vector<Seg> orig = out;
+            for (uint32_t k = 1; k <= count; ++k) {
+                double ang = c.f[2] * (double)k;
+                double ca = std::cos(deg2rad(ang)), sa = std::sin(deg2rad(ang));
+                for (const Seg &o : orig) {
+                    Seg g = o;
+                    g.tag += inc * k;
+                    rot_z(g.start, ca, sa);
+                    rot_z(g.end, ca, sa);
+                    rot_z(g.mid, ca, sa);
+                    recompute_dir(g);
+                    out.push_back(g);
+                }
+            }
+        }
+    }
+    if (out.empty()) {
+        err = "deck contains no GW (wire) cards";
+        return FNEC_INVALID_ARG;
+    }
+    return FNEC_OK;
+}
+
+Ground ground_model(const std::vector<Card> &cards)
+{
+    Ground g;
+    for (const Card &c : cards) {
+        if (!card_is(c, "GN"))
+            continue;
+        int t = as_i32(c.f[0]);
+        if (t == -1)
+            g.kind = 0;
+        else if (t == 1)
+            g.kind = 1;
+        else if (t == 0 || t == 2) {
+            g.kind = 2;
+            g.eps_r = c.nf > 4 ? c.f[4] : 13.0;
+            g.sigma = c.nf > 5 ? c.f[5] : 0.005;
+        } else
+            g.kind = 3;
+        return g;
+    }
+    for (const Card &c : cards)
+        if (card_is(c, "GE") && c.nf > 0 && as_i32(c.f[0]) == 1) {
+            g.kind = 1;
+            return g;
+        }
+    return g;
+}
+
+// ---- wire topology (geometry.rs:168-339) ----
+
+std::vector<std::pair<size_t, size_t>> wire_endpoints(const std::vector<Seg> &s)
+{
+    std::vector<std::pair<size_t, size_t>> out;
+    uint32_t cur = 0xFFFFFFFFu;
+    size_t first = 0;
+    for (size_t i = 0; i < s.size(); ++i) {
+        if (s[i].tag != cur) {
+            if (cur != 0xFFFFFFFFu)
+                out.push_back({first, i - 1});
+            cur = s[i].tag;
+            first = i;
+        }
+    }
+    if (cur != 0xFFFFFFFFu)
+        out.push_back({first, s.size() - 1});
+    return out;
+}
+
+double dist2(const double *a, const double *b)
+{
+    double d0 = a[0] - b[0], d1 = a[1] - b[1], d2 = a[2] - b[2];
+    return d0 * d0 + d1 * d1 + d2 * d2;
+}
+
+bool dir_aligned(const double *a, const double *b, double tol)
+{
+    double cr[3] = {a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0]};
+    double z[3] = {0, 0, 0};
+    double dot = a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
+    return dist2(cr, z) < tol * tol && dot > 0.0;
+}
+
+std::vector<std::pair<size_t, size_t>> merge_collinear(const std::vector<Seg> &s)
+{
+    auto base = wire_endpoints(s);
+    if (base.size() < 2)
+        return base;
+    const double tol = 1e-6;
+    std::vector<std::pair<size_t, size_t>> merged;
+    auto cur = base[0];
+    for (size_t i = 1; i < base.size(); ++i) {
+        const Seg &a = s[cur.second], &b = s[base[i].first];
+        bool contiguous = cur.second + 1 == base[i].first;
+        bool connects = dist2(a.end, b.start) < tol * tol;
+        bool collinear = dir_aligned(a.dir, b.dir, tol);
+        bool same_r = std::fabs(a.radius - b.radius) < tol;
+        if (contiguous && connects && collinear && same_r)
+            cur.second = base[i].second;
+        else {
+            merged.push_back(cur);
+            cur = base[i];
+        }
+    }
+    merged.push_back(cur);
+    return merged;
+}
+
+struct Junction {
+    size_t a, b;
+    double sign;
+};
+
+std::vector<Junction> detect_junctions(const std::vector<Seg> &s,
+                                       const std::vector<std::pair<size_t, size_t>> &wires, double tol)
+{
+    struct Node {
+        size_t seg;
+        const double *p;
+        bool start;
+    };
+    std::vector<Node> nodes;
+    for (auto &w : wires) {
+        nodes.push_back({w.first, s[w.first].start, true});
+        nodes.push_back({w.second, s[w.second].end, false});
+    }
+    std::vector<Junction> out;
+    for (size_t i = 0; i < nodes.size(); ++i)
+        for (size_t j = i + 1; j < nodes.size(); ++j) {
+            if (nodes[i].seg == nodes[j].seg)
+                continue;
+            double dx = nodes[i].p[0] - nodes[j].p[0], dy = nodes[i].p[1] - nodes[j].p[1],
+                   dz = nodes[i].p[2] - nodes[j].p[2];
+            if (std::sqrt(dx * dx + dy * dy + dz * dz) > tol)
+                continue;
+            out.push_back({nodes[i].seg, nodes[j].seg, nodes[i].start == nodes[j].start ? 1.0 : -1.0});
+        }
+    return out;
+}
+
+struct Paths {
+    bool ok = false;
+    bool nontrivial = false;
+    size_t count = 0;
+    std::vector<size_t> path_of;
+    std::vector<double> sign_of, s_of;
+    std::vector<size_t> free_ends; // two per path
+};
+
+size_t uf_find(std::vector<size_t> &parent, size_t x)
+{
+    size_t r = x;
+    while (parent[r] != r)
+        r = parent[r];
+    while (parent[x] != x) {
+        size_t nx = parent[x];
+        parent[x] = r;
+        x = nx;
+    }
+    return r;
+}
+
+Paths conductor_paths(const std::vector<Seg> &s)
+{
+    Paths P;
+    auto wires = wire_endpoints(s);
+    const size_t w = wires.size(), n = s.size();
+    P.path_of.assign(n, 0);
+    P.sign_of.assign(n, 1.0);
+    P.s_of.assign(n, 0.0);
+    if (w == 0) {
+        P.ok = true;
+        return P;
+    }
+    const double tol = 1e-6;
+    auto ep = [&](size_t wi, int end) -> const double * {
+        return end == 0 ? s[wires[wi].first].start : s[wires[wi].second].end;
+    };
+    std::vector<long> node_of(2 * w, -1);
+    std::vector<std::vector<size_t>> members;
+    for (size_t id = 0; id < 2 * w; ++id) {
+        if (node_of[id] >= 0)
+            continue;
+        const double *p = ep(id / 2, (int)(id & 1));
+        long node = (long)members.size();
+        members.push_back({id});
+        node_of[id] = node;
+        for (size_t j = 0; j < 2 * w; ++j) {
+            if (node_of[j] >= 0)
+                continue;
+            if (dist2(p, ep(j / 2, (int)(j & 1))) < tol * tol) {
+                node_of[j] = node;
+                members.back().push_back(j);
+            }
+        }
+    }
+    for (auto &m : members)
+        if (m.size() >= 3)
+            return P;
+    std::vector<size_t> parent(w);
+    for (size_t i = 0; i < w; ++i)
+        parent[i] = i;
+    for (auto &m : members)
+        if (m.size() == 2) {
+            size_t a = m[0] / 2, b = m[1] / 2;
+            if (a == b)
+                return P;
+            size_t ra = uf_find(parent, a), rb = uf_find(parent, b);
+            parent[ra] = rb;
+        }
+    std::map<size_t, std::vector<size_t>> comp;
+    for (size_t wi = 0; wi < w; ++wi)
+        comp[uf_find(parent, wi)].push_back(wi);
+    for (auto &kv : comp) {
+        std::vector<size_t> fe;
+        for (size_t wi : kv.second)
+            for (int end = 0; end < 2; ++end)
+                if (members[node_of[wi * 2 + end]].size() == 1)
+                    fe.push_back(wi * 2 + end);
+        if (fe.size() != 2)
+            return P;
+        size_t cur = fe[0] / 2;
+        int enter = (int)(fe[0] & 1);
+        std::vector<char> visited(w, 0);
+        std::vector<size_t> order;
+        double arc = 0.0;
+        for (;;) {
+            visited[cur] = 1;
+            size_t first = wires[cur].first, last = wires[cur].second;
+            bool fwd = enter == 0;
+            double sg = fwd ? 1.0 : -1.0;
+            for (size_t k = 0; k <= last - first; ++k) {
+                size_t m = fwd ? first + k : last - k;
+                double half = s[m].length * 0.5;
+                P.s_of[m] = arc + half;
+                arc += s[m].length;
+                order.push_back(m);
+                P.path_of[m] = P.count;
+                P.sign_of[m] = sg;
+            }
+            int exit_end = 1 - enter;
+            long exit_node = node_of[cur * 2 + exit_end];
+            if (members[exit_node].size() == 1)
+                break;
+            long nw = -1;
+            int ne = 0;
+            for (size_t mem : members[exit_node])
+                if (mem / 2 != cur && !visited[mem / 2]) {
+                    nw = (long)(mem / 2);
+                    ne = (int)(mem & 1);
+                }
+            if (nw < 0)
+                break;
+            cur = (size_t)nw;
+            enter = ne;
+        }
+        for (size_t m : order)
+            P.s_of[m] = P.s_of[m] - arc * 0.5;
+        for (size_t i = 0; i < order.size(); ++i) {
+            if (P.sign_of[order[i]] != 1.0)
+                P.nontrivial = true;
+            if (i > 0 && order[i] != order[i - 1] + 1)
+                P.nontrivial = true;
+        }
+        P.free_ends.push_back(order.front());
+        P.free_ends.push_back(order.back());
+        P.count++;
+    }
+    P.ok = true;
+    return P;
+}
+
+// ---- excitation (excitation.rs:78-341) ----
+
+int build_excitation(const std::vector<Card> &cards, const std::vector<Seg> &s, std::vector<fnec_c64> &v)
+{
+    v.assign(s.size(), fnec_c64{0.0, 0.0});
+    for (const Card &c : cards) {
+        if (!card_is(c, "EX") || ex_plane(c) || ex_current(c))
+            continue;
+        if (!ex_voltage(c))
+            return FNEC_INVALID_ARG;
+        int idx = find_seg(s, ex_tag(c), ex_seg(c));
+        if (idx < 0)
+            return FNEC_INVALID_ARG;
+        double dl = s[idx].length;
+        v[idx].re = v[idx].re + ex_vr(c) / dl;
+        v[idx].im = v[idx].im + ex_vi(c) / dl;
+    }
+    return FNEC_OK;
+}
+
+// (0, b) * (vr, vi) with num-complex multiplication order
+inline void add_src(fnec_c64 &acc, double b, double vr, double vi, double scale_sign)
+{
+    double re = 0.0 * vr - b * vi, im = 0.0 * vi + b * vr;
+    if (scale_sign != 1.0) {
+        re *= scale_sign;
+        im *= scale_sign;
+    }
+    acc.re = acc.re + re;
+    acc.im = acc.im + im;
+}
+
+int hallen_rhs(const std::vector<Card> &cards, const std::vector<Seg> &s, double freq,
+               std::vector<fnec_c64> &rhs, std::vector<double> &cosv)
+{
+    const size_t n = s.size();
+    const Card *first = nullptr;
+    for (const Card &c : cards) {
+        if (!card_is(c, "EX") || ex_plane(c) || ex_current(c))
+            continue;
+        if (!ex_voltage(c))
+            return FNEC_INVALID_ARG;
+        if (!first)
+            first = &c;
+    }
+    rhs.assign(n, fnec_c64{0.0, 0.0});
+    cosv.assign(n, 0.0);
+    if (!first)
+        return FNEC_OK;
+    int feed = find_seg(s, ex_tag(*first), ex_seg(*first));
+    if (feed < 0)
+        return FNEC_INVALID_ARG;
+    const double k = 2.0 * kPi * freq / kC0;
+    const double scale = 2.0 * kPi / kEta0;
+    const uint32_t driven = s[feed].tag;
+    std::map<uint32_t, std::pair<size_t, fnec_c64>> src;
+    src[driven] = {(size_t)feed, fnec_c64{ex_vr(*first), ex_vi(*first)}};
+    for (const Card &c : cards) {
+        if (!card_is(c, "EX") || !ex_voltage(c) || ex_tag(c) == driven)
+            continue;
+        int idx = find_seg(s, ex_tag(c), ex_seg(c));
+        if (idx >= 0)
+            src[ex_tag(c)] = {(size_t)idx, fnec_c64{ex_vr(c), ex_vi(c)}};
+    }
+    auto comps = merge_collinear(s);
+    std::vector<size_t> comp_of(n, 0);
+    for (size_t ci = 0; ci < comps.size(); ++ci)
+        for (size_t m = comps[ci].first; m <= comps[ci].second && m < n; ++m)
+            comp_of[m] = ci;
+    for (size_t m = 0; m < n; ++m) {
+        const Seg &a = s[comps[comp_of[m]].first], &b = s[comps[comp_of[m]].second];
+        double wm[3] = {(a.mid[0] + b.mid[0]) / 2.0, (a.mid[1] + b.mid[1]) / 2.0, (a.mid[2] + b.mid[2]) / 2.0};
+        double dl[3] = {s[m].mid[0] - wm[0], s[m].mid[1] - wm[1], s[m].mid[2] - wm[2]};
+        double sl = dl[0] * a.dir[0] + dl[1] * a.dir[1] + dl[2] * a.dir[2];
+        cosv[m] = std::cos(k * sl);
+    }
+    for (auto &kv : src) {
+        size_t fi = kv.second.first;
+        fnec_c64 v = kv.second.second;
+        auto cr = comps[comp_of[fi]];
+        for (size_t m = cr.first; m <= cr.second; ++m) {
+            double d[3] = {s[m].mid[0] - s[fi].mid[0], s[m].mid[1] - s[fi].mid[1], s[m].mid[2] - s[fi].mid[2]};
+            double sv = d[0] * s[fi].dir[0] + d[1] * s[fi].dir[1] + d[2] * s[fi].dir[2];
+            add_src(rhs[m], -scale * std::sin(k * std::fabs(sv)), v.re, v.im, 1.0);
+        }
+    }
+    return FNEC_OK;
+}
+
+int hallen_rhs_paths(const std::vector<Card> &cards, const std::vector<Seg> &s, double freq,
+                     const Paths &P, std::vector<fnec_c64> &rhs, std::vector<double> &cosv)
+{
+    const size_t n = s.size();
+    const double k = 2.0 * kPi * freq / kC0;
+    const double scale = 2.0 * kPi / kEta0;
+    cosv.assign(n, 0.0);
+    rhs.assign(n, fnec_c64{0.0, 0.0});
+    for (size_t m = 0; m < n; ++m)
+        cosv[m] = P.sign_of[m] * std::cos(k * P.s_of[m]);
+    for (const Card &c : cards) {
+        if (!card_is(c, "EX") || ex_plane(c) || ex_current(c))
+            continue;
+        if (!ex_voltage(c))
+            return FNEC_INVALID_ARG;
+        int fi = find_seg(s, ex_tag(c), ex_seg(c));
+        if (fi < 0)
+            return FNEC_INVALID_ARG;
+        const double vr = ex_vr(c), vi = ex_vi(c);
+        const size_t sp = P.path_of[fi];
+        const double ssrc = P.s_of[fi], fs = P.sign_of[fi];
+        for (size_t m = 0; m < n; ++m) {
+            if (P.path_of[m] != sp)
+                continue;
+            double ds = std::fabs(P.s_of[m] - ssrc);
+            add_src(rhs[m], -scale * std::sin(k * ds), vr, vi, P.sign_of[m] * fs);
+        }
+    }
+    return FNEC_OK;
+}
+
+// ---- Rust number formatting (nec_report/src/lib.rs:437-482) ----
+
+std::string fmt_fixed(double v, int prec)
+{
+    if (std::isnan(v))
+        return "NaN";
+    if (std::isinf(v))
+        return v < 0 ? "-inf" : "inf";
+    char b[64];
+    std::snprintf(b, sizeof(b), "%.*f", prec, v);
+    return b;
+}
+
+std::string fmt_exp(double v, int prec)
+{
+    if (std::isnan(v) || std::isinf(v))
+        return fmt_fixed(v, prec);
+    char b[64];
+    std::snprintf(b, sizeof(b), "%.*e", prec, v);
+    char *e = std::strchr(b, 'e');
+    int ex = std::atoi(e + 1);
+    *e = 0;
+    return std::string(b) + "e" + std::to_string(ex);
+}
+
+bool deck_unsupported(const std::vector<Card> &cards)
+{
+    for (const Card &c : cards) {
+        if (card_is(c, "LD") || card_is(c, "TL") || card_is(c, "NT"))
+            return true;
+        if (card_is(c, "EX") && (ex_plane(c) || ex_current(c)))
+            return true;
+    }
+    return false;
+}
+
+} // namespace
+
+extern "C" {
+
+fnec_deck *fnec_deck_parse(const char *text, char *err, size_t errlen)
+{
+    if (err && errlen)
+        err[0] = 0;
+    if (!text) {
+        if (err && errlen)
+            std::snprintf(err, errlen, "null deck text");
+        return nullptr;
+    }
+    fnec_deck *d = new fnec_deck;
+    const char *p = text;
+    int lineno = 0;
+    static const char *known[] = {"GW", "GE", "GM", "GR", "GN", "EX", "NE", "NH", "FR", "RP", "LD", "TL", "NT", "PT"};
+    while (*p) {
+        const char *eol = std::strchr(p, '\n');
+        std::string line = eol ? std::string(p, eol) : std::string(p);
+        p = eol ? eol + 1 : p + line.size();
+        ++lineno;
+        std::vector<std::string> toks;
+        size_t i = 0;
+        while (i < line.size()) {
+            while (i < line.size() && std::isspace((unsigned char)line[i]))
+                ++i;
+            if (i >= line.size())
+                break;
+            size_t j = i;
+            while (j < line.size() && !std::isspace((unsigned char)line[j]))
+                ++j;
+            toks.push_back(line.substr(i, j - i));
+            i = j;
+        }
+        if (toks.empty())
+            continue;
+        std::string mn = toks[0];
+        for (auto &c : mn)
+            c = (char)std::toupper((unsigned char)c);
+        if (mn == "CM" || mn == "CE")
+            continue;
+        if (mn == "EN")
+            break;
+        bool isknown = false;
+        for (const char *k : known)
+            if (mn == k)
+                isknown = true;
+        if (!isknown)
+            continue; // unknown card: warning only (nec_parser lib.rs:403-408)
+        int nf = (int)toks.size() - 1, need = 0, nparse = 0;
+        if (mn == "GW") { need = 9; nparse = 9; }
+        else if (mn == "GE") { nparse = nf > 0 ? 1 : 0; }
+        else if (mn == "GM") { need = 9; nparse = 9; }
+        else if (mn == "GR") { need = 3; nparse = 3; }
+        else if (mn == "GN") { need = 1; nparse = nf > 5 ? 6 : (nf > 4 ? 5 : 1); }
+        else if (mn == "EX" || mn == "NE" || mn == "NH") { need = 4; nparse = std::min(nf, 10); }
+        else if (mn == "FR") { need = 4; nparse = nf >= 6 ? 6 : 4; }
+        else if (mn == "RP") { need = 7; nparse = nf >= 8 ? 8 : 7; }
+        else if (mn == "LD") { need = 4; nparse = std::min(nf, 7); }
+        else if (mn == "TL") { need = 8; nparse = nf > 8 ? 9 : 8; }
+        if (nf < need) {
+            if (err && errlen)
+                std::snprintf(err, errlen, "line %d: %s needs %d fields, got %d", lineno, mn.c_str(), need, nf);
+            delete d;
+            return nullptr;
+        }
+        Card c;
+        c.mn = mn;
+        c.nf = nf;
+        for (int q = 0; q < nparse && q < 16; ++q)
+            if (!parse_f64(toks[1 + q], c.f[q])) {
+                if (err && errlen)
+                    std::snprintf(err, errlen, "line %d: %s field %d: cannot parse '%s'", lineno,
+                                  mn.c_str(), q + 1, toks[1 + q].c_str());
+                delete d;
+                return nullptr;
+            }
+        d->cards.push_back(c);
+    }
+    return d;
+}
+
+void fnec_deck_free(fnec_deck *d) { delete d; }
+
+size_t fnec_deck_frequencies(const fnec_deck *d, int semantics, double *out, size_t max)
+{
+    if (!d)
+        return 0;
+    size_t cnt = 0;
+    auto put = [&](double v) {
+        if (out && cnt < max)
+            out[cnt] = v;
+        ++cnt;
+    };
+    for (const Card &c : d->cards) {
+        if (!card_is(c, "FR"))
+            continue;
+        int fi = c.nf >= 6 ? 4 : 2;
+        uint32_t type = as_u32(c.f[0]), steps = std::max<uint32_t>(as_u32(c.f[1]), 1);
+        double f = c.f[fi], st = c.f[fi + 1];
+        auto powi = [](double a, int b) {
+            double r = 1.0;
+            for (;;) {
+                if (b & 1)
+                    r *= a;
+                b /= 2;
+                if (b == 0)
+                    break;
+                a *= a;
+            }
+            return r;
+        };
+        if (semantics == FNEC_SEM_PY) {
+            for (uint32_t i = 0; i < steps; ++i)
+                put((type == 1 ? f * powi(st, (int)i) : f + st * (double)i) * 1e6);
+            continue;
+        }
+        if (type == 0)
+            for (uint32_t i = 0; i < steps; ++i)
+                put((f + (double)i * st) * 1e6);
+        else if (type == 1)
+            for (uint32_t i = 0; i < steps; ++i)
+                put(f * powi(st, (int)i) * 1e6);
+        else
+            put(f * 1e6);
+        break;
+    }
+    return cnt;
+}
+
+size_t fnec_deck_segment_count(fnec_deck *d)
+{
+    if (!d)
+        return 0;
+    if (!d->geo_built) {
+        d->geo_built = true;
+        d->geo_status = build_geometry(d->cards, d->segs, d->geo_err);
+    }
+    if (d->geo_status) {
+        fnec::set_error(d->geo_err);
+        return 0;
+    }
+    return d->segs.size();
+}
+
+int fnec_deck_build(fnec_deck *d, int semantics, double freq_hz, fnec_deck_problem *out)
+{
+    if (!d || !out) {
+        fnec::set_error("null deck or output");
+        return FNEC_INVALID_ARG;
+    }
+    if (fnec_deck_segment_count(d) == 0)
+        return FNEC_INVALID_ARG;
+    if (deck_unsupported(d->cards)) {
+        fnec::set_error("LD/TL/NT, plane-wave and current-source decks stay on the CPU path");
+        return FNEC_UNAVAILABLE;
+    }
+    const std::vector<Seg> &s = d->segs;
+    const size_t n = s.size();
+    std::memset(out, 0, sizeof(*out));
+    d->fsegs.resize(n);
+    d->seg_tag.resize(n);
+    d->seg_index.resize(n);
+    for (size_t i = 0; i < n; ++i) {
+        for (int c = 0; c < 3; ++c) {
+            d->fsegs[i].mid[c] = s[i].mid[c];
+            d->fsegs[i].dir[c] = s[i].dir[c];
+        }
+        d->fsegs[i].length = s[i].length;
+        d->fsegs[i].radius = s[i].radius;
+        d->seg_tag[i] = s[i].tag;
+        d->seg_index[i] = s[i].idx;
+    }
+    Ground g = ground_model(d->cards);
+    fnec_ground fg{g.kind == 1 ? FNEC_GROUND_PEC : (g.kind == 2 ? FNEC_GROUND_FINITE : FNEC_GROUND_FREE),
+                   g.eps_r, g.sigma};
+    std::vector<fnec_c64> vvec;
+    if (semantics != FNEC_SEM_WORKER && build_excitation(d->cards, s, vvec)) {
+        fnec::set_error("EX: bad excitation");
+        return FNEC_INVALID_ARG;
+    }
+    d->wf.clear(); d->wl.clear(); d->ja.clear(); d->jb.clear(); d->js.clear();
+    d->path_of.clear(); d->free_ends.clear();
+    int route = 0;
+    if (semantics == FNEC_SEM_CLI) {
+        Paths P = conductor_paths(s);
+        if (P.ok && P.count > 0 && P.nontrivial) {
+            route = 1;
+            if (hallen_rhs_paths(d->cards, s, freq_hz, P, d->rhs, d->cosv)) {
+                fnec::set_error("EX: bad excitation");
+                return FNEC_INVALID_ARG;
+            }
+            d->path_of = P.path_of;
+            d->free_ends = P.free_ends;
+        }
+    }
+    if (route == 0) {
+        if (hallen_rhs(d->cards, s, freq_hz, d->rhs, d->cosv)) {
+            fnec::set_error("EX: bad excitation");
+            return FNEC_INVALID_ARG;
+        }
+        auto wires = semantics == FNEC_SEM_CLI ? merge_collinear(s) : wire_endpoints(s);
+        for (auto &w : wires) {
+            d->wf.push_back(w.first);
+            d->wl.push_back(w.second);
+        }
+        auto js = detect_junctions(s, wires, 1e-6);
+        std::vector<size_t> comp(n, 0);
+        for (size_t ci = 0; ci < wires.size(); ++ci)
+            for (size_t m = wires[ci].first; m <= wires[ci].second; ++m)
+                comp[m] = ci;
+        for (auto &j : js) {
+            if (semantics == FNEC_SEM_CLI && comp[j.a] == comp[j.b])
+                continue;
+            d->ja.push_back(j.a);
+            d->jb.push_back(j.b);
+            d->js.push_back(j.sign);
+        }
+    }
+    // feedpoints (solve_session.rs:877-944; nec_worker solve.rs:243-268; fnec_py lib.rs:84-92)
+    d->feed_idx.clear(); d->feed_v.clear(); d->feed_tag.clear(); d->feed_seg.clear();
+    for (const Card &c : d->cards) {
+        if (!card_is(c, "EX") || ex_plane(c))
+            continue;
+        if (semantics == FNEC_SEM_WORKER && ex_type(c) != 0)
+            continue;
+        int idx = find_seg(s, ex_tag(c), ex_seg(c));
+        if (idx < 0)
+            continue;
+        fnec_c64 v;
+        if (semantics == FNEC_SEM_WORKER)
+            v = fnec_c64{ex_vr(c), ex_vi(c)};
+        else
+            v = fnec_c64{vvec[idx].re * s[idx].length, vvec[idx].im * s[idx].length};
+        d->feed_idx.push_back((size_t)idx);
+        d->feed_v.push_back(v);
+        d->feed_tag.push_back(s[idx].tag);
+        d->feed_seg.push_back(s[idx].idx);
+        if (semantics != FNEC_SEM_CLI)
+            break;
+    }
+    // RP points: all RP cards, phi outer / theta inner (farfield.rs:677-699)
+    d->theta.clear();
+    d->phi.clear();
+    if (semantics == FNEC_SEM_CLI)
+        for (const Card &c : d->cards) {
+            if (!card_is(c, "RP"))
+                continue;
+            int a = c.nf >= 8 ? 4 : 3;
+            uint32_t nt = std::max<uint32_t>(as_u32(c.f[1]), 1), np = std::max<uint32_t>(as_u32(c.f[2]), 1);
+            for (uint32_t ip = 0; ip < np; ++ip) {
+                double ph = c.f[a + 1] + (double)ip * c.f[a + 3];
+                for (uint32_t it = 0; it < nt; ++it) {
+                    d->theta.push_back(c.f[a] + (double)it * c.f[a + 2]);
+                    d->phi.push_back(ph);
+                }
+            }
+        }
+    fnec_request &r = out->req;
+    r.segs = d->fsegs.data();
+    r.n = n;
+    r.rhs = d->rhs.data();
+    r.cos_vec = d->cosv.data();
+    r.wire_first = d->wf.empty() ? nullptr : d->wf.data();
+    r.wire_last = d->wl.empty() ? nullptr : d->wl.data();
+    r.n_wires = d->wf.size();
+    r.j_a = d->ja.empty() ? nullptr : d->ja.data();
+    r.j_b = d->jb.empty() ? nullptr : d->jb.data();
+    r.j_sign = d->js.empty() ? nullptr : d->js.data();
+    r.n_junc = d->ja.size();
+    r.freq_hz = freq_hz;
+    r.ground = fg;
+    r.feed_index = d->feed_idx.empty() ? 0 : d->feed_idx[0];
+    r.v_feed = d->feed_v.empty() ? fnec_c64{0.0, 0.0} : d->feed_v[0];
+    r.rp_theta = d->theta.empty() ? nullptr : d->theta.data();
+    r.rp_phi = d->phi.empty() ? nullptr : d->phi.data();
+    r.rp_npts = d->theta.size();
+    out->route = route;
+    out->path_of = d->path_of.empty() ? nullptr : d->path_of.data();
+    out->free_ends = d->free_ends.empty() ? nullptr : d->free_ends.data();
+    out->n_free = d->free_ends.size();
+    out->n_feeds = d->feed_idx.size();
+    out->feed_idx = d->feed_idx.empty() ? nullptr : d->feed_idx.data();
+    out->feed_v = d->feed_v.empty() ? nullptr : d->feed_v.data();
+    out->feed_tag = d->feed_tag.empty() ? nullptr : d->feed_tag.data();
+    out->feed_seg = d->feed_seg.empty() ? nullptr : d->feed_seg.data();
+    out->seg_tag = d->seg_tag.data();
+    out->seg_index = d->seg_index.data();
+    return FNEC_OK;
+}
+
+int fnec_deck_report(fnec_deck *d, size_t freq_index, int precision, char *outbuf, size_t cap,
+                     size_t *needed)
+{
+    double freqs[4096];
+    size_t nf = fnec_deck_frequencies(d, FNEC_SEM_CLI, freqs, 4096);
+    if (freq_index >= nf || freq_index >= 4096) {
+        fnec::set_error("no such frequency");
+        return FNEC_INVALID_ARG;
+    }
+    const double f = freqs[freq_index];
+    fnec_deck_problem pr;
+    int st = fnec_deck_build(d, FNEC_SEM_CLI, f, &pr);
+    if (st)
+        return st;
+    const fnec_request &r = pr.req;
+    const size_t n = r.n;
+    std::vector<fnec_c64> x(n + std::max<size_t>(pr.route ? (pr.n_free / 2) : r.n_wires, 1));
+    fnec_solve_stats stats;
+    if (pr.route == 1)
+        st = fnec_hip_solve_hallen_paths(r.segs, n, r.rhs, r.cos_vec, pr.path_of, pr.free_ends,
+                                         pr.n_free, f, &r.ground, precision, x.data(), &stats);
+    else
+        st = fnec_hip_solve_hallen(r.segs, n, r.rhs, r.cos_vec, r.wire_first, r.wire_last, r.n_wires,
+                                   r.j_a, r.j_b, r.j_sign, r.n_junc, f, &r.ground, precision,
+                                   x.data(), &stats);
+    if (st)
+        return st;
+    std::string o;
+    o += "FNEC FEEDPOINT REPORT\nFORMAT_VERSION 1\n";
+    o += "FREQ_MHZ " + fmt_fixed(f / 1e6, 6) + "\n";
+    o += "SOLVER_MODE hallen\nPULSE_RHS Nec2\n\nFEEDPOINTS\nTAG SEG V_RE V_IM I_RE I_IM Z_RE Z_IM\n";
+    double p_in = 0.0;
+    for (size_t i = 0; i < pr.n_feeds; ++i) {
+        fnec_c64 v = pr.feed_v[i], c = x[pr.feed_idx[i]], z;
+        if (std::hypot(c.re, c.im) > 1e-60) {
+            double ns = c.re * c.re + c.im * c.im;
+            z.re = (v.re * c.re + v.im * c.im) / ns;
+            z.im = (v.im * c.re - v.re * c.im) / ns;
+        } else
+            z = v;
+        p_in += 0.5 * (v.re * c.re - v.im * (-c.im));
+        o += std::to_string(pr.feed_tag[i]) + " " + std::to_string(pr.feed_seg[i]) + " " +
+             fmt_fixed(v.re, 6) + " " + fmt_fixed(v.im, 6) + " " + fmt_fixed(c.re, 6) + " " +
+             fmt_fixed(c.im, 6) + " " + fmt_fixed(z.re, 6) + " " + fmt_fixed(z.im, 6) + "\n";
+    }
+    int nsrc = 0;
+    for (const Card &c : d->cards)
+        if (card_is(c, "EX"))
+            ++nsrc;
+    if (nsrc) {
+        o += "\nSOURCES\nN_SOURCES " + std::to_string(nsrc) + "\nTYPE TAG SEG I4 V_RE V_IM\n";
+        for (const Card &c : d->cards)
+            if (card_is(c, "EX"))
+                o += std::to_string(as_u32(c.f[0])) + " " + std::to_string(ex_tag(c)) + " " +
+                     std::to_string(ex_seg(c)) + " " + std::to_string(as_u32(c.f[3])) + " " +
+                     fmt_fixed(ex_vr(c), 6) + " " + fmt_fixed(ex_vi(c), 6) + "\n";
+    }
+    o += "\nCURRENTS\nTAG SEG I_RE I_IM I_MAG I_PHASE\n";
+    for (size_t i = 0; i < n; ++i) {
+        fnec_c64 c = x[i];
+        o += std::to_string(pr.seg_tag[i]) + " " + std::to_string(pr.seg_index[i]) + " " +
+             fmt_exp(c.re, 6) + " " + fmt_exp(c.im, 6) + " " + fmt_exp(std::hypot(c.re, c.im), 6) +
+             " " + fmt_fixed(rad2deg(std::atan2(c.im, c.re)), 4) + "\n";
+    }
+    const size_t np = r.rp_npts;
+    bool normalize = false, avg = false;
+    for (const Card &c : d->cards)
+        if (card_is(c, "RP") && c.nf >= 8) {
+            uint32_t xn = as_u32(c.f[3]);
+            if ((xn / 1000) % 10 != 0)
+                normalize = true;
+            if (xn % 10 != 0)
+                avg = true;
+        }
+    std::vector<fnec_rp_row> rows(np);
+    if (np) {
+        double total = 0.0;
+        st = fnec_hip_rp(r.segs, x.data(), n, f, &r.ground, r.rp_theta, r.rp_phi, np, rows.data(), &total);
+        if (st)
+            return st;
+        if (r.ground.kind == FNEC_GROUND_FINITE && p_in > 0.0) {
+            // radiation efficiency (farfield.rs:308-322, solve_session.rs:1664-1684):
+            // the integral at k = 2 pi f / c0
+            double integral = 0.0;
+            st = fnec_hip_rp(r.segs, x.data(), n, f, &r.ground, nullptr, nullptr, 0, nullptr, &integral);
+            if (st)
+                return st;
+            const double k = 2.0 * kPi * f / kC0;
+            double p_rad = k * k * kEta0 / (32.0 * kPi * kPi) * integral;
+            double eta = p_rad / p_in;
+            eta = eta < 0.0 ? 0.0 : (eta > 1.0 ? 1.0 : eta);
+            double ddb = 10.0 * std::log10(eta);
+            for (auto &row : rows)
+                for (double *g : {&row.gain_total_dbi, &row.gain_theta_dbi, &row.gain_phi_dbi})
+                    if (*g > -900.0)
+                        *g += ddb;
+        }
+        o += "\nRADIATION_PATTERN\nN_POINTS " + std::to_string(np) +
+             "\nTHETA PHI GAIN_DB GAIN_V_DB GAIN_H_DB AXIAL_RATIO\n";
+        for (size_t i = 0; i < np; ++i)
+            o += fmt_fixed(r.rp_theta[i], 4) + " " + fmt_fixed(r.rp_phi[i], 4) + " " +
+                 fmt_fixed(rows[i].gain_total_dbi, 4) + " " + fmt_fixed(rows[i].gain_theta_dbi, 4) +
+                 " " + fmt_fixed(rows[i].gain_phi_dbi, 4) + " " + fmt_fixed(rows[i].axial_ratio, 4) + "\n";
+        if (normalize) {
+            double peak = -INFINITY;
+            for (auto &row : rows)
+                peak = std::fmax(peak, row.gain_total_dbi);
+            o += "\nNORMALIZED_PATTERN\nN_POINTS " + std::to_string(np) + "\nTHETA PHI GAIN_NORM_DB\n";
+            for (size_t i = 0; i < np; ++i) {
+                double g = rows[i].gain_total_dbi;
+                o += fmt_fixed(r.rp_theta[i], 4) + " " + fmt_fixed(r.rp_phi[i], 4) + " " +
+                     fmt_fixed(g > -900.0 ? g - peak : g, 4) + "\n";
+            }
+        }
+        if (avg) {
+            double num = 0.0, den = 0.0;
+            for (size_t i = 0; i < np; ++i) {
+                double g = rows[i].gain_total_dbi;
+                if (g <= -900.0)
+                    continue;
+                double w = std::sin(deg2rad(r.rp_theta[i]));
+                num += std::pow(10.0, g / 10.0) * w;
+                den += w;
+            }
+            if (den > 0.0)
+                o += "\nAVERAGE_POWER_GAIN " + fmt_fixed(num / den, 4) + "\n";
+        }
+    }
+    if (needed)
+        *needed = o.size() + 1;
+    if (outbuf && cap) {
+        size_t k = std::min(cap - 1, o.size());
+        std::memcpy(outbuf, o.data(), k);
+        outbuf[k] = 0;
+    }
+    return FNEC_OK;
+}
+
+} // extern "C"

@@ -250,7 +250,9 @@ __global__ __launch_bounds__(256) void k_gram(const Prob *__restrict__ probs,
                     continue;
                 if (diag && gi < gj)
                     continue;
-                const double re = accr[a][b][v], im = acci[a][b][v];
+                const double re = accr[a][b][v];
+                // the diagonal of Z^H Z is real: every term re*im - im*re cancels exactly
+                const double im = gi == gj ? 0.0 : acci[a][b][v];
                 P.ar[(size_t)gj * P.lda + gi] = re;
                 P.ai[(size_t)gj * P.lda + gi] = im;
                 if (gi != gj) {

@@ -207,6 +207,14 @@ int fnec_hip_debug_normal_eqs(const fnec_seg *segs, size_t n, const fnec_c64 *rh
                               double freq_hz, const fnec_ground *ground, fnec_c64 *aug_out);
 int fnec_hip_debug_mfma_probe(const double *a16x4, const double *b4x16, double *lanes_out);
 
+/* Per-kernel timing with HIP events on the solve stream (off by default).
+ * Kernel classes: "fill", "gram", "lu_panel", "lu_trsm", "lu_update",
+ * "backsub", "rp". out4 = {launches, total microseconds, algorithmic flops,
+ * algorithmic bytes} accumulated since the last reset. */
+void fnec_hip_profile_enable(int on);
+void fnec_hip_profile_reset(void);
+int fnec_hip_profile_read(const char *kernel, double *out4);
+
 #ifdef __cplusplus
 }
 #endif

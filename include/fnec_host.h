@@ -1,0 +1,78 @@
+/*
+ * fnec_host — the host side above the fnec_hip C ABI: a NEC deck parser subset
+ * and the geometry / excitation / topology builders the reference's callers
+ * run before they reach nec_accel, plus the CLI pipeline and report contract
+ * v1. Mirrors (file:line in the reference):
+ *   nec_parser::parse                       crates/nec_parser/src/lib.rs:84-330
+ *   build_geometry (GW/GM/GR)               crates/nec_solver/src/geometry.rs:676-879
+ *   ground_model_from_deck                  geometry.rs:71-111
+ *   wire_endpoints_from_segs / merge        geometry.rs:168-241
+ *   detect_wire_junctions                   geometry.rs:274-339
+ *   build_conductor_paths                   geometry.rs:404-584
+ *   build_excitation / build_hallen_rhs(_paths) crates/nec_solver/src/excitation.rs:78-341
+ *   solve_frequency_point (Hallén route)    apps/nec-cli/src/solve_session.rs:1186-1775
+ *   worker solve                            crates/nec_worker/src/solve.rs:150-289
+ *   render_text_report                      crates/nec_report/src/lib.rs:307-482
+ * Unsupported cards for this path (LD, TL, NT, plane-wave and current-source
+ * EX) make fnec_deck_build return FNEC_UNAVAILABLE, the signal on which the
+ * reference's callers keep the CPU path.
+ */
+#ifndef FNEC_HOST_H
+#define FNEC_HOST_H
+
+#include "fnec_hip.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { FNEC_SEM_CLI = 0, FNEC_SEM_WORKER = 1, FNEC_SEM_PY = 2 };
+
+typedef struct fnec_deck fnec_deck;
+
+/* Parse deck text. NULL on a fatal parse error (message in err). */
+fnec_deck *fnec_deck_parse(const char *text, char *err, size_t errlen);
+void fnec_deck_free(fnec_deck *d);
+
+/* Frequencies in Hz: the first FR card (CLI, worker) or all FR cards (Python
+ * binding). Returns the count; writes at most max values. */
+size_t fnec_deck_frequencies(const fnec_deck *d, int semantics, double *out, size_t max);
+
+/* Segment count of the built geometry, or 0 with the error in
+ * fnec_hip_last_error() when the geometry cannot be built. */
+size_t fnec_deck_segment_count(fnec_deck *d);
+
+/* Hallén inputs for one frequency, with the caller semantics of
+ * FNEC_SEM_CLI (merged collinear wires, cross-component junctions, or the
+ * conductor-path route for bent / reversed degree-2 chains) or
+ * FNEC_SEM_WORKER / FNEC_SEM_PY (raw wire ranges, all junctions). All
+ * pointers stay valid until the next call on the same deck. */
+typedef struct {
+    int route;                 /* 0: solve_hallen, 1: solve_hallen_paths */
+    fnec_request req;          /* segments, rhs, cos, wires, junctions, ground,
+                                  first feed (index + voltage), RP grid */
+    const size_t *path_of;     /* route 1 */
+    const size_t *free_ends;
+    size_t n_free;
+    size_t n_feeds;            /* feedpoint rows (CLI: every non plane-wave EX) */
+    const size_t *feed_idx;
+    const fnec_c64 *feed_v;    /* V/dl * dl (CLI, Python) or the EX voltage (worker) */
+    const uint32_t *feed_tag;
+    const uint32_t *feed_seg;
+    const uint32_t *seg_tag;   /* per segment tag and 1-based index */
+    const uint32_t *seg_index;
+} fnec_deck_problem;
+
+int fnec_deck_build(fnec_deck *d, int semantics, double freq_hz, fnec_deck_problem *out);
+
+/* CLI pipeline for one frequency of the deck on the GPU: solve, feedpoint
+ * rows, currents, RP with normalisation, finite-ground efficiency, and the
+ * report contract v1 text (FREQ_MHZ ... RADIATION_PATTERN ...). Writes up to
+ * cap bytes (NUL-terminated) and the full length to *needed. */
+int fnec_deck_report(fnec_deck *d, size_t freq_index, int precision, char *out, size_t cap,
+                     size_t *needed);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

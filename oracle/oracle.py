@@ -323,7 +323,32 @@ def deck_inputs(text: str, freq_hz: float, semantics: str = "worker") -> dict:
             keep = comp[ja] != comp[jb]
             ja, jb, js = ja[keep], jb[keep], js[keep]
         g = L.or_ground_model(d)
-        return {"segs": segs, "rhs": rhs, "cos": cosv, "wf": wf, "wl": wl, "ja": ja, "jb": jb,
-                "js": js, "ground": (g.kind, g.eps_r, g.sigma), "n": n}
+        out = {"segs": segs, "rhs": rhs, "cos": cosv, "wf": wf, "wl": wl, "ja": ja, "jb": jb,
+               "js": js, "ground": (g.kind, g.eps_r, g.sigma), "n": n, "route": 0}
+        if semantics == "cli":
+            L.or_conductor_paths.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,
+                                             C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]
+            L.or_build_hallen_rhs_paths.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_double, C.c_void_p,
+                                                    C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+            path_of = np.zeros(n, np.int32)
+            sign_of = np.zeros(n)
+            s_of = np.zeros(n)
+            fa = np.zeros(n + 1, np.int32)
+            fb = np.zeros(n + 1, np.int32)
+            nontriv = C.c_int(0)
+            npaths = L.or_conductor_paths(_p(segs), n, _p(path_of), _p(sign_of), _p(s_of), _p(fa), _p(fb),
+                                          n + 1, C.cast(C.pointer(nontriv), C.c_void_p))
+            if npaths > 0 and nontriv.value:
+                rhs2 = np.zeros(n, np.complex128)
+                cos2 = np.zeros(n)
+                st = L.or_build_hallen_rhs_paths(d, _p(segs), n, float(freq_hz), _p(path_of), _p(sign_of),
+                                                  _p(s_of), _p(rhs2), _p(cos2))
+                if st:
+                    raise RuntimeError(f"excitation error {st}")
+                fe = np.empty(2 * npaths, np.int32)
+                fe[0::2] = fa[:npaths]
+                fe[1::2] = fb[:npaths]
+                out.update(route=1, rhs=rhs2, cos=cos2, path_of=path_of, free_ends=fe)
+        return out
     finally:
         L.or_deck_free(d)
