@@ -52,7 +52,7 @@ class Ground(C.Structure):
 class SolveStats(C.Structure):
     _fields_ = [("abs_res", C.c_double), ("rel_res", C.c_double), ("diag_spread", C.c_double),
                 ("backward_error", C.c_double), ("refine_iters", C.c_int), ("n_swaps", C.c_int),
-                ("rows", C.c_int), ("unknowns", C.c_int), ("t_fill_us", C.c_double),
+                ("exact_panels", C.c_int), ("rows", C.c_int), ("unknowns", C.c_int), ("t_fill_us", C.c_double),
                 ("t_gram_us", C.c_double), ("t_lu_us", C.c_double), ("t_solve_us", C.c_double),
                 ("t_total_us", C.c_double)]
 
@@ -372,4 +372,63 @@ def profile(enable: bool | None = None, reset: bool = False) -> dict:
         buf = np.zeros(4)
         _check(L.fnec_hip_profile_read(k.encode(), _p(buf)))
         out[k] = tuple(float(v) for v in buf)
+    return out
+
+
+def make_request(p: dict, tag: int = 0, rp_theta=None, rp_phi=None) -> tuple:
+    """fnec_request for a Deck.problem dict (worker route); returns
+    (request, keepalive) — keep the second item alive while the request is used."""
+    keep = {k: np.ascontiguousarray(p[k]) for k in ("segs", "rhs", "cos", "wf", "wl", "ja", "jb", "js")}
+    q = Request()
+    q.segs = _p(keep["segs"])
+    q.n = len(keep["segs"])
+    q.rhs = _p(keep["rhs"])
+    q.cos_vec = _p(keep["cos"])
+    q.wire_first = _p(keep["wf"])
+    q.wire_last = _p(keep["wl"])
+    q.n_wires = len(keep["wf"])
+    q.j_a = _p(keep["ja"])
+    q.j_b = _p(keep["jb"])
+    q.j_sign = _p(keep["js"])
+    q.n_junc = len(keep["ja"])
+    q.freq_hz = p["freq_hz"]
+    q.ground = p["ground_struct"]
+    q.feed_index = int(p["feed_index"])
+    q.v_feed = C64(p["v_feed"].real, p["v_feed"].imag)
+    if rp_theta is not None:
+        keep["th"] = np.ascontiguousarray(rp_theta, np.float64)
+        keep["ph"] = np.ascontiguousarray(rp_phi, np.float64)
+        q.rp_theta = _p(keep["th"])
+        q.rp_phi = _p(keep["ph"])
+        q.rp_npts = len(keep["th"])
+    q.tag = tag
+    return q, keep
+
+
+def solve_batch(problems: list, precision: int = PREC_FP64, rp_grids: list | None = None):
+    """fnec_hip_solve_batch over Deck.problem dicts; returns a list of dicts
+    with z_in, x (currents + constants), stats and the RP rows if requested."""
+    reqs = (Request * len(problems))()
+    res = (Result * len(problems))()
+    keeps, xs, rps = [], [], []
+    for i, p in enumerate(problems):
+        th, ph = (rp_grids[i] if rp_grids and rp_grids[i] is not None else (None, None))
+        q, k = make_request(p, i, th, ph)
+        reqs[i] = q
+        keeps.append(k)
+        x = np.zeros(len(p["segs"]) + max(len(p["wf"]), 1), np.complex128)
+        xs.append(x)
+        res[i].x = _p(x)
+        if th is not None:
+            rows = np.zeros((len(th), 4))
+            rps.append(rows)
+            res[i].rp = _p(rows)
+        else:
+            rps.append(None)
+    _check(lib().fnec_hip_solve_batch(C.cast(reqs, C.c_void_p), len(problems), precision, C.cast(res, C.c_void_p)))
+    out = []
+    for i in range(len(problems)):
+        r = res[i]
+        out.append({"status": r.status, "z_in": complex(r.z_in.re, r.z_in.im), "x": xs[i], "rp": rps[i],
+                    "rp_total": r.rp_total, "stats": r.stats.as_dict()})
     return out

@@ -49,6 +49,11 @@ struct Prob {
     double *x;             // s complex, interleaved
     double *stats;         // [0] abs_res, [1] rel_res, [2] diag_spread
     double *scratch;       // residual partial sums
+    double *pr, *pi;       // panel workspace, lda x kPanel (fast no-swap path)
+    unsigned long long *colmax; // kPanel: max |Schur entry| below the diagonal, as bits
+    double *diagmag;       // kPanel: |u_jj|
+    int *fast_ok;          // [0]: the fast path factored the current panel
+    int pad1;
 };
 
 // Far-field problem (one RP grid + normalisation integral).

@@ -24,6 +24,9 @@ __global__ void k_gram(const Prob *, const int4 *);
 __global__ void k_gram_cols(const Prob *, const int2 *);
 __global__ void k_gram_fix(const Prob *, const int *);
 __global__ void k_lu_panel(const Prob *, const int *, int);
+__global__ void k_panel_diag(const Prob *, const int *, int);
+__global__ void k_panel_rows(const Prob *, const int2 *, int);
+__global__ void k_panel_commit(const Prob *, const int2 *, int);
 __global__ void k_lu_trsm(const Prob *, const int2 *, int);
 __global__ void k_lu_update(const Prob *, const int4 *, int);
 __global__ void k_backsub(const Prob *, const int *);
@@ -452,6 +455,7 @@ struct Plan {
     struct Step {
         int p;
         size_t panel_off, panel_cnt, trsm_off, trsm_cnt, upd_off, upd_cnt;
+        size_t rows_off, rows_cnt, commit_off, commit_cnt;
     };
     std::vector<Step> steps;
     // algorithmic work per launch (profiling)
@@ -461,7 +465,7 @@ struct Plan {
     size_t bytes = 0;
     size_t a_begin = 0, a_end = 0; // byte range of the A planes (zeroed per run)
     std::vector<size_t> off_seg, off_z, off_a, off_cos, off_rhs, off_runs, off_cons, off_sign,
-        off_ipiv, off_info, off_x, off_stats, off_scr;
+        off_ipiv, off_info, off_x, off_stats, off_scr, off_panel, off_pmeta;
     size_t off_probs = 0, off_ints = 0;
 };
 
@@ -480,6 +484,7 @@ size_t problem_bytes(int n, int s, int ncons, int nruns)
     b += 2 * 8 * lda * (s + 1) + 256;
     b += (size_t)n * 8 * kSegStride + 8 * n + 16 * n + 12 * nruns + 8 * ncons + 8 * ncons + 4 * s;
     b += 16 * s + 64 + 32 + 32 * ((n + 255) / 256 + 1) + 12 * 256;
+    b += 2 * 8 * lda * kPanel + 16 * kPanel + 16 + 512;
     return b;
 }
 
@@ -501,6 +506,8 @@ static void plan_build(Plan &pl, const std::vector<const HostProblem *> &hps, bo
     pl.off_x.resize(np);
     pl.off_stats.resize(np);
     pl.off_scr.resize(np);
+    pl.off_panel.resize(np);
+    pl.off_pmeta.resize(np);
     // A planes first, contiguous, so one zero-fill covers them
     pl.a_begin = 0;
     for (size_t b = 0; b < np; ++b) {
@@ -520,10 +527,14 @@ static void plan_build(Plan &pl, const std::vector<const HostProblem *> &hps, bo
         pl.off_cons[b] = take(cur, 4 * h.cons.size() + 4);
         pl.off_sign[b] = take(cur, 8 * h.cons_sign.size() + 8);
         pl.off_ipiv[b] = take(cur, 4 * (size_t)h.s);
-        pl.off_info[b] = take(cur, 8);
+        pl.off_info[b] = take(cur, 16);
         pl.off_x[b] = take(cur, 16 * (size_t)h.s);
         pl.off_stats[b] = take(cur, 32);
         pl.off_scr[b] = take(cur, 32 * ((size_t)(h.n + 255) / 256 + 1));
+        if (with_solve) {
+            pl.off_panel[b] = take(cur, 2 * 8 * (size_t)roundup(h.s, 64) * kPanel);
+            pl.off_pmeta[b] = take(cur, 8 * kPanel + 8 * kPanel + 16);
+        }
     }
     // work lists
     auto push4 = [&](int a, int b, int c, int d) {
@@ -621,6 +632,27 @@ static void plan_build(Plan &pl, const std::vector<const HostProblem *> &hps, bo
                         push4((int)b, y, z, 0);
             }
             st.upd_cnt = (pl.ints.size() - st.upd_off) / 4;
+            align4();
+            st.rows_off = pl.ints.size();
+            for (size_t b = 0; b < np; ++b) {
+                int s = hps[b]->s;
+                if (s <= p)
+                    continue;
+                int jb = std::min(kPanel, s - p);
+                for (int y = 0; y < (s - p - jb + 255) / 256; ++y)
+                    push2((int)b, y);
+            }
+            st.rows_cnt = (pl.ints.size() - st.rows_off) / 2;
+            align4();
+            st.commit_off = pl.ints.size();
+            for (size_t b = 0; b < np; ++b) {
+                int s = hps[b]->s;
+                if (s <= p)
+                    continue;
+                for (int y = 0; y < (s - p + 255) / 256; ++y)
+                    push2((int)b, y);
+            }
+            st.commit_cnt = (pl.ints.size() - st.commit_off) / 2;
             double fu = 0, fp = 0, ft = 0;
             for (size_t b = 0; b < np; ++b) {
                 const int s = hps[b]->s;
@@ -687,6 +719,13 @@ static int plan_upload(Plan &pl, const std::vector<const HostProblem *> &hps, ch
         P.x = (double *)(base + pl.off_x[b]);
         P.stats = (double *)(base + pl.off_stats[b]);
         P.scratch = (double *)(base + pl.off_scr[b]);
+        if (with_solve) {
+            P.pr = (double *)(base + pl.off_panel[b]);
+            P.pi = P.pr + (size_t)P.lda * kPanel;
+            P.colmax = (unsigned long long *)(base + pl.off_pmeta[b]);
+            P.diagmag = (double *)(base + pl.off_pmeta[b] + 8 * kPanel);
+            P.fast_ok = (int *)(base + pl.off_pmeta[b] + 16 * kPanel);
+        }
         HIPCHK(hipMemcpyAsync(base + pl.off_seg[b], h.seg.data(), 8 * h.seg.size(), hipMemcpyHostToDevice, st));
         HIPCHK(hipMemcpyAsync(base + pl.off_cos[b], h.cosv.data(), 8 * h.cosv.size(), hipMemcpyHostToDevice, st));
         HIPCHK(hipMemcpyAsync(base + pl.off_rhs[b], h.rhs.data(), 8 * h.rhs.size(), hipMemcpyHostToDevice, st));
@@ -699,9 +738,9 @@ static int plan_upload(Plan &pl, const std::vector<const HostProblem *> &hps, ch
     }
     // info words start at zero; they are written only by device code after this
     {
-        std::vector<char> zeros(8, 0);
+        std::vector<char> zeros(16, 0);
         for (size_t b = 0; b < np; ++b)
-            HIPCHK(hipMemcpyAsync(base + pl.off_info[b], zeros.data(), 8, hipMemcpyHostToDevice, st));
+            HIPCHK(hipMemcpyAsync(base + pl.off_info[b], zeros.data(), 16, hipMemcpyHostToDevice, st));
     }
     HIPCHK(hipMemcpyAsync(base + pl.off_probs, pl.probs.data(), sizeof(Prob) * np, hipMemcpyHostToDevice, st));
     if (!pl.ints.empty())
@@ -751,6 +790,13 @@ static int plan_launch(Plan &pl, char *base, bool with_solve)
         const auto &s = pl.steps[si];
         if (s.panel_cnt) {
             hipEvent_t e = prof_begin();
+            hipLaunchKernelGGL(k_panel_diag, dim3((unsigned)s.panel_cnt), dim3(256), 0, st, dp,
+                               wl_ptr(base, pl, s.panel_off), s.p);
+            if (s.rows_cnt)
+                hipLaunchKernelGGL(k_panel_rows, dim3((unsigned)s.rows_cnt), dim3(256), 0, st, dp,
+                                   (const int2 *)wl_ptr(base, pl, s.rows_off), s.p);
+            hipLaunchKernelGGL(k_panel_commit, dim3((unsigned)s.commit_cnt), dim3(256), 0, st, dp,
+                               (const int2 *)wl_ptr(base, pl, s.commit_off), s.p);
             hipLaunchKernelGGL(k_lu_panel, dim3((unsigned)s.panel_cnt), dim3(512), 0, st, dp,
                                wl_ptr(base, pl, s.panel_off), s.p);
             prof_end(PC_LU_PANEL, e, pl.step_panel_flops[si], 0);
@@ -822,9 +868,9 @@ int solve_problems(const std::vector<const HostProblem *> &hps, std::vector<fnec
     status.assign(np, FNEC_OK);
     stats.assign(np, fnec_solve_stats{});
     for (size_t b = 0; b < np; ++b) {
-        int info[2];
+        int info[4];
         double sv[4];
-        HIPCHK(hipMemcpy(info, base + pl.off_info[b], 8, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(info, base + pl.off_info[b], 16, hipMemcpyDeviceToHost));
         HIPCHK(hipMemcpy(sv, base + pl.off_stats[b], 32, hipMemcpyDeviceToHost));
         status[b] = info[0] ? FNEC_SINGULAR : FNEC_OK;
         fnec_solve_stats &s = stats[b];
@@ -832,6 +878,7 @@ int solve_problems(const std::vector<const HostProblem *> &hps, std::vector<fnec
         s.rel_res = sv[1];
         s.diag_spread = sv[2];
         s.n_swaps = info[1];
+        s.exact_panels = info[2];
         s.rows = hps[b]->r;
         s.unknowns = hps[b]->s;
         s.t_fill_us = t_fill;

@@ -400,7 +400,7 @@ __global__ __launch_bounds__(kPanelThreads) void k_lu_panel(const Prob *__restri
 {
     const Prob &P = probs[plist[blockIdx.x]];
     const int s = P.s;
-    if (P.info[0] != 0 || p >= s)
+    if (P.info[0] != 0 || p >= s || P.fast_ok[0])
         return;
     const int jb = min(kPanel, s - p);
     const int t = threadIdx.x;
@@ -411,6 +411,8 @@ __global__ __launch_bounds__(kPanelThreads) void k_lu_panel(const Prob *__restri
     __shared__ double prow_r[kPanel], prow_i[kPanel];
     __shared__ int s_piv;
     __shared__ double s_best;
+    if (t == 0)
+        P.info[2] += 1; // panels factored on the exact interchange path
     for (int j = 0; j < jb; ++j) {
         const int col = p + j;
         const double *cr = ar + (size_t)col * lda, *ci = ai + (size_t)col * lda;
@@ -486,6 +488,177 @@ __global__ __launch_bounds__(kPanelThreads) void k_lu_panel(const Prob *__restri
         }
         __syncthreads();
     }
+}
+
+// ---- fast panel path: factor without interchanges, then prove none was due ----
+//
+// Partial pivoting on these Hermitian positive-definite Gram matrices rarely
+// interchanges rows. The fast path factors the panel without interchanges
+// and records, for every panel column j, the largest |a_rj| below the
+// diagonal at elimination step j. The reference keeps row j as pivot exactly
+// when every such entry is strictly smaller than |u_jj| (a tie selects the
+// later row, linear.rs:1345-1352), so when that holds for all columns the
+// factorisation is the partial-pivoting one with no interchange and is
+// committed; otherwise the panel is left untouched for k_lu_panel.
+
+// One workgroup per problem: LU of the jb x jb diagonal block in LDS.
+__global__ __launch_bounds__(256) void k_panel_diag(const Prob *__restrict__ probs,
+                                                    const int *__restrict__ plist, int p)
+{
+    const Prob &P = probs[plist[blockIdx.x]];
+    const int s = P.s;
+    if (P.info[0] != 0 || p >= s)
+        return;
+    const int jb = min(kPanel, s - p);
+    const int lda = P.lda, t = threadIdx.x;
+    __shared__ double dr[kPanel][kPanel + 1], di[kPanel][kPanel + 1];
+    __shared__ double cmax[kPanel];
+    for (int e = t; e < jb * jb; e += 256) {
+        const int r = e % jb, c = e / jb;
+        dr[r][c] = P.ar[(size_t)(p + c) * lda + p + r];
+        di[r][c] = P.ai[(size_t)(p + c) * lda + p + r];
+    }
+    if (t < kPanel)
+        cmax[t] = 0.0;
+    __syncthreads();
+    for (int j = 0; j < jb; ++j) {
+        // |a_rj| of the rows below the diagonal at step j, then multipliers
+        const double ur = dr[j][j], ui = di[j][j];
+        const double nsq = ur * ur + ui * ui;
+        const double ir = ur / nsq, ii = -ui / nsq;
+        if (t == 0) {
+            double m = 0.0;
+            for (int r = j + 1; r < jb; ++r)
+                m = fmax(m, hypot(dr[r][j], di[r][j]));
+            cmax[j] = m;
+        }
+        __syncthreads();
+        for (int r = j + 1 + t; r < jb; r += 256) {
+            const double xr = dr[r][j], xi = di[r][j];
+            dr[r][j] = xr * ir - xi * ii;
+            di[r][j] = xr * ii + xi * ir;
+        }
+        __syncthreads();
+        const int nr = jb - j - 1;
+        for (int e = t; e < nr * nr; e += 256) {
+            const int r = j + 1 + e % nr, c = j + 1 + e / nr;
+            const double lr = dr[r][j], li = di[r][j];
+            const double vr = dr[j][c], vi = di[j][c];
+            dr[r][c] -= lr * vr - li * vi;
+            di[r][c] -= lr * vi + li * vr;
+        }
+        __syncthreads();
+    }
+    for (int e = t; e < jb * jb; e += 256) {
+        const int r = e % jb, c = e / jb;
+        P.pr[(size_t)c * lda + p + r] = dr[r][c];
+        P.pi[(size_t)c * lda + p + r] = di[r][c];
+    }
+    if (t < jb) {
+        P.diagmag[t] = hypot(dr[t][t], di[t][t]);
+        P.colmax[t] = (unsigned long long)__double_as_longlong(cmax[t]);
+    }
+}
+
+// Rows below the diagonal block: l_r U_D = a_r by forward substitution, one
+// thread per row; the intermediate t_rj is a_rj at elimination step j.
+__global__ __launch_bounds__(256) void k_panel_rows(const Prob *__restrict__ probs,
+                                                    const int2 *__restrict__ work, int p)
+{
+    const int2 wk = work[blockIdx.x];
+    const Prob &P = probs[wk.x];
+    const int s = P.s;
+    if (P.info[0] != 0 || p >= s)
+        return;
+    const int jb = min(kPanel, s - p);
+    const int lda = P.lda, t = threadIdx.x;
+    __shared__ double ur_[kPanel][kPanel + 1], ui_[kPanel][kPanel + 1];
+    __shared__ double cm[kPanel][8];
+    for (int e = t; e < jb * jb; e += 256) {
+        const int r = e % jb, c = e / jb;
+        ur_[r][c] = P.pr[(size_t)c * lda + p + r];
+        ui_[r][c] = P.pi[(size_t)c * lda + p + r];
+    }
+    __syncthreads();
+    const int row = p + jb + wk.y * 256 + t;
+    const bool active = row < s;
+    const int lane = t & 63, wave = t >> 6;
+    double lr[kPanel], li[kPanel];
+#pragma unroll
+    for (int j = 0; j < kPanel; ++j) {
+        lr[j] = 0.0;
+        li[j] = 0.0;
+        if (j < jb) {
+            double vr = 0.0, vi = 0.0;
+            if (active) {
+                vr = P.ar[(size_t)(p + j) * lda + row];
+                vi = P.ai[(size_t)(p + j) * lda + row];
+#pragma unroll
+                for (int m = 0; m < j; ++m) {
+                    vr -= lr[m] * ur_[m][j] - li[m] * ui_[m][j];
+                    vi -= lr[m] * ui_[m][j] + li[m] * ur_[m][j];
+                }
+            }
+            // |a_rj| at step j, reduced over the wave (inactive lanes give 0)
+            double v = active ? hypot(vr, vi) : 0.0;
+            for (int off = 32; off > 0; off >>= 1)
+                v = fmax(v, __shfl_xor(v, off, 64));
+            if (lane == 0)
+                cm[j][wave] = v;
+            if (active) {
+                const double dr = ur_[j][j], di = ui_[j][j];
+                const double nsq = dr * dr + di * di;
+                const double ir = dr / nsq, ii = -di / nsq;
+                lr[j] = vr * ir - vi * ii;
+                li[j] = vr * ii + vi * ir;
+                P.pr[(size_t)j * lda + row] = lr[j];
+                P.pi[(size_t)j * lda + row] = li[j];
+            }
+        }
+    }
+    __syncthreads();
+    if (t < jb) {
+        const double v = fmax(fmax(cm[t][0], cm[t][1]), fmax(cm[t][2], cm[t][3]));
+        atomicMax(&P.colmax[t], (unsigned long long)__double_as_longlong(v));
+    }
+}
+
+// Decide and commit: every workgroup reads the same per-column maxima, so all
+// agree; on success the panel rows of the workspace replace the matrix panel.
+__global__ __launch_bounds__(256) void k_panel_commit(const Prob *__restrict__ probs,
+                                                      const int2 *__restrict__ work, int p)
+{
+    const int2 wk = work[blockIdx.x];
+    const Prob &P = probs[wk.x];
+    const int s = P.s;
+    if (P.info[0] != 0 || p >= s)
+        return;
+    const int jb = min(kPanel, s - p);
+    const int lda = P.lda, t = threadIdx.x;
+    __shared__ int ok;
+    if (t == 0) {
+        int good = 1;
+        for (int j = 0; j < jb; ++j) {
+            const double m = __longlong_as_double((long long)P.colmax[j]);
+            const double d = P.diagmag[j];
+            if (!(m < d) || d < kSingular)
+                good = 0;
+        }
+        ok = good;
+        if (wk.y == 0)
+            P.fast_ok[0] = good;
+    }
+    __syncthreads();
+    if (!ok)
+        return;
+    const int row = p + wk.y * 256 + t;
+    if (row < s)
+        for (int j = 0; j < jb; ++j) {
+            P.ar[(size_t)(p + j) * lda + row] = P.pr[(size_t)j * lda + row];
+            P.ai[(size_t)(p + j) * lda + row] = P.pi[(size_t)j * lda + row];
+        }
+    if (wk.y == 0 && t < jb)
+        P.ipiv[p + t] = p + t;
 }
 
 // Row interchanges of the panel and U12 = L11^{-1} A12 for columns

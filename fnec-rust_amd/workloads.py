@@ -15,8 +15,8 @@ Frozen choices (each generator's docstring states its own):
   frequencies f_i = 3 * 10**(i/255) MHz.
 - config 4b / 5: a request is 1-8 wires whose segment counts split N as evenly
   as possible (the first N mod W wires get one more); a set is either parallel
-  (x-offset dipoles) or junctioned (a chain of bent wires sharing endpoints);
-  centre-fed on wire 1; radius U[0.5,3] mm.
+  (x-offset wires) or junctioned (a chain of bent wires sharing endpoints);
+  segment length lambda/U[10,40]; centre-fed on wire 1; radius U[0.5,3] mm.
 - config 5: 20k requests, N log-U[51,4096], Poisson arrivals, 30% with a
   2 x 2 deg RP grid (theta 0..180 x phi 0..358: 91 x 180 points), 10% over
   PEC ground (GN 1) with every wire above z = 1 m.
@@ -118,21 +118,26 @@ def config4a_frequencies() -> np.ndarray:
 
 def random_request_deck(rng: np.random.Generator, n: int, freq_mhz: float, pec: bool = False,
                         rp_grid: bool = False, junctioned: bool | None = None) -> str:
-    """One request: 1-8 wires with n segments in total, parallel or junctioned."""
+    """One request: 1-8 wires with n segments in total, parallel or junctioned.
+
+    Segment length follows NEC practice, lambda/U[10,40] (so at least ~110
+    radii: the thin-wire kernel stays valid and the reference CLI's
+    source length/radius guard is met); a wire's length is its segment count
+    times that length, so large N means long wires or big arrays."""
     w = int(rng.integers(1, 9))
     w = max(1, min(w, n // 3))
     counts = [n // w + (1 if i < n % w else 0) for i in range(w)]
     lam = C0 / (freq_mhz * 1e6)
+    seg = lam / rng.uniform(10.0, 40.0)
     if junctioned is None:
         junctioned = w > 1 and rng.random() < 0.5
     z0 = 1.0 + rng.uniform(0.0, 0.5) * lam if pec else 0.0
     cards = []
     if junctioned:
         # a chain of bent wires: wire i+1 starts at wire i's end
-        total = rng.uniform(0.8, 2.0) * lam
         p = np.array([0.0, 0.0, z0])
         for i, ns in enumerate(counts):
-            L = total * ns / n
+            L = ns * seg
             th = rng.uniform(0.2, 1.3)
             ph = rng.uniform(0, 2 * math.pi)
             d = np.array([math.sin(th) * math.cos(ph), math.sin(th) * math.sin(ph), math.cos(th)])
@@ -143,12 +148,13 @@ def random_request_deck(rng: np.random.Generator, n: int, freq_mhz: float, pec: 
             cards.append(_gw(i + 1, ns, p, q, a))
             p = q
     else:
+        x = 0.0
         for i, ns in enumerate(counts):
-            L = rng.uniform(0.3, 0.6) * lam * max(ns, 3) / max(counts)
-            x = i * rng.uniform(0.1, 0.3) * lam
+            L = ns * seg
             a = rng.uniform(0.5e-3, 3e-3)
             zc = z0 + L / 2 + 0.5 if pec else 0.0
             cards.append(_gw(i + 1, ns, (x, 0.0, zc - L / 2), (x, 0.0, zc + L / 2), a))
+            x += rng.uniform(0.1, 0.3) * lam
     feed = max(1, (counts[0] + 1) // 2)
     if pec:
         cards.append("GN 1")

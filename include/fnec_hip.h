@@ -81,6 +81,7 @@ typedef struct {
     double backward_error; /* mixed: final normwise backward error of the LS normal equations */
     int refine_iters;      /* mixed: refinement iterations used */
     int n_swaps;           /* row interchanges taken by partial pivoting */
+    int exact_panels;      /* LU panels that needed the interchange path */
     int rows;              /* R = N + constraint rows          */
     int unknowns;          /* S = N + homogeneous columns      */
     double t_fill_us, t_gram_us, t_lu_us, t_solve_us, t_total_us;
