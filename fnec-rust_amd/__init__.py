@@ -32,6 +32,7 @@ EXPORTS = [
     "fnec_hip_solve_batch", "fnec_hip_session_create", "fnec_hip_session_submit",
     "fnec_hip_session_poll", "fnec_hip_session_pending", "fnec_hip_session_destroy",
     "fnec_hip_request_footprint", "fnec_hip_debug_normal_eqs", "fnec_hip_debug_mfma_probe",
+    "fnec_hip_debug_mfma_probe_f32",
 ]
 
 
@@ -118,6 +119,7 @@ def lib():
         L.fnec_hip_debug_normal_eqs.argtypes = [vp, sz, vp, vp, vp, vp, sz, vp, vp, vp, sz, dbl,
                                                 C.POINTER(Ground), vp]
         L.fnec_hip_debug_mfma_probe.argtypes = [vp, vp, vp]
+        L.fnec_hip_debug_mfma_probe_f32.argtypes = [vp, vp, vp]
         _lib = L
     return _lib
 
@@ -235,6 +237,14 @@ def debug_mfma_probe(a16x4: np.ndarray, b4x16: np.ndarray) -> np.ndarray:
     b = np.ascontiguousarray(b4x16, np.float64)
     out = np.zeros((64, 4))
     _check(lib().fnec_hip_debug_mfma_probe(_p(a), _p(b), _p(out)))
+    return out
+
+
+def debug_mfma_probe_f32(a16x4: np.ndarray, b4x16: np.ndarray) -> np.ndarray:
+    a = np.ascontiguousarray(a16x4, np.float32)
+    b = np.ascontiguousarray(b4x16, np.float32)
+    out = np.zeros((64, 4), np.float32)
+    _check(lib().fnec_hip_debug_mfma_probe_f32(_p(a), _p(b), _p(out)))
     return out
 
 
@@ -368,7 +378,8 @@ def profile(enable: bool | None = None, reset: bool = False) -> dict:
     if enable is not None:
         L.fnec_hip_profile_enable(int(enable))
     out = {}
-    for k in ["fill", "gram", "lu_panel", "lu_trsm", "lu_update", "backsub", "rp"]:
+    for k in ["fill", "gram", "lu_panel", "lu_trsm", "lu_update", "backsub", "rp", "zsolve", "matvec",
+              "pcg_ctl"]:
         buf = np.zeros(4)
         _check(L.fnec_hip_profile_read(k.encode(), _p(buf)))
         out[k] = tuple(float(v) for v in buf)

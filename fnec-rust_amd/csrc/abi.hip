@@ -185,6 +185,15 @@ int fnec_hip_debug_mfma_probe(const double *a16x4, const double *b4x16, double *
     return mfma_probe(a16x4, b4x16, lanes_out);
 }
 
+int fnec_hip_debug_mfma_probe_f32(const float *a16x4, const float *b4x16, float *lanes_out)
+{
+    std::lock_guard<std::mutex> lk(global_mutex());
+    int st = ensure_ready();
+    if (st)
+        return st;
+    return mfma_probe32(a16x4, b4x16, lanes_out);
+}
+
 void fnec_hip_profile_enable(int on)
 {
     std::lock_guard<std::mutex> lk(global_mutex());

@@ -54,6 +54,53 @@ struct Prob {
     double *diagmag;       // kPanel: |u_jj|
     int *fast_ok;          // [0]: the fast path factored the current panel
     int pad1;
+    // ---- mixed-precision path (null on the plain FP64 path) ----
+    float *fr, *fi;        // FP32 copy of Z written by the fill, then its L\U factors
+    float *fpr, *fpi;      // FP32 panel workspace, ldz x kPanel
+    const int *rowcol;     // n: homogeneous column of each row, -1 for none
+    double *fh;            // n x ncons complex, column c = Z^{-H} (constraint row c)^H
+    double *lk;            // ncons x ncons complex, Cholesky factor of K = I + F F^H
+    double *ym;            // ncons x w complex, Y = Lk^{-1} H with H = -F B
+    double *ls;            // w x w complex, Cholesky factor of Y^H Y + lambda I
+    double *vr, *vz, *vp, *vq; // PCG vectors, s complex each
+    double *vt;            // M p, r complex
+    double *vv;            // triangular-solve vector, n complex
+    double *part;          // nchunk x n complex partial sums of Z p
+    double *colsq;         // n: squared column norms of Z
+    double *pcg;           // [0] rho, [1] |h|, [2] |r|, [3] |M|_F^2, [4] |x|
+    int *pstate;           // [0] converged, [1] iterations, [2] breakdown
+    int nchunk;
+    int maxit;
+    double tol;
+};
+
+constexpr int kMixMaxCons = 256; // constraint rows the mixed path accepts
+constexpr int kMixMaxW = 128;    // homogeneous columns the mixed path accepts
+constexpr int kMvRows = 256;     // rows per M p work item
+constexpr int kMvChunk = 256;    // columns per M p work item
+constexpr int kMvCols = 16;      // columns per M^H t work item
+
+// One matrix factored by the batched LU (lu.h): column-major split planes.
+template <typename T> struct LuDesc {
+    T *ar, *ai;       // lda x ncols, factored in place
+    T *pr, *pi;       // panel workspace, lda x kPanel (fast no-swap path)
+    int lda;
+    int m;            // order: rows and pivot columns
+    int ncols;        // columns carried along (m + 1 with an augmented right-hand side)
+    int pad0;
+    int *ipiv;        // m
+    int *info;        // [0] singular, [1] interchanges, [2] panels on the exact path
+    unsigned long long *colmax; // kPanel: max |entry| below the diagonal, as double bits
+    double *diagmag;  // kPanel: |u_jj|
+    int *fast_ok;     // [0]: the fast path factored the current panel
+    int pad1;
+};
+
+// Work lists of one panel step (offsets into the batch's int list).
+struct LuStep {
+    int p;
+    size_t panel_off, panel_cnt, trsm_off, trsm_cnt, upd_off, upd_cnt;
+    size_t rows_off, rows_cnt, commit_off, commit_cnt;
 };
 
 // Far-field problem (one RP grid + normalisation integral).

@@ -18,6 +18,7 @@ struct HostProblem {
     std::vector<double> cosv;  // n
     std::vector<double> rhs;   // 2n
     std::vector<int> runs;     // 3 per run: row begin, row end, column
+    std::vector<int> rowcol;   // n: homogeneous column of each row
     std::vector<int> cons;     // 2 per constraint row
     std::vector<double> cons_sign;
 };
@@ -58,6 +59,7 @@ int rp_host_from(RpHost &h, const fnec_seg *segs, const fnec_c64 *cur, size_t n,
 int rp_run(const std::vector<RpHost> &jobs, std::vector<fnec_rp_row *> &outs,
            std::vector<double> &totals);
 int mfma_probe(const double *a, const double *b, double *out);
+int mfma_probe32(const float *a, const float *b, float *out);
 void profile_enable(int on);
 void profile_reset();
 int profile_read(const char *name, double *out4);

@@ -15,6 +15,7 @@
 #include "../../include/fnec_hip.h"
 #include "common.h"
 #include "engine.h"
+#include "lu.h"
 
 namespace fnec {
 
@@ -23,12 +24,6 @@ __global__ void k_fill_z(const Prob *, const int4 *);
 __global__ void k_gram(const Prob *, const int4 *);
 __global__ void k_gram_cols(const Prob *, const int2 *);
 __global__ void k_gram_fix(const Prob *, const int *);
-__global__ void k_lu_panel(const Prob *, const int *, int);
-__global__ void k_panel_diag(const Prob *, const int *, int);
-__global__ void k_panel_rows(const Prob *, const int2 *, int);
-__global__ void k_panel_commit(const Prob *, const int2 *, int);
-__global__ void k_lu_trsm(const Prob *, const int2 *, int);
-__global__ void k_lu_update(const Prob *, const int4 *, int);
 __global__ void k_backsub(const Prob *, const int *);
 __global__ void k_residual_rows(const Prob *, const int2 *);
 __global__ void k_residual_finish(const Prob *, const int *);
@@ -36,6 +31,18 @@ __global__ void k_rp_points(const RpProb *, const int2 *);
 __global__ void k_rp_norm(const RpProb *, const int2 *);
 __global__ void k_rp_finish(const RpProb *, const int *);
 __global__ void k_zero(double *, size_t);
+// mixed.hip
+__global__ void k_zsolve(const Prob *, const int2 *, int);
+__global__ void k_pre_setup(const Prob *, const int *);
+__global__ void k_prec_mid(const Prob *, const int *);
+__global__ void k_mv_z(const Prob *, const int4 *);
+__global__ void k_mv_fin(const Prob *, const int2 *);
+__global__ void k_mv_zh(const Prob *, const int2 *, int);
+__global__ void k_mix_init(const Prob *, const int *);
+__global__ void k_mix_start(const Prob *, const int *);
+__global__ void k_mix_dir(const Prob *, const int *);
+__global__ void k_mix_step(const Prob *, const int *);
+__global__ void k_mfma_probe32(const float *, const float *, float *);
 __global__ void k_mfma_probe(const double *, const double *, double *);
 
 // ---------------------------------------------------------------------------
@@ -80,8 +87,10 @@ struct ProfPending {
     hipEvent_t a, b;
     double flops, bytes;
 };
-enum { PC_FILL, PC_GRAM, PC_LU_PANEL, PC_LU_TRSM, PC_LU_UPDATE, PC_BACKSUB, PC_RP, PC_N };
-static const char *kProfNames[PC_N] = {"fill", "gram", "lu_panel", "lu_trsm", "lu_update", "backsub", "rp"};
+enum { PC_FILL, PC_GRAM, PC_LU_PANEL, PC_LU_TRSM, PC_LU_UPDATE, PC_BACKSUB, PC_RP, PC_ZSOLVE,
+       PC_MV, PC_PCG, PC_N };
+static const char *kProfNames[PC_N] = {"fill", "gram", "lu_panel", "lu_trsm", "lu_update",
+                                       "backsub", "rp", "zsolve", "matvec", "pcg_ctl"};
 static bool g_prof = false;
 static ProfClass g_prof_acc[PC_N];
 static std::vector<ProfPending> g_prof_pending;
@@ -339,6 +348,7 @@ int build_problem(HostProblem &hp, const fnec_seg *segs, size_t n, const fnec_c6
         hp.runs.push_back(row_col[r]);
         r = e;
     }
+    hp.rowcol.assign(row_col.begin(), row_col.begin() + n);
     hp.cons = cons;
     hp.cons_sign = cons_sign;
     return FNEC_OK;
@@ -452,12 +462,7 @@ struct Plan {
     size_t cols_off = 0, cols_cnt = 0;
     size_t all_off = 0, all_cnt = 0;
     size_t res_off = 0, res_cnt = 0;
-    struct Step {
-        int p;
-        size_t panel_off, panel_cnt, trsm_off, trsm_cnt, upd_off, upd_cnt;
-        size_t rows_off, rows_cnt, commit_off, commit_cnt;
-    };
-    std::vector<Step> steps;
+    std::vector<LuStep> steps;
     // algorithmic work per launch (profiling)
     double fill_flops = 0, gram_flops = 0, backsub_flops = 0;
     std::vector<double> step_upd_flops, step_panel_flops, step_trsm_flops;
@@ -466,8 +471,26 @@ struct Plan {
     size_t a_begin = 0, a_end = 0; // byte range of the A planes (zeroed per run)
     std::vector<size_t> off_seg, off_z, off_a, off_cos, off_rhs, off_runs, off_cons, off_sign,
         off_ipiv, off_info, off_x, off_stats, off_scr, off_panel, off_pmeta;
-    size_t off_probs = 0, off_ints = 0;
+    size_t off_probs = 0, off_ints = 0, off_lud = 0;
+    std::vector<LuDesc<double>> lud;
+    // mixed-precision path
+    bool mixed = false;
+    struct MixOff {
+        size_t fr, fpr, rowcol, fh, lk, ym, ls, vr, vz, vp, vq, vt, vv, part, colsq, pcg, pstate;
+        int nchunk;
+    };
+    std::vector<MixOff> mix;
+    std::vector<LuStep> steps32;
+    std::vector<double> s32_upd_flops, s32_panel_flops, s32_trsm_flops;
+    std::vector<LuDesc<float>> lud32;
+    size_t off_lud32 = 0;
+    size_t fsol_off = 0, fsol_cnt = 0, vsol_off = 0, vsol_cnt = 0, mvz_off = 0, mvz_cnt = 0;
+    size_t mvfin_off = 0, mvfin_cnt = 0, mvzh_off = 0, mvzh_cnt = 0;
+    double zsolve_flops = 0, mv_flops = 0;
 };
+
+constexpr int kMixMaxIt = 12;        // CG iterations at most
+constexpr double kMixTol = 1e-14;    // |h - (M^H M + lambda) x| <= tol |h|
 
 static size_t take(size_t &cur, size_t bytes)
 {
@@ -486,6 +509,107 @@ size_t problem_bytes(int n, int s, int ncons, int nruns)
     b += 16 * s + 64 + 32 + 32 * ((n + 255) / 256 + 1) + 12 * 256;
     b += 2 * 8 * lda * kPanel + 16 * kPanel + 16 + 512;
     return b;
+}
+
+// Work lists of the batched LU of matrices with orders ms[b] carrying ncs[b]
+// columns: one LuStep per panel position, plus algorithmic flops per launch.
+static void lu_plan(std::vector<int> &ints, const std::vector<int> &ms, const std::vector<int> &ncs,
+                    std::vector<LuStep> &steps, std::vector<double> &upd_flops,
+                    std::vector<double> &panel_flops, std::vector<double> &trsm_flops)
+{
+    auto align4 = [&]() {
+        while (ints.size() % 4)
+            ints.push_back(0);
+    };
+    auto push2 = [&](int a, int b) {
+        ints.push_back(a);
+        ints.push_back(b);
+    };
+    const size_t np = ms.size();
+    int smax = 0;
+    for (int m : ms)
+        smax = std::max(smax, m);
+    steps.clear();
+    upd_flops.clear();
+    panel_flops.clear();
+    trsm_flops.clear();
+    for (int p = 0; p < smax; p += kPanel) {
+        LuStep st;
+        st.p = p;
+        align4();
+        st.panel_off = ints.size();
+        for (size_t b = 0; b < np; ++b)
+            if (ms[b] > p)
+                ints.push_back((int)b);
+        st.panel_cnt = ints.size() - st.panel_off;
+        align4();
+        st.trsm_off = ints.size();
+        for (size_t b = 0; b < np; ++b) {
+            const int s = ms[b];
+            if (s <= p)
+                continue;
+            const int jb = std::min(kPanel, s - p);
+            const int ncols = ncs[b] - p - jb;
+            for (int y = 0; y < (ncols + 63) / 64; ++y)
+                push2((int)b, y);
+        }
+        st.trsm_cnt = (ints.size() - st.trsm_off) / 2;
+        align4();
+        st.upd_off = ints.size();
+        for (size_t b = 0; b < np; ++b) {
+            const int s = ms[b];
+            if (s <= p)
+                continue;
+            const int jb = std::min(kPanel, s - p);
+            const int nrows = s - p - jb, ncols = ncs[b] - p - jb;
+            if (nrows <= 0 || ncols <= 0)
+                continue;
+            for (int y = 0; y < (nrows + kTile - 1) / kTile; ++y)
+                for (int z = 0; z < (ncols + kTile - 1) / kTile; ++z) {
+                    ints.push_back((int)b);
+                    ints.push_back(y);
+                    ints.push_back(z);
+                    ints.push_back(0);
+                }
+        }
+        st.upd_cnt = (ints.size() - st.upd_off) / 4;
+        align4();
+        st.rows_off = ints.size();
+        for (size_t b = 0; b < np; ++b) {
+            const int s = ms[b];
+            if (s <= p)
+                continue;
+            const int jb = std::min(kPanel, s - p);
+            for (int y = 0; y < (s - p - jb + 255) / 256; ++y)
+                push2((int)b, y);
+        }
+        st.rows_cnt = (ints.size() - st.rows_off) / 2;
+        align4();
+        st.commit_off = ints.size();
+        for (size_t b = 0; b < np; ++b) {
+            const int s = ms[b];
+            if (s <= p)
+                continue;
+            for (int y = 0; y < (s - p + 255) / 256; ++y)
+                push2((int)b, y);
+        }
+        st.commit_cnt = (ints.size() - st.commit_off) / 2;
+        double fu = 0, fp = 0, ft = 0;
+        for (size_t b = 0; b < np; ++b) {
+            const int s = ms[b];
+            if (s <= p)
+                continue;
+            const int jb = std::min(kPanel, s - p);
+            const double nrows = s - p - jb, ncols = ncs[b] - p - jb, m = s - p;
+            fu += 8.0 * nrows * ncols * jb;
+            fp += 8.0 * m * jb * jb / 2.0;
+            ft += 8.0 * ncols * jb * jb / 2.0;
+        }
+        upd_flops.push_back(fu);
+        panel_flops.push_back(fp);
+        trsm_flops.push_back(ft);
+        steps.push_back(st);
+    }
 }
 
 static void plan_build(Plan &pl, const std::vector<const HostProblem *> &hps, bool with_solve)
@@ -508,12 +632,13 @@ static void plan_build(Plan &pl, const std::vector<const HostProblem *> &hps, bo
     pl.off_scr.resize(np);
     pl.off_panel.resize(np);
     pl.off_pmeta.resize(np);
+    pl.mix.clear();
     // A planes first, contiguous, so one zero-fill covers them
     pl.a_begin = 0;
     for (size_t b = 0; b < np; ++b) {
         const HostProblem &h = *hps[b];
         size_t lda = (size_t)roundup(h.s, 64);
-        pl.off_a[b] = with_solve ? take(cur, 2 * 8 * lda * (h.s + 1)) : 0;
+        pl.off_a[b] = with_solve && !pl.mixed ? take(cur, 2 * 8 * lda * (h.s + 1)) : 0;
     }
     pl.a_end = cur;
     for (size_t b = 0; b < np; ++b) {
@@ -532,8 +657,32 @@ static void plan_build(Plan &pl, const std::vector<const HostProblem *> &hps, bo
         pl.off_stats[b] = take(cur, 32);
         pl.off_scr[b] = take(cur, 32 * ((size_t)(h.n + 255) / 256 + 1));
         if (with_solve) {
-            pl.off_panel[b] = take(cur, 2 * 8 * (size_t)roundup(h.s, 64) * kPanel);
+            if (!pl.mixed)
+                pl.off_panel[b] = take(cur, 2 * 8 * (size_t)roundup(h.s, 64) * kPanel);
             pl.off_pmeta[b] = take(cur, 8 * kPanel + 8 * kPanel + 16);
+        }
+        if (with_solve && pl.mixed) {
+            Plan::MixOff m;
+            const size_t n = h.n, nc = std::max(h.ncons, 1), w = h.w, sz = h.s;
+            m.nchunk = (h.n + kMvChunk - 1) / kMvChunk;
+            m.fr = take(cur, 2 * 4 * ldz * n);
+            m.fpr = take(cur, 2 * 4 * ldz * kPanel);
+            m.rowcol = take(cur, 4 * n);
+            m.fh = take(cur, 16 * n * nc);
+            m.lk = take(cur, 16 * nc * nc);
+            m.ym = take(cur, 16 * nc * w);
+            m.ls = take(cur, 16 * w * w);
+            m.vr = take(cur, 16 * sz);
+            m.vz = take(cur, 16 * sz);
+            m.vp = take(cur, 16 * sz);
+            m.vq = take(cur, 16 * sz);
+            m.vt = take(cur, 16 * (n + nc));
+            m.vv = take(cur, 16 * n);
+            m.part = take(cur, 16 * (size_t)m.nchunk * n);
+            m.colsq = take(cur, 8 * n);
+            m.pcg = take(cur, 64);
+            m.pstate = take(cur, 16);
+            pl.mix.push_back(m);
         }
     }
     // work lists
@@ -571,7 +720,60 @@ static void plan_build(Plan &pl, const std::vector<const HostProblem *> &hps, bo
         pl.gram_flops += 4.0 * n * n * n;      // Hermitian-minimal Z^H Z
         pl.backsub_flops += 4.0 * s * s;       // 8 S^2 / 2 for the upper triangle
     }
-    if (with_solve) {
+    if (with_solve && pl.mixed) {
+        align4();
+        pl.all_off = pl.ints.size();
+        for (size_t b = 0; b < np; ++b)
+            pl.ints.push_back((int)b);
+        pl.all_cnt = np;
+        std::vector<int> ms(np), ncs(np);
+        for (size_t b = 0; b < np; ++b)
+            ms[b] = ncs[b] = hps[b]->n;
+        lu_plan(pl.ints, ms, ncs, pl.steps32, pl.s32_upd_flops, pl.s32_panel_flops,
+                pl.s32_trsm_flops);
+        align4();
+        pl.fsol_off = pl.ints.size();
+        for (size_t b = 0; b < np; ++b)
+            for (int c = 0; c < hps[b]->ncons; ++c)
+                push2((int)b, c);
+        pl.fsol_cnt = (pl.ints.size() - pl.fsol_off) / 2;
+        align4();
+        pl.vsol_off = pl.ints.size();
+        for (size_t b = 0; b < np; ++b)
+            push2((int)b, -1);
+        pl.vsol_cnt = np;
+        align4();
+        pl.mvz_off = pl.ints.size();
+        for (size_t b = 0; b < np; ++b)
+            for (int y = 0; y < (hps[b]->n + kMvRows - 1) / kMvRows; ++y)
+                for (int z = 0; z < pl.mix[b].nchunk; ++z)
+                    push4((int)b, y, z, 0);
+        pl.mvz_cnt = (pl.ints.size() - pl.mvz_off) / 4;
+        align4();
+        pl.mvfin_off = pl.ints.size();
+        for (size_t b = 0; b < np; ++b)
+            for (int y = 0; y < (hps[b]->n + kMvRows - 1) / kMvRows; ++y)
+                push2((int)b, y);
+        pl.mvfin_cnt = (pl.ints.size() - pl.mvfin_off) / 2;
+        align4();
+        pl.mvzh_off = pl.ints.size();
+        for (size_t b = 0; b < np; ++b)
+            for (int y = 0; y < (hps[b]->n + kMvCols - 1) / kMvCols; ++y)
+                push2((int)b, y);
+        pl.mvzh_cnt = (pl.ints.size() - pl.mvzh_off) / 2;
+        align4();
+        pl.res_off = pl.ints.size();
+        for (size_t b = 0; b < np; ++b)
+            for (int y = 0; y < (hps[b]->n + 255) / 256; ++y)
+                push2((int)b, y);
+        pl.res_cnt = (pl.ints.size() - pl.res_off) / 2;
+        pl.zsolve_flops = pl.mv_flops = 0;
+        for (size_t b = 0; b < np; ++b) {
+            const double n = hps[b]->n, r = hps[b]->r;
+            pl.zsolve_flops += 8.0 * n * n;   // L and U (or U^H and L^H): n^2 CMAC
+            pl.mv_flops += 2.0 * 8.0 * n * r; // M p and M^H t
+        }
+    } else if (with_solve) {
         align4();
         pl.gram_off = pl.ints.size();
         for (size_t b = 0; b < np; ++b) {
@@ -592,83 +794,13 @@ static void plan_build(Plan &pl, const std::vector<const HostProblem *> &hps, bo
         for (size_t b = 0; b < np; ++b)
             pl.ints.push_back((int)b);
         pl.all_cnt = np;
-        int smax = 0;
-        for (auto *h : hps)
-            smax = std::max(smax, h->s);
-        pl.steps.clear();
-        for (int p = 0; p < smax; p += kPanel) {
-            Plan::Step st;
-            st.p = p;
-            align4();
-            st.panel_off = pl.ints.size();
-            for (size_t b = 0; b < np; ++b)
-                if (hps[b]->s > p)
-                    pl.ints.push_back((int)b);
-            st.panel_cnt = pl.ints.size() - st.panel_off;
-            align4();
-            st.trsm_off = pl.ints.size();
-            for (size_t b = 0; b < np; ++b) {
-                int s = hps[b]->s;
-                if (s <= p)
-                    continue;
-                int jb = std::min(kPanel, s - p);
-                int ncols = s + 1 - p - jb;
-                for (int y = 0; y < (ncols + 63) / 64; ++y)
-                    push2((int)b, y);
-            }
-            st.trsm_cnt = (pl.ints.size() - st.trsm_off) / 2;
-            align4();
-            st.upd_off = pl.ints.size();
-            for (size_t b = 0; b < np; ++b) {
-                int s = hps[b]->s;
-                if (s <= p)
-                    continue;
-                int jb = std::min(kPanel, s - p);
-                int nrows = s - p - jb, ncols = s + 1 - p - jb;
-                if (nrows <= 0)
-                    continue;
-                for (int y = 0; y < (nrows + kTile - 1) / kTile; ++y)
-                    for (int z = 0; z < (ncols + kTile - 1) / kTile; ++z)
-                        push4((int)b, y, z, 0);
-            }
-            st.upd_cnt = (pl.ints.size() - st.upd_off) / 4;
-            align4();
-            st.rows_off = pl.ints.size();
-            for (size_t b = 0; b < np; ++b) {
-                int s = hps[b]->s;
-                if (s <= p)
-                    continue;
-                int jb = std::min(kPanel, s - p);
-                for (int y = 0; y < (s - p - jb + 255) / 256; ++y)
-                    push2((int)b, y);
-            }
-            st.rows_cnt = (pl.ints.size() - st.rows_off) / 2;
-            align4();
-            st.commit_off = pl.ints.size();
-            for (size_t b = 0; b < np; ++b) {
-                int s = hps[b]->s;
-                if (s <= p)
-                    continue;
-                for (int y = 0; y < (s - p + 255) / 256; ++y)
-                    push2((int)b, y);
-            }
-            st.commit_cnt = (pl.ints.size() - st.commit_off) / 2;
-            double fu = 0, fp = 0, ft = 0;
-            for (size_t b = 0; b < np; ++b) {
-                const int s = hps[b]->s;
-                if (s <= p)
-                    continue;
-                const int jb = std::min(kPanel, s - p);
-                const double nrows = s - p - jb, ncols = s + 1 - p - jb, m = s - p;
-                fu += 8.0 * nrows * ncols * jb;
-                fp += 8.0 * m * jb * jb / 2.0;
-                ft += 8.0 * ncols * jb * jb / 2.0;
-            }
-            pl.step_upd_flops.push_back(fu);
-            pl.step_panel_flops.push_back(fp);
-            pl.step_trsm_flops.push_back(ft);
-            pl.steps.push_back(st);
+        std::vector<int> ms(np), ncs(np);
+        for (size_t b = 0; b < np; ++b) {
+            ms[b] = hps[b]->s;
+            ncs[b] = hps[b]->s + 1;
         }
+        lu_plan(pl.ints, ms, ncs, pl.steps, pl.step_upd_flops, pl.step_panel_flops,
+                pl.step_trsm_flops);
         align4();
         pl.res_off = pl.ints.size();
         for (size_t b = 0; b < np; ++b)
@@ -677,6 +809,10 @@ static void plan_build(Plan &pl, const std::vector<const HostProblem *> &hps, bo
         pl.res_cnt = (pl.ints.size() - pl.res_off) / 2;
     }
     pl.off_probs = take(cur, sizeof(Prob) * np);
+    if (with_solve && !pl.mixed)
+        pl.off_lud = take(cur, sizeof(LuDesc<double>) * np);
+    if (with_solve && pl.mixed)
+        pl.off_lud32 = take(cur, sizeof(LuDesc<float>) * np);
     pl.off_ints = take(cur, 4 * pl.ints.size() + 16);
     pl.bytes = cur + 256;
 }
@@ -686,6 +822,8 @@ static int plan_upload(Plan &pl, const std::vector<const HostProblem *> &hps, ch
 {
     hipStream_t st = g_ctx.stream;
     const size_t np = hps.size();
+    pl.lud.clear();
+    pl.lud32.clear();
     for (size_t b = 0; b < np; ++b) {
         const HostProblem &h = *hps[b];
         Prob &P = pl.probs[b];
@@ -705,7 +843,7 @@ static int plan_upload(Plan &pl, const std::vector<const HostProblem *> &hps, ch
         P.seg = (const double *)(base + pl.off_seg[b]);
         P.zr = (double *)(base + pl.off_z[b]);
         P.zi = P.zr + (size_t)P.ldz * h.n;
-        if (with_solve) {
+        if (with_solve && !pl.mixed) {
             P.ar = (double *)(base + pl.off_a[b]);
             P.ai = P.ar + (size_t)P.lda * (h.s + 1);
         }
@@ -720,11 +858,70 @@ static int plan_upload(Plan &pl, const std::vector<const HostProblem *> &hps, ch
         P.stats = (double *)(base + pl.off_stats[b]);
         P.scratch = (double *)(base + pl.off_scr[b]);
         if (with_solve) {
-            P.pr = (double *)(base + pl.off_panel[b]);
-            P.pi = P.pr + (size_t)P.lda * kPanel;
             P.colmax = (unsigned long long *)(base + pl.off_pmeta[b]);
             P.diagmag = (double *)(base + pl.off_pmeta[b] + 8 * kPanel);
             P.fast_ok = (int *)(base + pl.off_pmeta[b] + 16 * kPanel);
+        }
+        if (with_solve && pl.mixed) {
+            const Plan::MixOff &m = pl.mix[b];
+            const size_t plane = (size_t)P.ldz * h.n;
+            P.fr = (float *)(base + m.fr);
+            P.fi = P.fr + plane;
+            P.fpr = (float *)(base + m.fpr);
+            P.fpi = P.fpr + (size_t)P.ldz * kPanel;
+            P.rowcol = (const int *)(base + m.rowcol);
+            P.fh = (double *)(base + m.fh);
+            P.lk = (double *)(base + m.lk);
+            P.ym = (double *)(base + m.ym);
+            P.ls = (double *)(base + m.ls);
+            P.vr = (double *)(base + m.vr);
+            P.vz = (double *)(base + m.vz);
+            P.vp = (double *)(base + m.vp);
+            P.vq = (double *)(base + m.vq);
+            P.vt = (double *)(base + m.vt);
+            P.vv = (double *)(base + m.vv);
+            P.part = (double *)(base + m.part);
+            P.colsq = (double *)(base + m.colsq);
+            P.pcg = (double *)(base + m.pcg);
+            P.pstate = (int *)(base + m.pstate);
+            P.nchunk = m.nchunk;
+            P.maxit = kMixMaxIt;
+            P.tol = kMixTol;
+            LuDesc<float> D;
+            std::memset(&D, 0, sizeof(D));
+            D.ar = P.fr;
+            D.ai = P.fi;
+            D.pr = P.fpr;
+            D.pi = P.fpi;
+            D.lda = P.ldz;
+            D.m = h.n;
+            D.ncols = h.n;
+            D.ipiv = P.ipiv;
+            D.info = P.info;
+            D.colmax = P.colmax;
+            D.diagmag = P.diagmag;
+            D.fast_ok = P.fast_ok;
+            pl.lud32.push_back(D);
+            HIPCHK(hipMemcpyAsync(base + m.rowcol, h.rowcol.data(), 4 * h.rowcol.size(),
+                                  hipMemcpyHostToDevice, st));
+        } else if (with_solve) {
+            P.pr = (double *)(base + pl.off_panel[b]);
+            P.pi = P.pr + (size_t)P.lda * kPanel;
+            LuDesc<double> D;
+            std::memset(&D, 0, sizeof(D));
+            D.ar = P.ar;
+            D.ai = P.ai;
+            D.pr = P.pr;
+            D.pi = P.pi;
+            D.lda = P.lda;
+            D.m = P.s;
+            D.ncols = P.s + 1;
+            D.ipiv = P.ipiv;
+            D.info = P.info;
+            D.colmax = P.colmax;
+            D.diagmag = P.diagmag;
+            D.fast_ok = P.fast_ok;
+            pl.lud.push_back(D);
         }
         HIPCHK(hipMemcpyAsync(base + pl.off_seg[b], h.seg.data(), 8 * h.seg.size(), hipMemcpyHostToDevice, st));
         HIPCHK(hipMemcpyAsync(base + pl.off_cos[b], h.cosv.data(), 8 * h.cosv.size(), hipMemcpyHostToDevice, st));
@@ -743,6 +940,12 @@ static int plan_upload(Plan &pl, const std::vector<const HostProblem *> &hps, ch
             HIPCHK(hipMemcpyAsync(base + pl.off_info[b], zeros.data(), 16, hipMemcpyHostToDevice, st));
     }
     HIPCHK(hipMemcpyAsync(base + pl.off_probs, pl.probs.data(), sizeof(Prob) * np, hipMemcpyHostToDevice, st));
+    if (with_solve && !pl.mixed)
+        HIPCHK(hipMemcpyAsync(base + pl.off_lud, pl.lud.data(), sizeof(LuDesc<double>) * np,
+                              hipMemcpyHostToDevice, st));
+    if (with_solve && pl.mixed)
+        HIPCHK(hipMemcpyAsync(base + pl.off_lud32, pl.lud32.data(), sizeof(LuDesc<float>) * np,
+                              hipMemcpyHostToDevice, st));
     if (!pl.ints.empty())
         HIPCHK(hipMemcpyAsync(base + pl.off_ints, pl.ints.data(), 4 * pl.ints.size(), hipMemcpyHostToDevice, st));
     return FNEC_OK;
@@ -751,6 +954,81 @@ static int plan_upload(Plan &pl, const std::vector<const HostProblem *> &hps, ch
 static inline const int *wl_ptr(char *base, const Plan &pl, size_t off)
 {
     return (const int *)(base + pl.off_ints) + off;
+}
+
+// Mixed path: FP32 LU of Z, preconditioner set-up, CG iterations, residual.
+static int mixed_launch(Plan &pl, char *base)
+{
+    hipStream_t st = g_ctx.stream;
+    const Prob *dp = (const Prob *)(base + pl.off_probs);
+    const int *ints = wl_ptr(base, pl, 0);
+    const int *all = ints + pl.all_off;
+    const unsigned np = (unsigned)pl.all_cnt;
+    HIPCHK(hipEventRecord(g_ctx.ev[2], st));
+    const LuDesc<float> *lud = (const LuDesc<float> *)(base + pl.off_lud32);
+    for (size_t si = 0; si < pl.steps32.size(); ++si) {
+        const LuStep &s = pl.steps32[si];
+        if (s.panel_cnt) {
+            hipEvent_t e = prof_begin();
+            lu_launch_panel(lud, s, ints, st);
+            prof_end(PC_LU_PANEL, e, pl.s32_panel_flops[si], 0);
+        }
+        if (s.trsm_cnt) {
+            hipEvent_t e = prof_begin();
+            lu_launch_trsm(lud, s, ints, st);
+            prof_end(PC_LU_TRSM, e, pl.s32_trsm_flops[si], 0);
+        }
+        if (s.upd_cnt) {
+            hipEvent_t e = prof_begin();
+            lu_launch_update(lud, s, ints, st);
+            prof_end(PC_LU_UPDATE, e, pl.s32_upd_flops[si], 0);
+        }
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(g_ctx.ev[3], st));
+    const int2 *vsol = (const int2 *)(ints + pl.vsol_off);
+    hipLaunchKernelGGL(k_mix_init, dim3(np), dim3(1024), 0, st, dp, all);
+    if (pl.fsol_cnt) {
+        hipEvent_t e = prof_begin();
+        hipLaunchKernelGGL(k_zsolve, dim3((unsigned)pl.fsol_cnt), dim3(1024), 0, st, dp,
+                           (const int2 *)(ints + pl.fsol_off), 1);
+        prof_end(PC_ZSOLVE, e, 0, 0);
+    }
+    hipLaunchKernelGGL(k_pre_setup, dim3(np), dim3(1024), 0, st, dp, all);
+    hipLaunchKernelGGL(k_mv_zh, dim3((unsigned)pl.mvzh_cnt), dim3(256), 0, st, dp,
+                       (const int2 *)(ints + pl.mvzh_off), 1);
+    hipLaunchKernelGGL(k_mix_start, dim3(np), dim3(1024), 0, st, dp, all);
+    for (int it = 0; it < kMixMaxIt; ++it) {
+        hipEvent_t e = prof_begin();
+        hipLaunchKernelGGL(k_zsolve, dim3(np), dim3(1024), 0, st, dp, vsol, 1);
+        prof_end(PC_ZSOLVE, e, pl.zsolve_flops, 0);
+        e = prof_begin();
+        hipLaunchKernelGGL(k_prec_mid, dim3(np), dim3(1024), 0, st, dp, all);
+        prof_end(PC_PCG, e, 0, 0);
+        e = prof_begin();
+        hipLaunchKernelGGL(k_zsolve, dim3(np), dim3(1024), 0, st, dp, vsol, 0);
+        prof_end(PC_ZSOLVE, e, pl.zsolve_flops, 0);
+        e = prof_begin();
+        hipLaunchKernelGGL(k_mix_dir, dim3(np), dim3(1024), 0, st, dp, all);
+        prof_end(PC_PCG, e, 0, 0);
+        e = prof_begin();
+        hipLaunchKernelGGL(k_mv_z, dim3((unsigned)pl.mvz_cnt), dim3(kMvRows), 0, st, dp,
+                           (const int4 *)(ints + pl.mvz_off));
+        hipLaunchKernelGGL(k_mv_fin, dim3((unsigned)pl.mvfin_cnt), dim3(kMvRows), 0, st, dp,
+                           (const int2 *)(ints + pl.mvfin_off));
+        hipLaunchKernelGGL(k_mv_zh, dim3((unsigned)pl.mvzh_cnt), dim3(256), 0, st, dp,
+                           (const int2 *)(ints + pl.mvzh_off), 0);
+        prof_end(PC_MV, e, pl.mv_flops, 0);
+        e = prof_begin();
+        hipLaunchKernelGGL(k_mix_step, dim3(np), dim3(1024), 0, st, dp, all);
+        prof_end(PC_PCG, e, 0, 0);
+    }
+    hipLaunchKernelGGL(k_residual_rows, dim3((unsigned)pl.res_cnt), dim3(256), 0, st, dp,
+                       (const int2 *)(ints + pl.res_off));
+    hipLaunchKernelGGL(k_residual_finish, dim3(np), dim3(64), 0, st, dp, all);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(g_ctx.ev[4], st));
+    return FNEC_OK;
 }
 
 // Launch the whole solve on the context stream; records phase events.
@@ -769,6 +1047,8 @@ static int plan_launch(Plan &pl, char *base, bool with_solve)
     HIPCHK(hipEventRecord(g_ctx.ev[1], st));
     if (!with_solve)
         return FNEC_OK;
+    if (pl.mixed)
+        return mixed_launch(pl, base);
     {
         size_t count = (pl.a_end - pl.a_begin) / 8;
         if (count)
@@ -786,31 +1066,23 @@ static int plan_launch(Plan &pl, char *base, bool with_solve)
                        wl_ptr(base, pl, pl.all_off));
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(g_ctx.ev[2], st));
+    const LuDesc<double> *lud = (const LuDesc<double> *)(base + pl.off_lud);
+    const int *ints = wl_ptr(base, pl, 0);
     for (size_t si = 0; si < pl.steps.size(); ++si) {
         const auto &s = pl.steps[si];
         if (s.panel_cnt) {
             hipEvent_t e = prof_begin();
-            hipLaunchKernelGGL(k_panel_diag, dim3((unsigned)s.panel_cnt), dim3(256), 0, st, dp,
-                               wl_ptr(base, pl, s.panel_off), s.p);
-            if (s.rows_cnt)
-                hipLaunchKernelGGL(k_panel_rows, dim3((unsigned)s.rows_cnt), dim3(256), 0, st, dp,
-                                   (const int2 *)wl_ptr(base, pl, s.rows_off), s.p);
-            hipLaunchKernelGGL(k_panel_commit, dim3((unsigned)s.commit_cnt), dim3(256), 0, st, dp,
-                               (const int2 *)wl_ptr(base, pl, s.commit_off), s.p);
-            hipLaunchKernelGGL(k_lu_panel, dim3((unsigned)s.panel_cnt), dim3(512), 0, st, dp,
-                               wl_ptr(base, pl, s.panel_off), s.p);
+            lu_launch_panel(lud, s, ints, st);
             prof_end(PC_LU_PANEL, e, pl.step_panel_flops[si], 0);
         }
         if (s.trsm_cnt) {
             hipEvent_t e = prof_begin();
-            hipLaunchKernelGGL(k_lu_trsm, dim3((unsigned)s.trsm_cnt), dim3(64), 0, st, dp,
-                               (const int2 *)wl_ptr(base, pl, s.trsm_off), s.p);
+            lu_launch_trsm(lud, s, ints, st);
             prof_end(PC_LU_TRSM, e, pl.step_trsm_flops[si], 0);
         }
         if (s.upd_cnt) {
             hipEvent_t e = prof_begin();
-            hipLaunchKernelGGL(k_lu_update, dim3((unsigned)s.upd_cnt), dim3(256), 0, st, dp,
-                               (const int4 *)wl_ptr(base, pl, s.upd_off), s.p);
+            lu_launch_update(lud, s, ints, st);
             prof_end(PC_LU_UPDATE, e, pl.step_upd_flops[si], 0);
         }
     }
@@ -844,11 +1116,17 @@ int solve_problems(const std::vector<const HostProblem *> &hps, std::vector<fnec
                    std::vector<fnec_solve_stats> &stats, std::vector<int> &status,
                    std::vector<double *> *device_x, int precision)
 {
-    if (precision != FNEC_PREC_FP64) {
-        set_error("mixed precision is not available in this build");
-        return FNEC_INVALID_ARG;
-    }
+    const bool mixed = precision == FNEC_PREC_MIXED;
+    if (mixed)
+        for (const HostProblem *h : hps)
+            if (h->w < 1 || h->w > kMixMaxW || h->ncons > kMixMaxCons) {
+                set_error("mixed precision accepts 1.." + std::to_string(kMixMaxW) +
+                          " homogeneous columns and at most " + std::to_string(kMixMaxCons) +
+                          " constraint rows");
+                return FNEC_INVALID_ARG;
+            }
     Plan pl;
+    pl.mixed = mixed;
     plan_build(pl, hps, true);
     char *base = nullptr;
     int st = arena_reserve(pl.bytes, &base);
@@ -874,6 +1152,20 @@ int solve_problems(const std::vector<const HostProblem *> &hps, std::vector<fnec
         HIPCHK(hipMemcpy(sv, base + pl.off_stats[b], 32, hipMemcpyDeviceToHost));
         status[b] = info[0] ? FNEC_SINGULAR : FNEC_OK;
         fnec_solve_stats &s = stats[b];
+        if (mixed && !info[0]) {
+            double pcg[8];
+            int ps[4];
+            HIPCHK(hipMemcpy(pcg, base + pl.mix[b].pcg, 64, hipMemcpyDeviceToHost));
+            HIPCHK(hipMemcpy(ps, base + pl.mix[b].pstate, 16, hipMemcpyDeviceToHost));
+            s.refine_iters = ps[1];
+            const double den = pcg[3] * pcg[4] + pcg[1];
+            s.backward_error = den > 0.0 ? pcg[2] / den : 0.0;
+            if (ps[2] != 0 || !(pcg[2] <= 1e-8 * pcg[1])) {
+                set_error(ps[2] ? "mixed-precision iteration broke down"
+                                : "mixed-precision iteration did not converge");
+                status[b] = FNEC_INTERNAL;
+            }
+        }
         s.abs_res = sv[0];
         s.rel_res = sv[1];
         s.diag_spread = sv[2];
@@ -886,7 +1178,7 @@ int solve_problems(const std::vector<const HostProblem *> &hps, std::vector<fnec
         s.t_lu_us = t_lu;
         s.t_solve_us = t_sol;
         s.t_total_us = t_fill + t_gram + t_lu + t_sol;
-        if (x_out.size() > b && x_out[b] && !info[0])
+        if (x_out.size() > b && x_out[b] && status[b] == FNEC_OK)
             HIPCHK(hipMemcpy(x_out[b], base + pl.off_x[b], 16 * (size_t)hps[b]->s, hipMemcpyDeviceToHost));
         if (device_x)
             device_x->push_back((double *)(base + pl.off_x[b]));
@@ -1120,6 +1412,23 @@ int mfma_probe(const double *a, const double *b, double *out)
                        (const double *)(base + 1024), (double *)(base + 2048));
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(out, base + 2048, 256 * 8, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    return FNEC_OK;
+}
+
+int mfma_probe32(const float *a, const float *b, float *out)
+{
+    char *base = nullptr;
+    int st = arena_reserve(4096 * 4, &base);
+    if (st)
+        return st;
+    hipStream_t s = g_ctx.stream;
+    HIPCHK(hipMemcpyAsync(base, a, 64 * 4, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(base + 1024, b, 64 * 4, hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_mfma_probe32, dim3(1), dim3(64), 0, s, (const float *)base,
+                       (const float *)(base + 1024), (float *)(base + 2048));
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(out, base + 2048, 256 * 4, hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
     return FNEC_OK;
 }

@@ -5,10 +5,7 @@
 //   k_gram          G = Z^H Z on v_mfma_f64_16x16x4f64, lower-triangle tiles.
 //   k_gram_cols     homogeneous columns, M^H y (linear.rs:735-780).
 //   k_gram_fix      W x W block, constraint rows, lambda (linear.rs:744-785).
-//   k_lu_panel      panel factorisation with the reference pivot rule
-//                   (last row of maximal hypot, linear.rs:1345-1356).
-//   k_lu_trsm       row interchanges + unit-lower triangular solve of U12.
-//   k_lu_update     trailing update A22 -= L21 U12 on MFMA.
+//   (the blocked LU kernels are templates in lu.h, instantiated by engine.hip)
 //   k_backsub       U x = b' (blocked, one workgroup per problem).
 //   k_residual_*    ||Z I - C cos - rhs|| and diag spread (solve_session.rs:118-207).
 //   k_rp_*          far field + normalisation integral (farfield.rs:265-647).
@@ -157,6 +154,10 @@ __global__ __launch_bounds__(256) void k_fill_z(const Prob *__restrict__ probs,
         const size_t o = (size_t)j * P.ldz + i;
         P.zr[o] = re;
         P.zi[o] = im;
+        if (P.fr) { // mixed path: FP32 copy for the factorisation
+            P.fr[o] = (float)re;
+            P.fi[o] = (float)im;
+        }
     }
 }
 
@@ -386,418 +387,6 @@ __global__ __launch_bounds__(256) void k_gram_fix(const Prob *__restrict__ probs
     __syncthreads();
     for (int i = t; i < s; i += 256)
         P.ar[(size_t)i * P.lda + i] += kLambda;
-}
-
-// ---------------------------------------------------------------------------
-// LU with partial pivoting on [G | M^H y]
-// ---------------------------------------------------------------------------
-
-constexpr int kPanelThreads = 512;
-
-// One workgroup per problem factors columns [p, p + jb) of rows [p, s).
-__global__ __launch_bounds__(kPanelThreads) void k_lu_panel(const Prob *__restrict__ probs,
-                                                            const int *__restrict__ plist, int p)
-{
-    const Prob &P = probs[plist[blockIdx.x]];
-    const int s = P.s;
-    if (P.info[0] != 0 || p >= s || P.fast_ok[0])
-        return;
-    const int jb = min(kPanel, s - p);
-    const int t = threadIdx.x;
-    const int lda = P.lda;
-    double *ar = P.ar, *ai = P.ai;
-    __shared__ double rv[kPanelThreads];
-    __shared__ int ri[kPanelThreads];
-    __shared__ double prow_r[kPanel], prow_i[kPanel];
-    __shared__ int s_piv;
-    __shared__ double s_best;
-    if (t == 0)
-        P.info[2] += 1; // panels factored on the exact interchange path
-    for (int j = 0; j < jb; ++j) {
-        const int col = p + j;
-        const double *cr = ar + (size_t)col * lda, *ci = ai + (size_t)col * lda;
-        double best = -1.0;
-        int bidx = col;
-        for (int r = col + t; r < s; r += kPanelThreads) {
-            const double v = hypot(cr[r], ci[r]);
-            if (!(v < best)) { // later rows win ties (Iterator::max_by)
-                best = v;
-                bidx = r;
-            }
-        }
-        rv[t] = best;
-        ri[t] = bidx;
-        __syncthreads();
-        for (int off = kPanelThreads / 2; off > 0; off >>= 1) {
-            if (t < off) {
-                const double v2 = rv[t + off];
-                const int i2 = ri[t + off];
-                if (v2 > rv[t] || (v2 == rv[t] && i2 > ri[t])) {
-                    rv[t] = v2;
-                    ri[t] = i2;
-                }
-            }
-            __syncthreads();
-        }
-        if (t == 0) {
-            s_piv = ri[0];
-            s_best = rv[0];
-        }
-        __syncthreads();
-        const int piv = s_piv;
-        if (s_best < kSingular) { // NaN passes, as in linear.rs:1354
-            if (t == 0)
-                P.info[0] = 1;
-            return;
-        }
-        if (piv != col) {
-            for (int c = p + t; c < p + jb; c += kPanelThreads) {
-                const size_t o1 = (size_t)c * lda + col, o2 = (size_t)c * lda + piv;
-                const double xr = ar[o1], xi = ai[o1];
-                ar[o1] = ar[o2];
-                ai[o1] = ai[o2];
-                ar[o2] = xr;
-                ai[o2] = xi;
-            }
-            if (t == 0)
-                P.info[1] += 1;
-        }
-        if (t == 0)
-            P.ipiv[col] = piv;
-        __syncthreads();
-        for (int c = col + 1 + t; c < p + jb; c += kPanelThreads) {
-            prow_r[c - col - 1] = ar[(size_t)c * lda + col];
-            prow_i[c - col - 1] = ai[(size_t)c * lda + col];
-        }
-        __syncthreads();
-        const double dr = cr[col], di = ci[col];
-        const double nsq = dr * dr + di * di;
-        const double ir = dr / nsq, ii = -di / nsq; // 1/d
-        const int ncols = p + jb - col - 1;
-        for (int r = col + 1 + t; r < s; r += kPanelThreads) {
-            const double xr = cr[r], xi = ci[r];
-            const double lr = xr * ir - xi * ii, li = xr * ii + xi * ir;
-            ar[(size_t)col * lda + r] = lr;
-            ai[(size_t)col * lda + r] = li;
-            for (int c = 0; c < ncols; ++c) {
-                const size_t o = (size_t)(col + 1 + c) * lda + r;
-                const double ur = prow_r[c], ui = prow_i[c];
-                ar[o] -= lr * ur - li * ui;
-                ai[o] -= lr * ui + li * ur;
-            }
-        }
-        __syncthreads();
-    }
-}
-
-// ---- fast panel path: factor without interchanges, then prove none was due ----
-//
-// Partial pivoting on these Hermitian positive-definite Gram matrices rarely
-// interchanges rows. The fast path factors the panel without interchanges
-// and records, for every panel column j, the largest |a_rj| below the
-// diagonal at elimination step j. The reference keeps row j as pivot exactly
-// when every such entry is strictly smaller than |u_jj| (a tie selects the
-// later row, linear.rs:1345-1352), so when that holds for all columns the
-// factorisation is the partial-pivoting one with no interchange and is
-// committed; otherwise the panel is left untouched for k_lu_panel.
-
-// One workgroup per problem: LU of the jb x jb diagonal block in LDS.
-__global__ __launch_bounds__(256) void k_panel_diag(const Prob *__restrict__ probs,
-                                                    const int *__restrict__ plist, int p)
-{
-    const Prob &P = probs[plist[blockIdx.x]];
-    const int s = P.s;
-    if (P.info[0] != 0 || p >= s)
-        return;
-    const int jb = min(kPanel, s - p);
-    const int lda = P.lda, t = threadIdx.x;
-    __shared__ double dr[kPanel][kPanel + 1], di[kPanel][kPanel + 1];
-    __shared__ double cmax[kPanel];
-    for (int e = t; e < jb * jb; e += 256) {
-        const int r = e % jb, c = e / jb;
-        dr[r][c] = P.ar[(size_t)(p + c) * lda + p + r];
-        di[r][c] = P.ai[(size_t)(p + c) * lda + p + r];
-    }
-    if (t < kPanel)
-        cmax[t] = 0.0;
-    __syncthreads();
-    for (int j = 0; j < jb; ++j) {
-        // |a_rj| of the rows below the diagonal at step j, then multipliers
-        const double ur = dr[j][j], ui = di[j][j];
-        const double nsq = ur * ur + ui * ui;
-        const double ir = ur / nsq, ii = -ui / nsq;
-        if (t == 0) {
-            double m = 0.0;
-            for (int r = j + 1; r < jb; ++r)
-                m = fmax(m, hypot(dr[r][j], di[r][j]));
-            cmax[j] = m;
-        }
-        __syncthreads();
-        for (int r = j + 1 + t; r < jb; r += 256) {
-            const double xr = dr[r][j], xi = di[r][j];
-            dr[r][j] = xr * ir - xi * ii;
-            di[r][j] = xr * ii + xi * ir;
-        }
-        __syncthreads();
-        const int nr = jb - j - 1;
-        for (int e = t; e < nr * nr; e += 256) {
-            const int r = j + 1 + e % nr, c = j + 1 + e / nr;
-            const double lr = dr[r][j], li = di[r][j];
-            const double vr = dr[j][c], vi = di[j][c];
-            dr[r][c] -= lr * vr - li * vi;
-            di[r][c] -= lr * vi + li * vr;
-        }
-        __syncthreads();
-    }
-    for (int e = t; e < jb * jb; e += 256) {
-        const int r = e % jb, c = e / jb;
-        P.pr[(size_t)c * lda + p + r] = dr[r][c];
-        P.pi[(size_t)c * lda + p + r] = di[r][c];
-    }
-    if (t < jb) {
-        P.diagmag[t] = hypot(dr[t][t], di[t][t]);
-        P.colmax[t] = (unsigned long long)__double_as_longlong(cmax[t]);
-    }
-}
-
-// Rows below the diagonal block: l_r U_D = a_r by forward substitution, one
-// thread per row; the intermediate t_rj is a_rj at elimination step j.
-__global__ __launch_bounds__(256) void k_panel_rows(const Prob *__restrict__ probs,
-                                                    const int2 *__restrict__ work, int p)
-{
-    const int2 wk = work[blockIdx.x];
-    const Prob &P = probs[wk.x];
-    const int s = P.s;
-    if (P.info[0] != 0 || p >= s)
-        return;
-    const int jb = min(kPanel, s - p);
-    const int lda = P.lda, t = threadIdx.x;
-    __shared__ double ur_[kPanel][kPanel + 1], ui_[kPanel][kPanel + 1];
-    __shared__ double cm[kPanel][8];
-    for (int e = t; e < jb * jb; e += 256) {
-        const int r = e % jb, c = e / jb;
-        ur_[r][c] = P.pr[(size_t)c * lda + p + r];
-        ui_[r][c] = P.pi[(size_t)c * lda + p + r];
-    }
-    __syncthreads();
-    const int row = p + jb + wk.y * 256 + t;
-    const bool active = row < s;
-    const int lane = t & 63, wave = t >> 6;
-    double lr[kPanel], li[kPanel];
-#pragma unroll
-    for (int j = 0; j < kPanel; ++j) {
-        lr[j] = 0.0;
-        li[j] = 0.0;
-        if (j < jb) {
-            double vr = 0.0, vi = 0.0;
-            if (active) {
-                vr = P.ar[(size_t)(p + j) * lda + row];
-                vi = P.ai[(size_t)(p + j) * lda + row];
-#pragma unroll
-                for (int m = 0; m < j; ++m) {
-                    vr -= lr[m] * ur_[m][j] - li[m] * ui_[m][j];
-                    vi -= lr[m] * ui_[m][j] + li[m] * ur_[m][j];
-                }
-            }
-            // |a_rj| at step j, reduced over the wave (inactive lanes give 0)
-            double v = active ? hypot(vr, vi) : 0.0;
-            for (int off = 32; off > 0; off >>= 1)
-                v = fmax(v, __shfl_xor(v, off, 64));
-            if (lane == 0)
-                cm[j][wave] = v;
-            if (active) {
-                const double dr = ur_[j][j], di = ui_[j][j];
-                const double nsq = dr * dr + di * di;
-                const double ir = dr / nsq, ii = -di / nsq;
-                lr[j] = vr * ir - vi * ii;
-                li[j] = vr * ii + vi * ir;
-                P.pr[(size_t)j * lda + row] = lr[j];
-                P.pi[(size_t)j * lda + row] = li[j];
-            }
-        }
-    }
-    __syncthreads();
-    if (t < jb) {
-        const double v = fmax(fmax(cm[t][0], cm[t][1]), fmax(cm[t][2], cm[t][3]));
-        atomicMax(&P.colmax[t], (unsigned long long)__double_as_longlong(v));
-    }
-}
-
-// Decide and commit: every workgroup reads the same per-column maxima, so all
-// agree; on success the panel rows of the workspace replace the matrix panel.
-__global__ __launch_bounds__(256) void k_panel_commit(const Prob *__restrict__ probs,
-                                                      const int2 *__restrict__ work, int p)
-{
-    const int2 wk = work[blockIdx.x];
-    const Prob &P = probs[wk.x];
-    const int s = P.s;
-    if (P.info[0] != 0 || p >= s)
-        return;
-    const int jb = min(kPanel, s - p);
-    const int lda = P.lda, t = threadIdx.x;
-    __shared__ int ok;
-    if (t == 0) {
-        int good = 1;
-        for (int j = 0; j < jb; ++j) {
-            const double m = __longlong_as_double((long long)P.colmax[j]);
-            const double d = P.diagmag[j];
-            if (!(m < d) || d < kSingular)
-                good = 0;
-        }
-        ok = good;
-        if (wk.y == 0)
-            P.fast_ok[0] = good;
-    }
-    __syncthreads();
-    if (!ok)
-        return;
-    const int row = p + wk.y * 256 + t;
-    if (row < s)
-        for (int j = 0; j < jb; ++j) {
-            P.ar[(size_t)(p + j) * lda + row] = P.pr[(size_t)j * lda + row];
-            P.ai[(size_t)(p + j) * lda + row] = P.pi[(size_t)j * lda + row];
-        }
-    if (wk.y == 0 && t < jb)
-        P.ipiv[p + t] = p + t;
-}
-
-// Row interchanges of the panel and U12 = L11^{-1} A12 for columns
-// [p + jb, s] (the augmented right-hand side included). One thread per column.
-__global__ __launch_bounds__(64) void k_lu_trsm(const Prob *__restrict__ probs,
-                                                const int2 *__restrict__ work, int p)
-{
-    const int2 wk = work[blockIdx.x];
-    const Prob &P = probs[wk.x];
-    const int s = P.s;
-    if (P.info[0] != 0 || p >= s)
-        return;
-    const int jb = min(kPanel, s - p);
-    const int lda = P.lda;
-    __shared__ double lr[kPanel][kPanel + 1], li[kPanel][kPanel + 1];
-    const int t = threadIdx.x;
-    for (int e = t; e < jb * jb; e += 64) {
-        const int rr = e % jb, cc = e / jb;
-        lr[rr][cc] = P.ar[(size_t)(p + cc) * lda + p + rr];
-        li[rr][cc] = P.ai[(size_t)(p + cc) * lda + p + rr];
-    }
-    __syncthreads();
-    const int c = p + jb + wk.y * 64 + t;
-    if (c > s)
-        return;
-    double *cr = P.ar + (size_t)c * lda, *ci = P.ai + (size_t)c * lda;
-    for (int j = 0; j < jb; ++j) {
-        const int piv = P.ipiv[p + j];
-        if (piv != p + j) {
-            const double xr = cr[p + j], xi = ci[p + j];
-            cr[p + j] = cr[piv];
-            ci[p + j] = ci[piv];
-            cr[piv] = xr;
-            ci[piv] = xi;
-        }
-    }
-    double ur[kPanel], ui[kPanel];
-#pragma unroll
-    for (int j = 0; j < kPanel; ++j) {
-        if (j < jb) {
-            double vr = cr[p + j], vi = ci[p + j];
-            for (int m = 0; m < j; ++m) {
-                vr -= lr[j][m] * ur[m] - li[j][m] * ui[m];
-                vi -= lr[j][m] * ui[m] + li[j][m] * ur[m];
-            }
-            ur[j] = vr;
-            ui[j] = vi;
-            cr[p + j] = vr;
-            ci[p + j] = vi;
-        }
-    }
-}
-
-// Trailing update A22 -= L21 U12 over rows [p+jb, s) x columns [p+jb, s].
-// Work item: (problem, row tile, column tile). Same wave layout as k_gram;
-// K = jb <= 32 is staged whole in LDS.
-__global__ __launch_bounds__(256) void k_lu_update(const Prob *__restrict__ probs,
-                                                   const int4 *__restrict__ work, int p)
-{
-    const int4 wk = work[blockIdx.x];
-    const Prob &P = probs[wk.x];
-    const int s = P.s;
-    if (P.info[0] != 0 || p >= s)
-        return;
-    const int jb = min(kPanel, s - p);
-    const int lda = P.lda;
-    const int r0 = p + jb + wk.y * kTile, c0 = p + jb + wk.z * kTile;
-    __shared__ double sa_r[kPanel][kTile + kLdsPad], sa_i[kPanel][kTile + kLdsPad];
-    __shared__ double sb_r[kPanel][kTile + kLdsPad], sb_i[kPanel][kTile + kLdsPad];
-    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-    const int wr = wave >> 1, wc = wave & 1;
-    // A tile: rows r0.., k columns p..p+jb (column-major: contiguous rows)
-    for (int e = t; e < kPanel * kTile; e += 256) {
-        const int kk = e / kTile, rr = e % kTile;
-        const int gr = r0 + rr;
-        double vr = 0.0, vi = 0.0;
-        if (kk < jb && gr < s) {
-            vr = P.ar[(size_t)(p + kk) * lda + gr];
-            vi = P.ai[(size_t)(p + kk) * lda + gr];
-        }
-        sa_r[kk][rr] = vr;
-        sa_i[kk][rr] = vi;
-    }
-    // B tile: k rows p..p+jb, columns c0.. (contiguous k within a column)
-    for (int e = t; e < kPanel * kTile; e += 256) {
-        const int cc = e / kPanel, kk = e % kPanel;
-        const int gc = c0 + cc;
-        double vr = 0.0, vi = 0.0;
-        if (kk < jb && gc <= s) {
-            vr = P.ar[(size_t)gc * lda + p + kk];
-            vi = P.ai[(size_t)gc * lda + p + kk];
-        }
-        sb_r[kk][cc] = vr;
-        sb_i[kk][cc] = vi;
-    }
-    __syncthreads();
-    d4 accr[2][2], acci[2][2];
-#pragma unroll
-    for (int a = 0; a < 2; ++a)
-#pragma unroll
-        for (int b = 0; b < 2; ++b) {
-            accr[a][b] = d4{0.0, 0.0, 0.0, 0.0};
-            acci[a][b] = d4{0.0, 0.0, 0.0, 0.0};
-        }
-    const int lrow = lane & 15, lk = lane >> 4;
-#pragma unroll
-    for (int kk = 0; kk < kPanel; kk += 4) {
-        double xr[2], xi[2], yr[2], yi[2];
-#pragma unroll
-        for (int a = 0; a < 2; ++a) {
-            xr[a] = sa_r[kk + lk][wr * 32 + a * 16 + lrow];
-            xi[a] = sa_i[kk + lk][wr * 32 + a * 16 + lrow];
-            yr[a] = sb_r[kk + lk][wc * 32 + a * 16 + lrow];
-            yi[a] = sb_i[kk + lk][wc * 32 + a * 16 + lrow];
-        }
-#pragma unroll
-        for (int a = 0; a < 2; ++a)
-#pragma unroll
-            for (int b = 0; b < 2; ++b) {
-                accr[a][b] = mfma(xr[a], yr[b], accr[a][b]);
-                accr[a][b] = mfma(-xi[a], yi[b], accr[a][b]);
-                acci[a][b] = mfma(xr[a], yi[b], acci[a][b]);
-                acci[a][b] = mfma(xi[a], yr[b], acci[a][b]);
-            }
-    }
-#pragma unroll
-    for (int a = 0; a < 2; ++a)
-#pragma unroll
-        for (int b = 0; b < 2; ++b)
-#pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                const int gi = r0 + wr * 32 + a * 16 + acc_row(lane, v);
-                const int gj = c0 + wc * 32 + b * 16 + lrow;
-                if (gi >= s || gj > s)
-                    continue;
-                const size_t o = (size_t)gj * lda + gi;
-                P.ar[o] -= accr[a][b][v];
-                P.ai[o] -= acci[a][b][v];
-            }
 }
 
 // Back substitution U x = b' with b' the transformed augmented column.

@@ -1,0 +1,507 @@
+// Blocked complex LU with partial pivoting, templated on the scalar type.
+//
+// double: the plain path factors the augmented normal matrix [G | M^H y]
+//         (linear.rs:1321-1383), ncols = m + 1.
+// float:  the mixed path factors the Z matrix itself (ncols = m); the factors
+//         precondition an FP64 iteration.
+//
+// Matrices are column-major split planes (re, im) with leading dimension lda.
+// Row interchanges are applied to the panel and to the columns right of it
+// only; the multipliers of earlier panels keep their original row order, so
+// triangular solves apply each panel's interchanges just before its
+// elimination (the LINPACK xGESL order).
+//
+// Kernels per panel step p (jb = min(kPanel, m - p) columns):
+//   k_panel_diag / k_panel_rows / k_panel_commit  no-interchange fast path:
+//       factor without pivoting, then prove that partial pivoting (last row
+//       of maximal modulus, linear.rs:1345-1352) would have kept every
+//       diagonal row; commit only then.
+//   k_lu_panel   exact path with interchanges when the fast path declined.
+//   k_lu_trsm    interchanges + U12 = L11^{-1} A12 (one thread per column).
+//   k_lu_update  A22 -= L21 U12 on MFMA (16x16x4, four real products per
+//                complex product, 64x64 output tiles, K = jb staged in LDS).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <math.h>
+
+#include "common.h"
+
+namespace fnec {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+template <typename T> struct Mfma;
+
+// v_mfma_f64_16x16x4f64: lane l supplies A[l%16][l/16] and B[l/16][l%16];
+// accumulator item v of lane l is D[(l/16) + 4v][l%16].
+template <> struct Mfma<double> {
+    typedef d4 acc;
+    static __device__ __forceinline__ acc zero() { return d4{0.0, 0.0, 0.0, 0.0}; }
+    static __device__ __forceinline__ acc fma(double a, double b, acc c)
+    {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int lane, int v) { return (lane >> 4) + 4 * v; }
+};
+
+// v_mfma_f32_16x16x4f32: same operand layout; accumulator item v of lane l
+// is D[4(l/16) + v][l%16].
+template <> struct Mfma<float> {
+    typedef f4 acc;
+    static __device__ __forceinline__ acc zero() { return f4{0.f, 0.f, 0.f, 0.f}; }
+    static __device__ __forceinline__ acc fma(float a, float b, acc c)
+    {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int lane, int v) { return 4 * (lane >> 4) + v; }
+};
+
+__device__ __forceinline__ double cmod(double re, double im) { return hypot(re, im); }
+__device__ __forceinline__ float cmod(float re, float im) { return hypotf(re, im); }
+
+constexpr int kPanelThreads = 512;
+constexpr int kLuPad = 4;
+
+// One workgroup per problem factors columns [p, p + jb) of rows [p, m).
+template <typename T>
+__global__ __launch_bounds__(kPanelThreads) void k_lu_panel(const LuDesc<T> *__restrict__ descs,
+                                                            const int *__restrict__ plist, int p)
+{
+    const LuDesc<T> &D = descs[plist[blockIdx.x]];
+    const int s = D.m;
+    if (D.info[0] != 0 || p >= s || D.fast_ok[0])
+        return;
+    const int jb = min(kPanel, s - p);
+    const int t = threadIdx.x;
+    const int lda = D.lda;
+    T *ar = D.ar, *ai = D.ai;
+    __shared__ T rv[kPanelThreads];
+    __shared__ int ri[kPanelThreads];
+    __shared__ T prow_r[kPanel], prow_i[kPanel];
+    __shared__ int s_piv;
+    __shared__ T s_best;
+    if (t == 0)
+        D.info[2] += 1; // panels factored on the exact interchange path
+    for (int j = 0; j < jb; ++j) {
+        const int col = p + j;
+        const T *cr = ar + (size_t)col * lda, *ci = ai + (size_t)col * lda;
+        T best = -1;
+        int bidx = col;
+        for (int r = col + t; r < s; r += kPanelThreads) {
+            const T v = cmod(cr[r], ci[r]);
+            if (!(v < best)) { // later rows win ties (Iterator::max_by)
+                best = v;
+                bidx = r;
+            }
+        }
+        rv[t] = best;
+        ri[t] = bidx;
+        __syncthreads();
+        for (int off = kPanelThreads / 2; off > 0; off >>= 1) {
+            if (t < off) {
+                const T v2 = rv[t + off];
+                const int i2 = ri[t + off];
+                if (v2 > rv[t] || (v2 == rv[t] && i2 > ri[t])) {
+                    rv[t] = v2;
+                    ri[t] = i2;
+                }
+            }
+            __syncthreads();
+        }
+        if (t == 0) {
+            s_piv = ri[0];
+            s_best = rv[0];
+        }
+        __syncthreads();
+        const int piv = s_piv;
+        if (s_best < (T)kSingular) { // NaN passes, as in linear.rs:1354
+            if (t == 0)
+                D.info[0] = 1;
+            return;
+        }
+        if (piv != col) {
+            for (int c = p + t; c < p + jb; c += kPanelThreads) {
+                const size_t o1 = (size_t)c * lda + col, o2 = (size_t)c * lda + piv;
+                const T xr = ar[o1], xi = ai[o1];
+                ar[o1] = ar[o2];
+                ai[o1] = ai[o2];
+                ar[o2] = xr;
+                ai[o2] = xi;
+            }
+            if (t == 0)
+                D.info[1] += 1;
+        }
+        if (t == 0)
+            D.ipiv[col] = piv;
+        __syncthreads();
+        for (int c = col + 1 + t; c < p + jb; c += kPanelThreads) {
+            prow_r[c - col - 1] = ar[(size_t)c * lda + col];
+            prow_i[c - col - 1] = ai[(size_t)c * lda + col];
+        }
+        __syncthreads();
+        const T dr = cr[col], di = ci[col];
+        const T nsq = dr * dr + di * di;
+        const T ir = dr / nsq, ii = -di / nsq; // 1/d
+        const int ncols = p + jb - col - 1;
+        for (int r = col + 1 + t; r < s; r += kPanelThreads) {
+            const T xr = cr[r], xi = ci[r];
+            const T lr = xr * ir - xi * ii, li = xr * ii + xi * ir;
+            ar[(size_t)col * lda + r] = lr;
+            ai[(size_t)col * lda + r] = li;
+            for (int c = 0; c < ncols; ++c) {
+                const size_t o = (size_t)(col + 1 + c) * lda + r;
+                const T ur = prow_r[c], ui = prow_i[c];
+                ar[o] -= lr * ur - li * ui;
+                ai[o] -= lr * ui + li * ur;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// ---- fast panel path: factor without interchanges, then prove none was due ----
+//
+// The fast path records, for every panel column j, the largest |a_rj| below
+// the diagonal at elimination step j. Partial pivoting keeps row j exactly
+// when every such entry is strictly smaller than |u_jj| (a tie selects the
+// later row), so when that holds for all columns the no-interchange
+// factorisation is the partial-pivoting one and is committed; otherwise the
+// panel is left untouched for k_lu_panel.
+
+// One workgroup per problem: LU of the jb x jb diagonal block in LDS.
+template <typename T>
+__global__ __launch_bounds__(256) void k_panel_diag(const LuDesc<T> *__restrict__ descs,
+                                                    const int *__restrict__ plist, int p)
+{
+    const LuDesc<T> &D = descs[plist[blockIdx.x]];
+    const int s = D.m;
+    if (D.info[0] != 0 || p >= s)
+        return;
+    const int jb = min(kPanel, s - p);
+    const int lda = D.lda, t = threadIdx.x;
+    __shared__ T dr[kPanel][kPanel + 1], di[kPanel][kPanel + 1];
+    __shared__ T cmax[kPanel];
+    for (int e = t; e < jb * jb; e += 256) {
+        const int r = e % jb, c = e / jb;
+        dr[r][c] = D.ar[(size_t)(p + c) * lda + p + r];
+        di[r][c] = D.ai[(size_t)(p + c) * lda + p + r];
+    }
+    if (t < kPanel)
+        cmax[t] = 0;
+    __syncthreads();
+    for (int j = 0; j < jb; ++j) {
+        const T ur = dr[j][j], ui = di[j][j];
+        const T nsq = ur * ur + ui * ui;
+        const T ir = ur / nsq, ii = -ui / nsq;
+        if (t == 0) {
+            T m = 0;
+            for (int r = j + 1; r < jb; ++r)
+                m = fmax(m, cmod(dr[r][j], di[r][j]));
+            cmax[j] = m;
+        }
+        __syncthreads();
+        for (int r = j + 1 + t; r < jb; r += 256) {
+            const T xr = dr[r][j], xi = di[r][j];
+            dr[r][j] = xr * ir - xi * ii;
+            di[r][j] = xr * ii + xi * ir;
+        }
+        __syncthreads();
+        const int nr = jb - j - 1;
+        for (int e = t; e < nr * nr; e += 256) {
+            const int r = j + 1 + e % nr, c = j + 1 + e / nr;
+            const T lr = dr[r][j], li = di[r][j];
+            const T vr = dr[j][c], vi = di[j][c];
+            dr[r][c] -= lr * vr - li * vi;
+            di[r][c] -= lr * vi + li * vr;
+        }
+        __syncthreads();
+    }
+    for (int e = t; e < jb * jb; e += 256) {
+        const int r = e % jb, c = e / jb;
+        D.pr[(size_t)c * lda + p + r] = dr[r][c];
+        D.pi[(size_t)c * lda + p + r] = di[r][c];
+    }
+    if (t < jb) {
+        D.diagmag[t] = (double)cmod(dr[t][t], di[t][t]);
+        D.colmax[t] = (unsigned long long)__double_as_longlong((double)cmax[t]);
+    }
+}
+
+// Rows below the diagonal block: l_r U_D = a_r by forward substitution, one
+// thread per row; the intermediate t_rj is a_rj at elimination step j.
+template <typename T>
+__global__ __launch_bounds__(256) void k_panel_rows(const LuDesc<T> *__restrict__ descs,
+                                                    const int2 *__restrict__ work, int p)
+{
+    const int2 wk = work[blockIdx.x];
+    const LuDesc<T> &D = descs[wk.x];
+    const int s = D.m;
+    if (D.info[0] != 0 || p >= s)
+        return;
+    const int jb = min(kPanel, s - p);
+    const int lda = D.lda, t = threadIdx.x;
+    __shared__ T ur_[kPanel][kPanel + 1], ui_[kPanel][kPanel + 1];
+    __shared__ T cm[kPanel][8];
+    for (int e = t; e < jb * jb; e += 256) {
+        const int r = e % jb, c = e / jb;
+        ur_[r][c] = D.pr[(size_t)c * lda + p + r];
+        ui_[r][c] = D.pi[(size_t)c * lda + p + r];
+    }
+    __syncthreads();
+    const int row = p + jb + wk.y * 256 + t;
+    const bool active = row < s;
+    const int lane = t & 63, wave = t >> 6;
+    T lr[kPanel], li[kPanel];
+#pragma unroll
+    for (int j = 0; j < kPanel; ++j) {
+        lr[j] = 0;
+        li[j] = 0;
+        if (j < jb) {
+            T vr = 0, vi = 0;
+            if (active) {
+                vr = D.ar[(size_t)(p + j) * lda + row];
+                vi = D.ai[(size_t)(p + j) * lda + row];
+#pragma unroll
+                for (int m = 0; m < j; ++m) {
+                    vr -= lr[m] * ur_[m][j] - li[m] * ui_[m][j];
+                    vi -= lr[m] * ui_[m][j] + li[m] * ur_[m][j];
+                }
+            }
+            // |a_rj| at step j, reduced over the wave (inactive lanes give 0)
+            T v = active ? cmod(vr, vi) : (T)0;
+            for (int off = 32; off > 0; off >>= 1)
+                v = fmax(v, __shfl_xor(v, off, 64));
+            if (lane == 0)
+                cm[j][wave] = v;
+            if (active) {
+                const T dr = ur_[j][j], di = ui_[j][j];
+                const T nsq = dr * dr + di * di;
+                const T ir = dr / nsq, ii = -di / nsq;
+                lr[j] = vr * ir - vi * ii;
+                li[j] = vr * ii + vi * ir;
+                D.pr[(size_t)j * lda + row] = lr[j];
+                D.pi[(size_t)j * lda + row] = li[j];
+            }
+        }
+    }
+    __syncthreads();
+    if (t < jb) {
+        const T v = fmax(fmax(cm[t][0], cm[t][1]), fmax(cm[t][2], cm[t][3]));
+        atomicMax(&D.colmax[t], (unsigned long long)__double_as_longlong((double)v));
+    }
+}
+
+// Decide and commit: every workgroup reads the same per-column maxima, so all
+// agree; on success the panel rows of the workspace replace the matrix panel.
+template <typename T>
+__global__ __launch_bounds__(256) void k_panel_commit(const LuDesc<T> *__restrict__ descs,
+                                                      const int2 *__restrict__ work, int p)
+{
+    const int2 wk = work[blockIdx.x];
+    const LuDesc<T> &D = descs[wk.x];
+    const int s = D.m;
+    if (D.info[0] != 0 || p >= s)
+        return;
+    const int jb = min(kPanel, s - p);
+    const int lda = D.lda, t = threadIdx.x;
+    __shared__ int ok;
+    if (t == 0) {
+        int good = 1;
+        for (int j = 0; j < jb; ++j) {
+            const double m = __longlong_as_double((long long)D.colmax[j]);
+            const double d = D.diagmag[j];
+            if (!(m < d) || d < kSingular)
+                good = 0;
+        }
+        ok = good;
+        if (wk.y == 0)
+            D.fast_ok[0] = good;
+    }
+    __syncthreads();
+    if (!ok)
+        return;
+    const int row = p + wk.y * 256 + t;
+    if (row < s)
+        for (int j = 0; j < jb; ++j) {
+            D.ar[(size_t)(p + j) * lda + row] = D.pr[(size_t)j * lda + row];
+            D.ai[(size_t)(p + j) * lda + row] = D.pi[(size_t)j * lda + row];
+        }
+    if (wk.y == 0 && t < jb)
+        D.ipiv[p + t] = p + t;
+}
+
+// Row interchanges of the panel and U12 = L11^{-1} A12 for columns
+// [p + jb, ncols). One thread per column.
+template <typename T>
+__global__ __launch_bounds__(64) void k_lu_trsm(const LuDesc<T> *__restrict__ descs,
+                                                const int2 *__restrict__ work, int p)
+{
+    const int2 wk = work[blockIdx.x];
+    const LuDesc<T> &D = descs[wk.x];
+    const int s = D.m;
+    if (D.info[0] != 0 || p >= s)
+        return;
+    const int jb = min(kPanel, s - p);
+    const int lda = D.lda;
+    __shared__ T lr[kPanel][kPanel + 1], li[kPanel][kPanel + 1];
+    const int t = threadIdx.x;
+    for (int e = t; e < jb * jb; e += 64) {
+        const int rr = e % jb, cc = e / jb;
+        lr[rr][cc] = D.ar[(size_t)(p + cc) * lda + p + rr];
+        li[rr][cc] = D.ai[(size_t)(p + cc) * lda + p + rr];
+    }
+    __syncthreads();
+    const int c = p + jb + wk.y * 64 + t;
+    if (c >= D.ncols)
+        return;
+    T *cr = D.ar + (size_t)c * lda, *ci = D.ai + (size_t)c * lda;
+    for (int j = 0; j < jb; ++j) {
+        const int piv = D.ipiv[p + j];
+        if (piv != p + j) {
+            const T xr = cr[p + j], xi = ci[p + j];
+            cr[p + j] = cr[piv];
+            ci[p + j] = ci[piv];
+            cr[piv] = xr;
+            ci[piv] = xi;
+        }
+    }
+    T ur[kPanel], ui[kPanel];
+#pragma unroll
+    for (int j = 0; j < kPanel; ++j) {
+        if (j < jb) {
+            T vr = cr[p + j], vi = ci[p + j];
+            for (int m = 0; m < j; ++m) {
+                vr -= lr[j][m] * ur[m] - li[j][m] * ui[m];
+                vi -= lr[j][m] * ui[m] + li[j][m] * ur[m];
+            }
+            ur[j] = vr;
+            ui[j] = vi;
+            cr[p + j] = vr;
+            ci[p + j] = vi;
+        }
+    }
+}
+
+// Trailing update A22 -= L21 U12 over rows [p+jb, m) x columns [p+jb, ncols).
+// Work item: (problem, row tile, column tile). Four waves in a 2x2
+// arrangement each own a 32x32 block = 2x2 MFMA 16x16 tiles with separate
+// real and imaginary accumulators; K = jb <= 32 is staged whole in LDS.
+template <typename T>
+__global__ __launch_bounds__(256) void k_lu_update(const LuDesc<T> *__restrict__ descs,
+                                                   const int4 *__restrict__ work, int p)
+{
+    typedef Mfma<T> MF;
+    const int4 wk = work[blockIdx.x];
+    const LuDesc<T> &D = descs[wk.x];
+    const int s = D.m, nc = D.ncols;
+    if (D.info[0] != 0 || p >= s)
+        return;
+    const int jb = min(kPanel, s - p);
+    const int lda = D.lda;
+    const int r0 = p + jb + wk.y * kTile, c0 = p + jb + wk.z * kTile;
+    __shared__ T sa_r[kPanel][kTile + kLuPad], sa_i[kPanel][kTile + kLuPad];
+    __shared__ T sb_r[kPanel][kTile + kLuPad], sb_i[kPanel][kTile + kLuPad];
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const int wr = wave >> 1, wc = wave & 1;
+    // A tile: rows r0.., k columns p..p+jb (column-major: contiguous rows)
+    for (int e = t; e < kPanel * kTile; e += 256) {
+        const int kk = e / kTile, rr = e % kTile;
+        const int gr = r0 + rr;
+        T vr = 0, vi = 0;
+        if (kk < jb && gr < s) {
+            vr = D.ar[(size_t)(p + kk) * lda + gr];
+            vi = D.ai[(size_t)(p + kk) * lda + gr];
+        }
+        sa_r[kk][rr] = vr;
+        sa_i[kk][rr] = vi;
+    }
+    // B tile: k rows p..p+jb, columns c0.. (contiguous k within a column)
+    for (int e = t; e < kPanel * kTile; e += 256) {
+        const int cc = e / kPanel, kk = e % kPanel;
+        const int gc = c0 + cc;
+        T vr = 0, vi = 0;
+        if (kk < jb && gc < nc) {
+            vr = D.ar[(size_t)gc * lda + p + kk];
+            vi = D.ai[(size_t)gc * lda + p + kk];
+        }
+        sb_r[kk][cc] = vr;
+        sb_i[kk][cc] = vi;
+    }
+    __syncthreads();
+    typename MF::acc accr[2][2], acci[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+            accr[a][b] = MF::zero();
+            acci[a][b] = MF::zero();
+        }
+    const int lrow = lane & 15, lk = lane >> 4;
+#pragma unroll
+    for (int kk = 0; kk < kPanel; kk += 4) {
+        T xr[2], xi[2], yr[2], yi[2];
+#pragma unroll
+        for (int a = 0; a < 2; ++a) {
+            xr[a] = sa_r[kk + lk][wr * 32 + a * 16 + lrow];
+            xi[a] = sa_i[kk + lk][wr * 32 + a * 16 + lrow];
+            yr[a] = sb_r[kk + lk][wc * 32 + a * 16 + lrow];
+            yi[a] = sb_i[kk + lk][wc * 32 + a * 16 + lrow];
+        }
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int b = 0; b < 2; ++b) {
+                accr[a][b] = MF::fma(xr[a], yr[b], accr[a][b]);
+                accr[a][b] = MF::fma(-xi[a], yi[b], accr[a][b]);
+                acci[a][b] = MF::fma(xr[a], yi[b], acci[a][b]);
+                acci[a][b] = MF::fma(xi[a], yr[b], acci[a][b]);
+            }
+    }
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int gi = r0 + wr * 32 + a * 16 + MF::row(lane, v);
+                const int gj = c0 + wc * 32 + b * 16 + lrow;
+                if (gi >= s || gj >= nc)
+                    continue;
+                const size_t o = (size_t)gj * lda + gi;
+                D.ar[o] -= accr[a][b][v];
+                D.ai[o] -= acci[a][b][v];
+            }
+}
+
+// Host side: one panel step of the batched factorisation on stream st.
+template <typename T>
+void lu_launch_panel(const LuDesc<T> *d, const LuStep &s, const int *ints, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_panel_diag<T>, dim3((unsigned)s.panel_cnt), dim3(256), 0, st, d,
+                       ints + s.panel_off, s.p);
+    if (s.rows_cnt)
+        hipLaunchKernelGGL(k_panel_rows<T>, dim3((unsigned)s.rows_cnt), dim3(256), 0, st, d,
+                           (const int2 *)(ints + s.rows_off), s.p);
+    hipLaunchKernelGGL(k_panel_commit<T>, dim3((unsigned)s.commit_cnt), dim3(256), 0, st, d,
+                       (const int2 *)(ints + s.commit_off), s.p);
+    hipLaunchKernelGGL(k_lu_panel<T>, dim3((unsigned)s.panel_cnt), dim3(kPanelThreads), 0, st, d,
+                       ints + s.panel_off, s.p);
+}
+
+template <typename T>
+void lu_launch_trsm(const LuDesc<T> *d, const LuStep &s, const int *ints, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_lu_trsm<T>, dim3((unsigned)s.trsm_cnt), dim3(64), 0, st, d,
+                       (const int2 *)(ints + s.trsm_off), s.p);
+}
+
+template <typename T>
+void lu_launch_update(const LuDesc<T> *d, const LuStep &s, const int *ints, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_lu_update<T>, dim3((unsigned)s.upd_cnt), dim3(256), 0, st, d,
+                       (const int4 *)(ints + s.upd_off), s.p);
+}
+
+} // namespace fnec

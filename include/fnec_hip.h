@@ -207,10 +207,13 @@ int fnec_hip_debug_normal_eqs(const fnec_seg *segs, size_t n, const fnec_c64 *rh
                               const size_t *j_b, const double *j_sign, size_t n_junc,
                               double freq_hz, const fnec_ground *ground, fnec_c64 *aug_out);
 int fnec_hip_debug_mfma_probe(const double *a16x4, const double *b4x16, double *lanes_out);
+/* The same probe for the FP32 MFMA (v_mfma_f32_16x16x4f32) used by the mixed path. */
+int fnec_hip_debug_mfma_probe_f32(const float *a16x4, const float *b4x16, float *lanes_out);
 
 /* Per-kernel timing with HIP events on the solve stream (off by default).
  * Kernel classes: "fill", "gram", "lu_panel", "lu_trsm", "lu_update",
- * "backsub", "rp". out4 = {launches, total microseconds, algorithmic flops,
+ * "backsub", "rp", and on the mixed path "zsolve" (Z / Z^H solves with the
+ * FP32 factors), "matvec" (M p and M^H t in FP64), "pcg_ctl". out4 = {launches, total microseconds, algorithmic flops,
  * algorithmic bytes} accumulated since the last reset. */
 void fnec_hip_profile_enable(int on);
 void fnec_hip_profile_reset(void);

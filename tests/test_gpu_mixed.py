@@ -1,0 +1,145 @@
+"""GPU tests of the mixed-precision solve (FNEC_PREC_MIXED) through the C ABI.
+
+The mixed path factors Z in FP32 on MFMA and iterates the regularised normal
+equations of linear.rs:672-797 in FP64 (preconditioned CG). Gates:
+- feedpoint impedance within 1e-6 (relative) of the FP64 reference, the
+  accuracy contract of the mixed path;
+- currents within 1e-7 of the exact (extended-precision) solution, or of the
+  plain FP64 GPU solve where no exact solution is formed (large N);
+- normwise backward error of the normal equations <= 1e-12;
+- bitwise determinism run to run and alone vs inside a batch.
+"""
+import importlib
+import os
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN
+from test_gpu_parity import exact_solution, rel
+
+pytestmark = pytest.mark.gpu
+
+F = importlib.import_module("fnec-rust_amd")
+W = importlib.import_module("fnec-rust_amd.workloads")
+
+
+@pytest.fixture(scope="module", autouse=True)
+def device():
+    F.init()
+
+
+def deck_text(name):
+    return open(os.path.join(GOLDEN, "corpus", name + ".nec")).read()
+
+
+def test_mfma_f32_lane_layout():
+    a = (np.arange(64, dtype=np.float32).reshape(16, 4) + 1.0).astype(np.float32)
+    b = (np.arange(64, dtype=np.float32).reshape(4, 16) * 0.5 + 2.0).astype(np.float32)
+    lanes = F.debug_mfma_probe_f32(a, b)
+    d = np.zeros((16, 16), np.float32)
+    for lane in range(64):
+        for v in range(4):
+            d[4 * (lane // 16) + v, lane % 16] = lanes[lane, v]
+    assert np.array_equal(d, a @ b)
+
+
+MIXED_CASES = [
+    ("dipole-freesp-rp-large-grid", "cli"),
+    ("dipole-ground-51seg", "cli"),
+    ("dipole-gn0-fresnel-51seg", "cli"),
+    ("yagi-5elm-51seg", "cli"),
+    ("dipole-loaded", "worker"),
+    ("dipole-loaded", "cli"),
+    ("multi-source", "cli"),
+]
+
+
+@pytest.mark.parametrize("name,sem", MIXED_CASES)
+def test_mixed_matches_oracle(oracle, name, sem):
+    txt = deck_text(name)
+    r = oracle.run_deck(txt, sem)
+    n = r["n"]
+    inp = oracle.deck_inputs(txt, r["freq_hz"], sem)
+    kind, er, sg = inp["ground"]
+    z = oracle.fill_z(inp["segs"], r["freq_hz"], oracle.Ground(kind if kind != 3 else 0, er, sg))
+    xe = exact_solution(oracle, inp, z)
+    p = F.Deck(txt).problem(sem, r["freq_hz"])
+    x, st = F.solve_problem(p, precision=F.PREC_MIXED, with_stats=True)
+    e_exact = rel(x[:n], xe[:n])
+    print(f"{name}/{sem}: mixed vs exact {e_exact:.2e}, iterations {st['refine_iters']}, "
+          f"backward error {st['backward_error']:.2e}")
+    assert e_exact < 1e-7
+    assert st["backward_error"] <= 1e-12
+    for k, idx in enumerate(r["feed_idx"]):
+        zin = r["feed_v"][k] / x[idx]
+        assert abs(zin - r["feed_z"][k]) / abs(r["feed_z"][k]) <= 1e-6
+
+
+def test_mixed_gain_matches_oracle(oracle):
+    txt = deck_text("dipole-ground-rp-51seg")
+    r = oracle.run_deck(txt, "cli")
+    p = F.Deck(txt).problem("cli", r["freq_hz"])
+    x = F.solve_problem(p, precision=F.PREC_MIXED)
+    rows, total = F.rp(p["segs"], x[:r["n"]], r["freq_hz"], r["rp_theta"], r["rp_phi"], p["ground_struct"])
+    live = r["rp"][:, 0] > -999.0
+    gl = 10.0 ** (rows[live, 0] / 10.0)
+    gl_ref = 10.0 ** (r["rp"][live, 0] / 10.0)
+    assert np.max(np.abs(gl - gl_ref)) / np.max(gl_ref) <= 1e-6
+
+
+def random_problems(seed, sizes, sem="worker"):
+    rng = np.random.default_rng(seed)
+    out = []
+    for i, n in enumerate(sizes):
+        txt = W.random_request_deck(rng, n, float(rng.uniform(3.0, 30.0)), pec=bool(i % 3 == 2),
+                                    junctioned=bool(i % 2))
+        d = F.Deck(txt)
+        out.append(d.problem(sem, d.frequencies(sem)[0]))
+    return out
+
+
+def test_mixed_matches_plain_on_serving_requests():
+    probs = random_problems(11, [51, 97, 180, 333, 640, 1024, 1500, 2300])
+    for p in probs:
+        x64 = F.solve_problem(p, precision=F.PREC_FP64)
+        xm, st = F.solve_problem(p, precision=F.PREC_MIXED, with_stats=True)
+        fi = int(p["feed_index"])
+        dz = abs(x64[fi] - xm[fi]) / abs(x64[fi])
+        di = rel(xm, x64)
+        print(f"N {len(p['segs'])}: rel dZin {dz:.2e} rel dI {di:.2e} its {st['refine_iters']} "
+              f"bwd {st['backward_error']:.1e}")
+        assert dz <= 1e-6
+        assert di <= 1e-7
+        assert st["backward_error"] <= 1e-12
+
+
+def test_mixed_batch_equals_single_bitwise():
+    probs = random_problems(12, [64, 300, 129, 800, 51])
+    batch = F.solve_batch(probs, precision=F.PREC_MIXED)
+    for p, b in zip(probs, batch):
+        assert b["status"] == F.OK
+        x = F.solve_problem(p, precision=F.PREC_MIXED)
+        assert np.array_equal(x, b["x"])
+    again = F.solve_batch(probs, precision=F.PREC_MIXED)
+    for a, b in zip(batch, again):
+        assert np.array_equal(a["x"], b["x"])
+
+
+def test_config3_mixed_large_solve():
+    """Configuration 3: 32 random wires x 256 segments (N = 8192, one KCL
+    junction) at 7 MHz. Mixed vs the plain FP64 GPU solve of the same deck."""
+    txt = W.config3_deck(rp=False)
+    d = F.Deck(txt)
+    f = d.frequencies("worker")[0]
+    p = d.problem("worker", f)
+    assert len(p["segs"]) == 8192 and len(p["ja"]) == 1
+    x64 = F.solve_problem(p, precision=F.PREC_FP64)
+    xm, st = F.solve_problem(p, precision=F.PREC_MIXED, with_stats=True)
+    fi = int(p["feed_index"])
+    dz = abs(x64[fi] - xm[fi]) / abs(x64[fi])
+    print(f"config 3: rel dZin {dz:.2e}, rel dI {rel(xm, x64):.2e}, its {st['refine_iters']}, "
+          f"backward error {st['backward_error']:.2e}")
+    assert dz <= 1e-6
+    assert rel(xm[:8192], x64[:8192]) <= 1e-7
+    assert st["backward_error"] <= 1e-12
