@@ -9,7 +9,7 @@ one window of requests submitted to the serving session and drained;
 
 Printed (one JSON line): served solves/s (value), p50/p99 latency, the plain
 FP64 one-request-at-a-time GPU path on a sample of the same stream (the
-north-star denominator) with the speedup over it and the max relative
+reference point of the speedup) with the speedup over it and the max relative
 Z_in difference between the served and the plain path, the roofline of the
 dominant kernel from HIP events recorded on its stream during the timed
 region, and a CPU baseline (the oracle, timed on this host on a bounded
@@ -40,6 +40,7 @@ METRIC = "solves/s + p99 latency, mixed-N queue, vs plain FP64 GPU at rel Î”Zinâ
 PEAK_FP64_TFLOPS = 78.6
 PEAK_FP32_TFLOPS = 157.3
 PEAK_HBM_GBS = 8000.0
+HBM_BOUND = ("zsolve", "matvec", "backsub")
 
 RP_THETA = np.repeat(np.arange(0.0, 181.0, 2.0)[None, :], 180, axis=0).ravel()  # phi outer, theta inner
 RP_PHI = np.repeat(np.arange(0.0, 360.0, 2.0), 91)
@@ -150,7 +151,7 @@ def cpu_baseline(stream, budget_s, threads):
     import oracle as O  # noqa: E402
     O.build()
     rng = np.random.default_rng(7)
-    small = [r for r in stream if r["n"] <= 420]
+    small = [r for r in stream if r["n"] <= 600]
     order = rng.permutation(len(small))
 
     def one(r):
@@ -165,17 +166,18 @@ def cpu_baseline(stream, budget_s, threads):
     samples = []
     t0 = time.perf_counter()
     with cf.ThreadPoolExecutor(max_workers=threads) as ex:
-        futs = []
-        for i in order:
-            futs.append(ex.submit(one, small[i]))
-            if len(futs) >= 4 * threads:
-                break
-        for f in cf.as_completed(futs):
-            samples.append(f.result())
-            if time.perf_counter() - t0 > budget_s:
-                break
-        for f in futs:
-            f.cancel()
+        pending = set()
+        it = iter(order)
+        for _ in range(threads):
+            pending.add(ex.submit(one, small[next(it)]))
+        while pending:
+            done, pending = cf.wait(pending, return_when=cf.FIRST_COMPLETED)
+            for f in done:
+                samples.append(f.result())
+                if time.perf_counter() - t0 < budget_s:
+                    nxt = next(it, None)
+                    if nxt is not None:
+                        pending.add(ex.submit(one, small[nxt]))
     wall = time.perf_counter() - t0
     ns = np.array([s[0] for s in samples], float)
     rps = np.array([s[1] for s in samples], float)
@@ -193,7 +195,7 @@ def cpu_baseline(stream, budget_s, threads):
         "unit": "solves/s",
         "cores": threads,
         "kind": "port",
-        "sample": (f"{len(samples)} requests with N<=420 from the same stream solved by the oracle "
+        "sample": (f"{len(samples)} requests with N<=600 from the same stream solved by the oracle "
                    f"(worker semantics + RP) on {threads} threads in {wall:.1f} s; per-request cost "
                    f"fitted as a*N^3 + b*N*[rp] and averaged over all {len(stream)} requests of the "
                    f"stream (extrapolated)"),
@@ -207,7 +209,7 @@ def main():
     ap.add_argument("--steps", type=int, default=12)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--window", type=int, default=256, help="requests per step")
-    ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
+    ap.add_argument("--precision", choices=["fp64", "mixed"], default="mixed")
     ap.add_argument("--plain-sample", type=int, default=256)
     ap.add_argument("--max-batch", type=int, default=256)
     ap.add_argument("--budget-gib", type=float, default=96.0)
@@ -242,25 +244,32 @@ def main():
     print(f"[bench] plain FP64 one-at-a-time: {plain_rate:.1f} solves/s on {len(sample_items)}",
           file=sys.stderr, flush=True)
 
-    # dominant kernel by time during the timed region
-    dom = max((k for k in prof if prof[k][0] > 0), key=lambda k: prof[k][1])
-    launches, tot_us, flops, _ = prof[dom]
+    # dominant kernel class by time during the timed region
+    dom = max((k for k in prof if prof[k][0] > 0 and prof[k][2] > 0), key=lambda k: prof[k][1])
+    launches, tot_us, flops, nbytes = prof[dom]
     avg_us = tot_us / launches
-    achieved = flops / launches / (avg_us * 1e-6) / 1e12
-    peak = PEAK_FP32_TFLOPS if precision == F.PREC_MIXED and dom in ("gram", "lu_update") else PEAK_FP64_TFLOPS
+    if dom in HBM_BOUND:
+        achieved = nbytes / launches / (avg_us * 1e-6) / 1e9
+        peak, unit, bound = PEAK_HBM_GBS, "GB/s", "hbm"
+    else:
+        achieved = flops / launches / (avg_us * 1e-6) / 1e12
+        mfma = dom in ("gram", "lu_update")
+        peak = PEAK_FP32_TFLOPS if precision == F.PREC_MIXED and dom == "lu_update" else PEAK_FP64_TFLOPS
+        unit, bound = "TFLOP/s", "mfma" if mfma else "valu"
     roofline = {
         "kernel": dom,
-        "bound": "mfma" if dom in ("gram", "lu_update") else "valu",
+        "bound": bound,
         "achieved": round(achieved, 3),
         "peak": peak,
-        "unit": "TFLOP/s",
+        "unit": unit,
         "frac": round(achieved / peak, 4),
         "traffic": None,
         "avg_launch_us": round(avg_us, 2),
         "launches": int(launches),
         "kernels": {k: {"launches": int(v[0]), "total_ms": round(v[1] / 1e3, 3),
-                        "tflops": round(v[2] / (v[1] * 1e-6) / 1e12, 3) if v[1] else None}
-                    for k, v in prof.items()},
+                        "tflops": round(v[2] / (v[1] * 1e-6) / 1e12, 3) if v[1] and v[2] else None,
+                        "gbs": round(v[3] / (v[1] * 1e-6) / 1e9, 1) if v[1] and v[3] else None}
+                    for k, v in prof.items() if v[0] > 0},
     }
     cpu = None
     if not args.no_cpu_baseline:

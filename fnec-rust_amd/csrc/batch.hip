@@ -193,11 +193,10 @@ extern "C" {
 
 size_t fnec_hip_request_footprint(size_t n, size_t n_wires, size_t n_junc, size_t rp_npts, int precision)
 {
-    (void)precision;
     const size_t w = n_wires ? n_wires : 1;
     const size_t s = n + w;
     const size_t ncons = 2 * w + n_junc;
-    size_t b = problem_bytes((int)n, (int)s, (int)ncons, (int)w + 1);
+    size_t b = problem_bytes((int)n, (int)s, (int)ncons, (int)w + 1, precision);
     b += rp_npts * (8 + 8 + 32 + 32) + 8 * ((181 * 360 + 255) / 256) + 1024;
     return b;
 }

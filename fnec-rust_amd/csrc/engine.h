@@ -35,7 +35,7 @@ void set_error(const std::string &e);
 const char *last_error();
 int ensure_ready();
 std::mutex &global_mutex();
-size_t problem_bytes(int n, int s, int ncons, int nruns);
+size_t problem_bytes(int n, int s, int ncons, int nruns, int precision);
 
 int build_problem(HostProblem &hp, const fnec_seg *segs, size_t n, const fnec_c64 *rhs,
                   const double *cos_vec, const std::vector<int> &row_col, int ncols,

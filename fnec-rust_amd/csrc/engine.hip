@@ -42,6 +42,7 @@ __global__ void k_mix_init(const Prob *, const int *);
 __global__ void k_mix_start(const Prob *, const int *);
 __global__ void k_mix_dir(const Prob *, const int *);
 __global__ void k_mix_step(const Prob *, const int *);
+__global__ void k_mix_flags(const Prob *, unsigned, int *);
 __global__ void k_mfma_probe32(const float *, const float *, float *);
 __global__ void k_mfma_probe(const double *, const double *, double *);
 
@@ -117,13 +118,24 @@ static hipEvent_t prof_begin()
     return e;
 }
 
-static void prof_end(int cls, hipEvent_t a, double flops, double bytes)
+// Returns the index of the pending record (-1 when profiling is off) so that
+// work known only after the run can be attributed to it.
+static long prof_end(int cls, hipEvent_t a, double flops, double bytes)
 {
     if (!g_prof || !a)
-        return;
+        return -1;
     hipEvent_t b = prof_event();
     (void)hipEventRecord(b, g_ctx.stream);
     g_prof_pending.push_back({cls, a, b, flops, bytes});
+    return (long)g_prof_pending.size() - 1;
+}
+
+static void prof_set_work(long idx, double flops, double bytes)
+{
+    if (idx >= 0 && idx < (long)g_prof_pending.size()) {
+        g_prof_pending[idx].flops = flops;
+        g_prof_pending[idx].bytes = bytes;
+    }
 }
 
 // Called after a stream synchronisation: fold pending event pairs.
@@ -486,10 +498,16 @@ struct Plan {
     size_t off_lud32 = 0;
     size_t fsol_off = 0, fsol_cnt = 0, vsol_off = 0, vsol_cnt = 0, mvz_off = 0, mvz_cnt = 0;
     size_t mvfin_off = 0, mvfin_cnt = 0, mvzh_off = 0, mvzh_cnt = 0;
-    double zsolve_flops = 0, mv_flops = 0;
+    double zsolve_flops = 0, mv_flops = 0, zsolve_bytes = 0, mv_bytes = 0;
+    std::vector<long> it_zs1, it_zs0, it_mv; // profiling records per CG iteration
+    int iters_launched = 0;
+    double fsol_flops = 0;
+    size_t off_flags = 0;
 };
 
 constexpr int kMixMaxIt = 12;        // CG iterations at most
+constexpr int kMixFirstChunk = 4;    // iterations launched before the first convergence check
+constexpr int kMixNextChunk = 2;
 constexpr double kMixTol = 1e-14;    // |h - (M^H M + lambda) x| <= tol |h|
 
 static size_t take(size_t &cur, size_t bytes)
@@ -499,15 +517,25 @@ static size_t take(size_t &cur, size_t bytes)
     return o;
 }
 
-size_t problem_bytes(int n, int s, int ncons, int nruns)
+size_t problem_bytes(int n, int s, int ncons, int nruns, int precision)
 {
-    size_t ldz = (size_t)roundup(n, 64), lda = (size_t)roundup(s, 64);
+    const size_t ldz = (size_t)roundup(n, 64), lda = (size_t)roundup(s, 64);
+    const size_t nn = n, ss = s, nc = std::max(ncons, 1), w = s - n;
     size_t b = 0;
-    b += 2 * 8 * ldz * n + 256;
-    b += 2 * 8 * lda * (s + 1) + 256;
-    b += (size_t)n * 8 * kSegStride + 8 * n + 16 * n + 12 * nruns + 8 * ncons + 8 * ncons + 4 * s;
-    b += 16 * s + 64 + 32 + 32 * ((n + 255) / 256 + 1) + 12 * 256;
-    b += 2 * 8 * lda * kPanel + 16 * kPanel + 16 + 512;
+    b += 2 * 8 * ldz * nn + 256;                                     // Z (FP64)
+    b += nn * 8 * kSegStride + 8 * nn + 16 * nn + 12 * nruns + 16 * nc + 4 * ss + 2048;
+    b += 16 * ss + 64 + 32 + 32 * ((nn + 255) / 256 + 1);            // x, info, stats, residual
+    b += 16 * kPanel + 16 + 256;                                     // panel metadata
+    if (precision == FNEC_PREC_MIXED) {
+        const size_t nchunk = (nn + kMvChunk - 1) / kMvChunk;
+        b += 2 * 4 * ldz * nn + 2 * 4 * ldz * kPanel;                // FP32 factors + panel
+        b += 4 * nn + 16 * nn * nc + 16 * nc * nc + 16 * nc * w + 16 * w * w;
+        b += 4 * 16 * ss + 16 * (nn + nc) + 16 * nn + 16 * nchunk * nn + 8 * nn + 80;
+        b += 17 * 256;                                               // alignment of 17 buffers
+    } else {
+        b += 2 * 8 * lda * (ss + 1) + 256;                           // [G | M^H y]
+        b += 2 * 8 * lda * kPanel + 256;                             // panel workspace
+    }
     return b;
 }
 
@@ -767,11 +795,15 @@ static void plan_build(Plan &pl, const std::vector<const HostProblem *> &hps, bo
             for (int y = 0; y < (hps[b]->n + 255) / 256; ++y)
                 push2((int)b, y);
         pl.res_cnt = (pl.ints.size() - pl.res_off) / 2;
-        pl.zsolve_flops = pl.mv_flops = 0;
+        pl.off_flags = take(cur, 4 * np + 16);
+        pl.zsolve_flops = pl.mv_flops = pl.zsolve_bytes = pl.mv_bytes = pl.fsol_flops = 0;
         for (size_t b = 0; b < np; ++b) {
             const double n = hps[b]->n, r = hps[b]->r;
-            pl.zsolve_flops += 8.0 * n * n;   // L and U (or U^H and L^H): n^2 CMAC
-            pl.mv_flops += 2.0 * 8.0 * n * r; // M p and M^H t
+            pl.zsolve_flops += 8.0 * n * n;    // L and U (or U^H and L^H): n^2 CMAC
+            pl.zsolve_bytes += 8.0 * n * n;    // both FP32 complex triangles read once
+            pl.mv_flops += 2.0 * 8.0 * n * r;  // M p and M^H t
+            pl.mv_bytes += 2.0 * 16.0 * n * n; // FP64 Z read twice
+            pl.fsol_flops += 8.0 * n * n * hps[b]->ncons;
         }
     } else if (with_solve) {
         align4();
@@ -992,37 +1024,59 @@ static int mixed_launch(Plan &pl, char *base)
         hipEvent_t e = prof_begin();
         hipLaunchKernelGGL(k_zsolve, dim3((unsigned)pl.fsol_cnt), dim3(1024), 0, st, dp,
                            (const int2 *)(ints + pl.fsol_off), 1);
-        prof_end(PC_ZSOLVE, e, 0, 0);
+        prof_end(PC_ZSOLVE, e, pl.fsol_flops, pl.fsol_flops);
     }
     hipLaunchKernelGGL(k_pre_setup, dim3(np), dim3(1024), 0, st, dp, all);
     hipLaunchKernelGGL(k_mv_zh, dim3((unsigned)pl.mvzh_cnt), dim3(256), 0, st, dp,
                        (const int2 *)(ints + pl.mvzh_off), 1);
     hipLaunchKernelGGL(k_mix_start, dim3(np), dim3(1024), 0, st, dp, all);
-    for (int it = 0; it < kMixMaxIt; ++it) {
-        hipEvent_t e = prof_begin();
-        hipLaunchKernelGGL(k_zsolve, dim3(np), dim3(1024), 0, st, dp, vsol, 1);
-        prof_end(PC_ZSOLVE, e, pl.zsolve_flops, 0);
-        e = prof_begin();
-        hipLaunchKernelGGL(k_prec_mid, dim3(np), dim3(1024), 0, st, dp, all);
-        prof_end(PC_PCG, e, 0, 0);
-        e = prof_begin();
-        hipLaunchKernelGGL(k_zsolve, dim3(np), dim3(1024), 0, st, dp, vsol, 0);
-        prof_end(PC_ZSOLVE, e, pl.zsolve_flops, 0);
-        e = prof_begin();
-        hipLaunchKernelGGL(k_mix_dir, dim3(np), dim3(1024), 0, st, dp, all);
-        prof_end(PC_PCG, e, 0, 0);
-        e = prof_begin();
-        hipLaunchKernelGGL(k_mv_z, dim3((unsigned)pl.mvz_cnt), dim3(kMvRows), 0, st, dp,
-                           (const int4 *)(ints + pl.mvz_off));
-        hipLaunchKernelGGL(k_mv_fin, dim3((unsigned)pl.mvfin_cnt), dim3(kMvRows), 0, st, dp,
-                           (const int2 *)(ints + pl.mvfin_off));
-        hipLaunchKernelGGL(k_mv_zh, dim3((unsigned)pl.mvzh_cnt), dim3(256), 0, st, dp,
-                           (const int2 *)(ints + pl.mvzh_off), 0);
-        prof_end(PC_MV, e, pl.mv_flops, 0);
-        e = prof_begin();
-        hipLaunchKernelGGL(k_mix_step, dim3(np), dim3(1024), 0, st, dp, all);
-        prof_end(PC_PCG, e, 0, 0);
+    // CG iterations in chunks; between chunks one 4-byte-per-problem read-back
+    // tells whether any problem still iterates.
+    int *flags_dev = (int *)(base + pl.off_flags);
+    std::vector<int> flags(np);
+    pl.it_zs1.clear();
+    pl.it_zs0.clear();
+    pl.it_mv.clear();
+    int it = 0;
+    while (it < kMixMaxIt) {
+        const int chunk = it == 0 ? kMixFirstChunk : kMixNextChunk;
+        for (int c = 0; c < chunk && it < kMixMaxIt; ++c, ++it) {
+            hipEvent_t e = prof_begin();
+            hipLaunchKernelGGL(k_zsolve, dim3(np), dim3(1024), 0, st, dp, vsol, 1);
+            pl.it_zs1.push_back(prof_end(PC_ZSOLVE, e, pl.zsolve_flops, pl.zsolve_bytes));
+            e = prof_begin();
+            hipLaunchKernelGGL(k_prec_mid, dim3(np), dim3(1024), 0, st, dp, all);
+            prof_end(PC_PCG, e, 0, 0);
+            e = prof_begin();
+            hipLaunchKernelGGL(k_zsolve, dim3(np), dim3(1024), 0, st, dp, vsol, 0);
+            pl.it_zs0.push_back(prof_end(PC_ZSOLVE, e, pl.zsolve_flops, pl.zsolve_bytes));
+            e = prof_begin();
+            hipLaunchKernelGGL(k_mix_dir, dim3(np), dim3(1024), 0, st, dp, all);
+            prof_end(PC_PCG, e, 0, 0);
+            e = prof_begin();
+            hipLaunchKernelGGL(k_mv_z, dim3((unsigned)pl.mvz_cnt), dim3(kMvRows), 0, st, dp,
+                               (const int4 *)(ints + pl.mvz_off));
+            hipLaunchKernelGGL(k_mv_fin, dim3((unsigned)pl.mvfin_cnt), dim3(kMvRows), 0, st, dp,
+                               (const int2 *)(ints + pl.mvfin_off));
+            hipLaunchKernelGGL(k_mv_zh, dim3((unsigned)pl.mvzh_cnt), dim3(256), 0, st, dp,
+                               (const int2 *)(ints + pl.mvzh_off), 0);
+            pl.it_mv.push_back(prof_end(PC_MV, e, pl.mv_flops, pl.mv_bytes));
+            e = prof_begin();
+            hipLaunchKernelGGL(k_mix_step, dim3(np), dim3(1024), 0, st, dp, all);
+            prof_end(PC_PCG, e, 0, 0);
+        }
+        if (it >= kMixMaxIt)
+            break;
+        hipLaunchKernelGGL(k_mix_flags, dim3((np + 255) / 256), dim3(256), 0, st, dp, np, flags_dev);
+        HIPCHK(hipMemcpyAsync(flags.data(), flags_dev, 4 * np, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        bool all_done = true;
+        for (unsigned b = 0; b < np; ++b)
+            all_done = all_done && flags[b] != 0;
+        if (all_done)
+            break;
     }
+    pl.iters_launched = it;
     hipLaunchKernelGGL(k_residual_rows, dim3((unsigned)pl.res_cnt), dim3(256), 0, st, dp,
                        (const int2 *)(ints + pl.res_off));
     hipLaunchKernelGGL(k_residual_finish, dim3(np), dim3(64), 0, st, dp, all);
@@ -1139,6 +1193,29 @@ int solve_problems(const std::vector<const HostProblem *> &hps, std::vector<fnec
     if (st)
         return st;
     HIPCHK(hipStreamSynchronize(g_ctx.stream));
+    if (mixed && g_prof) {
+        // the work of CG iteration i is done only by problems that ran it
+        std::vector<int> iters(hps.size(), 0);
+        for (size_t b = 0; b < hps.size(); ++b) {
+            int ps[4];
+            HIPCHK(hipMemcpy(ps, base + pl.mix[b].pstate, 16, hipMemcpyDeviceToHost));
+            iters[b] = ps[1];
+        }
+        for (int i = 0; i < pl.iters_launched; ++i) {
+            double zf = 0, zb = 0, mf = 0, mb = 0;
+            for (size_t b = 0; b < hps.size(); ++b)
+                if (iters[b] > i) {
+                    const double n = hps[b]->n, r = hps[b]->r;
+                    zf += 8.0 * n * n;
+                    zb += 8.0 * n * n;
+                    mf += 16.0 * n * r;
+                    mb += 32.0 * n * n;
+                }
+            prof_set_work(pl.it_zs1[i], zf, zb);
+            prof_set_work(pl.it_zs0[i], zf, zb);
+            prof_set_work(pl.it_mv[i], mf, mb);
+        }
+    }
     prof_collect();
     const float t_fill = ev_ms(0, 1) * 1000.f, t_gram = ev_ms(1, 2) * 1000.f,
                 t_lu = ev_ms(2, 3) * 1000.f, t_sol = ev_ms(3, 4) * 1000.f;

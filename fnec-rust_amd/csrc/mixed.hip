@@ -818,6 +818,15 @@ __global__ __launch_bounds__(kCtlThreads) void k_mix_step(const Prob *__restrict
     }
 }
 
+// Convergence flags of a batch, gathered into one array for the host.
+__global__ __launch_bounds__(256) void k_mix_flags(const Prob *__restrict__ probs, unsigned np,
+                                                   int *__restrict__ flags)
+{
+    const unsigned b = blockIdx.x * 256 + threadIdx.x;
+    if (b < np)
+        flags[b] = (probs[b].info[0] != 0 || probs[b].pstate[0] != 0) ? 1 : 0;
+}
+
 // FP32 MFMA operand-layout probe: D = A B for A (16x4), B (4x16) row-major;
 // writes the raw per-lane accumulator.
 __global__ __launch_bounds__(64) void k_mfma_probe32(const float *__restrict__ a,

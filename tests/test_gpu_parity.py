@@ -3,7 +3,7 @@
 Tolerances (relative unless stated):
 - Z fill elements: 1e-12 (FP64 quadrature; sincos/sqrt/div rounding only).
 - Gram [G | M^H y]: 1e-12 normwise (summation order differs from the oracle).
-- Currents / Z_in: 1e-9 (north-star plain-FP64 bar). The attainable floor is
+- Currents / Z_in: 1e-9 (the plain-FP64 accuracy contract). The attainable floor is
   kappa(G) * eps for any change of summation order: kappa(G) ~ 5e6 for the
   51-segment dipole and ~2e8 at N = 2048, i.e. 1e-10 .. 5e-10; decks whose own
   reference output carries ~1e-8 rounding noise (two mirror-symmetric

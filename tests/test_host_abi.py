@@ -151,3 +151,7 @@ def test_request_footprint_monotone():
     assert 0 < a < b < c
     # dominated by Z and the augmented normal matrix: ~ 32 N^2 bytes
     assert 32 * 200 * 200 < b < 64 * 264 * 264
+    # mixed precision keeps Z (FP64) and its FP32 factors but no normal matrix
+    m = F.lib().fnec_hip_request_footprint(2000, 4, 0, 0, 1)
+    p = F.lib().fnec_hip_request_footprint(2000, 4, 0, 0, 0)
+    assert 24 * 2000 * 2000 < m < p
