@@ -69,6 +69,10 @@ struct Prob {
     double *colsq;         // n: squared column norms of Z
     double *pcg;           // [0] rho, [1] |h|, [2] |r|, [3] |M|_F^2, [4] |x|
     int *pstate;           // [0] converged, [1] iterations, [2] breakdown
+    int *perm;             // n: P Z = L U row permutation of the FP32 factors
+    double *tws;           // n x (ncons + 1) complex: triangular-solve workspace per vector
+    int *tflags;           // (ncons + 1) x 2 x ceil(n / 64): block completion epochs
+    float *dinv;           // ceil(n / 64) x 2 x 64 x 64 complex: L_ii^{-1}, U_ii^{-1}
     int nchunk;
     int maxit;
     double tol;

@@ -32,7 +32,10 @@ __global__ void k_rp_norm(const RpProb *, const int2 *);
 __global__ void k_rp_finish(const RpProb *, const int *);
 __global__ void k_zero(double *, size_t);
 // mixed.hip
-__global__ void k_zsolve(const Prob *, const int2 *, int);
+__global__ void k_trsv(const Prob *, const int4 *, int, int, unsigned long long *);
+__global__ void k_perm(const Prob *, const int *);
+__global__ void k_trsv_multi(const Prob *, const int4 *, int, unsigned long long *);
+__global__ void k_diag_inv(const Prob *, const int2 *);
 __global__ void k_pre_setup(const Prob *, const int *);
 __global__ void k_prec_mid(const Prob *, const int *);
 __global__ void k_mv_z(const Prob *, const int4 *);
@@ -74,6 +77,13 @@ struct Context {
     // cached device arena, grown on demand
     char *arena = nullptr;
     size_t arena_bytes = 0;
+    // far-field jobs read the currents in place from the solve arena, so they
+    // get a device arena of their own
+    char *rp_arena = nullptr;
+    size_t rp_arena_bytes = 0;
+    // pinned host staging for the batch upload and the result read-back
+    char *h_up = nullptr, *h_back = nullptr;
+    size_t h_up_bytes = 0, h_back_bytes = 0;
 };
 
 static Context g_ctx;
@@ -89,9 +99,9 @@ struct ProfPending {
     double flops, bytes;
 };
 enum { PC_FILL, PC_GRAM, PC_LU_PANEL, PC_LU_TRSM, PC_LU_UPDATE, PC_BACKSUB, PC_RP, PC_ZSOLVE,
-       PC_MV, PC_PCG, PC_N };
+       PC_MV, PC_PCG, PC_FSETUP, PC_N };
 static const char *kProfNames[PC_N] = {"fill", "gram", "lu_panel", "lu_trsm", "lu_update",
-                                       "backsub", "rp", "zsolve", "matvec", "pcg_ctl"};
+                                       "backsub", "rp", "zsolve", "matvec", "pcg_ctl", "fsetup"};
 static bool g_prof = false;
 static ProfClass g_prof_acc[PC_N];
 static std::vector<ProfPending> g_prof_pending;
@@ -225,20 +235,42 @@ int ensure_ready()
 
 hipStream_t stream() { return g_ctx.stream; }
 
-static int arena_reserve(size_t bytes, char **out)
+// Pinned host buffer of at least `bytes`, grown on demand (the stream is
+// idle whenever a batch is planned, so the old buffer is free to release).
+static char *pinned(char *&buf, size_t &have, size_t bytes)
 {
-    if (bytes > g_ctx.arena_bytes) {
-        if (g_ctx.arena) {
+    if (bytes <= have)
+        return buf;
+    if (buf) {
+        (void)hipStreamSynchronize(g_ctx.stream);
+        (void)hipHostFree(buf);
+        buf = nullptr;
+        have = 0;
+    }
+    const size_t want = bytes + bytes / 4 + (1u << 20);
+    if (hipHostMalloc((void **)&buf, want, hipHostMallocDefault) != hipSuccess) {
+        (void)hipGetLastError();
+        buf = nullptr;
+        return nullptr;
+    }
+    have = want;
+    return buf;
+}
+
+static int grow_arena(char *&arena, size_t &have, size_t bytes, char **out)
+{
+    if (bytes > have) {
+        if (arena) {
             HIPCHK(hipStreamSynchronize(g_ctx.stream));
-            HIPCHK(hipFree(g_ctx.arena));
-            g_ctx.arena = nullptr;
-            g_ctx.arena_bytes = 0;
+            HIPCHK(hipFree(arena));
+            arena = nullptr;
+            have = 0;
         }
         size_t want = bytes + bytes / 4 + (64u << 20);
-        hipError_t e = hipMalloc(&g_ctx.arena, want);
+        hipError_t e = hipMalloc(&arena, want);
         if (e != hipSuccess) {
             (void)hipGetLastError();
-            e = hipMalloc(&g_ctx.arena, bytes);
+            e = hipMalloc(&arena, bytes);
             if (e != hipSuccess) {
                 (void)hipGetLastError();
                 set_error("device memory exhausted");
@@ -246,10 +278,15 @@ static int arena_reserve(size_t bytes, char **out)
             }
             want = bytes;
         }
-        g_ctx.arena_bytes = want;
+        have = want;
     }
-    *out = g_ctx.arena;
+    *out = arena;
     return FNEC_OK;
+}
+
+static int arena_reserve(size_t bytes, char **out)
+{
+    return grow_arena(g_ctx.arena, g_ctx.arena_bytes, bytes, out);
 }
 
 // ---------------------------------------------------------------------------
@@ -489,6 +526,7 @@ struct Plan {
     bool mixed = false;
     struct MixOff {
         size_t fr, fpr, rowcol, fh, lk, ym, ls, vr, vz, vp, vq, vt, vv, part, colsq, pcg, pstate;
+        size_t perm, tws, tflags, dinv;
         int nchunk;
     };
     std::vector<MixOff> mix;
@@ -497,6 +535,9 @@ struct Plan {
     std::vector<LuDesc<float>> lud32;
     size_t off_lud32 = 0;
     size_t fsol_off = 0, fsol_cnt = 0, vsol_off = 0, vsol_cnt = 0, mvz_off = 0, mvz_cnt = 0;
+    size_t flags_begin = 0, flags_end = 0, off_tickets = 0; // zeroed at the start of a batch
+    size_t dinv_off = 0, dinv_cnt = 0;
+    size_t in_begin = 0, io_begin = 0, io_end = 0; // staged upload [in_begin, io_end), read-back [io_begin, io_end)
     size_t mvfin_off = 0, mvfin_cnt = 0, mvzh_off = 0, mvzh_cnt = 0;
     double zsolve_flops = 0, mv_flops = 0, zsolve_bytes = 0, mv_bytes = 0;
     std::vector<long> it_zs1, it_zs0, it_mv; // profiling records per CG iteration
@@ -508,6 +549,8 @@ struct Plan {
 constexpr int kMixMaxIt = 12;        // CG iterations at most
 constexpr int kMixFirstChunk = 4;    // iterations launched before the first convergence check
 constexpr int kMixNextChunk = 2;
+constexpr int kMaxTrsvLaunches = 2 * kMixMaxIt + 1; // F set-up + two per CG iteration
+constexpr int kMultiRhsHost = 16;                   // right-hand sides per set-up chain (k_trsv_multi)
 constexpr double kMixTol = 1e-14;    // |h - (M^H M + lambda) x| <= tol |h|
 
 static size_t take(size_t &cur, size_t bytes)
@@ -531,7 +574,9 @@ size_t problem_bytes(int n, int s, int ncons, int nruns, int precision)
         b += 2 * 4 * ldz * nn + 2 * 4 * ldz * kPanel;                // FP32 factors + panel
         b += 4 * nn + 16 * nn * nc + 16 * nc * nc + 16 * nc * w + 16 * w * w;
         b += 4 * 16 * ss + 16 * (nn + nc) + 16 * nn + 16 * nchunk * nn + 8 * nn + 80;
-        b += 17 * 256;                                               // alignment of 17 buffers
+        b += 4 * nn + 16 * nn * (nc + 1) + 8 * (nc + 1) * ((nn + 63) / 64);  // perm, solves
+        b += ((nn + 63) / 64) * 2 * 64 * 64 * 8;                     // diagonal-block inverses
+        b += 21 * 256;                                               // alignment of 21 buffers
     } else {
         b += 2 * 8 * lda * (ss + 1) + 256;                           // [G | M^H y]
         b += 2 * 8 * lda * kPanel + 256;                             // panel workspace
@@ -543,7 +588,8 @@ size_t problem_bytes(int n, int s, int ncons, int nruns, int precision)
 // columns: one LuStep per panel position, plus algorithmic flops per launch.
 static void lu_plan(std::vector<int> &ints, const std::vector<int> &ms, const std::vector<int> &ncs,
                     std::vector<LuStep> &steps, std::vector<double> &upd_flops,
-                    std::vector<double> &panel_flops, std::vector<double> &trsm_flops)
+                    std::vector<double> &panel_flops, std::vector<double> &trsm_flops,
+                    bool swap_left = false)
 {
     auto align4 = [&]() {
         while (ints.size() % 4)
@@ -580,6 +626,9 @@ static void lu_plan(std::vector<int> &ints, const std::vector<int> &ms, const st
             const int ncols = ncs[b] - p - jb;
             for (int y = 0; y < (ncols + 63) / 64; ++y)
                 push2((int)b, y);
+            if (swap_left)
+                for (int y = 0; y < (p + 63) / 64; ++y)
+                    push2((int)b, -1 - y);
         }
         st.trsm_cnt = (ints.size() - st.trsm_off) / 2;
         align4();
@@ -673,16 +722,7 @@ static void plan_build(Plan &pl, const std::vector<const HostProblem *> &hps, bo
         const HostProblem &h = *hps[b];
         size_t ldz = (size_t)roundup(h.n, 64);
         pl.off_z[b] = take(cur, 2 * 8 * ldz * h.n);
-        pl.off_seg[b] = take(cur, (size_t)h.n * 8 * kSegStride);
-        pl.off_cos[b] = take(cur, 8 * (size_t)h.n);
-        pl.off_rhs[b] = take(cur, 16 * (size_t)h.n);
-        pl.off_runs[b] = take(cur, 4 * h.runs.size() + 4);
-        pl.off_cons[b] = take(cur, 4 * h.cons.size() + 4);
-        pl.off_sign[b] = take(cur, 8 * h.cons_sign.size() + 8);
         pl.off_ipiv[b] = take(cur, 4 * (size_t)h.s);
-        pl.off_info[b] = take(cur, 16);
-        pl.off_x[b] = take(cur, 16 * (size_t)h.s);
-        pl.off_stats[b] = take(cur, 32);
         pl.off_scr[b] = take(cur, 32 * ((size_t)(h.n + 255) / 256 + 1));
         if (with_solve) {
             if (!pl.mixed)
@@ -695,7 +735,6 @@ static void plan_build(Plan &pl, const std::vector<const HostProblem *> &hps, bo
             m.nchunk = (h.n + kMvChunk - 1) / kMvChunk;
             m.fr = take(cur, 2 * 4 * ldz * n);
             m.fpr = take(cur, 2 * 4 * ldz * kPanel);
-            m.rowcol = take(cur, 4 * n);
             m.fh = take(cur, 16 * n * nc);
             m.lk = take(cur, 16 * nc * nc);
             m.ym = take(cur, 16 * nc * w);
@@ -708,10 +747,36 @@ static void plan_build(Plan &pl, const std::vector<const HostProblem *> &hps, bo
             m.vv = take(cur, 16 * n);
             m.part = take(cur, 16 * (size_t)m.nchunk * n);
             m.colsq = take(cur, 8 * n);
-            m.pcg = take(cur, 64);
-            m.pstate = take(cur, 16);
+            m.perm = take(cur, 4 * n);
+            m.dinv = take(cur, ((n + 63) / 64) * 2 * 64 * 64 * 8);
+            m.tws = take(cur, 16 * n * (nc + 1));
             pl.mix.push_back(m);
         }
+    }
+    if (with_solve && pl.mixed) {
+        // block-completion flags of the triangular solves and the launch
+        // tickets, contiguous so that one fill clears them per batch
+        pl.flags_begin = cur = (cur + 255) & ~(size_t)255;
+        for (size_t b = 0; b < np; ++b) {
+            const size_t nb = (hps[b]->n + 63) / 64, nc = std::max(hps[b]->ncons, 0);
+            pl.mix[b].tflags = take(cur, 4 * (nc + 1) * 2 * nb);
+        }
+        pl.off_tickets = take(cur, 8 * kMaxTrsvLaunches);
+        pl.flags_end = cur = (cur + 7) & ~(size_t)7;
+    }
+    // host inputs, contiguous: one staged copy uploads them (with the
+    // problem table, the work lists and the zeroed result words below)
+    pl.in_begin = cur = (cur + 255) & ~(size_t)255;
+    for (size_t b = 0; b < np; ++b) {
+        const HostProblem &h = *hps[b];
+        pl.off_seg[b] = take(cur, (size_t)h.n * 8 * kSegStride);
+        pl.off_cos[b] = take(cur, 8 * (size_t)h.n);
+        pl.off_rhs[b] = take(cur, 16 * (size_t)h.n);
+        pl.off_runs[b] = take(cur, 4 * h.runs.size() + 4);
+        pl.off_cons[b] = take(cur, 4 * h.cons.size() + 4);
+        pl.off_sign[b] = take(cur, 8 * h.cons_sign.size() + 8);
+        if (with_solve && pl.mixed)
+            pl.mix[b].rowcol = take(cur, 4 * (size_t)h.n);
     }
     // work lists
     auto push4 = [&](int a, int b, int c, int d) {
@@ -758,18 +823,40 @@ static void plan_build(Plan &pl, const std::vector<const HostProblem *> &hps, bo
         for (size_t b = 0; b < np; ++b)
             ms[b] = ncs[b] = hps[b]->n;
         lu_plan(pl.ints, ms, ncs, pl.steps32, pl.s32_upd_flops, pl.s32_panel_flops,
-                pl.s32_trsm_flops);
+                pl.s32_trsm_flops, true);
+        // triangular-solve items (problem, vector, phase, block): phase 0
+        // blocks in forward order, then phase 1 blocks in backward order,
+        // interleaved across chains so that early tickets rarely wait
+        auto trsv_list = [&](bool fsetup, size_t &off, size_t &cnt) {
+            align4();
+            off = pl.ints.size();
+            std::vector<int3> chains; // (problem, vector, blocks)
+            int maxnb = 0;
+            for (size_t b = 0; b < np; ++b) {
+                const int nb = (hps[b]->n + 63) / 64;
+                maxnb = std::max(maxnb, nb);
+                if (fsetup) {
+                    for (int g = 0; g < (hps[b]->ncons + kMultiRhsHost - 1) / kMultiRhsHost; ++g)
+                        chains.push_back(make_int3((int)b, g, nb));
+                } else {
+                    chains.push_back(make_int3((int)b, -1, nb));
+                }
+            }
+            for (int ph = 0; ph < 2; ++ph)
+                for (int st = 0; st < maxnb; ++st)
+                    for (const int3 &c : chains)
+                        if (st < c.z)
+                            push4(c.x, c.y, ph, ph == 0 ? st : c.z - 1 - st);
+            cnt = (pl.ints.size() - off) / 4;
+        };
         align4();
-        pl.fsol_off = pl.ints.size();
+        pl.dinv_off = pl.ints.size();
         for (size_t b = 0; b < np; ++b)
-            for (int c = 0; c < hps[b]->ncons; ++c)
-                push2((int)b, c);
-        pl.fsol_cnt = (pl.ints.size() - pl.fsol_off) / 2;
-        align4();
-        pl.vsol_off = pl.ints.size();
-        for (size_t b = 0; b < np; ++b)
-            push2((int)b, -1);
-        pl.vsol_cnt = np;
+            for (int y = 0; y < (hps[b]->n + 63) / 64; ++y)
+                push2((int)b, y);
+        pl.dinv_cnt = (pl.ints.size() - pl.dinv_off) / 2;
+        trsv_list(true, pl.fsol_off, pl.fsol_cnt);
+        trsv_list(false, pl.vsol_off, pl.vsol_cnt);
         align4();
         pl.mvz_off = pl.ints.size();
         for (size_t b = 0; b < np; ++b)
@@ -846,6 +933,24 @@ static void plan_build(Plan &pl, const std::vector<const HostProblem *> &hps, bo
     if (with_solve && pl.mixed)
         pl.off_lud32 = take(cur, sizeof(LuDesc<float>) * np);
     pl.off_ints = take(cur, 4 * pl.ints.size() + 16);
+    // result words, packed (16-byte alignment): uploaded as zeros, read back
+    // with one copy after the run
+    pl.io_begin = cur = (cur + 255) & ~(size_t)255;
+    auto take16 = [&](size_t bytes) {
+        const size_t o = (cur + 15) & ~(size_t)15;
+        cur = o + bytes;
+        return o;
+    };
+    for (size_t b = 0; b < np; ++b) {
+        pl.off_info[b] = take16(16);
+        pl.off_stats[b] = take16(32);
+        if (with_solve && pl.mixed) {
+            pl.mix[b].pcg = take16(64);
+            pl.mix[b].pstate = take16(16);
+        }
+        pl.off_x[b] = take16(16 * (size_t)hps[b]->s);
+    }
+    pl.io_end = cur = (cur + 255) & ~(size_t)255;
     pl.bytes = cur + 256;
 }
 
@@ -916,6 +1021,10 @@ static int plan_upload(Plan &pl, const std::vector<const HostProblem *> &hps, ch
             P.colsq = (double *)(base + m.colsq);
             P.pcg = (double *)(base + m.pcg);
             P.pstate = (int *)(base + m.pstate);
+            P.perm = (int *)(base + m.perm);
+            P.tws = (double *)(base + m.tws);
+            P.tflags = (int *)(base + m.tflags);
+            P.dinv = (float *)(base + m.dinv);
             P.nchunk = m.nchunk;
             P.maxit = kMixMaxIt;
             P.tol = kMixTol;
@@ -934,8 +1043,7 @@ static int plan_upload(Plan &pl, const std::vector<const HostProblem *> &hps, ch
             D.diagmag = P.diagmag;
             D.fast_ok = P.fast_ok;
             pl.lud32.push_back(D);
-            HIPCHK(hipMemcpyAsync(base + m.rowcol, h.rowcol.data(), 4 * h.rowcol.size(),
-                                  hipMemcpyHostToDevice, st));
+
         } else if (with_solve) {
             P.pr = (double *)(base + pl.off_panel[b]);
             P.pi = P.pr + (size_t)P.lda * kPanel;
@@ -955,31 +1063,38 @@ static int plan_upload(Plan &pl, const std::vector<const HostProblem *> &hps, ch
             D.fast_ok = P.fast_ok;
             pl.lud.push_back(D);
         }
-        HIPCHK(hipMemcpyAsync(base + pl.off_seg[b], h.seg.data(), 8 * h.seg.size(), hipMemcpyHostToDevice, st));
-        HIPCHK(hipMemcpyAsync(base + pl.off_cos[b], h.cosv.data(), 8 * h.cosv.size(), hipMemcpyHostToDevice, st));
-        HIPCHK(hipMemcpyAsync(base + pl.off_rhs[b], h.rhs.data(), 8 * h.rhs.size(), hipMemcpyHostToDevice, st));
-        if (!h.runs.empty())
-            HIPCHK(hipMemcpyAsync(base + pl.off_runs[b], h.runs.data(), 4 * h.runs.size(), hipMemcpyHostToDevice, st));
-        if (!h.cons.empty()) {
-            HIPCHK(hipMemcpyAsync(base + pl.off_cons[b], h.cons.data(), 4 * h.cons.size(), hipMemcpyHostToDevice, st));
-            HIPCHK(hipMemcpyAsync(base + pl.off_sign[b], h.cons_sign.data(), 8 * h.cons_sign.size(), hipMemcpyHostToDevice, st));
-        }
     }
-    // info words start at zero; they are written only by device code after this
-    {
-        std::vector<char> zeros(16, 0);
-        for (size_t b = 0; b < np; ++b)
-            HIPCHK(hipMemcpyAsync(base + pl.off_info[b], zeros.data(), 16, hipMemcpyHostToDevice, st));
+    // one staged copy: host inputs, problem table, work lists, zeroed results
+    const size_t nbytes = pl.io_end - pl.in_begin;
+    char *hs = pinned(g_ctx.h_up, g_ctx.h_up_bytes, nbytes);
+    if (!hs) {
+        set_error("pinned host memory exhausted");
+        return FNEC_RESOURCE_EXHAUSTED;
     }
-    HIPCHK(hipMemcpyAsync(base + pl.off_probs, pl.probs.data(), sizeof(Prob) * np, hipMemcpyHostToDevice, st));
+    auto put = [&](size_t off, const void *src, size_t len) {
+        if (len)
+            std::memcpy(hs + (off - pl.in_begin), src, len);
+    };
+    for (size_t b = 0; b < np; ++b) {
+        const HostProblem &h = *hps[b];
+        put(pl.off_seg[b], h.seg.data(), 8 * h.seg.size());
+        put(pl.off_cos[b], h.cosv.data(), 8 * h.cosv.size());
+        put(pl.off_rhs[b], h.rhs.data(), 8 * h.rhs.size());
+        put(pl.off_runs[b], h.runs.data(), 4 * h.runs.size());
+        put(pl.off_cons[b], h.cons.data(), 4 * h.cons.size());
+        put(pl.off_sign[b], h.cons_sign.data(), 8 * h.cons_sign.size());
+        if (with_solve && pl.mixed)
+            put(pl.mix[b].rowcol, h.rowcol.data(), 4 * h.rowcol.size());
+    }
+    put(pl.off_probs, pl.probs.data(), sizeof(Prob) * np);
     if (with_solve && !pl.mixed)
-        HIPCHK(hipMemcpyAsync(base + pl.off_lud, pl.lud.data(), sizeof(LuDesc<double>) * np,
-                              hipMemcpyHostToDevice, st));
+        put(pl.off_lud, pl.lud.data(), sizeof(LuDesc<double>) * np);
     if (with_solve && pl.mixed)
-        HIPCHK(hipMemcpyAsync(base + pl.off_lud32, pl.lud32.data(), sizeof(LuDesc<float>) * np,
-                              hipMemcpyHostToDevice, st));
-    if (!pl.ints.empty())
-        HIPCHK(hipMemcpyAsync(base + pl.off_ints, pl.ints.data(), 4 * pl.ints.size(), hipMemcpyHostToDevice, st));
+        put(pl.off_lud32, pl.lud32.data(), sizeof(LuDesc<float>) * np);
+    put(pl.off_ints, pl.ints.data(), 4 * pl.ints.size());
+    // result words start at zero; only device code writes them after this
+    std::memset(hs + (pl.io_begin - pl.in_begin), 0, pl.io_end - pl.io_begin);
+    HIPCHK(hipMemcpyAsync(base + pl.in_begin, hs, nbytes, hipMemcpyHostToDevice, st));
     return FNEC_OK;
 }
 
@@ -1018,13 +1133,24 @@ static int mixed_launch(Plan &pl, char *base)
     }
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(g_ctx.ev[3], st));
-    const int2 *vsol = (const int2 *)(ints + pl.vsol_off);
+    const int4 *vsol = (const int4 *)(ints + pl.vsol_off);
+    unsigned long long *tickets = (unsigned long long *)(base + pl.off_tickets);
+    int ntrsv = 0; // launches of k_trsv so far: epoch ntrsv + 1, ticket counter ntrsv
+    {
+        const size_t count = (pl.flags_end - pl.flags_begin) / 8;
+        hipLaunchKernelGGL(k_zero, dim3((unsigned)std::min<size_t>(1024, (count + 255) / 256)), dim3(256), 0,
+                           st, (double *)(base + pl.flags_begin), count);
+    }
+    hipLaunchKernelGGL(k_perm, dim3(np), dim3(256), 0, st, dp, all);
+    hipLaunchKernelGGL(k_diag_inv, dim3((unsigned)pl.dinv_cnt), dim3(64), 0, st, dp,
+                       (const int2 *)(ints + pl.dinv_off));
     hipLaunchKernelGGL(k_mix_init, dim3(np), dim3(1024), 0, st, dp, all);
     if (pl.fsol_cnt) {
         hipEvent_t e = prof_begin();
-        hipLaunchKernelGGL(k_zsolve, dim3((unsigned)pl.fsol_cnt), dim3(1024), 0, st, dp,
-                           (const int2 *)(ints + pl.fsol_off), 1);
-        prof_end(PC_ZSOLVE, e, pl.fsol_flops, pl.fsol_flops);
+        hipLaunchKernelGGL(k_trsv_multi, dim3((unsigned)pl.fsol_cnt), dim3(256), 0, st, dp,
+                           (const int4 *)(ints + pl.fsol_off), ntrsv + 1, tickets + ntrsv);
+        ++ntrsv;
+        prof_end(PC_FSETUP, e, pl.fsol_flops, pl.fsol_flops);
     }
     hipLaunchKernelGGL(k_pre_setup, dim3(np), dim3(1024), 0, st, dp, all);
     hipLaunchKernelGGL(k_mv_zh, dim3((unsigned)pl.mvzh_cnt), dim3(256), 0, st, dp,
@@ -1042,13 +1168,17 @@ static int mixed_launch(Plan &pl, char *base)
         const int chunk = it == 0 ? kMixFirstChunk : kMixNextChunk;
         for (int c = 0; c < chunk && it < kMixMaxIt; ++c, ++it) {
             hipEvent_t e = prof_begin();
-            hipLaunchKernelGGL(k_zsolve, dim3(np), dim3(1024), 0, st, dp, vsol, 1);
+            hipLaunchKernelGGL(k_trsv, dim3((unsigned)pl.vsol_cnt), dim3(256), 0, st, dp, vsol, 1,
+                               ntrsv + 1, tickets + ntrsv);
+            ++ntrsv;
             pl.it_zs1.push_back(prof_end(PC_ZSOLVE, e, pl.zsolve_flops, pl.zsolve_bytes));
             e = prof_begin();
             hipLaunchKernelGGL(k_prec_mid, dim3(np), dim3(1024), 0, st, dp, all);
             prof_end(PC_PCG, e, 0, 0);
             e = prof_begin();
-            hipLaunchKernelGGL(k_zsolve, dim3(np), dim3(1024), 0, st, dp, vsol, 0);
+            hipLaunchKernelGGL(k_trsv, dim3((unsigned)pl.vsol_cnt), dim3(256), 0, st, dp, vsol, 0,
+                               ntrsv + 1, tickets + ntrsv);
+            ++ntrsv;
             pl.it_zs0.push_back(prof_end(PC_ZSOLVE, e, pl.zsolve_flops, pl.zsolve_bytes));
             e = prof_begin();
             hipLaunchKernelGGL(k_mix_dir, dim3(np), dim3(1024), 0, st, dp, all);
@@ -1192,15 +1322,21 @@ int solve_problems(const std::vector<const HostProblem *> &hps, std::vector<fnec
     st = plan_launch(pl, base, true);
     if (st)
         return st;
+    // all result words in one copy
+    const size_t nback = pl.io_end - pl.io_begin;
+    char *hb = pinned(g_ctx.h_back, g_ctx.h_back_bytes, nback);
+    if (!hb) {
+        set_error("pinned host memory exhausted");
+        return FNEC_RESOURCE_EXHAUSTED;
+    }
+    HIPCHK(hipMemcpyAsync(hb, base + pl.io_begin, nback, hipMemcpyDeviceToHost, g_ctx.stream));
     HIPCHK(hipStreamSynchronize(g_ctx.stream));
+    auto back = [&](size_t off) { return hb + (off - pl.io_begin); };
     if (mixed && g_prof) {
         // the work of CG iteration i is done only by problems that ran it
         std::vector<int> iters(hps.size(), 0);
-        for (size_t b = 0; b < hps.size(); ++b) {
-            int ps[4];
-            HIPCHK(hipMemcpy(ps, base + pl.mix[b].pstate, 16, hipMemcpyDeviceToHost));
-            iters[b] = ps[1];
-        }
+        for (size_t b = 0; b < hps.size(); ++b)
+            iters[b] = ((const int *)back(pl.mix[b].pstate))[1];
         for (int i = 0; i < pl.iters_launched; ++i) {
             double zf = 0, zb = 0, mf = 0, mb = 0;
             for (size_t b = 0; b < hps.size(); ++b)
@@ -1223,17 +1359,13 @@ int solve_problems(const std::vector<const HostProblem *> &hps, std::vector<fnec
     status.assign(np, FNEC_OK);
     stats.assign(np, fnec_solve_stats{});
     for (size_t b = 0; b < np; ++b) {
-        int info[4];
-        double sv[4];
-        HIPCHK(hipMemcpy(info, base + pl.off_info[b], 16, hipMemcpyDeviceToHost));
-        HIPCHK(hipMemcpy(sv, base + pl.off_stats[b], 32, hipMemcpyDeviceToHost));
+        const int *info = (const int *)back(pl.off_info[b]);
+        const double *sv = (const double *)back(pl.off_stats[b]);
         status[b] = info[0] ? FNEC_SINGULAR : FNEC_OK;
         fnec_solve_stats &s = stats[b];
         if (mixed && !info[0]) {
-            double pcg[8];
-            int ps[4];
-            HIPCHK(hipMemcpy(pcg, base + pl.mix[b].pcg, 64, hipMemcpyDeviceToHost));
-            HIPCHK(hipMemcpy(ps, base + pl.mix[b].pstate, 16, hipMemcpyDeviceToHost));
+            const double *pcg = (const double *)back(pl.mix[b].pcg);
+            const int *ps = (const int *)back(pl.mix[b].pstate);
             s.refine_iters = ps[1];
             const double den = pcg[3] * pcg[4] + pcg[1];
             s.backward_error = den > 0.0 ? pcg[2] / den : 0.0;
@@ -1256,7 +1388,7 @@ int solve_problems(const std::vector<const HostProblem *> &hps, std::vector<fnec
         s.t_solve_us = t_sol;
         s.t_total_us = t_fill + t_gram + t_lu + t_sol;
         if (x_out.size() > b && x_out[b] && status[b] == FNEC_OK)
-            HIPCHK(hipMemcpy(x_out[b], base + pl.off_x[b], 16 * (size_t)hps[b]->s, hipMemcpyDeviceToHost));
+            std::memcpy(x_out[b], back(pl.off_x[b]), 16 * (size_t)hps[b]->s);
         if (device_x)
             device_x->push_back((double *)(base + pl.off_x[b]));
     }
@@ -1338,16 +1470,11 @@ int rp_run(const std::vector<RpHost> &jobs, std::vector<fnec_rp_row *> &outs, st
         o_tot(nj), o_out(nj);
     std::vector<int> ints;
     size_t pts_off, norm_off, all_off;
+    // device-only work buffers
     for (size_t j = 0; j < nj; ++j) {
         const RpHost &h = jobs[j];
-        o_seg[j] = take(cur, 8 * h.seg.size());
-        o_cur[j] = h.cur_dev ? 0 : take(cur, 8 * h.cur.size());
-        o_th[j] = take(cur, 8 * h.theta.size() + 8);
-        o_ph[j] = take(cur, 8 * h.phi.size() + 8);
         o_field[j] = take(cur, 32 * h.theta.size() + 32);
         o_part[j] = take(cur, 8 * (size_t)((h.ndir + 255) / 256) + 8);
-        o_tot[j] = take(cur, 8);
-        o_out[j] = take(cur, 32 * h.theta.size() + 32);
     }
     pts_off = ints.size();
     for (size_t j = 0; j < nj; ++j)
@@ -1366,15 +1493,41 @@ int rp_run(const std::vector<RpHost> &jobs, std::vector<fnec_rp_row *> &outs, st
     all_off = ints.size();
     for (size_t j = 0; j < nj; ++j)
         ints.push_back((int)j);
+    // inputs (one staged copy), then outputs (one read-back)
+    const size_t in_begin = cur = (cur + 255) & ~(size_t)255;
+    for (size_t j = 0; j < nj; ++j) {
+        const RpHost &h = jobs[j];
+        o_seg[j] = take(cur, 8 * h.seg.size());
+        o_cur[j] = h.cur_dev ? 0 : take(cur, 8 * h.cur.size());
+        o_th[j] = take(cur, 8 * h.theta.size() + 8);
+        o_ph[j] = take(cur, 8 * h.phi.size() + 8);
+    }
     size_t o_probs = take(cur, sizeof(RpProb) * nj);
     size_t o_ints = take(cur, 4 * ints.size() + 16);
+    const size_t out_begin = cur = (cur + 255) & ~(size_t)255;
+    for (size_t j = 0; j < nj; ++j) {
+        o_tot[j] = (cur + 15) & ~(size_t)15;
+        cur = o_tot[j] + 16;
+        o_out[j] = cur;
+        cur += 32 * jobs[j].theta.size();
+    }
+    const size_t out_end = cur = (cur + 255) & ~(size_t)255;
     size_t total = cur + 256;
     char *base = nullptr;
-    int st = arena_reserve(total, &base);
+    int st = grow_arena(g_ctx.rp_arena, g_ctx.rp_arena_bytes, total, &base);
     if (st)
         return st;
     hipStream_t s = g_ctx.stream;
     std::vector<RpProb> probs(nj);
+    char *hs = pinned(g_ctx.h_up, g_ctx.h_up_bytes, out_begin - in_begin);
+    if (!hs) {
+        set_error("pinned host memory exhausted");
+        return FNEC_RESOURCE_EXHAUSTED;
+    }
+    auto put = [&](size_t off, const void *src, size_t len) {
+        if (len)
+            std::memcpy(hs + (off - in_begin), src, len);
+    };
     for (size_t j = 0; j < nj; ++j) {
         const RpHost &h = jobs[j];
         RpProb &R = probs[j];
@@ -1395,16 +1548,15 @@ int rp_run(const std::vector<RpHost> &jobs, std::vector<fnec_rp_row *> &outs, st
         R.npartial = (h.ndir + 255) / 256;
         R.total = (double *)(base + o_tot[j]);
         R.out = (double *)(base + o_out[j]);
-        HIPCHK(hipMemcpyAsync(base + o_seg[j], h.seg.data(), 8 * h.seg.size(), hipMemcpyHostToDevice, s));
+        put(o_seg[j], h.seg.data(), 8 * h.seg.size());
         if (!h.cur_dev)
-            HIPCHK(hipMemcpyAsync(base + o_cur[j], h.cur.data(), 8 * h.cur.size(), hipMemcpyHostToDevice, s));
-        if (!h.theta.empty()) {
-            HIPCHK(hipMemcpyAsync(base + o_th[j], h.theta.data(), 8 * h.theta.size(), hipMemcpyHostToDevice, s));
-            HIPCHK(hipMemcpyAsync(base + o_ph[j], h.phi.data(), 8 * h.phi.size(), hipMemcpyHostToDevice, s));
-        }
+            put(o_cur[j], h.cur.data(), 8 * h.cur.size());
+        put(o_th[j], h.theta.data(), 8 * h.theta.size());
+        put(o_ph[j], h.phi.data(), 8 * h.phi.size());
     }
-    HIPCHK(hipMemcpyAsync(base + o_probs, probs.data(), sizeof(RpProb) * nj, hipMemcpyHostToDevice, s));
-    HIPCHK(hipMemcpyAsync(base + o_ints, ints.data(), 4 * ints.size(), hipMemcpyHostToDevice, s));
+    put(o_probs, probs.data(), sizeof(RpProb) * nj);
+    put(o_ints, ints.data(), 4 * ints.size());
+    HIPCHK(hipMemcpyAsync(base + in_begin, hs, out_begin - in_begin, hipMemcpyHostToDevice, s));
     const RpProb *dp = (const RpProb *)(base + o_probs);
     const int *wi = (const int *)(base + o_ints);
     double rp_terms = 0;
@@ -1419,14 +1571,20 @@ int rp_run(const std::vector<RpHost> &jobs, std::vector<fnec_rp_row *> &outs, st
     prof_end(PC_RP, e_rp, rp_terms * 34.0, 0);
     hipLaunchKernelGGL(k_rp_finish, dim3((unsigned)nj), dim3(256), 0, s, dp, wi + all_off);
     HIPCHK(hipGetLastError());
-    totals.assign(nj, 0.0);
-    for (size_t j = 0; j < nj; ++j) {
-        if (outs[j] && !jobs[j].theta.empty())
-            HIPCHK(hipMemcpyAsync(outs[j], base + o_out[j], 32 * jobs[j].theta.size(), hipMemcpyDeviceToHost, s));
-        HIPCHK(hipMemcpyAsync(&totals[j], base + o_tot[j], 8, hipMemcpyDeviceToHost, s));
+    char *hb = pinned(g_ctx.h_back, g_ctx.h_back_bytes, out_end - out_begin);
+    if (!hb) {
+        set_error("pinned host memory exhausted");
+        return FNEC_RESOURCE_EXHAUSTED;
     }
+    HIPCHK(hipMemcpyAsync(hb, base + out_begin, out_end - out_begin, hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
     prof_collect();
+    totals.assign(nj, 0.0);
+    for (size_t j = 0; j < nj; ++j) {
+        totals[j] = *(const double *)(hb + (o_tot[j] - out_begin));
+        if (outs[j] && !jobs[j].theta.empty())
+            std::memcpy(outs[j], hb + (o_out[j] - out_begin), 32 * jobs[j].theta.size());
+    }
     return FNEC_OK;
 }
 

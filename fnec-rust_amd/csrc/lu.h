@@ -6,10 +6,11 @@
 //         precondition an FP64 iteration.
 //
 // Matrices are column-major split planes (re, im) with leading dimension lda.
-// Row interchanges are applied to the panel and to the columns right of it
-// only; the multipliers of earlier panels keep their original row order, so
-// triangular solves apply each panel's interchanges just before its
-// elimination (the LINPACK xGESL order).
+// Row interchanges are applied to the panel and to the columns right of it;
+// with swap_left (the mixed path) also to the multipliers of earlier panels,
+// so that the factors read P Z = L U with one row permutation P. Without it
+// (the plain path, which needs U and the transformed right-hand side only)
+// earlier multipliers keep their original row order.
 //
 // Kernels per panel step p (jb = min(kPanel, m - p) columns):
 //   k_panel_diag / k_panel_rows / k_panel_commit  no-interchange fast path:
@@ -58,8 +59,11 @@ template <> struct Mfma<float> {
     static __device__ __forceinline__ int row(int lane, int v) { return 4 * (lane >> 4) + v; }
 };
 
+// |z|: hypot for FP64 (the reference's norm(), linear.rs:1346); for the FP32
+// factorisation, which only ranks pivot candidates, a plain sqrt keeps the
+// call inline (hypotf is an out-of-line call that forces register spills).
 __device__ __forceinline__ double cmod(double re, double im) { return hypot(re, im); }
-__device__ __forceinline__ float cmod(float re, float im) { return hypotf(re, im); }
+__device__ __forceinline__ float cmod(float re, float im) { return __builtin_sqrtf(re * re + im * im); }
 
 constexpr int kPanelThreads = 512;
 constexpr int kLuPad = 4;
@@ -170,67 +174,65 @@ __global__ __launch_bounds__(kPanelThreads) void k_lu_panel(const LuDesc<T> *__r
 // factorisation is the partial-pivoting one and is committed; otherwise the
 // panel is left untouched for k_lu_panel.
 
-// One workgroup per problem: LU of the jb x jb diagonal block in LDS.
+// One wavefront per problem: LU of the jb x jb diagonal block in LDS, one
+// lane per row (jb <= 64), so no workgroup barrier is needed.
 template <typename T>
-__global__ __launch_bounds__(256) void k_panel_diag(const LuDesc<T> *__restrict__ descs,
-                                                    const int *__restrict__ plist, int p)
+__global__ __launch_bounds__(64) void k_panel_diag(const LuDesc<T> *__restrict__ descs,
+                                                   const int *__restrict__ plist, int p)
 {
     const LuDesc<T> &D = descs[plist[blockIdx.x]];
     const int s = D.m;
     if (D.info[0] != 0 || p >= s)
         return;
     const int jb = min(kPanel, s - p);
-    const int lda = D.lda, t = threadIdx.x;
+    const int lda = D.lda, lane = threadIdx.x;
     __shared__ T dr[kPanel][kPanel + 1], di[kPanel][kPanel + 1];
-    __shared__ T cmax[kPanel];
-    for (int e = t; e < jb * jb; e += 256) {
+    for (int e = lane; e < jb * jb; e += 64) {
         const int r = e % jb, c = e / jb;
         dr[r][c] = D.ar[(size_t)(p + c) * lda + p + r];
         di[r][c] = D.ai[(size_t)(p + c) * lda + p + r];
     }
-    if (t < kPanel)
-        cmax[t] = 0;
     __syncthreads();
+    T cmax_mine = 0; // lane j keeps column j's maximum
     for (int j = 0; j < jb; ++j) {
         const T ur = dr[j][j], ui = di[j][j];
         const T nsq = ur * ur + ui * ui;
         const T ir = ur / nsq, ii = -ui / nsq;
-        if (t == 0) {
-            T m = 0;
-            for (int r = j + 1; r < jb; ++r)
-                m = fmax(m, cmod(dr[r][j], di[r][j]));
-            cmax[j] = m;
-        }
-        __syncthreads();
-        for (int r = j + 1 + t; r < jb; r += 256) {
-            const T xr = dr[r][j], xi = di[r][j];
-            dr[r][j] = xr * ir - xi * ii;
-            di[r][j] = xr * ii + xi * ir;
-        }
-        __syncthreads();
-        const int nr = jb - j - 1;
-        for (int e = t; e < nr * nr; e += 256) {
-            const int r = j + 1 + e % nr, c = j + 1 + e / nr;
-            const T lr = dr[r][j], li = di[r][j];
-            const T vr = dr[j][c], vi = di[j][c];
-            dr[r][c] -= lr * vr - li * vi;
-            di[r][c] -= lr * vi + li * vr;
+        // |a_rj| below the diagonal at step j (max is order independent)
+        T v = (lane > j && lane < jb) ? cmod(dr[lane][j], di[lane][j]) : (T)0;
+        for (int off = 32; off > 0; off >>= 1)
+            v = fmax(v, __shfl_xor(v, off, 64));
+        if (lane == j)
+            cmax_mine = v;
+        if (lane > j && lane < jb) {
+            const T xr = dr[lane][j], xi = di[lane][j];
+            const T lr = xr * ir - xi * ii, li = xr * ii + xi * ir;
+            dr[lane][j] = lr;
+            di[lane][j] = li;
+            for (int c = j + 1; c < jb; ++c) {
+                const T vr = dr[j][c], vi = di[j][c];
+                dr[lane][c] -= lr * vr - li * vi;
+                di[lane][c] -= lr * vi + li * vr;
+            }
         }
         __syncthreads();
     }
-    for (int e = t; e < jb * jb; e += 256) {
+    for (int e = lane; e < jb * jb; e += 64) {
         const int r = e % jb, c = e / jb;
         D.pr[(size_t)c * lda + p + r] = dr[r][c];
         D.pi[(size_t)c * lda + p + r] = di[r][c];
     }
-    if (t < jb) {
-        D.diagmag[t] = (double)cmod(dr[t][t], di[t][t]);
-        D.colmax[t] = (unsigned long long)__double_as_longlong((double)cmax[t]);
+    if (lane < jb) {
+        D.diagmag[lane] = (double)cmod(dr[lane][lane], di[lane][lane]);
+        D.colmax[lane] = (unsigned long long)__double_as_longlong((double)cmax_mine);
     }
 }
 
 // Rows below the diagonal block: l_r U_D = a_r by forward substitution, one
-// thread per row; the intermediate t_rj is a_rj at elimination step j.
+// thread per row; the intermediate t_rj is a_rj at elimination step j. The
+// row's panel entries are loaded up front so the loads overlap. Arithmetic is
+// FP64 for both factor types (the FP32 instantiation of this fully unrolled
+// recurrence spills registers; FP64 does not, and is at least as accurate).
 template <typename T>
 __global__ __launch_bounds__(256) void k_panel_rows(const LuDesc<T> *__restrict__ descs,
                                                     const int2 *__restrict__ work, int p)
@@ -242,27 +244,32 @@ __global__ __launch_bounds__(256) void k_panel_rows(const LuDesc<T> *__restrict_
         return;
     const int jb = min(kPanel, s - p);
     const int lda = D.lda, t = threadIdx.x;
-    __shared__ T ur_[kPanel][kPanel + 1], ui_[kPanel][kPanel + 1];
-    __shared__ T cm[kPanel][8];
+    __shared__ double ur_[kPanel][kPanel + 1], ui_[kPanel][kPanel + 1];
+    __shared__ double cm[kPanel][8];
     for (int e = t; e < jb * jb; e += 256) {
         const int r = e % jb, c = e / jb;
-        ur_[r][c] = D.pr[(size_t)c * lda + p + r];
-        ui_[r][c] = D.pi[(size_t)c * lda + p + r];
+        ur_[r][c] = (double)D.pr[(size_t)c * lda + p + r];
+        ui_[r][c] = (double)D.pi[(size_t)c * lda + p + r];
     }
-    __syncthreads();
     const int row = p + jb + wk.y * 256 + t;
     const bool active = row < s;
-    const int lane = t & 63, wave = t >> 6;
-    T lr[kPanel], li[kPanel];
+    double lr[kPanel], li[kPanel];
 #pragma unroll
     for (int j = 0; j < kPanel; ++j) {
-        lr[j] = 0;
-        li[j] = 0;
+        lr[j] = 0.0;
+        li[j] = 0.0;
+        if (active && j < jb) {
+            lr[j] = (double)D.ar[(size_t)(p + j) * lda + row];
+            li[j] = (double)D.ai[(size_t)(p + j) * lda + row];
+        }
+    }
+    __syncthreads();
+    const int lane = t & 63, wave = t >> 6;
+#pragma unroll
+    for (int j = 0; j < kPanel; ++j) {
         if (j < jb) {
-            T vr = 0, vi = 0;
+            double vr = lr[j], vi = li[j];
             if (active) {
-                vr = D.ar[(size_t)(p + j) * lda + row];
-                vi = D.ai[(size_t)(p + j) * lda + row];
 #pragma unroll
                 for (int m = 0; m < j; ++m) {
                     vr -= lr[m] * ur_[m][j] - li[m] * ui_[m][j];
@@ -270,26 +277,34 @@ __global__ __launch_bounds__(256) void k_panel_rows(const LuDesc<T> *__restrict_
                 }
             }
             // |a_rj| at step j, reduced over the wave (inactive lanes give 0)
-            T v = active ? cmod(vr, vi) : (T)0;
+            double v = active ? hypot(vr, vi) : 0.0;
             for (int off = 32; off > 0; off >>= 1)
                 v = fmax(v, __shfl_xor(v, off, 64));
             if (lane == 0)
                 cm[j][wave] = v;
             if (active) {
-                const T dr = ur_[j][j], di = ui_[j][j];
-                const T nsq = dr * dr + di * di;
-                const T ir = dr / nsq, ii = -di / nsq;
+                const double dr = ur_[j][j], di = ui_[j][j];
+                const double nsq = dr * dr + di * di;
+                const double ir = dr / nsq, ii = -di / nsq;
                 lr[j] = vr * ir - vi * ii;
                 li[j] = vr * ii + vi * ir;
-                D.pr[(size_t)j * lda + row] = lr[j];
-                D.pi[(size_t)j * lda + row] = li[j];
+            } else {
+                lr[j] = 0.0;
+                li[j] = 0.0;
             }
         }
     }
+    if (active)
+#pragma unroll
+        for (int j = 0; j < kPanel; ++j)
+            if (j < jb) {
+                D.pr[(size_t)j * lda + row] = (T)lr[j];
+                D.pi[(size_t)j * lda + row] = (T)li[j];
+            }
     __syncthreads();
     if (t < jb) {
-        const T v = fmax(fmax(cm[t][0], cm[t][1]), fmax(cm[t][2], cm[t][3]));
-        atomicMax(&D.colmax[t], (unsigned long long)__double_as_longlong((double)v));
+        const double v = fmax(fmax(cm[t][0], cm[t][1]), fmax(cm[t][2], cm[t][3]));
+        atomicMax(&D.colmax[t], (unsigned long long)__double_as_longlong(v));
     }
 }
 
@@ -306,34 +321,41 @@ __global__ __launch_bounds__(256) void k_panel_commit(const LuDesc<T> *__restric
         return;
     const int jb = min(kPanel, s - p);
     const int lda = D.lda, t = threadIdx.x;
-    __shared__ int ok;
-    if (t == 0) {
-        int good = 1;
-        for (int j = 0; j < jb; ++j) {
-            const double m = __longlong_as_double((long long)D.colmax[j]);
-            const double d = D.diagmag[j];
-            if (!(m < d) || d < kSingular)
-                good = 0;
-        }
-        ok = good;
-        if (wk.y == 0)
-            D.fast_ok[0] = good;
+    int bad = 0;
+    if (t < jb) {
+        const double m = __longlong_as_double((long long)D.colmax[t]);
+        const double d = D.diagmag[t];
+        bad = (!(m < d) || d < kSingular) ? 1 : 0;
     }
-    __syncthreads();
+    const int ok = !__syncthreads_or(bad);
+    if (t == 0 && wk.y == 0)
+        D.fast_ok[0] = ok;
     if (!ok)
         return;
     const int row = p + wk.y * 256 + t;
-    if (row < s)
-        for (int j = 0; j < jb; ++j) {
-            D.ar[(size_t)(p + j) * lda + row] = D.pr[(size_t)j * lda + row];
-            D.ai[(size_t)(p + j) * lda + row] = D.pi[(size_t)j * lda + row];
-        }
+    if (row < s) {
+        T vr[kPanel], vi[kPanel];
+#pragma unroll
+        for (int j = 0; j < kPanel; ++j)
+            if (j < jb) {
+                vr[j] = D.pr[(size_t)j * lda + row];
+                vi[j] = D.pi[(size_t)j * lda + row];
+            }
+#pragma unroll
+        for (int j = 0; j < kPanel; ++j)
+            if (j < jb) {
+                D.ar[(size_t)(p + j) * lda + row] = vr[j];
+                D.ai[(size_t)(p + j) * lda + row] = vi[j];
+            }
+    }
     if (wk.y == 0 && t < jb)
         D.ipiv[p + t] = p + t;
 }
 
 // Row interchanges of the panel and U12 = L11^{-1} A12 for columns
-// [p + jb, ncols). One thread per column.
+// [p + jb, ncols). One thread per column. Work items with a negative column
+// block (-1 - b) apply the panel's interchanges to columns [64 b, 64 b + 64)
+// left of the panel (requested by lu_plan when swap_left is set).
 template <typename T>
 __global__ __launch_bounds__(64) void k_lu_trsm(const LuDesc<T> *__restrict__ descs,
                                                 const int2 *__restrict__ work, int p)
@@ -346,32 +368,62 @@ __global__ __launch_bounds__(64) void k_lu_trsm(const LuDesc<T> *__restrict__ de
     const int jb = min(kPanel, s - p);
     const int lda = D.lda;
     __shared__ T lr[kPanel][kPanel + 1], li[kPanel][kPanel + 1];
+    __shared__ int piv_s[kPanel];
     const int t = threadIdx.x;
+    if (t < jb)
+        piv_s[t] = D.ipiv[p + t];
     for (int e = t; e < jb * jb; e += 64) {
         const int rr = e % jb, cc = e / jb;
         lr[rr][cc] = D.ar[(size_t)(p + cc) * lda + p + rr];
         li[rr][cc] = D.ai[(size_t)(p + cc) * lda + p + rr];
     }
     __syncthreads();
+    const bool any_swap = __syncthreads_or(t < jb && piv_s[t] != p + t);
+    if (wk.y < 0) {
+        // columns left of the panel (L of earlier panels): interchanges only,
+        // so that the factors read P Z = L U with one row permutation P
+        const int cl = (-wk.y - 1) * 64 + t;
+        if (!any_swap || cl >= p)
+            return;
+        T *cr = D.ar + (size_t)cl * lda, *ci = D.ai + (size_t)cl * lda;
+        for (int j = 0; j < jb; ++j) {
+            const int piv = piv_s[j];
+            if (piv != p + j) {
+                const T xr = cr[p + j], xi = ci[p + j];
+                cr[p + j] = cr[piv];
+                ci[p + j] = ci[piv];
+                cr[piv] = xr;
+                ci[piv] = xi;
+            }
+        }
+        return;
+    }
     const int c = p + jb + wk.y * 64 + t;
     if (c >= D.ncols)
         return;
     T *cr = D.ar + (size_t)c * lda, *ci = D.ai + (size_t)c * lda;
-    for (int j = 0; j < jb; ++j) {
-        const int piv = D.ipiv[p + j];
-        if (piv != p + j) {
-            const T xr = cr[p + j], xi = ci[p + j];
-            cr[p + j] = cr[piv];
-            ci[p + j] = ci[piv];
-            cr[piv] = xr;
-            ci[piv] = xi;
+    if (any_swap)
+        for (int j = 0; j < jb; ++j) {
+            const int piv = piv_s[j];
+            if (piv != p + j) {
+                const T xr = cr[p + j], xi = ci[p + j];
+                cr[p + j] = cr[piv];
+                ci[p + j] = ci[piv];
+                cr[piv] = xr;
+                ci[piv] = xi;
+            }
         }
-    }
     T ur[kPanel], ui[kPanel];
+#pragma unroll
+    for (int j = 0; j < kPanel; ++j)
+        if (j < jb) {
+            ur[j] = cr[p + j];
+            ui[j] = ci[p + j];
+        }
 #pragma unroll
     for (int j = 0; j < kPanel; ++j) {
         if (j < jb) {
-            T vr = cr[p + j], vi = ci[p + j];
+            T vr = ur[j], vi = ui[j];
             for (int m = 0; m < j; ++m) {
                 vr -= lr[j][m] * ur[m] - li[j][m] * ui[m];
                 vi -= lr[j][m] * ui[m] + li[j][m] * ur[m];
@@ -479,7 +531,7 @@ __global__ __launch_bounds__(256) void k_lu_update(const LuDesc<T> *__restrict__
 template <typename T>
 void lu_launch_panel(const LuDesc<T> *d, const LuStep &s, const int *ints, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_panel_diag<T>, dim3((unsigned)s.panel_cnt), dim3(256), 0, st, d,
+    hipLaunchKernelGGL(k_panel_diag<T>, dim3((unsigned)s.panel_cnt), dim3(64), 0, st, d,
                        ints + s.panel_off, s.p);
     if (s.rows_cnt)
         hipLaunchKernelGGL(k_panel_rows<T>, dim3((unsigned)s.rows_cnt), dim3(256), 0, st, d,

@@ -16,9 +16,11 @@
 // residual h - (M^H M + lambda) x and every vector update are FP64.
 //
 // Kernels:
-//   k_zsolve     Z v = b or Z^H v = b with the FP32 factors, FP64 vector;
-//                one workgroup per (problem, vector), panel-interleaved
-//                interchanges as recorded by the LU (lu.h).
+//   k_trsv       Z v = b or Z^H v = b with the FP32 factors, FP64 vectors;
+//                one workgroup per 64-row block, blocks chained by flags.
+//   k_trsv_multi Z^H solves of the constraint rows, 16 right-hand sides per chain.
+//   k_perm       the row permutation of the factors.
+//   k_diag_inv   inverses of the 64 x 64 diagonal blocks of L and U.
 //   k_pre_setup  K, H, their Cholesky / Schur factors (one workgroup per problem).
 //   k_prec_mid   the dense middle of P^{-1} between the two Z solves.
 //   k_mv_z, k_mv_fin, k_mv_zh   t = M p and q = M^H t + lambda p (FP64 Z).
@@ -30,7 +32,6 @@
 
 namespace fnec {
 
-constexpr int kSolveThreads = 1024;
 constexpr int kCtlThreads = 1024;
 
 __device__ __forceinline__ double2 cm(double2 a, double2 b)
@@ -115,174 +116,438 @@ __device__ __forceinline__ double cons_coef(const Prob &P, int c, int j)
 }
 
 // ---------------------------------------------------------------------------
-// Triangular solves with the FP32 factors
+// Multi-workgroup triangular solves (one launch per Z or Z^H solve)
 // ---------------------------------------------------------------------------
+//
+// The factors read P Z = L U (the LU applies every panel's interchanges to
+// the earlier multipliers too), perm[i] = row of Z that became row i.
+//   Z v = b  : phase 0 y = L^{-1} (P b) into ws, phase 1 v = U^{-1} y into vec.
+//   Z^H v = b: phase 0 w = U^{-H} b into ws, phase 1 u = L^{-H} w into ws,
+//              v = P^T u scattered into vec.
+// Work item (problem, vector, phase, block); each workgroup solves one
+// 64-row block. Blocks publish completion by writing the launch epoch into a
+// flag; a block waits for the flags of the blocks it reads. Workgroups take
+// their item from an atomic ticket in list order, and every item's
+// dependencies precede it in the list, so a workgroup only ever waits on
+// items already taken by running workgroups. Every wait is bounded: past
+// kSpinLimit polls the problem is marked failed and the workgroup moves on.
 
-// Work item: (problem, vector). vector < 0: the CG vector vv (skipped once the
-// problem has converged); vector c >= 0: column c of fh, initialised here to
-// the conjugate transpose of constraint row c and solved with Z^H.
-// mode 0: Z v = b  = forward L with interchanges, then backward U.
-// mode 1: Z^H v = b = forward U^H, then backward L^H with interchanges undone.
-__global__ __launch_bounds__(kSolveThreads) void k_zsolve(const Prob *__restrict__ probs,
-                                                          const int2 *__restrict__ work, int mode)
+constexpr int kTb = 64;              // rows per block
+constexpr long kSpinLimit = 1L << 20; // polls (>= 0.1 s); a legitimate wait is a few ms
+
+__device__ __forceinline__ int *trsv_flag(const Prob &P, int vec, int phase, int blk)
+{
+    const int nb = (P.n + kTb - 1) / kTb;
+    return P.tflags + ((size_t)(vec + 1) * 2 + phase) * nb + blk;
+}
+
+// Thread 0 polls the flag, then the workgroup synchronises; returns false on
+// timeout (the problem is then marked failed).
+__device__ bool wait_flag(const Prob &P, const int *flag, int epoch, int *s_ok)
+{
+    if (threadIdx.x == 0) {
+        long spins = 0;
+        while (__hip_atomic_load(flag, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) != epoch) {
+            __builtin_amdgcn_s_sleep(2);
+            ++spins;
+            // another block of this problem gave up: its flags will never come
+            if ((spins & 255) == 0 &&
+                __hip_atomic_load(&P.pstate[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
+                *s_ok = 0;
+                break;
+            }
+            if (spins > kSpinLimit) {
+                *s_ok = 0;
+                __hip_atomic_store(&P.pstate[2], 4, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                break;
+            }
+        }
+    }
+    __syncthreads();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    return *s_ok != 0;
+}
+
+__global__ __launch_bounds__(256) void k_trsv(const Prob *__restrict__ probs,
+                                              const int4 *__restrict__ work, int mode, int epoch,
+                                              unsigned long long *__restrict__ ticket)
+{
+    __shared__ int s_item, s_ok;
+    __shared__ float dr[kTb][kTb + 1], di[kTb][kTb + 1];
+    __shared__ double2 xs[kTb];
+    __shared__ double2 part[4][kTb];
+    const int t = threadIdx.x, lane = t & 63, q = t >> 6;
+    if (t == 0) {
+        s_item = (int)atomicAdd(ticket, 1ULL);
+        s_ok = 1;
+    }
+    __syncthreads();
+    const int4 wk = work[s_item];
+    const Prob &P = probs[wk.x];
+    const int vecid = wk.y, phase = wk.z, blk = wk.w;
+    if (P.info[0] != 0 || (vecid < 0 && P.pstate[0] != 0))
+        return;
+    const int n = P.n, ld = P.ldz, nb = (n + kTb - 1) / kTb;
+    const int i0 = blk * kTb, jb = min(kTb, n - i0);
+    const float *Fr = P.fr, *Fi = P.fi;
+    double2 *vec = (double2 *)(vecid < 0 ? P.vv : P.fh + 2 * (size_t)n * vecid);
+    double2 *ws = (double2 *)P.tws + (size_t)n * (vecid + 1);
+    // right-hand side of this block
+    double2 acc = make_double2(0.0, 0.0);
+    if (phase == 0) {
+        if (t < jb) {
+            const int r = i0 + t;
+            if (mode == 0)
+                acc = vec[P.perm[r]];
+            else
+                acc = vecid >= 0 ? make_double2(cons_coef(P, vecid, r), 0.0) : vec[r];
+        }
+    } else {
+        if (!wait_flag(P, trsv_flag(P, vecid, 0, nb - 1), epoch, &s_ok))
+            return;
+        if (t < jb)
+            acc = ws[i0 + t];
+    }
+    // stage the inverse of the diagonal block: L_ii^{-1} (mode 0 phase 0, mode 1
+    // phase 1) or U_ii^{-1}; dr[r][c] = inverse[r][c]
+    {
+        const int which = (mode == 0) == (phase == 0) ? 0 : 1;
+        const float *src = P.dinv + ((size_t)blk * 2 + which) * 2 * kTb * kTb;
+        for (int e = t; e < kTb * kTb; e += 256) {
+            const int r = e % kTb, c = e / kTb;
+            dr[r][c] = src[2 * e];
+            di[r][c] = src[2 * e + 1];
+        }
+    }
+    // off-diagonal blocks, in the order their flags become ready
+    const bool fwd = phase == 0;
+    const int jfirst = fwd ? 0 : nb - 1, jlast = fwd ? blk : blk, jstep = fwd ? 1 : -1;
+    double2 pacc = make_double2(0.0, 0.0);   // mode 0: row lane, columns 16q..16q+15
+    double2 racc[16];                        // mode 1: rows 16q..16q+15, column lane
+#pragma unroll
+    for (int k = 0; k < 16; ++k)
+        racc[k] = make_double2(0.0, 0.0);
+    for (int j = jfirst; j != jlast; j += jstep) {
+        if (!wait_flag(P, trsv_flag(P, vecid, phase, j), epoch, &s_ok))
+            return;
+        const int j0 = j * kTb, jn = min(kTb, n - j0);
+        // solved values of block j: phase 0 in ws; phase 1 in vec (mode 0) or ws (mode 1)
+        const double2 *src = (phase == 1 && mode == 0) ? vec : ws;
+        if (mode == 0) {
+            if (t < jn)
+                xs[t] = src[j0 + t];
+            __syncthreads();
+            if (lane < jb) {
+                const int r = i0 + lane;
+                for (int c = 16 * q; c < min(16 * q + 16, jn); ++c) {
+                    const size_t o = (size_t)(j0 + c) * ld + r;
+                    const double ar = Fr[o], ai = Fi[o];
+                    const double2 x = xs[c];
+                    pacc.x += ar * x.x - ai * x.y;
+                    pacc.y += ar * x.y + ai * x.x;
+                }
+            }
+            __syncthreads();
+        } else {
+            // rows r of this block, column c = lane of block j:
+            // U^H[r][c] = conj(U[c][r]) (phase 0), L^H[r][c] = conj(L[c][r]) (phase 1)
+            const double2 x = lane < jn ? src[j0 + lane] : make_double2(0.0, 0.0);
+            if (lane < jn)
+#pragma unroll
+                for (int k = 0; k < 16; ++k) {
+                    const int r = 16 * q + k;
+                    if (r < jb) {
+                        const size_t o = (size_t)(i0 + r) * ld + j0 + lane;
+                        const double ar = Fr[o], ai = Fi[o];
+                        racc[k].x += ar * x.x + ai * x.y;
+                        racc[k].y += ar * x.y - ai * x.x;
+                    }
+                }
+        }
+    }
+    // combine partial sums into acc (rows on lanes of wave 0)
+    if (mode == 0) {
+        part[q][lane] = pacc;
+    } else {
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            const double2 v = wave_sum2(racc[k]);
+            if (lane == 0)
+                part[0][16 * q + k] = v;
+        }
+    }
+    __syncthreads();
+    if (q == 0) {
+        double2 sub;
+        if (mode == 0)
+            sub = cadd(cadd(part[0][lane], part[1][lane]), cadd(part[2][lane], part[3][lane]));
+        else
+            sub = part[0][lane];
+        xs[lane] = lane < jb ? csub(acc, sub) : make_double2(0.0, 0.0);
+    }
+    __syncthreads();
+    // diagonal block: x = T_ii^{-1} acc with T_ii^{-1} = inverse (mode 0) or its
+    // conjugate transpose (mode 1); rows on lanes, columns split over waves
+    {
+        double2 sdiag = make_double2(0.0, 0.0);
+        for (int c = 16 * q; c < 16 * q + 16; ++c) {
+            const double2 tv = mode == 0 ? make_double2(dr[lane][c], di[lane][c])
+                                         : make_double2(dr[c][lane], -di[c][lane]);
+            sdiag = cadd(sdiag, cm(tv, xs[c]));
+        }
+        part[q][lane] = sdiag;
+    }
+    __syncthreads();
+    if (q == 0) {
+        const double2 x = cadd(cadd(part[0][lane], part[1][lane]), cadd(part[2][lane], part[3][lane]));
+        if (lane < jb) {
+            const int r = i0 + lane;
+            if (phase == 0 || mode == 1)
+                ws[r] = x;
+            if (phase == 1)
+                vec[mode == 0 ? r : P.perm[r]] = x;
+        }
+        // publish: this wave wrote every output of the block
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        if (lane == 0)
+            __hip_atomic_store(trsv_flag(P, vecid, phase, blk), epoch, __ATOMIC_RELEASE,
+                               __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+// Z^H solves of up to kMultiRhs right-hand sides at once (the constraint rows
+// of the preconditioner set-up): item (problem, group, phase, block), group g
+// covering constraint rows [16 g, 16 g + 16). Same chaining as k_trsv (mode
+// 1); each factor block is staged in LDS once and applied to every vector.
+constexpr int kMultiRhs = 16;
+
+__global__ __launch_bounds__(256) void k_trsv_multi(const Prob *__restrict__ probs,
+                                                    const int4 *__restrict__ work, int epoch,
+                                                    unsigned long long *__restrict__ ticket)
+{
+    __shared__ int s_item, s_ok;
+    __shared__ float fr_[kTb][kTb + 1], fi_[kTb][kTb + 1]; // factor block, then the inverse
+    __shared__ double2 xv[kTb][kMultiRhs];                   // solved values of block j / acc
+    const int t = threadIdx.x, lane = t & 63, q = t >> 6;
+    if (t == 0) {
+        s_item = (int)atomicAdd(ticket, 1ULL);
+        s_ok = 1;
+    }
+    __syncthreads();
+    const int4 wk = work[s_item];
+    const Prob &P = probs[wk.x];
+    const int grp = wk.y, phase = wk.z, blk = wk.w;
+    if (P.info[0] != 0)
+        return;
+    const int n = P.n, ld = P.ldz, nb = (n + kTb - 1) / kTb;
+    const int v0 = grp * kMultiRhs, nv = min(kMultiRhs, P.ncons - v0);
+    const int i0 = blk * kTb, jb = min(kTb, n - i0);
+    const float *Fr = P.fr, *Fi = P.fi;
+    // thread (row = lane, vectors q, q + 4, q + 8, q + 12)
+    double2 acc[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int v = q + 4 * k;
+        acc[k] = make_double2(0.0, 0.0);
+        if (v < nv && lane < jb) {
+            if (phase == 0)
+                acc[k] = make_double2(cons_coef(P, v0 + v, i0 + lane), 0.0);
+            else
+                acc[k] = make_double2(1.0, 0.0); // placeholder, set after the wait
+        }
+    }
+    if (phase == 1) {
+        if (!wait_flag(P, trsv_flag(P, grp, 0, nb - 1), epoch, &s_ok))
+            return;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int v = q + 4 * k;
+            if (v < nv && lane < jb)
+                acc[k] = ((const double2 *)P.tws)[(size_t)n * (v0 + v + 1) + i0 + lane];
+        }
+    }
+    const bool fwd = phase == 0;
+    const int jfirst = fwd ? 0 : nb - 1, jstep = fwd ? 1 : -1;
+    for (int j = jfirst; j != blk; j += jstep) {
+        if (!wait_flag(P, trsv_flag(P, grp, phase, j), epoch, &s_ok))
+            return;
+        const int j0 = j * kTb, jn = min(kTb, n - j0);
+        // factor block, transposed on the way in: fr_[r][c] = F[j0 + c][i0 + r]
+        for (int e = t; e < kTb * kTb; e += 256) {
+            const int c = e % kTb, r = e / kTb;
+            float a = 0.f, b = 0.f;
+            if (c < jn && r < jb) {
+                const size_t o = (size_t)(i0 + r) * ld + j0 + c;
+                a = Fr[o];
+                b = Fi[o];
+            }
+            fr_[r][c] = a;
+            fi_[r][c] = b;
+        }
+        for (int e = t; e < kTb * kMultiRhs; e += 256) {
+            const int c = e / kMultiRhs, v = e % kMultiRhs;
+            xv[c][v] = (c < jn && v < nv)
+                           ? ((const double2 *)P.tws)[(size_t)n * (v0 + v + 1) + j0 + c]
+                           : make_double2(0.0, 0.0);
+        }
+        __syncthreads();
+        // U^H[r][c] = conj(U[c][r]) (phase 0), L^H[r][c] = conj(L[c][r]) (phase 1)
+        for (int c = 0; c < kTb; ++c) {
+            const double ar = fr_[lane][c], ai = fi_[lane][c];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const double2 x = xv[c][q + 4 * k];
+                acc[k].x -= ar * x.x + ai * x.y;
+                acc[k].y -= ar * x.y - ai * x.x;
+            }
+        }
+        __syncthreads();
+    }
+    // diagonal block: x = (T_ii^{-1})^H acc with the stored inverse of U_ii
+    // (phase 0) or L_ii (phase 1)
+    {
+        const float *src = P.dinv + ((size_t)blk * 2 + (phase == 0 ? 1 : 0)) * 2 * kTb * kTb;
+        for (int e = t; e < kTb * kTb; e += 256) {
+            const int r = e % kTb, c = e / kTb;
+            fr_[r][c] = src[2 * e];
+            fi_[r][c] = src[2 * e + 1];
+        }
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            xv[lane][q + 4 * k] = lane < jb ? acc[k] : make_double2(0.0, 0.0);
+        __syncthreads();
+        double2 x[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            x[k] = make_double2(0.0, 0.0);
+        for (int c = 0; c < kTb; ++c) {
+            const double ar = fr_[c][lane], ai = fi_[c][lane]; // conj(inv[c][r])
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const double2 a = xv[c][q + 4 * k];
+                x[k].x += ar * a.x + ai * a.y;
+                x[k].y += ar * a.y - ai * a.x;
+            }
+        }
+        double2 *ws = (double2 *)P.tws;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int v = q + 4 * k;
+            if (v < nv && lane < jb) {
+                const int r = i0 + lane;
+                ws[(size_t)n * (v0 + v + 1) + r] = x[k];
+                if (phase == 1)
+                    ((double2 *)P.fh)[(size_t)n * (v0 + v) + P.perm[r]] = x[k];
+            }
+        }
+        // publish after every wave's stores are complete
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        __syncthreads();
+        if (t == 0)
+            __hip_atomic_store(trsv_flag(P, grp, phase, blk), epoch, __ATOMIC_RELEASE,
+                               __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+// Inverses of the 64 x 64 diagonal blocks of the FP32 factors: L_ii^{-1}
+// (unit lower) and U_ii^{-1} (upper), computed in FP32 one column per lane,
+// stored column-major per block with zero padding beyond the matrix order.
+// Work item: (problem, block).
+__global__ __launch_bounds__(64) void k_diag_inv(const Prob *__restrict__ probs,
+                                                 const int2 *__restrict__ work)
 {
     const int2 wk = work[blockIdx.x];
     const Prob &P = probs[wk.x];
     if (P.info[0] != 0)
         return;
-    if (wk.y < 0 && P.pstate[0] != 0)
-        return;
-    const int n = P.n, ld = P.ldz, t = threadIdx.x, lane = t & 63, wave = t >> 6;
-    const int nt = kSolveThreads, nw = kSolveThreads / 64;
-    const float *Fr = P.fr, *Fi = P.fi;
-    double2 *vec = (double2 *)(wk.y < 0 ? P.vv : P.fh + 2 * (size_t)n * wk.y);
-    __shared__ float dr[kPanel][kPanel + 1], di[kPanel][kPanel + 1];
-    __shared__ double2 xs[kPanel];
-    __shared__ double2 accs[kPanel];
-    if (wk.y >= 0) {
-        for (int r = t; r < n; r += nt)
-            vec[r] = make_double2(cons_coef(P, wk.y, r), 0.0);
+    const int n = P.n, ld = P.ldz, blk = wk.y, c = threadIdx.x;
+    const int i0 = blk * kTb, jb = min(kTb, n - i0);
+    __shared__ float ar[kTb][kTb + 1], ai[kTb][kTb + 1];
+    __shared__ float xr[kTb][kTb + 1], xi[kTb][kTb + 1];
+    for (int e = c; e < jb * jb; e += 64) {
+        const int r = e % jb, k = e / jb;
+        ar[r][k] = P.fr[(size_t)(i0 + k) * ld + i0 + r];
+        ai[r][k] = P.fi[(size_t)(i0 + k) * ld + i0 + r];
+    }
+    __syncthreads();
+    for (int which = 0; which < 2; ++which) {
+        // column c of the inverse, held in xr/xi[.][c] (this lane's own column)
+        if (which == 0) {
+            for (int r = 0; r < kTb; ++r) {
+                float sr = 0.f, si = 0.f;
+                if (c < jb && r < jb && r > c) {
+                    for (int k = c; k < r; ++k) {
+                        const float lr = ar[r][k], li = ai[r][k];
+                        const float vr = xr[k][c], vi = xi[k][c];
+                        sr -= lr * vr - li * vi;
+                        si -= lr * vi + li * vr;
+                    }
+                } else if (c < jb && r == c) {
+                    sr = 1.f;
+                }
+                xr[r][c] = sr;
+                xi[r][c] = si;
+            }
+        } else {
+            for (int r = kTb - 1; r >= 0; --r) {
+                float sr = 0.f, si = 0.f;
+                if (c < jb && r < jb && r <= c) {
+                    sr = r == c ? 1.f : 0.f;
+                    for (int k = r + 1; k <= c; ++k) {
+                        const float ur = ar[r][k], ui = ai[r][k];
+                        const float vr = xr[k][c], vi = xi[k][c];
+                        sr -= ur * vr - ui * vi;
+                        si -= ur * vi + ui * vr;
+                    }
+                    const float dr = ar[r][r], di = ai[r][r];
+                    const float ns = dr * dr + di * di;
+                    const float qr = (sr * dr + si * di) / ns, qi = (si * dr - sr * di) / ns;
+                    sr = qr;
+                    si = qi;
+                }
+                xr[r][c] = sr;
+                xi[r][c] = si;
+            }
+        }
+        __syncthreads();
+        float *dst = P.dinv + ((size_t)blk * 2 + which) * 2 * kTb * kTb;
+        for (int e = c; e < kTb * kTb; e += 64) {
+            const int r = e % kTb, k = e / kTb;
+            dst[2 * e] = xr[r][k];
+            dst[2 * e + 1] = xi[r][k];
+        }
         __syncthreads();
     }
-    auto stage = [&](int r0, int jb) {
-        for (int e = t; e < jb * jb; e += nt) {
-            const int r = e % jb, c = e / jb;
-            dr[r][c] = Fr[(size_t)(r0 + c) * ld + r0 + r];
-            di[r][c] = Fi[(size_t)(r0 + c) * ld + r0 + r];
-        }
-    };
-    if (mode == 0) {
-        // forward: per panel, interchanges then unit-lower elimination
-        for (int p = 0; p < n; p += kPanel) {
-            const int jb = min(kPanel, n - p);
-            stage(p, jb);
-            if (t == 0)
-                for (int j = 0; j < jb; ++j) {
-                    const int piv = P.ipiv[p + j];
-                    if (piv != p + j) {
-                        const double2 a = vec[p + j];
-                        vec[p + j] = vec[piv];
-                        vec[piv] = a;
-                    }
-                }
-            __syncthreads();
-            if (wave == 0) {
-                double2 x = lane < jb ? vec[p + lane] : make_double2(0.0, 0.0);
-                for (int k = 0; k < jb; ++k) {
-                    const double2 xk = make_double2(__shfl(x.x, k, 64), __shfl(x.y, k, 64));
-                    if (lane > k && lane < jb)
-                        x = csub(x, cm(make_double2(dr[lane][k], di[lane][k]), xk));
-                }
-                if (lane < jb) {
-                    vec[p + lane] = x;
-                    xs[lane] = x;
+}
+
+// perm[i]: row of Z that the LU moved to row i (ipiv applied in order).
+// One workgroup per problem; interchanges are rare, so one wave finds them
+// with a ballot and lane 0 applies them in order.
+__global__ __launch_bounds__(256) void k_perm(const Prob *__restrict__ probs,
+                                              const int *__restrict__ plist)
+{
+    const Prob &P = probs[plist[blockIdx.x]];
+    if (P.info[0] != 0)
+        return;
+    const int n = P.n, t = threadIdx.x;
+    for (int i = t; i < n; i += 256)
+        P.perm[i] = i;
+    __syncthreads();
+    if (t < 64) {
+        for (int j0 = 0; j0 < n; j0 += 64) {
+            const int j = j0 + t;
+            const bool sw = j < n && P.ipiv[j] != j;
+            unsigned long long m = __ballot(sw);
+            while (m) {
+                const int k = __ffsll((long long)m) - 1;
+                m &= m - 1;
+                if (t == 0) {
+                    const int a = j0 + k, b = P.ipiv[j0 + k];
+                    const int x = P.perm[a];
+                    P.perm[a] = P.perm[b];
+                    P.perm[b] = x;
                 }
             }
-            __syncthreads();
-            for (int r = p + jb + t; r < n; r += nt) {
-                double2 acc = vec[r];
-                for (int k = 0; k < jb; ++k) {
-                    const size_t o = (size_t)(p + k) * ld + r;
-                    acc = csub(acc, cm(make_double2(Fr[o], Fi[o]), xs[k]));
-                }
-                vec[r] = acc;
-            }
-            __syncthreads();
-        }
-        // backward: U, blocks from the bottom
-        for (int r1 = n; r1 > 0; r1 -= kPanel) {
-            const int r0 = max(0, r1 - kPanel), jb = r1 - r0;
-            stage(r0, jb);
-            __syncthreads();
-            if (wave == 0) {
-                double2 x = lane < jb ? vec[r0 + lane] : make_double2(0.0, 0.0);
-                for (int k = jb - 1; k >= 0; --k) {
-                    if (lane == k)
-                        x = cdivd(x, make_double2(dr[k][k], di[k][k]));
-                    const double2 xk = make_double2(__shfl(x.x, k, 64), __shfl(x.y, k, 64));
-                    if (lane < k)
-                        x = csub(x, cm(make_double2(dr[lane][k], di[lane][k]), xk));
-                }
-                if (lane < jb) {
-                    vec[r0 + lane] = x;
-                    xs[lane] = x;
-                }
-            }
-            __syncthreads();
-            for (int m = t; m < r0; m += nt) {
-                double2 acc = vec[m];
-                for (int k = 0; k < jb; ++k) {
-                    const size_t o = (size_t)(r0 + k) * ld + m;
-                    acc = csub(acc, cm(make_double2(Fr[o], Fi[o]), xs[k]));
-                }
-                vec[m] = acc;
-            }
-            __syncthreads();
-        }
-    } else {
-        // forward: U^H, row i of U^H is the conjugate of column i of U
-        for (int r0 = 0; r0 < n; r0 += kPanel) {
-            const int jb = min(kPanel, n - r0);
-            stage(r0, jb);
-            for (int i = wave; i < jb; i += nw) {
-                const size_t col = (size_t)(r0 + i) * ld;
-                double2 s = make_double2(0.0, 0.0);
-                for (int c = lane; c < r0; c += 64)
-                    s = cadd(s, cmc(make_double2(Fr[col + c], Fi[col + c]), vec[c]));
-                s = wave_sum2(s);
-                if (lane == 0)
-                    accs[i] = csub(vec[r0 + i], s);
-            }
-            __syncthreads();
-            if (wave == 0) {
-                double2 x = lane < jb ? accs[lane] : make_double2(0.0, 0.0);
-                for (int k = 0; k < jb; ++k) {
-                    if (lane == k)
-                        x = cdivd(x, make_double2(dr[k][k], -di[k][k]));
-                    const double2 xk = make_double2(__shfl(x.x, k, 64), __shfl(x.y, k, 64));
-                    if (lane > k && lane < jb) // U^H[lane][k] = conj(U[k][lane])
-                        x = csub(x, cmc(make_double2(dr[k][lane], di[k][lane]), xk));
-                }
-                if (lane < jb)
-                    vec[r0 + lane] = x;
-            }
-            __syncthreads();
-        }
-        // backward: L^H per panel (last first), then the panel's interchanges
-        for (int p = ((n - 1) / kPanel) * kPanel; p >= 0; p -= kPanel) {
-            const int jb = min(kPanel, n - p), pe = p + jb;
-            stage(p, jb);
-            for (int i = wave; i < jb; i += nw) {
-                const size_t col = (size_t)(p + i) * ld;
-                double2 s = make_double2(0.0, 0.0);
-                for (int c = pe + lane; c < n; c += 64)
-                    s = cadd(s, cmc(make_double2(Fr[col + c], Fi[col + c]), vec[c]));
-                s = wave_sum2(s);
-                if (lane == 0)
-                    accs[i] = csub(vec[p + i], s);
-            }
-            __syncthreads();
-            if (wave == 0) {
-                double2 x = lane < jb ? accs[lane] : make_double2(0.0, 0.0);
-                for (int k = jb - 1; k >= 0; --k) {
-                    const double2 xk = make_double2(__shfl(x.x, k, 64), __shfl(x.y, k, 64));
-                    if (lane < k) // L^H[lane][k] = conj(L[k][lane])
-                        x = csub(x, cmc(make_double2(dr[k][lane], di[k][lane]), xk));
-                }
-                if (lane < jb)
-                    vec[p + lane] = x;
-            }
-            __syncthreads();
-            if (t == 0)
-                for (int j = jb - 1; j >= 0; --j) {
-                    const int piv = P.ipiv[p + j];
-                    if (piv != p + j) {
-                        const double2 a = vec[p + j];
-                        vec[p + j] = vec[piv];
-                        vec[piv] = a;
-                    }
-                }
-            __syncthreads();
         }
     }
 }

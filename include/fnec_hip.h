@@ -213,7 +213,8 @@ int fnec_hip_debug_mfma_probe_f32(const float *a16x4, const float *b4x16, float 
 /* Per-kernel timing with HIP events on the solve stream (off by default).
  * Kernel classes: "fill", "gram", "lu_panel", "lu_trsm", "lu_update",
  * "backsub", "rp", and on the mixed path "zsolve" (Z / Z^H solves with the
- * FP32 factors), "matvec" (M p and M^H t in FP64), "pcg_ctl". out4 = {launches, total microseconds, algorithmic flops,
+ * FP32 factors), "matvec" (M p and M^H t in FP64), "pcg_ctl", "fsetup"
+ * (Z^H solves of the constraint rows). out4 = {launches, total microseconds, algorithmic flops,
  * algorithmic bytes} accumulated since the last reset. */
 void fnec_hip_profile_enable(int on);
 void fnec_hip_profile_reset(void);

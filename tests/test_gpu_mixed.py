@@ -143,3 +143,20 @@ def test_config3_mixed_large_solve():
     assert dz <= 1e-6
     assert rel(xm[:8192], x64[:8192]) <= 1e-7
     assert st["backward_error"] <= 1e-12
+
+
+def test_mixed_with_row_interchanges():
+    """Segments half a radius long: the FP32 LU of Z interchanges rows
+    (kappa(Z) ~ 3e3), which exercises the factor permutation in the solves."""
+    a, n = 0.01, 40
+    ln = n * a * 0.5
+    txt = (f"GW 1 {n} 0 0 {-ln / 2} 0 0 {ln / 2} {a}\nGW 2 {n} 0.05 0 {-ln / 2} 0.05 0 {ln / 2} {a}\n"
+           f"EX 0 1 {n // 2} 0 1 0\nFR 0 1 0 0 30 0\nEN\n")
+    d = F.Deck(txt)
+    p = d.problem("worker", d.frequencies("worker")[0])
+    x64 = F.solve_problem(p, precision=F.PREC_FP64)
+    xm, st = F.solve_problem(p, precision=F.PREC_MIXED, with_stats=True)
+    assert st["n_swaps"] > 0
+    fi = int(p["feed_index"])
+    assert abs(x64[fi] - xm[fi]) / abs(x64[fi]) <= 1e-6
+    assert rel(xm, x64) <= 1e-6
