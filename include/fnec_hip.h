@@ -5,8 +5,9 @@
  * feedpoint impedance / currents, and the RP far-field pattern.
  *
  * Plain C ABI: caller-owned buffers, no pointer retained after a call returns,
- * no exceptions across the boundary. Thread-safe: each call runs on its own
- * stream from a pool; the device context is initialised once per process.
+ * no exceptions across the boundary. Thread-safe: calls are serialised by a
+ * process-wide lock onto one device stream; the device context and its
+ * memory arenas are created once per process and reused.
  *
  * Status codes map onto the reference's error enums:
  *   FNEC_OK                  Ok(..)
@@ -110,8 +111,11 @@ int fnec_hip_fill_z(const fnec_seg *segs, size_t n, double freq_hz, const fnec_g
  * LU with partial pivoting. Z never leaves the device.
  * n_wires == 0 means one range (0, n-1), as in the reference.
  * x_out: n + max(n_wires,1) values (currents, then one constant per wire).
- * precision: FNEC_PREC_FP64 (plain) or FNEC_PREC_MIXED (FP32 MFMA factorisation
- * with FP64 refinement). stats may be NULL. */
+ * precision: FNEC_PREC_FP64 (plain: FP64 Gram + LU of the normal equations,
+ * the reference's algorithm) or FNEC_PREC_MIXED (FP32 MFMA LU of Z used to
+ * precondition FP64 conjugate gradients on the same regularised normal
+ * equations, converged to a relative residual of 1e-14; stats report the
+ * iterations and the normwise backward error). stats may be NULL. */
 int fnec_hip_solve_hallen(const fnec_seg *segs, size_t n, const fnec_c64 *rhs, const double *cos_vec,
                           const size_t *wire_first, const size_t *wire_last, size_t n_wires,
                           const size_t *j_a, const size_t *j_b, const double *j_sign, size_t n_junc,

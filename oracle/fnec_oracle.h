@@ -11,6 +11,15 @@
  *
  * Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg load it.
  * The product library never links, loads or calls anything here.
+ *
+ * Pinning (tests/test_oracle.py): the reference's golden corpus and README
+ * excerpt to the printed digits, its unit-test closed forms, mpmath
+ * evaluations of the fill formula and of the regularised normal equations,
+ * an independently assembled Gram matrix and KCL system, the Hertzian-dipole
+ * and PEC-image closed forms, the split-dipole invariant of the conductor-path
+ * route, and independent evaluations of the residual / diagonal spread.
+ * Parity unpinned: or_radiation_efficiency (the GN 0/2 gain correction of
+ * farfield.rs:308-322); no reference fixture or closed form covers it.
  */
 #ifndef FNEC_ORACLE_H
 #define FNEC_ORACLE_H

@@ -381,3 +381,42 @@ def test_axial_ratio_limits(oracle):
     assert abs(ar((1, 0), (0, 1)) + ar((1, 0), (0, -1))) < 1e-9
     e = ar((2, 0), (0, 1))
     assert 0 < abs(e) < 1
+
+
+def test_conductor_path_route_reproduces_straight_dipole(oracle):
+    """CLI route for a degree-2 reversed junction (solve_session.rs:1338-1378,
+    linear.rs:823-894, excitation.rs:264-341, geometry.rs:404-584): a dipole
+    made of two 25-segment halves that both start at the centre is one
+    conductor path; its currents equal the straight 50-segment dipole's with
+    the sign of the reversed half flipped (the invariant behind the
+    reference's split-dipole tests, general_junction.rs:75-108, held here to
+    rounding)."""
+    ln, a = 10.564, 0.001
+    straight = f"GW 1 50 0 0 {-ln / 2} 0 0 {ln / 2} {a}\nEX 0 1 26 0 1 0\nFR 0 1 0 0 14.2 0\nEN\n"
+    halves = (f"GW 1 25 0 0 0 0 0 {ln / 2} {a}\nGW 2 25 0 0 0 0 0 {-ln / 2} {a}\n"
+              "EX 0 1 1 0 1 0\nFR 0 1 0 0 14.2 0\nEN\n")
+    rs = oracle.run_deck(straight, "cli")
+    rh = oracle.run_deck(halves, "cli")
+    assert rh["used_paths"] and not rs["used_paths"]
+    i_s = rs["currents"]
+    mapped = np.concatenate([i_s[25:50], -i_s[24::-1]])
+    assert np.linalg.norm(rh["currents"] - mapped) / np.linalg.norm(mapped) < 1e-10
+    assert abs(rh["feed_z"][0] - rs["feed_z"][0]) / abs(rs["feed_z"][0]) < 1e-10
+
+
+def test_residual_and_diagonal_spread(oracle):
+    """The diag: line (solve_session.rs:118-207) against an independent numpy
+    evaluation: ||Z I - C cos - rhs|| from the separately assembled M, and
+    max|Z_ii| / min|Z_ii|."""
+    txt = deck("yagi-5elm-51seg")
+    r = oracle.run_deck(txt, "worker")
+    inp = oracle.deck_inputs(txt, r["freq_hz"], "worker")
+    z = oracle.fill_z(inp["segs"], r["freq_hz"])
+    M, y = build_m_numpy(z, inp["cos"], inp["rhs"], list(inp["wf"]), list(inp["wl"]), list(inp["ja"]),
+                         list(inp["jb"]), list(inp["js"]))
+    n = r["n"]
+    res = (M @ r["x"] - y)[:n]
+    assert abs(np.linalg.norm(res) - r["abs_res"]) <= 1e-6 * r["abs_res"] + 1e-18
+    assert abs(np.linalg.norm(res) / np.linalg.norm(y[:n]) - r["rel_res"]) <= 1e-6 * r["rel_res"] + 1e-18
+    d = np.abs(np.diag(z))
+    assert abs(d.max() / d.min() - r["diag_spread"]) <= 1e-12 * r["diag_spread"]
