@@ -589,7 +589,7 @@ size_t problem_bytes(int n, int s, int ncons, int nruns, int precision)
 static void lu_plan(std::vector<int> &ints, const std::vector<int> &ms, const std::vector<int> &ncs,
                     std::vector<LuStep> &steps, std::vector<double> &upd_flops,
                     std::vector<double> &panel_flops, std::vector<double> &trsm_flops,
-                    bool swap_left = false)
+                    bool swap_left = false, int utile = kTile)
 {
     auto align4 = [&]() {
         while (ints.size() % 4)
@@ -641,8 +641,8 @@ static void lu_plan(std::vector<int> &ints, const std::vector<int> &ms, const st
             const int nrows = s - p - jb, ncols = ncs[b] - p - jb;
             if (nrows <= 0 || ncols <= 0)
                 continue;
-            for (int y = 0; y < (nrows + kTile - 1) / kTile; ++y)
-                for (int z = 0; z < (ncols + kTile - 1) / kTile; ++z) {
+            for (int y = 0; y < (nrows + utile - 1) / utile; ++y)
+                for (int z = 0; z < (ncols + utile - 1) / utile; ++z) {
                     ints.push_back((int)b);
                     ints.push_back(y);
                     ints.push_back(z);
@@ -823,7 +823,7 @@ static void plan_build(Plan &pl, const std::vector<const HostProblem *> &hps, bo
         for (size_t b = 0; b < np; ++b)
             ms[b] = ncs[b] = hps[b]->n;
         lu_plan(pl.ints, ms, ncs, pl.steps32, pl.s32_upd_flops, pl.s32_panel_flops,
-                pl.s32_trsm_flops, true);
+                pl.s32_trsm_flops, true, kTile32);
         // triangular-solve items (problem, vector, phase, block): phase 0
         // blocks in forward order, then phase 1 blocks in backward order,
         // interleaved across chains so that early tickets rarely wait
@@ -1127,7 +1127,7 @@ static int mixed_launch(Plan &pl, char *base)
         }
         if (s.upd_cnt) {
             hipEvent_t e = prof_begin();
-            lu_launch_update(lud, s, ints, st);
+            lu_launch_update_f32(lud, s, ints, st);
             prof_end(PC_LU_UPDATE, e, pl.s32_upd_flops[si], 0);
         }
     }

@@ -21,6 +21,8 @@
 //   k_lu_trsm    interchanges + U12 = L11^{-1} A12 (one thread per column).
 //   k_lu_update  A22 -= L21 U12 on MFMA (16x16x4, four real products per
 //                complex product, 64x64 output tiles, K = jb staged in LDS).
+//   k_lu_update_f32  the FP32 variant: 128x128 tiles, accumulators seeded
+//                with C so each element is read and written once per step.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -525,6 +527,151 @@ __global__ __launch_bounds__(256) void k_lu_update(const LuDesc<T> *__restrict__
                 D.ar[o] -= accr[a][b][v];
                 D.ai[o] -= acci[a][b][v];
             }
+}
+
+// FP32 trailing update for the mixed path: A22 -= L21 U12 with 128 x 128
+// output tiles. Four waves in a 2 x 2 arrangement each own a 64 x 64 block =
+// 4 x 4 MFMA 16x16x4 tiles; the accumulators start from C (float4 loads in
+// the accumulator's own row layout: 4 consecutive rows per lane) and the A
+// operand enters negated, so the MFMA chain produces C - L21 U12 directly and
+// C is read and written once per panel step. Panels here are full (jb = 32):
+// the last, partial panel has no rows below it.
+constexpr int kTile32 = 128;
+constexpr int kLds32 = kTile32 + 4;
+
+__global__ __launch_bounds__(256) void k_lu_update_f32(const LuDesc<float> *__restrict__ descs,
+                                                       const int4 *__restrict__ work, int p)
+{
+    typedef Mfma<float> MF;
+    const int4 wk = work[blockIdx.x];
+    const LuDesc<float> &D = descs[wk.x];
+    const int s = D.m, nc = D.ncols;
+    if (D.info[0] != 0 || p + kPanel >= s)
+        return;
+    const int lda = D.lda;
+    const int r0 = p + kPanel + wk.y * kTile32, c0 = p + kPanel + wk.z * kTile32;
+    __shared__ float sa_r[kPanel][kLds32], sa_i[kPanel][kLds32];
+    __shared__ float sb_r[kPanel][kLds32], sb_i[kPanel][kLds32];
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const int wr = wave >> 1, wc = wave & 1;
+    const float *ar = D.ar, *ai = D.ai;
+    // A: column p + k, rows r0 .. r0 + 127 (contiguous): 32 float4 per column
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int e = t + 256 * i;
+        const int k = e >> 5, r4 = (e & 31) * 4;
+        const int gr = r0 + r4;
+        float4 vr = make_float4(0.f, 0.f, 0.f, 0.f), vi = vr;
+        const size_t o = (size_t)(p + k) * lda + gr;
+        if (gr + 3 < s) {
+            vr = *(const float4 *)(ar + o);
+            vi = *(const float4 *)(ai + o);
+        } else if (gr < s) {
+            float tr[4] = {0.f, 0.f, 0.f, 0.f}, ti[4] = {0.f, 0.f, 0.f, 0.f};
+            for (int q = 0; q < 4 && gr + q < s; ++q) {
+                tr[q] = ar[o + q];
+                ti[q] = ai[o + q];
+            }
+            vr = make_float4(tr[0], tr[1], tr[2], tr[3]);
+            vi = make_float4(ti[0], ti[1], ti[2], ti[3]);
+        }
+        *(float4 *)&sa_r[k][r4] = vr;
+        *(float4 *)&sa_i[k][r4] = vi;
+    }
+    // B: column c0 + c, rows p .. p + 31 (contiguous): 8 float4 per column
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int e = t + 256 * i;
+        const int c = e >> 3, k4 = (e & 7) * 4;
+        const int gc = c0 + c;
+        float4 vr = make_float4(0.f, 0.f, 0.f, 0.f), vi = vr;
+        if (gc < nc) {
+            const size_t o = (size_t)gc * lda + p + k4;
+            vr = *(const float4 *)(ar + o);
+            vi = *(const float4 *)(ai + o);
+        }
+        sb_r[k4][c] = vr.x;
+        sb_r[k4 + 1][c] = vr.y;
+        sb_r[k4 + 2][c] = vr.z;
+        sb_r[k4 + 3][c] = vr.w;
+        sb_i[k4][c] = vi.x;
+        sb_i[k4 + 1][c] = vi.y;
+        sb_i[k4 + 2][c] = vi.z;
+        sb_i[k4 + 3][c] = vi.w;
+    }
+    // accumulators = C
+    typename MF::acc accr[4][4], acci[4][4];
+    const int lrow = lane & 15, lk = lane >> 4;
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            const int gi = r0 + wr * 64 + a * 16 + 4 * lk; // rows gi .. gi + 3
+            const int gj = c0 + wc * 64 + b * 16 + lrow;
+            f4 cr = f4{0.f, 0.f, 0.f, 0.f}, ci = cr;
+            if (gj < nc && gi < s) {
+                const size_t o = (size_t)gj * lda + gi;
+                if (gi + 3 < s) {
+                    const float4 x = *(const float4 *)(ar + o), y = *(const float4 *)(ai + o);
+                    cr = f4{x.x, x.y, x.z, x.w};
+                    ci = f4{y.x, y.y, y.z, y.w};
+                } else {
+                    for (int q = 0; q < 4 && gi + q < s; ++q) {
+                        cr[q] = ar[o + q];
+                        ci[q] = ai[o + q];
+                    }
+                }
+            }
+            accr[a][b] = cr;
+            acci[a][b] = ci;
+        }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < kPanel; kk += 4) {
+        float xr[4], xi[4], yr[4], yi[4];
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+            xr[a] = sa_r[kk + lk][wr * 64 + a * 16 + lrow];
+            xi[a] = sa_i[kk + lk][wr * 64 + a * 16 + lrow];
+            yr[a] = sb_r[kk + lk][wc * 64 + a * 16 + lrow];
+            yi[a] = sb_i[kk + lk][wc * 64 + a * 16 + lrow];
+        }
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+                // C - (xr + i xi)(yr + i yi)
+                accr[a][b] = MF::fma(-xr[a], yr[b], accr[a][b]);
+                accr[a][b] = MF::fma(xi[a], yi[b], accr[a][b]);
+                acci[a][b] = MF::fma(-xr[a], yi[b], acci[a][b]);
+                acci[a][b] = MF::fma(-xi[a], yr[b], acci[a][b]);
+            }
+    }
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            const int gi = r0 + wr * 64 + a * 16 + 4 * lk;
+            const int gj = c0 + wc * 64 + b * 16 + lrow;
+            if (gj >= nc || gi >= s)
+                continue;
+            const size_t o = (size_t)gj * lda + gi;
+            if (gi + 3 < s) {
+                *(float4 *)(D.ar + o) = make_float4(accr[a][b][0], accr[a][b][1], accr[a][b][2], accr[a][b][3]);
+                *(float4 *)(D.ai + o) = make_float4(acci[a][b][0], acci[a][b][1], acci[a][b][2], acci[a][b][3]);
+            } else {
+                for (int q = 0; q < 4 && gi + q < s; ++q) {
+                    D.ar[o + q] = accr[a][b][q];
+                    D.ai[o + q] = acci[a][b][q];
+                }
+            }
+        }
+}
+
+inline void lu_launch_update_f32(const LuDesc<float> *d, const LuStep &s, const int *ints, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_lu_update_f32, dim3((unsigned)s.upd_cnt), dim3(256), 0, st, d,
+                       (const int4 *)(ints + s.upd_off), s.p);
 }
 
 // Host side: one panel step of the batched factorisation on stream st.
