@@ -203,6 +203,36 @@ def cpu_baseline(stream, budget_s, threads):
     }
 
 
+def measured_traffic(kernel, precision):
+    """HBM bytes per launch of `kernel` measured with rocprofv3 FETCH_SIZE /
+    WRITE_SIZE on this workload (scaled by the copy-probe calibration), from
+    the committed summary in profiles/; None when no measurement matches."""
+    path = os.path.join(ROOT, "profiles", "serving_stream_traffic.json")
+    try:
+        with open(path) as f:
+            d = json.load(f)
+    except (OSError, ValueError):
+        return None
+    e = d.get("kernels", {}).get(kernel)
+    if not e or e.get("precision") != ("mixed" if precision == F.PREC_MIXED else "fp64"):
+        return None
+    return {"bytes_per_launch": e["hbm_bytes_per_launch"], "source": "profiles/serving_stream_traffic.json"}
+
+
+def copy_probe():
+    """A plain device-to-device copy of 256 MiB, five times: reads and writes
+    exactly 256 MiB per dispatch, the reference size for scaling FETCH_SIZE /
+    WRITE_SIZE counter readings into HBM bytes."""
+    import torch
+
+    x = torch.ones(64 << 20, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(5):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes": x.numel() * 4, "dispatches": 5}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -215,7 +245,12 @@ def main():
     ap.add_argument("--budget-gib", type=float, default=96.0)
     ap.add_argument("--cpu-budget-s", type=float, default=20.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run a plain 256 MiB device copy (calibrates HBM byte counters)")
     args = ap.parse_args()
+    if args.copy_probe:
+        copy_probe()
+        return
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; multi-GPU runs are not part of it "
               "(requests shard across GPUs with no device communication)", file=sys.stderr)
@@ -244,18 +279,22 @@ def main():
     print(f"[bench] plain FP64 one-at-a-time: {plain_rate:.1f} solves/s on {len(sample_items)}",
           file=sys.stderr, flush=True)
 
-    # dominant kernel class by time during the timed region
+    # dominant kernel class by time during the timed region; its bound is the
+    # roofline side its algorithmic intensity falls on (flops per HBM byte vs
+    # the ridge peak_flops / peak_bandwidth)
     dom = max((k for k in prof if prof[k][0] > 0 and prof[k][2] > 0), key=lambda k: prof[k][1])
     launches, tot_us, flops, nbytes = prof[dom]
     avg_us = tot_us / launches
-    if dom in HBM_BOUND:
+    mfma = dom in ("gram", "lu_update")
+    peak_f = PEAK_FP32_TFLOPS if precision == F.PREC_MIXED and dom == "lu_update" else PEAK_FP64_TFLOPS
+    ridge = peak_f * 1e12 / (PEAK_HBM_GBS * 1e9)
+    if dom in HBM_BOUND or (nbytes and flops / nbytes < ridge):
         achieved = nbytes / launches / (avg_us * 1e-6) / 1e9
         peak, unit, bound = PEAK_HBM_GBS, "GB/s", "hbm"
     else:
         achieved = flops / launches / (avg_us * 1e-6) / 1e12
-        mfma = dom in ("gram", "lu_update")
-        peak = PEAK_FP32_TFLOPS if precision == F.PREC_MIXED and dom == "lu_update" else PEAK_FP64_TFLOPS
-        unit, bound = "TFLOP/s", "mfma" if mfma else "valu"
+        peak, unit, bound = peak_f, "TFLOP/s", "mfma" if mfma else "valu"
+    traffic = measured_traffic(dom, precision)
     roofline = {
         "kernel": dom,
         "bound": bound,
@@ -263,12 +302,17 @@ def main():
         "peak": peak,
         "unit": unit,
         "frac": round(achieved / peak, 4),
-        "traffic": None,
+        "traffic": traffic["bytes_per_launch"] if traffic else None,
+        "traffic_source": traffic["source"] if traffic else None,
+        "intensity_flop_per_byte": round(flops / nbytes, 2) if nbytes else None,
+        "ridge_flop_per_byte": round(ridge, 2),
         "avg_launch_us": round(avg_us, 2),
         "launches": int(launches),
+        "algorithmic_bytes_per_launch": round(nbytes / launches) if nbytes else None,
         "kernels": {k: {"launches": int(v[0]), "total_ms": round(v[1] / 1e3, 3),
                         "tflops": round(v[2] / (v[1] * 1e-6) / 1e12, 3) if v[1] and v[2] else None,
-                        "gbs": round(v[3] / (v[1] * 1e-6) / 1e9, 1) if v[1] and v[3] else None}
+                        "gbs": round(v[3] / (v[1] * 1e-6) / 1e9, 1) if v[1] and v[3] else None,
+                        "bytes_per_launch": round(v[3] / v[0]) if v[3] else None}
                     for k, v in prof.items() if v[0] > 0},
     }
     cpu = None

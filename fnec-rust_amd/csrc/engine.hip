@@ -531,7 +531,7 @@ struct Plan {
     };
     std::vector<MixOff> mix;
     std::vector<LuStep> steps32;
-    std::vector<double> s32_upd_flops, s32_panel_flops, s32_trsm_flops;
+    std::vector<double> s32_upd_flops, s32_panel_flops, s32_trsm_flops, s32_upd_bytes;
     std::vector<LuDesc<float>> lud32;
     size_t off_lud32 = 0;
     size_t fsol_off = 0, fsol_cnt = 0, vsol_off = 0, vsol_cnt = 0, mvz_off = 0, mvz_cnt = 0;
@@ -589,7 +589,8 @@ size_t problem_bytes(int n, int s, int ncons, int nruns, int precision)
 static void lu_plan(std::vector<int> &ints, const std::vector<int> &ms, const std::vector<int> &ncs,
                     std::vector<LuStep> &steps, std::vector<double> &upd_flops,
                     std::vector<double> &panel_flops, std::vector<double> &trsm_flops,
-                    bool swap_left = false, int utile = kTile)
+                    bool swap_left = false, int utile = kTile, std::vector<double> *upd_bytes = nullptr,
+                    double elem_bytes = 16.0)
 {
     auto align4 = [&]() {
         while (ints.size() % 4)
@@ -671,7 +672,7 @@ static void lu_plan(std::vector<int> &ints, const std::vector<int> &ms, const st
                 push2((int)b, y);
         }
         st.commit_cnt = (ints.size() - st.commit_off) / 2;
-        double fu = 0, fp = 0, ft = 0;
+        double fu = 0, fp = 0, ft = 0, bu = 0;
         for (size_t b = 0; b < np; ++b) {
             const int s = ms[b];
             if (s <= p)
@@ -681,7 +682,12 @@ static void lu_plan(std::vector<int> &ints, const std::vector<int> &ms, const st
             fu += 8.0 * nrows * ncols * jb;
             fp += 8.0 * m * jb * jb / 2.0;
             ft += 8.0 * ncols * jb * jb / 2.0;
+            // trailing block read and written once, panel and U12 read once
+            if (nrows > 0 && ncols > 0)
+                bu += elem_bytes * (2.0 * nrows * ncols + nrows * jb + jb * ncols);
         }
+        if (upd_bytes)
+            upd_bytes->push_back(bu);
         upd_flops.push_back(fu);
         panel_flops.push_back(fp);
         trsm_flops.push_back(ft);
@@ -823,7 +829,7 @@ static void plan_build(Plan &pl, const std::vector<const HostProblem *> &hps, bo
         for (size_t b = 0; b < np; ++b)
             ms[b] = ncs[b] = hps[b]->n;
         lu_plan(pl.ints, ms, ncs, pl.steps32, pl.s32_upd_flops, pl.s32_panel_flops,
-                pl.s32_trsm_flops, true, kTile32);
+                pl.s32_trsm_flops, true, kTile32, &pl.s32_upd_bytes, 8.0);
         // triangular-solve items (problem, vector, phase, block): phase 0
         // blocks in forward order, then phase 1 blocks in backward order,
         // interleaved across chains so that early tickets rarely wait
@@ -1128,7 +1134,7 @@ static int mixed_launch(Plan &pl, char *base)
         if (s.upd_cnt) {
             hipEvent_t e = prof_begin();
             lu_launch_update_f32(lud, s, ints, st);
-            prof_end(PC_LU_UPDATE, e, pl.s32_upd_flops[si], 0);
+            prof_end(PC_LU_UPDATE, e, pl.s32_upd_flops[si], pl.s32_upd_bytes[si]);
         }
     }
     HIPCHK(hipGetLastError());
