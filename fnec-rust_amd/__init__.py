@@ -32,7 +32,7 @@ EXPORTS = [
     "fnec_hip_solve_batch", "fnec_hip_session_create", "fnec_hip_session_submit",
     "fnec_hip_session_poll", "fnec_hip_session_pending", "fnec_hip_session_destroy",
     "fnec_hip_request_footprint", "fnec_hip_debug_normal_eqs", "fnec_hip_debug_mfma_probe",
-    "fnec_hip_debug_mfma_probe_f32",
+    "fnec_hip_debug_mfma_probe_f32", "fnec_hip_set_tunable", "fnec_hip_get_tunable",
 ]
 
 
@@ -120,6 +120,9 @@ def lib():
                                                 C.POINTER(Ground), vp]
         L.fnec_hip_debug_mfma_probe.argtypes = [vp, vp, vp]
         L.fnec_hip_debug_mfma_probe_f32.argtypes = [vp, vp, vp]
+        L.fnec_hip_set_tunable.argtypes = [C.c_char_p, C.c_long]
+        L.fnec_hip_get_tunable.argtypes = [C.c_char_p]
+        L.fnec_hip_get_tunable.restype = C.c_long
         _lib = L
     return _lib
 
@@ -238,6 +241,14 @@ def debug_mfma_probe(a16x4: np.ndarray, b4x16: np.ndarray) -> np.ndarray:
     out = np.zeros((64, 4))
     _check(lib().fnec_hip_debug_mfma_probe(_p(a), _p(b), _p(out)))
     return out
+
+
+def set_tunable(name: str, value: int) -> None:
+    _check(lib().fnec_hip_set_tunable(name.encode(), int(value)))
+
+
+def get_tunable(name: str) -> int:
+    return int(lib().fnec_hip_get_tunable(name.encode()))
 
 
 def debug_mfma_probe_f32(a16x4: np.ndarray, b4x16: np.ndarray) -> np.ndarray:

@@ -194,6 +194,18 @@ int fnec_hip_debug_mfma_probe_f32(const float *a16x4, const float *b4x16, float 
     return mfma_probe32(a16x4, b4x16, lanes_out);
 }
 
+int fnec_hip_set_tunable(const char *name, long value)
+{
+    std::lock_guard<std::mutex> lk(global_mutex());
+    return set_tunable(name, value);
+}
+
+long fnec_hip_get_tunable(const char *name)
+{
+    std::lock_guard<std::mutex> lk(global_mutex());
+    return get_tunable(name);
+}
+
 void fnec_hip_profile_enable(int on)
 {
     std::lock_guard<std::mutex> lk(global_mutex());

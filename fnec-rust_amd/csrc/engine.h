@@ -60,6 +60,8 @@ int rp_run(const std::vector<RpHost> &jobs, std::vector<fnec_rp_row *> &outs,
            std::vector<double> &totals);
 int mfma_probe(const double *a, const double *b, double *out);
 int mfma_probe32(const float *a, const float *b, float *out);
+int set_tunable(const char *name, long value);
+long get_tunable(const char *name);
 void profile_enable(int on);
 void profile_reset();
 int profile_read(const char *name, double *out4);

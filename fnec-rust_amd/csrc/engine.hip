@@ -32,9 +32,11 @@ __global__ void k_rp_norm(const RpProb *, const int2 *);
 __global__ void k_rp_finish(const RpProb *, const int *);
 __global__ void k_zero(double *, size_t);
 // mixed.hip
-__global__ void k_trsv(const Prob *, const int4 *, int, int, unsigned long long *);
+__global__ void k_trsv0(const Prob *, const int4 *, int, unsigned long long *);
+__global__ void k_trsv1(const Prob *, const int4 *, int, unsigned long long *);
 __global__ void k_perm(const Prob *, const int *);
 __global__ void k_trsv_multi(const Prob *, const int4 *, int, unsigned long long *);
+__global__ void k_zsolve1(const Prob *, const int *, int);
 __global__ void k_diag_inv(const Prob *, const int2 *);
 __global__ void k_pre_setup(const Prob *, const int *);
 __global__ void k_prec_mid(const Prob *, const int *);
@@ -537,12 +539,14 @@ struct Plan {
     size_t fsol_off = 0, fsol_cnt = 0, vsol_off = 0, vsol_cnt = 0, mvz_off = 0, mvz_cnt = 0;
     size_t flags_begin = 0, flags_end = 0, off_tickets = 0; // zeroed at the start of a batch
     size_t dinv_off = 0, dinv_cnt = 0;
+    size_t one_off = 0, one_cnt = 0; // problems solved by one workgroup (n <= kSolve1MaxNHost)
+    int one_maxn = 0;
     size_t in_begin = 0, io_begin = 0, io_end = 0; // staged upload [in_begin, io_end), read-back [io_begin, io_end)
     size_t mvfin_off = 0, mvfin_cnt = 0, mvzh_off = 0, mvzh_cnt = 0;
     double zsolve_flops = 0, mv_flops = 0, zsolve_bytes = 0, mv_bytes = 0;
     std::vector<long> it_zs1, it_zs0, it_mv; // profiling records per CG iteration
     int iters_launched = 0;
-    double fsol_flops = 0;
+    double fsol_flops = 0, fsol_bytes = 0;
     size_t off_flags = 0;
 };
 
@@ -551,6 +555,11 @@ constexpr int kMixFirstChunk = 4;    // iterations launched before the first con
 constexpr int kMixNextChunk = 2;
 constexpr int kMaxTrsvLaunches = 2 * kMixMaxIt + 1; // F set-up + two per CG iteration
 constexpr int kMultiRhsHost = 16;                   // right-hand sides per set-up chain (k_trsv_multi)
+constexpr int kSolve1MaxNHost = 8192;               // largest n the one-workgroup solve holds in LDS
+
+// Tunables (fnec_hip_set_tunable): the autotuner's knobs.
+static int g_solve1_max_n = kSolve1MaxNHost; // CG solves of n <= this run one workgroup per problem
+static int g_mix_first_chunk = kMixFirstChunk;
 constexpr double kMixTol = 1e-14;    // |h - (M^H M + lambda) x| <= tol |h|
 
 static size_t take(size_t &cur, size_t bytes)
@@ -590,8 +599,10 @@ static void lu_plan(std::vector<int> &ints, const std::vector<int> &ms, const st
                     std::vector<LuStep> &steps, std::vector<double> &upd_flops,
                     std::vector<double> &panel_flops, std::vector<double> &trsm_flops,
                     bool swap_left = false, int utile = kTile, std::vector<double> *upd_bytes = nullptr,
-                    double elem_bytes = 16.0)
+                    double elem_bytes = 16.0, int utile_n = 0)
 {
+    if (utile_n <= 0)
+        utile_n = utile;
     auto align4 = [&]() {
         while (ints.size() % 4)
             ints.push_back(0);
@@ -643,7 +654,7 @@ static void lu_plan(std::vector<int> &ints, const std::vector<int> &ms, const st
             if (nrows <= 0 || ncols <= 0)
                 continue;
             for (int y = 0; y < (nrows + utile - 1) / utile; ++y)
-                for (int z = 0; z < (ncols + utile - 1) / utile; ++z) {
+                for (int z = 0; z < (ncols + utile_n - 1) / utile_n; ++z) {
                     ints.push_back((int)b);
                     ints.push_back(y);
                     ints.push_back(z);
@@ -829,7 +840,7 @@ static void plan_build(Plan &pl, const std::vector<const HostProblem *> &hps, bo
         for (size_t b = 0; b < np; ++b)
             ms[b] = ncs[b] = hps[b]->n;
         lu_plan(pl.ints, ms, ncs, pl.steps32, pl.s32_upd_flops, pl.s32_panel_flops,
-                pl.s32_trsm_flops, true, kTile32, &pl.s32_upd_bytes, 8.0);
+                pl.s32_trsm_flops, true, kTile32, &pl.s32_upd_bytes, 8.0, kTile32N);
         // triangular-solve items (problem, vector, phase, block): phase 0
         // blocks in forward order, then phase 1 blocks in backward order,
         // interleaved across chains so that early tickets rarely wait
@@ -840,6 +851,8 @@ static void plan_build(Plan &pl, const std::vector<const HostProblem *> &hps, bo
             int maxnb = 0;
             for (size_t b = 0; b < np; ++b) {
                 const int nb = (hps[b]->n + 63) / 64;
+                if (!fsetup && hps[b]->n <= g_solve1_max_n)
+                    continue; // CG solves of this problem run in k_zsolve1
                 maxnb = std::max(maxnb, nb);
                 if (fsetup) {
                     for (int g = 0; g < (hps[b]->ncons + kMultiRhsHost - 1) / kMultiRhsHost; ++g)
@@ -863,6 +876,15 @@ static void plan_build(Plan &pl, const std::vector<const HostProblem *> &hps, bo
         pl.dinv_cnt = (pl.ints.size() - pl.dinv_off) / 2;
         trsv_list(true, pl.fsol_off, pl.fsol_cnt);
         trsv_list(false, pl.vsol_off, pl.vsol_cnt);
+        align4();
+        pl.one_off = pl.ints.size();
+        pl.one_maxn = 0;
+        for (size_t b = 0; b < np; ++b)
+            if (hps[b]->n <= g_solve1_max_n) {
+                pl.ints.push_back((int)b);
+                pl.one_maxn = std::max(pl.one_maxn, hps[b]->n);
+            }
+        pl.one_cnt = pl.ints.size() - pl.one_off;
         align4();
         pl.mvz_off = pl.ints.size();
         for (size_t b = 0; b < np; ++b)
@@ -889,7 +911,7 @@ static void plan_build(Plan &pl, const std::vector<const HostProblem *> &hps, bo
                 push2((int)b, y);
         pl.res_cnt = (pl.ints.size() - pl.res_off) / 2;
         pl.off_flags = take(cur, 4 * np + 16);
-        pl.zsolve_flops = pl.mv_flops = pl.zsolve_bytes = pl.mv_bytes = pl.fsol_flops = 0;
+        pl.zsolve_flops = pl.mv_flops = pl.zsolve_bytes = pl.mv_bytes = pl.fsol_flops = pl.fsol_bytes = 0;
         for (size_t b = 0; b < np; ++b) {
             const double n = hps[b]->n, r = hps[b]->r;
             pl.zsolve_flops += 8.0 * n * n;    // L and U (or U^H and L^H): n^2 CMAC
@@ -897,6 +919,7 @@ static void plan_build(Plan &pl, const std::vector<const HostProblem *> &hps, bo
             pl.mv_flops += 2.0 * 8.0 * n * r;  // M p and M^H t
             pl.mv_bytes += 2.0 * 16.0 * n * n; // FP64 Z read twice
             pl.fsol_flops += 8.0 * n * n * hps[b]->ncons;
+            pl.fsol_bytes += 8.0 * n * n * ((hps[b]->ncons + kMultiRhsHost - 1) / kMultiRhsHost);
         }
     } else if (with_solve) {
         align4();
@@ -1142,6 +1165,27 @@ static int mixed_launch(Plan &pl, char *base)
     const int4 *vsol = (const int4 *)(ints + pl.vsol_off);
     unsigned long long *tickets = (unsigned long long *)(base + pl.off_tickets);
     int ntrsv = 0; // launches of k_trsv so far: epoch ntrsv + 1, ticket counter ntrsv
+    // CG solves: one workgroup per problem with its vector in LDS for
+    // n <= kSolve1MaxNHost; the flag-chained multi-workgroup solve above that.
+    // The choice depends on the problem only, so batching never changes a result.
+    const size_t lds1 = 16 * (size_t)pl.one_maxn;
+    if (pl.one_cnt && lds1 > 64 * 1024)
+        HIPCHK(hipFuncSetAttribute((const void *)k_zsolve1, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)lds1));
+    auto zsolve = [&](int mode) {
+        if (pl.one_cnt)
+            hipLaunchKernelGGL(k_zsolve1, dim3((unsigned)pl.one_cnt), dim3(1024), lds1, st, dp,
+                               ints + pl.one_off, mode);
+        if (pl.vsol_cnt) {
+            if (mode == 0)
+                hipLaunchKernelGGL(k_trsv0, dim3((unsigned)pl.vsol_cnt), dim3(256), 0, st, dp, vsol,
+                                   ntrsv + 1, tickets + ntrsv);
+            else
+                hipLaunchKernelGGL(k_trsv1, dim3((unsigned)pl.vsol_cnt), dim3(256), 0, st, dp, vsol,
+                                   ntrsv + 1, tickets + ntrsv);
+            ++ntrsv;
+        }
+    };
     {
         const size_t count = (pl.flags_end - pl.flags_begin) / 8;
         hipLaunchKernelGGL(k_zero, dim3((unsigned)std::min<size_t>(1024, (count + 255) / 256)), dim3(256), 0,
@@ -1156,7 +1200,7 @@ static int mixed_launch(Plan &pl, char *base)
         hipLaunchKernelGGL(k_trsv_multi, dim3((unsigned)pl.fsol_cnt), dim3(256), 0, st, dp,
                            (const int4 *)(ints + pl.fsol_off), ntrsv + 1, tickets + ntrsv);
         ++ntrsv;
-        prof_end(PC_FSETUP, e, pl.fsol_flops, pl.fsol_flops);
+        prof_end(PC_FSETUP, e, pl.fsol_flops, pl.fsol_bytes);
     }
     hipLaunchKernelGGL(k_pre_setup, dim3(np), dim3(1024), 0, st, dp, all);
     hipLaunchKernelGGL(k_mv_zh, dim3((unsigned)pl.mvzh_cnt), dim3(256), 0, st, dp,
@@ -1171,20 +1215,16 @@ static int mixed_launch(Plan &pl, char *base)
     pl.it_mv.clear();
     int it = 0;
     while (it < kMixMaxIt) {
-        const int chunk = it == 0 ? kMixFirstChunk : kMixNextChunk;
+        const int chunk = it == 0 ? g_mix_first_chunk : kMixNextChunk;
         for (int c = 0; c < chunk && it < kMixMaxIt; ++c, ++it) {
             hipEvent_t e = prof_begin();
-            hipLaunchKernelGGL(k_trsv, dim3((unsigned)pl.vsol_cnt), dim3(256), 0, st, dp, vsol, 1,
-                               ntrsv + 1, tickets + ntrsv);
-            ++ntrsv;
+            zsolve(1);
             pl.it_zs1.push_back(prof_end(PC_ZSOLVE, e, pl.zsolve_flops, pl.zsolve_bytes));
             e = prof_begin();
             hipLaunchKernelGGL(k_prec_mid, dim3(np), dim3(1024), 0, st, dp, all);
             prof_end(PC_PCG, e, 0, 0);
             e = prof_begin();
-            hipLaunchKernelGGL(k_trsv, dim3((unsigned)pl.vsol_cnt), dim3(256), 0, st, dp, vsol, 0,
-                               ntrsv + 1, tickets + ntrsv);
-            ++ntrsv;
+            zsolve(0);
             pl.it_zs0.push_back(prof_end(PC_ZSOLVE, e, pl.zsolve_flops, pl.zsolve_bytes));
             e = prof_begin();
             hipLaunchKernelGGL(k_mix_dir, dim3(np), dim3(1024), 0, st, dp, all);
@@ -1672,6 +1712,31 @@ int mfma_probe32(const float *a, const float *b, float *out)
     HIPCHK(hipMemcpyAsync(out, base + 2048, 256 * 4, hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
     return FNEC_OK;
+}
+
+int set_tunable(const char *name, long value)
+{
+    const std::string k = name ? name : "";
+    if (k == "solve1_max_n" && value >= 0 && value <= kSolve1MaxNHost) {
+        g_solve1_max_n = (int)value;
+        return FNEC_OK;
+    }
+    if (k == "mix_first_chunk" && value >= 1 && value <= kMixMaxIt) {
+        g_mix_first_chunk = (int)value;
+        return FNEC_OK;
+    }
+    set_error("unknown tunable or value out of range");
+    return FNEC_INVALID_ARG;
+}
+
+long get_tunable(const char *name)
+{
+    const std::string k = name ? name : "";
+    if (k == "solve1_max_n")
+        return g_solve1_max_n;
+    if (k == "mix_first_chunk")
+        return g_mix_first_chunk;
+    return -1;
 }
 
 std::mutex &global_mutex() { return g_mu; }

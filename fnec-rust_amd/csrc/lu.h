@@ -21,7 +21,7 @@
 //   k_lu_trsm    interchanges + U12 = L11^{-1} A12 (one thread per column).
 //   k_lu_update  A22 -= L21 U12 on MFMA (16x16x4, four real products per
 //                complex product, 64x64 output tiles, K = jb staged in LDS).
-//   k_lu_update_f32  the FP32 variant: 128x128 tiles, accumulators seeded
+//   k_lu_update_f32  the FP32 variant: 128x64 tiles, accumulators seeded
 //                with C so each element is read and written once per step.
 #pragma once
 
@@ -529,37 +529,41 @@ __global__ __launch_bounds__(256) void k_lu_update(const LuDesc<T> *__restrict__
             }
 }
 
-// FP32 trailing update for the mixed path: A22 -= L21 U12 with 128 x 128
-// output tiles. Four waves in a 2 x 2 arrangement each own a 64 x 64 block =
-// 4 x 4 MFMA 16x16x4 tiles; the accumulators start from C (float4 loads in
+// FP32 trailing update for the mixed path: A22 -= L21 U12 with TM x TN
+// output tiles. Four waves in a 2 x 2 arrangement each own a TM/2 x TN/2
+// block of MFMA 16x16x4 tiles; the accumulators start from C (float4 loads in
 // the accumulator's own row layout: 4 consecutive rows per lane) and the A
 // operand enters negated, so the MFMA chain produces C - L21 U12 directly and
 // C is read and written once per panel step. Panels here are full (jb = 32):
 // the last, partial panel has no rows below it.
-constexpr int kTile32 = 128;
-constexpr int kLds32 = kTile32 + 4;
+constexpr int kTile32 = 128;   // rows per FP32 update tile
+constexpr int kTile32N = 64;   // columns per FP32 update tile
 
+template <int TM, int TN>
 __global__ __launch_bounds__(256) void k_lu_update_f32(const LuDesc<float> *__restrict__ descs,
                                                        const int4 *__restrict__ work, int p)
 {
     typedef Mfma<float> MF;
+    constexpr int WM = TM / 2, WN = TN / 2;       // per-wave block (2 x 2 waves)
+    constexpr int AM = WM / 16, AN = WN / 16;     // MFMA tiles per wave
+    constexpr int LA = TM + 4, LB = TN + 4;
     const int4 wk = work[blockIdx.x];
     const LuDesc<float> &D = descs[wk.x];
     const int s = D.m, nc = D.ncols;
     if (D.info[0] != 0 || p + kPanel >= s)
         return;
     const int lda = D.lda;
-    const int r0 = p + kPanel + wk.y * kTile32, c0 = p + kPanel + wk.z * kTile32;
-    __shared__ float sa_r[kPanel][kLds32], sa_i[kPanel][kLds32];
-    __shared__ float sb_r[kPanel][kLds32], sb_i[kPanel][kLds32];
+    const int r0 = p + kPanel + wk.y * TM, c0 = p + kPanel + wk.z * TN;
+    __shared__ float sa_r[kPanel][LA], sa_i[kPanel][LA];
+    __shared__ float sb_r[kPanel][LB], sb_i[kPanel][LB];
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const int wr = wave >> 1, wc = wave & 1;
     const float *ar = D.ar, *ai = D.ai;
-    // A: column p + k, rows r0 .. r0 + 127 (contiguous): 32 float4 per column
+    // A: column p + k, rows r0 .. r0 + TM - 1 (contiguous): TM / 4 float4 per column
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
+    for (int i = 0; i < (kPanel * TM / 4) / 256; ++i) {
         const int e = t + 256 * i;
-        const int k = e >> 5, r4 = (e & 31) * 4;
+        const int k = e / (TM / 4), r4 = (e % (TM / 4)) * 4;
         const int gr = r0 + r4;
         float4 vr = make_float4(0.f, 0.f, 0.f, 0.f), vi = vr;
         const size_t o = (size_t)(p + k) * lda + gr;
@@ -580,7 +584,7 @@ __global__ __launch_bounds__(256) void k_lu_update_f32(const LuDesc<float> *__re
     }
     // B: column c0 + c, rows p .. p + 31 (contiguous): 8 float4 per column
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
+    for (int i = 0; i < (kPanel * TN / 4) / 256; ++i) {
         const int e = t + 256 * i;
         const int c = e >> 3, k4 = (e & 7) * 4;
         const int gc = c0 + c;
@@ -600,14 +604,14 @@ __global__ __launch_bounds__(256) void k_lu_update_f32(const LuDesc<float> *__re
         sb_i[k4 + 3][c] = vi.w;
     }
     // accumulators = C
-    typename MF::acc accr[4][4], acci[4][4];
+    typename MF::acc accr[AM][AN], acci[AM][AN];
     const int lrow = lane & 15, lk = lane >> 4;
 #pragma unroll
-    for (int a = 0; a < 4; ++a)
+    for (int a = 0; a < AM; ++a)
 #pragma unroll
-        for (int b = 0; b < 4; ++b) {
-            const int gi = r0 + wr * 64 + a * 16 + 4 * lk; // rows gi .. gi + 3
-            const int gj = c0 + wc * 64 + b * 16 + lrow;
+        for (int b = 0; b < AN; ++b) {
+            const int gi = r0 + wr * WM + a * 16 + 4 * lk; // rows gi .. gi + 3
+            const int gj = c0 + wc * WN + b * 16 + lrow;
             f4 cr = f4{0.f, 0.f, 0.f, 0.f}, ci = cr;
             if (gj < nc && gi < s) {
                 const size_t o = (size_t)gj * lda + gi;
@@ -628,18 +632,21 @@ __global__ __launch_bounds__(256) void k_lu_update_f32(const LuDesc<float> *__re
     __syncthreads();
 #pragma unroll
     for (int kk = 0; kk < kPanel; kk += 4) {
-        float xr[4], xi[4], yr[4], yi[4];
+        float xr[AM], xi[AM], yr[AN], yi[AN];
 #pragma unroll
-        for (int a = 0; a < 4; ++a) {
-            xr[a] = sa_r[kk + lk][wr * 64 + a * 16 + lrow];
-            xi[a] = sa_i[kk + lk][wr * 64 + a * 16 + lrow];
-            yr[a] = sb_r[kk + lk][wc * 64 + a * 16 + lrow];
-            yi[a] = sb_i[kk + lk][wc * 64 + a * 16 + lrow];
+        for (int a = 0; a < AM; ++a) {
+            xr[a] = sa_r[kk + lk][wr * WM + a * 16 + lrow];
+            xi[a] = sa_i[kk + lk][wr * WM + a * 16 + lrow];
         }
 #pragma unroll
-        for (int a = 0; a < 4; ++a)
+        for (int b = 0; b < AN; ++b) {
+            yr[b] = sb_r[kk + lk][wc * WN + b * 16 + lrow];
+            yi[b] = sb_i[kk + lk][wc * WN + b * 16 + lrow];
+        }
 #pragma unroll
-            for (int b = 0; b < 4; ++b) {
+        for (int a = 0; a < AM; ++a)
+#pragma unroll
+            for (int b = 0; b < AN; ++b) {
                 // C - (xr + i xi)(yr + i yi)
                 accr[a][b] = MF::fma(-xr[a], yr[b], accr[a][b]);
                 accr[a][b] = MF::fma(xi[a], yi[b], accr[a][b]);
@@ -648,11 +655,11 @@ __global__ __launch_bounds__(256) void k_lu_update_f32(const LuDesc<float> *__re
             }
     }
 #pragma unroll
-    for (int a = 0; a < 4; ++a)
+    for (int a = 0; a < AM; ++a)
 #pragma unroll
-        for (int b = 0; b < 4; ++b) {
-            const int gi = r0 + wr * 64 + a * 16 + 4 * lk;
-            const int gj = c0 + wc * 64 + b * 16 + lrow;
+        for (int b = 0; b < AN; ++b) {
+            const int gi = r0 + wr * WM + a * 16 + 4 * lk;
+            const int gj = c0 + wc * WN + b * 16 + lrow;
             if (gj >= nc || gi >= s)
                 continue;
             const size_t o = (size_t)gj * lda + gi;
@@ -670,8 +677,8 @@ __global__ __launch_bounds__(256) void k_lu_update_f32(const LuDesc<float> *__re
 
 inline void lu_launch_update_f32(const LuDesc<float> *d, const LuStep &s, const int *ints, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_lu_update_f32, dim3((unsigned)s.upd_cnt), dim3(256), 0, st, d,
-                       (const int4 *)(ints + s.upd_off), s.p);
+    hipLaunchKernelGGL((k_lu_update_f32<kTile32, kTile32N>), dim3((unsigned)s.upd_cnt), dim3(256), 0,
+                       st, d, (const int4 *)(ints + s.upd_off), s.p);
 }
 
 // Host side: one panel step of the batched factorisation on stream st.

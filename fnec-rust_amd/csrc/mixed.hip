@@ -16,8 +16,9 @@
 // residual h - (M^H M + lambda) x and every vector update are FP64.
 //
 // Kernels:
-//   k_trsv       Z v = b or Z^H v = b with the FP32 factors, FP64 vectors;
+//   k_trsv0/1    Z v = b or Z^H v = b with the FP32 factors, FP64 vectors;
 //                one workgroup per 64-row block, blocks chained by flags.
+//   k_zsolve1    the same solves with one workgroup per problem (batches).
 //   k_trsv_multi Z^H solves of the constraint rows, 16 right-hand sides per chain.
 //   k_perm       the row permutation of the factors.
 //   k_diag_inv   inverses of the 64 x 64 diagonal blocks of L and U.
@@ -168,9 +169,9 @@ __device__ bool wait_flag(const Prob &P, const int *flag, int epoch, int *s_ok)
     return *s_ok != 0;
 }
 
-__global__ __launch_bounds__(256) void k_trsv(const Prob *__restrict__ probs,
-                                              const int4 *__restrict__ work, int mode, int epoch,
-                                              unsigned long long *__restrict__ ticket)
+template <int mode>
+__device__ __forceinline__ void trsv_body(const Prob *__restrict__ probs, const int4 *__restrict__ work,
+                                          int epoch, unsigned long long *__restrict__ ticket)
 {
     __shared__ int s_item, s_ok;
     __shared__ float dr[kTb][kTb + 1], di[kTb][kTb + 1];
@@ -313,6 +314,19 @@ __global__ __launch_bounds__(256) void k_trsv(const Prob *__restrict__ probs,
             __hip_atomic_store(trsv_flag(P, vecid, phase, blk), epoch, __ATOMIC_RELEASE,
                                __HIP_MEMORY_SCOPE_AGENT);
     }
+}
+
+// One kernel per mode, so each instance keeps only its own accumulators.
+__global__ __launch_bounds__(256) void k_trsv0(const Prob *__restrict__ probs, const int4 *__restrict__ work,
+                                               int epoch, unsigned long long *__restrict__ ticket)
+{
+    trsv_body<0>(probs, work, epoch, ticket);
+}
+
+__global__ __launch_bounds__(256) void k_trsv1(const Prob *__restrict__ probs, const int4 *__restrict__ work,
+                                               int epoch, unsigned long long *__restrict__ ticket)
+{
+    trsv_body<1>(probs, work, epoch, ticket);
 }
 
 // Z^H solves of up to kMultiRhs right-hand sides at once (the constraint rows
@@ -517,6 +531,202 @@ __global__ __launch_bounds__(64) void k_diag_inv(const Prob *__restrict__ probs,
             dst[2 * e + 1] = xi[r][k];
         }
         __syncthreads();
+    }
+}
+
+// Single-workgroup Z or Z^H solve of one problem (the batched form: one
+// workgroup per problem, no inter-workgroup waiting). The whole vector lives
+// in LDS (16 n bytes; the host sends problems with n <= 8192); 64-row blocks use the diagonal-block
+// inverses. Z v = b: L forward and U backward, right-looking (column-major
+// factors are read down columns, coalesced); Z^H v = b: U^H forward and L^H
+// backward, left-looking (a row of U^H or L^H is a column of U or L).
+// Work item: plist[blockIdx.x] = problem.
+constexpr int kSolve1Threads = 1024;
+
+__device__ __forceinline__ double2 lds2(const double2 *v, int i) { return v[i]; }
+
+__global__ __launch_bounds__(kSolve1Threads) void k_zsolve1(const Prob *__restrict__ probs,
+                                                            const int *__restrict__ plist, int mode)
+{
+    extern __shared__ double2 vec[];           // n entries
+    __shared__ double2 part[16][kTb];
+    __shared__ double2 xb[kTb];
+    const Prob &P = probs[plist[blockIdx.x]];
+    if (P.info[0] != 0 || P.pstate[0] != 0)
+        return;
+    const int n = P.n, ld = P.ldz, nb = (n + kTb - 1) / kTb;
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6; // 16 waves
+    const float *Fr = P.fr, *Fi = P.fi;
+    double2 *io = (double2 *)P.vv;
+    for (int i = t; i < n; i += kSolve1Threads)
+        vec[i] = mode == 0 ? io[P.perm[i]] : io[i];
+    __syncthreads();
+    // x_blk = op(inverse) * xb, op = identity (mode 0) or conjugate transpose (mode 1)
+    auto apply_inv = [&](int blk, int which) {
+        const float *src = P.dinv + ((size_t)blk * 2 + which) * 2 * kTb * kTb;
+        const int r = t & 63, g = t >> 6; // 16 column groups of 4
+        double2 acc = make_double2(0.0, 0.0);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int c = 4 * g + q;
+            const size_t e = mode == 0 ? (size_t)c * kTb + r : (size_t)r * kTb + c; // [r][c] or [c][r]
+            const double ar = src[2 * e], ai = mode == 0 ? src[2 * e + 1] : -src[2 * e + 1];
+            const double2 x = xb[c];
+            acc.x += ar * x.x - ai * x.y;
+            acc.y += ar * x.y + ai * x.x;
+        }
+        part[g][r] = acc;
+        __syncthreads();
+        if (t < kTb) {
+            double2 sum = make_double2(0.0, 0.0);
+            for (int q = 0; q < 16; ++q)
+                sum = cadd(sum, part[q][t]);
+            const int i = blk * kTb + t;
+            if (i < n)
+                vec[i] = sum;
+        }
+        __syncthreads();
+    };
+    // vec[r] -= F[r, j0 .. j0 + jn) xb for rows [ra, rb): four consecutive rows
+    // per thread (float4 loads down each column; ld and the row offsets used
+    // here are multiples of 4 except at the matrix end, handled one by one)
+    auto rows_update = [&](int j0, int jn, int ra, int rb) {
+        const int a4 = (ra + 3) & ~3; // first 4-aligned row
+        for (int r = ra + t; r < min(a4, rb); r += kSolve1Threads) {
+            double sr = 0.0, si = 0.0;
+            for (int c = 0; c < jn; ++c) {
+                const size_t o = (size_t)(j0 + c) * ld + r;
+                const double2 x = xb[c];
+                sr += (double)Fr[o] * x.x - (double)Fi[o] * x.y;
+                si += (double)Fr[o] * x.y + (double)Fi[o] * x.x;
+            }
+            vec[r] = make_double2(vec[r].x - sr, vec[r].y - si);
+        }
+        for (int r4 = a4 + 4 * t; r4 < rb; r4 += 4 * kSolve1Threads) {
+            double sr[4] = {0.0, 0.0, 0.0, 0.0}, si[4] = {0.0, 0.0, 0.0, 0.0};
+            if (r4 + 3 < rb) {
+#pragma unroll 4
+                for (int c = 0; c < jn; ++c) {
+                    const size_t o = (size_t)(j0 + c) * ld + r4;
+                    const float4 fr = *(const float4 *)(Fr + o), fi = *(const float4 *)(Fi + o);
+                    const double2 x = xb[c];
+                    sr[0] += (double)fr.x * x.x - (double)fi.x * x.y;
+                    si[0] += (double)fr.x * x.y + (double)fi.x * x.x;
+                    sr[1] += (double)fr.y * x.x - (double)fi.y * x.y;
+                    si[1] += (double)fr.y * x.y + (double)fi.y * x.x;
+                    sr[2] += (double)fr.z * x.x - (double)fi.z * x.y;
+                    si[2] += (double)fr.z * x.y + (double)fi.z * x.x;
+                    sr[3] += (double)fr.w * x.x - (double)fi.w * x.y;
+                    si[3] += (double)fr.w * x.y + (double)fi.w * x.x;
+                }
+                for (int q = 0; q < 4; ++q)
+                    vec[r4 + q] = make_double2(vec[r4 + q].x - sr[q], vec[r4 + q].y - si[q]);
+            } else {
+                for (int r = r4; r < rb; ++r) {
+                    double ur = 0.0, ui = 0.0;
+                    for (int c = 0; c < jn; ++c) {
+                        const size_t o = (size_t)(j0 + c) * ld + r;
+                        const double2 x = xb[c];
+                        ur += (double)Fr[o] * x.x - (double)Fi[o] * x.y;
+                        ui += (double)Fr[o] * x.y + (double)Fi[o] * x.x;
+                    }
+                    vec[r] = make_double2(vec[r].x - ur, vec[r].y - ui);
+                }
+            }
+        }
+    };
+    // rows of U^H (mode 1 phase 0) or L^H (phase 1) against solved entries
+    // [ca, cb): a row is a contiguous column of the factor; four entries per lane
+    auto row_dot = [&](size_t col, int ca, int cb, double &sr, double &si) {
+        sr = 0.0;
+        si = 0.0;
+        const int a4 = min((ca + 3) & ~3, cb);
+        for (int c = ca + lane; c < a4; c += 64) {
+            const double ar = Fr[col + c], ai = Fi[col + c];
+            const double2 x = vec[c];
+            sr += ar * x.x + ai * x.y;
+            si += ar * x.y - ai * x.x;
+        }
+        const int b4 = a4 + ((cb - a4) & ~3);
+        for (int c = a4 + 4 * lane; c < b4; c += 256) {
+            const float4 fr = *(const float4 *)(Fr + col + c), fi = *(const float4 *)(Fi + col + c);
+            const double2 x0 = vec[c], x1 = vec[c + 1], x2 = vec[c + 2], x3 = vec[c + 3];
+            sr += (double)fr.x * x0.x + (double)fi.x * x0.y + (double)fr.y * x1.x + (double)fi.y * x1.y +
+                  (double)fr.z * x2.x + (double)fi.z * x2.y + (double)fr.w * x3.x + (double)fi.w * x3.y;
+            si += (double)fr.x * x0.y - (double)fi.x * x0.x + (double)fr.y * x1.y - (double)fi.y * x1.x +
+                  (double)fr.z * x2.y - (double)fi.z * x2.x + (double)fr.w * x3.y - (double)fi.w * x3.x;
+        }
+        for (int c = b4 + lane; c < cb; c += 64) {
+            const double ar = Fr[col + c], ai = Fi[col + c];
+            const double2 x = vec[c];
+            sr += ar * x.x + ai * x.y;
+            si += ar * x.y - ai * x.x;
+        }
+    };
+    if (mode == 0) {
+        // forward L (unit lower): solve block j, then update every row below it
+        for (int j = 0; j < nb; ++j) {
+            const int j0 = j * kTb, jn = min(kTb, n - j0);
+            if (t < kTb)
+                xb[t] = t < jn ? vec[j0 + t] : make_double2(0.0, 0.0);
+            __syncthreads();
+            apply_inv(j, 0);
+            if (t < kTb)
+                xb[t] = t < jn ? vec[j0 + t] : make_double2(0.0, 0.0);
+            __syncthreads();
+            rows_update(j0, jn, j0 + jn, n);
+            __syncthreads();
+        }
+        // backward U: solve block j (from the bottom), then update every row above it
+        for (int j = nb - 1; j >= 0; --j) {
+            const int j0 = j * kTb, jn = min(kTb, n - j0);
+            if (t < kTb)
+                xb[t] = t < jn ? vec[j0 + t] : make_double2(0.0, 0.0);
+            __syncthreads();
+            apply_inv(j, 1);
+            if (t < kTb)
+                xb[t] = t < jn ? vec[j0 + t] : make_double2(0.0, 0.0);
+            __syncthreads();
+            rows_update(j0, jn, 0, j0);
+            __syncthreads();
+        }
+        for (int i = t; i < n; i += kSolve1Threads)
+            io[i] = vec[i];
+    } else {
+        // forward U^H: rows of block i against every solved entry above it
+        for (int i = 0; i < nb; ++i) {
+            const int i0 = i * kTb, in = min(kTb, n - i0);
+            for (int rr = wave; rr < in; rr += 16) {
+                double sr, si; // conj(U[c][r]) x_c over the solved entries c < i0
+                row_dot((size_t)(i0 + rr) * ld, 0, i0, sr, si);
+                sr = wave_sum1(sr);
+                si = wave_sum1(si);
+                if (lane == 0)
+                    xb[rr] = make_double2(vec[i0 + rr].x - sr, vec[i0 + rr].y - si);
+            }
+            if (t < kTb && t >= in)
+                xb[t] = make_double2(0.0, 0.0);
+            __syncthreads();
+            apply_inv(i, 1); // (U_ii^{-1})^H
+        }
+        // backward L^H: rows of block i against every solved entry below it
+        for (int i = nb - 1; i >= 0; --i) {
+            const int i0 = i * kTb, in = min(kTb, n - i0);
+            for (int rr = wave; rr < in; rr += 16) {
+                double sr, si; // conj(L[c][r]) x_c over the solved entries below the block
+                row_dot((size_t)(i0 + rr) * ld, i0 + in, n, sr, si);
+                sr = wave_sum1(sr);
+                si = wave_sum1(si);
+                if (lane == 0)
+                    xb[rr] = make_double2(vec[i0 + rr].x - sr, vec[i0 + rr].y - si);
+            }
+            if (t < kTb && t >= in)
+                xb[t] = make_double2(0.0, 0.0);
+            __syncthreads();
+            apply_inv(i, 0); // (L_ii^{-1})^H
+        }
+        for (int i = t; i < n; i += kSolve1Threads)
+            io[P.perm[i]] = vec[i];
     }
 }
 

@@ -214,6 +214,18 @@ int fnec_hip_debug_mfma_probe(const double *a16x4, const double *b4x16, double *
 /* The same probe for the FP32 MFMA (v_mfma_f32_16x16x4f32) used by the mixed path. */
 int fnec_hip_debug_mfma_probe_f32(const float *a16x4, const float *b4x16, float *lanes_out);
 
+/* ---- tunables (no counterpart in the reference) ----
+ * Knobs of the batched engine, set by the autotuner (fnec-rust_amd autotune())
+ * or by hand; they change speed only, never results of a given problem.
+ *   "solve1_max_n"    CG solves of problems with n <= value run one workgroup
+ *                     per problem (vector in LDS); larger n use the
+ *                     flag-chained multi-workgroup solve. 0 .. 8192.
+ *   "mix_first_chunk" CG iterations launched before the first convergence
+ *                     read-back. 1 .. 12.
+ * set: FNEC_OK or FNEC_INVALID_ARG; get: the value, -1 for an unknown name. */
+int fnec_hip_set_tunable(const char *name, long value);
+long fnec_hip_get_tunable(const char *name);
+
 /* Per-kernel timing with HIP events on the solve stream (off by default).
  * Kernel classes: "fill", "gram", "lu_panel", "lu_trsm", "lu_update",
  * "backsub", "rp", and on the mixed path "zsolve" (Z / Z^H solves with the
