@@ -390,7 +390,7 @@ def profile(enable: bool | None = None, reset: bool = False) -> dict:
         L.fnec_hip_profile_enable(int(enable))
     out = {}
     for k in ["fill", "gram", "lu_panel", "lu_trsm", "lu_update", "backsub", "rp", "zsolve", "matvec",
-              "pcg_ctl", "fsetup"]:
+              "pcg_ctl", "fsetup", "lu_update_narrow"]:
         buf = np.zeros(4)
         _check(L.fnec_hip_profile_read(k.encode(), _p(buf)))
         out[k] = tuple(float(v) for v in buf)

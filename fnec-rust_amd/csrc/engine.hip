@@ -101,9 +101,10 @@ struct ProfPending {
     double flops, bytes;
 };
 enum { PC_FILL, PC_GRAM, PC_LU_PANEL, PC_LU_TRSM, PC_LU_UPDATE, PC_BACKSUB, PC_RP, PC_ZSOLVE,
-       PC_MV, PC_PCG, PC_FSETUP, PC_N };
+       PC_MV, PC_PCG, PC_FSETUP, PC_LU_UPDN, PC_N };
 static const char *kProfNames[PC_N] = {"fill", "gram", "lu_panel", "lu_trsm", "lu_update",
-                                       "backsub", "rp", "zsolve", "matvec", "pcg_ctl", "fsetup"};
+                                       "backsub", "rp", "zsolve", "matvec", "pcg_ctl", "fsetup",
+                                       "lu_update_narrow"};
 static bool g_prof = false;
 static ProfClass g_prof_acc[PC_N];
 static std::vector<ProfPending> g_prof_pending;
@@ -532,8 +533,16 @@ struct Plan {
         int nchunk;
     };
     std::vector<MixOff> mix;
-    std::vector<LuStep> steps32;
-    std::vector<double> s32_upd_flops, s32_panel_flops, s32_trsm_flops, s32_upd_bytes;
+    struct Outer {                 // one 64-column step of the two-level FP32 LU
+        int P;
+        LuStep a, b;               // inner panels at P and P + 32
+        bool has_b, has_outer;
+        size_t ua_off, ua_cnt;     // narrow update of columns [P + 32, P + 64)
+        size_t ot_off, ot_cnt;     // outer interchanges + L11 solve of columns [P + 64, n)
+        size_t ou_off, ou_cnt;     // outer update, K = 64
+        double f_panel, f_trsm, f_ua, b_ua, f_ou, b_ou;
+    };
+    std::vector<Outer> outer32;
     std::vector<LuDesc<float>> lud32;
     size_t off_lud32 = 0;
     size_t fsol_off = 0, fsol_cnt = 0, vsol_off = 0, vsol_cnt = 0, mvz_off = 0, mvz_cnt = 0;
@@ -706,6 +715,149 @@ static void lu_plan(std::vector<int> &ints, const std::vector<int> &ms, const st
     }
 }
 
+// Two-level plan of the FP32 LU (mixed path): 64-column outer steps made of
+// two 32-column panels, so that the trailing update runs with K = 64 and reads
+// and writes the trailing matrix once per 64 columns.
+static void lu_plan_outer(std::vector<int> &ints, const std::vector<int> &ms, std::vector<Plan::Outer> &out)
+{
+    auto align4 = [&]() {
+        while (ints.size() % 4)
+            ints.push_back(0);
+    };
+    auto push2 = [&](int a, int b) {
+        ints.push_back(a);
+        ints.push_back(b);
+    };
+    auto push4 = [&](int a, int b, int c, int d) {
+        ints.push_back(a);
+        ints.push_back(b);
+        ints.push_back(c);
+        ints.push_back(d);
+    };
+    const size_t np = ms.size();
+    int smax = 0;
+    for (int m : ms)
+        smax = std::max(smax, m);
+    out.clear();
+    auto panel_lists = [&](LuStep &st, int p) {
+        align4();
+        st.panel_off = ints.size();
+        for (size_t b = 0; b < np; ++b)
+            if (ms[b] > p)
+                ints.push_back((int)b);
+        st.panel_cnt = ints.size() - st.panel_off;
+        align4();
+        st.rows_off = ints.size();
+        for (size_t b = 0; b < np; ++b) {
+            const int s = ms[b];
+            if (s <= p)
+                continue;
+            const int jb = std::min(kPanel, s - p);
+            for (int y = 0; y < (s - p - jb + 255) / 256; ++y)
+                push2((int)b, y);
+        }
+        st.rows_cnt = (ints.size() - st.rows_off) / 2;
+        align4();
+        st.commit_off = ints.size();
+        for (size_t b = 0; b < np; ++b) {
+            const int s = ms[b];
+            if (s <= p)
+                continue;
+            for (int y = 0; y < (s - p + 255) / 256; ++y)
+                push2((int)b, y);
+        }
+        st.commit_cnt = (ints.size() - st.commit_off) / 2;
+    };
+    for (int P = 0; P < smax; P += 2 * kPanel) {
+        Plan::Outer o{};
+        o.P = P;
+        o.a.p = P;
+        o.b.p = P + kPanel;
+        o.has_b = false;
+        o.has_outer = false;
+        // inner panel a
+        panel_lists(o.a, P);
+        align4();
+        o.a.trsm_off = ints.size();
+        for (size_t b = 0; b < np; ++b) {
+            const int s = ms[b];
+            if (s <= P)
+                continue;
+            if (s > P + kPanel)
+                push2((int)b, 0); // columns [P + 32, min(P + 64, s))
+            for (int y = 0; y < (P + 63) / 64; ++y)
+                push2((int)b, -1 - y); // interchanges left of the panel
+        }
+        o.a.trsm_cnt = (ints.size() - o.a.trsm_off) / 2;
+        align4();
+        o.ua_off = ints.size();
+        for (size_t b = 0; b < np; ++b) {
+            const int s = ms[b];
+            if (s <= P + kPanel)
+                continue;
+            for (int y = 0; y < (s - P - kPanel + kTile32 - 1) / kTile32; ++y)
+                push4((int)b, y, 0, 0);
+        }
+        o.ua_cnt = (ints.size() - o.ua_off) / 4;
+        // inner panel b
+        o.has_b = smax > P + kPanel;
+        if (o.has_b) {
+            panel_lists(o.b, P + kPanel);
+            align4();
+            o.b.trsm_off = ints.size();
+            for (size_t b = 0; b < np; ++b) {
+                const int s = ms[b];
+                if (s <= P + kPanel)
+                    continue;
+                for (int y = 0; y < (P + kPanel + 63) / 64; ++y)
+                    push2((int)b, -1 - y);
+            }
+            o.b.trsm_cnt = (ints.size() - o.b.trsm_off) / 2;
+        }
+        // outer solve and update
+        o.has_outer = smax > P + 2 * kPanel;
+        if (o.has_outer) {
+            align4();
+            o.ot_off = ints.size();
+            for (size_t b = 0; b < np; ++b) {
+                const int s = ms[b];
+                for (int y = 0; y < (s - P - 2 * kPanel + 63) / 64 && s > P + 2 * kPanel; ++y)
+                    push2((int)b, y);
+            }
+            o.ot_cnt = (ints.size() - o.ot_off) / 2;
+            align4();
+            o.ou_off = ints.size();
+            for (size_t b = 0; b < np; ++b) {
+                const int s = ms[b];
+                if (s <= P + 2 * kPanel)
+                    continue;
+                const int rem = s - P - 2 * kPanel;
+                for (int y = 0; y < (rem + kTile32 - 1) / kTile32; ++y)
+                    for (int z = 0; z < (rem + kTile32N - 1) / kTile32N; ++z)
+                        push4((int)b, y, z, 0);
+            }
+            o.ou_cnt = (ints.size() - o.ou_off) / 4;
+        }
+        // algorithmic work of the launches (8 flops per complex multiply-add;
+        // 8 bytes per FP32 complex element)
+        for (size_t b = 0; b < np; ++b) {
+            const double s = ms[b];
+            if (s <= P)
+                continue;
+            const double ja = std::min<double>(kPanel, s - P), jbb = std::max(0.0, std::min<double>(kPanel, s - P - kPanel));
+            o.f_panel += 8.0 * (s - P) * ja * ja / 2.0 + 8.0 * (s - P - ja) * jbb * jbb / 2.0;
+            const double rows_a = s - P - ja, cols_a = jbb;
+            o.f_ua += 8.0 * rows_a * cols_a * ja;
+            o.b_ua += 8.0 * (2.0 * rows_a * cols_a + rows_a * ja + ja * cols_a);
+            const double rem = std::max(0.0, s - P - 2.0 * kPanel);
+            o.f_trsm += 8.0 * cols_a * ja * ja / 2.0 + 8.0 * rem * 64.0 * 64.0 / 2.0;
+            o.f_ou += 8.0 * rem * rem * 64.0;
+            o.b_ou += 8.0 * (2.0 * rem * rem + 2.0 * rem * 64.0);
+        }
+        out.push_back(o);
+    }
+}
+
 static void plan_build(Plan &pl, const std::vector<const HostProblem *> &hps, bool with_solve)
 {
     const size_t np = hps.size();
@@ -836,11 +988,10 @@ static void plan_build(Plan &pl, const std::vector<const HostProblem *> &hps, bo
         for (size_t b = 0; b < np; ++b)
             pl.ints.push_back((int)b);
         pl.all_cnt = np;
-        std::vector<int> ms(np), ncs(np);
+        std::vector<int> ms(np);
         for (size_t b = 0; b < np; ++b)
-            ms[b] = ncs[b] = hps[b]->n;
-        lu_plan(pl.ints, ms, ncs, pl.steps32, pl.s32_upd_flops, pl.s32_panel_flops,
-                pl.s32_trsm_flops, true, kTile32, &pl.s32_upd_bytes, 8.0, kTile32N);
+            ms[b] = hps[b]->n;
+        lu_plan_outer(pl.ints, ms, pl.outer32);
         // triangular-solve items (problem, vector, phase, block): phase 0
         // blocks in forward order, then phase 1 blocks in backward order,
         // interleaved across chains so that early tickets rarely wait
@@ -1142,22 +1293,40 @@ static int mixed_launch(Plan &pl, char *base)
     const unsigned np = (unsigned)pl.all_cnt;
     HIPCHK(hipEventRecord(g_ctx.ev[2], st));
     const LuDesc<float> *lud = (const LuDesc<float> *)(base + pl.off_lud32);
-    for (size_t si = 0; si < pl.steps32.size(); ++si) {
-        const LuStep &s = pl.steps32[si];
-        if (s.panel_cnt) {
-            hipEvent_t e = prof_begin();
-            lu_launch_panel(lud, s, ints, st);
-            prof_end(PC_LU_PANEL, e, pl.s32_panel_flops[si], 0);
+    for (const Plan::Outer &o : pl.outer32) {
+        hipEvent_t e = prof_begin();
+        lu_launch_panel(lud, o.a, ints, st);
+        if (o.has_b)
+            prof_end(PC_LU_PANEL, e, 0, 0);
+        else
+            prof_end(PC_LU_PANEL, e, o.f_panel, 0);
+        e = prof_begin();
+        lu_launch_trsm_f32(lud, o.a.trsm_off, o.a.trsm_cnt, ints, o.P, o.P + 2 * kPanel, st);
+        prof_end(PC_LU_TRSM, e, 0, 0);
+        if (o.ua_cnt) {
+            e = prof_begin();
+            lu_launch_update_narrow_f32(lud, o.ua_off, o.ua_cnt, ints, o.P, st);
+            prof_end(PC_LU_UPDN, e, o.f_ua, o.b_ua);
         }
-        if (s.trsm_cnt) {
-            hipEvent_t e = prof_begin();
-            lu_launch_trsm(lud, s, ints, st);
-            prof_end(PC_LU_TRSM, e, pl.s32_trsm_flops[si], 0);
+        if (o.has_b) {
+            e = prof_begin();
+            lu_launch_panel(lud, o.b, ints, st);
+            prof_end(PC_LU_PANEL, e, o.f_panel, 0);
+            e = prof_begin();
+            lu_launch_trsm_f32(lud, o.b.trsm_off, o.b.trsm_cnt, ints, o.P + kPanel, 0, st);
+            prof_end(PC_LU_TRSM, e, 0, 0);
         }
-        if (s.upd_cnt) {
-            hipEvent_t e = prof_begin();
-            lu_launch_update_f32(lud, s, ints, st);
-            prof_end(PC_LU_UPDATE, e, pl.s32_upd_flops[si], pl.s32_upd_bytes[si]);
+        if (o.has_outer) {
+            e = prof_begin();
+            if (o.ot_cnt)
+                lu_launch_trsm64_f32(lud, o.ot_off, o.ot_cnt, ints, o.P, st);
+            prof_end(PC_LU_TRSM, e, o.f_trsm, 0);
+            if (o.ou_cnt) {
+                e = prof_begin();
+                lu_launch_update_f32(lud, o.ou_off, o.ou_cnt, ints, o.P, 2, o.P + 2 * kPanel,
+                                     o.P + 2 * kPanel, 0x7fffffff, st);
+                prof_end(PC_LU_UPDATE, e, o.f_ou, o.b_ou);
+            }
         }
     }
     HIPCHK(hipGetLastError());

@@ -22,7 +22,13 @@
 //   k_lu_update  A22 -= L21 U12 on MFMA (16x16x4, four real products per
 //                complex product, 64x64 output tiles, K = jb staged in LDS).
 //   k_lu_update_f32  the FP32 variant: 128x64 tiles, accumulators seeded
-//                with C so each element is read and written once per step.
+//                with C, K = 32 nk columns of L21 per launch.
+//   k_lu_trsm64_f32  FP32 outer step: interchanges of two 32-column panels
+//                and the 64x64 L11 solve for the columns right of both.
+// The FP32 factorisation (mixed path) is two-level: each 64-column step
+// factors two 32-column panels (the second after a narrow K = 32 update of
+// its columns only), then updates the trailing matrix once with K = 64, so
+// the trailing matrix is read and written once per 64 columns.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -360,7 +366,7 @@ __global__ __launch_bounds__(256) void k_panel_commit(const LuDesc<T> *__restric
 // left of the panel (requested by lu_plan when swap_left is set).
 template <typename T>
 __global__ __launch_bounds__(64) void k_lu_trsm(const LuDesc<T> *__restrict__ descs,
-                                                const int2 *__restrict__ work, int p)
+                                                const int2 *__restrict__ work, int p, int cend)
 {
     const int2 wk = work[blockIdx.x];
     const LuDesc<T> &D = descs[wk.x];
@@ -401,7 +407,7 @@ __global__ __launch_bounds__(64) void k_lu_trsm(const LuDesc<T> *__restrict__ de
         return;
     }
     const int c = p + jb + wk.y * 64 + t;
-    if (c >= D.ncols)
+    if (c >= D.ncols || c >= cend)
         return;
     T *cr = D.ar + (size_t)c * lda, *ci = D.ai + (size_t)c * lda;
     if (any_swap)
@@ -534,14 +540,17 @@ __global__ __launch_bounds__(256) void k_lu_update(const LuDesc<T> *__restrict__
 // block of MFMA 16x16x4 tiles; the accumulators start from C (float4 loads in
 // the accumulator's own row layout: 4 consecutive rows per lane) and the A
 // operand enters negated, so the MFMA chain produces C - L21 U12 directly and
-// C is read and written once per panel step. Panels here are full (jb = 32):
-// the last, partial panel has no rows below it.
+// C is read and written once per launch whatever K. The update covers rows
+// [rbeg, m) x columns [cbeg, min(ncols, cend)) with L21 = columns
+// [kbeg, kbeg + 32 nk); those panels are full (a partial panel is always the
+// last one and has no rows below it).
 constexpr int kTile32 = 128;   // rows per FP32 update tile
 constexpr int kTile32N = 64;   // columns per FP32 update tile
 
 template <int TM, int TN>
 __global__ __launch_bounds__(256) void k_lu_update_f32(const LuDesc<float> *__restrict__ descs,
-                                                       const int4 *__restrict__ work, int p)
+                                                       const int4 *__restrict__ work, int kbeg, int nk,
+                                                       int rbeg, int cbeg, int cend)
 {
     typedef Mfma<float> MF;
     constexpr int WM = TM / 2, WN = TN / 2;       // per-wave block (2 x 2 waves)
@@ -549,60 +558,16 @@ __global__ __launch_bounds__(256) void k_lu_update_f32(const LuDesc<float> *__re
     constexpr int LA = TM + 4, LB = TN + 4;
     const int4 wk = work[blockIdx.x];
     const LuDesc<float> &D = descs[wk.x];
-    const int s = D.m, nc = D.ncols;
-    if (D.info[0] != 0 || p + kPanel >= s)
+    const int s = D.m, nc = min(D.ncols, cend);
+    if (D.info[0] != 0 || rbeg >= s || kbeg + kPanel * nk > s)
         return;
     const int lda = D.lda;
-    const int r0 = p + kPanel + wk.y * TM, c0 = p + kPanel + wk.z * TN;
+    const int r0 = rbeg + wk.y * TM, c0 = cbeg + wk.z * TN;
     __shared__ float sa_r[kPanel][LA], sa_i[kPanel][LA];
     __shared__ float sb_r[kPanel][LB], sb_i[kPanel][LB];
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const int wr = wave >> 1, wc = wave & 1;
     const float *ar = D.ar, *ai = D.ai;
-    // A: column p + k, rows r0 .. r0 + TM - 1 (contiguous): TM / 4 float4 per column
-#pragma unroll
-    for (int i = 0; i < (kPanel * TM / 4) / 256; ++i) {
-        const int e = t + 256 * i;
-        const int k = e / (TM / 4), r4 = (e % (TM / 4)) * 4;
-        const int gr = r0 + r4;
-        float4 vr = make_float4(0.f, 0.f, 0.f, 0.f), vi = vr;
-        const size_t o = (size_t)(p + k) * lda + gr;
-        if (gr + 3 < s) {
-            vr = *(const float4 *)(ar + o);
-            vi = *(const float4 *)(ai + o);
-        } else if (gr < s) {
-            float tr[4] = {0.f, 0.f, 0.f, 0.f}, ti[4] = {0.f, 0.f, 0.f, 0.f};
-            for (int q = 0; q < 4 && gr + q < s; ++q) {
-                tr[q] = ar[o + q];
-                ti[q] = ai[o + q];
-            }
-            vr = make_float4(tr[0], tr[1], tr[2], tr[3]);
-            vi = make_float4(ti[0], ti[1], ti[2], ti[3]);
-        }
-        *(float4 *)&sa_r[k][r4] = vr;
-        *(float4 *)&sa_i[k][r4] = vi;
-    }
-    // B: column c0 + c, rows p .. p + 31 (contiguous): 8 float4 per column
-#pragma unroll
-    for (int i = 0; i < (kPanel * TN / 4) / 256; ++i) {
-        const int e = t + 256 * i;
-        const int c = e >> 3, k4 = (e & 7) * 4;
-        const int gc = c0 + c;
-        float4 vr = make_float4(0.f, 0.f, 0.f, 0.f), vi = vr;
-        if (gc < nc) {
-            const size_t o = (size_t)gc * lda + p + k4;
-            vr = *(const float4 *)(ar + o);
-            vi = *(const float4 *)(ai + o);
-        }
-        sb_r[k4][c] = vr.x;
-        sb_r[k4 + 1][c] = vr.y;
-        sb_r[k4 + 2][c] = vr.z;
-        sb_r[k4 + 3][c] = vr.w;
-        sb_i[k4][c] = vi.x;
-        sb_i[k4 + 1][c] = vi.y;
-        sb_i[k4 + 2][c] = vi.z;
-        sb_i[k4 + 3][c] = vi.w;
-    }
     // accumulators = C
     typename MF::acc accr[AM][AN], acci[AM][AN];
     const int lrow = lane & 15, lk = lane >> 4;
@@ -629,30 +594,79 @@ __global__ __launch_bounds__(256) void k_lu_update_f32(const LuDesc<float> *__re
             accr[a][b] = cr;
             acci[a][b] = ci;
         }
-    __syncthreads();
+    for (int chunk = 0; chunk < nk; ++chunk) {
+        const int k0 = kbeg + kPanel * chunk;
+        if (chunk)
+            __syncthreads(); // previous chunk's operands consumed
+        // A: column k0 + k, rows r0 .. r0 + TM - 1 (contiguous): TM / 4 float4 per column
 #pragma unroll
-    for (int kk = 0; kk < kPanel; kk += 4) {
-        float xr[AM], xi[AM], yr[AN], yi[AN];
-#pragma unroll
-        for (int a = 0; a < AM; ++a) {
-            xr[a] = sa_r[kk + lk][wr * WM + a * 16 + lrow];
-            xi[a] = sa_i[kk + lk][wr * WM + a * 16 + lrow];
+        for (int i = 0; i < (kPanel * TM / 4) / 256; ++i) {
+            const int e = t + 256 * i;
+            const int k = e / (TM / 4), r4 = (e % (TM / 4)) * 4;
+            const int gr = r0 + r4;
+            float4 vr = make_float4(0.f, 0.f, 0.f, 0.f), vi = vr;
+            const size_t o = (size_t)(k0 + k) * lda + gr;
+            if (gr + 3 < s) {
+                vr = *(const float4 *)(ar + o);
+                vi = *(const float4 *)(ai + o);
+            } else if (gr < s) {
+                float tr[4] = {0.f, 0.f, 0.f, 0.f}, ti[4] = {0.f, 0.f, 0.f, 0.f};
+                for (int q = 0; q < 4 && gr + q < s; ++q) {
+                    tr[q] = ar[o + q];
+                    ti[q] = ai[o + q];
+                }
+                vr = make_float4(tr[0], tr[1], tr[2], tr[3]);
+                vi = make_float4(ti[0], ti[1], ti[2], ti[3]);
+            }
+            *(float4 *)&sa_r[k][r4] = vr;
+            *(float4 *)&sa_i[k][r4] = vi;
         }
+        // B: column c0 + c, rows k0 .. k0 + 31 (contiguous): 8 float4 per column
 #pragma unroll
-        for (int b = 0; b < AN; ++b) {
-            yr[b] = sb_r[kk + lk][wc * WN + b * 16 + lrow];
-            yi[b] = sb_i[kk + lk][wc * WN + b * 16 + lrow];
+        for (int i = 0; i < (kPanel * TN / 4) / 256; ++i) {
+            const int e = t + 256 * i;
+            const int c = e >> 3, k4 = (e & 7) * 4;
+            const int gc = c0 + c;
+            float4 vr = make_float4(0.f, 0.f, 0.f, 0.f), vi = vr;
+            if (gc < nc) {
+                const size_t o = (size_t)gc * lda + k0 + k4;
+                vr = *(const float4 *)(ar + o);
+                vi = *(const float4 *)(ai + o);
+            }
+            sb_r[k4][c] = vr.x;
+            sb_r[k4 + 1][c] = vr.y;
+            sb_r[k4 + 2][c] = vr.z;
+            sb_r[k4 + 3][c] = vr.w;
+            sb_i[k4][c] = vi.x;
+            sb_i[k4 + 1][c] = vi.y;
+            sb_i[k4 + 2][c] = vi.z;
+            sb_i[k4 + 3][c] = vi.w;
         }
+        __syncthreads();
 #pragma unroll
-        for (int a = 0; a < AM; ++a)
+        for (int kk = 0; kk < kPanel; kk += 4) {
+            float xr[AM], xi[AM], yr[AN], yi[AN];
+#pragma unroll
+            for (int a = 0; a < AM; ++a) {
+                xr[a] = sa_r[kk + lk][wr * WM + a * 16 + lrow];
+                xi[a] = sa_i[kk + lk][wr * WM + a * 16 + lrow];
+            }
 #pragma unroll
             for (int b = 0; b < AN; ++b) {
-                // C - (xr + i xi)(yr + i yi)
-                accr[a][b] = MF::fma(-xr[a], yr[b], accr[a][b]);
-                accr[a][b] = MF::fma(xi[a], yi[b], accr[a][b]);
-                acci[a][b] = MF::fma(-xr[a], yi[b], acci[a][b]);
-                acci[a][b] = MF::fma(-xi[a], yr[b], acci[a][b]);
+                yr[b] = sb_r[kk + lk][wc * WN + b * 16 + lrow];
+                yi[b] = sb_i[kk + lk][wc * WN + b * 16 + lrow];
             }
+#pragma unroll
+            for (int a = 0; a < AM; ++a)
+#pragma unroll
+                for (int b = 0; b < AN; ++b) {
+                    // C - (xr + i xi)(yr + i yi)
+                    accr[a][b] = MF::fma(-xr[a], yr[b], accr[a][b]);
+                    accr[a][b] = MF::fma(xi[a], yi[b], accr[a][b]);
+                    acci[a][b] = MF::fma(-xr[a], yi[b], acci[a][b]);
+                    acci[a][b] = MF::fma(-xi[a], yr[b], acci[a][b]);
+                }
+        }
     }
 #pragma unroll
     for (int a = 0; a < AM; ++a)
@@ -675,10 +689,96 @@ __global__ __launch_bounds__(256) void k_lu_update_f32(const LuDesc<float> *__re
         }
 }
 
-inline void lu_launch_update_f32(const LuDesc<float> *d, const LuStep &s, const int *ints, hipStream_t st)
+// Outer step of the two-level FP32 LU: for columns [P + 64, ncols) apply the
+// interchanges of both 32-column panels in order, then U12 = L11^{-1} A12
+// with the 64 x 64 unit-lower L11 at (P, P). One thread per column.
+__global__ __launch_bounds__(64) void k_lu_trsm64_f32(const LuDesc<float> *__restrict__ descs,
+                                                      const int2 *__restrict__ work, int P)
 {
-    hipLaunchKernelGGL((k_lu_update_f32<kTile32, kTile32N>), dim3((unsigned)s.upd_cnt), dim3(256), 0,
-                       st, d, (const int4 *)(ints + s.upd_off), s.p);
+    constexpr int NB = 2 * kPanel;
+    const int2 wk = work[blockIdx.x];
+    const LuDesc<float> &D = descs[wk.x];
+    const int s = D.m;
+    if (D.info[0] != 0 || P + NB >= s)
+        return;
+    const int lda = D.lda, t = threadIdx.x;
+    __shared__ float lr[NB][NB + 1], li[NB][NB + 1];
+    __shared__ int piv_s[NB];
+    piv_s[t] = D.ipiv[P + t];
+    for (int e = t; e < NB * NB; e += 64) {
+        const int rr = e % NB, cc = e / NB;
+        lr[rr][cc] = D.ar[(size_t)(P + cc) * lda + P + rr];
+        li[rr][cc] = D.ai[(size_t)(P + cc) * lda + P + rr];
+    }
+    __syncthreads();
+    const bool any_swap = __syncthreads_or(piv_s[t] != P + t);
+    const int c = P + NB + wk.y * 64 + t;
+    if (c >= D.ncols)
+        return;
+    float *cr = D.ar + (size_t)c * lda, *ci = D.ai + (size_t)c * lda;
+    if (any_swap)
+        for (int j = 0; j < NB; ++j) {
+            const int piv = piv_s[j];
+            if (piv != P + j) {
+                const float xr = cr[P + j], xi = ci[P + j];
+                cr[P + j] = cr[piv];
+                ci[P + j] = ci[piv];
+                cr[piv] = xr;
+                ci[piv] = xi;
+            }
+        }
+    float ur[NB], ui[NB];
+#pragma unroll
+    for (int j = 0; j < NB; ++j) {
+        ur[j] = cr[P + j];
+        ui[j] = ci[P + j];
+    }
+#pragma unroll
+    for (int j = 0; j < NB; ++j) {
+        float vr = ur[j], vi = ui[j];
+#pragma unroll
+        for (int m = 0; m < j; ++m) {
+            vr -= lr[j][m] * ur[m] - li[j][m] * ui[m];
+            vi -= lr[j][m] * ui[m] + li[j][m] * ur[m];
+        }
+        ur[j] = vr;
+        ui[j] = vi;
+    }
+#pragma unroll
+    for (int j = 0; j < NB; ++j) {
+        cr[P + j] = ur[j];
+        ci[P + j] = ui[j];
+    }
+}
+
+inline void lu_launch_update_f32(const LuDesc<float> *d, size_t off, size_t cnt, const int *ints,
+                                 int kbeg, int nk, int rbeg, int cbeg, int cend, hipStream_t st)
+{
+    hipLaunchKernelGGL((k_lu_update_f32<kTile32, kTile32N>), dim3((unsigned)cnt), dim3(256), 0, st, d,
+                       (const int4 *)(ints + off), kbeg, nk, rbeg, cbeg, cend);
+}
+
+// The 32-column update inside a 64-column outer step (rows [P + 32, m) x
+// columns [P + 32, P + 64), K = 32): 128 x 32 tiles, one column tile.
+inline void lu_launch_update_narrow_f32(const LuDesc<float> *d, size_t off, size_t cnt, const int *ints,
+                                        int P, hipStream_t st)
+{
+    hipLaunchKernelGGL((k_lu_update_f32<kTile32, kPanel>), dim3((unsigned)cnt), dim3(256), 0, st, d,
+                       (const int4 *)(ints + off), P, 1, P + kPanel, P + kPanel, P + 2 * kPanel);
+}
+
+inline void lu_launch_trsm_f32(const LuDesc<float> *d, size_t off, size_t cnt, const int *ints, int p,
+                               int cend, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_lu_trsm<float>, dim3((unsigned)cnt), dim3(64), 0, st, d,
+                       (const int2 *)(ints + off), p, cend);
+}
+
+inline void lu_launch_trsm64_f32(const LuDesc<float> *d, size_t off, size_t cnt, const int *ints, int P,
+                                 hipStream_t st)
+{
+    hipLaunchKernelGGL(k_lu_trsm64_f32, dim3((unsigned)cnt), dim3(64), 0, st, d,
+                       (const int2 *)(ints + off), P);
 }
 
 // Host side: one panel step of the batched factorisation on stream st.
@@ -700,7 +800,7 @@ template <typename T>
 void lu_launch_trsm(const LuDesc<T> *d, const LuStep &s, const int *ints, hipStream_t st)
 {
     hipLaunchKernelGGL(k_lu_trsm<T>, dim3((unsigned)s.trsm_cnt), dim3(64), 0, st, d,
-                       (const int2 *)(ints + s.trsm_off), s.p);
+                       (const int2 *)(ints + s.trsm_off), s.p, 0x7fffffff);
 }
 
 template <typename T>

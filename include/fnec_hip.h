@@ -230,7 +230,9 @@ long fnec_hip_get_tunable(const char *name);
  * Kernel classes: "fill", "gram", "lu_panel", "lu_trsm", "lu_update",
  * "backsub", "rp", and on the mixed path "zsolve" (Z / Z^H solves with the
  * FP32 factors), "matvec" (M p and M^H t in FP64), "pcg_ctl", "fsetup"
- * (Z^H solves of the constraint rows). out4 = {launches, total microseconds, algorithmic flops,
+ * (Z^H solves of the constraint rows), "lu_update_narrow" (the K = 32 update
+ * of the second 32-column panel inside each 64-column step of the FP32 LU;
+ * "lu_update" is then the K = 64 trailing update). out4 = {launches, total microseconds, algorithmic flops,
  * algorithmic bytes} accumulated since the last reset. */
 void fnec_hip_profile_enable(int on);
 void fnec_hip_profile_reset(void);
