@@ -12,7 +12,8 @@ FP64 one-request-at-a-time GPU path on a sample of the same stream (the
 reference point of the speedup) with the speedup over it and the max relative
 Z_in difference between the served and the plain path, the roofline of the
 dominant kernel from HIP events recorded on its stream during the timed
-region, and a CPU baseline (the oracle, timed on this host on a bounded
+region (events around that kernel class only; the per-class breakdown comes
+from untimed windows profiled before it), and a CPU baseline (the oracle, timed on this host on a bounded
 sample and extrapolated over the stream's N distribution).
 """
 from __future__ import annotations
@@ -84,7 +85,10 @@ def build_requests(stream):
     return items
 
 
-def serve(items, window, steps, warmup, precision, budget, max_batch, profile):
+def serve(items, window, steps, warmup, prof_windows, precision, budget, max_batch):
+    """Warmup windows, then ``prof_windows`` untimed windows with events on
+    every kernel class (the per-class breakdown; it names the dominant class),
+    then the timed windows with events around the dominant class only."""
     L = F.lib()
     sess = L.fnec_hip_session_create(budget, precision, max_batch)
     if not sess:
@@ -117,16 +121,22 @@ def serve(items, window, steps, warmup, precision, budget, max_batch, profile):
 
     for w in range(warmup):
         run_window(items[w * window:(w + 1) * window])
-    if profile:
-        F.profile(enable=True, reset=True)
+    F.profile(enable=True, reset=True, only="")
+    for w in range(warmup, warmup + prof_windows):
+        run_window(items[w * window:(w + 1) * window])
+    prof_all = F.profile(enable=False)
+    dom = max((k for k in prof_all if prof_all[k][0] > 0 and prof_all[k][2] > 0),
+              key=lambda k: prof_all[k][1])
+    F.profile(enable=True, reset=True, only=dom)
+    first = warmup + prof_windows
     lat = []
     t0 = time.perf_counter()
     for s in range(steps):
-        lat += run_window(items[(warmup + s) * window:(warmup + s + 1) * window])
+        lat += run_window(items[(first + s) * window:(first + s + 1) * window])
     elapsed = time.perf_counter() - t0
-    prof = F.profile(enable=False) if profile else None
+    prof_timed = F.profile(enable=False, only="")
     L.fnec_hip_session_destroy(sess)
-    return elapsed, np.array(lat), results, prof
+    return elapsed, np.array(lat), results, dom, prof_all, prof_timed
 
 
 def plain_fp64(items, sample):
@@ -239,6 +249,8 @@ def main():
     ap.add_argument("--steps", type=int, default=12)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--window", type=int, default=256, help="requests per step")
+    ap.add_argument("--profile-windows", type=int, default=2,
+                    help="untimed windows with events on every kernel class, before the timed ones")
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="mixed")
     ap.add_argument("--plain-sample", type=int, default=256)
     ap.add_argument("--max-batch", type=int, default=256)
@@ -257,7 +269,7 @@ def main():
         sys.exit(2)
     F.init()
     precision = F.PREC_MIXED if args.precision == "mixed" else F.PREC_FP64
-    total = (args.warmup + args.steps) * args.window
+    total = (args.warmup + args.profile_windows + args.steps) * args.window
     t_gen = time.perf_counter()
     stream = W.config5_stream(count=total)
     items = build_requests(stream)
@@ -265,13 +277,15 @@ def main():
     print(f"[bench] built {len(items)} requests in {t_gen:.1f}s; serving {args.steps} x {args.window}",
           file=sys.stderr, flush=True)
 
-    elapsed, lat, zs_served, prof = serve(items, args.window, args.steps, args.warmup, precision,
-                                          int(args.budget_gib * 2 ** 30), args.max_batch, True)
+    elapsed, lat, zs_served, dom, prof_all, prof = serve(items, args.window, args.steps, args.warmup,
+                                                         args.profile_windows, precision,
+                                                         int(args.budget_gib * 2 ** 30), args.max_batch)
+    first = (args.warmup + args.profile_windows) * args.window
     timed = args.steps * args.window
     value = timed / elapsed
     print(f"[bench] served {timed} requests in {elapsed:.2f}s", file=sys.stderr, flush=True)
 
-    sample_items = items[args.warmup * args.window:args.warmup * args.window + args.plain_sample]
+    sample_items = items[first:first + args.plain_sample]
     t_plain, zs_plain = plain_fp64(sample_items, args.plain_sample)
     plain_rate = len(sample_items) / t_plain
     served_sample_time = sum(1.0 for _ in sample_items) / value
@@ -279,10 +293,10 @@ def main():
     print(f"[bench] plain FP64 one-at-a-time: {plain_rate:.1f} solves/s on {len(sample_items)}",
           file=sys.stderr, flush=True)
 
-    # dominant kernel class by time during the timed region; its bound is the
-    # roofline side its algorithmic intensity falls on (flops per HBM byte vs
-    # the ridge peak_flops / peak_bandwidth)
-    dom = max((k for k in prof if prof[k][0] > 0 and prof[k][2] > 0), key=lambda k: prof[k][1])
+    # dominant kernel class by time in the profiled pass; its events in the
+    # timed region give the achieved rate, and its bound is the roofline side
+    # its algorithmic intensity falls on (flops per HBM byte vs the ridge
+    # peak_flops / peak_bandwidth)
     launches, tot_us, flops, nbytes = prof[dom]
     avg_us = tot_us / launches
     mfma = dom in ("gram", "lu_update")
@@ -309,11 +323,12 @@ def main():
         "avg_launch_us": round(avg_us, 2),
         "launches": int(launches),
         "algorithmic_bytes_per_launch": round(nbytes / launches) if nbytes else None,
+        "kernels_source": f"{args.profile_windows} untimed windows of {args.window} with events on every class",
         "kernels": {k: {"launches": int(v[0]), "total_ms": round(v[1] / 1e3, 3),
                         "tflops": round(v[2] / (v[1] * 1e-6) / 1e12, 3) if v[1] and v[2] else None,
                         "gbs": round(v[3] / (v[1] * 1e-6) / 1e9, 1) if v[1] and v[3] else None,
                         "bytes_per_launch": round(v[3] / v[0]) if v[3] else None}
-                    for k, v in prof.items() if v[0] > 0},
+                    for k, v in prof_all.items() if v[0] > 0},
     }
     cpu = None
     if not args.no_cpu_baseline:

@@ -380,17 +380,25 @@ def solve_problem(p: dict, precision: int = PREC_FP64, with_stats: bool = False)
                         p["freq_hz"], p["ground_struct"], precision, with_stats)
 
 
-def profile(enable: bool | None = None, reset: bool = False) -> dict:
-    """Per-kernel event timings: {class: (launches, total_us, flops, bytes)}."""
+PROFILE_CLASSES = ("fill", "gram", "lu_panel", "lu_trsm", "lu_update", "backsub", "rp", "zsolve", "matvec",
+                   "pcg_ctl", "fsetup", "lu_update_narrow")
+
+
+def profile(enable: bool | None = None, reset: bool = False, only: str | None = None) -> dict:
+    """Per-kernel event timings: {class: (launches, total_us, flops, bytes)}.
+
+    ``only`` restricts event recording to one class ("" for every class)."""
     L = lib()
     L.fnec_hip_profile_read.argtypes = [C.c_char_p, C.c_void_p]
+    L.fnec_hip_profile_only.argtypes = [C.c_char_p]
+    if only is not None:
+        _check(L.fnec_hip_profile_only(only.encode()))
     if reset:
         L.fnec_hip_profile_reset()
     if enable is not None:
         L.fnec_hip_profile_enable(int(enable))
     out = {}
-    for k in ["fill", "gram", "lu_panel", "lu_trsm", "lu_update", "backsub", "rp", "zsolve", "matvec",
-              "pcg_ctl", "fsetup", "lu_update_narrow"]:
+    for k in PROFILE_CLASSES:
         buf = np.zeros(4)
         _check(L.fnec_hip_profile_read(k.encode(), _p(buf)))
         out[k] = tuple(float(v) for v in buf)

@@ -212,6 +212,15 @@ void fnec_hip_profile_enable(int on)
     profile_enable(on);
 }
 
+int fnec_hip_profile_only(const char *kernel)
+{
+    std::lock_guard<std::mutex> lk(global_mutex());
+    const int st = profile_only(kernel);
+    if (st)
+        set_error("unknown kernel class");
+    return st;
+}
+
 void fnec_hip_profile_reset(void)
 {
     std::lock_guard<std::mutex> lk(global_mutex());

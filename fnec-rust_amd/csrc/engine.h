@@ -63,6 +63,7 @@ int mfma_probe32(const float *a, const float *b, float *out);
 int set_tunable(const char *name, long value);
 long get_tunable(const char *name);
 void profile_enable(int on);
+int profile_only(const char *name);
 void profile_reset();
 int profile_read(const char *name, double *out4);
 

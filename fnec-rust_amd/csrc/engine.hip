@@ -122,9 +122,11 @@ static hipEvent_t prof_event()
     return e;
 }
 
-static hipEvent_t prof_begin()
+static int g_prof_only = -1; // record only this class (-1: every class)
+
+static hipEvent_t prof_begin(int cls)
 {
-    if (!g_prof)
+    if (!g_prof || (g_prof_only >= 0 && cls != g_prof_only))
         return nullptr;
     hipEvent_t e = prof_event();
     (void)hipEventRecord(e, g_ctx.stream);
@@ -170,6 +172,19 @@ static void prof_collect()
 }
 
 void profile_enable(int on) { g_prof = on != 0; }
+int profile_only(const char *name)
+{
+    if (!name || !*name) {
+        g_prof_only = -1;
+        return FNEC_OK;
+    }
+    for (int i = 0; i < PC_N; ++i)
+        if (!std::strcmp(name, kProfNames[i])) {
+            g_prof_only = i;
+            return FNEC_OK;
+        }
+    return FNEC_INVALID_ARG;
+}
 void profile_reset()
 {
     for (auto &c : g_prof_acc)
@@ -1294,35 +1309,35 @@ static int mixed_launch(Plan &pl, char *base)
     HIPCHK(hipEventRecord(g_ctx.ev[2], st));
     const LuDesc<float> *lud = (const LuDesc<float> *)(base + pl.off_lud32);
     for (const Plan::Outer &o : pl.outer32) {
-        hipEvent_t e = prof_begin();
+        hipEvent_t e = prof_begin(PC_LU_PANEL);
         lu_launch_panel(lud, o.a, ints, st);
         if (o.has_b)
             prof_end(PC_LU_PANEL, e, 0, 0);
         else
             prof_end(PC_LU_PANEL, e, o.f_panel, 0);
-        e = prof_begin();
+        e = prof_begin(PC_LU_TRSM);
         lu_launch_trsm_f32(lud, o.a.trsm_off, o.a.trsm_cnt, ints, o.P, o.P + 2 * kPanel, st);
         prof_end(PC_LU_TRSM, e, 0, 0);
         if (o.ua_cnt) {
-            e = prof_begin();
+            e = prof_begin(PC_LU_UPDN);
             lu_launch_update_narrow_f32(lud, o.ua_off, o.ua_cnt, ints, o.P, st);
             prof_end(PC_LU_UPDN, e, o.f_ua, o.b_ua);
         }
         if (o.has_b) {
-            e = prof_begin();
+            e = prof_begin(PC_LU_PANEL);
             lu_launch_panel(lud, o.b, ints, st);
             prof_end(PC_LU_PANEL, e, o.f_panel, 0);
-            e = prof_begin();
+            e = prof_begin(PC_LU_TRSM);
             lu_launch_trsm_f32(lud, o.b.trsm_off, o.b.trsm_cnt, ints, o.P + kPanel, 0, st);
             prof_end(PC_LU_TRSM, e, 0, 0);
         }
         if (o.has_outer) {
-            e = prof_begin();
+            e = prof_begin(PC_LU_TRSM);
             if (o.ot_cnt)
                 lu_launch_trsm64_f32(lud, o.ot_off, o.ot_cnt, ints, o.P, st);
             prof_end(PC_LU_TRSM, e, o.f_trsm, 0);
             if (o.ou_cnt) {
-                e = prof_begin();
+                e = prof_begin(PC_LU_UPDATE);
                 lu_launch_update_f32(lud, o.ou_off, o.ou_cnt, ints, o.P, 2, o.P + 2 * kPanel,
                                      o.P + 2 * kPanel, 0x7fffffff, st);
                 prof_end(PC_LU_UPDATE, e, o.f_ou, o.b_ou);
@@ -1365,7 +1380,7 @@ static int mixed_launch(Plan &pl, char *base)
                        (const int2 *)(ints + pl.dinv_off));
     hipLaunchKernelGGL(k_mix_init, dim3(np), dim3(1024), 0, st, dp, all);
     if (pl.fsol_cnt) {
-        hipEvent_t e = prof_begin();
+        hipEvent_t e = prof_begin(PC_FSETUP);
         hipLaunchKernelGGL(k_trsv_multi, dim3((unsigned)pl.fsol_cnt), dim3(256), 0, st, dp,
                            (const int4 *)(ints + pl.fsol_off), ntrsv + 1, tickets + ntrsv);
         ++ntrsv;
@@ -1386,19 +1401,19 @@ static int mixed_launch(Plan &pl, char *base)
     while (it < kMixMaxIt) {
         const int chunk = it == 0 ? g_mix_first_chunk : kMixNextChunk;
         for (int c = 0; c < chunk && it < kMixMaxIt; ++c, ++it) {
-            hipEvent_t e = prof_begin();
+            hipEvent_t e = prof_begin(PC_ZSOLVE);
             zsolve(1);
             pl.it_zs1.push_back(prof_end(PC_ZSOLVE, e, pl.zsolve_flops, pl.zsolve_bytes));
-            e = prof_begin();
+            e = prof_begin(PC_PCG);
             hipLaunchKernelGGL(k_prec_mid, dim3(np), dim3(1024), 0, st, dp, all);
             prof_end(PC_PCG, e, 0, 0);
-            e = prof_begin();
+            e = prof_begin(PC_ZSOLVE);
             zsolve(0);
             pl.it_zs0.push_back(prof_end(PC_ZSOLVE, e, pl.zsolve_flops, pl.zsolve_bytes));
-            e = prof_begin();
+            e = prof_begin(PC_PCG);
             hipLaunchKernelGGL(k_mix_dir, dim3(np), dim3(1024), 0, st, dp, all);
             prof_end(PC_PCG, e, 0, 0);
-            e = prof_begin();
+            e = prof_begin(PC_MV);
             hipLaunchKernelGGL(k_mv_z, dim3((unsigned)pl.mvz_cnt), dim3(kMvRows), 0, st, dp,
                                (const int4 *)(ints + pl.mvz_off));
             hipLaunchKernelGGL(k_mv_fin, dim3((unsigned)pl.mvfin_cnt), dim3(kMvRows), 0, st, dp,
@@ -1406,7 +1421,7 @@ static int mixed_launch(Plan &pl, char *base)
             hipLaunchKernelGGL(k_mv_zh, dim3((unsigned)pl.mvzh_cnt), dim3(256), 0, st, dp,
                                (const int2 *)(ints + pl.mvzh_off), 0);
             pl.it_mv.push_back(prof_end(PC_MV, e, pl.mv_flops, pl.mv_bytes));
-            e = prof_begin();
+            e = prof_begin(PC_PCG);
             hipLaunchKernelGGL(k_mix_step, dim3(np), dim3(1024), 0, st, dp, all);
             prof_end(PC_PCG, e, 0, 0);
         }
@@ -1437,7 +1452,7 @@ static int plan_launch(Plan &pl, char *base, bool with_solve)
     const Prob *dp = (const Prob *)(base + pl.off_probs);
     HIPCHK(hipEventRecord(g_ctx.ev[0], st));
     if (pl.fill_cnt) {
-        hipEvent_t e = prof_begin();
+        hipEvent_t e = prof_begin(PC_FILL);
         hipLaunchKernelGGL(k_fill_z, dim3((unsigned)pl.fill_cnt), dim3(256), 0, st, dp,
                            (const int4 *)wl_ptr(base, pl, pl.fill_off));
         prof_end(PC_FILL, e, pl.fill_flops, 0);
@@ -1454,7 +1469,7 @@ static int plan_launch(Plan &pl, char *base, bool with_solve)
             hipLaunchKernelGGL(k_zero, dim3(1024), dim3(256), 0, st, (double *)(base + pl.a_begin), count);
     }
     {
-        hipEvent_t e = prof_begin();
+        hipEvent_t e = prof_begin(PC_GRAM);
         hipLaunchKernelGGL(k_gram, dim3((unsigned)pl.gram_cnt), dim3(256), 0, st, dp,
                            (const int4 *)wl_ptr(base, pl, pl.gram_off));
         prof_end(PC_GRAM, e, pl.gram_flops, 0);
@@ -1470,17 +1485,17 @@ static int plan_launch(Plan &pl, char *base, bool with_solve)
     for (size_t si = 0; si < pl.steps.size(); ++si) {
         const auto &s = pl.steps[si];
         if (s.panel_cnt) {
-            hipEvent_t e = prof_begin();
+            hipEvent_t e = prof_begin(PC_LU_PANEL);
             lu_launch_panel(lud, s, ints, st);
             prof_end(PC_LU_PANEL, e, pl.step_panel_flops[si], 0);
         }
         if (s.trsm_cnt) {
-            hipEvent_t e = prof_begin();
+            hipEvent_t e = prof_begin(PC_LU_TRSM);
             lu_launch_trsm(lud, s, ints, st);
             prof_end(PC_LU_TRSM, e, pl.step_trsm_flops[si], 0);
         }
         if (s.upd_cnt) {
-            hipEvent_t e = prof_begin();
+            hipEvent_t e = prof_begin(PC_LU_UPDATE);
             lu_launch_update(lud, s, ints, st);
             prof_end(PC_LU_UPDATE, e, pl.step_upd_flops[si], 0);
         }
@@ -1488,7 +1503,7 @@ static int plan_launch(Plan &pl, char *base, bool with_solve)
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(g_ctx.ev[3], st));
     {
-        hipEvent_t e = prof_begin();
+        hipEvent_t e = prof_begin(PC_BACKSUB);
         hipLaunchKernelGGL(k_backsub, dim3((unsigned)pl.all_cnt), dim3(256), 0, st, dp,
                            wl_ptr(base, pl, pl.all_off));
         prof_end(PC_BACKSUB, e, pl.backsub_flops, 0);
@@ -1777,7 +1792,7 @@ int rp_run(const std::vector<RpHost> &jobs, std::vector<fnec_rp_row *> &outs, st
     double rp_terms = 0;
     for (const auto &h : jobs)
         rp_terms += (double)h.n * (h.ground ? 2.0 : 1.0) * ((double)h.theta.size() + h.ndir);
-    hipEvent_t e_rp = prof_begin();
+    hipEvent_t e_rp = prof_begin(PC_RP);
     if (pts_cnt)
         hipLaunchKernelGGL(k_rp_points, dim3((unsigned)pts_cnt), dim3(256), 0, s, dp, (const int2 *)(wi + pts_off));
     if (norm_cnt)

@@ -235,6 +235,10 @@ long fnec_hip_get_tunable(const char *name);
  * "lu_update" is then the K = 64 trailing update). out4 = {launches, total microseconds, algorithmic flops,
  * algorithmic bytes} accumulated since the last reset. */
 void fnec_hip_profile_enable(int on);
+/* Restrict event recording to one kernel class (NULL or "": every class), so
+ * that a timed run carries events only around the launches it reports.
+ * FNEC_INVALID_ARG for an unknown class name. */
+int fnec_hip_profile_only(const char *kernel);
 void fnec_hip_profile_reset(void);
 int fnec_hip_profile_read(const char *kernel, double *out4);
 

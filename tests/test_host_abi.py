@@ -2,6 +2,7 @@
 the library loads, exports every function include/*.h declares, and the
 C++ deck -> Hallén-input builders (include/fnec_host.h) produce the same
 doubles as the oracle's restatement of the reference builders."""
+import ctypes as C  # noqa: N812
 import glob
 import importlib
 import os
@@ -41,6 +42,16 @@ def test_library_exports_every_declared_function():
     assert len(names) >= 20
     missing = [n for n in names if not hasattr(L, n)]
     assert not missing, missing
+
+
+def test_profile_class_filter():
+    L = F.lib()
+    L.fnec_hip_profile_only.argtypes = [C.c_char_p]
+    assert L.fnec_hip_profile_only(b"no_such_kernel") == F.INVALID_ARG
+    for k in F.PROFILE_CLASSES:
+        assert L.fnec_hip_profile_only(k.encode()) == F.OK
+    assert L.fnec_hip_profile_only(b"") == F.OK
+    assert set(F.profile()) == set(F.PROFILE_CLASSES)
 
 
 def test_no_gpu_reports_zero_devices():
