@@ -73,6 +73,8 @@ struct Prob {
     double *tws;           // n x (ncons + 1) complex: triangular-solve workspace per vector
     int *tflags;           // (ncons + 1) x 2 x ceil(n / 64): block completion epochs
     float *dinv;           // ceil(n / 64) x 2 x 64 x 64 complex: L_ii^{-1}, U_ii^{-1}
+    unsigned long long *gx; // 2 x ceil(n / 64) x 64 x 4 words: solved blocks a gang publishes
+                            // (forward, backward sweep), one data dword + the launch epoch per word
     int nchunk;
     int maxit;
     double tol;

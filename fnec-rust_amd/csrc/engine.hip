@@ -36,7 +36,7 @@ __global__ void k_trsv0(const Prob *, const int4 *, int, unsigned long long *);
 __global__ void k_trsv1(const Prob *, const int4 *, int, unsigned long long *);
 __global__ void k_perm(const Prob *, const int *);
 __global__ void k_trsv_multi(const Prob *, const int4 *, int, unsigned long long *);
-__global__ void k_zsolve1(const Prob *, const int *, int);
+__global__ void k_zsolve(const Prob *, const int4 *, int, int, unsigned long long *);
 __global__ void k_diag_inv(const Prob *, const int2 *);
 __global__ void k_pre_setup(const Prob *, const int *);
 __global__ void k_prec_mid(const Prob *, const int *);
@@ -544,7 +544,7 @@ struct Plan {
     bool mixed = false;
     struct MixOff {
         size_t fr, fpr, rowcol, fh, lk, ym, ls, vr, vz, vp, vq, vt, vv, part, colsq, pcg, pstate;
-        size_t perm, tws, tflags, dinv;
+        size_t perm, tws, tflags, dinv, gx;
         int nchunk;
     };
     std::vector<MixOff> mix;
@@ -563,8 +563,8 @@ struct Plan {
     size_t fsol_off = 0, fsol_cnt = 0, vsol_off = 0, vsol_cnt = 0, mvz_off = 0, mvz_cnt = 0;
     size_t flags_begin = 0, flags_end = 0, off_tickets = 0; // zeroed at the start of a batch
     size_t dinv_off = 0, dinv_cnt = 0;
-    size_t one_off = 0, one_cnt = 0; // problems solved by one workgroup (n <= kSolve1MaxNHost)
-    int one_maxn = 0;
+    size_t zs_off = 0, zs_cnt = 0; // k_zsolve items (problem, member, G) for n <= g_solve1_max_n
+    size_t zs_lds = 0;             // dynamic LDS of a k_zsolve launch
     size_t in_begin = 0, io_begin = 0, io_end = 0; // staged upload [in_begin, io_end), read-back [io_begin, io_end)
     size_t mvfin_off = 0, mvfin_cnt = 0, mvzh_off = 0, mvzh_cnt = 0;
     double zsolve_flops = 0, mv_flops = 0, zsolve_bytes = 0, mv_bytes = 0;
@@ -577,13 +577,26 @@ struct Plan {
 constexpr int kMixMaxIt = 12;        // CG iterations at most
 constexpr int kMixFirstChunk = 4;    // iterations launched before the first convergence check
 constexpr int kMixNextChunk = 2;
-constexpr int kMaxTrsvLaunches = 2 * kMixMaxIt + 1; // F set-up + two per CG iteration
+constexpr int kMaxTrsvLaunches = 4 * kMixMaxIt + 1; // F set-up + two kernels x two solves per CG iteration
 constexpr int kMultiRhsHost = 16;                   // right-hand sides per set-up chain (k_trsv_multi)
 constexpr int kSolve1MaxNHost = 8192;               // largest n the one-workgroup solve holds in LDS
 
 // Tunables (fnec_hip_set_tunable): the autotuner's knobs.
 static int g_solve1_max_n = kSolve1MaxNHost; // CG solves of n <= this run one workgroup per problem
 static int g_mix_first_chunk = kMixFirstChunk;
+// Gang size of a CG solve: G = clamp(floor(n^2 / gang_n^2), 1, min(gang_max, blocks)),
+// so a problem's solve time shrinks towards the chain latency of its blocks
+// instead of one workgroup streaming the whole factor.
+static int g_gang_n = 768;
+static int g_gang_max = 4;
+static int gang_size(int n)
+{
+    if (g_gang_n <= 0)
+        return 1;
+    const long q = (long)n * n / ((long)g_gang_n * g_gang_n);
+    const int nb = (n + 63) / 64;
+    return (int)std::max<long>(1, std::min<long>(q, std::min(g_gang_max, nb)));
+}
 constexpr double kMixTol = 1e-14;    // |h - (M^H M + lambda) x| <= tol |h|
 
 static size_t take(size_t &cur, size_t bytes)
@@ -609,7 +622,8 @@ size_t problem_bytes(int n, int s, int ncons, int nruns, int precision)
         b += 4 * 16 * ss + 16 * (nn + nc) + 16 * nn + 16 * nchunk * nn + 8 * nn + 80;
         b += 4 * nn + 16 * nn * (nc + 1) + 8 * (nc + 1) * ((nn + 63) / 64);  // perm, solves
         b += ((nn + 63) / 64) * 2 * 64 * 64 * 8;                     // diagonal-block inverses
-        b += 21 * 256;                                               // alignment of 21 buffers
+        b += 8 * 2 * 256 * ((nn + 63) / 64);                         // gang-published blocks
+        b += 22 * 256;                                               // alignment of 22 buffers
     } else {
         b += 2 * 8 * lda * (ss + 1) + 256;                           // [G | M^H y]
         b += 2 * 8 * lda * kPanel + 256;                             // panel workspace
@@ -944,6 +958,7 @@ static void plan_build(Plan &pl, const std::vector<const HostProblem *> &hps, bo
         for (size_t b = 0; b < np; ++b) {
             const size_t nb = (hps[b]->n + 63) / 64, nc = std::max(hps[b]->ncons, 0);
             pl.mix[b].tflags = take(cur, 4 * (nc + 1) * 2 * nb);
+            pl.mix[b].gx = take(cur, 8 * 2 * nb * 64 * 4);
         }
         pl.off_tickets = take(cur, 8 * kMaxTrsvLaunches);
         pl.flags_end = cur = (cur + 7) & ~(size_t)7;
@@ -1018,7 +1033,7 @@ static void plan_build(Plan &pl, const std::vector<const HostProblem *> &hps, bo
             for (size_t b = 0; b < np; ++b) {
                 const int nb = (hps[b]->n + 63) / 64;
                 if (!fsetup && hps[b]->n <= g_solve1_max_n)
-                    continue; // CG solves of this problem run in k_zsolve1
+                    continue; // CG solves of this problem run in k_zsolve
                 maxnb = std::max(maxnb, nb);
                 if (fsetup) {
                     for (int g = 0; g < (hps[b]->ncons + kMultiRhsHost - 1) / kMultiRhsHost; ++g)
@@ -1043,14 +1058,29 @@ static void plan_build(Plan &pl, const std::vector<const HostProblem *> &hps, bo
         trsv_list(true, pl.fsol_off, pl.fsol_cnt);
         trsv_list(false, pl.vsol_off, pl.vsol_cnt);
         align4();
-        pl.one_off = pl.ints.size();
-        pl.one_maxn = 0;
-        for (size_t b = 0; b < np; ++b)
-            if (hps[b]->n <= g_solve1_max_n) {
-                pl.ints.push_back((int)b);
-                pl.one_maxn = std::max(pl.one_maxn, hps[b]->n);
+        // k_zsolve items, gangs first and larger problems first: a gang's
+        // members take consecutive tickets and the longest solves start first
+        pl.zs_off = pl.ints.size();
+        pl.zs_lds = 0;
+        {
+            std::vector<std::pair<int, int>> order; // (G, problem)
+            for (size_t b = 0; b < np; ++b)
+                if (hps[b]->n <= g_solve1_max_n)
+                    order.push_back({gang_size(hps[b]->n), (int)b});
+            std::stable_sort(order.begin(), order.end(), [&](const std::pair<int, int> &a, const std::pair<int, int> &c) {
+                if (a.first != c.first)
+                    return a.first > c.first;
+                return hps[a.second]->n > hps[c.second]->n;
+            });
+            for (const auto &o : order) {
+                const int n = hps[o.second]->n, G = o.first;
+                for (int mbr = 0; mbr < G; ++mbr)
+                    push4(o.second, mbr, G, 0);
+                const size_t lds = G == 1 ? 16 * (size_t)n : 16 * 64 * (size_t)(((n + 63) / 64 + G - 1) / G);
+                pl.zs_lds = std::max(pl.zs_lds, lds);
             }
-        pl.one_cnt = pl.ints.size() - pl.one_off;
+        }
+        pl.zs_cnt = (pl.ints.size() - pl.zs_off) / 4;
         align4();
         pl.mvz_off = pl.ints.size();
         for (size_t b = 0; b < np; ++b)
@@ -1219,6 +1249,7 @@ static int plan_upload(Plan &pl, const std::vector<const HostProblem *> &hps, ch
             P.perm = (int *)(base + m.perm);
             P.tws = (double *)(base + m.tws);
             P.tflags = (int *)(base + m.tflags);
+            P.gx = (unsigned long long *)(base + m.gx);
             P.dinv = (float *)(base + m.dinv);
             P.nchunk = m.nchunk;
             P.maxit = kMixMaxIt;
@@ -1352,14 +1383,15 @@ static int mixed_launch(Plan &pl, char *base)
     // CG solves: one workgroup per problem with its vector in LDS for
     // n <= kSolve1MaxNHost; the flag-chained multi-workgroup solve above that.
     // The choice depends on the problem only, so batching never changes a result.
-    const size_t lds1 = 16 * (size_t)pl.one_maxn;
-    if (pl.one_cnt && lds1 > 64 * 1024)
-        HIPCHK(hipFuncSetAttribute((const void *)k_zsolve1, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                   (int)lds1));
+    if (pl.zs_cnt && pl.zs_lds > 64 * 1024)
+        HIPCHK(hipFuncSetAttribute((const void *)k_zsolve, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)pl.zs_lds));
     auto zsolve = [&](int mode) {
-        if (pl.one_cnt)
-            hipLaunchKernelGGL(k_zsolve1, dim3((unsigned)pl.one_cnt), dim3(1024), lds1, st, dp,
-                               ints + pl.one_off, mode);
+        if (pl.zs_cnt) {
+            hipLaunchKernelGGL(k_zsolve, dim3((unsigned)pl.zs_cnt), dim3(1024), pl.zs_lds, st, dp,
+                               (const int4 *)(ints + pl.zs_off), mode, ntrsv + 1, tickets + ntrsv);
+            ++ntrsv;
+        }
         if (pl.vsol_cnt) {
             if (mode == 0)
                 hipLaunchKernelGGL(k_trsv0, dim3((unsigned)pl.vsol_cnt), dim3(256), 0, st, dp, vsol,
@@ -1909,6 +1941,14 @@ int set_tunable(const char *name, long value)
         g_mix_first_chunk = (int)value;
         return FNEC_OK;
     }
+    if (k == "gang_n" && value >= 0 && value <= 1 << 16) {
+        g_gang_n = (int)value;
+        return FNEC_OK;
+    }
+    if (k == "gang_max" && value >= 1 && value <= 64) {
+        g_gang_max = (int)value;
+        return FNEC_OK;
+    }
     set_error("unknown tunable or value out of range");
     return FNEC_INVALID_ARG;
 }
@@ -1920,6 +1960,10 @@ long get_tunable(const char *name)
         return g_solve1_max_n;
     if (k == "mix_first_chunk")
         return g_mix_first_chunk;
+    if (k == "gang_n")
+        return g_gang_n;
+    if (k == "gang_max")
+        return g_gang_max;
     return -1;
 }
 

@@ -545,15 +545,9 @@ constexpr int kSolve1Threads = 1024;
 
 __device__ __forceinline__ double2 lds2(const double2 *v, int i) { return v[i]; }
 
-__global__ __launch_bounds__(kSolve1Threads) void k_zsolve1(const Prob *__restrict__ probs,
-                                                            const int *__restrict__ plist, int mode)
+__device__ __forceinline__ void zsolve1_body(const Prob &P, int mode, double2 *__restrict__ vec,
+                                             double2 (*__restrict__ part)[kTb], double2 *__restrict__ xb)
 {
-    extern __shared__ double2 vec[];           // n entries
-    __shared__ double2 part[16][kTb];
-    __shared__ double2 xb[kTb];
-    const Prob &P = probs[plist[blockIdx.x]];
-    if (P.info[0] != 0 || P.pstate[0] != 0)
-        return;
     const int n = P.n, ld = P.ldz, nb = (n + kTb - 1) / kTb;
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6; // 16 waves
     const float *Fr = P.fr, *Fi = P.fi;
@@ -728,6 +722,480 @@ __global__ __launch_bounds__(kSolve1Threads) void k_zsolve1(const Prob *__restri
         for (int i = t; i < n; i += kSolve1Threads)
             io[P.perm[i]] = vec[i];
     }
+}
+
+// ---------------------------------------------------------------------------
+// Gang-scheduled Z / Z^H solves: G workgroups share one problem
+// ---------------------------------------------------------------------------
+//
+// Member m of a gang owns the 64-row blocks m, m + G, m + 2G, ... and keeps
+// only those entries of the vector in LDS. Both triangular sweeps are
+// right-looking: when block j is solved, its owner publishes x_j in P.gx
+// (epoch-tagged words, below), and every member subtracts B(b, j) x_j from
+// its owned blocks b beyond j, where
+//   B(b, j) = F[b rows, j columns]            (Z v = b: L forward, U backward)
+//   B(b, j) = conj(F[j rows, b columns])^T    (Z^H v = b: U^H forward, L^H backward).
+// The owner of block j + 1 first applies x_j to that block alone, solves and
+// publishes it, and only then updates its other blocks, so the chain of
+// solved blocks does not wait for the bulk of the updates; the operands of
+// that step are loaded before x_j is waited for. Reductions run in a fixed
+// order, so a result depends on (n, G) only.
+
+constexpr int kGangWaves = kSolve1Threads / 64;
+
+// own[k] -= B(b_k, j) x for the owned local blocks k in [ka, kb), b_k = m + k G.
+// Z (mode 0): a lane holds 4 consecutive rows (float4 down a column) and one
+// column of every 4; the 4 column lanes of a quad are summed with shuffles.
+// When fewer than 16 blocks remain, each block's columns are split over
+// several waves whose partial sums meet in red[][] in a fixed order.
+// Z^H (mode 1): a row of B is a column segment of F; a lane reads 4
+// consecutive entries of it (float4) at 4 column offsets and the quad sums.
+__device__ void gang_sweep(const Prob &P, int mode, int m, int G, int j, int ka, int kb,
+                           const double2 *__restrict__ x, double2 *__restrict__ own,
+                           double2 (*__restrict__ red)[kTb])
+{
+    const int nS = kb - ka;
+    if (nS <= 0)
+        return;
+    const int n = P.n, ld = P.ldz;
+    const int j0 = j * kTb, jn = min(kTb, n - j0);
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    int U = 1; // units per block: nS * U <= 16 when nS < 16
+    while (U < 16 && nS * U * 2 <= kGangWaves)
+        U *= 2;
+    const int nunits = nS * U;
+    const float *__restrict__ Fr = P.fr;
+    const float *__restrict__ Fi = P.fi;
+    if (mode == 0) {
+        const int rq = lane >> 2, cs = lane & 3;
+        const int ncol = kTb / U, iters = ncol / 4;
+        for (int u = wave; u < nunits; u += kGangWaves) {
+            const int k = ka + u / U, p = u % U;
+            const int r = (m + k * G) * kTb + 4 * rq; // first row of this lane's quad
+            double sr0 = 0.0, si0 = 0.0, sr1 = 0.0, si1 = 0.0, sr2 = 0.0, si2 = 0.0, sr3 = 0.0, si3 = 0.0;
+            const int cbeg = p * ncol;
+#pragma unroll 4
+            for (int i = 0; i < iters; ++i) {
+                const int c = cbeg + 4 * i + cs;
+                if (c < jn) {
+                    const size_t o = (size_t)(j0 + c) * ld + r;
+                    float4 fr = *(const float4 *)(Fr + o), fi = *(const float4 *)(Fi + o);
+                    if (r + 3 >= n) { // rows past n: column padding, never written
+                        if (r >= n) fr.x = fi.x = 0.f;
+                        if (r + 1 >= n) fr.y = fi.y = 0.f;
+                        if (r + 2 >= n) fr.z = fi.z = 0.f;
+                        fr.w = fi.w = 0.f;
+                    }
+                    const double2 xv = x[c];
+                    sr0 += (double)fr.x * xv.x - (double)fi.x * xv.y;
+                    si0 += (double)fr.x * xv.y + (double)fi.x * xv.x;
+                    sr1 += (double)fr.y * xv.x - (double)fi.y * xv.y;
+                    si1 += (double)fr.y * xv.y + (double)fi.y * xv.x;
+                    sr2 += (double)fr.z * xv.x - (double)fi.z * xv.y;
+                    si2 += (double)fr.z * xv.y + (double)fi.z * xv.x;
+                    sr3 += (double)fr.w * xv.x - (double)fi.w * xv.y;
+                    si3 += (double)fr.w * xv.y + (double)fi.w * xv.x;
+                }
+            }
+#pragma unroll
+            for (int d = 1; d <= 2; d <<= 1) {
+                sr0 += __shfl_xor(sr0, d);
+                si0 += __shfl_xor(si0, d);
+                sr1 += __shfl_xor(sr1, d);
+                si1 += __shfl_xor(si1, d);
+                sr2 += __shfl_xor(sr2, d);
+                si2 += __shfl_xor(si2, d);
+                sr3 += __shfl_xor(sr3, d);
+                si3 += __shfl_xor(si3, d);
+            }
+            // lane cs of the quad takes row 4 rq + cs
+            const double vr = cs == 0 ? sr0 : cs == 1 ? sr1 : cs == 2 ? sr2 : sr3;
+            const double vi = cs == 0 ? si0 : cs == 1 ? si1 : cs == 2 ? si2 : si3;
+            if (U == 1) {
+                double2 &o = own[k * kTb + 4 * rq + cs];
+                o = make_double2(o.x - vr, o.y - vi);
+            } else {
+                red[u][4 * rq + cs] = make_double2(vr, vi);
+            }
+        }
+        if (U > 1) {
+            __syncthreads();
+            for (int e = t; e < nS * kTb; e += kSolve1Threads) {
+                const int kk = e / kTb, rr = e % kTb;
+                double2 sum = red[kk * U][rr];
+                for (int p = 1; p < U; ++p)
+                    sum = cadd(sum, red[kk * U + p][rr]);
+                double2 &o = own[(ka + kk) * kTb + rr];
+                o = make_double2(o.x - sum.x, o.y - sum.y);
+            }
+        }
+    } else {
+        const int cq = lane & 3, rs = lane >> 2;
+        const int nrow = kTb / U; // rows per unit
+        for (int u = wave; u < nunits; u += kGangWaves) {
+            const int k = ka + u / U, p = u % U;
+            const int b0 = (m + k * G) * kTb;
+            for (int i = 0; i * 16 < nrow; ++i) {
+                const int rr = p * nrow + i * 16 + rs; // row of B within the block
+                const bool live = i * 16 + rs < nrow && b0 + rr < n;
+                double sr = 0.0, si = 0.0;
+                if (live) {
+                    const size_t col = (size_t)(b0 + rr) * ld + j0;
+#pragma unroll 2
+                    for (int h = 0; h < 4; ++h) {
+                        const int c = 16 * h + 4 * cq;
+                        if (c < jn) {
+                            float4 fr = *(const float4 *)(Fr + col + c), fi = *(const float4 *)(Fi + col + c);
+                            if (c + 3 >= jn) { // entries past n: column padding
+                                if (c + 1 >= jn) fr.y = fi.y = 0.f;
+                                if (c + 2 >= jn) fr.z = fi.z = 0.f;
+                                fr.w = fi.w = 0.f;
+                            }
+                            const double2 x0 = x[c], x1 = x[c + 1], x2 = x[c + 2], x3 = x[c + 3];
+                            sr += (double)fr.x * x0.x + (double)fi.x * x0.y + (double)fr.y * x1.x +
+                                  (double)fi.y * x1.y + (double)fr.z * x2.x + (double)fi.z * x2.y +
+                                  (double)fr.w * x3.x + (double)fi.w * x3.y;
+                            si += (double)fr.x * x0.y - (double)fi.x * x0.x + (double)fr.y * x1.y -
+                                  (double)fi.y * x1.x + (double)fr.z * x2.y - (double)fi.z * x2.x +
+                                  (double)fr.w * x3.y - (double)fi.w * x3.x;
+                        }
+                    }
+                }
+                sr += __shfl_xor(sr, 1);
+                si += __shfl_xor(si, 1);
+                sr += __shfl_xor(sr, 2);
+                si += __shfl_xor(si, 2);
+                if (live && cq == 0) {
+                    double2 &o = own[k * kTb + rr];
+                    o = make_double2(o.x - sr, o.y - si);
+                }
+            }
+        }
+    }
+}
+
+// Publication of solved blocks inside a gang. Each 32-bit half of an entry
+// travels in one 64-bit word together with the launch epoch, written and read
+// with relaxed agent-scope atomics: a word is single-copy atomic and coherent
+// across the device, so a reader that sees the epoch in every word of an
+// entry has its value, with no fence and no separate flag. The words are
+// cleared before a batch and every launch of a batch has its own epoch.
+__device__ __forceinline__ void ll_publish(unsigned long long *w, double2 v, unsigned epoch)
+{
+    const unsigned long long a = (unsigned long long)__double_as_longlong(v.x);
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v.y);
+    const unsigned long long e = (unsigned long long)epoch << 32;
+    __hip_atomic_store(w + 0, e | (a & 0xffffffffull), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(w + 1, e | (a >> 32), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(w + 2, e | (b & 0xffffffffull), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(w + 3, e | (b >> 32), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Wave-0 lanes t < valid poll the words of entry t until all carry epoch;
+// the entry goes to x[t] (0 for t >= valid). Bounded like wait_flag: past
+// kSpinLimit polls, or when another workgroup of the problem gave up, the
+// problem is marked failed. Ends with a workgroup barrier; returns false on failure.
+__device__ bool ll_fetch(const Prob &P, const unsigned long long *w, int valid, unsigned epoch, double2 *x,
+                         int *s_ok)
+{
+    const int t = threadIdx.x;
+    if (t < kTb) {
+        double2 v = make_double2(0.0, 0.0);
+        if (t < valid) {
+            const unsigned long long *p = w + 4 * t;
+            long spins = 0;
+            unsigned long long q0, q1, q2, q3;
+            for (;;) {
+                q0 = __hip_atomic_load(p + 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                q1 = __hip_atomic_load(p + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                q2 = __hip_atomic_load(p + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                q3 = __hip_atomic_load(p + 3, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if ((unsigned)(q0 >> 32) == epoch && (unsigned)(q1 >> 32) == epoch &&
+                    (unsigned)(q2 >> 32) == epoch && (unsigned)(q3 >> 32) == epoch)
+                    break;
+                ++spins;
+                if ((spins & 255) == 0 &&
+                    __hip_atomic_load(&P.pstate[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
+                    *s_ok = 0;
+                    break;
+                }
+                if (spins > kSpinLimit) {
+                    *s_ok = 0;
+                    __hip_atomic_store(&P.pstate[2], 4, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(1);
+            }
+            v.x = __longlong_as_double((long long)((q0 & 0xffffffffull) | (q1 << 32)));
+            v.y = __longlong_as_double((long long)((q2 & 0xffffffffull) | (q3 << 32)));
+        }
+        x[t] = v;
+    }
+    __syncthreads();
+    return *s_ok != 0;
+}
+
+// Operands of one look-ahead step (block jn updated with x_j, then solved),
+// loaded into registers before x_j is waited for, so that the chain of solved
+// blocks pays only the flag and x_j latencies.
+//   b: B(jn, j), one float4 per plane per thread. Z: thread (column c = t/16,
+//      rows 4 (t%16) .. +3). Z^H: thread (row r = t/16, columns 4 (t%16) .. +3).
+//   d: the inverse diagonal block of jn, thread (row t%64, columns 4 (t/64) .. +3).
+struct GangAhead {
+    float4 br, bi;
+    float dr[4], di[4];
+};
+
+__device__ __forceinline__ void gang_prefetch(const Prob &P, int mode, int which, int jn, int j, GangAhead &a)
+{
+    const int n = P.n, ld = P.ldz, t = threadIdx.x;
+    const int j0 = j * kTb, jc = min(kTb, n - j0), b0 = jn * kTb;
+    a.br = a.bi = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (mode == 0) {
+        const int c = t >> 4, r = b0 + 4 * (t & 15);
+        if (c < jc) {
+            const size_t o = (size_t)(j0 + c) * ld + r;
+            a.br = *(const float4 *)(P.fr + o);
+            a.bi = *(const float4 *)(P.fi + o);
+            if (r + 3 >= n) { // rows past n: column padding, never written
+                if (r >= n) a.br.x = a.bi.x = 0.f;
+                if (r + 1 >= n) a.br.y = a.bi.y = 0.f;
+                if (r + 2 >= n) a.br.z = a.bi.z = 0.f;
+                a.br.w = a.bi.w = 0.f;
+            }
+        }
+    } else {
+        const int r = t >> 4, c = 4 * (t & 15);
+        if (b0 + r < n && c < jc) {
+            const size_t o = (size_t)(b0 + r) * ld + j0 + c;
+            a.br = *(const float4 *)(P.fr + o);
+            a.bi = *(const float4 *)(P.fi + o);
+            if (c + 3 >= jc) {
+                if (c + 1 >= jc) a.br.y = a.bi.y = 0.f;
+                if (c + 2 >= jc) a.br.z = a.bi.z = 0.f;
+                a.br.w = a.bi.w = 0.f;
+            }
+        }
+    }
+    const int jnc = min(kTb, n - b0);
+    const float *src = P.dinv + ((size_t)jn * 2 + which) * 2 * kTb * kTb;
+    const int r = t & 63, g = t >> 6;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int c = 4 * g + q;
+        a.dr[q] = a.di[q] = 0.f;
+        if (c < jnc) {
+            const size_t e = mode == 0 ? (size_t)c * kTb + r : (size_t)r * kTb + c;
+            a.dr[q] = src[2 * e];
+            a.di[q] = mode == 0 ? src[2 * e + 1] : -src[2 * e + 1];
+        }
+    }
+}
+
+// x = op(D) v for the block whose inverse sits in a.d (v: its entries in LDS,
+// overwritten with x); x also goes to dst (LDS) and is published in pub.
+__device__ void gang_solve_block(const Prob &P, int j, const GangAhead &a, double2 *__restrict__ v,
+                                 double2 *__restrict__ dst, double2 (*__restrict__ red)[kTb],
+                                 unsigned long long *pub, unsigned epoch)
+{
+    const int n = P.n, j0 = j * kTb, jn = min(kTb, n - j0);
+    const int t = threadIdx.x, r = t & 63, g = t >> 6; // 16 column groups of 4
+    double2 acc = make_double2(0.0, 0.0);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const double2 xv = v[4 * g + q];
+        const double ar = a.dr[q], ai = a.di[q];
+        acc.x += ar * xv.x - ai * xv.y;
+        acc.y += ar * xv.y + ai * xv.x;
+    }
+    red[g][r] = acc;
+    __syncthreads();
+    if (t < kTb) {
+        double2 sum = red[0][t];
+        for (int q = 1; q < kGangWaves; ++q)
+            sum = cadd(sum, red[q][t]);
+        if (t >= jn)
+            sum = make_double2(0.0, 0.0);
+        v[t] = sum;
+        dst[t] = sum;
+        if (t < jn)
+            ll_publish(pub + (size_t)j0 * 4 + 4 * t, sum, epoch);
+    }
+    __syncthreads();
+}
+
+// v -= B(jn, j) x with B from a.b (one block, the look-ahead update).
+__device__ void gang_ahead_update(int mode, const GangAhead &a, const double2 *__restrict__ x,
+                                  double2 *__restrict__ v, double2 (*__restrict__ red)[kTb])
+{
+    const int t = threadIdx.x;
+    if (mode == 0) {
+        const int c = t >> 4, rq = t & 15, w = t >> 6, cl = c & 3;
+        const double2 xv = x[c];
+        double sr[4], si[4];
+        const float fr[4] = {a.br.x, a.br.y, a.br.z, a.br.w}, fi[4] = {a.bi.x, a.bi.y, a.bi.z, a.bi.w};
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            sr[q] = (double)fr[q] * xv.x - (double)fi[q] * xv.y;
+            si[q] = (double)fr[q] * xv.y + (double)fi[q] * xv.x;
+        }
+#pragma unroll
+        for (int d = 16; d <= 32; d <<= 1)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                sr[q] += __shfl_xor(sr[q], d);
+                si[q] += __shfl_xor(si[q], d);
+            }
+        // the 4 column lanes of a row quad hold equal sums; lane cl keeps row 4 rq + cl
+        const double vr = cl == 0 ? sr[0] : cl == 1 ? sr[1] : cl == 2 ? sr[2] : sr[3];
+        const double vi = cl == 0 ? si[0] : cl == 1 ? si[1] : cl == 2 ? si[2] : si[3];
+        red[w][4 * rq + cl] = make_double2(vr, vi);
+        __syncthreads();
+        if (t < kTb) {
+            double2 sum = red[0][t];
+            for (int q = 1; q < kGangWaves; ++q)
+                sum = cadd(sum, red[q][t]);
+            v[t] = make_double2(v[t].x - sum.x, v[t].y - sum.y);
+        }
+    } else {
+        const int r = t >> 4, c = 4 * (t & 15);
+        const double2 x0 = x[c], x1 = x[c + 1], x2 = x[c + 2], x3 = x[c + 3];
+        double sr = (double)a.br.x * x0.x + (double)a.bi.x * x0.y + (double)a.br.y * x1.x +
+                    (double)a.bi.y * x1.y + (double)a.br.z * x2.x + (double)a.bi.z * x2.y +
+                    (double)a.br.w * x3.x + (double)a.bi.w * x3.y;
+        double si = (double)a.br.x * x0.y - (double)a.bi.x * x0.x + (double)a.br.y * x1.y -
+                    (double)a.bi.y * x1.x + (double)a.br.z * x2.y - (double)a.bi.z * x2.x +
+                    (double)a.br.w * x3.y - (double)a.bi.w * x3.x;
+#pragma unroll
+        for (int d = 1; d <= 8; d <<= 1) {
+            sr += __shfl_xor(sr, d);
+            si += __shfl_xor(si, d);
+        }
+        if ((t & 15) == 0)
+            v[r] = make_double2(v[r].x - sr, v[r].y - si);
+    }
+    __syncthreads();
+}
+
+__device__ void zsolve_gang_body(const Prob &P, int mode, int m, int G, int epoch, double2 *__restrict__ own,
+                                 double2 (*__restrict__ red)[kTb], double2 *__restrict__ xa,
+                                 double2 *__restrict__ xn, int *s_ok)
+{
+    const int n = P.n, nb = (n + kTb - 1) / kTb;
+    const int t = threadIdx.x;
+    double2 *io = (double2 *)P.vv;
+    const int nown = (nb - m + G - 1) / G;
+    for (int e = t; e < nown * kTb; e += kSolve1Threads) {
+        const int i = (m + (e / kTb) * G) * kTb + e % kTb;
+        own[e] = i < n ? (mode == 0 ? io[P.perm[i]] : io[i]) : make_double2(0.0, 0.0);
+    }
+    __syncthreads();
+    GangAhead ahd;
+    for (int phase = 0; phase < 2; ++phase) {
+        const bool fwd = phase == 0;
+        const int which = (mode == 0) == fwd ? 0 : 1;
+        unsigned long long *pub = P.gx + (size_t)phase * nb * kTb * 4;
+        // owned local blocks beyond block j in the sweep direction: [ka, kb)
+        auto range = [&](int j, int &ka, int &kb) {
+            if (fwd) {
+                ka = j < m ? 0 : (j - m) / G + 1;
+                kb = nown;
+            } else {
+                ka = 0;
+                kb = j > m ? (j - m + G - 1) / G : 0;
+            }
+        };
+        auto next = [&](int j) { return fwd ? j + 1 : j - 1; };
+        auto mine = [&](int j) { return j >= 0 && j < nb && j % G == m; };
+        auto fetch = [&](int j) { // x_j published by another member -> xa
+            return ll_fetch(P, pub + (size_t)j * kTb * 4, min(kTb, n - j * kTb), (unsigned)epoch, xa, s_ok);
+        };
+        int j = fwd ? 0 : nb - 1;
+        bool have = false; // ahd holds the operands of the next look-ahead step
+        if (mine(j)) {
+            gang_prefetch(P, mode, which, j, j, ahd); // only the inverse is used
+            gang_solve_block(P, j, ahd, own + ((j - m) / G) * kTb, xa, red, pub, (unsigned)epoch);
+        } else {
+            if (mine(next(j))) {
+                gang_prefetch(P, mode, which, next(j), j, ahd);
+                have = true;
+            }
+            if (!fetch(j))
+                return;
+        }
+        for (int s = 0; s < nb; ++s) {
+            j = fwd ? s : nb - 1 - s;
+            int ka, kb;
+            range(j, ka, kb);
+            if (ka >= kb)
+                break; // nothing of this member lies beyond block j
+            const int jn = next(j); // in range: an owned block lies beyond j
+            if (mine(jn)) {
+                const int kn = (jn - m) / G; // the first owned block beyond j
+                if (!have)
+                    gang_prefetch(P, mode, which, jn, j, ahd);
+                have = false;
+                gang_ahead_update(mode, ahd, xa, own + kn * kTb, red);
+                gang_solve_block(P, jn, ahd, own + kn * kTb, xn, red, pub, (unsigned)epoch);
+                if (fwd)
+                    gang_sweep(P, mode, m, G, j, kn + 1, kb, xa, own, red);
+                else
+                    gang_sweep(P, mode, m, G, j, ka, kn, xa, own, red);
+                __syncthreads();
+                if (t < kTb)
+                    xa[t] = xn[t];
+                __syncthreads();
+            } else {
+                gang_sweep(P, mode, m, G, j, ka, kb, xa, own, red);
+                __syncthreads();
+                int na, nbk;
+                range(jn, na, nbk);
+                if (na >= nbk)
+                    break;
+                if (mine(next(jn))) {
+                    gang_prefetch(P, mode, which, next(jn), jn, ahd);
+                    have = true;
+                }
+                if (!fetch(jn))
+                    return;
+            }
+        }
+        __syncthreads();
+    }
+    for (int e = t; e < nown * kTb; e += kSolve1Threads) {
+        const int i = (m + (e / kTb) * G) * kTb + e % kTb;
+        if (i < n)
+            io[mode == 0 ? i : P.perm[i]] = own[e];
+    }
+}
+
+// Batched Z or Z^H solves. Work item (problem, member, G): G = 1 runs the
+// one-workgroup solve with the whole vector in LDS; G > 1 is one member of a
+// gang. Workgroups take items from an atomic ticket in list order; a gang's
+// members are consecutive, so every member a workgroup waits on has taken its
+// ticket earlier or will take the next ones while the waiting workgroups hold
+// fewer slots than the device has, and every wait is bounded.
+__global__ __launch_bounds__(kSolve1Threads) void k_zsolve(const Prob *__restrict__ probs,
+                                                           const int4 *__restrict__ items, int mode, int epoch,
+                                                           unsigned long long *__restrict__ ticket)
+{
+    extern __shared__ double2 vec[]; // G = 1: n entries; gang member: its owned blocks
+    __shared__ double2 part[kGangWaves][kTb];
+    __shared__ double2 xa[kTb], xn[kTb];
+    __shared__ int s_item, s_ok;
+    if (threadIdx.x == 0) {
+        s_item = (int)atomicAdd(ticket, 1ULL);
+        s_ok = 1;
+    }
+    __syncthreads();
+    const int4 it = items[s_item];
+    const Prob &P = probs[it.x];
+    if (P.info[0] != 0 || P.pstate[0] != 0)
+        return;
+    if (it.z == 1)
+        zsolve1_body(P, mode, vec, part, xa);
+    else
+        zsolve_gang_body(P, mode, it.y, it.z, epoch, vec, part, xa, xn, &s_ok);
 }
 
 // perm[i]: row of Z that the LU moved to row i (ipiv applied in order).

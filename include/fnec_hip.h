@@ -216,10 +216,16 @@ int fnec_hip_debug_mfma_probe_f32(const float *a16x4, const float *b4x16, float 
 
 /* ---- tunables (no counterpart in the reference) ----
  * Knobs of the batched engine, set by the autotuner (fnec-rust_amd autotune())
- * or by hand; they change speed only, never results of a given problem.
- *   "solve1_max_n"    CG solves of problems with n <= value run one workgroup
- *                     per problem (vector in LDS); larger n use the
- *                     flag-chained multi-workgroup solve. 0 .. 8192.
+ * or by hand. They change speed, and the rounding of a mixed-path solve
+ * within its accuracy contract (the summation order of the CG solves follows
+ * the gang size); a result never depends on what else is in the batch.
+ *   "solve1_max_n"    CG solves of problems with n <= value run in k_zsolve
+ *                     (one workgroup or one gang per problem); larger n use
+ *                     the flag-chained one-block-per-workgroup solve. 0 .. 8192.
+ *   "gang_n", "gang_max"  gang size of a CG solve:
+ *                     G = clamp(floor(n^2 / gang_n^2), 1, min(gang_max, ceil(n / 64)));
+ *                     gang_n = 0 keeps one workgroup per problem.
+ *                     gang_n 0 .. 65536, gang_max 1 .. 64.
  *   "mix_first_chunk" CG iterations launched before the first convergence
  *                     read-back. 1 .. 12.
  * set: FNEC_OK or FNEC_INVALID_ARG; get: the value, -1 for an unknown name. */

@@ -126,6 +126,35 @@ def test_mixed_batch_equals_single_bitwise():
         assert np.array_equal(a["x"], b["x"])
 
 
+@pytest.mark.parametrize("gang_n,gang_max", [(64, 16), (150, 3), (0, 16)])
+def test_gang_solves_match_plain(gang_n, gang_max):
+    """CG solves split over gangs of workgroups (forced onto small problems
+    through the tunables; gang_n = 0 keeps one workgroup per problem): the
+    result meets the mixed-path gates and a batch equals the single solves
+    bitwise, since the gang size depends on n only."""
+    probs = random_problems(13, [65, 130, 333, 700, 1100, 191])
+    old = F.get_tunable("gang_n"), F.get_tunable("gang_max")
+    try:
+        F.set_tunable("gang_n", gang_n)
+        F.set_tunable("gang_max", gang_max)
+        singles = []
+        for p in probs:
+            x64 = F.solve_problem(p, precision=F.PREC_FP64)
+            xm, st = F.solve_problem(p, precision=F.PREC_MIXED, with_stats=True)
+            fi = int(p["feed_index"])
+            assert abs(x64[fi] - xm[fi]) / abs(x64[fi]) <= 1e-6
+            assert rel(xm, x64) <= 1e-7
+            assert st["backward_error"] <= 1e-12
+            singles.append(xm)
+        batch = F.solve_batch(probs, precision=F.PREC_MIXED)
+        for x, b in zip(singles, batch):
+            assert b["status"] == F.OK
+            assert np.array_equal(x, b["x"])
+    finally:
+        F.set_tunable("gang_n", old[0])
+        F.set_tunable("gang_max", old[1])
+
+
 def test_config3_mixed_large_solve():
     """Configuration 3: 32 random wires x 256 segments (N = 8192, one KCL
     junction) at 7 MHz. Mixed vs the plain FP64 GPU solve of the same deck."""
