@@ -79,17 +79,24 @@ struct OwnedRequest {
 int run_batch(const fnec_request *const *reqs, size_t count, int precision, fnec_result *results,
               std::vector<std::vector<fnec_c64>> *xs, std::vector<std::vector<fnec_rp_row>> *rps)
 {
-    std::vector<HostProblem> hps(count);
+    // problem descriptions keep their capacity across batches (the engine
+    // runs one batch at a time under the global lock)
+    static std::vector<HostProblem> hps;
+    if (hps.size() < count)
+        hps.resize(count);
     std::vector<int> pre(count, FNEC_OK);
-    for (size_t i = 0; i < count; ++i) {
-        const fnec_request &r = *reqs[i];
-        results[i].tag = r.tag;
-        results[i].status = FNEC_OK;
-        pre[i] = problem_from_hallen(hps[i], r.segs, r.n, r.rhs, r.cos_vec, r.wire_first, r.wire_last,
-                                     r.n_wires, r.j_a, r.j_b, r.j_sign, r.n_junc, r.freq_hz, &r.ground);
-        if (!pre[i] && r.feed_index >= r.n) {
-            set_error("feed index outside the segment list");
-            pre[i] = FNEC_INVALID_ARG;
+    {
+        HostTimer ht(kHostPrep);
+        for (size_t i = 0; i < count; ++i) {
+            const fnec_request &r = *reqs[i];
+            results[i].tag = r.tag;
+            results[i].status = FNEC_OK;
+            pre[i] = problem_from_hallen(hps[i], r.segs, r.n, r.rhs, r.cos_vec, r.wire_first, r.wire_last,
+                                         r.n_wires, r.j_a, r.j_b, r.j_sign, r.n_junc, r.freq_hz, &r.ground);
+            if (!pre[i] && r.feed_index >= r.n) {
+                set_error("feed index outside the segment list");
+                pre[i] = FNEC_INVALID_ARG;
+            }
         }
     }
     std::vector<const HostProblem *> ok;
@@ -110,70 +117,91 @@ int run_batch(const fnec_request *const *reqs, size_t count, int precision, fnec
         xloc[q].resize(ok[q]->s);
         xo[q] = xloc[q].data();
     }
-    std::vector<fnec_solve_stats> stats;
-    std::vector<int> status;
-    std::vector<double *> dev_x;
-    int st = solve_problems(ok, xo, stats, status, &dev_x, precision);
-    if (st)
-        return st;
-    // far-field jobs for solved requests that asked for a grid
+    // far-field jobs read the currents in place on the device; their inputs
+    // are prepared and uploaded while the device works on the solve
     std::vector<RpHost> jobs;
-    std::vector<size_t> job_of;
-    std::vector<std::vector<fnec_rp_row>> rpbuf;
-    for (size_t q = 0; q < ok.size(); ++q) {
-        const size_t i = idx[q];
-        const fnec_request &r = *reqs[i];
-        fnec_result &res = results[i];
-        res.status = status[q];
-        res.stats = stats[q];
-        if (status[q])
-            continue;
-        const fnec_c64 cur = xloc[q][r.feed_index];
-        res.feed_current = cur;
-        // Z = V / I when |I| > 1e-60, else V (nec_worker/src/solve.rs:254-260)
-        if (std::hypot(cur.re, cur.im) > 1e-60) {
-            const double ns = cur.re * cur.re + cur.im * cur.im;
-            res.z_in.re = (r.v_feed.re * cur.re + r.v_feed.im * cur.im) / ns;
-            res.z_in.im = (r.v_feed.im * cur.re - r.v_feed.re * cur.im) / ns;
-        } else {
-            res.z_in = r.v_feed;
-        }
-        if (res.x)
-            std::memcpy(res.x, xloc[q].data(), sizeof(fnec_c64) * xloc[q].size());
-        if (xs)
-            (*xs)[i] = xloc[q];
-        if (r.rp_npts && r.rp_theta && r.rp_phi) {
+    std::vector<size_t> job_q;
+    std::vector<std::pair<size_t, int>> job_err; // (q, status) of grids that could not be set up
+    RpStaged staged;
+    int rp_st = FNEC_OK;
+    auto overlap = [&](const std::vector<double *> &dev_x) {
+        HostTimer ht(kHostRp);
+        for (size_t q = 0; q < ok.size(); ++q) {
+            const fnec_request &r = *reqs[idx[q]];
+            if (!(r.rp_npts && r.rp_theta && r.rp_phi))
+                continue;
             RpHost h;
             h.cur_dev = dev_x[q];
-            int e = rp_host_from(h, r.segs, nullptr, r.n, r.freq_hz, &r.ground, r.rp_theta, r.rp_phi,
-                                 r.rp_npts);
+            const int e = rp_host_from(h, r.segs, nullptr, r.n, r.freq_hz, &r.ground, r.rp_theta, r.rp_phi,
+                                       r.rp_npts);
             if (e) {
-                res.status = e;
+                job_err.push_back({q, e});
                 continue;
             }
             jobs.push_back(std::move(h));
-            job_of.push_back(i);
+            job_q.push_back(q);
         }
+        if (!jobs.empty())
+            rp_st = rp_stage(jobs, staged);
+    };
+    std::vector<fnec_solve_stats> stats;
+    std::vector<int> status;
+    std::vector<double *> dev_x;
+    int st = solve_problems(ok, xo, stats, status, &dev_x, precision, overlap);
+    if (st)
+        return st;
+    if (rp_st)
+        return rp_st;
+    {
+        HostTimer ht(kHostPost);
+        for (size_t q = 0; q < ok.size(); ++q) {
+            const size_t i = idx[q];
+            const fnec_request &r = *reqs[i];
+            fnec_result &res = results[i];
+            res.status = status[q];
+            res.stats = stats[q];
+            if (status[q])
+                continue;
+            const fnec_c64 cur = xloc[q][r.feed_index];
+            res.feed_current = cur;
+            // Z = V / I when |I| > 1e-60, else V (nec_worker/src/solve.rs:254-260)
+            if (std::hypot(cur.re, cur.im) > 1e-60) {
+                const double ns = cur.re * cur.re + cur.im * cur.im;
+                res.z_in.re = (r.v_feed.re * cur.re + r.v_feed.im * cur.im) / ns;
+                res.z_in.im = (r.v_feed.im * cur.re - r.v_feed.re * cur.im) / ns;
+            } else {
+                res.z_in = r.v_feed;
+            }
+            if (res.x)
+                std::memcpy(res.x, xloc[q].data(), sizeof(fnec_c64) * xloc[q].size());
+            if (xs)
+                (*xs)[i] = std::move(xloc[q]);
+        }
+        for (const auto &qe : job_err)
+            if (!status[qe.first])
+                results[idx[qe.first]].status = qe.second;
     }
     if (!jobs.empty()) {
-        rpbuf.resize(jobs.size());
-        std::vector<fnec_rp_row *> outs(jobs.size());
+        // rows are read back only into the caller's buffers (or the internal
+        // copies a caller asked for); a failed solve's pattern is dropped
+        std::vector<fnec_rp_row *> outs(jobs.size(), nullptr);
+        if (rps)
+            for (size_t j = 0; j < jobs.size(); ++j)
+                if (!status[job_q[j]])
+                    (*rps)[idx[job_q[j]]].resize(jobs[j].npts);
         for (size_t j = 0; j < jobs.size(); ++j) {
-            rpbuf[j].resize(jobs[j].theta.size());
-            outs[j] = rpbuf[j].data();
+            const size_t i = idx[job_q[j]];
+            if (status[job_q[j]])
+                continue;
+            outs[j] = results[i].rp ? results[i].rp : (rps ? (*rps)[i].data() : nullptr);
         }
         std::vector<double> totals;
-        st = rp_run(jobs, outs, totals);
+        st = rp_finish(staged, outs, totals);
         if (st)
             return st;
-        for (size_t j = 0; j < jobs.size(); ++j) {
-            fnec_result &res = results[job_of[j]];
-            res.rp_total = totals[j];
-            if (res.rp)
-                std::memcpy(res.rp, rpbuf[j].data(), sizeof(fnec_rp_row) * rpbuf[j].size());
-            if (rps)
-                (*rps)[job_of[j]] = std::move(rpbuf[j]);
-        }
+        for (size_t j = 0; j < jobs.size(); ++j)
+            if (!status[job_q[j]])
+                results[idx[job_q[j]]].rp_total = totals[j];
     }
     return FNEC_OK;
 }

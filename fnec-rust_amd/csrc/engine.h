@@ -1,6 +1,7 @@
 // Internal interface between the C ABI (abi.hip) and the engine (engine.hip).
 #pragma once
 
+#include <functional>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -23,12 +24,43 @@ struct HostProblem {
     std::vector<double> cons_sign;
 };
 
-// Host-side description of one far-field job.
+// Host-side description of one far-field job. The grid is borrowed from
+// the caller for the duration of the call that runs the job.
 struct RpHost {
     int n = 0, ground = 0, ndir = 0;
     double k = 0.0, eps_r = 0.0, sigma = 0.0;
-    std::vector<double> seg, cur, theta, phi;
+    std::vector<double> seg, cur;
+    const double *theta = nullptr, *phi = nullptr;
+    size_t npts = 0;
     const double *cur_dev = nullptr; // currents already on the device
+};
+
+// A batch of far-field jobs whose inputs are on their way to the device
+// (rp_stage), waiting for its kernels and read-back (rp_finish).
+struct RpStaged {
+    size_t nj = 0;
+    char *base = nullptr;
+    std::vector<size_t> o_tot, o_out, npts;
+    size_t out_begin = 0, rows_begin = 0, out_end = 0;
+    size_t pts_off = 0, pts_cnt = 0, norm_off = 0, norm_cnt = 0, all_off = 0;
+    size_t o_probs = 0, o_ints = 0;
+    double rp_terms = 0.0;
+};
+
+// Host-phase timing for the per-class profile (a no-op while profiling is
+// off): problem and plan building, staging copies, far-field preparation,
+// result processing, and time blocked on the stream.
+enum { kHostPrep = 12, kHostStage, kHostRp, kHostPost, kHostWait };
+class HostTimer {
+  public:
+    explicit HostTimer(int cls);
+    ~HostTimer();
+    HostTimer(const HostTimer &) = delete;
+    HostTimer &operator=(const HostTimer &) = delete;
+
+  private:
+    int cls_;
+    double t0_;
 };
 
 void set_error(const std::string &e);
@@ -49,13 +81,22 @@ int problem_from_paths(HostProblem &hp, const fnec_seg *segs, size_t n, const fn
                        const double *cos_vec, const size_t *path_of, const size_t *free_ends,
                        size_t nfree, double freq_hz, const fnec_ground *ground);
 
+// overlap (optional) runs once on the host while the device works on the
+// solve (after the factorisation and the first iterations are queued, before
+// the host first waits), with the device addresses of the solution vectors.
 int solve_problems(const std::vector<const HostProblem *> &hps, std::vector<fnec_c64 *> &x_out,
                    std::vector<fnec_solve_stats> &stats, std::vector<int> &status,
-                   std::vector<double *> *device_x, int precision);
+                   std::vector<double *> *device_x, int precision,
+                   const std::function<void(const std::vector<double *> &)> &overlap = {});
 int fill_only(const HostProblem &hp, fnec_c64 *z_out);
 int normal_eqs_only(const HostProblem &hp, fnec_c64 *aug_out);
 int rp_host_from(RpHost &h, const fnec_seg *segs, const fnec_c64 *cur, size_t n, double freq_hz,
                  const fnec_ground *ground, const double *theta, const double *phi, size_t npts);
+// Far-field jobs in one batch: rp_stage uploads the inputs on the copy
+// stream; rp_finish runs the kernels after the solve stream's work and reads
+// back the totals, and the rows of the jobs whose outs[j] is not null.
+int rp_stage(const std::vector<RpHost> &jobs, RpStaged &st);
+int rp_finish(RpStaged &st, const std::vector<fnec_rp_row *> &outs, std::vector<double> &totals);
 int rp_run(const std::vector<RpHost> &jobs, std::vector<fnec_rp_row *> &outs,
            std::vector<double> &totals);
 int mfma_probe(const double *a, const double *b, double *out);

@@ -8,6 +8,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <functional>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -83,9 +84,13 @@ struct Context {
     // get a device arena of their own
     char *rp_arena = nullptr;
     size_t rp_arena_bytes = 0;
-    // pinned host staging for the batch upload and the result read-back
-    char *h_up = nullptr, *h_back = nullptr;
-    size_t h_up_bytes = 0, h_back_bytes = 0;
+    // pinned host staging for the batch upload and the result read-back, and
+    // for far-field inputs staged while a solve runs
+    char *h_up = nullptr, *h_back = nullptr, *h_rp = nullptr;
+    size_t h_up_bytes = 0, h_back_bytes = 0, h_rp_bytes = 0;
+    // uploads that overlap the solve stream's kernels (default priority)
+    hipStream_t copy_stream = nullptr;
+    hipEvent_t ev_rp = nullptr;
 };
 
 static Context g_ctx;
@@ -100,11 +105,15 @@ struct ProfPending {
     hipEvent_t a, b;
     double flops, bytes;
 };
+// Classes from PC_HOST_PREP on are host-side phases timed with a steady clock
+// (launches = occurrences, no flops or bytes).
 enum { PC_FILL, PC_GRAM, PC_LU_PANEL, PC_LU_TRSM, PC_LU_UPDATE, PC_BACKSUB, PC_RP, PC_ZSOLVE,
-       PC_MV, PC_PCG, PC_FSETUP, PC_LU_UPDN, PC_N };
+       PC_MV, PC_PCG, PC_FSETUP, PC_LU_UPDN, PC_HOST_PREP, PC_HOST_STAGE, PC_HOST_RP, PC_HOST_POST,
+       PC_HOST_WAIT, PC_N };
 static const char *kProfNames[PC_N] = {"fill", "gram", "lu_panel", "lu_trsm", "lu_update",
                                        "backsub", "rp", "zsolve", "matvec", "pcg_ctl", "fsetup",
-                                       "lu_update_narrow"};
+                                       "lu_update_narrow", "host_prep", "host_stage", "host_rp",
+                                       "host_post", "host_wait"};
 static bool g_prof = false;
 static ProfClass g_prof_acc[PC_N];
 static std::vector<ProfPending> g_prof_pending;
@@ -123,6 +132,24 @@ static hipEvent_t prof_event()
 }
 
 static int g_prof_only = -1; // record only this class (-1: every class)
+
+static_assert(PC_HOST_PREP == kHostPrep && PC_HOST_WAIT == kHostWait, "host classes");
+
+static double host_now_us()
+{
+    using namespace std::chrono;
+    return (double)duration_cast<nanoseconds>(steady_clock::now().time_since_epoch()).count() * 1e-3;
+}
+
+HostTimer::HostTimer(int cls) : cls_(cls), t0_(g_prof ? host_now_us() : 0.0) {}
+HostTimer::~HostTimer()
+{
+    if (!g_prof || (g_prof_only >= 0 && cls_ != g_prof_only))
+        return;
+    ProfClass &c = g_prof_acc[cls_];
+    c.launches += 1;
+    c.total_us += host_now_us() - t0_;
+}
 
 static hipEvent_t prof_begin(int cls)
 {
@@ -229,7 +256,8 @@ static int ctx_init()
         g_ctx.status = FNEC_UNAVAILABLE;
         return g_ctx.status;
     }
-    if (hipSetDevice(g_ctx.device) != hipSuccess || hipStreamCreate(&g_ctx.stream) != hipSuccess) {
+    if (hipSetDevice(g_ctx.device) != hipSuccess || hipStreamCreate(&g_ctx.stream) != hipSuccess ||
+        hipStreamCreate(&g_ctx.copy_stream) != hipSuccess || hipEventCreate(&g_ctx.ev_rp) != hipSuccess) {
         set_error("device initialisation failed");
         g_ctx.status = FNEC_INTERNAL;
         return g_ctx.status;
@@ -261,6 +289,7 @@ static char *pinned(char *&buf, size_t &have, size_t bytes)
         return buf;
     if (buf) {
         (void)hipStreamSynchronize(g_ctx.stream);
+        (void)hipStreamSynchronize(g_ctx.copy_stream);
         (void)hipHostFree(buf);
         buf = nullptr;
         have = 0;
@@ -280,6 +309,7 @@ static int grow_arena(char *&arena, size_t &have, size_t bytes, char **out)
     if (bytes > have) {
         if (arena) {
             HIPCHK(hipStreamSynchronize(g_ctx.stream));
+            HIPCHK(hipStreamSynchronize(g_ctx.copy_stream));
             HIPCHK(hipFree(arena));
             arena = nullptr;
             have = 0;
@@ -558,6 +588,7 @@ struct Plan {
         double f_panel, f_trsm, f_ua, b_ua, f_ou, b_ou;
     };
     std::vector<Outer> outer32;
+    std::function<void()> overlap; // host work run once while the device computes
     std::vector<LuDesc<float>> lud32;
     size_t off_lud32 = 0;
     size_t fsol_off = 0, fsol_cnt = 0, vsol_off = 0, vsol_cnt = 0, mvz_off = 0, mvz_cnt = 0;
@@ -1461,7 +1492,15 @@ static int mixed_launch(Plan &pl, char *base)
             break;
         hipLaunchKernelGGL(k_mix_flags, dim3((np + 255) / 256), dim3(256), 0, st, dp, np, flags_dev);
         HIPCHK(hipMemcpyAsync(flags.data(), flags_dev, 4 * np, hipMemcpyDeviceToHost, st));
-        HIPCHK(hipStreamSynchronize(st));
+        if (pl.overlap) {
+            auto f = std::move(pl.overlap);
+            pl.overlap = nullptr;
+            f();
+        }
+        {
+            HostTimer ht(kHostWait);
+            HIPCHK(hipStreamSynchronize(st));
+        }
         bool all_done = true;
         for (unsigned b = 0; b < np; ++b)
             all_done = all_done && flags[b] != 0;
@@ -1560,7 +1599,8 @@ static float ev_ms(int a, int b)
 // Solve a batch of problems; x_out[b] receives s complex values (may be null).
 int solve_problems(const std::vector<const HostProblem *> &hps, std::vector<fnec_c64 *> &x_out,
                    std::vector<fnec_solve_stats> &stats, std::vector<int> &status,
-                   std::vector<double *> *device_x, int precision)
+                   std::vector<double *> *device_x, int precision,
+                   const std::function<void(const std::vector<double *> &)> &overlap)
 {
     const bool mixed = precision == FNEC_PREC_MIXED;
     if (mixed)
@@ -1573,17 +1613,40 @@ int solve_problems(const std::vector<const HostProblem *> &hps, std::vector<fnec
             }
     Plan pl;
     pl.mixed = mixed;
-    plan_build(pl, hps, true);
+    // the work lists keep their capacity across batches
+    static std::vector<int> ints_cache;
+    pl.ints.swap(ints_cache);
+    struct IntsBack {
+        Plan &p;
+        ~IntsBack() { p.ints.swap(ints_cache); }
+    } ints_back{pl};
+    {
+        HostTimer ht(kHostPrep);
+        plan_build(pl, hps, true);
+    }
     char *base = nullptr;
     int st = arena_reserve(pl.bytes, &base);
     if (st)
         return st;
-    st = plan_upload(pl, hps, base, true);
+    {
+        HostTimer ht(kHostStage);
+        st = plan_upload(pl, hps, base, true);
+    }
     if (st)
         return st;
+    std::vector<double *> dev_x(hps.size());
+    for (size_t b = 0; b < hps.size(); ++b)
+        dev_x[b] = (double *)(base + pl.off_x[b]);
+    if (overlap)
+        pl.overlap = [&]() { overlap(dev_x); };
     st = plan_launch(pl, base, true);
     if (st)
         return st;
+    if (pl.overlap) {
+        auto f = std::move(pl.overlap);
+        pl.overlap = nullptr;
+        f();
+    }
     // all result words in one copy
     const size_t nback = pl.io_end - pl.io_begin;
     char *hb = pinned(g_ctx.h_back, g_ctx.h_back_bytes, nback);
@@ -1592,7 +1655,11 @@ int solve_problems(const std::vector<const HostProblem *> &hps, std::vector<fnec
         return FNEC_RESOURCE_EXHAUSTED;
     }
     HIPCHK(hipMemcpyAsync(hb, base + pl.io_begin, nback, hipMemcpyDeviceToHost, g_ctx.stream));
-    HIPCHK(hipStreamSynchronize(g_ctx.stream));
+    {
+        HostTimer ht(kHostWait);
+        HIPCHK(hipStreamSynchronize(g_ctx.stream));
+    }
+    HostTimer ht_post(kHostPost);
     auto back = [&](size_t off) { return hb + (off - pl.io_begin); };
     if (mixed && g_prof) {
         // the work of CG iteration i is done only by problems that ran it
@@ -1652,7 +1719,7 @@ int solve_problems(const std::vector<const HostProblem *> &hps, std::vector<fnec
         if (x_out.size() > b && x_out[b] && status[b] == FNEC_OK)
             std::memcpy(x_out[b], back(pl.off_x[b]), 16 * (size_t)hps[b]->s);
         if (device_x)
-            device_x->push_back((double *)(base + pl.off_x[b]));
+            device_x->push_back(dev_x[b]);
     }
     return FNEC_OK;
 }
@@ -1724,64 +1791,69 @@ int normal_eqs_only(const HostProblem &hp, fnec_c64 *aug_out)
 // far field
 // ---------------------------------------------------------------------------
 
-int rp_run(const std::vector<RpHost> &jobs, std::vector<fnec_rp_row *> &outs, std::vector<double> &totals)
+int rp_stage(const std::vector<RpHost> &jobs, RpStaged &S)
 {
+    HostTimer ht(kHostRp);
     const size_t nj = jobs.size();
+    S = RpStaged{};
+    S.nj = nj;
     size_t cur = 0;
-    std::vector<size_t> o_seg(nj), o_cur(nj), o_th(nj), o_ph(nj), o_field(nj), o_part(nj),
-        o_tot(nj), o_out(nj);
-    std::vector<int> ints;
-    size_t pts_off, norm_off, all_off;
+    std::vector<size_t> o_seg(nj), o_cur(nj), o_th(nj), o_ph(nj), o_field(nj), o_part(nj);
+    S.o_tot.resize(nj);
+    S.o_out.resize(nj);
+    S.npts.resize(nj);
+    static std::vector<int> ints; // capacity kept across batches
+    ints.clear();
     // device-only work buffers
     for (size_t j = 0; j < nj; ++j) {
         const RpHost &h = jobs[j];
-        o_field[j] = take(cur, 32 * h.theta.size() + 32);
+        S.npts[j] = h.npts;
+        o_field[j] = take(cur, 32 * h.npts + 32);
         o_part[j] = take(cur, 8 * (size_t)((h.ndir + 255) / 256) + 8);
     }
-    pts_off = ints.size();
+    S.pts_off = ints.size();
     for (size_t j = 0; j < nj; ++j)
-        for (int y = 0; y < ((int)jobs[j].theta.size() + 255) / 256; ++y) {
+        for (int y = 0; y < ((int)jobs[j].npts + 255) / 256; ++y) {
             ints.push_back((int)j);
             ints.push_back(y);
         }
-    size_t pts_cnt = (ints.size() - pts_off) / 2;
-    norm_off = ints.size();
+    S.pts_cnt = (ints.size() - S.pts_off) / 2;
+    S.norm_off = ints.size();
     for (size_t j = 0; j < nj; ++j)
         for (int y = 0; y < (jobs[j].ndir + 255) / 256; ++y) {
             ints.push_back((int)j);
             ints.push_back(y);
         }
-    size_t norm_cnt = (ints.size() - norm_off) / 2;
-    all_off = ints.size();
+    S.norm_cnt = (ints.size() - S.norm_off) / 2;
+    S.all_off = ints.size();
     for (size_t j = 0; j < nj; ++j)
         ints.push_back((int)j);
-    // inputs (one staged copy), then outputs (one read-back)
+    // inputs (one staged copy), then outputs: the totals, then the rows
     const size_t in_begin = cur = (cur + 255) & ~(size_t)255;
     for (size_t j = 0; j < nj; ++j) {
         const RpHost &h = jobs[j];
         o_seg[j] = take(cur, 8 * h.seg.size());
         o_cur[j] = h.cur_dev ? 0 : take(cur, 8 * h.cur.size());
-        o_th[j] = take(cur, 8 * h.theta.size() + 8);
-        o_ph[j] = take(cur, 8 * h.phi.size() + 8);
+        o_th[j] = take(cur, 8 * h.npts + 8);
+        o_ph[j] = take(cur, 8 * h.npts + 8);
     }
-    size_t o_probs = take(cur, sizeof(RpProb) * nj);
-    size_t o_ints = take(cur, 4 * ints.size() + 16);
-    const size_t out_begin = cur = (cur + 255) & ~(size_t)255;
+    S.o_probs = take(cur, sizeof(RpProb) * nj);
+    S.o_ints = take(cur, 4 * ints.size() + 16);
+    S.out_begin = cur = (cur + 255) & ~(size_t)255;
+    for (size_t j = 0; j < nj; ++j)
+        S.o_tot[j] = S.out_begin + 16 * j;
+    S.rows_begin = cur = (S.out_begin + 16 * nj + 255) & ~(size_t)255;
     for (size_t j = 0; j < nj; ++j) {
-        o_tot[j] = (cur + 15) & ~(size_t)15;
-        cur = o_tot[j] + 16;
-        o_out[j] = cur;
-        cur += 32 * jobs[j].theta.size();
+        S.o_out[j] = cur;
+        cur += 32 * jobs[j].npts;
     }
-    const size_t out_end = cur = (cur + 255) & ~(size_t)255;
-    size_t total = cur + 256;
-    char *base = nullptr;
-    int st = grow_arena(g_ctx.rp_arena, g_ctx.rp_arena_bytes, total, &base);
+    S.out_end = cur = (cur + 255) & ~(size_t)255;
+    int st = grow_arena(g_ctx.rp_arena, g_ctx.rp_arena_bytes, cur + 256, &S.base);
     if (st)
         return st;
-    hipStream_t s = g_ctx.stream;
+    char *base = S.base;
     std::vector<RpProb> probs(nj);
-    char *hs = pinned(g_ctx.h_up, g_ctx.h_up_bytes, out_begin - in_begin);
+    char *hs = pinned(g_ctx.h_rp, g_ctx.h_rp_bytes, S.out_begin - in_begin);
     if (!hs) {
         set_error("pinned host memory exhausted");
         return FNEC_RESOURCE_EXHAUSTED;
@@ -1795,7 +1867,7 @@ int rp_run(const std::vector<RpHost> &jobs, std::vector<fnec_rp_row *> &outs, st
         RpProb &R = probs[j];
         std::memset(&R, 0, sizeof(R));
         R.n = h.n;
-        R.npts = (int)h.theta.size();
+        R.npts = (int)h.npts;
         R.ground = h.ground;
         R.ndir = h.ndir;
         R.k = h.k;
@@ -1808,46 +1880,76 @@ int rp_run(const std::vector<RpHost> &jobs, std::vector<fnec_rp_row *> &outs, st
         R.field = (double *)(base + o_field[j]);
         R.partial = (double *)(base + o_part[j]);
         R.npartial = (h.ndir + 255) / 256;
-        R.total = (double *)(base + o_tot[j]);
-        R.out = (double *)(base + o_out[j]);
+        R.total = (double *)(base + S.o_tot[j]);
+        R.out = (double *)(base + S.o_out[j]);
         put(o_seg[j], h.seg.data(), 8 * h.seg.size());
         if (!h.cur_dev)
             put(o_cur[j], h.cur.data(), 8 * h.cur.size());
-        put(o_th[j], h.theta.data(), 8 * h.theta.size());
-        put(o_ph[j], h.phi.data(), 8 * h.phi.size());
+        put(o_th[j], h.theta, 8 * h.npts);
+        put(o_ph[j], h.phi, 8 * h.npts);
+        S.rp_terms += (double)h.n * (h.ground ? 2.0 : 1.0) * ((double)h.npts + h.ndir);
     }
-    put(o_probs, probs.data(), sizeof(RpProb) * nj);
-    put(o_ints, ints.data(), 4 * ints.size());
-    HIPCHK(hipMemcpyAsync(base + in_begin, hs, out_begin - in_begin, hipMemcpyHostToDevice, s));
-    const RpProb *dp = (const RpProb *)(base + o_probs);
-    const int *wi = (const int *)(base + o_ints);
-    double rp_terms = 0;
-    for (const auto &h : jobs)
-        rp_terms += (double)h.n * (h.ground ? 2.0 : 1.0) * ((double)h.theta.size() + h.ndir);
+    put(S.o_probs, probs.data(), sizeof(RpProb) * nj);
+    put(S.o_ints, ints.data(), 4 * ints.size());
+    // the copy stream carries the upload so that it overlaps the solve's kernels
+    HIPCHK(hipMemcpyAsync(base + in_begin, hs, S.out_begin - in_begin, hipMemcpyHostToDevice, g_ctx.copy_stream));
+    HIPCHK(hipEventRecord(g_ctx.ev_rp, g_ctx.copy_stream));
+    return FNEC_OK;
+}
+
+int rp_finish(RpStaged &S, const std::vector<fnec_rp_row *> &outs, std::vector<double> &totals)
+{
+    const size_t nj = S.nj;
+    totals.assign(nj, 0.0);
+    if (!nj)
+        return FNEC_OK;
+    hipStream_t s = g_ctx.stream;
+    char *base = S.base;
+    HIPCHK(hipStreamWaitEvent(s, g_ctx.ev_rp, 0));
+    const RpProb *dp = (const RpProb *)(base + S.o_probs);
+    const int *wi = (const int *)(base + S.o_ints);
     hipEvent_t e_rp = prof_begin(PC_RP);
-    if (pts_cnt)
-        hipLaunchKernelGGL(k_rp_points, dim3((unsigned)pts_cnt), dim3(256), 0, s, dp, (const int2 *)(wi + pts_off));
-    if (norm_cnt)
-        hipLaunchKernelGGL(k_rp_norm, dim3((unsigned)norm_cnt), dim3(256), 0, s, dp, (const int2 *)(wi + norm_off));
+    if (S.pts_cnt)
+        hipLaunchKernelGGL(k_rp_points, dim3((unsigned)S.pts_cnt), dim3(256), 0, s, dp,
+                           (const int2 *)(wi + S.pts_off));
+    if (S.norm_cnt)
+        hipLaunchKernelGGL(k_rp_norm, dim3((unsigned)S.norm_cnt), dim3(256), 0, s, dp,
+                           (const int2 *)(wi + S.norm_off));
     // RP unit: one (direction, segment[, image]) term = 32 flops + sin + cos
-    prof_end(PC_RP, e_rp, rp_terms * 34.0, 0);
-    hipLaunchKernelGGL(k_rp_finish, dim3((unsigned)nj), dim3(256), 0, s, dp, wi + all_off);
+    prof_end(PC_RP, e_rp, S.rp_terms * 34.0, 0);
+    hipLaunchKernelGGL(k_rp_finish, dim3((unsigned)nj), dim3(256), 0, s, dp, wi + S.all_off);
     HIPCHK(hipGetLastError());
-    char *hb = pinned(g_ctx.h_back, g_ctx.h_back_bytes, out_end - out_begin);
+    bool rows = false;
+    for (size_t j = 0; j < nj; ++j)
+        rows = rows || (outs.size() > j && outs[j] && S.npts[j]);
+    const size_t end = rows ? S.out_end : S.rows_begin;
+    char *hb = pinned(g_ctx.h_back, g_ctx.h_back_bytes, end - S.out_begin);
     if (!hb) {
         set_error("pinned host memory exhausted");
         return FNEC_RESOURCE_EXHAUSTED;
     }
-    HIPCHK(hipMemcpyAsync(hb, base + out_begin, out_end - out_begin, hipMemcpyDeviceToHost, s));
-    HIPCHK(hipStreamSynchronize(s));
+    HIPCHK(hipMemcpyAsync(hb, base + S.out_begin, end - S.out_begin, hipMemcpyDeviceToHost, s));
+    {
+        HostTimer ht(kHostWait);
+        HIPCHK(hipStreamSynchronize(s));
+    }
+    HostTimer ht_post(kHostPost);
     prof_collect();
-    totals.assign(nj, 0.0);
     for (size_t j = 0; j < nj; ++j) {
-        totals[j] = *(const double *)(hb + (o_tot[j] - out_begin));
-        if (outs[j] && !jobs[j].theta.empty())
-            std::memcpy(outs[j], hb + (o_out[j] - out_begin), 32 * jobs[j].theta.size());
+        totals[j] = *(const double *)(hb + (S.o_tot[j] - S.out_begin));
+        if (outs.size() > j && outs[j] && S.npts[j])
+            std::memcpy(outs[j], hb + (S.o_out[j] - S.out_begin), 32 * S.npts[j]);
     }
     return FNEC_OK;
+}
+
+int rp_run(const std::vector<RpHost> &jobs, std::vector<fnec_rp_row *> &outs, std::vector<double> &totals)
+{
+    RpStaged S;
+    int st = rp_stage(jobs, S);
+    if (st)
+        return st;
+    return rp_finish(S, outs, totals);
 }
 
 int rp_host_from(RpHost &h, const fnec_seg *segs, const fnec_c64 *cur, size_t n, double freq_hz,
@@ -1879,8 +1981,9 @@ int rp_host_from(RpHost &h, const fnec_seg *segs, const fnec_c64 *cur, size_t n,
             h.cur[2 * i + 1] = cur[i].im;
         }
     }
-    h.theta.assign(theta, theta + npts);
-    h.phi.assign(phi, phi + npts);
+    h.theta = theta;
+    h.phi = phi;
+    h.npts = npts;
     h.ground = 0;
     h.eps_r = h.sigma = 0.0;
     if (ground && ground->kind == FNEC_GROUND_PEC)

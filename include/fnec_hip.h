@@ -238,7 +238,11 @@ long fnec_hip_get_tunable(const char *name);
  * FP32 factors), "matvec" (M p and M^H t in FP64), "pcg_ctl", "fsetup"
  * (Z^H solves of the constraint rows), "lu_update_narrow" (the K = 32 update
  * of the second 32-column panel inside each 64-column step of the FP32 LU;
- * "lu_update" is then the K = 64 trailing update). out4 = {launches, total microseconds, algorithmic flops,
+ * "lu_update" is then the K = 64 trailing update), and the host-side phases
+ * "host_prep" (problem and plan building), "host_stage" (staging copies),
+ * "host_rp" (far-field set-up), "host_post" (result processing) and
+ * "host_wait" (time blocked on the stream), timed with a steady clock (flops
+ * and bytes 0). out4 = {launches, total microseconds, algorithmic flops,
  * algorithmic bytes} accumulated since the last reset. */
 void fnec_hip_profile_enable(int on);
 /* Restrict event recording to one kernel class (NULL or "": every class), so
