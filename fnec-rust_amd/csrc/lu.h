@@ -537,10 +537,12 @@ __global__ __launch_bounds__(256) void k_lu_update(const LuDesc<T> *__restrict__
 
 // FP32 trailing update for the mixed path: A22 -= L21 U12 with TM x TN
 // output tiles. Four waves in a 2 x 2 arrangement each own a TM/2 x TN/2
-// block of MFMA 16x16x4 tiles; the accumulators start from C (float4 loads in
-// the accumulator's own row layout: 4 consecutive rows per lane) and the A
-// operand enters negated, so the MFMA chain produces C - L21 U12 directly and
-// C is read and written once per launch whatever K. The update covers rows
+// block of MFMA 16x16x4 tiles; the A operand enters negated, so the MFMA
+// chain produces -L21 U12, and C (float4 loads in the accumulator's own row
+// layout: 4 consecutive rows per lane) is added in the epilogue; C is read and
+// written once per launch whatever K. The loads of C and of the next K
+// chunk's operands are issued before the current chunk's products, so their
+// latency overlaps the MFMA work. The update covers rows
 // [rbeg, m) x columns [cbeg, min(ncols, cend)) with L21 = columns
 // [kbeg, kbeg + 32 nk); those panels are full (a partial panel is always the
 // last one and has no rows below it).
@@ -556,6 +558,8 @@ __global__ __launch_bounds__(256) void k_lu_update_f32(const LuDesc<float> *__re
     constexpr int WM = TM / 2, WN = TN / 2;       // per-wave block (2 x 2 waves)
     constexpr int AM = WM / 16, AN = WN / 16;     // MFMA tiles per wave
     constexpr int LA = TM + 4, LB = TN + 4;
+    constexpr int NA = (kPanel * TM / 4) / 256;   // float4 of A per thread and chunk
+    constexpr int NB = (kPanel * TN / 4) / 256;   // float4 of B per thread and chunk
     const int4 wk = work[blockIdx.x];
     const LuDesc<float> &D = descs[wk.x];
     const int s = D.m, nc = min(D.ncols, cend);
@@ -568,81 +572,105 @@ __global__ __launch_bounds__(256) void k_lu_update_f32(const LuDesc<float> *__re
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const int wr = wave >> 1, wc = wave & 1;
     const float *ar = D.ar, *ai = D.ai;
-    // accumulators = C
-    typename MF::acc accr[AM][AN], acci[AM][AN];
     const int lrow = lane & 15, lk = lane >> 4;
+    // operands of one K chunk in registers: A column k0 + k, rows r0 .. (TM / 4
+    // float4 per column); B column c0 + c, rows k0 .. k0 + 31 (8 float4 per column)
+    float4 va_r[NA], va_i[NA], vb_r[NB], vb_i[NB];
+    auto load_ab = [&](int k0) {
 #pragma unroll
-    for (int a = 0; a < AM; ++a)
-#pragma unroll
-        for (int b = 0; b < AN; ++b) {
-            const int gi = r0 + wr * WM + a * 16 + 4 * lk; // rows gi .. gi + 3
-            const int gj = c0 + wc * WN + b * 16 + lrow;
-            f4 cr = f4{0.f, 0.f, 0.f, 0.f}, ci = cr;
-            if (gj < nc && gi < s) {
-                const size_t o = (size_t)gj * lda + gi;
-                if (gi + 3 < s) {
-                    const float4 x = *(const float4 *)(ar + o), y = *(const float4 *)(ai + o);
-                    cr = f4{x.x, x.y, x.z, x.w};
-                    ci = f4{y.x, y.y, y.z, y.w};
-                } else {
-                    for (int q = 0; q < 4 && gi + q < s; ++q) {
-                        cr[q] = ar[o + q];
-                        ci[q] = ai[o + q];
-                    }
-                }
-            }
-            accr[a][b] = cr;
-            acci[a][b] = ci;
-        }
-    for (int chunk = 0; chunk < nk; ++chunk) {
-        const int k0 = kbeg + kPanel * chunk;
-        if (chunk)
-            __syncthreads(); // previous chunk's operands consumed
-        // A: column k0 + k, rows r0 .. r0 + TM - 1 (contiguous): TM / 4 float4 per column
-#pragma unroll
-        for (int i = 0; i < (kPanel * TM / 4) / 256; ++i) {
+        for (int i = 0; i < NA; ++i) {
             const int e = t + 256 * i;
             const int k = e / (TM / 4), r4 = (e % (TM / 4)) * 4;
             const int gr = r0 + r4;
-            float4 vr = make_float4(0.f, 0.f, 0.f, 0.f), vi = vr;
             const size_t o = (size_t)(k0 + k) * lda + gr;
+            va_r[i] = va_i[i] = make_float4(0.f, 0.f, 0.f, 0.f);
             if (gr + 3 < s) {
-                vr = *(const float4 *)(ar + o);
-                vi = *(const float4 *)(ai + o);
+                va_r[i] = *(const float4 *)(ar + o);
+                va_i[i] = *(const float4 *)(ai + o);
             } else if (gr < s) {
                 float tr[4] = {0.f, 0.f, 0.f, 0.f}, ti[4] = {0.f, 0.f, 0.f, 0.f};
                 for (int q = 0; q < 4 && gr + q < s; ++q) {
                     tr[q] = ar[o + q];
                     ti[q] = ai[o + q];
                 }
-                vr = make_float4(tr[0], tr[1], tr[2], tr[3]);
-                vi = make_float4(ti[0], ti[1], ti[2], ti[3]);
+                va_r[i] = make_float4(tr[0], tr[1], tr[2], tr[3]);
+                va_i[i] = make_float4(ti[0], ti[1], ti[2], ti[3]);
             }
-            *(float4 *)&sa_r[k][r4] = vr;
-            *(float4 *)&sa_i[k][r4] = vi;
         }
-        // B: column c0 + c, rows k0 .. k0 + 31 (contiguous): 8 float4 per column
 #pragma unroll
-        for (int i = 0; i < (kPanel * TN / 4) / 256; ++i) {
+        for (int i = 0; i < NB; ++i) {
             const int e = t + 256 * i;
             const int c = e >> 3, k4 = (e & 7) * 4;
             const int gc = c0 + c;
-            float4 vr = make_float4(0.f, 0.f, 0.f, 0.f), vi = vr;
+            vb_r[i] = vb_i[i] = make_float4(0.f, 0.f, 0.f, 0.f);
             if (gc < nc) {
                 const size_t o = (size_t)gc * lda + k0 + k4;
-                vr = *(const float4 *)(ar + o);
-                vi = *(const float4 *)(ai + o);
+                vb_r[i] = *(const float4 *)(ar + o);
+                vb_i[i] = *(const float4 *)(ai + o);
             }
-            sb_r[k4][c] = vr.x;
-            sb_r[k4 + 1][c] = vr.y;
-            sb_r[k4 + 2][c] = vr.z;
-            sb_r[k4 + 3][c] = vr.w;
-            sb_i[k4][c] = vi.x;
-            sb_i[k4 + 1][c] = vi.y;
-            sb_i[k4 + 2][c] = vi.z;
-            sb_i[k4 + 3][c] = vi.w;
         }
-        __syncthreads();
+    };
+    auto store_ab = [&]() {
+#pragma unroll
+        for (int i = 0; i < NA; ++i) {
+            const int e = t + 256 * i;
+            const int k = e / (TM / 4), r4 = (e % (TM / 4)) * 4;
+            *(float4 *)&sa_r[k][r4] = va_r[i];
+            *(float4 *)&sa_i[k][r4] = va_i[i];
+        }
+#pragma unroll
+        for (int i = 0; i < NB; ++i) {
+            const int e = t + 256 * i;
+            const int c = e >> 3, k4 = (e & 7) * 4;
+            sb_r[k4][c] = vb_r[i].x;
+            sb_r[k4 + 1][c] = vb_r[i].y;
+            sb_r[k4 + 2][c] = vb_r[i].z;
+            sb_r[k4 + 3][c] = vb_r[i].w;
+            sb_i[k4][c] = vb_i[i].x;
+            sb_i[k4 + 1][c] = vb_i[i].y;
+            sb_i[k4 + 2][c] = vb_i[i].z;
+            sb_i[k4 + 3][c] = vb_i[i].w;
+        }
+    };
+    load_ab(kbeg);
+    // C, read after the first operands so that its loads stay in flight during
+    // the first chunk's products (4 consecutive rows per lane: the
+    // accumulator's row layout); it is added in the epilogue
+    f4 cr[AM][AN], ci[AM][AN];
+#pragma unroll
+    for (int a = 0; a < AM; ++a)
+#pragma unroll
+        for (int b = 0; b < AN; ++b) {
+            const int gi = r0 + wr * WM + a * 16 + 4 * lk; // rows gi .. gi + 3
+            const int gj = c0 + wc * WN + b * 16 + lrow;
+            cr[a][b] = ci[a][b] = f4{0.f, 0.f, 0.f, 0.f};
+            if (gj < nc && gi < s) {
+                const size_t o = (size_t)gj * lda + gi;
+                if (gi + 3 < s) {
+                    const float4 x = *(const float4 *)(ar + o), y = *(const float4 *)(ai + o);
+                    cr[a][b] = f4{x.x, x.y, x.z, x.w};
+                    ci[a][b] = f4{y.x, y.y, y.z, y.w};
+                } else {
+                    for (int q = 0; q < 4 && gi + q < s; ++q) {
+                        cr[a][b][q] = ar[o + q];
+                        ci[a][b][q] = ai[o + q];
+                    }
+                }
+            }
+        }
+    typename MF::acc accr[AM][AN], acci[AM][AN];
+#pragma unroll
+    for (int a = 0; a < AM; ++a)
+#pragma unroll
+        for (int b = 0; b < AN; ++b) {
+            accr[a][b] = MF::zero();
+            acci[a][b] = MF::zero();
+        }
+    store_ab();
+    __syncthreads();
+    for (int chunk = 0; chunk < nk; ++chunk) {
+        if (chunk + 1 < nk)
+            load_ab(kbeg + kPanel * (chunk + 1)); // in flight during this chunk's products
 #pragma unroll
         for (int kk = 0; kk < kPanel; kk += 4) {
             float xr[AM], xi[AM], yr[AN], yi[AN];
@@ -660,12 +688,17 @@ __global__ __launch_bounds__(256) void k_lu_update_f32(const LuDesc<float> *__re
             for (int a = 0; a < AM; ++a)
 #pragma unroll
                 for (int b = 0; b < AN; ++b) {
-                    // C - (xr + i xi)(yr + i yi)
+                    // -(xr + i xi)(yr + i yi)
                     accr[a][b] = MF::fma(-xr[a], yr[b], accr[a][b]);
                     accr[a][b] = MF::fma(xi[a], yi[b], accr[a][b]);
                     acci[a][b] = MF::fma(-xr[a], yi[b], acci[a][b]);
                     acci[a][b] = MF::fma(-xi[a], yr[b], acci[a][b]);
                 }
+        }
+        if (chunk + 1 < nk) {
+            __syncthreads(); // this chunk's operands consumed
+            store_ab();
+            __syncthreads();
         }
     }
 #pragma unroll
@@ -677,13 +710,19 @@ __global__ __launch_bounds__(256) void k_lu_update_f32(const LuDesc<float> *__re
             if (gj >= nc || gi >= s)
                 continue;
             const size_t o = (size_t)gj * lda + gi;
+            f4 vr, vi;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                vr[q] = cr[a][b][q] + accr[a][b][q];
+                vi[q] = ci[a][b][q] + acci[a][b][q];
+            }
             if (gi + 3 < s) {
-                *(float4 *)(D.ar + o) = make_float4(accr[a][b][0], accr[a][b][1], accr[a][b][2], accr[a][b][3]);
-                *(float4 *)(D.ai + o) = make_float4(acci[a][b][0], acci[a][b][1], acci[a][b][2], acci[a][b][3]);
+                *(float4 *)(D.ar + o) = make_float4(vr[0], vr[1], vr[2], vr[3]);
+                *(float4 *)(D.ai + o) = make_float4(vi[0], vi[1], vi[2], vi[3]);
             } else {
                 for (int q = 0; q < 4 && gi + q < s; ++q) {
-                    D.ar[o + q] = accr[a][b][q];
-                    D.ai[o + q] = acci[a][b][q];
+                    D.ar[o + q] = vr[q];
+                    D.ai[o + q] = vi[q];
                 }
             }
         }
