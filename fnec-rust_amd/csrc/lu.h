@@ -546,21 +546,43 @@ __global__ __launch_bounds__(256) void k_lu_update(const LuDesc<T> *__restrict__
 // [rbeg, m) x columns [cbeg, min(ncols, cend)) with L21 = columns
 // [kbeg, kbeg + 32 nk); those panels are full (a partial panel is always the
 // last one and has no rows below it).
+// Entries q >= valid of a float4 of consecutive rows set to 0 (a row quad
+// that starts below the order m ends inside the column, whose length lda is
+// a multiple of 64; the rows past m are padding and never enter a result).
+__device__ __forceinline__ void mask_rows(float4 &x, float4 &y, int valid)
+{
+    if (valid < 4) {
+        if (valid < 2) x.y = y.y = 0.f;
+        if (valid < 3) x.z = y.z = 0.f;
+        x.w = y.w = 0.f;
+    }
+}
+
 constexpr int kTile32 = 128;   // rows per FP32 update tile
 constexpr int kTile32N = 64;   // columns per FP32 update tile
 
+constexpr int kXcds = 8;
+
 template <int TM, int TN>
 __global__ __launch_bounds__(256) void k_lu_update_f32(const LuDesc<float> *__restrict__ descs,
-                                                       const int4 *__restrict__ work, int kbeg, int nk,
+                                                       const int4 *__restrict__ work, int cnt, int kbeg, int nk,
                                                        int rbeg, int cbeg, int cend)
 {
     typedef Mfma<float> MF;
     constexpr int WM = TM / 2, WN = TN / 2;       // per-wave block (2 x 2 waves)
     constexpr int AM = WM / 16, AN = WN / 16;     // MFMA tiles per wave
-    constexpr int LA = TM + 4, LB = TN + 4;
+    // A rows padded by 16 floats: the 4 k-rows a wave reads hit disjoint banks
+    constexpr int LA = TM + 16, LB = TN + 4;
     constexpr int NA = (kPanel * TM / 4) / 256;   // float4 of A per thread and chunk
     constexpr int NB = (kPanel * TN / 4) / 256;   // float4 of B per thread and chunk
-    const int4 wk = work[blockIdx.x];
+    // Workgroups are dealt to the XCDs round-robin by index; XCD x takes the
+    // contiguous item range [x per, (x + 1) per), so the tiles that share
+    // panels of one problem share that XCD's L2.
+    const int per = (cnt + kXcds - 1) / kXcds;
+    const int item = (blockIdx.x % kXcds) * per + blockIdx.x / kXcds;
+    if (item >= cnt)
+        return;
+    const int4 wk = work[item];
     const LuDesc<float> &D = descs[wk.x];
     const int s = D.m, nc = min(D.ncols, cend);
     if (D.info[0] != 0 || rbeg >= s || kbeg + kPanel * nk > s)
@@ -584,17 +606,10 @@ __global__ __launch_bounds__(256) void k_lu_update_f32(const LuDesc<float> *__re
             const int gr = r0 + r4;
             const size_t o = (size_t)(k0 + k) * lda + gr;
             va_r[i] = va_i[i] = make_float4(0.f, 0.f, 0.f, 0.f);
-            if (gr + 3 < s) {
+            if (gr < s) {
                 va_r[i] = *(const float4 *)(ar + o);
                 va_i[i] = *(const float4 *)(ai + o);
-            } else if (gr < s) {
-                float tr[4] = {0.f, 0.f, 0.f, 0.f}, ti[4] = {0.f, 0.f, 0.f, 0.f};
-                for (int q = 0; q < 4 && gr + q < s; ++q) {
-                    tr[q] = ar[o + q];
-                    ti[q] = ai[o + q];
-                }
-                va_r[i] = make_float4(tr[0], tr[1], tr[2], tr[3]);
-                va_i[i] = make_float4(ti[0], ti[1], ti[2], ti[3]);
+                mask_rows(va_r[i], va_i[i], s - gr);
             }
         }
 #pragma unroll
@@ -646,16 +661,10 @@ __global__ __launch_bounds__(256) void k_lu_update_f32(const LuDesc<float> *__re
             cr[a][b] = ci[a][b] = f4{0.f, 0.f, 0.f, 0.f};
             if (gj < nc && gi < s) {
                 const size_t o = (size_t)gj * lda + gi;
-                if (gi + 3 < s) {
-                    const float4 x = *(const float4 *)(ar + o), y = *(const float4 *)(ai + o);
-                    cr[a][b] = f4{x.x, x.y, x.z, x.w};
-                    ci[a][b] = f4{y.x, y.y, y.z, y.w};
-                } else {
-                    for (int q = 0; q < 4 && gi + q < s; ++q) {
-                        cr[a][b][q] = ar[o + q];
-                        ci[a][b][q] = ai[o + q];
-                    }
-                }
+                float4 x = *(const float4 *)(ar + o), y = *(const float4 *)(ai + o);
+                mask_rows(x, y, s - gi);
+                cr[a][b] = f4{x.x, x.y, x.z, x.w};
+                ci[a][b] = f4{y.x, y.y, y.z, y.w};
             }
         }
     typename MF::acc accr[AM][AN], acci[AM][AN];
@@ -716,15 +725,10 @@ __global__ __launch_bounds__(256) void k_lu_update_f32(const LuDesc<float> *__re
                 vr[q] = cr[a][b][q] + accr[a][b][q];
                 vi[q] = ci[a][b][q] + acci[a][b][q];
             }
-            if (gi + 3 < s) {
-                *(float4 *)(D.ar + o) = make_float4(vr[0], vr[1], vr[2], vr[3]);
-                *(float4 *)(D.ai + o) = make_float4(vi[0], vi[1], vi[2], vi[3]);
-            } else {
-                for (int q = 0; q < 4 && gi + q < s; ++q) {
-                    D.ar[o + q] = vr[q];
-                    D.ai[o + q] = vi[q];
-                }
-            }
+            // rows past m (at most 3, inside the padded column) receive 0:
+            // their C and L21 entries entered as 0
+            *(float4 *)(D.ar + o) = make_float4(vr[0], vr[1], vr[2], vr[3]);
+            *(float4 *)(D.ai + o) = make_float4(vi[0], vi[1], vi[2], vi[3]);
         }
 }
 
@@ -793,8 +797,9 @@ __global__ __launch_bounds__(64) void k_lu_trsm64_f32(const LuDesc<float> *__res
 inline void lu_launch_update_f32(const LuDesc<float> *d, size_t off, size_t cnt, const int *ints,
                                  int kbeg, int nk, int rbeg, int cbeg, int cend, hipStream_t st)
 {
-    hipLaunchKernelGGL((k_lu_update_f32<kTile32, kTile32N>), dim3((unsigned)cnt), dim3(256), 0, st, d,
-                       (const int4 *)(ints + off), kbeg, nk, rbeg, cbeg, cend);
+    const unsigned grid = (unsigned)((cnt + kXcds - 1) / kXcds * kXcds);
+    hipLaunchKernelGGL((k_lu_update_f32<kTile32, kTile32N>), dim3(grid), dim3(256), 0, st, d,
+                       (const int4 *)(ints + off), (int)cnt, kbeg, nk, rbeg, cbeg, cend);
 }
 
 // The 32-column update inside a 64-column outer step (rows [P + 32, m) x
@@ -802,8 +807,9 @@ inline void lu_launch_update_f32(const LuDesc<float> *d, size_t off, size_t cnt,
 inline void lu_launch_update_narrow_f32(const LuDesc<float> *d, size_t off, size_t cnt, const int *ints,
                                         int P, hipStream_t st)
 {
-    hipLaunchKernelGGL((k_lu_update_f32<kTile32, kPanel>), dim3((unsigned)cnt), dim3(256), 0, st, d,
-                       (const int4 *)(ints + off), P, 1, P + kPanel, P + kPanel, P + 2 * kPanel);
+    const unsigned grid = (unsigned)((cnt + kXcds - 1) / kXcds * kXcds);
+    hipLaunchKernelGGL((k_lu_update_f32<kTile32, kPanel>), dim3(grid), dim3(256), 0, st, d,
+                       (const int4 *)(ints + off), (int)cnt, P, 1, P + kPanel, P + kPanel, P + 2 * kPanel);
 }
 
 inline void lu_launch_trsm_f32(const LuDesc<float> *d, size_t off, size_t cnt, const int *ints, int p,
