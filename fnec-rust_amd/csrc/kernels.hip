@@ -114,8 +114,19 @@ __global__ __launch_bounds__(256) void k_fill_z(const Prob *__restrict__ probs,
     __syncthreads();
     const int row = t & 63;
     const int i = i0 + row;
-    if (i >= P.n)
+    if (i >= P.n) {
+        // mixed path: the FP32 copy's padding rows [n, ldz) are 0, so that the
+        // factorisation's kernels read and write whole float4 row quads
+        if (P.fr)
+            for (int q = 0; q < 4; ++q) {
+                const int j = j0 + (t >> 6) * 4 + q;
+                if (j >= P.n)
+                    break;
+                P.fr[(size_t)j * P.ldz + i] = 0.f;
+                P.fi[(size_t)j * P.ldz + i] = 0.f;
+            }
         return;
+    }
     const double ox = so[row][0], oy = so[row][1], oz = so[row][2];
     const double odx = so[row][3], ody = so[row][4], odz = so[row][5];
     const double k = P.k;

@@ -545,19 +545,10 @@ __global__ __launch_bounds__(256) void k_lu_update(const LuDesc<T> *__restrict__
 // latency overlaps the MFMA work. The update covers rows
 // [rbeg, m) x columns [cbeg, min(ncols, cend)) with L21 = columns
 // [kbeg, kbeg + 32 nk); those panels are full (a partial panel is always the
-// last one and has no rows below it).
-// Entries q >= valid of a float4 of consecutive rows set to 0 (a row quad
-// that starts below the order m ends inside the column, whose length lda is
-// a multiple of 64; the rows past m are padding and never enter a result).
-__device__ __forceinline__ void mask_rows(float4 &x, float4 &y, int valid)
-{
-    if (valid < 4) {
-        if (valid < 2) x.y = y.y = 0.f;
-        if (valid < 3) x.z = y.z = 0.f;
-        x.w = y.w = 0.f;
-    }
-}
-
+// last one and has no rows below it). Row quads are read and written whole: a
+// quad that starts below m ends inside the column (lda is a multiple of 64),
+// and the padding rows past m hold 0 (the fill writes them, and here their C
+// and L21 entries are 0, so they stay 0).
 constexpr int kTile32 = 128;   // rows per FP32 update tile
 constexpr int kTile32N = 64;   // columns per FP32 update tile
 
@@ -606,10 +597,9 @@ __global__ __launch_bounds__(256) void k_lu_update_f32(const LuDesc<float> *__re
             const int gr = r0 + r4;
             const size_t o = (size_t)(k0 + k) * lda + gr;
             va_r[i] = va_i[i] = make_float4(0.f, 0.f, 0.f, 0.f);
-            if (gr < s) {
+            if (gr < s) { // whole quad: padding rows hold 0
                 va_r[i] = *(const float4 *)(ar + o);
                 va_i[i] = *(const float4 *)(ai + o);
-                mask_rows(va_r[i], va_i[i], s - gr);
             }
         }
 #pragma unroll
@@ -661,8 +651,7 @@ __global__ __launch_bounds__(256) void k_lu_update_f32(const LuDesc<float> *__re
             cr[a][b] = ci[a][b] = f4{0.f, 0.f, 0.f, 0.f};
             if (gj < nc && gi < s) {
                 const size_t o = (size_t)gj * lda + gi;
-                float4 x = *(const float4 *)(ar + o), y = *(const float4 *)(ai + o);
-                mask_rows(x, y, s - gi);
+                const float4 x = *(const float4 *)(ar + o), y = *(const float4 *)(ai + o);
                 cr[a][b] = f4{x.x, x.y, x.z, x.w};
                 ci[a][b] = f4{y.x, y.y, y.z, y.w};
             }
@@ -725,8 +714,8 @@ __global__ __launch_bounds__(256) void k_lu_update_f32(const LuDesc<float> *__re
                 vr[q] = cr[a][b][q] + accr[a][b][q];
                 vi[q] = ci[a][b][q] + acci[a][b][q];
             }
-            // rows past m (at most 3, inside the padded column) receive 0:
-            // their C and L21 entries entered as 0
+            // rows past m (at most 3, inside the padded column) keep 0:
+            // their C and L21 entries are 0
             *(float4 *)(D.ar + o) = make_float4(vr[0], vr[1], vr[2], vr[3]);
             *(float4 *)(D.ai + o) = make_float4(vi[0], vi[1], vi[2], vi[3]);
         }
