@@ -38,6 +38,7 @@ __global__ void k_trsv1(const Prob *, const int4 *, int, unsigned long long *);
 __global__ void k_perm(const Prob *, const int *);
 __global__ void k_trsv_multi(const Prob *, const int4 *, int, unsigned long long *);
 __global__ void k_zsolve(const Prob *, const int4 *, int, int, unsigned long long *);
+__global__ void k_gen_items(const int4 *, int4 *);
 __global__ void k_diag_inv(const Prob *, const int2 *);
 __global__ void k_pre_setup(const Prob *, const int *);
 __global__ void k_prec_mid(const Prob *, const int *);
@@ -554,7 +555,12 @@ struct Plan {
     std::vector<Prob> probs;
     // work lists, concatenated into one int buffer
     std::vector<int> ints;
-    size_t fill_off = 0, fill_cnt = 0;
+    size_t fill_off = 0, fill_cnt = 0; // fill items in the generated region (item units)
+    // Regular work lists (problem, y, z) generated on the device by
+    // k_gen_items from one segment (first item, problem, ny, nz) per problem
+    // and launch: the host builds O(problems) records instead of O(tiles) items.
+    std::vector<int4> gensegs;
+    size_t gen_items = 0, off_gen = 0, gensegs_off = 0;
     size_t gram_off = 0, gram_cnt = 0;
     size_t cols_off = 0, cols_cnt = 0;
     size_t all_off = 0, all_cnt = 0;
@@ -778,7 +784,8 @@ static void lu_plan(std::vector<int> &ints, const std::vector<int> &ms, const st
 // Two-level plan of the FP32 LU (mixed path): 64-column outer steps made of
 // two 32-column panels, so that the trailing update runs with K = 64 and reads
 // and writes the trailing matrix once per 64 columns.
-static void lu_plan_outer(std::vector<int> &ints, const std::vector<int> &ms, std::vector<Plan::Outer> &out)
+static void lu_plan_outer(std::vector<int> &ints, const std::vector<int> &ms, std::vector<Plan::Outer> &out,
+                          std::vector<int4> &gensegs, size_t &gen_items)
 {
     auto align4 = [&]() {
         while (ints.size() % 4)
@@ -885,18 +892,18 @@ static void lu_plan_outer(std::vector<int> &ints, const std::vector<int> &ms, st
                     push2((int)b, y);
             }
             o.ot_cnt = (ints.size() - o.ot_off) / 2;
-            align4();
-            o.ou_off = ints.size();
+            // (problem, row tile, column tile) items, generated on the device
+            o.ou_off = gen_items;
             for (size_t b = 0; b < np; ++b) {
                 const int s = ms[b];
                 if (s <= P + 2 * kPanel)
                     continue;
                 const int rem = s - P - 2 * kPanel;
-                for (int y = 0; y < (rem + kTile32 - 1) / kTile32; ++y)
-                    for (int z = 0; z < (rem + kTile32N - 1) / kTile32N; ++z)
-                        push4((int)b, y, z, 0);
+                const int ny = (rem + kTile32 - 1) / kTile32, nz = (rem + kTile32N - 1) / kTile32N;
+                gensegs.push_back(make_int4((int)gen_items, (int)b, ny, nz));
+                gen_items += (size_t)ny * nz;
             }
-            o.ou_cnt = (ints.size() - o.ou_off) / 4;
+            o.ou_cnt = gen_items - o.ou_off;
         }
         // algorithmic work of the launches (8 flops per complex multiply-add;
         // 8 bytes per FP32 complex element)
@@ -1025,15 +1032,16 @@ static void plan_build(Plan &pl, const std::vector<const HostProblem *> &hps, bo
             pl.ints.push_back(0);
     };
     pl.ints.clear();
+    pl.gensegs.clear();
+    pl.gen_items = 0;
     align4();
-    pl.fill_off = pl.ints.size();
+    pl.fill_off = pl.gen_items;
     for (size_t b = 0; b < np; ++b) {
-        int rt = (hps[b]->n + 63) / 64, ct = (hps[b]->n + 15) / 16;
-        for (int y = 0; y < rt; ++y)
-            for (int z = 0; z < ct; ++z)
-                push4((int)b, y, z, 0);
+        const int rt = (hps[b]->n + 63) / 64, ct = (hps[b]->n + 15) / 16;
+        pl.gensegs.push_back(make_int4((int)pl.gen_items, (int)b, rt, ct));
+        pl.gen_items += (size_t)rt * ct;
     }
-    pl.fill_cnt = (pl.ints.size() - pl.fill_off) / 4;
+    pl.fill_cnt = pl.gen_items - pl.fill_off;
     pl.fill_flops = pl.gram_flops = pl.backsub_flops = 0;
     for (size_t b = 0; b < np; ++b) {
         const double n = hps[b]->n, s = hps[b]->s;
@@ -1052,7 +1060,7 @@ static void plan_build(Plan &pl, const std::vector<const HostProblem *> &hps, bo
         std::vector<int> ms(np);
         for (size_t b = 0; b < np; ++b)
             ms[b] = hps[b]->n;
-        lu_plan_outer(pl.ints, ms, pl.outer32);
+        lu_plan_outer(pl.ints, ms, pl.outer32, pl.gensegs, pl.gen_items);
         // triangular-solve items (problem, vector, phase, block): phase 0
         // blocks in forward order, then phase 1 blocks in backward order,
         // interleaved across chains so that early tickets rarely wait
@@ -1188,6 +1196,10 @@ static void plan_build(Plan &pl, const std::vector<const HostProblem *> &hps, bo
         pl.off_lud = take(cur, sizeof(LuDesc<double>) * np);
     if (with_solve && pl.mixed)
         pl.off_lud32 = take(cur, sizeof(LuDesc<float>) * np);
+    align4();
+    pl.gensegs_off = pl.ints.size();
+    for (const int4 &g : pl.gensegs)
+        push4(g.x, g.y, g.z, g.w);
     pl.off_ints = take(cur, 4 * pl.ints.size() + 16);
     // result words, packed (16-byte alignment): uploaded as zeros, read back
     // with one copy after the run
@@ -1207,6 +1219,8 @@ static void plan_build(Plan &pl, const std::vector<const HostProblem *> &hps, bo
         pl.off_x[b] = take16(16 * (size_t)hps[b]->s);
     }
     pl.io_end = cur = (cur + 255) & ~(size_t)255;
+    // generated work items (device only, written by k_gen_items)
+    pl.off_gen = take(cur, 16 * pl.gen_items + 16);
     pl.bytes = cur + 256;
 }
 
@@ -1355,6 +1369,20 @@ static int plan_upload(Plan &pl, const std::vector<const HostProblem *> &hps, ch
     return FNEC_OK;
 }
 
+// Generated work items as ints (item i at 4 i).
+static inline const int *gen_ptr(char *base, const Plan &pl) { return (const int *)(base + pl.off_gen); }
+
+// Expand the segment records into the generated work lists (before any
+// kernel that reads them, on the same stream).
+static void gen_launch(const Plan &pl, char *base, hipStream_t st)
+{
+    if (pl.gensegs.empty())
+        return;
+    hipLaunchKernelGGL(k_gen_items, dim3((unsigned)pl.gensegs.size()), dim3(256), 0, st,
+                       (const int4 *)((const int *)(base + pl.off_ints) + pl.gensegs_off),
+                       (int4 *)(base + pl.off_gen));
+}
+
 static inline const int *wl_ptr(char *base, const Plan &pl, size_t off)
 {
     return (const int *)(base + pl.off_ints) + off;
@@ -1400,7 +1428,7 @@ static int mixed_launch(Plan &pl, char *base)
             prof_end(PC_LU_TRSM, e, o.f_trsm, 0);
             if (o.ou_cnt) {
                 e = prof_begin(PC_LU_UPDATE);
-                lu_launch_update_f32(lud, o.ou_off, o.ou_cnt, ints, o.P, 2, o.P + 2 * kPanel,
+                lu_launch_update_f32(lud, 4 * o.ou_off, o.ou_cnt, gen_ptr(base, pl), o.P, 2, o.P + 2 * kPanel,
                                      o.P + 2 * kPanel, 0x7fffffff, st);
                 prof_end(PC_LU_UPDATE, e, o.f_ou, o.b_ou);
             }
@@ -1522,10 +1550,11 @@ static int plan_launch(Plan &pl, char *base, bool with_solve)
     hipStream_t st = g_ctx.stream;
     const Prob *dp = (const Prob *)(base + pl.off_probs);
     HIPCHK(hipEventRecord(g_ctx.ev[0], st));
+    gen_launch(pl, base, st);
     if (pl.fill_cnt) {
         hipEvent_t e = prof_begin(PC_FILL);
         hipLaunchKernelGGL(k_fill_z, dim3((unsigned)pl.fill_cnt), dim3(256), 0, st, dp,
-                           (const int4 *)wl_ptr(base, pl, pl.fill_off));
+                           (const int4 *)gen_ptr(base, pl) + pl.fill_off);
         prof_end(PC_FILL, e, pl.fill_flops, 0);
     }
     HIPCHK(hipGetLastError());
@@ -1763,8 +1792,9 @@ int normal_eqs_only(const HostProblem &hp, fnec_c64 *aug_out)
         return st;
     hipStream_t s = g_ctx.stream;
     const Prob *dp = (const Prob *)(base + pl.off_probs);
+    gen_launch(pl, base, s);
     hipLaunchKernelGGL(k_fill_z, dim3((unsigned)pl.fill_cnt), dim3(256), 0, s, dp,
-                       (const int4 *)wl_ptr(base, pl, pl.fill_off));
+                       (const int4 *)gen_ptr(base, pl) + pl.fill_off);
     size_t count = (pl.a_end - pl.a_begin) / 8;
     hipLaunchKernelGGL(k_zero, dim3(1024), dim3(256), 0, s, (double *)(base + pl.a_begin), count);
     hipLaunchKernelGGL(k_gram, dim3((unsigned)pl.gram_cnt), dim3(256), 0, s, dp,

@@ -86,6 +86,16 @@ __device__ __forceinline__ void self_term(double h, double a, double k, double &
     im = si * h;
 }
 
+// Regular work lists from segment records (first item, problem, ny, nz):
+// items (problem, y, z, 0) in y-major order. One workgroup per segment.
+__global__ __launch_bounds__(256) void k_gen_items(const int4 *__restrict__ segs, int4 *__restrict__ out)
+{
+    const int4 sg = segs[blockIdx.x];
+    const int total = sg.z * sg.w;
+    for (int e = threadIdx.x; e < total; e += 256)
+        out[(size_t)sg.x + e] = make_int4(sg.y, e / sg.w, e % sg.w, 0);
+}
+
 constexpr int kFillRows = 64;
 constexpr int kFillCols = 16;
 
