@@ -125,8 +125,8 @@ def serve(items, window, steps, warmup, prof_windows, precision, budget, max_bat
     for w in range(warmup, warmup + prof_windows):
         run_window(items[w * window:(w + 1) * window])
     prof_all = F.profile(enable=False)
-    dom = max((k for k in prof_all if prof_all[k][0] > 0 and prof_all[k][2] > 0),
-              key=lambda k: prof_all[k][1])
+    timed_cls = [k for k in prof_all if prof_all[k][0] > 0 and prof_all[k][2] > 0]
+    dom = max(timed_cls, key=lambda k: prof_all[k][1]) if timed_cls else "lu_update"
     F.profile(enable=True, reset=True, only=dom)
     first = warmup + prof_windows
     lat = []

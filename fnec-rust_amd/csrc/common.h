@@ -100,6 +100,7 @@ template <typename T> struct LuDesc {
     double *diagmag;  // kPanel: |u_jj|
     int *fast_ok;     // [0]: the fast path factored the current panel
     int pad1;
+    T *linv;          // FP32 two-level LU: L11^{-1} of the current 64-column step, 2 x 64 x 64
 };
 
 // Work lists of one panel step (offsets into the batch's int list).

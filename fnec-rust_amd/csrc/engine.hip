@@ -92,6 +92,10 @@ struct Context {
     // uploads that overlap the solve stream's kernels (default priority)
     hipStream_t copy_stream = nullptr;
     hipEvent_t ev_rp = nullptr;
+    // the LU's panel chain runs one step ahead on this stream (default
+    // priority), ordered against the solve stream with events from the pool
+    hipStream_t side_stream = nullptr;
+    std::vector<hipEvent_t> sync_events;
 };
 
 static Context g_ctx;
@@ -152,23 +156,23 @@ HostTimer::~HostTimer()
     c.total_us += host_now_us() - t0_;
 }
 
-static hipEvent_t prof_begin(int cls)
+static hipEvent_t prof_begin(int cls, hipStream_t st = nullptr)
 {
     if (!g_prof || (g_prof_only >= 0 && cls != g_prof_only))
         return nullptr;
     hipEvent_t e = prof_event();
-    (void)hipEventRecord(e, g_ctx.stream);
+    (void)hipEventRecord(e, st ? st : g_ctx.stream);
     return e;
 }
 
 // Returns the index of the pending record (-1 when profiling is off) so that
 // work known only after the run can be attributed to it.
-static long prof_end(int cls, hipEvent_t a, double flops, double bytes)
+static long prof_end(int cls, hipEvent_t a, double flops, double bytes, hipStream_t st = nullptr)
 {
     if (!g_prof || !a)
         return -1;
     hipEvent_t b = prof_event();
-    (void)hipEventRecord(b, g_ctx.stream);
+    (void)hipEventRecord(b, st ? st : g_ctx.stream);
     g_prof_pending.push_back({cls, a, b, flops, bytes});
     return (long)g_prof_pending.size() - 1;
 }
@@ -258,7 +262,8 @@ static int ctx_init()
         return g_ctx.status;
     }
     if (hipSetDevice(g_ctx.device) != hipSuccess || hipStreamCreate(&g_ctx.stream) != hipSuccess ||
-        hipStreamCreate(&g_ctx.copy_stream) != hipSuccess || hipEventCreate(&g_ctx.ev_rp) != hipSuccess) {
+        hipStreamCreate(&g_ctx.copy_stream) != hipSuccess || hipEventCreate(&g_ctx.ev_rp) != hipSuccess ||
+        hipStreamCreate(&g_ctx.side_stream) != hipSuccess) {
         set_error("device initialisation failed");
         g_ctx.status = FNEC_INTERNAL;
         return g_ctx.status;
@@ -580,18 +585,20 @@ struct Plan {
     bool mixed = false;
     struct MixOff {
         size_t fr, fpr, rowcol, fh, lk, ym, ls, vr, vz, vp, vq, vt, vv, part, colsq, pcg, pstate;
-        size_t perm, tws, tflags, dinv, gx;
+        size_t perm, tws, tflags, dinv, gx, linv;
         int nchunk;
     };
     std::vector<MixOff> mix;
     struct Outer {                 // one 64-column step of the two-level FP32 LU
         int P;
         LuStep a, b;               // inner panels at P and P + 32
-        bool has_b, has_outer;
+        bool has_b;
         size_t ua_off, ua_cnt;     // narrow update of columns [P + 32, P + 64)
-        size_t ot_off, ot_cnt;     // outer interchanges + L11 solve of columns [P + 64, n)
-        size_t ou_off, ou_cnt;     // outer update, K = 64
-        double f_panel, f_trsm, f_ua, b_ua, f_ou, b_ou;
+        size_t li_off, li_cnt;     // problems whose L11^{-1} this step needs (order > P + 64)
+        size_t ot_off, ot_cnt;     // interchanges left of P, and of + L11 solve for columns [P + 64, n)
+        size_t on_off, on_cnt;     // outer update (K = 64) of the next step's columns [P + 64, P + 128)
+        size_t of_off, of_cnt;     // outer update of the columns [P + 128, n)
+        double f_panel, f_trsm, f_ua, b_ua, f_on, b_on, f_of, b_of;
     };
     std::vector<Outer> outer32;
     std::function<void()> overlap; // host work run once while the device computes
@@ -658,9 +665,9 @@ size_t problem_bytes(int n, int s, int ncons, int nruns, int precision)
         b += 4 * nn + 16 * nn * nc + 16 * nc * nc + 16 * nc * w + 16 * w * w;
         b += 4 * 16 * ss + 16 * (nn + nc) + 16 * nn + 16 * nchunk * nn + 8 * nn + 80;
         b += 4 * nn + 16 * nn * (nc + 1) + 8 * (nc + 1) * ((nn + 63) / 64);  // perm, solves
-        b += ((nn + 63) / 64) * 2 * 64 * 64 * 8;                     // diagonal-block inverses
+        b += ((nn + 63) / 64) * 2 * 64 * 64 * 8 + 2 * 64 * 64 * 4;   // diagonal-block inverses, L11^{-1}
         b += 8 * 2 * 256 * ((nn + 63) / 64);                         // gang-published blocks
-        b += 22 * 256;                                               // alignment of 22 buffers
+        b += 23 * 256;                                               // alignment of 23 buffers
     } else {
         b += 2 * 8 * lda * (ss + 1) + 256;                           // [G | M^H y]
         b += 2 * 8 * lda * kPanel + 256;                             // panel workspace
@@ -841,20 +848,14 @@ static void lu_plan_outer(std::vector<int> &ints, const std::vector<int> &ms, st
         o.a.p = P;
         o.b.p = P + kPanel;
         o.has_b = false;
-        o.has_outer = false;
-        // inner panel a
+        // inner panel a; its interchanges reach the columns left of P in the
+        // outer step's launch (ot), after the previous trailing update
         panel_lists(o.a, P);
         align4();
         o.a.trsm_off = ints.size();
-        for (size_t b = 0; b < np; ++b) {
-            const int s = ms[b];
-            if (s <= P)
-                continue;
-            if (s > P + kPanel)
+        for (size_t b = 0; b < np; ++b)
+            if (ms[b] > P + kPanel)
                 push2((int)b, 0); // columns [P + 32, min(P + 64, s))
-            for (int y = 0; y < (P + 63) / 64; ++y)
-                push2((int)b, -1 - y); // interchanges left of the panel
-        }
         o.a.trsm_cnt = (ints.size() - o.a.trsm_off) / 2;
         align4();
         o.ua_off = ints.size();
@@ -866,45 +867,59 @@ static void lu_plan_outer(std::vector<int> &ints, const std::vector<int> &ms, st
                 push4((int)b, y, 0, 0);
         }
         o.ua_cnt = (ints.size() - o.ua_off) / 4;
-        // inner panel b
+        // inner panel b; its interchanges on panel a's columns [P, P + 32)
         o.has_b = smax > P + kPanel;
         if (o.has_b) {
             panel_lists(o.b, P + kPanel);
             align4();
             o.b.trsm_off = ints.size();
-            for (size_t b = 0; b < np; ++b) {
-                const int s = ms[b];
-                if (s <= P + kPanel)
-                    continue;
-                for (int y = 0; y < (P + kPanel + 63) / 64; ++y)
-                    push2((int)b, -1 - y);
-            }
+            for (size_t b = 0; b < np; ++b)
+                if (ms[b] > P + kPanel)
+                    push2((int)b, -1 - P / 64);
             o.b.trsm_cnt = (ints.size() - o.b.trsm_off) / 2;
         }
-        // outer solve and update
-        o.has_outer = smax > P + 2 * kPanel;
-        if (o.has_outer) {
-            align4();
-            o.ot_off = ints.size();
-            for (size_t b = 0; b < np; ++b) {
-                const int s = ms[b];
-                for (int y = 0; y < (s - P - 2 * kPanel + 63) / 64 && s > P + 2 * kPanel; ++y)
-                    push2((int)b, y);
-            }
-            o.ot_cnt = (ints.size() - o.ot_off) / 2;
-            // (problem, row tile, column tile) items, generated on the device
-            o.ou_off = gen_items;
-            for (size_t b = 0; b < np; ++b) {
-                const int s = ms[b];
-                if (s <= P + 2 * kPanel)
-                    continue;
-                const int rem = s - P - 2 * kPanel;
-                const int ny = (rem + kTile32 - 1) / kTile32, nz = (rem + kTile32N - 1) / kTile32N;
-                gensegs.push_back(make_int4((int)gen_items, (int)b, ny, nz));
-                gen_items += (size_t)ny * nz;
-            }
-            o.ou_cnt = gen_items - o.ou_off;
+        align4();
+        o.li_off = ints.size();
+        for (size_t b = 0; b < np; ++b)
+            if (ms[b] > P + 2 * kPanel)
+                ints.push_back((int)b);
+        o.li_cnt = ints.size() - o.li_off;
+        // outer step: interchanges left of P, interchanges + L11 solve right of P + 64
+        align4();
+        o.ot_off = ints.size();
+        for (size_t b = 0; b < np; ++b) {
+            const int s = ms[b];
+            if (s <= P)
+                continue;
+            for (int y = 0; y < P / 64; ++y)
+                push2((int)b, -1 - y);
+            for (int y = 0; y < (s - P - 2 * kPanel + kTrsm64Cols - 1) / kTrsm64Cols && s > P + 2 * kPanel; ++y)
+                push2((int)b, y);
         }
+        o.ot_cnt = (ints.size() - o.ot_off) / 2;
+        // outer update items (problem, row tile, column tile), generated on the
+        // device: the next step's 64 columns, then the rest
+        o.on_off = gen_items;
+        for (size_t b = 0; b < np; ++b) {
+            const int s = ms[b];
+            if (s <= P + 2 * kPanel)
+                continue;
+            const int ny = (s - P - 2 * kPanel + kTile32 - 1) / kTile32;
+            gensegs.push_back(make_int4((int)gen_items, (int)b, ny, 1));
+            gen_items += (size_t)ny;
+        }
+        o.on_cnt = gen_items - o.on_off;
+        o.of_off = gen_items;
+        for (size_t b = 0; b < np; ++b) {
+            const int s = ms[b];
+            if (s <= P + 4 * kPanel)
+                continue;
+            const int ny = (s - P - 2 * kPanel + kTile32 - 1) / kTile32;
+            const int nz = (s - P - 4 * kPanel + kTile32N - 1) / kTile32N;
+            gensegs.push_back(make_int4((int)gen_items, (int)b, ny, nz));
+            gen_items += (size_t)ny * nz;
+        }
+        o.of_cnt = gen_items - o.of_off;
         // algorithmic work of the launches (8 flops per complex multiply-add;
         // 8 bytes per FP32 complex element)
         for (size_t b = 0; b < np; ++b) {
@@ -917,9 +932,12 @@ static void lu_plan_outer(std::vector<int> &ints, const std::vector<int> &ms, st
             o.f_ua += 8.0 * rows_a * cols_a * ja;
             o.b_ua += 8.0 * (2.0 * rows_a * cols_a + rows_a * ja + ja * cols_a);
             const double rem = std::max(0.0, s - P - 2.0 * kPanel);
+            const double cn = std::min(64.0, rem), cf = std::max(0.0, rem - 64.0);
             o.f_trsm += 8.0 * cols_a * ja * ja / 2.0 + 8.0 * rem * 64.0 * 64.0 / 2.0;
-            o.f_ou += 8.0 * rem * rem * 64.0;
-            o.b_ou += 8.0 * (2.0 * rem * rem + 2.0 * rem * 64.0);
+            o.f_on += 8.0 * rem * cn * 64.0;
+            o.b_on += 8.0 * (2.0 * rem * cn + rem * 64.0 + 64.0 * cn);
+            o.f_of += 8.0 * rem * cf * 64.0;
+            o.b_of += 8.0 * (2.0 * rem * cf + rem * 64.0 + 64.0 * cf);
         }
         out.push_back(o);
     }
@@ -985,6 +1003,7 @@ static void plan_build(Plan &pl, const std::vector<const HostProblem *> &hps, bo
             m.colsq = take(cur, 8 * n);
             m.perm = take(cur, 4 * n);
             m.dinv = take(cur, ((n + 63) / 64) * 2 * 64 * 64 * 8);
+            m.linv = take(cur, 2 * 64 * 64 * 4);
             m.tws = take(cur, 16 * n * (nc + 1));
             pl.mix.push_back(m);
         }
@@ -1313,6 +1332,7 @@ static int plan_upload(Plan &pl, const std::vector<const HostProblem *> &hps, ch
             D.colmax = P.colmax;
             D.diagmag = P.diagmag;
             D.fast_ok = P.fast_ok;
+            D.linv = (float *)(base + m.linv);
             pl.lud32.push_back(D);
 
         } else if (with_solve) {
@@ -1398,40 +1418,82 @@ static int mixed_launch(Plan &pl, char *base)
     const unsigned np = (unsigned)pl.all_cnt;
     HIPCHK(hipEventRecord(g_ctx.ev[2], st));
     const LuDesc<float> *lud = (const LuDesc<float> *)(base + pl.off_lud32);
-    for (const Plan::Outer &o : pl.outer32) {
-        hipEvent_t e = prof_begin(PC_LU_PANEL);
-        lu_launch_panel(lud, o.a, ints, st);
-        if (o.has_b)
-            prof_end(PC_LU_PANEL, e, 0, 0);
-        else
-            prof_end(PC_LU_PANEL, e, o.f_panel, 0);
-        e = prof_begin(PC_LU_TRSM);
-        lu_launch_trsm_f32(lud, o.a.trsm_off, o.a.trsm_cnt, ints, o.P, o.P + 2 * kPanel, st);
-        prof_end(PC_LU_TRSM, e, 0, 0);
+    // Look-ahead: the panel chain of step k + 1 (panels, their column solves
+    // and the narrow update, all inside the columns [P + 64, P + 128)) runs on
+    // the side stream once the main stream has updated those columns, while
+    // the main stream updates the columns right of them. The interchanges of a
+    // step reach the columns left of it in the main stream's outer launch,
+    // after the trailing update that reads those columns.
+    const size_t nsteps = pl.outer32.size();
+    while (g_ctx.sync_events.size() < 2 * nsteps + 1) {
+        hipEvent_t e = nullptr;
+        HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        g_ctx.sync_events.push_back(e);
+    }
+    hipEvent_t *ev_near = g_ctx.sync_events.data();          // [k]: main updated step k + 1's columns
+    hipEvent_t *ev_chain = g_ctx.sync_events.data() + nsteps; // [k]: step k's panel chain done
+    hipStream_t side = g_ctx.side_stream;
+    auto chain = [&](const Plan::Outer &o) {
+        hipEvent_t e = prof_begin(PC_LU_PANEL, side);
+        lu_launch_panel(lud, o.a, ints, side);
+        prof_end(PC_LU_PANEL, e, o.has_b ? 0.0 : o.f_panel, 0, side);
+        if (o.a.trsm_cnt) {
+            e = prof_begin(PC_LU_TRSM, side);
+            lu_launch_trsm_f32(lud, o.a.trsm_off, o.a.trsm_cnt, ints, o.P, o.P + 2 * kPanel, side);
+            prof_end(PC_LU_TRSM, e, 0, 0, side);
+        }
         if (o.ua_cnt) {
-            e = prof_begin(PC_LU_UPDN);
-            lu_launch_update_narrow_f32(lud, o.ua_off, o.ua_cnt, ints, o.P, st);
-            prof_end(PC_LU_UPDN, e, o.f_ua, o.b_ua);
+            e = prof_begin(PC_LU_UPDN, side);
+            lu_launch_update_narrow_f32(lud, o.ua_off, o.ua_cnt, ints, o.P, side);
+            prof_end(PC_LU_UPDN, e, o.f_ua, o.b_ua, side);
         }
         if (o.has_b) {
-            e = prof_begin(PC_LU_PANEL);
-            lu_launch_panel(lud, o.b, ints, st);
-            prof_end(PC_LU_PANEL, e, o.f_panel, 0);
-            e = prof_begin(PC_LU_TRSM);
-            lu_launch_trsm_f32(lud, o.b.trsm_off, o.b.trsm_cnt, ints, o.P + kPanel, 0, st);
-            prof_end(PC_LU_TRSM, e, 0, 0);
-        }
-        if (o.has_outer) {
-            e = prof_begin(PC_LU_TRSM);
-            if (o.ot_cnt)
-                lu_launch_trsm64_f32(lud, o.ot_off, o.ot_cnt, ints, o.P, st);
-            prof_end(PC_LU_TRSM, e, o.f_trsm, 0);
-            if (o.ou_cnt) {
-                e = prof_begin(PC_LU_UPDATE);
-                lu_launch_update_f32(lud, 4 * o.ou_off, o.ou_cnt, gen_ptr(base, pl), o.P, 2, o.P + 2 * kPanel,
-                                     o.P + 2 * kPanel, 0x7fffffff, st);
-                prof_end(PC_LU_UPDATE, e, o.f_ou, o.b_ou);
+            e = prof_begin(PC_LU_PANEL, side);
+            lu_launch_panel(lud, o.b, ints, side);
+            prof_end(PC_LU_PANEL, e, o.f_panel, 0, side);
+            if (o.b.trsm_cnt) {
+                e = prof_begin(PC_LU_TRSM, side);
+                lu_launch_trsm_f32(lud, o.b.trsm_off, o.b.trsm_cnt, ints, o.P + kPanel, 0, side);
+                prof_end(PC_LU_TRSM, e, 0, 0, side);
             }
+        }
+        if (o.li_cnt) {
+            e = prof_begin(PC_LU_TRSM, side);
+            lu_launch_l11_inv(lud, o.li_off, o.li_cnt, ints, o.P, side);
+            prof_end(PC_LU_TRSM, e, 0, 0, side);
+        }
+    };
+    HIPCHK(hipEventRecord(ev_chain[nsteps], st)); // the fill is done
+    HIPCHK(hipStreamWaitEvent(side, ev_chain[nsteps], 0));
+    for (size_t k = 0; k < nsteps; ++k) {
+        const Plan::Outer &o = pl.outer32[k];
+        if (k == 0) {
+            chain(o);
+            HIPCHK(hipEventRecord(ev_chain[0], side));
+        }
+        HIPCHK(hipStreamWaitEvent(st, ev_chain[k], 0));
+        if (o.ot_cnt) {
+            hipEvent_t e = prof_begin(PC_LU_TRSM);
+            lu_launch_trsm64_f32(lud, o.ot_off, o.ot_cnt, ints, o.P, st);
+            prof_end(PC_LU_TRSM, e, o.f_trsm, 0);
+        }
+        if (o.on_cnt) {
+            hipEvent_t e = prof_begin(PC_LU_UPDATE);
+            lu_launch_update_f32(lud, 4 * o.on_off, o.on_cnt, gen_ptr(base, pl), o.P, 2, o.P + 2 * kPanel,
+                                 o.P + 2 * kPanel, o.P + 4 * kPanel, st);
+            prof_end(PC_LU_UPDATE, e, o.f_on, o.b_on);
+        }
+        if (k + 1 < nsteps) {
+            HIPCHK(hipEventRecord(ev_near[k], st));
+            HIPCHK(hipStreamWaitEvent(side, ev_near[k], 0));
+            chain(pl.outer32[k + 1]);
+            HIPCHK(hipEventRecord(ev_chain[k + 1], side));
+        }
+        if (o.of_cnt) {
+            hipEvent_t e = prof_begin(PC_LU_UPDATE);
+            lu_launch_update_f32(lud, 4 * o.of_off, o.of_cnt, gen_ptr(base, pl), o.P, 2, o.P + 2 * kPanel,
+                                 o.P + 4 * kPanel, 0x7fffffff, st);
+            prof_end(PC_LU_UPDATE, e, o.f_of, o.b_of);
         }
     }
     HIPCHK(hipGetLastError());

@@ -23,8 +23,9 @@
 //                complex product, 64x64 output tiles, K = jb staged in LDS).
 //   k_lu_update_f32  the FP32 variant: 128x64 tiles, accumulators seeded
 //                with C, K = 32 nk columns of L21 per launch.
+//   k_l11_inv    FP32 outer step: L11^{-1} of the 64 x 64 diagonal block.
 //   k_lu_trsm64_f32  FP32 outer step: interchanges of two 32-column panels
-//                and the 64x64 L11 solve for the columns right of both.
+//                and U12 = L11^{-1} A12 on MFMA for the columns right of both.
 // The FP32 factorisation (mixed path) is two-level: each 64-column step
 // factors two 32-column panels (the second after a narrow K = 32 update of
 // its columns only), then updates the trailing matrix once with K = 64, so
@@ -721,35 +722,85 @@ __global__ __launch_bounds__(256) void k_lu_update_f32(const LuDesc<float> *__re
         }
 }
 
-// Outer step of the two-level FP32 LU: for columns [P + 64, ncols) apply the
-// interchanges of both 32-column panels in order, then U12 = L11^{-1} A12
-// with the 64 x 64 unit-lower L11 at (P, P). One thread per column.
-__global__ __launch_bounds__(64) void k_lu_trsm64_f32(const LuDesc<float> *__restrict__ descs,
-                                                      const int2 *__restrict__ work, int P)
+// L11^{-1} of a 64-column outer step of the FP32 LU (L11 unit lower at
+// (P, P), after both panels' interchanges) into D.linv, column-major, real
+// then imaginary plane. One thread per column c of the inverse solves
+// L x = e_c in registers against L11 staged in LDS; the result goes back
+// through LDS so the stores are coalesced.
+__global__ __launch_bounds__(64) void k_l11_inv(const LuDesc<float> *__restrict__ descs,
+                                                const int *__restrict__ plist, int P)
 {
     constexpr int NB = 2 * kPanel;
-    const int2 wk = work[blockIdx.x];
-    const LuDesc<float> &D = descs[wk.x];
-    const int s = D.m;
-    if (D.info[0] != 0 || P + NB >= s)
+    const LuDesc<float> &D = descs[plist[blockIdx.x]];
+    if (D.info[0] != 0 || P + NB >= D.m)
         return;
-    const int lda = D.lda, t = threadIdx.x;
     __shared__ float lr[NB][NB + 1], li[NB][NB + 1];
-    __shared__ int piv_s[NB];
-    piv_s[t] = D.ipiv[P + t];
+    const int t = threadIdx.x, lda = D.lda;
     for (int e = t; e < NB * NB; e += 64) {
         const int rr = e % NB, cc = e / NB;
         lr[rr][cc] = D.ar[(size_t)(P + cc) * lda + P + rr];
         li[rr][cc] = D.ai[(size_t)(P + cc) * lda + P + rr];
     }
     __syncthreads();
-    const bool any_swap = __syncthreads_or(piv_s[t] != P + t);
-    const int c = P + NB + wk.y * 64 + t;
-    if (c >= D.ncols)
+    float xr[NB], xi[NB];
+#pragma unroll
+    for (int j = 0; j < NB; ++j) {
+        float vr = j == t ? 1.f : 0.f, vi = 0.f;
+#pragma unroll
+        for (int m = 0; m < j; ++m) {
+            vr -= lr[j][m] * xr[m] - li[j][m] * xi[m];
+            vi -= lr[j][m] * xi[m] + li[j][m] * xr[m];
+        }
+        xr[j] = vr;
+        xi[j] = vi;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < NB; ++j) {
+        lr[j][t] = xr[j];
+        li[j][t] = xi[j];
+    }
+    __syncthreads();
+    for (int e = t; e < NB * NB; e += 64) {
+        const int rr = e % NB, cc = e / NB;
+        D.linv[e] = lr[rr][cc];
+        D.linv[NB * NB + e] = li[rr][cc];
+    }
+}
+
+// Outer step of the two-level FP32 LU. Right items (problem, g >= 0): for the
+// 64 columns [P + 64 + 64 g, + 64) apply the interchanges of both 32-column
+// panels in order, then U12 = L11^{-1} A12 in place as a 64 x 64 x 64 complex
+// product on MFMA (four waves, 2 x 2 blocks of 32 x 32), with L11^{-1} from
+// k_l11_inv. Left items (problem, -1 - g): the same interchanges on the
+// columns [64 g, 64 g + 64) left of P (the multipliers of earlier steps), so
+// the factors read P Z = L U.
+constexpr int kTrsm64Cols = 64;
+
+__global__ __launch_bounds__(256) void k_lu_trsm64_f32(const LuDesc<float> *__restrict__ descs,
+                                                       const int2 *__restrict__ work, int P)
+{
+    typedef Mfma<float> MF;
+    constexpr int NB = 2 * kPanel, LPA = NB + 16, LPB = NB + 4; // two workgroups per CU
+    const int2 wk = work[blockIdx.x];
+    const LuDesc<float> &D = descs[wk.x];
+    const int s = D.m;
+    if (D.info[0] != 0 || P >= s)
         return;
-    float *cr = D.ar + (size_t)c * lda, *ci = D.ai + (size_t)c * lda;
-    if (any_swap)
-        for (int j = 0; j < NB; ++j) {
+    const bool left = wk.y < 0;
+    if (!left && P + NB >= s)
+        return;
+    const int lda = D.lda, t = threadIdx.x;
+    const int jbt = min(NB, s - P); // interchanges of this step
+    __shared__ float sa_r[NB][LPA], sa_i[NB][LPA]; // L11^{-1}: [k][row]
+    __shared__ float sb_r[NB][LPB], sb_i[NB][LPB]; // A12 block: [k][column]
+    __shared__ int piv_s[NB];
+    if (t < NB)
+        piv_s[t] = t < jbt ? D.ipiv[P + t] : P + t;
+    __syncthreads();
+    const bool any_swap = __syncthreads_or(t < NB && piv_s[t] != P + t);
+    auto swap_rows = [&](float *cr, float *ci) {
+        for (int j = 0; j < jbt; ++j) {
             const int piv = piv_s[j];
             if (piv != P + j) {
                 const float xr = cr[P + j], xi = ci[P + j];
@@ -759,28 +810,87 @@ __global__ __launch_bounds__(64) void k_lu_trsm64_f32(const LuDesc<float> *__res
                 ci[piv] = xi;
             }
         }
-    float ur[NB], ui[NB];
-#pragma unroll
-    for (int j = 0; j < NB; ++j) {
-        ur[j] = cr[P + j];
-        ui[j] = ci[P + j];
+    };
+    if (left) {
+        const int cl = (-wk.y - 1) * 64 + t;
+        if (t < 64 && any_swap && cl < P)
+            swap_rows(D.ar + (size_t)cl * lda, D.ai + (size_t)cl * lda);
+        return;
     }
+    const int c0 = P + NB + wk.y * kTrsm64Cols;
+    const int nc = min(kTrsm64Cols, D.ncols - c0);
+    if (any_swap && t < nc)
+        swap_rows(D.ar + (size_t)(c0 + t) * lda, D.ai + (size_t)(c0 + t) * lda);
+    __syncthreads(); // the interchanged rows are visible to the workgroup
+    // stage L11^{-1} ([k][row] from its column-major copy) and the A12 block
+    // (rows P .. P + 63 of the columns c0 ..), one float4 of 4 rows per access
 #pragma unroll
-    for (int j = 0; j < NB; ++j) {
-        float vr = ur[j], vi = ui[j];
-#pragma unroll
-        for (int m = 0; m < j; ++m) {
-            vr -= lr[j][m] * ur[m] - li[j][m] * ui[m];
-            vi -= lr[j][m] * ui[m] + li[j][m] * ur[m];
+    for (int i = 0; i < 4; ++i) {
+        const int e = t + 256 * i;       // 1024 float4 per plane
+        const int k = e >> 4, r4 = (e & 15) * 4;
+        *(float4 *)&sa_r[k][r4] = *(const float4 *)(D.linv + k * NB + r4);
+        *(float4 *)&sa_i[k][r4] = *(const float4 *)(D.linv + NB * NB + k * NB + r4);
+        const int c = e >> 4;            // column of the block, rows r4 .. r4 + 3
+        float4 br = make_float4(0.f, 0.f, 0.f, 0.f), bi = br;
+        if (c < nc) {
+            const size_t o = (size_t)(c0 + c) * lda + P + r4;
+            br = *(const float4 *)(D.ar + o);
+            bi = *(const float4 *)(D.ai + o);
         }
-        ur[j] = vr;
-        ui[j] = vi;
+        sb_r[r4][c] = br.x;
+        sb_r[r4 + 1][c] = br.y;
+        sb_r[r4 + 2][c] = br.z;
+        sb_r[r4 + 3][c] = br.w;
+        sb_i[r4][c] = bi.x;
+        sb_i[r4 + 1][c] = bi.y;
+        sb_i[r4 + 2][c] = bi.z;
+        sb_i[r4 + 3][c] = bi.w;
     }
+    __syncthreads();
+    const int lane = t & 63, wave = t >> 6, wr = wave >> 1, wc = wave & 1;
+    const int lrow = lane & 15, lk = lane >> 4;
+    typename MF::acc accr[2][2], acci[2][2];
 #pragma unroll
-    for (int j = 0; j < NB; ++j) {
-        cr[P + j] = ur[j];
-        ci[P + j] = ui[j];
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+            accr[a][b] = MF::zero();
+            acci[a][b] = MF::zero();
+        }
+#pragma unroll 4
+    for (int kk = 0; kk < NB; kk += 4) {
+        float xr[2], xi[2], yr[2], yi[2];
+#pragma unroll
+        for (int a = 0; a < 2; ++a) {
+            xr[a] = sa_r[kk + lk][wr * 32 + a * 16 + lrow];
+            xi[a] = sa_i[kk + lk][wr * 32 + a * 16 + lrow];
+            yr[a] = sb_r[kk + lk][wc * 32 + a * 16 + lrow];
+            yi[a] = sb_i[kk + lk][wc * 32 + a * 16 + lrow];
+        }
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int b = 0; b < 2; ++b) {
+                // (xr + i xi)(yr + i yi)
+                accr[a][b] = MF::fma(xr[a], yr[b], accr[a][b]);
+                accr[a][b] = MF::fma(-xi[a], yi[b], accr[a][b]);
+                acci[a][b] = MF::fma(xr[a], yi[b], acci[a][b]);
+                acci[a][b] = MF::fma(xi[a], yr[b], acci[a][b]);
+            }
     }
+    // the whole block was staged before any write: the product goes back in place
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+            const int r = wr * 32 + a * 16 + 4 * lk; // rows r .. r + 3 of the block
+            const int c = wc * 32 + b * 16 + lrow;
+            if (c >= nc)
+                continue;
+            const size_t o = (size_t)(c0 + c) * lda + P + r;
+            *(float4 *)(D.ar + o) = make_float4(accr[a][b][0], accr[a][b][1], accr[a][b][2], accr[a][b][3]);
+            *(float4 *)(D.ai + o) = make_float4(acci[a][b][0], acci[a][b][1], acci[a][b][2], acci[a][b][3]);
+        }
 }
 
 inline void lu_launch_update_f32(const LuDesc<float> *d, size_t off, size_t cnt, const int *ints,
@@ -811,8 +921,14 @@ inline void lu_launch_trsm_f32(const LuDesc<float> *d, size_t off, size_t cnt, c
 inline void lu_launch_trsm64_f32(const LuDesc<float> *d, size_t off, size_t cnt, const int *ints, int P,
                                  hipStream_t st)
 {
-    hipLaunchKernelGGL(k_lu_trsm64_f32, dim3((unsigned)cnt), dim3(64), 0, st, d,
+    hipLaunchKernelGGL(k_lu_trsm64_f32, dim3((unsigned)cnt), dim3(256), 0, st, d,
                        (const int2 *)(ints + off), P);
+}
+
+inline void lu_launch_l11_inv(const LuDesc<float> *d, size_t off, size_t cnt, const int *ints, int P,
+                              hipStream_t st)
+{
+    hipLaunchKernelGGL(k_l11_inv, dim3((unsigned)cnt), dim3(64), 0, st, d, ints + off, P);
 }
 
 // Host side: one panel step of the batched factorisation on stream st.
