@@ -50,7 +50,9 @@ __device__ __forceinline__ void quad8(double ox, double oy, double oz, double sx
                                       double a2, double k, double &re, double &im)
 {
     double sr = 0.0, si = 0.0;
-#pragma unroll
+    // rolled: each point inlines an FP64 sincos, and the fully unrolled fill
+    // kernel outgrows the instruction cache
+#pragma unroll 1
     for (int m = 0; m < 8; ++m) {
         double sc = c_gl8n[m] * h;
         double px = ox - (sx + sc * dx);
@@ -71,7 +73,7 @@ __device__ __forceinline__ void quad8(double ox, double oy, double oz, double sx
 __device__ __forceinline__ void self_term(double h, double a, double k, double &re, double &im)
 {
     double sr = 0.0, si = 0.0;
-#pragma unroll
+#pragma unroll 1
     for (int m = 0; m < 4; ++m) {
         double l = c_gl4n[m] * h;
         double r = sqrt(l * l + a * a);
@@ -140,6 +142,7 @@ __global__ __launch_bounds__(256) void k_fill_z(const Prob *__restrict__ probs,
     const double ox = so[row][0], oy = so[row][1], oz = so[row][2];
     const double odx = so[row][3], ody = so[row][4], odz = so[row][5];
     const double k = P.k;
+#pragma unroll 1
     for (int q = 0; q < 4; ++q) {
         const int col = (t >> 6) * 4 + q;
         const int j = j0 + col;

@@ -724,47 +724,41 @@ __global__ __launch_bounds__(256) void k_lu_update_f32(const LuDesc<float> *__re
 
 // L11^{-1} of a 64-column outer step of the FP32 LU (L11 unit lower at
 // (P, P), after both panels' interchanges) into D.linv, column-major, real
-// then imaginary plane. One thread per column c of the inverse solves
-// L x = e_c in registers against L11 staged in LDS; the result goes back
-// through LDS so the stores are coalesced.
-__global__ __launch_bounds__(64) void k_l11_inv(const LuDesc<float> *__restrict__ descs,
-                                                const int *__restrict__ plist, int P)
+// then imaginary plane. Right-looking on X = I in LDS: at step j every
+// X[i][c] with i > j, c <= j takes -= L[i][j] X[j][c], spread over the
+// workgroup, one barrier per step.
+__global__ __launch_bounds__(256) void k_l11_inv(const LuDesc<float> *__restrict__ descs,
+                                                 const int *__restrict__ plist, int P)
 {
     constexpr int NB = 2 * kPanel;
     const LuDesc<float> &D = descs[plist[blockIdx.x]];
     if (D.info[0] != 0 || P + NB >= D.m)
         return;
-    __shared__ float lr[NB][NB + 1], li[NB][NB + 1];
+    __shared__ float lr[NB][NB + 1], li[NB][NB + 1]; // L11 [row][col]
+    __shared__ float xr[NB][NB + 1], xi[NB][NB + 1]; // L11^{-1} [row][col]
     const int t = threadIdx.x, lda = D.lda;
-    for (int e = t; e < NB * NB; e += 64) {
+    for (int e = t; e < NB * NB; e += 256) {
         const int rr = e % NB, cc = e / NB;
         lr[rr][cc] = D.ar[(size_t)(P + cc) * lda + P + rr];
         li[rr][cc] = D.ai[(size_t)(P + cc) * lda + P + rr];
+        xr[rr][cc] = rr == cc ? 1.f : 0.f;
+        xi[rr][cc] = 0.f;
     }
     __syncthreads();
-    float xr[NB], xi[NB];
-#pragma unroll
-    for (int j = 0; j < NB; ++j) {
-        float vr = j == t ? 1.f : 0.f, vi = 0.f;
-#pragma unroll
-        for (int m = 0; m < j; ++m) {
-            vr -= lr[j][m] * xr[m] - li[j][m] * xi[m];
-            vi -= lr[j][m] * xi[m] + li[j][m] * xr[m];
+    for (int j = 0; j < NB - 1; ++j) {
+        const int w = j + 1, cnt = (NB - 1 - j) * w;
+        for (int e = t; e < cnt; e += 256) {
+            const int i = j + 1 + e / w, c = e % w;
+            const float a = lr[i][j], b = li[i][j], x = xr[j][c], y = xi[j][c];
+            xr[i][c] -= a * x - b * y;
+            xi[i][c] -= a * y + b * x;
         }
-        xr[j] = vr;
-        xi[j] = vi;
+        __syncthreads();
     }
-    __syncthreads();
-#pragma unroll
-    for (int j = 0; j < NB; ++j) {
-        lr[j][t] = xr[j];
-        li[j][t] = xi[j];
-    }
-    __syncthreads();
-    for (int e = t; e < NB * NB; e += 64) {
+    for (int e = t; e < NB * NB; e += 256) {
         const int rr = e % NB, cc = e / NB;
-        D.linv[e] = lr[rr][cc];
-        D.linv[NB * NB + e] = li[rr][cc];
+        D.linv[e] = xr[rr][cc];
+        D.linv[NB * NB + e] = xi[rr][cc];
     }
 }
 
@@ -928,7 +922,7 @@ inline void lu_launch_trsm64_f32(const LuDesc<float> *d, size_t off, size_t cnt,
 inline void lu_launch_l11_inv(const LuDesc<float> *d, size_t off, size_t cnt, const int *ints, int P,
                               hipStream_t st)
 {
-    hipLaunchKernelGGL(k_l11_inv, dim3((unsigned)cnt), dim3(64), 0, st, d, ints + off, P);
+    hipLaunchKernelGGL(k_l11_inv, dim3((unsigned)cnt), dim3(256), 0, st, d, ints + off, P);
 }
 
 // Host side: one panel step of the batched factorisation on stream st.
