@@ -665,7 +665,7 @@ size_t problem_bytes(int n, int s, int ncons, int nruns, int precision)
         b += 4 * nn + 16 * nn * nc + 16 * nc * nc + 16 * nc * w + 16 * w * w;
         b += 4 * 16 * ss + 16 * (nn + nc) + 16 * nn + 16 * nchunk * nn + 8 * nn + 80;
         b += 4 * nn + 16 * nn * (nc + 1) + 8 * (nc + 1) * ((nn + 63) / 64);  // perm, solves
-        b += ((nn + 63) / 64) * 2 * 64 * 64 * 8 + 2 * 64 * 64 * 4;   // diagonal-block inverses, L11^{-1}
+        b += ((nn + 63) / 64) * 2 * 64 * 64 * 8 + (2 * 64 * 64 + 2 * 32 * 32) * 4; // block inverses
         b += 8 * 2 * 256 * ((nn + 63) / 64);                         // gang-published blocks
         b += 23 * 256;                                               // alignment of 23 buffers
     } else {
@@ -1003,7 +1003,7 @@ static void plan_build(Plan &pl, const std::vector<const HostProblem *> &hps, bo
             m.colsq = take(cur, 8 * n);
             m.perm = take(cur, 4 * n);
             m.dinv = take(cur, ((n + 63) / 64) * 2 * 64 * 64 * 8);
-            m.linv = take(cur, 2 * 64 * 64 * 4);
+            m.linv = take(cur, (2 * 64 * 64 + 2 * 32 * 32) * 4); // L11^{-1}, then the panel's U_D^{-1}
             m.tws = take(cur, 16 * n * (nc + 1));
             pl.mix.push_back(m);
         }
@@ -1435,7 +1435,7 @@ static int mixed_launch(Plan &pl, char *base)
     hipStream_t side = g_ctx.side_stream;
     auto chain = [&](const Plan::Outer &o) {
         hipEvent_t e = prof_begin(PC_LU_PANEL, side);
-        lu_launch_panel(lud, o.a, ints, side);
+        lu_launch_panel_f32(lud, o.a, ints, side);
         prof_end(PC_LU_PANEL, e, o.has_b ? 0.0 : o.f_panel, 0, side);
         if (o.a.trsm_cnt) {
             e = prof_begin(PC_LU_TRSM, side);
@@ -1449,7 +1449,7 @@ static int mixed_launch(Plan &pl, char *base)
         }
         if (o.has_b) {
             e = prof_begin(PC_LU_PANEL, side);
-            lu_launch_panel(lud, o.b, ints, side);
+            lu_launch_panel_f32(lud, o.b, ints, side);
             prof_end(PC_LU_PANEL, e, o.f_panel, 0, side);
             if (o.b.trsm_cnt) {
                 e = prof_begin(PC_LU_TRSM, side);

@@ -317,9 +317,205 @@ __global__ __launch_bounds__(256) void k_panel_rows(const LuDesc<T> *__restrict_
     }
 }
 
+// ---- FP32 fast panel (mixed path) ----
+// The multipliers satisfy |t_rj| = |l_rj| |u_jj|, so "no interchange was due"
+// reads max_r |l_rj| < 1 (strict: a tie selects the later row). The diagonal
+// block is factored in LDS and U_D^{-1} formed; the rows below are then
+// l_r = a_r U_D^{-1}, a (rows x 32) x (32 x 32) complex product on MFMA.
+
+// One wavefront per problem: LU of the diagonal block (one lane per row, the
+// block padded to 32 x 32 with the identity), max |l| per column, then
+// U_D^{-1} right-looking across the wave into D.linv + 2 x 64 x 64 ([k][c]).
+__global__ __launch_bounds__(64) void k_panel_diag_f32(const LuDesc<float> *__restrict__ descs,
+                                                       const int *__restrict__ plist, int p)
+{
+    constexpr int NB = kPanel;
+    const LuDesc<float> &D = descs[plist[blockIdx.x]];
+    const int s = D.m;
+    if (D.info[0] != 0 || p >= s)
+        return;
+    const int jb = min(NB, s - p);
+    const int lda = D.lda, lane = threadIdx.x;
+    __shared__ float dr[NB][NB + 1], di[NB][NB + 1];
+    __shared__ float xr[NB][NB + 1], xi[NB][NB + 1];
+    for (int e = lane; e < NB * NB; e += 64) {
+        const int r = e % NB, c = e / NB;
+        const bool in = r < jb && c < jb;
+        dr[r][c] = in ? D.ar[(size_t)(p + c) * lda + p + r] : (r == c ? 1.f : 0.f);
+        di[r][c] = in ? D.ai[(size_t)(p + c) * lda + p + r] : 0.f;
+        xr[r][c] = r == c ? 1.f : 0.f;
+        xi[r][c] = 0.f;
+    }
+    __syncthreads();
+    float lmax = 0.f; // lane j: max |l_ij| of column j within the block
+    for (int j = 0; j < NB; ++j) {
+        const float ur = dr[j][j], ui = di[j][j];
+        const float nsq = ur * ur + ui * ui;
+        const float ir = ur / nsq, ii = -ui / nsq;
+        float m = 0.f;
+        if (lane > j && lane < NB) {
+            const float x = dr[lane][j], y = di[lane][j];
+            const float lr = x * ir - y * ii, li = x * ii + y * ir;
+            dr[lane][j] = lr;
+            di[lane][j] = li;
+            m = lr * lr + li * li;
+#pragma unroll 4
+            for (int c = j + 1; c < NB; ++c) {
+                const float vr = dr[j][c], vi = di[j][c];
+                dr[lane][c] -= lr * vr - li * vi;
+                di[lane][c] -= lr * vi + li * vr;
+            }
+        }
+        for (int off = 32; off > 0; off >>= 1)
+            m = fmaxf(m, __shfl_xor(m, off, 64));
+        if (lane == j)
+            lmax = m;
+        __syncthreads();
+    }
+    // U_D^{-1}: row i of X is final once scaled by 1/u_ii; then every row
+    // k < i takes -= u_ki X[i][:]
+    for (int i = NB - 1; i >= 0; --i) {
+        const float ur = dr[i][i], ui = di[i][i];
+        const float nsq = ur * ur + ui * ui;
+        const float ir = ur / nsq, ii = -ui / nsq;
+        if (lane >= i && lane < NB) {
+            const float x = xr[i][lane], y = xi[i][lane];
+            xr[i][lane] = x * ir - y * ii;
+            xi[i][lane] = x * ii + y * ir;
+        }
+        __syncthreads();
+        const int w = NB - i;
+        for (int e = lane; e < i * w; e += 64) {
+            const int k = e / w, c = i + e % w;
+            const float a = dr[k][i], b = di[k][i], x = xr[i][c], y = xi[i][c];
+            xr[k][c] -= a * x - b * y;
+            xi[k][c] -= a * y + b * x;
+        }
+        __syncthreads();
+    }
+    for (int e = lane; e < jb * jb; e += 64) {
+        const int r = e % jb, c = e / jb;
+        D.pr[(size_t)c * lda + p + r] = dr[r][c];
+        D.pi[(size_t)c * lda + p + r] = di[r][c];
+    }
+    float *ui_r = D.linv + 2 * 64 * 64, *ui_i = ui_r + NB * NB;
+    for (int e = lane; e < NB * NB; e += 64) {
+        ui_r[e] = xr[e / NB][e % NB];
+        ui_i[e] = xi[e / NB][e % NB];
+    }
+    if (lane < jb) {
+        D.diagmag[lane] = (double)__builtin_sqrtf(dr[lane][lane] * dr[lane][lane] + di[lane][lane] * di[lane][lane]);
+        D.colmax[lane] = (unsigned long long)__double_as_longlong((double)__builtin_sqrtf(lmax));
+    }
+}
+
+// Rows [p + 32 + 256 y, + 256) below a full diagonal block: l_r = a_r U_D^{-1}
+// on MFMA (four waves of 64 rows x 32 columns, K = 32) into the panel
+// workspace, and max |l| per column into colmax. Row quads are whole (the
+// padding rows past m hold 0).
+__global__ __launch_bounds__(256) void k_panel_rows_f32(const LuDesc<float> *__restrict__ descs,
+                                                        const int2 *__restrict__ work, int p)
+{
+    typedef Mfma<float> MF;
+    constexpr int NB = kPanel, TM = 256, LA = TM + 16, LB = NB + 4;
+    const int2 wk = work[blockIdx.x];
+    const LuDesc<float> &D = descs[wk.x];
+    const int s = D.m;
+    if (D.info[0] != 0 || p + NB >= s)
+        return;
+    const int lda = D.lda, t = threadIdx.x;
+    const int r0 = p + NB + wk.y * TM;
+    __shared__ float sa_r[NB][LA], sa_i[NB][LA]; // a: [k][row]
+    __shared__ float sb_r[NB][LB], sb_i[NB][LB]; // U_D^{-1}: [k][c]
+    __shared__ unsigned cmx[NB];
+    if (t < NB)
+        cmx[t] = 0u;
+#pragma unroll
+    for (int i = 0; i < (NB * TM / 4) / 256; ++i) {
+        const int e = t + 256 * i;
+        const int k = e / (TM / 4), r4 = (e % (TM / 4)) * 4;
+        const int gr = r0 + r4;
+        float4 vr = make_float4(0.f, 0.f, 0.f, 0.f), vi = vr;
+        if (gr < s) {
+            const size_t o = (size_t)(p + k) * lda + gr;
+            vr = *(const float4 *)(D.ar + o);
+            vi = *(const float4 *)(D.ai + o);
+        }
+        *(float4 *)&sa_r[k][r4] = vr;
+        *(float4 *)&sa_i[k][r4] = vi;
+    }
+    {
+        const float *ui_r = D.linv + 2 * 64 * 64, *ui_i = ui_r + NB * NB;
+        for (int e = t; e < NB * NB; e += 256) {
+            sb_r[e / NB][e % NB] = ui_r[e];
+            sb_i[e / NB][e % NB] = ui_i[e];
+        }
+    }
+    __syncthreads();
+    const int lane = t & 63, wave = t >> 6, lrow = lane & 15, lk = lane >> 4;
+    typename MF::acc accr[4][2], acci[4][2];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+            accr[a][b] = MF::zero();
+            acci[a][b] = MF::zero();
+        }
+#pragma unroll 2
+    for (int kk = 0; kk < NB; kk += 4) {
+        float xr[4], xi[4], yr[2], yi[2];
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+            xr[a] = sa_r[kk + lk][wave * 64 + a * 16 + lrow];
+            xi[a] = sa_i[kk + lk][wave * 64 + a * 16 + lrow];
+        }
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+            yr[b] = sb_r[kk + lk][b * 16 + lrow];
+            yi[b] = sb_i[kk + lk][b * 16 + lrow];
+        }
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int b = 0; b < 2; ++b) {
+                accr[a][b] = MF::fma(xr[a], yr[b], accr[a][b]);
+                accr[a][b] = MF::fma(-xi[a], yi[b], accr[a][b]);
+                acci[a][b] = MF::fma(xr[a], yi[b], acci[a][b]);
+                acci[a][b] = MF::fma(xi[a], yr[b], acci[a][b]);
+            }
+    }
+    float m[2] = {0.f, 0.f};
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+            const int r = r0 + wave * 64 + a * 16 + 4 * lk; // rows r .. r + 3
+            const int c = b * 16 + lrow;
+            if (r >= s)
+                continue;
+            const size_t o = (size_t)c * lda + r;
+            *(float4 *)(D.pr + o) = make_float4(accr[a][b][0], accr[a][b][1], accr[a][b][2], accr[a][b][3]);
+            *(float4 *)(D.pi + o) = make_float4(acci[a][b][0], acci[a][b][1], acci[a][b][2], acci[a][b][3]);
+#pragma unroll
+            for (int v = 0; v < 4; ++v)
+                m[b] = fmaxf(m[b], accr[a][b][v] * accr[a][b][v] + acci[a][b][v] * acci[a][b][v]);
+        }
+#pragma unroll
+    for (int b = 0; b < 2; ++b) {
+        m[b] = fmaxf(m[b], __shfl_xor(m[b], 16, 64));
+        m[b] = fmaxf(m[b], __shfl_xor(m[b], 32, 64));
+        if (lk == 0)
+            atomicMax(&cmx[b * 16 + lrow], __float_as_uint(m[b])); // non-negative: bits order as values
+    }
+    __syncthreads();
+    if (t < NB)
+        atomicMax(&D.colmax[t], (unsigned long long)__double_as_longlong(
+                                    (double)__builtin_sqrtf(__uint_as_float(cmx[t]))));
+}
+
 // Decide and commit: every workgroup reads the same per-column maxima, so all
 // agree; on success the panel rows of the workspace replace the matrix panel.
-template <typename T>
+template <typename T, bool kUnitL = false>
 __global__ __launch_bounds__(256) void k_panel_commit(const LuDesc<T> *__restrict__ descs,
                                                       const int2 *__restrict__ work, int p)
 {
@@ -334,7 +530,9 @@ __global__ __launch_bounds__(256) void k_panel_commit(const LuDesc<T> *__restric
     if (t < jb) {
         const double m = __longlong_as_double((long long)D.colmax[t]);
         const double d = D.diagmag[t];
-        bad = (!(m < d) || d < kSingular) ? 1 : 0;
+        // colmax holds max |t_rj| (compared with |u_jj|) or, for the FP32
+        // panel, max |l_rj| = max |t_rj| / |u_jj| (compared with 1)
+        bad = (!(m < (kUnitL ? 1.0 : d)) || d < kSingular) ? 1 : 0;
     }
     const int ok = !__syncthreads_or(bad);
     if (t == 0 && wk.y == 0)
@@ -937,6 +1135,20 @@ void lu_launch_panel(const LuDesc<T> *d, const LuStep &s, const int *ints, hipSt
     hipLaunchKernelGGL(k_panel_commit<T>, dim3((unsigned)s.commit_cnt), dim3(256), 0, st, d,
                        (const int2 *)(ints + s.commit_off), s.p);
     hipLaunchKernelGGL(k_lu_panel<T>, dim3((unsigned)s.panel_cnt), dim3(kPanelThreads), 0, st, d,
+                       ints + s.panel_off, s.p);
+}
+
+// FP32 panel step of the mixed path (U_D^{-1} on the diagonal block, rows on MFMA).
+inline void lu_launch_panel_f32(const LuDesc<float> *d, const LuStep &s, const int *ints, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_panel_diag_f32, dim3((unsigned)s.panel_cnt), dim3(64), 0, st, d, ints + s.panel_off,
+                       s.p);
+    if (s.rows_cnt)
+        hipLaunchKernelGGL(k_panel_rows_f32, dim3((unsigned)s.rows_cnt), dim3(256), 0, st, d,
+                           (const int2 *)(ints + s.rows_off), s.p);
+    hipLaunchKernelGGL((k_panel_commit<float, true>), dim3((unsigned)s.commit_cnt), dim3(256), 0, st, d,
+                       (const int2 *)(ints + s.commit_off), s.p);
+    hipLaunchKernelGGL(k_lu_panel<float>, dim3((unsigned)s.panel_cnt), dim3(kPanelThreads), 0, st, d,
                        ints + s.panel_off, s.p);
 }
 
