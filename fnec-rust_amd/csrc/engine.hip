@@ -36,9 +36,9 @@ __global__ void k_zero(double *, size_t);
 __global__ void k_trsv0(const Prob *, const int4 *, int, unsigned long long *);
 __global__ void k_trsv1(const Prob *, const int4 *, int, unsigned long long *);
 __global__ void k_perm(const Prob *, const int *);
-__global__ void k_trsv_multi(const Prob *, const int4 *, int, unsigned long long *);
 __global__ void k_zsolve(const Prob *, const int4 *, int, int, unsigned long long *);
 __global__ void k_gen_items(const int4 *, int4 *);
+__global__ void k_fsetup(const Prob *, const int4 *, int, unsigned long long *);
 __global__ void k_diag_inv(const Prob *, const int2 *);
 __global__ void k_pre_setup(const Prob *, const int *);
 __global__ void k_prec_mid(const Prob *, const int *);
@@ -615,14 +615,20 @@ struct Plan {
     std::vector<long> it_zs1, it_zs0, it_mv; // profiling records per CG iteration
     int iters_launched = 0;
     double fsol_flops = 0, fsol_bytes = 0;
+    // k_fsetup launches: one per group of 16 constraint rows (group g covers
+    // every problem with more than 16 g rows); items (problem, member, G, g)
+    std::vector<size_t> fs_off, fs_cnt;
+    size_t fs_lds = 0;
     size_t off_flags = 0;
 };
 
 constexpr int kMixMaxIt = 12;        // CG iterations at most
 constexpr int kMixFirstChunk = 4;    // iterations launched before the first convergence check
 constexpr int kMixNextChunk = 2;
-constexpr int kMaxTrsvLaunches = 4 * kMixMaxIt + 1; // F set-up + two kernels x two solves per CG iteration
-constexpr int kMultiRhsHost = 16;                   // right-hand sides per set-up chain (k_trsv_multi)
+constexpr int kFsRhsHost = 16; // constraint rows per k_fsetup gang
+// launches with a ticket and an epoch: the set-up groups, then two kernels x
+// two solves per CG iteration
+constexpr int kMaxTrsvLaunches = kMixMaxCons / kFsRhsHost + 4 * kMixMaxIt + 1;
 constexpr int kSolve1MaxNHost = 8192;               // largest n the one-workgroup solve holds in LDS
 
 // Tunables (fnec_hip_set_tunable): the autotuner's knobs.
@@ -666,7 +672,7 @@ size_t problem_bytes(int n, int s, int ncons, int nruns, int precision)
         b += 4 * 16 * ss + 16 * (nn + nc) + 16 * nn + 16 * nchunk * nn + 8 * nn + 80;
         b += 4 * nn + 16 * nn * (nc + 1) + 8 * (nc + 1) * ((nn + 63) / 64);  // perm, solves
         b += ((nn + 63) / 64) * 2 * 64 * 64 * 8 + (2 * 64 * 64 + 2 * 32 * 32) * 4; // block inverses
-        b += 8 * 2 * 256 * ((nn + 63) / 64);                         // gang-published blocks
+        b += 8 * 2 * 256 * 16 * ((nn + 63) / 64);                    // gang-published blocks
         b += 23 * 256;                                               // alignment of 23 buffers
     } else {
         b += 2 * 8 * lda * (ss + 1) + 256;                           // [G | M^H y]
@@ -1015,7 +1021,7 @@ static void plan_build(Plan &pl, const std::vector<const HostProblem *> &hps, bo
         for (size_t b = 0; b < np; ++b) {
             const size_t nb = (hps[b]->n + 63) / 64, nc = std::max(hps[b]->ncons, 0);
             pl.mix[b].tflags = take(cur, 4 * (nc + 1) * 2 * nb);
-            pl.mix[b].gx = take(cur, 8 * 2 * nb * 64 * 4);
+            pl.mix[b].gx = take(cur, 8 * 2 * nb * 64 * kFsRhsHost * 4); // 16 vectors (set-up), 1 (CG)
         }
         pl.off_tickets = take(cur, 8 * kMaxTrsvLaunches);
         pl.flags_end = cur = (cur + 7) & ~(size_t)7;
@@ -1080,25 +1086,21 @@ static void plan_build(Plan &pl, const std::vector<const HostProblem *> &hps, bo
         for (size_t b = 0; b < np; ++b)
             ms[b] = hps[b]->n;
         lu_plan_outer(pl.ints, ms, pl.outer32, pl.gensegs, pl.gen_items);
-        // triangular-solve items (problem, vector, phase, block): phase 0
-        // blocks in forward order, then phase 1 blocks in backward order,
-        // interleaved across chains so that early tickets rarely wait
-        auto trsv_list = [&](bool fsetup, size_t &off, size_t &cnt) {
+        // CG solves of problems above solve1_max_n (flag-chained k_trsv0/1):
+        // items (problem, -1, phase, block), phase 0 blocks in forward order,
+        // then phase 1 blocks in backward order, interleaved across problems
+        // so that early tickets rarely wait
+        auto trsv_list = [&](size_t &off, size_t &cnt) {
             align4();
             off = pl.ints.size();
             std::vector<int3> chains; // (problem, vector, blocks)
             int maxnb = 0;
             for (size_t b = 0; b < np; ++b) {
                 const int nb = (hps[b]->n + 63) / 64;
-                if (!fsetup && hps[b]->n <= g_solve1_max_n)
+                if (hps[b]->n <= g_solve1_max_n)
                     continue; // CG solves of this problem run in k_zsolve
                 maxnb = std::max(maxnb, nb);
-                if (fsetup) {
-                    for (int g = 0; g < (hps[b]->ncons + kMultiRhsHost - 1) / kMultiRhsHost; ++g)
-                        chains.push_back(make_int3((int)b, g, nb));
-                } else {
-                    chains.push_back(make_int3((int)b, -1, nb));
-                }
+                chains.push_back(make_int3((int)b, -1, nb));
             }
             for (int ph = 0; ph < 2; ++ph)
                 for (int st = 0; st < maxnb; ++st)
@@ -1113,8 +1115,42 @@ static void plan_build(Plan &pl, const std::vector<const HostProblem *> &hps, bo
             for (int y = 0; y < (hps[b]->n + 63) / 64; ++y)
                 push2((int)b, y);
         pl.dinv_cnt = (pl.ints.size() - pl.dinv_off) / 2;
-        trsv_list(true, pl.fsol_off, pl.fsol_cnt);
-        trsv_list(false, pl.vsol_off, pl.vsol_cnt);
+        // set-up gangs: at most 8 owned blocks per member (16 vectors per row in LDS)
+        pl.fs_off.clear();
+        pl.fs_cnt.clear();
+        pl.fs_lds = 0;
+        pl.fsol_cnt = 0;
+        {
+            int maxg = 0;
+            for (size_t b = 0; b < np; ++b)
+                maxg = std::max(maxg, (hps[b]->ncons + kFsRhsHost - 1) / kFsRhsHost);
+            std::vector<std::pair<int, int>> order; // (G, problem)
+            for (size_t b = 0; b < np; ++b) {
+                const int nb = (hps[b]->n + 63) / 64;
+                order.push_back({std::max(gang_size(hps[b]->n), (nb + 7) / 8), (int)b});
+            }
+            std::stable_sort(order.begin(), order.end(), [&](const std::pair<int, int> &a, const std::pair<int, int> &c) {
+                if (a.first != c.first)
+                    return a.first > c.first;
+                return hps[a.second]->n > hps[c.second]->n;
+            });
+            for (int g = 0; g < maxg; ++g) {
+                align4();
+                pl.fs_off.push_back(pl.ints.size());
+                for (const auto &o : order) {
+                    const HostProblem &h = *hps[o.second];
+                    if (h.ncons <= g * kFsRhsHost)
+                        continue;
+                    const int G = o.first, nb = (h.n + 63) / 64;
+                    for (int mbr = 0; mbr < G; ++mbr)
+                        push4(o.second, mbr, G, g);
+                    pl.fs_lds = std::max(pl.fs_lds, (size_t)16 * 64 * kFsRhsHost * ((nb + G - 1) / G));
+                }
+                pl.fs_cnt.push_back((pl.ints.size() - pl.fs_off.back()) / 4);
+                pl.fsol_cnt += pl.fs_cnt.back();
+            }
+        }
+        trsv_list(pl.vsol_off, pl.vsol_cnt);
         align4();
         // k_zsolve items, gangs first and larger problems first: a gang's
         // members take consecutive tickets and the longest solves start first
@@ -1173,7 +1209,7 @@ static void plan_build(Plan &pl, const std::vector<const HostProblem *> &hps, bo
             pl.mv_flops += 2.0 * 8.0 * n * r;  // M p and M^H t
             pl.mv_bytes += 2.0 * 16.0 * n * n; // FP64 Z read twice
             pl.fsol_flops += 8.0 * n * n * hps[b]->ncons;
-            pl.fsol_bytes += 8.0 * n * n * ((hps[b]->ncons + kMultiRhsHost - 1) / kMultiRhsHost);
+            pl.fsol_bytes += 8.0 * n * n * ((hps[b]->ncons + kFsRhsHost - 1) / kFsRhsHost);
         }
     } else if (with_solve) {
         align4();
@@ -1534,9 +1570,15 @@ static int mixed_launch(Plan &pl, char *base)
     hipLaunchKernelGGL(k_mix_init, dim3(np), dim3(1024), 0, st, dp, all);
     if (pl.fsol_cnt) {
         hipEvent_t e = prof_begin(PC_FSETUP);
-        hipLaunchKernelGGL(k_trsv_multi, dim3((unsigned)pl.fsol_cnt), dim3(256), 0, st, dp,
-                           (const int4 *)(ints + pl.fsol_off), ntrsv + 1, tickets + ntrsv);
-        ++ntrsv;
+        if (pl.fs_lds > 64 * 1024)
+            HIPCHK(hipFuncSetAttribute((const void *)k_fsetup, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)pl.fs_lds));
+        for (size_t g = 0; g < pl.fs_off.size(); ++g)
+            if (pl.fs_cnt[g]) {
+                hipLaunchKernelGGL(k_fsetup, dim3((unsigned)pl.fs_cnt[g]), dim3(1024), pl.fs_lds, st, dp,
+                                   (const int4 *)(ints + pl.fs_off[g]), ntrsv + 1, tickets + ntrsv);
+                ++ntrsv;
+            }
         prof_end(PC_FSETUP, e, pl.fsol_flops, pl.fsol_bytes);
     }
     hipLaunchKernelGGL(k_pre_setup, dim3(np), dim3(1024), 0, st, dp, all);

@@ -18,8 +18,9 @@
 // Kernels:
 //   k_trsv0/1    Z v = b or Z^H v = b with the FP32 factors, FP64 vectors;
 //                one workgroup per 64-row block, blocks chained by flags.
-//   k_zsolve1    the same solves with one workgroup per problem (batches).
-//   k_trsv_multi Z^H solves of the constraint rows, 16 right-hand sides per chain.
+//   k_zsolve     the same solves in batches: one workgroup per problem, or a
+//                gang of workgroups sharing a large problem.
+//   k_fsetup     Z^H solves of the constraint rows by gangs, 16 right-hand sides.
 //   k_perm       the row permutation of the factors.
 //   k_diag_inv   inverses of the 64 x 64 diagonal blocks of L and U.
 //   k_pre_setup  K, H, their Cholesky / Schur factors (one workgroup per problem).
@@ -327,140 +328,6 @@ __global__ __launch_bounds__(256) void k_trsv1(const Prob *__restrict__ probs, c
                                                int epoch, unsigned long long *__restrict__ ticket)
 {
     trsv_body<1>(probs, work, epoch, ticket);
-}
-
-// Z^H solves of up to kMultiRhs right-hand sides at once (the constraint rows
-// of the preconditioner set-up): item (problem, group, phase, block), group g
-// covering constraint rows [16 g, 16 g + 16). Same chaining as k_trsv (mode
-// 1); each factor block is staged in LDS once and applied to every vector.
-constexpr int kMultiRhs = 16;
-
-__global__ __launch_bounds__(256) void k_trsv_multi(const Prob *__restrict__ probs,
-                                                    const int4 *__restrict__ work, int epoch,
-                                                    unsigned long long *__restrict__ ticket)
-{
-    __shared__ int s_item, s_ok;
-    __shared__ float fr_[kTb][kTb + 1], fi_[kTb][kTb + 1]; // factor block, then the inverse
-    __shared__ double2 xv[kTb][kMultiRhs];                   // solved values of block j / acc
-    const int t = threadIdx.x, lane = t & 63, q = t >> 6;
-    if (t == 0) {
-        s_item = (int)atomicAdd(ticket, 1ULL);
-        s_ok = 1;
-    }
-    __syncthreads();
-    const int4 wk = work[s_item];
-    const Prob &P = probs[wk.x];
-    const int grp = wk.y, phase = wk.z, blk = wk.w;
-    if (P.info[0] != 0)
-        return;
-    const int n = P.n, ld = P.ldz, nb = (n + kTb - 1) / kTb;
-    const int v0 = grp * kMultiRhs, nv = min(kMultiRhs, P.ncons - v0);
-    const int i0 = blk * kTb, jb = min(kTb, n - i0);
-    const float *Fr = P.fr, *Fi = P.fi;
-    // thread (row = lane, vectors q, q + 4, q + 8, q + 12)
-    double2 acc[4];
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        const int v = q + 4 * k;
-        acc[k] = make_double2(0.0, 0.0);
-        if (v < nv && lane < jb) {
-            if (phase == 0)
-                acc[k] = make_double2(cons_coef(P, v0 + v, i0 + lane), 0.0);
-            else
-                acc[k] = make_double2(1.0, 0.0); // placeholder, set after the wait
-        }
-    }
-    if (phase == 1) {
-        if (!wait_flag(P, trsv_flag(P, grp, 0, nb - 1), epoch, &s_ok))
-            return;
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            const int v = q + 4 * k;
-            if (v < nv && lane < jb)
-                acc[k] = ((const double2 *)P.tws)[(size_t)n * (v0 + v + 1) + i0 + lane];
-        }
-    }
-    const bool fwd = phase == 0;
-    const int jfirst = fwd ? 0 : nb - 1, jstep = fwd ? 1 : -1;
-    for (int j = jfirst; j != blk; j += jstep) {
-        if (!wait_flag(P, trsv_flag(P, grp, phase, j), epoch, &s_ok))
-            return;
-        const int j0 = j * kTb, jn = min(kTb, n - j0);
-        // factor block, transposed on the way in: fr_[r][c] = F[j0 + c][i0 + r]
-        for (int e = t; e < kTb * kTb; e += 256) {
-            const int c = e % kTb, r = e / kTb;
-            float a = 0.f, b = 0.f;
-            if (c < jn && r < jb) {
-                const size_t o = (size_t)(i0 + r) * ld + j0 + c;
-                a = Fr[o];
-                b = Fi[o];
-            }
-            fr_[r][c] = a;
-            fi_[r][c] = b;
-        }
-        for (int e = t; e < kTb * kMultiRhs; e += 256) {
-            const int c = e / kMultiRhs, v = e % kMultiRhs;
-            xv[c][v] = (c < jn && v < nv)
-                           ? ((const double2 *)P.tws)[(size_t)n * (v0 + v + 1) + j0 + c]
-                           : make_double2(0.0, 0.0);
-        }
-        __syncthreads();
-        // U^H[r][c] = conj(U[c][r]) (phase 0), L^H[r][c] = conj(L[c][r]) (phase 1)
-        for (int c = 0; c < kTb; ++c) {
-            const double ar = fr_[lane][c], ai = fi_[lane][c];
-#pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                const double2 x = xv[c][q + 4 * k];
-                acc[k].x -= ar * x.x + ai * x.y;
-                acc[k].y -= ar * x.y - ai * x.x;
-            }
-        }
-        __syncthreads();
-    }
-    // diagonal block: x = (T_ii^{-1})^H acc with the stored inverse of U_ii
-    // (phase 0) or L_ii (phase 1)
-    {
-        const float *src = P.dinv + ((size_t)blk * 2 + (phase == 0 ? 1 : 0)) * 2 * kTb * kTb;
-        for (int e = t; e < kTb * kTb; e += 256) {
-            const int r = e % kTb, c = e / kTb;
-            fr_[r][c] = src[2 * e];
-            fi_[r][c] = src[2 * e + 1];
-        }
-#pragma unroll
-        for (int k = 0; k < 4; ++k)
-            xv[lane][q + 4 * k] = lane < jb ? acc[k] : make_double2(0.0, 0.0);
-        __syncthreads();
-        double2 x[4];
-#pragma unroll
-        for (int k = 0; k < 4; ++k)
-            x[k] = make_double2(0.0, 0.0);
-        for (int c = 0; c < kTb; ++c) {
-            const double ar = fr_[c][lane], ai = fi_[c][lane]; // conj(inv[c][r])
-#pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                const double2 a = xv[c][q + 4 * k];
-                x[k].x += ar * a.x + ai * a.y;
-                x[k].y += ar * a.y - ai * a.x;
-            }
-        }
-        double2 *ws = (double2 *)P.tws;
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            const int v = q + 4 * k;
-            if (v < nv && lane < jb) {
-                const int r = i0 + lane;
-                ws[(size_t)n * (v0 + v + 1) + r] = x[k];
-                if (phase == 1)
-                    ((double2 *)P.fh)[(size_t)n * (v0 + v) + P.perm[r]] = x[k];
-            }
-        }
-        // publish after every wave's stores are complete
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-        __syncthreads();
-        if (t == 0)
-            __hip_atomic_store(trsv_flag(P, grp, phase, blk), epoch, __ATOMIC_RELEASE,
-                               __HIP_MEMORY_SCOPE_AGENT);
-    }
 }
 
 // Inverses of the 64 x 64 diagonal blocks of the FP32 factors: L_ii^{-1}
@@ -1166,6 +1033,195 @@ __device__ void zsolve_gang_body(const Prob &P, int mode, int m, int G, int epoc
         const int i = (m + (e / kTb) * G) * kTb + e % kTb;
         if (i < n)
             io[mode == 0 ? i : P.perm[i]] = own[e];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Preconditioner set-up: Z^H solves of the constraint rows by gangs
+// ---------------------------------------------------------------------------
+//
+// F^H = Z^{-H} E^H for up to kFsRhs constraint rows per item: the Z^H form of
+// the gang solve above, with kFsRhs vectors per row. A thread owns one
+// (row, vector) pair; the factor row segment it reads is the same for the 16
+// vector lanes of a row. The published words carry kFsRhs entries per row.
+// Work item (problem, member, G, group): constraint rows [16 group, + 16).
+constexpr int kFsRhs = 16;
+
+__device__ bool ll_fetch_multi(const Prob &P, const unsigned long long *w, int valid_rows, unsigned epoch,
+                               double2 *x, int *s_ok)
+{
+    const int t = threadIdx.x, r = t >> 4; // entry t = (row r, vector t % 16)
+    double2 v = make_double2(0.0, 0.0);
+    if (r < valid_rows) {
+        const unsigned long long *p = w + 4 * t;
+        long spins = 0;
+        unsigned long long q0, q1, q2, q3;
+        for (;;) {
+            q0 = __hip_atomic_load(p + 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            q1 = __hip_atomic_load(p + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            q2 = __hip_atomic_load(p + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            q3 = __hip_atomic_load(p + 3, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if ((unsigned)(q0 >> 32) == epoch && (unsigned)(q1 >> 32) == epoch &&
+                (unsigned)(q2 >> 32) == epoch && (unsigned)(q3 >> 32) == epoch)
+                break;
+            ++spins;
+            if ((spins & 255) == 0 &&
+                __hip_atomic_load(&P.pstate[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
+                *s_ok = 0;
+                break;
+            }
+            if (spins > kSpinLimit) {
+                *s_ok = 0;
+                __hip_atomic_store(&P.pstate[2], 4, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+        v.x = __longlong_as_double((long long)((q0 & 0xffffffffull) | (q1 << 32)));
+        v.y = __longlong_as_double((long long)((q2 & 0xffffffffull) | (q3 << 32)));
+    }
+    x[t] = v;
+    __syncthreads();
+    return *s_ok != 0;
+}
+
+__global__ __launch_bounds__(kSolve1Threads) void k_fsetup(const Prob *__restrict__ probs,
+                                                           const int4 *__restrict__ items, int epoch,
+                                                           unsigned long long *__restrict__ ticket)
+{
+    extern __shared__ double2 own[]; // owned blocks: [block][row][vector]
+    __shared__ double2 xa[kTb * kFsRhs];
+    __shared__ int s_item, s_ok;
+    const int t = threadIdx.x, r = t >> 4, v = t & 15;
+    if (t == 0) {
+        s_item = (int)atomicAdd(ticket, 1ULL);
+        s_ok = 1;
+    }
+    __syncthreads();
+    const int4 it = items[s_item];
+    const Prob &P = probs[it.x];
+    if (P.info[0] != 0)
+        return;
+    const int m = it.y, G = it.z, v0 = it.w * kFsRhs;
+    const int n = P.n, ld = P.ldz, nb = (n + kTb - 1) / kTb;
+    const int nv = min(kFsRhs, P.ncons - v0);
+    const int nown = (nb - m + G - 1) / G;
+    const float *__restrict__ Fr = P.fr;
+    const float *__restrict__ Fi = P.fi;
+    constexpr int BS = kTb * kFsRhs; // entries per block
+    for (int k = 0; k < nown; ++k) {
+        const int i = (m + k * G) * kTb + r;
+        own[k * BS + t] = (i < n && v < nv) ? make_double2(cons_coef(P, v0 + v, i), 0.0) : make_double2(0.0, 0.0);
+    }
+    __syncthreads();
+    // own_b[r][v] -= sum_c conj(F[b0 + r][j0 + c]) x[c][v]: a row of U^H / L^H
+    // is a column segment of the factor
+    auto update = [&](int j, int k, const double2 *x) {
+        const int b0 = (m + k * G) * kTb, j0 = j * kTb, jc = min(kTb, n - j0);
+        if (b0 + r >= n)
+            return;
+        const size_t col = (size_t)(b0 + r) * ld + j0;
+        double sr = 0.0, si = 0.0;
+#pragma unroll 4
+        for (int c4 = 0; c4 < kTb; c4 += 4) {
+            if (c4 >= jc)
+                break;
+            float4 fr = *(const float4 *)(Fr + col + c4), fi = *(const float4 *)(Fi + col + c4);
+            if (c4 + 3 >= jc) { // entries past n: column padding
+                if (c4 + 1 >= jc) fr.y = fi.y = 0.f;
+                if (c4 + 2 >= jc) fr.z = fi.z = 0.f;
+                fr.w = fi.w = 0.f;
+            }
+            const double2 x0 = x[(c4 + 0) * kFsRhs + v], x1 = x[(c4 + 1) * kFsRhs + v];
+            const double2 x2 = x[(c4 + 2) * kFsRhs + v], x3 = x[(c4 + 3) * kFsRhs + v];
+            sr += (double)fr.x * x0.x + (double)fi.x * x0.y + (double)fr.y * x1.x + (double)fi.y * x1.y +
+                  (double)fr.z * x2.x + (double)fi.z * x2.y + (double)fr.w * x3.x + (double)fi.w * x3.y;
+            si += (double)fr.x * x0.y - (double)fi.x * x0.x + (double)fr.y * x1.y - (double)fi.y * x1.x +
+                  (double)fr.z * x2.y - (double)fi.z * x2.x + (double)fr.w * x3.y - (double)fi.w * x3.x;
+        }
+        double2 &o = own[k * BS + t];
+        o = make_double2(o.x - sr, o.y - si);
+    };
+    for (int phase = 0; phase < 2; ++phase) {
+        const bool fwd = phase == 0;
+        const int which = fwd ? 1 : 0; // (U_jj^{-1})^H, then (L_jj^{-1})^H
+        unsigned long long *pub = P.gx + (size_t)phase * nb * BS * 4;
+        auto range = [&](int j, int &ka, int &kb) {
+            if (fwd) {
+                ka = j < m ? 0 : (j - m) / G + 1;
+                kb = nown;
+            } else {
+                ka = 0;
+                kb = j > m ? (j - m + G - 1) / G : 0;
+            }
+        };
+        // x_j = (D_j^{-1})^H own_j for an owned block (in place), published
+        auto solve = [&](int j) {
+            const int k = (j - m) / G, j0 = j * kTb, jc = min(kTb, n - j0);
+            const float *src = P.dinv + ((size_t)j * 2 + which) * 2 * kTb * kTb;
+            double sr = 0.0, si = 0.0;
+            for (int c = 0; c < jc; ++c) {
+                const size_t e = (size_t)r * kTb + c; // conj(D[c][r])
+                const double ar = src[2 * e], ai = -src[2 * e + 1];
+                const double2 x = own[k * BS + c * kFsRhs + v];
+                sr += ar * x.x - ai * x.y;
+                si += ar * x.y + ai * x.x;
+            }
+            __syncthreads(); // every thread has read the block
+            const double2 xv = r < jc ? make_double2(sr, si) : make_double2(0.0, 0.0);
+            own[k * BS + t] = xv;
+            if (r < jc)
+                ll_publish(pub + ((size_t)j * BS + t) * 4, xv, (unsigned)epoch);
+            __syncthreads();
+        };
+        int j = fwd ? 0 : nb - 1;
+        const double2 *xj;
+        if (j % G == m) {
+            solve(j);
+            xj = own + ((j - m) / G) * BS;
+        } else {
+            if (!ll_fetch_multi(P, pub + (size_t)j * BS * 4, min(kTb, n - j * kTb), (unsigned)epoch, xa, &s_ok))
+                return;
+            xj = xa;
+        }
+        for (int s = 0; s < nb; ++s) {
+            j = fwd ? s : nb - 1 - s;
+            int ka, kb;
+            range(j, ka, kb);
+            if (ka >= kb)
+                break;
+            const int jn = fwd ? j + 1 : j - 1;
+            const bool ahead = jn % G == m;
+            if (ahead) {
+                const int kn = (jn - m) / G;
+                update(j, kn, xj);
+                __syncthreads();
+                solve(jn);
+                for (int k = (fwd ? kn + 1 : ka); k < (fwd ? kb : kn); ++k)
+                    update(j, k, xj);
+                __syncthreads();
+                xj = own + kn * BS;
+            } else {
+                for (int k = ka; k < kb; ++k)
+                    update(j, k, xj);
+                __syncthreads();
+                int na, nbk;
+                range(jn, na, nbk);
+                if (na >= nbk)
+                    break;
+                if (!ll_fetch_multi(P, pub + (size_t)jn * BS * 4, min(kTb, n - jn * kTb), (unsigned)epoch, xa,
+                                    &s_ok))
+                    return;
+                xj = xa;
+            }
+        }
+        __syncthreads();
+    }
+    // F^H columns: fh[(v0 + v) n + perm[i]]
+    for (int k = 0; k < nown; ++k) {
+        const int i = (m + k * G) * kTb + r;
+        if (i < n && v < nv)
+            ((double2 *)P.fh)[(size_t)n * (v0 + v) + P.perm[i]] = own[k * BS + t];
     }
 }
 

@@ -155,6 +155,31 @@ def test_gang_solves_match_plain(gang_n, gang_max):
         F.set_tunable("gang_max", old[1])
 
 
+def test_chained_block_solves_match_plain():
+    """CG solves on the flag-chained one-block-per-workgroup kernels (the path
+    for problems above solve1_max_n, forced here with solve1_max_n = 0): the
+    mixed-path gates hold and a batch equals the single solves bitwise."""
+    probs = random_problems(14, [65, 333, 1100, 191])
+    old = F.get_tunable("solve1_max_n")
+    try:
+        F.set_tunable("solve1_max_n", 0)
+        singles = []
+        for p in probs:
+            x64 = F.solve_problem(p, precision=F.PREC_FP64)
+            xm, st = F.solve_problem(p, precision=F.PREC_MIXED, with_stats=True)
+            fi = int(p["feed_index"])
+            assert abs(x64[fi] - xm[fi]) / abs(x64[fi]) <= 1e-6
+            assert rel(xm, x64) <= 1e-7
+            assert st["backward_error"] <= 1e-12
+            singles.append(xm)
+        batch = F.solve_batch(probs, precision=F.PREC_MIXED)
+        for x, b in zip(singles, batch):
+            assert b["status"] == F.OK
+            assert np.array_equal(x, b["x"])
+    finally:
+        F.set_tunable("solve1_max_n", old)
+
+
 def test_config3_mixed_large_solve():
     """Configuration 3: 32 random wires x 256 segments (N = 8192, one KCL
     junction) at 7 MHz. Mixed vs the plain FP64 GPU solve of the same deck."""
