@@ -80,6 +80,47 @@ struct Prob {
     double tol;
 };
 
+// sin and cos of a phase: Cody-Waite reduction by pi/2 in three parts (the
+// first two carry 33 significant bits, so n P1 and n P2 are exact for
+// |n| < 2^20), then the Taylor polynomials of sin (to r^15) and cos (to r^16)
+// on |r| <= pi/4, whose truncation errors stay below 6e-17 and 3e-18. The
+// result is within about 2 ulp. Arguments beyond 1e5 (never a phase k R of
+// the decks this path accepts) take the library routine.
+__device__ __forceinline__ void sincos_phase(double x, double &s, double &c)
+{
+    if (!(fabs(x) < 1e5)) {
+        sincos(x, &s, &c);
+        return;
+    }
+    const double n = rint(x * 0.6366197723675814);      // 2 / pi
+    double r = fma(-n, 1.5707963267341256, x);           // P1 = 0x1.921fb544p+0
+    r = fma(-n, 6.077100506303966e-11, r);               // P2 = 0x1.0b4611a6p-34
+    r = fma(-n, 2.0222662487959506e-21, r);              // P3 = pi/2 - P1 - P2
+    const double r2 = r * r;
+    double ps = fma(r2, -7.647163731819816e-13, 1.6059043836821613e-10); // 1/15!, 1/13!
+    ps = fma(r2, ps, -2.505210838544172e-08);
+    ps = fma(r2, ps, 2.7557319223985893e-06);
+    ps = fma(r2, ps, -0.0001984126984126984);
+    ps = fma(r2, ps, 0.008333333333333333);
+    ps = fma(r2, ps, -0.16666666666666666);
+    const double sr = fma(r * r2, ps, r);
+    double pc = fma(r2, 4.779477332387385e-14, -1.1470745597729725e-11); // 1/16!, 1/14!
+    pc = fma(r2, pc, 2.08767569878681e-09);
+    pc = fma(r2, pc, -2.755731922398589e-07);
+    pc = fma(r2, pc, 2.48015873015873e-05);
+    pc = fma(r2, pc, -0.001388888888888889);
+    pc = fma(r2, pc, 0.041666666666666664);
+    pc = fma(r2, pc, -0.5);
+    const double cr = fma(r2, pc, 1.0);
+    const int q = (int)n & 3; // quadrant of x
+    s = (q & 1) ? cr : sr;
+    c = (q & 1) ? sr : cr;
+    if (q & 2)
+        s = -s;
+    if ((q + 1) & 2)
+        c = -c;
+}
+
 constexpr int kMixMaxCons = 256; // constraint rows the mixed path accepts
 constexpr int kMixMaxW = 128;    // homogeneous columns the mixed path accepts
 constexpr int kMvRows = 256;     // rows per M p work item

@@ -50,8 +50,7 @@ __device__ __forceinline__ void quad8(double ox, double oy, double oz, double sx
                                       double a2, double k, double &re, double &im)
 {
     double sr = 0.0, si = 0.0;
-    // rolled: each point inlines an FP64 sincos, and the fully unrolled fill
-    // kernel outgrows the instruction cache
+    // rolled: keeps the fill kernel's code small
 #pragma unroll 1
     for (int m = 0; m < 8; ++m) {
         double sc = c_gl8n[m] * h;
@@ -60,7 +59,7 @@ __device__ __forceinline__ void quad8(double ox, double oy, double oz, double sx
         double pz = oz - (sz + sc * dz);
         double r = sqrt(px * px + py * py + pz * pz + a2);
         double s, c;
-        sincos(k * r, &s, &c);
+        sincos_phase(k * r, s, c);
         double wr = c_gl8w[m] / r;
         sr += wr * c;
         si -= wr * s;
@@ -78,7 +77,7 @@ __device__ __forceinline__ void self_term(double h, double a, double k, double &
         double l = c_gl4n[m] * h;
         double r = sqrt(l * l + a * a);
         double s, c;
-        sincos(k * r, &s, &c);
+        sincos_phase(k * r, s, c);
         double inv = 1.0 / r;
         sr += c_gl4w[m] * (c * inv - inv);
         si -= c_gl4w[m] * (s * inv);
@@ -656,7 +655,7 @@ __device__ __forceinline__ void ff_chunk(const double *__restrict__ sg, int cnt,
         const double mx = g[0], my = g[1], mz = g[2], dx = g[3], dy = g[4], dz = g[5];
         const double len = g[6], ir = g[7], ii = g[8];
         double s, c;
-        sincos(k * (mx * u.rx + my * u.ry + mz * u.rz), &s, &c);
+        sincos_phase(k * (mx * u.rx + my * u.ry + mz * u.rz), s, c);
         const double pr = len * c, pi = len * s;
         const double wr = ir * pr - ii * pi, wi = ir * pi + ii * pr;
         const double qt = dx * u.tx + dy * u.ty + dz * u.tz;
@@ -668,7 +667,7 @@ __device__ __forceinline__ void ff_chunk(const double *__restrict__ sg, int cnt,
         if (ground != 0) {
             // image: position (x, y, -z), direction (-dx, -dy, dz) (farfield.rs:64-82)
             double s2, c2;
-            sincos(k * (mx * u.rx + my * u.ry - mz * u.rz), &s2, &c2);
+            sincos_phase(k * (mx * u.rx + my * u.ry - mz * u.rz), s2, c2);
             const double p2r = len * c2, p2i = len * s2;
             const double vr = ir * p2r - ii * p2i, vi = ir * p2i + ii * p2r;
             const double it = -dx * u.tx - dy * u.ty + dz * u.tz;
