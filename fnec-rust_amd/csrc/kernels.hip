@@ -57,10 +57,11 @@ __device__ __forceinline__ void quad8(double ox, double oy, double oz, double sx
         double px = ox - (sx + sc * dx);
         double py = oy - (sy + sc * dy);
         double pz = oz - (sz + sc * dz);
-        double r = sqrt(px * px + py * py + pz * pz + a2);
+        const double r2 = px * px + py * py + pz * pz + a2;
+        const double ir = rsqrt(r2), r = r2 * ir; // one reciprocal square root for R and 1/R
         double s, c;
         sincos_phase(k * r, s, c);
-        double wr = c_gl8w[m] / r;
+        double wr = c_gl8w[m] * ir;
         sr += wr * c;
         si -= wr * s;
     }
