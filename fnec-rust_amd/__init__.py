@@ -383,8 +383,9 @@ def solve_problem(p: dict, precision: int = PREC_FP64, with_stats: bool = False)
 PROFILE_CLASSES = ("fill", "gram", "lu_panel", "lu_trsm", "lu_update", "backsub", "rp", "zsolve", "matvec",
                    "pcg_ctl", "fsetup", "lu_update_narrow",
                    # host-side phases (steady clock): problem and plan building, staging
-                   # copies, far-field set-up, result processing, time blocked on the stream
-                   "host_prep", "host_stage", "host_rp", "host_post", "host_wait")
+                   # copies, far-field set-up, result processing, time blocked on the stream,
+                   # kernel enqueueing up to the first wait
+                   "host_prep", "host_stage", "host_rp", "host_post", "host_wait", "host_enqueue")
 
 
 def profile(enable: bool | None = None, reset: bool = False, only: str | None = None) -> dict:

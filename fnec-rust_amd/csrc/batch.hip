@@ -14,6 +14,7 @@
 #include <cstring>
 #include <deque>
 #include <mutex>
+#include <string>
 #include <thread>
 #include <vector>
 
@@ -124,8 +125,21 @@ int run_batch(const fnec_request *const *reqs, size_t count, int precision, fnec
     std::vector<std::pair<size_t, int>> job_err; // (q, status) of grids that could not be set up
     RpStaged staged;
     int rp_st = FNEC_OK;
-    auto overlap = [&](const std::vector<double *> &dev_x) {
+    std::string rp_err;
+    // set up on a host thread of its own while this thread queues the solve
+    std::thread rp_thread;
+    struct Join {
+        std::thread &t;
+        ~Join()
+        {
+            if (t.joinable())
+                t.join();
+        }
+    } rp_join{rp_thread};
+    std::vector<double *> rp_dev_x;
+    auto rp_setup = [&]() {
         HostTimer ht(kHostRp);
+        const std::vector<double *> &dev_x = rp_dev_x;
         for (size_t q = 0; q < ok.size(); ++q) {
             const fnec_request &r = *reqs[idx[q]];
             if (!(r.rp_npts && r.rp_theta && r.rp_phi))
@@ -143,15 +157,32 @@ int run_batch(const fnec_request *const *reqs, size_t count, int precision, fnec
         }
         if (!jobs.empty())
             rp_st = rp_stage(jobs, staged);
+        if (rp_st)
+            rp_err = last_error(); // the message is per thread
+    };
+    bool any_rp = false;
+    for (size_t q = 0; q < ok.size(); ++q) {
+        const fnec_request &r = *reqs[idx[q]];
+        any_rp = any_rp || (r.rp_npts && r.rp_theta && r.rp_phi);
+    }
+    auto overlap = [&](const std::vector<double *> &dev_x) {
+        if (!any_rp)
+            return;
+        rp_dev_x = dev_x;
+        rp_thread = std::thread(rp_setup);
     };
     std::vector<fnec_solve_stats> stats;
     std::vector<int> status;
     std::vector<double *> dev_x;
     int st = solve_problems(ok, xo, stats, status, &dev_x, precision, overlap);
+    if (rp_thread.joinable())
+        rp_thread.join();
     if (st)
         return st;
-    if (rp_st)
+    if (rp_st) {
+        set_error(rp_err);
         return rp_st;
+    }
     {
         HostTimer ht(kHostPost);
         for (size_t q = 0; q < ok.size(); ++q) {

@@ -49,8 +49,8 @@ struct RpStaged {
 
 // Host-phase timing for the per-class profile (a no-op while profiling is
 // off): problem and plan building, staging copies, far-field preparation,
-// result processing, and time blocked on the stream.
-enum { kHostPrep = 12, kHostStage, kHostRp, kHostPost, kHostWait };
+// result processing, time blocked on the stream, and kernel enqueueing.
+enum { kHostPrep = 12, kHostStage, kHostRp, kHostPost, kHostWait, kHostEnq };
 class HostTimer {
   public:
     explicit HostTimer(int cls);
@@ -81,9 +81,9 @@ int problem_from_paths(HostProblem &hp, const fnec_seg *segs, size_t n, const fn
                        const double *cos_vec, const size_t *path_of, const size_t *free_ends,
                        size_t nfree, double freq_hz, const fnec_ground *ground);
 
-// overlap (optional) runs once on the host while the device works on the
-// solve (after the factorisation and the first iterations are queued, before
-// the host first waits), with the device addresses of the solution vectors.
+// overlap (optional) is called once, before any of the solve's kernels is
+// queued, with the device addresses of the solution vectors; it may start host
+// work that runs while the solve is queued and computed (the caller joins it).
 int solve_problems(const std::vector<const HostProblem *> &hps, std::vector<fnec_c64 *> &x_out,
                    std::vector<fnec_solve_stats> &stats, std::vector<int> &status,
                    std::vector<double *> *device_x, int precision,

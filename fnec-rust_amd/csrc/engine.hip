@@ -9,6 +9,7 @@
 #include <cstdio>
 #include <cstring>
 #include <functional>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -100,6 +101,18 @@ struct Context {
 
 static Context g_ctx;
 
+// Wait for a stream by polling it: a blocking synchronisation can sleep the
+// host thread and wake it a few milliseconds after the stream drained, which
+// leaves the device idle between a solve's dependent phases.
+static hipError_t stream_wait(hipStream_t st)
+{
+    for (;;) {
+        const hipError_t e = hipStreamQuery(st);
+        if (e != hipErrorNotReady)
+            return e;
+    }
+}
+
 // Optional per-kernel timing: HIP events around the launches of each tracked
 // kernel class on the solve stream, with the algorithmic work of each launch.
 struct ProfClass {
@@ -114,11 +127,11 @@ struct ProfPending {
 // (launches = occurrences, no flops or bytes).
 enum { PC_FILL, PC_GRAM, PC_LU_PANEL, PC_LU_TRSM, PC_LU_UPDATE, PC_BACKSUB, PC_RP, PC_ZSOLVE,
        PC_MV, PC_PCG, PC_FSETUP, PC_LU_UPDN, PC_HOST_PREP, PC_HOST_STAGE, PC_HOST_RP, PC_HOST_POST,
-       PC_HOST_WAIT, PC_N };
+       PC_HOST_WAIT, PC_HOST_ENQ, PC_N };
 static const char *kProfNames[PC_N] = {"fill", "gram", "lu_panel", "lu_trsm", "lu_update",
                                        "backsub", "rp", "zsolve", "matvec", "pcg_ctl", "fsetup",
                                        "lu_update_narrow", "host_prep", "host_stage", "host_rp",
-                                       "host_post", "host_wait"};
+                                       "host_post", "host_wait", "host_enqueue"};
 static bool g_prof = false;
 static ProfClass g_prof_acc[PC_N];
 static std::vector<ProfPending> g_prof_pending;
@@ -138,7 +151,7 @@ static hipEvent_t prof_event()
 
 static int g_prof_only = -1; // record only this class (-1: every class)
 
-static_assert(PC_HOST_PREP == kHostPrep && PC_HOST_WAIT == kHostWait, "host classes");
+static_assert(PC_HOST_PREP == kHostPrep && PC_HOST_WAIT == kHostWait && PC_HOST_ENQ == kHostEnq, "host classes");
 
 static double host_now_us()
 {
@@ -601,7 +614,6 @@ struct Plan {
         double f_panel, f_trsm, f_ua, b_ua, f_on, b_on, f_of, b_of;
     };
     std::vector<Outer> outer32;
-    std::function<void()> overlap; // host work run once while the device computes
     std::vector<LuDesc<float>> lud32;
     size_t off_lud32 = 0;
     size_t fsol_off = 0, fsol_cnt = 0, vsol_off = 0, vsol_cnt = 0, mvz_off = 0, mvz_cnt = 0;
@@ -1447,6 +1459,8 @@ static inline const int *wl_ptr(char *base, const Plan &pl, size_t off)
 // Mixed path: FP32 LU of Z, preconditioner set-up, CG iterations, residual.
 static int mixed_launch(Plan &pl, char *base)
 {
+    HostTimer *enq = new HostTimer(kHostEnq); // until the first wait on the stream
+    std::unique_ptr<HostTimer> enq_own(enq);
     hipStream_t st = g_ctx.stream;
     const Prob *dp = (const Prob *)(base + pl.off_probs);
     const int *ints = wl_ptr(base, pl, 0);
@@ -1624,14 +1638,10 @@ static int mixed_launch(Plan &pl, char *base)
             break;
         hipLaunchKernelGGL(k_mix_flags, dim3((np + 255) / 256), dim3(256), 0, st, dp, np, flags_dev);
         HIPCHK(hipMemcpyAsync(flags.data(), flags_dev, 4 * np, hipMemcpyDeviceToHost, st));
-        if (pl.overlap) {
-            auto f = std::move(pl.overlap);
-            pl.overlap = nullptr;
-            f();
-        }
+        enq_own.reset();
         {
             HostTimer ht(kHostWait);
-            HIPCHK(hipStreamSynchronize(st));
+            HIPCHK(stream_wait(st));
         }
         bool all_done = true;
         for (unsigned b = 0; b < np; ++b)
@@ -1770,16 +1780,13 @@ int solve_problems(const std::vector<const HostProblem *> &hps, std::vector<fnec
     std::vector<double *> dev_x(hps.size());
     for (size_t b = 0; b < hps.size(); ++b)
         dev_x[b] = (double *)(base + pl.off_x[b]);
+    // the caller's concurrent work starts before the launches: queueing them
+    // blocks on the device queue, so the host has no lead to spend later
     if (overlap)
-        pl.overlap = [&]() { overlap(dev_x); };
+        overlap(dev_x);
     st = plan_launch(pl, base, true);
     if (st)
         return st;
-    if (pl.overlap) {
-        auto f = std::move(pl.overlap);
-        pl.overlap = nullptr;
-        f();
-    }
     // all result words in one copy
     const size_t nback = pl.io_end - pl.io_begin;
     char *hb = pinned(g_ctx.h_back, g_ctx.h_back_bytes, nback);
@@ -1790,7 +1797,7 @@ int solve_problems(const std::vector<const HostProblem *> &hps, std::vector<fnec
     HIPCHK(hipMemcpyAsync(hb, base + pl.io_begin, nback, hipMemcpyDeviceToHost, g_ctx.stream));
     {
         HostTimer ht(kHostWait);
-        HIPCHK(hipStreamSynchronize(g_ctx.stream));
+        HIPCHK(stream_wait(g_ctx.stream));
     }
     HostTimer ht_post(kHostPost);
     auto back = [&](size_t off) { return hb + (off - pl.io_begin); };
@@ -2065,7 +2072,7 @@ int rp_finish(RpStaged &S, const std::vector<fnec_rp_row *> &outs, std::vector<d
     HIPCHK(hipMemcpyAsync(hb, base + S.out_begin, end - S.out_begin, hipMemcpyDeviceToHost, s));
     {
         HostTimer ht(kHostWait);
-        HIPCHK(hipStreamSynchronize(s));
+        HIPCHK(stream_wait(s));
     }
     HostTimer ht_post(kHostPost);
     prof_collect();

@@ -240,8 +240,9 @@ long fnec_hip_get_tunable(const char *name);
  * of the second 32-column panel inside each 64-column step of the FP32 LU;
  * "lu_update" is then the K = 64 trailing update), and the host-side phases
  * "host_prep" (problem and plan building), "host_stage" (staging copies),
- * "host_rp" (far-field set-up), "host_post" (result processing) and
- * "host_wait" (time blocked on the stream), timed with a steady clock (flops
+ * "host_rp" (far-field set-up), "host_post" (result processing),
+ * "host_wait" (time blocked on the stream) and "host_enqueue" (queueing a
+ * solve's kernels up to its first wait), timed with a steady clock (flops
  * and bytes 0). out4 = {launches, total microseconds, algorithmic flops,
  * algorithmic bytes} accumulated since the last reset. */
 void fnec_hip_profile_enable(int on);
