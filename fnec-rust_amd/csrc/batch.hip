@@ -42,11 +42,20 @@ struct OwnedRequest {
     double t_submit = 0.0;
     size_t footprint = 0;
 
+    // Every array is reassigned (or cleared when the request has none): an
+    // object is reused for later requests.
     void own(const fnec_request &r)
     {
         segs.assign(r.segs, r.segs + r.n);
         rhs.assign(r.rhs, r.rhs + r.n);
         cosv.assign(r.cos_vec, r.cos_vec + r.n);
+        wf.clear();
+        wl.clear();
+        ja.clear();
+        jb.clear();
+        js.clear();
+        theta.clear();
+        phi.clear();
         if (r.n_wires) {
             wf.assign(r.wire_first, r.wire_first + r.n_wires);
             wl.assign(r.wire_last, r.wire_last + r.n_wires);
@@ -246,6 +255,9 @@ struct fnec_session {
     std::mutex mu;
     std::deque<OwnedRequest *> queue;
     std::deque<fnec_result> done;
+    // finished requests, reused by later submits: their arrays keep their
+    // capacity, so a submit copies into memory that is already mapped
+    std::vector<OwnedRequest *> spare;
 };
 
 extern "C" {
@@ -301,7 +313,16 @@ int fnec_hip_session_submit(fnec_session *s, const fnec_request *req)
         set_error("request footprint exceeds the session budget");
         return FNEC_RESOURCE_EXHAUSTED;
     }
-    OwnedRequest *o = new OwnedRequest;
+    OwnedRequest *o = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        if (!s->spare.empty()) {
+            o = s->spare.back();
+            s->spare.pop_back();
+        }
+    }
+    if (!o)
+        o = new OwnedRequest;
     o->own(*req);
     o->t_submit = now_us();
     o->footprint = fp;
@@ -359,7 +380,7 @@ int fnec_hip_session_poll(fnec_session *s, fnec_result *out, int max, int wait_u
             res[i].t_submit_us = batch[i]->t_submit;
             res[i].t_done_us = t_done;
             s->done.push_back(res[i]);
-            delete batch[i];
+            s->spare.push_back(batch[i]);
         }
     }
     int k = 0;
@@ -376,6 +397,8 @@ void fnec_hip_session_destroy(fnec_session *s)
     if (!s)
         return;
     for (auto *o : s->queue)
+        delete o;
+    for (auto *o : s->spare)
         delete o;
     delete s;
 }

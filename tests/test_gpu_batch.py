@@ -116,6 +116,39 @@ def test_session_equals_batch_under_any_split(precision):
         assert got[i][1] == b["z_in"]
 
 
+def test_session_reuses_request_storage_across_polls():
+    """A session reuses a finished request's storage for later submits: rounds
+    of submit + poll with requests of different shapes (junctions or none, a
+    far-field grid or none) return the batch result bitwise every time."""
+    probs = problems(25, [51, 300, 77, 900, 150, 64, 500, 120])
+    batch = F.solve_batch(probs, precision=1)
+    L = F.lib()
+    sess = L.fnec_hip_session_create(1 << 40, 1, 256)
+    assert sess
+    try:
+        out = (F.Result * 16)()
+        keep = []
+        for rnd, order in enumerate([[0, 1], [2, 3, 4], [5], [6, 7, 0], [3, 1]]):
+            for i in order:
+                grid = (TH2, PH2) if (i + rnd) % 3 == 0 else (None, None)
+                q, k = F.make_request(probs[i], tag=100 * rnd + i, rp_theta=grid[0], rp_phi=grid[1])
+                keep.append((q, k))
+                assert L.fnec_hip_session_submit(sess, C.byref(q)) == 0
+            got = {}
+            for _ in range(20):
+                k = L.fnec_hip_session_poll(sess, C.cast(out, C.c_void_p), 16, 0)
+                for j in range(k):
+                    got[out[j].tag] = (out[j].status, complex(out[j].z_in.re, out[j].z_in.im))
+                if len(got) == len(order):
+                    break
+            for i in order:
+                status, zin = got[100 * rnd + i]
+                assert status == F.OK
+                assert zin == batch[i]["z_in"]
+    finally:
+        L.fnec_hip_session_destroy(sess)
+
+
 def test_session_rejects_request_over_budget():
     probs = problems(24, [400])
     st, _ = run_session(probs, 1, 1 << 20, 8)
