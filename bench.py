@@ -36,6 +36,7 @@ sys.path.insert(0, ROOT)
 
 F = importlib.import_module("fnec-rust_amd")
 W = importlib.import_module("fnec-rust_amd.workloads")
+AT = importlib.import_module("fnec-rust_amd.tuning")
 
 METRIC = "solves/s + p99 latency, mixed-N queue, vs plain FP64 GPU at rel ΔZin≤1e-6"
 PEAK_FP64_TFLOPS = 78.6
@@ -85,28 +86,40 @@ def build_requests(stream):
     return items
 
 
-def serve(items, window, steps, warmup, prof_windows, precision, budget, max_batch):
+def serve(items, window, steps, warmup, prof_windows, precision, budget, max_batch, depth):
     """Warmup windows, then ``prof_windows`` untimed windows with events on
     every kernel class (the per-class breakdown; it names the dominant class),
-    then the timed windows with events around the dominant class only."""
+    then the timed windows with events around the dominant class only. In each
+    phase a window of requests is submitted whenever fewer than ``depth``
+    windows are outstanding, so new requests arrive while earlier ones are on
+    the device."""
     L = F.lib()
     sess = L.fnec_hip_session_create(budget, precision, max_batch)
     if not sess:
         raise F.FnecError(F.INVALID_ARG, "session create failed")
     out = (F.Result * 4096)()
     results = {}
+    batches = {}
 
-    def run_window(chunk):
+    def run_phase(first, nwin):
         t_sub = {}
-        for it in chunk:
-            t_sub[it["req"].tag] = time.perf_counter()
-            st = L.fnec_hip_session_submit(sess, C.byref(it["req"]))
-            if st:
-                F._check(st)
-        remaining = len(chunk)
         lat = []
-        while remaining:
-            k = L.fnec_hip_session_poll(sess, C.cast(out, C.c_void_p), 4096, 1000)
+        bt = {}
+
+        def submit(w):
+            for it in items[(first + w) * window:(first + w + 1) * window]:
+                t_sub[it["req"].tag] = time.perf_counter()
+                st = L.fnec_hip_session_submit(sess, C.byref(it["req"]))
+                if st:
+                    F._check(st)
+
+        nxt = outstanding = 0
+        while nxt < min(depth, nwin):
+            submit(nxt)
+            nxt += 1
+            outstanding += window
+        while outstanding:
+            k = L.fnec_hip_session_poll(sess, C.cast(out, C.c_void_p), 4096, 2000)
             if k < 0:
                 F._check(-k)
             now = time.perf_counter()
@@ -116,27 +129,47 @@ def serve(items, window, steps, warmup, prof_windows, precision, budget, max_bat
                     raise F.FnecError(r.status, f"request {r.tag} failed")
                 results[r.tag] = complex(r.z_in.re, r.z_in.im)
                 lat.append(now - t_sub[r.tag])
-            remaining -= k
-        return lat
+                b = bt.setdefault(r.batch, [r.t_start_us, r.t_done_us, 0])
+                b[2] += 1
+            outstanding -= k
+            while nxt < nwin and outstanding <= (depth - 1) * window:
+                submit(nxt)
+                nxt += 1
+                outstanding += window
+        return lat, bt
 
-    for w in range(warmup):
-        run_window(items[w * window:(w + 1) * window])
+    run_phase(0, warmup)
     F.profile(enable=True, reset=True, only="")
-    for w in range(warmup, warmup + prof_windows):
-        run_window(items[w * window:(w + 1) * window])
+    run_phase(warmup, prof_windows)
     prof_all = F.profile(enable=False)
     timed_cls = [k for k in prof_all if prof_all[k][0] > 0 and prof_all[k][2] > 0]
     dom = max(timed_cls, key=lambda k: prof_all[k][1]) if timed_cls else "lu_update"
     F.profile(enable=True, reset=True, only=dom)
-    first = warmup + prof_windows
-    lat = []
     t0 = time.perf_counter()
-    for s in range(steps):
-        lat += run_window(items[(first + s) * window:(first + s + 1) * window])
+    lat, batches = run_phase(warmup + prof_windows, steps)
     elapsed = time.perf_counter() - t0
     prof_timed = F.profile(enable=False, only="")
     L.fnec_hip_session_destroy(sess)
-    return elapsed, np.array(lat), results, dom, prof_all, prof_timed
+    return elapsed, np.array(lat), results, dom, prof_all, prof_timed, batches
+
+
+def batch_overlap(batches):
+    """Batches formed in the timed region, their mean size, and the share of
+    the span from the first batch start to the last batch end during which
+    two or more batches were on the device."""
+    if not batches:
+        return None
+    ev = sorted([(v[0], 1) for v in batches.values()] + [(v[1], -1) for v in batches.values()])
+    span = ev[-1][0] - ev[0][0]
+    busy2 = 0.0
+    level = 0
+    for (t, d), (t_next, _) in zip(ev, ev[1:] + [(ev[-1][0], 0)]):
+        level += d
+        if level >= 2:
+            busy2 += t_next - t
+    sizes = [v[2] for v in batches.values()]
+    return {"batches": len(batches), "mean_batch": round(float(np.mean(sizes)), 1),
+            "two_or_more_on_device_frac": round(busy2 / span, 3) if span > 0 else 0.0}
 
 
 def plain_fp64(items, sample):
@@ -254,6 +287,11 @@ def main():
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="mixed")
     ap.add_argument("--plain-sample", type=int, default=256)
     ap.add_argument("--max-batch", type=int, default=256)
+    ap.add_argument("--lanes", type=int, default=2, help="batches on the device at once")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the engine's default knobs (no autotune on a window of the stream)")
+    ap.add_argument("--depth", type=int, default=None,
+                    help="windows outstanding at once (default: lanes)")
     ap.add_argument("--budget-gib", type=float, default=96.0)
     ap.add_argument("--cpu-budget-s", type=float, default=20.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -268,6 +306,8 @@ def main():
               "(requests shard across GPUs with no device communication)", file=sys.stderr)
         sys.exit(2)
     F.init()
+    F.set_tunable("lanes", args.lanes)
+    depth = args.depth or args.lanes
     precision = F.PREC_MIXED if args.precision == "mixed" else F.PREC_FP64
     total = (args.warmup + args.profile_windows + args.steps) * args.window
     t_gen = time.perf_counter()
@@ -277,9 +317,17 @@ def main():
     print(f"[bench] built {len(items)} requests in {t_gen:.1f}s; serving {args.steps} x {args.window}",
           file=sys.stderr, flush=True)
 
-    elapsed, lat, zs_served, dom, prof_all, prof = serve(items, args.window, args.steps, args.warmup,
-                                                         args.profile_windows, precision,
-                                                         int(args.budget_gib * 2 ** 30), args.max_batch)
+    tuned = None
+    if not args.no_autotune:
+        t_at = time.perf_counter()
+        at = F.autotune([it["req"] for it in items[:args.window]], precision, repeats=1, rounds=1)
+        tuned = {"knobs": at["knobs"], "window_ms": round(at["seconds"] * 1e3, 2),
+                 "start_window_ms": round(at["start_seconds"] * 1e3, 2), "trials": len(at["trials"]),
+                 "tuning_s": round(time.perf_counter() - t_at, 2)}
+        print(f"[bench] autotune: {tuned}", file=sys.stderr, flush=True)
+    elapsed, lat, zs_served, dom, prof_all, prof, batches = serve(
+        items, args.window, args.steps, args.warmup, args.profile_windows, precision,
+        int(args.budget_gib * 2 ** 30), args.max_batch, depth)
     first = (args.warmup + args.profile_windows) * args.window
     timed = args.steps * args.window
     value = timed / elapsed
@@ -353,10 +401,14 @@ def main():
                         f"{timed} timed requests in windows of {args.window}",
             "precision": args.precision,
             "max_batch": args.max_batch,
+            "tunables": tuned["knobs"] if tuned else {k: F.get_tunable(k) for k in AT.SPACE},
         },
+        "autotune": tuned,
         "latency_ms": {"p50": round(float(np.percentile(lat, 50)) * 1e3, 3),
                        "p99": round(float(np.percentile(lat, 99)) * 1e3, 3),
-                       "arrival": f"windows of {args.window} submitted at once"},
+                       "arrival": f"windows of {args.window} requests; a window is submitted "
+                                  f"whenever fewer than {depth} are outstanding"},
+        "serving": dict(lanes=args.lanes, **(batch_overlap(batches) or {})),
         "plain_fp64_gpu": {"value": round(plain_rate, 3), "unit": "solves/s",
                            "sample": len(sample_items)},
         "speedup_vs_plain_fp64": round(value / plain_rate, 3),

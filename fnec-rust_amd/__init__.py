@@ -77,7 +77,8 @@ class Request(C.Structure):
 class Result(C.Structure):
     _fields_ = [("tag", C.c_uint64), ("status", C.c_int), ("z_in", C64), ("feed_current", C64),
                 ("x", C.c_void_p), ("rp", C.c_void_p), ("rp_total", C.c_double),
-                ("t_submit_us", C.c_double), ("t_done_us", C.c_double), ("stats", SolveStats)]
+                ("t_submit_us", C.c_double), ("t_start_us", C.c_double), ("t_done_us", C.c_double),
+                ("batch", C.c_uint64), ("stats", SolveStats)]
 
 
 def build(force: bool = True) -> str:
@@ -466,3 +467,12 @@ def solve_batch(problems: list, precision: int = PREC_FP64, rp_grids: list | Non
         out.append({"status": r.status, "z_in": complex(r.z_in.re, r.z_in.im), "x": xs[i], "rp": rps[i],
                     "rp_total": r.rp_total, "stats": r.stats.as_dict()})
     return out
+
+
+def autotune(problems: list, precision: int = PREC_MIXED, **kw) -> dict:
+    """Pick the engine's knobs (update tile, CG gang shape, solve size limit,
+    CG read-back chunk) by timing batched solves of `problems`; see
+    tuning.autotune for the options."""
+    from .tuning import autotune as _autotune
+
+    return _autotune(problems, precision, **kw)

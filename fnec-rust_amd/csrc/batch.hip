@@ -89,9 +89,13 @@ struct OwnedRequest {
 int run_batch(const fnec_request *const *reqs, size_t count, int precision, fnec_result *results,
               std::vector<std::vector<fnec_c64>> *xs, std::vector<std::vector<fnec_rp_row>> *rps)
 {
-    // problem descriptions keep their capacity across batches (the engine
-    // runs one batch at a time under the global lock)
-    static std::vector<HostProblem> hps;
+    // the whole batch runs on one lane: the far-field helper thread binds it too
+    LaneScope lane_scope;
+    if (lane_scope.status())
+        return lane_scope.status();
+    void *lane = lane_scope.lane();
+    // problem descriptions keep their capacity across this thread's batches
+    static thread_local std::vector<HostProblem> hps;
     if (hps.size() < count)
         hps.resize(count);
     std::vector<int> pre(count, FNEC_OK);
@@ -147,6 +151,7 @@ int run_batch(const fnec_request *const *reqs, size_t count, int precision, fnec
     } rp_join{rp_thread};
     std::vector<double *> rp_dev_x;
     auto rp_setup = [&]() {
+        LaneScope bind(lane);
         HostTimer ht(kHostRp);
         const std::vector<double *> &dev_x = rp_dev_x;
         for (size_t q = 0; q < ok.size(); ++q) {
@@ -248,17 +253,116 @@ int run_batch(const fnec_request *const *reqs, size_t count, int precision, fnec
 
 } // namespace
 
+// Serving session. Worker threads (one per engine lane) take admitted
+// requests off the queue as batches and run them; a batch runs on its own
+// lane, so two batches' kernels overlap on the device while new requests are
+// submitted. Requests are admitted in arrival order while the footprint of
+// every batch in flight stays within the budget.
 struct fnec_session {
     size_t budget = 0;
     int precision = FNEC_PREC_FP64;
     int max_batch = 64;
     std::mutex mu;
+    std::condition_variable cv_work, cv_done;
     std::deque<OwnedRequest *> queue;
     std::deque<fnec_result> done;
     // finished requests, reused by later submits: their arrays keep their
     // capacity, so a submit copies into memory that is already mapped
     std::vector<OwnedRequest *> spare;
+    size_t in_flight_bytes = 0; // footprint of the batches being solved
+    size_t running = 0;         // requests in those batches
+    uint64_t batches = 0;       // batches formed so far (the next batch id)
+    double last_submit = 0.0;
+    bool stop = false;
+    std::string err; // message of the last failed batch
+    std::vector<std::thread> workers;
 };
+
+namespace {
+
+// Batch formation: a worker that finds admissible requests waits until no
+// request has been submitted for kSettleUs (a burst of submits ends up in one
+// batch), the batch is full, or the oldest request has waited kLingerUs.
+constexpr double kSettleUs = 500.0;
+constexpr double kLingerUs = 20000.0;
+
+void session_worker(fnec_session *s)
+{
+    std::unique_lock<std::mutex> lk(s->mu);
+    for (;;) {
+        s->cv_work.wait(lk, [&] {
+            return s->stop ||
+                   (!s->queue.empty() && s->in_flight_bytes + s->queue.front()->footprint <= s->budget);
+        });
+        if (s->stop)
+            return;
+        while (!s->stop && !s->queue.empty() && (int)s->queue.size() < s->max_batch) {
+            const double now = now_us();
+            const double quiet = now - s->last_submit, age = now - s->queue.front()->t_submit;
+            if (quiet >= kSettleUs || age >= kLingerUs)
+                break;
+            const double w = std::max(1.0, std::min(kSettleUs - quiet, kLingerUs - age));
+            s->cv_work.wait_for(lk, std::chrono::microseconds((long)w));
+        }
+        if (s->stop)
+            return;
+        std::vector<OwnedRequest *> batch;
+        size_t bytes = 0;
+        while (!s->queue.empty() && (int)batch.size() < s->max_batch) {
+            OwnedRequest *o = s->queue.front();
+            if (s->in_flight_bytes + bytes + o->footprint > s->budget)
+                break;
+            bytes += o->footprint;
+            batch.push_back(o);
+            s->queue.pop_front();
+        }
+        if (batch.empty())
+            continue; // another worker took the requests, or they do not fit yet
+        s->in_flight_bytes += bytes;
+        s->running += batch.size();
+        const uint64_t id = s->batches++;
+        lk.unlock();
+
+        const double t_start = now_us();
+        std::vector<const fnec_request *> rp(batch.size());
+        std::vector<fnec_result> res(batch.size());
+        std::memset(res.data(), 0, sizeof(fnec_result) * res.size());
+        for (size_t i = 0; i < batch.size(); ++i)
+            rp[i] = &batch[i]->req;
+        int st;
+        {
+            std::lock_guard<std::mutex> g(global_mutex());
+            st = ensure_ready();
+        }
+        if (!st)
+            st = run_batch(rp.data(), rp.size(), s->precision, res.data(), nullptr, nullptr);
+        std::string msg = st ? std::string(last_error()) : std::string();
+        for (size_t i = 0; i < batch.size() && !st; ++i)
+            if (res[i].status && msg.empty())
+                msg = last_error();
+        const double t_done = now_us();
+
+        lk.lock();
+        if (!msg.empty())
+            s->err = msg;
+        for (size_t i = 0; i < batch.size(); ++i) {
+            if (st)
+                res[i].status = st;
+            res[i].t_submit_us = batch[i]->t_submit;
+            res[i].t_start_us = t_start;
+            res[i].t_done_us = t_done;
+            res[i].batch = id;
+            s->done.push_back(res[i]);
+            s->spare.push_back(batch[i]);
+        }
+        s->in_flight_bytes -= bytes;
+        s->running -= batch.size();
+        s->cv_done.notify_all();
+        s->cv_work.notify_all();
+    }
+}
+
+} // namespace
 
 extern "C" {
 
@@ -298,6 +402,9 @@ fnec_session *fnec_hip_session_create(size_t budget_bytes, int precision, int ma
     s->budget = budget_bytes;
     s->precision = precision;
     s->max_batch = max_batch > 0 ? max_batch : 64;
+    const long lanes = std::max(1L, get_tunable("lanes"));
+    for (long i = 0; i < lanes; ++i)
+        s->workers.emplace_back(session_worker, s);
     return s;
 }
 
@@ -324,10 +431,11 @@ int fnec_hip_session_submit(fnec_session *s, const fnec_request *req)
     if (!o)
         o = new OwnedRequest;
     o->own(*req);
-    o->t_submit = now_us();
     o->footprint = fp;
     std::lock_guard<std::mutex> lk(s->mu);
+    o->t_submit = s->last_submit = now_us();
     s->queue.push_back(o);
+    s->cv_work.notify_one();
     return FNEC_OK;
 }
 
@@ -336,59 +444,28 @@ int fnec_hip_session_pending(fnec_session *s)
     if (!s)
         return 0;
     std::lock_guard<std::mutex> lk(s->mu);
-    return (int)(s->queue.size() + s->done.size());
+    return (int)(s->queue.size() + s->running + s->done.size());
 }
 
 int fnec_hip_session_poll(fnec_session *s, fnec_result *out, int max, int wait_us)
 {
-    (void)wait_us;
     if (!s || (max > 0 && !out)) {
         set_error("invalid poll");
         return -FNEC_INVALID_ARG;
     }
-    std::vector<OwnedRequest *> batch;
-    {
-        std::lock_guard<std::mutex> lk(s->mu);
-        size_t used = 0;
-        while (!s->queue.empty() && (int)batch.size() < s->max_batch) {
-            OwnedRequest *o = s->queue.front();
-            if (!batch.empty() && used + o->footprint > s->budget)
-                break;
-            used += o->footprint;
-            batch.push_back(o);
-            s->queue.pop_front();
-        }
-    }
-    if (!batch.empty()) {
-        std::vector<const fnec_request *> rp(batch.size());
-        std::vector<fnec_result> res(batch.size());
-        std::memset(res.data(), 0, sizeof(fnec_result) * res.size());
-        for (size_t i = 0; i < batch.size(); ++i)
-            rp[i] = &batch[i]->req;
-        int st;
-        {
-            std::lock_guard<std::mutex> lk(global_mutex());
-            st = ensure_ready();
-            if (!st)
-                st = run_batch(rp.data(), rp.size(), s->precision, res.data(), nullptr, nullptr);
-        }
-        const double t_done = now_us();
-        std::lock_guard<std::mutex> lk(s->mu);
-        for (size_t i = 0; i < batch.size(); ++i) {
-            if (st)
-                res[i].status = st;
-            res[i].t_submit_us = batch[i]->t_submit;
-            res[i].t_done_us = t_done;
-            s->done.push_back(res[i]);
-            s->spare.push_back(batch[i]);
-        }
-    }
+    std::unique_lock<std::mutex> lk(s->mu);
+    if (s->done.empty() && wait_us > 0 && (s->running || !s->queue.empty()))
+        s->cv_done.wait_for(lk, std::chrono::microseconds(wait_us), [&] { return !s->done.empty(); });
     int k = 0;
-    std::lock_guard<std::mutex> lk(s->mu);
+    bool failed = false;
     while (k < max && !s->done.empty()) {
-        out[k++] = s->done.front();
+        out[k] = s->done.front();
+        failed = failed || out[k].status != FNEC_OK;
+        ++k;
         s->done.pop_front();
     }
+    if (failed)
+        set_error(s->err);
     return k;
 }
 
@@ -396,6 +473,13 @@ void fnec_hip_session_destroy(fnec_session *s)
 {
     if (!s)
         return;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->stop = true;
+    }
+    s->cv_work.notify_all();
+    for (auto &t : s->workers)
+        t.join(); // a batch already on the device finishes first
     for (auto *o : s->queue)
         delete o;
     for (auto *o : s->spare)

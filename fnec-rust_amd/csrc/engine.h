@@ -63,6 +63,28 @@ class HostTimer {
     double t0_;
 };
 
+// Binds one of the engine's lanes (streams, events, device arenas, pinned
+// staging) to this thread for the scope's lifetime. The default form takes a
+// free lane, waiting while every lane runs a batch, and is a no-op when the
+// thread already holds one; the other binds the lane of the thread that
+// started this one (a helper thread working on that thread's batch). The
+// device context must be initialised (ensure_ready) first.
+class LaneScope {
+  public:
+    LaneScope();
+    explicit LaneScope(void *lane);
+    ~LaneScope();
+    LaneScope(const LaneScope &) = delete;
+    LaneScope &operator=(const LaneScope &) = delete;
+    int status() const { return st_; }
+    void *lane() const;
+
+  private:
+    int st_ = 0;
+    void *owned_ = nullptr, *prev_ = nullptr;
+    bool bound_ = false;
+};
+
 void set_error(const std::string &e);
 const char *last_error();
 int ensure_ready();

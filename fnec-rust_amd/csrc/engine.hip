@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <condition_variable>
 #include <cstdio>
 #include <cstring>
 #include <functional>
@@ -77,6 +78,22 @@ struct Context {
     bool ready = false;
     int status = FNEC_UNAVAILABLE;
     int device = -1;
+};
+
+static Context g_ctx;
+
+struct ProfPending {
+    int cls;
+    hipEvent_t a, b;
+    double flops, bytes;
+};
+
+// One lane runs one batch at a time: its own streams, events, device arenas,
+// pinned staging and host caches. Batches on different lanes overlap on the
+// device (the serving session keeps two in flight). A thread binds a lane with
+// a LaneScope for the duration of an engine call.
+struct Lane {
+    bool init = false, busy = false;
     hipStream_t stream = nullptr;
     hipEvent_t ev[8];
     // cached device arena, grown on demand
@@ -90,16 +107,100 @@ struct Context {
     // for far-field inputs staged while a solve runs
     char *h_up = nullptr, *h_back = nullptr, *h_rp = nullptr;
     size_t h_up_bytes = 0, h_back_bytes = 0, h_rp_bytes = 0;
-    // uploads that overlap the solve stream's kernels (default priority)
-    hipStream_t copy_stream = nullptr;
-    hipEvent_t ev_rp = nullptr;
     // the LU's panel chain runs one step ahead on this stream (default
-    // priority), ordered against the solve stream with events from the pool
+    // priority), ordered against the solve stream with events from the pool;
+    // far-field inputs are uploaded on it while the solve runs (ev_rp)
     hipStream_t side_stream = nullptr;
+    hipEvent_t ev_rp = nullptr;
     std::vector<hipEvent_t> sync_events;
+    // per-kernel timing records of this lane's launches
+    std::vector<ProfPending> prof_pending;
+    // work lists keep their capacity across batches
+    std::vector<int> plan_ints, rp_ints;
 };
 
-static Context g_ctx;
+constexpr int kMaxLanes = 4;
+static Lane g_lanes[kMaxLanes];
+static int g_lanes_on = 2; // lanes in use (tunable "lanes")
+static std::mutex g_lane_mu;
+static std::condition_variable g_lane_cv;
+static thread_local Lane *t_lane = nullptr;
+
+// The lane bound to this thread (every engine entry point binds one).
+static inline Lane &LN() { return *t_lane; }
+
+static int lane_init(Lane &L)
+{
+    if (L.init)
+        return FNEC_OK;
+    if (hipStreamCreate(&L.stream) != hipSuccess || hipStreamCreate(&L.side_stream) != hipSuccess ||
+        hipEventCreate(&L.ev_rp) != hipSuccess) {
+        set_error("stream creation failed");
+        return FNEC_INTERNAL;
+    }
+    for (auto &e : L.ev)
+        if (hipEventCreate(&e) != hipSuccess) {
+            set_error("event creation failed");
+            return FNEC_INTERNAL;
+        }
+    L.init = true;
+    return FNEC_OK;
+}
+
+LaneScope::LaneScope()
+{
+    if (t_lane)
+        return; // this thread already holds a lane (nested engine call)
+    if (!g_ctx.ready || g_ctx.status != FNEC_OK) {
+        set_error("device not initialised");
+        st_ = FNEC_UNAVAILABLE;
+        return;
+    }
+    Lane *got = nullptr;
+    {
+        std::unique_lock<std::mutex> lk(g_lane_mu);
+        g_lane_cv.wait(lk, [&] {
+            for (int i = 0; i < g_lanes_on; ++i)
+                if (!g_lanes[i].busy) {
+                    got = &g_lanes[i];
+                    return true;
+                }
+            return false;
+        });
+        got->busy = true;
+    }
+    (void)hipSetDevice(g_ctx.device);
+    st_ = lane_init(*got);
+    if (st_) {
+        std::lock_guard<std::mutex> lk(g_lane_mu);
+        got->busy = false;
+        g_lane_cv.notify_all();
+        return;
+    }
+    t_lane = got;
+    owned_ = got;
+}
+
+LaneScope::LaneScope(void *lane) : prev_(t_lane)
+{
+    (void)hipSetDevice(g_ctx.device);
+    t_lane = (Lane *)lane;
+    bound_ = true;
+}
+
+LaneScope::~LaneScope()
+{
+    if (bound_)
+        t_lane = (Lane *)prev_;
+    if (!owned_)
+        return;
+    t_lane = nullptr;
+    std::lock_guard<std::mutex> lk(g_lane_mu);
+    ((Lane *)owned_)->busy = false;
+    g_lane_cv.notify_all();
+}
+
+void *LaneScope::lane() const { return t_lane; }
 
 // Wait for a stream by polling it: a blocking synchronisation can sleep the
 // host thread and wake it a few milliseconds after the stream drained, which
@@ -118,11 +219,6 @@ static hipError_t stream_wait(hipStream_t st)
 struct ProfClass {
     double launches = 0, total_us = 0, flops = 0, bytes = 0;
 };
-struct ProfPending {
-    int cls;
-    hipEvent_t a, b;
-    double flops, bytes;
-};
 // Classes from PC_HOST_PREP on are host-side phases timed with a steady clock
 // (launches = occurrences, no flops or bytes).
 enum { PC_FILL, PC_GRAM, PC_LU_PANEL, PC_LU_TRSM, PC_LU_UPDATE, PC_BACKSUB, PC_RP, PC_ZSOLVE,
@@ -134,11 +230,12 @@ static const char *kProfNames[PC_N] = {"fill", "gram", "lu_panel", "lu_trsm", "l
                                        "host_post", "host_wait", "host_enqueue"};
 static bool g_prof = false;
 static ProfClass g_prof_acc[PC_N];
-static std::vector<ProfPending> g_prof_pending;
 static std::vector<hipEvent_t> g_prof_pool;
+static std::mutex g_prof_mu; // accumulators and the event pool (lanes record concurrently)
 
 static hipEvent_t prof_event()
 {
+    std::lock_guard<std::mutex> lk(g_prof_mu);
     if (!g_prof_pool.empty()) {
         hipEvent_t e = g_prof_pool.back();
         g_prof_pool.pop_back();
@@ -164,6 +261,7 @@ HostTimer::~HostTimer()
 {
     if (!g_prof || (g_prof_only >= 0 && cls_ != g_prof_only))
         return;
+    std::lock_guard<std::mutex> lk(g_prof_mu);
     ProfClass &c = g_prof_acc[cls_];
     c.launches += 1;
     c.total_us += host_now_us() - t0_;
@@ -174,7 +272,7 @@ static hipEvent_t prof_begin(int cls, hipStream_t st = nullptr)
     if (!g_prof || (g_prof_only >= 0 && cls != g_prof_only))
         return nullptr;
     hipEvent_t e = prof_event();
-    (void)hipEventRecord(e, st ? st : g_ctx.stream);
+    (void)hipEventRecord(e, st ? st : LN().stream);
     return e;
 }
 
@@ -185,23 +283,26 @@ static long prof_end(int cls, hipEvent_t a, double flops, double bytes, hipStrea
     if (!g_prof || !a)
         return -1;
     hipEvent_t b = prof_event();
-    (void)hipEventRecord(b, st ? st : g_ctx.stream);
-    g_prof_pending.push_back({cls, a, b, flops, bytes});
-    return (long)g_prof_pending.size() - 1;
+    (void)hipEventRecord(b, st ? st : LN().stream);
+    LN().prof_pending.push_back({cls, a, b, flops, bytes});
+    return (long)LN().prof_pending.size() - 1;
 }
 
 static void prof_set_work(long idx, double flops, double bytes)
 {
-    if (idx >= 0 && idx < (long)g_prof_pending.size()) {
-        g_prof_pending[idx].flops = flops;
-        g_prof_pending[idx].bytes = bytes;
+    std::vector<ProfPending> &pend = LN().prof_pending;
+    if (idx >= 0 && idx < (long)pend.size()) {
+        pend[idx].flops = flops;
+        pend[idx].bytes = bytes;
     }
 }
 
-// Called after a stream synchronisation: fold pending event pairs.
+// Called after the lane's stream synchronisation: fold its pending event pairs.
 static void prof_collect()
 {
-    for (auto &p : g_prof_pending) {
+    std::vector<ProfPending> &pend = LN().prof_pending;
+    std::lock_guard<std::mutex> lk(g_prof_mu);
+    for (auto &p : pend) {
         float ms = 0.f;
         if (hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
             ProfClass &c = g_prof_acc[p.cls];
@@ -213,7 +314,7 @@ static void prof_collect()
         g_prof_pool.push_back(p.a);
         g_prof_pool.push_back(p.b);
     }
-    g_prof_pending.clear();
+    pend.clear();
 }
 
 void profile_enable(int on) { g_prof = on != 0; }
@@ -232,11 +333,13 @@ int profile_only(const char *name)
 }
 void profile_reset()
 {
+    std::lock_guard<std::mutex> lk(g_prof_mu);
     for (auto &c : g_prof_acc)
         c = ProfClass{};
 }
 int profile_read(const char *name, double *out4)
 {
+    std::lock_guard<std::mutex> lk(g_prof_mu);
     for (int i = 0; i < PC_N; ++i)
         if (!std::strcmp(name, kProfNames[i])) {
             out4[0] = g_prof_acc[i].launches;
@@ -274,18 +377,12 @@ static int ctx_init()
         g_ctx.status = FNEC_UNAVAILABLE;
         return g_ctx.status;
     }
-    if (hipSetDevice(g_ctx.device) != hipSuccess || hipStreamCreate(&g_ctx.stream) != hipSuccess ||
-        hipStreamCreate(&g_ctx.copy_stream) != hipSuccess || hipEventCreate(&g_ctx.ev_rp) != hipSuccess ||
-        hipStreamCreate(&g_ctx.side_stream) != hipSuccess) {
+    if (hipSetDevice(g_ctx.device) != hipSuccess) {
         set_error("device initialisation failed");
         g_ctx.status = FNEC_INTERNAL;
         return g_ctx.status;
     }
-    for (auto &e : g_ctx.ev)
-        if (hipEventCreate(&e) != hipSuccess) {
-            g_ctx.status = FNEC_INTERNAL;
-            return g_ctx.status;
-        }
+    // lanes create their streams when first bound
     g_ctx.status = FNEC_OK;
     return g_ctx.status;
 }
@@ -298,8 +395,6 @@ int ensure_ready()
     return st;
 }
 
-hipStream_t stream() { return g_ctx.stream; }
-
 // Pinned host buffer of at least `bytes`, grown on demand (the stream is
 // idle whenever a batch is planned, so the old buffer is free to release).
 static char *pinned(char *&buf, size_t &have, size_t bytes)
@@ -307,8 +402,8 @@ static char *pinned(char *&buf, size_t &have, size_t bytes)
     if (bytes <= have)
         return buf;
     if (buf) {
-        (void)hipStreamSynchronize(g_ctx.stream);
-        (void)hipStreamSynchronize(g_ctx.copy_stream);
+        (void)hipStreamSynchronize(LN().stream);
+        (void)hipStreamSynchronize(LN().side_stream);
         (void)hipHostFree(buf);
         buf = nullptr;
         have = 0;
@@ -327,8 +422,8 @@ static int grow_arena(char *&arena, size_t &have, size_t bytes, char **out)
 {
     if (bytes > have) {
         if (arena) {
-            HIPCHK(hipStreamSynchronize(g_ctx.stream));
-            HIPCHK(hipStreamSynchronize(g_ctx.copy_stream));
+            HIPCHK(hipStreamSynchronize(LN().stream));
+            HIPCHK(hipStreamSynchronize(LN().side_stream));
             HIPCHK(hipFree(arena));
             arena = nullptr;
             have = 0;
@@ -353,7 +448,7 @@ static int grow_arena(char *&arena, size_t &have, size_t bytes, char **out)
 
 static int arena_reserve(size_t bytes, char **out)
 {
-    return grow_arena(g_ctx.arena, g_ctx.arena_bytes, bytes, out);
+    return grow_arena(LN().arena, LN().arena_bytes, bytes, out);
 }
 
 // ---------------------------------------------------------------------------
@@ -571,6 +666,7 @@ int problem_from_paths(HostProblem &hp, const fnec_seg *segs, size_t n, const fn
 
 struct Plan {
     std::vector<Prob> probs;
+    int upd_tm = kTile32; // row tile of the FP32 trailing update (64 or 128)
     // work lists, concatenated into one int buffer
     std::vector<int> ints;
     size_t fill_off = 0, fill_cnt = 0; // fill items in the generated region (item units)
@@ -649,6 +745,7 @@ static int g_mix_first_chunk = kMixFirstChunk;
 // Gang size of a CG solve: G = clamp(floor(n^2 / gang_n^2), 1, min(gang_max, blocks)),
 // so a problem's solve time shrinks towards the chain latency of its blocks
 // instead of one workgroup streaming the whole factor.
+static int g_update_tile_m = kTile32; // row tile of the FP32 trailing update
 static int g_gang_n = 768;
 static int g_gang_max = 4;
 static int gang_size(int n)
@@ -810,7 +907,7 @@ static void lu_plan(std::vector<int> &ints, const std::vector<int> &ms, const st
 // two 32-column panels, so that the trailing update runs with K = 64 and reads
 // and writes the trailing matrix once per 64 columns.
 static void lu_plan_outer(std::vector<int> &ints, const std::vector<int> &ms, std::vector<Plan::Outer> &out,
-                          std::vector<int4> &gensegs, size_t &gen_items)
+                          std::vector<int4> &gensegs, size_t &gen_items, int tm)
 {
     auto align4 = [&]() {
         while (ints.size() % 4)
@@ -922,7 +1019,7 @@ static void lu_plan_outer(std::vector<int> &ints, const std::vector<int> &ms, st
             const int s = ms[b];
             if (s <= P + 2 * kPanel)
                 continue;
-            const int ny = (s - P - 2 * kPanel + kTile32 - 1) / kTile32;
+            const int ny = (s - P - 2 * kPanel + tm - 1) / tm;
             gensegs.push_back(make_int4((int)gen_items, (int)b, ny, 1));
             gen_items += (size_t)ny;
         }
@@ -932,7 +1029,7 @@ static void lu_plan_outer(std::vector<int> &ints, const std::vector<int> &ms, st
             const int s = ms[b];
             if (s <= P + 4 * kPanel)
                 continue;
-            const int ny = (s - P - 2 * kPanel + kTile32 - 1) / kTile32;
+            const int ny = (s - P - 2 * kPanel + tm - 1) / tm;
             const int nz = (s - P - 4 * kPanel + kTile32N - 1) / kTile32N;
             gensegs.push_back(make_int4((int)gen_items, (int)b, ny, nz));
             gen_items += (size_t)ny * nz;
@@ -1097,7 +1194,8 @@ static void plan_build(Plan &pl, const std::vector<const HostProblem *> &hps, bo
         std::vector<int> ms(np);
         for (size_t b = 0; b < np; ++b)
             ms[b] = hps[b]->n;
-        lu_plan_outer(pl.ints, ms, pl.outer32, pl.gensegs, pl.gen_items);
+        pl.upd_tm = g_update_tile_m;
+        lu_plan_outer(pl.ints, ms, pl.outer32, pl.gensegs, pl.gen_items, pl.upd_tm);
         // CG solves of problems above solve1_max_n (flag-chained k_trsv0/1):
         // items (problem, -1, phase, block), phase 0 blocks in forward order,
         // then phase 1 blocks in backward order, interleaved across problems
@@ -1294,7 +1392,7 @@ static void plan_build(Plan &pl, const std::vector<const HostProblem *> &hps, bo
 static int plan_upload(Plan &pl, const std::vector<const HostProblem *> &hps, char *base,
                        bool with_solve)
 {
-    hipStream_t st = g_ctx.stream;
+    hipStream_t st = LN().stream;
     const size_t np = hps.size();
     pl.lud.clear();
     pl.lud32.clear();
@@ -1405,7 +1503,7 @@ static int plan_upload(Plan &pl, const std::vector<const HostProblem *> &hps, ch
     }
     // one staged copy: host inputs, problem table, work lists, zeroed results
     const size_t nbytes = pl.io_end - pl.in_begin;
-    char *hs = pinned(g_ctx.h_up, g_ctx.h_up_bytes, nbytes);
+    char *hs = pinned(LN().h_up, LN().h_up_bytes, nbytes);
     if (!hs) {
         set_error("pinned host memory exhausted");
         return FNEC_RESOURCE_EXHAUSTED;
@@ -1461,12 +1559,12 @@ static int mixed_launch(Plan &pl, char *base)
 {
     HostTimer *enq = new HostTimer(kHostEnq); // until the first wait on the stream
     std::unique_ptr<HostTimer> enq_own(enq);
-    hipStream_t st = g_ctx.stream;
+    hipStream_t st = LN().stream;
     const Prob *dp = (const Prob *)(base + pl.off_probs);
     const int *ints = wl_ptr(base, pl, 0);
     const int *all = ints + pl.all_off;
     const unsigned np = (unsigned)pl.all_cnt;
-    HIPCHK(hipEventRecord(g_ctx.ev[2], st));
+    HIPCHK(hipEventRecord(LN().ev[2], st));
     const LuDesc<float> *lud = (const LuDesc<float> *)(base + pl.off_lud32);
     // Look-ahead: the panel chain of step k + 1 (panels, their column solves
     // and the narrow update, all inside the columns [P + 64, P + 128)) runs on
@@ -1475,14 +1573,14 @@ static int mixed_launch(Plan &pl, char *base)
     // step reach the columns left of it in the main stream's outer launch,
     // after the trailing update that reads those columns.
     const size_t nsteps = pl.outer32.size();
-    while (g_ctx.sync_events.size() < 2 * nsteps + 1) {
+    while (LN().sync_events.size() < 2 * nsteps + 1) {
         hipEvent_t e = nullptr;
         HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-        g_ctx.sync_events.push_back(e);
+        LN().sync_events.push_back(e);
     }
-    hipEvent_t *ev_near = g_ctx.sync_events.data();          // [k]: main updated step k + 1's columns
-    hipEvent_t *ev_chain = g_ctx.sync_events.data() + nsteps; // [k]: step k's panel chain done
-    hipStream_t side = g_ctx.side_stream;
+    hipEvent_t *ev_near = LN().sync_events.data();          // [k]: main updated step k + 1's columns
+    hipEvent_t *ev_chain = LN().sync_events.data() + nsteps; // [k]: step k's panel chain done
+    hipStream_t side = LN().side_stream;
     auto chain = [&](const Plan::Outer &o) {
         hipEvent_t e = prof_begin(PC_LU_PANEL, side);
         lu_launch_panel_f32(lud, o.a, ints, side);
@@ -1530,7 +1628,7 @@ static int mixed_launch(Plan &pl, char *base)
         if (o.on_cnt) {
             hipEvent_t e = prof_begin(PC_LU_UPDATE);
             lu_launch_update_f32(lud, 4 * o.on_off, o.on_cnt, gen_ptr(base, pl), o.P, 2, o.P + 2 * kPanel,
-                                 o.P + 2 * kPanel, o.P + 4 * kPanel, st);
+                                 o.P + 2 * kPanel, o.P + 4 * kPanel, pl.upd_tm, st);
             prof_end(PC_LU_UPDATE, e, o.f_on, o.b_on);
         }
         if (k + 1 < nsteps) {
@@ -1542,12 +1640,12 @@ static int mixed_launch(Plan &pl, char *base)
         if (o.of_cnt) {
             hipEvent_t e = prof_begin(PC_LU_UPDATE);
             lu_launch_update_f32(lud, 4 * o.of_off, o.of_cnt, gen_ptr(base, pl), o.P, 2, o.P + 2 * kPanel,
-                                 o.P + 4 * kPanel, 0x7fffffff, st);
+                                 o.P + 4 * kPanel, 0x7fffffff, pl.upd_tm, st);
             prof_end(PC_LU_UPDATE, e, o.f_of, o.b_of);
         }
     }
     HIPCHK(hipGetLastError());
-    HIPCHK(hipEventRecord(g_ctx.ev[3], st));
+    HIPCHK(hipEventRecord(LN().ev[3], st));
     const int4 *vsol = (const int4 *)(ints + pl.vsol_off);
     unsigned long long *tickets = (unsigned long long *)(base + pl.off_tickets);
     int ntrsv = 0; // launches of k_trsv so far: epoch ntrsv + 1, ticket counter ntrsv
@@ -1654,16 +1752,16 @@ static int mixed_launch(Plan &pl, char *base)
                        (const int2 *)(ints + pl.res_off));
     hipLaunchKernelGGL(k_residual_finish, dim3(np), dim3(64), 0, st, dp, all);
     HIPCHK(hipGetLastError());
-    HIPCHK(hipEventRecord(g_ctx.ev[4], st));
+    HIPCHK(hipEventRecord(LN().ev[4], st));
     return FNEC_OK;
 }
 
 // Launch the whole solve on the context stream; records phase events.
 static int plan_launch(Plan &pl, char *base, bool with_solve)
 {
-    hipStream_t st = g_ctx.stream;
+    hipStream_t st = LN().stream;
     const Prob *dp = (const Prob *)(base + pl.off_probs);
-    HIPCHK(hipEventRecord(g_ctx.ev[0], st));
+    HIPCHK(hipEventRecord(LN().ev[0], st));
     gen_launch(pl, base, st);
     if (pl.fill_cnt) {
         hipEvent_t e = prof_begin(PC_FILL);
@@ -1672,7 +1770,7 @@ static int plan_launch(Plan &pl, char *base, bool with_solve)
         prof_end(PC_FILL, e, pl.fill_flops, 0);
     }
     HIPCHK(hipGetLastError());
-    HIPCHK(hipEventRecord(g_ctx.ev[1], st));
+    HIPCHK(hipEventRecord(LN().ev[1], st));
     if (!with_solve)
         return FNEC_OK;
     if (pl.mixed)
@@ -1693,7 +1791,7 @@ static int plan_launch(Plan &pl, char *base, bool with_solve)
     hipLaunchKernelGGL(k_gram_fix, dim3((unsigned)pl.all_cnt), dim3(256), 0, st, dp,
                        wl_ptr(base, pl, pl.all_off));
     HIPCHK(hipGetLastError());
-    HIPCHK(hipEventRecord(g_ctx.ev[2], st));
+    HIPCHK(hipEventRecord(LN().ev[2], st));
     const LuDesc<double> *lud = (const LuDesc<double> *)(base + pl.off_lud);
     const int *ints = wl_ptr(base, pl, 0);
     for (size_t si = 0; si < pl.steps.size(); ++si) {
@@ -1715,7 +1813,7 @@ static int plan_launch(Plan &pl, char *base, bool with_solve)
         }
     }
     HIPCHK(hipGetLastError());
-    HIPCHK(hipEventRecord(g_ctx.ev[3], st));
+    HIPCHK(hipEventRecord(LN().ev[3], st));
     {
         hipEvent_t e = prof_begin(PC_BACKSUB);
         hipLaunchKernelGGL(k_backsub, dim3((unsigned)pl.all_cnt), dim3(256), 0, st, dp,
@@ -1727,14 +1825,14 @@ static int plan_launch(Plan &pl, char *base, bool with_solve)
     hipLaunchKernelGGL(k_residual_finish, dim3((unsigned)pl.all_cnt), dim3(64), 0, st, dp,
                        wl_ptr(base, pl, pl.all_off));
     HIPCHK(hipGetLastError());
-    HIPCHK(hipEventRecord(g_ctx.ev[4], st));
+    HIPCHK(hipEventRecord(LN().ev[4], st));
     return FNEC_OK;
 }
 
 static float ev_ms(int a, int b)
 {
     float ms = 0.f;
-    if (hipEventElapsedTime(&ms, g_ctx.ev[a], g_ctx.ev[b]) != hipSuccess)
+    if (hipEventElapsedTime(&ms, LN().ev[a], LN().ev[b]) != hipSuccess)
         return 0.f;
     return ms;
 }
@@ -1745,6 +1843,9 @@ int solve_problems(const std::vector<const HostProblem *> &hps, std::vector<fnec
                    std::vector<double *> *device_x, int precision,
                    const std::function<void(const std::vector<double *> &)> &overlap)
 {
+    LaneScope lane_scope;
+    if (lane_scope.status())
+        return lane_scope.status();
     const bool mixed = precision == FNEC_PREC_MIXED;
     if (mixed)
         for (const HostProblem *h : hps)
@@ -1757,12 +1858,13 @@ int solve_problems(const std::vector<const HostProblem *> &hps, std::vector<fnec
     Plan pl;
     pl.mixed = mixed;
     // the work lists keep their capacity across batches
-    static std::vector<int> ints_cache;
+    std::vector<int> &ints_cache = LN().plan_ints;
     pl.ints.swap(ints_cache);
     struct IntsBack {
         Plan &p;
-        ~IntsBack() { p.ints.swap(ints_cache); }
-    } ints_back{pl};
+        std::vector<int> &c;
+        ~IntsBack() { p.ints.swap(c); }
+    } ints_back{pl, ints_cache};
     {
         HostTimer ht(kHostPrep);
         plan_build(pl, hps, true);
@@ -1789,15 +1891,15 @@ int solve_problems(const std::vector<const HostProblem *> &hps, std::vector<fnec
         return st;
     // all result words in one copy
     const size_t nback = pl.io_end - pl.io_begin;
-    char *hb = pinned(g_ctx.h_back, g_ctx.h_back_bytes, nback);
+    char *hb = pinned(LN().h_back, LN().h_back_bytes, nback);
     if (!hb) {
         set_error("pinned host memory exhausted");
         return FNEC_RESOURCE_EXHAUSTED;
     }
-    HIPCHK(hipMemcpyAsync(hb, base + pl.io_begin, nback, hipMemcpyDeviceToHost, g_ctx.stream));
+    HIPCHK(hipMemcpyAsync(hb, base + pl.io_begin, nback, hipMemcpyDeviceToHost, LN().stream));
     {
         HostTimer ht(kHostWait);
-        HIPCHK(stream_wait(g_ctx.stream));
+        HIPCHK(stream_wait(LN().stream));
     }
     HostTimer ht_post(kHostPost);
     auto back = [&](size_t off) { return hb + (off - pl.io_begin); };
@@ -1866,6 +1968,9 @@ int solve_problems(const std::vector<const HostProblem *> &hps, std::vector<fnec
 
 int fill_only(const HostProblem &hp, fnec_c64 *z_out)
 {
+    LaneScope lane_scope;
+    if (lane_scope.status())
+        return lane_scope.status();
     Plan pl;
     std::vector<const HostProblem *> v{&hp};
     plan_build(pl, v, false);
@@ -1879,9 +1984,9 @@ int fill_only(const HostProblem &hp, fnec_c64 *z_out)
         return st;
     const int n = hp.n, ldz = roundup(n, 64);
     std::vector<double> zr((size_t)ldz * n), zi((size_t)ldz * n);
-    HIPCHK(hipMemcpyAsync(zr.data(), base + pl.off_z[0], 8 * zr.size(), hipMemcpyDeviceToHost, g_ctx.stream));
-    HIPCHK(hipMemcpyAsync(zi.data(), base + pl.off_z[0] + 8 * zr.size(), 8 * zi.size(), hipMemcpyDeviceToHost, g_ctx.stream));
-    HIPCHK(hipStreamSynchronize(g_ctx.stream));
+    HIPCHK(hipMemcpyAsync(zr.data(), base + pl.off_z[0], 8 * zr.size(), hipMemcpyDeviceToHost, LN().stream));
+    HIPCHK(hipMemcpyAsync(zi.data(), base + pl.off_z[0] + 8 * zr.size(), 8 * zi.size(), hipMemcpyDeviceToHost, LN().stream));
+    HIPCHK(hipStreamSynchronize(LN().stream));
     for (int i = 0; i < n; ++i)
         for (int j = 0; j < n; ++j) {
             z_out[(size_t)i * n + j].re = zr[(size_t)j * ldz + i];
@@ -1892,6 +1997,9 @@ int fill_only(const HostProblem &hp, fnec_c64 *z_out)
 
 int normal_eqs_only(const HostProblem &hp, fnec_c64 *aug_out)
 {
+    LaneScope lane_scope;
+    if (lane_scope.status())
+        return lane_scope.status();
     Plan pl;
     std::vector<const HostProblem *> v{&hp};
     plan_build(pl, v, true);
@@ -1901,7 +2009,7 @@ int normal_eqs_only(const HostProblem &hp, fnec_c64 *aug_out)
         return st;
     if ((st = plan_upload(pl, v, base, true)))
         return st;
-    hipStream_t s = g_ctx.stream;
+    hipStream_t s = LN().stream;
     const Prob *dp = (const Prob *)(base + pl.off_probs);
     gen_launch(pl, base, s);
     hipLaunchKernelGGL(k_fill_z, dim3((unsigned)pl.fill_cnt), dim3(256), 0, s, dp,
@@ -1934,6 +2042,9 @@ int normal_eqs_only(const HostProblem &hp, fnec_c64 *aug_out)
 
 int rp_stage(const std::vector<RpHost> &jobs, RpStaged &S)
 {
+    LaneScope lane_scope;
+    if (lane_scope.status())
+        return lane_scope.status();
     HostTimer ht(kHostRp);
     const size_t nj = jobs.size();
     S = RpStaged{};
@@ -1943,7 +2054,7 @@ int rp_stage(const std::vector<RpHost> &jobs, RpStaged &S)
     S.o_tot.resize(nj);
     S.o_out.resize(nj);
     S.npts.resize(nj);
-    static std::vector<int> ints; // capacity kept across batches
+    std::vector<int> &ints = LN().rp_ints; // capacity kept across batches
     ints.clear();
     // device-only work buffers
     for (size_t j = 0; j < nj; ++j) {
@@ -1989,12 +2100,12 @@ int rp_stage(const std::vector<RpHost> &jobs, RpStaged &S)
         cur += 32 * jobs[j].npts;
     }
     S.out_end = cur = (cur + 255) & ~(size_t)255;
-    int st = grow_arena(g_ctx.rp_arena, g_ctx.rp_arena_bytes, cur + 256, &S.base);
+    int st = grow_arena(LN().rp_arena, LN().rp_arena_bytes, cur + 256, &S.base);
     if (st)
         return st;
     char *base = S.base;
     std::vector<RpProb> probs(nj);
-    char *hs = pinned(g_ctx.h_rp, g_ctx.h_rp_bytes, S.out_begin - in_begin);
+    char *hs = pinned(LN().h_rp, LN().h_rp_bytes, S.out_begin - in_begin);
     if (!hs) {
         set_error("pinned host memory exhausted");
         return FNEC_RESOURCE_EXHAUSTED;
@@ -2033,20 +2144,23 @@ int rp_stage(const std::vector<RpHost> &jobs, RpStaged &S)
     put(S.o_probs, probs.data(), sizeof(RpProb) * nj);
     put(S.o_ints, ints.data(), 4 * ints.size());
     // the copy stream carries the upload so that it overlaps the solve's kernels
-    HIPCHK(hipMemcpyAsync(base + in_begin, hs, S.out_begin - in_begin, hipMemcpyHostToDevice, g_ctx.copy_stream));
-    HIPCHK(hipEventRecord(g_ctx.ev_rp, g_ctx.copy_stream));
+    HIPCHK(hipMemcpyAsync(base + in_begin, hs, S.out_begin - in_begin, hipMemcpyHostToDevice, LN().side_stream));
+    HIPCHK(hipEventRecord(LN().ev_rp, LN().side_stream));
     return FNEC_OK;
 }
 
 int rp_finish(RpStaged &S, const std::vector<fnec_rp_row *> &outs, std::vector<double> &totals)
 {
+    LaneScope lane_scope;
+    if (lane_scope.status())
+        return lane_scope.status();
     const size_t nj = S.nj;
     totals.assign(nj, 0.0);
     if (!nj)
         return FNEC_OK;
-    hipStream_t s = g_ctx.stream;
+    hipStream_t s = LN().stream;
     char *base = S.base;
-    HIPCHK(hipStreamWaitEvent(s, g_ctx.ev_rp, 0));
+    HIPCHK(hipStreamWaitEvent(s, LN().ev_rp, 0));
     const RpProb *dp = (const RpProb *)(base + S.o_probs);
     const int *wi = (const int *)(base + S.o_ints);
     hipEvent_t e_rp = prof_begin(PC_RP);
@@ -2064,7 +2178,7 @@ int rp_finish(RpStaged &S, const std::vector<fnec_rp_row *> &outs, std::vector<d
     for (size_t j = 0; j < nj; ++j)
         rows = rows || (outs.size() > j && outs[j] && S.npts[j]);
     const size_t end = rows ? S.out_end : S.rows_begin;
-    char *hb = pinned(g_ctx.h_back, g_ctx.h_back_bytes, end - S.out_begin);
+    char *hb = pinned(LN().h_back, LN().h_back_bytes, end - S.out_begin);
     if (!hb) {
         set_error("pinned host memory exhausted");
         return FNEC_RESOURCE_EXHAUSTED;
@@ -2086,6 +2200,9 @@ int rp_finish(RpStaged &S, const std::vector<fnec_rp_row *> &outs, std::vector<d
 
 int rp_run(const std::vector<RpHost> &jobs, std::vector<fnec_rp_row *> &outs, std::vector<double> &totals)
 {
+    LaneScope lane_scope;
+    if (lane_scope.status())
+        return lane_scope.status();
     RpStaged S;
     int st = rp_stage(jobs, S);
     if (st)
@@ -2142,11 +2259,14 @@ int rp_host_from(RpHost &h, const fnec_seg *segs, const fnec_c64 *cur, size_t n,
 
 int mfma_probe(const double *a, const double *b, double *out)
 {
+    LaneScope lane_scope;
+    if (lane_scope.status())
+        return lane_scope.status();
     char *base = nullptr;
     int st = arena_reserve(4096 * 4, &base);
     if (st)
         return st;
-    hipStream_t s = g_ctx.stream;
+    hipStream_t s = LN().stream;
     HIPCHK(hipMemcpyAsync(base, a, 64 * 8, hipMemcpyHostToDevice, s));
     HIPCHK(hipMemcpyAsync(base + 1024, b, 64 * 8, hipMemcpyHostToDevice, s));
     hipLaunchKernelGGL(k_mfma_probe, dim3(1), dim3(64), 0, s, (const double *)base,
@@ -2159,11 +2279,14 @@ int mfma_probe(const double *a, const double *b, double *out)
 
 int mfma_probe32(const float *a, const float *b, float *out)
 {
+    LaneScope lane_scope;
+    if (lane_scope.status())
+        return lane_scope.status();
     char *base = nullptr;
     int st = arena_reserve(4096 * 4, &base);
     if (st)
         return st;
-    hipStream_t s = g_ctx.stream;
+    hipStream_t s = LN().stream;
     HIPCHK(hipMemcpyAsync(base, a, 64 * 4, hipMemcpyHostToDevice, s));
     HIPCHK(hipMemcpyAsync(base + 1024, b, 64 * 4, hipMemcpyHostToDevice, s));
     hipLaunchKernelGGL(k_mfma_probe32, dim3(1), dim3(64), 0, s, (const float *)base,
@@ -2193,6 +2316,16 @@ int set_tunable(const char *name, long value)
         g_gang_max = (int)value;
         return FNEC_OK;
     }
+    if (k == "update_tile_m" && (value == 64 || value == 128)) {
+        g_update_tile_m = (int)value;
+        return FNEC_OK;
+    }
+    if (k == "lanes" && value >= 1 && value <= kMaxLanes) {
+        std::lock_guard<std::mutex> lk(g_lane_mu);
+        g_lanes_on = (int)value;
+        g_lane_cv.notify_all();
+        return FNEC_OK;
+    }
     set_error("unknown tunable or value out of range");
     return FNEC_INVALID_ARG;
 }
@@ -2208,6 +2341,10 @@ long get_tunable(const char *name)
         return g_gang_n;
     if (k == "gang_max")
         return g_gang_max;
+    if (k == "lanes")
+        return g_lanes_on;
+    if (k == "update_tile_m")
+        return g_update_tile_m;
     return -1;
 }
 

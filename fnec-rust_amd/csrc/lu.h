@@ -1085,12 +1085,17 @@ __global__ __launch_bounds__(256) void k_lu_trsm64_f32(const LuDesc<float> *__re
         }
 }
 
+// tm: row tile, 128 or 64 (the work list's row-tile count follows it).
 inline void lu_launch_update_f32(const LuDesc<float> *d, size_t off, size_t cnt, const int *ints,
-                                 int kbeg, int nk, int rbeg, int cbeg, int cend, hipStream_t st)
+                                 int kbeg, int nk, int rbeg, int cbeg, int cend, int tm, hipStream_t st)
 {
     const unsigned grid = (unsigned)((cnt + kXcds - 1) / kXcds * kXcds);
-    hipLaunchKernelGGL((k_lu_update_f32<kTile32, kTile32N>), dim3(grid), dim3(256), 0, st, d,
-                       (const int4 *)(ints + off), (int)cnt, kbeg, nk, rbeg, cbeg, cend);
+    if (tm == 64)
+        hipLaunchKernelGGL((k_lu_update_f32<64, kTile32N>), dim3(grid), dim3(256), 0, st, d,
+                           (const int4 *)(ints + off), (int)cnt, kbeg, nk, rbeg, cbeg, cend);
+    else
+        hipLaunchKernelGGL((k_lu_update_f32<kTile32, kTile32N>), dim3(grid), dim3(256), 0, st, d,
+                           (const int4 *)(ints + off), (int)cnt, kbeg, nk, rbeg, cbeg, cend);
 }
 
 // The 32-column update inside a 64-column outer step (rows [P + 32, m) x

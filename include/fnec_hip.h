@@ -177,7 +177,10 @@ typedef struct {
     fnec_c64 *x;             /* caller buffer of n + max(n_wires,1), or NULL */
     fnec_rp_row *rp;         /* caller buffer of rp_npts rows, or NULL */
     double rp_total;
-    double t_submit_us, t_done_us; /* session clock (steady, microseconds) */
+    /* session clock (steady, microseconds): submit, start of the request's
+     * batch (taken off the queue), and completion of that batch */
+    double t_submit_us, t_start_us, t_done_us;
+    uint64_t batch;          /* session batch the request ran in (0, 1, ...) */
     fnec_solve_stats stats;
 } fnec_result;
 
@@ -186,11 +189,21 @@ typedef struct {
 int fnec_hip_solve_batch(const fnec_request *reqs, size_t count, int precision, fnec_result *results);
 
 /* ---- serving session (continuous batching, nec_worker serving mode) ----
- * Requests are admitted by device-memory footprint against budget_bytes;
- * admitted requests are batched as they arrive and stay device-resident (Z,
- * LU factors, currents) until their result is polled. submit copies the
- * request's inputs; poll returns finished results (up to max) and does not
- * block unless wait_us > 0. */
+ * The session's worker threads (one per engine lane, tunable "lanes", default
+ * 2) take queued requests as batches and solve them; each batch runs on a lane
+ * of its own (streams, device arena), so a batch formed from requests that
+ * arrived while another is on the device runs beside it and their kernels
+ * overlap. Requests are admitted in arrival order while the device footprint
+ * of all batches in flight stays within budget_bytes; a batch holds Z, the LU
+ * factors and the currents on the device until it completes. A batch closes
+ * when no request has been submitted for 500 us, when it holds max_batch
+ * requests, or when its oldest request has waited 20 ms. submit copies the
+ * request's inputs and returns at once; poll returns finished results (up to
+ * max), waiting up to wait_us for the first one while requests are queued or
+ * running; pending counts queued, running and unpolled requests. A result's
+ * status is that of its request; after a failed one, fnec_hip_last_error()
+ * on the polling thread gives the batch's message. destroy waits for the
+ * batches on the device and drops queued requests. */
 typedef struct fnec_session fnec_session;
 fnec_session *fnec_hip_session_create(size_t budget_bytes, int precision, int max_batch);
 int fnec_hip_session_submit(fnec_session *s, const fnec_request *req);
@@ -215,8 +228,9 @@ int fnec_hip_debug_mfma_probe(const double *a16x4, const double *b4x16, double *
 int fnec_hip_debug_mfma_probe_f32(const float *a16x4, const float *b4x16, float *lanes_out);
 
 /* ---- tunables (no counterpart in the reference) ----
- * Knobs of the batched engine, set by the autotuner (fnec-rust_amd autotune())
- * or by hand. They change speed, and the rounding of a mixed-path solve
+ * Knobs of the batched engine, set by the autotuner (autotune() in the
+ * fnec-rust_amd package: timed batches of a representative workload) or by
+ * hand. A knob is read when a batch is planned. They change speed, and the rounding of a mixed-path solve
  * within its accuracy contract (the summation order of the CG solves follows
  * the gang size); a result never depends on what else is in the batch.
  *   "solve1_max_n"    CG solves of problems with n <= value run in k_zsolve
@@ -228,6 +242,12 @@ int fnec_hip_debug_mfma_probe_f32(const float *a16x4, const float *b4x16, float 
  *                     gang_n 0 .. 65536, gang_max 1 .. 64.
  *   "mix_first_chunk" CG iterations launched before the first convergence
  *                     read-back. 1 .. 12.
+ *   "update_tile_m"   row tile of the mixed path's FP32 trailing-matrix
+ *                     update (k_lu_update_f32 work items): 128 or 64.
+ *   "lanes"           batches that may run on the device at once (each on a
+ *                     lane: two streams, a device arena, pinned staging);
+ *                     also the worker count of a session created afterwards.
+ *                     1 .. 4, default 2.
  * set: FNEC_OK or FNEC_INVALID_ARG; get: the value, -1 for an unknown name. */
 int fnec_hip_set_tunable(const char *name, long value);
 long fnec_hip_get_tunable(const char *name);

@@ -8,10 +8,13 @@
   next to the solves (many small problems with 2 x 2 degree grids);
 - the session returns every request once, with the same result as a batch,
   whatever the admission splits (max_batch, HBM budget), and rejects a request
-  whose footprint exceeds its budget.
+  whose footprint exceeds its budget;
+- requests that arrive while a batch is on the device run beside it on a
+  second lane, and each result is bitwise the result of the request alone.
 """
 import ctypes as C  # noqa: N812
 import importlib
+import time
 
 import numpy as np
 import pytest
@@ -80,15 +83,15 @@ def run_session(probs, precision, budget, max_batch, order=None):
                 return st, None
         out = (F.Result * 64)()
         got = {}
-        for _ in range(10 * len(probs) + 10):
-            k = L.fnec_hip_session_poll(sess, C.cast(out, C.c_void_p), 64, 0)
+        deadline = time.monotonic() + 120.0
+        while len(got) < len(probs) and time.monotonic() < deadline:
+            k = L.fnec_hip_session_poll(sess, C.cast(out, C.c_void_p), 64, 200000)
             assert k >= 0
             for j in range(k):
                 r = out[j]
                 assert r.tag not in got
                 got[r.tag] = (r.status, complex(r.z_in.re, r.z_in.im), r.t_submit_us, r.t_done_us)
-            if len(got) == len(probs):
-                break
+        assert len(got) == len(probs)
         assert L.fnec_hip_session_pending(sess) == 0
         return 0, got
     finally:
@@ -135,18 +138,65 @@ def test_session_reuses_request_storage_across_polls():
                 keep.append((q, k))
                 assert L.fnec_hip_session_submit(sess, C.byref(q)) == 0
             got = {}
-            for _ in range(20):
-                k = L.fnec_hip_session_poll(sess, C.cast(out, C.c_void_p), 16, 0)
+            deadline = time.monotonic() + 60.0
+            while len(got) < len(order) and time.monotonic() < deadline:
+                k = L.fnec_hip_session_poll(sess, C.cast(out, C.c_void_p), 16, 200000)
                 for j in range(k):
                     got[out[j].tag] = (out[j].status, complex(out[j].z_in.re, out[j].z_in.im))
-                if len(got) == len(order):
-                    break
             for i in order:
                 status, zin = got[100 * rnd + i]
                 assert status == F.OK
                 assert zin == batch[i]["z_in"]
     finally:
         L.fnec_hip_session_destroy(sess)
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_session_overlaps_batches_and_results_match_alone(precision):
+    """Continuous batching: requests submitted while a batch of large solves is
+    on the device form a second batch that runs beside it on another lane (its
+    start precedes the first batch's end), and every result, far-field total
+    included, is bitwise the result of the same request solved alone."""
+    assert F.get_tunable("lanes") >= 2
+    heavy = problems(26, [4096] * 6)
+    light = problems(27, [51, 300, 77, 900, 150, 64, 500, 120])
+    probs = heavy + light
+    grids = [(TH2, PH2) if i % 3 == 1 else None for i in range(len(probs))]
+    alone = [F.solve_batch([p], precision=precision, rp_grids=[g])[0] for p, g in zip(probs, grids)]
+    L = F.lib()
+    sess = L.fnec_hip_session_create(1 << 40, precision, 256)
+    assert sess
+    try:
+        reqs = [F.make_request(p, tag=i, rp_theta=g[0] if g else None, rp_phi=g[1] if g else None)
+                for i, (p, g) in enumerate(zip(probs, grids))]
+        for i in range(len(heavy)):
+            assert L.fnec_hip_session_submit(sess, C.byref(reqs[i][0])) == 0
+        time.sleep(0.003)
+        for i in range(len(heavy), len(probs)):
+            assert L.fnec_hip_session_submit(sess, C.byref(reqs[i][0])) == 0
+        out = (F.Result * 64)()
+        got = {}
+        deadline = time.monotonic() + 120.0
+        while len(got) < len(probs) and time.monotonic() < deadline:
+            k = L.fnec_hip_session_poll(sess, C.cast(out, C.c_void_p), 64, 200000)
+            assert k >= 0
+            for j in range(k):
+                r = out[j]
+                got[r.tag] = (r.status, complex(r.z_in.re, r.z_in.im), complex(r.feed_current.re, r.feed_current.im),
+                              r.rp_total, r.batch, r.t_start_us, r.t_done_us)
+        assert len(got) == len(probs)
+    finally:
+        L.fnec_hip_session_destroy(sess)
+    for i, a in enumerate(alone):
+        status, zin, cur, rp_total, _, _, _ = got[i]
+        assert status == F.OK and a["status"] == F.OK
+        assert zin == a["z_in"]
+        assert cur == a["x"][int(probs[i]["feed_index"])]
+        if grids[i] is not None:
+            assert rp_total == a["rp_total"]
+    heavy_end = max(got[i][6] for i in range(len(heavy)))
+    light_start = min(got[i][5] for i in range(len(heavy), len(probs)))
+    assert light_start < heavy_end
 
 
 def test_session_rejects_request_over_budget():

@@ -214,3 +214,41 @@ def test_mixed_with_row_interchanges():
     fi = int(p["feed_index"])
     assert abs(x64[fi] - xm[fi]) / abs(x64[fi]) <= 1e-6
     assert rel(xm, x64) <= 1e-6
+
+
+def test_update_tile_64_is_bitwise_the_128_tile():
+    """The FP32 trailing update with 64-row tiles accumulates every element in
+    the same order as with 128-row tiles: the factors, and so the solutions,
+    are bitwise the same."""
+    probs = random_problems(14, [51, 200, 700, 1300, 2100])
+    assert F.get_tunable("update_tile_m") == 128
+    a = F.solve_batch(probs, precision=F.PREC_MIXED)
+    F.set_tunable("update_tile_m", 64)
+    try:
+        b = F.solve_batch(probs, precision=F.PREC_MIXED)
+    finally:
+        F.set_tunable("update_tile_m", 128)
+    for x, y in zip(a, b):
+        assert x["status"] == F.OK and y["status"] == F.OK
+        assert np.array_equal(x["x"], y["x"])
+
+
+def test_autotune_keeps_results_within_contract():
+    """autotune() on a mixed-N batch: the chosen knobs are no slower than the
+    start and the batch solved with them is within 1e-6 (Z_in) of the start."""
+    A = importlib.import_module("fnec-rust_amd.tuning")
+    probs = random_problems(13, [51, 300, 700, 1200, 2000, 90, 450])
+    start = {k: F.get_tunable(k) for k in A.SPACE}
+    try:
+        before = F.solve_batch(probs, precision=F.PREC_MIXED)
+        out = F.autotune(probs, repeats=1, rounds=1)
+        print("autotune:", out["knobs"], f"{out['start_seconds'] * 1e3:.1f} -> {out['seconds'] * 1e3:.1f} ms")
+        assert out["seconds"] <= out["start_seconds"]
+        assert {k: F.get_tunable(k) for k in A.SPACE} == out["knobs"]
+        after = F.solve_batch(probs, precision=F.PREC_MIXED)
+        for a, b in zip(before, after):
+            assert b["status"] == F.OK
+            assert abs(b["z_in"] - a["z_in"]) <= 1e-6 * abs(a["z_in"])
+    finally:
+        for k, v in start.items():
+            F.set_tunable(k, v)
