@@ -86,7 +86,7 @@ def build_requests(stream):
     return items
 
 
-def serve(items, window, steps, warmup, prof_windows, precision, budget, max_batch, depth):
+def serve(items, window, steps, warmup, prof_windows, precision, budget, max_batch, depth, timed_class=None):
     """Warmup windows, then ``prof_windows`` untimed windows with events on
     every kernel class (the per-class breakdown; it names the dominant class),
     then the timed windows with events around the dominant class only. In each
@@ -144,6 +144,8 @@ def serve(items, window, steps, warmup, prof_windows, precision, budget, max_bat
     prof_all = F.profile(enable=False)
     timed_cls = [k for k in prof_all if prof_all[k][0] > 0 and prof_all[k][2] > 0]
     dom = max(timed_cls, key=lambda k: prof_all[k][1]) if timed_cls else "lu_update"
+    if timed_class:
+        dom = timed_class
     F.profile(enable=True, reset=True, only=dom)
     t0 = time.perf_counter()
     lat, batches = run_phase(warmup + prof_windows, steps)
@@ -259,6 +261,8 @@ def measured_traffic(kernel, precision):
     e = d.get("kernels", {}).get(kernel)
     if not e or e.get("precision") != ("mixed" if precision == F.PREC_MIXED else "fp64"):
         return None
+    if kernel == "lu_update" and e.get("symbol") != f"k_lu_update_f32<{F.get_tunable('update_tile_m')}, 64>":
+        return None  # measured with the other row tile
     return {"bytes_per_launch": e["hbm_bytes_per_launch"], "source": "profiles/serving_stream_traffic.json"}
 
 
@@ -288,6 +292,8 @@ def main():
     ap.add_argument("--plain-sample", type=int, default=256)
     ap.add_argument("--max-batch", type=int, default=256)
     ap.add_argument("--lanes", type=int, default=2, help="batches on the device at once")
+    ap.add_argument("--profile-class", default=None,
+                    help="kernel class timed in the timed windows (default: the dominant one)")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the engine's default knobs (no autotune on a window of the stream)")
     ap.add_argument("--depth", type=int, default=None,
@@ -327,7 +333,7 @@ def main():
         print(f"[bench] autotune: {tuned}", file=sys.stderr, flush=True)
     elapsed, lat, zs_served, dom, prof_all, prof, batches = serve(
         items, args.window, args.steps, args.warmup, args.profile_windows, precision,
-        int(args.budget_gib * 2 ** 30), args.max_batch, depth)
+        int(args.budget_gib * 2 ** 30), args.max_batch, depth, args.profile_class)
     first = (args.warmup + args.profile_windows) * args.window
     timed = args.steps * args.window
     value = timed / elapsed

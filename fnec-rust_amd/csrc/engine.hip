@@ -745,8 +745,8 @@ static int g_mix_first_chunk = kMixFirstChunk;
 // Gang size of a CG solve: G = clamp(floor(n^2 / gang_n^2), 1, min(gang_max, blocks)),
 // so a problem's solve time shrinks towards the chain latency of its blocks
 // instead of one workgroup streaming the whole factor.
-static int g_update_tile_m = kTile32; // row tile of the FP32 trailing update
-static int g_gang_n = 768;
+static int g_update_tile_m = 64; // row tile of the FP32 trailing update (64 or 128)
+static int g_gang_n = 512;
 static int g_gang_max = 4;
 static int gang_size(int n)
 {

@@ -239,11 +239,13 @@ int fnec_hip_debug_mfma_probe_f32(const float *a16x4, const float *b4x16, float 
  *   "gang_n", "gang_max"  gang size of a CG solve:
  *                     G = clamp(floor(n^2 / gang_n^2), 1, min(gang_max, ceil(n / 64)));
  *                     gang_n = 0 keeps one workgroup per problem.
- *                     gang_n 0 .. 65536, gang_max 1 .. 64.
+ *                     gang_n 0 .. 65536 (default 512), gang_max 1 .. 64
+ *                     (default 4).
  *   "mix_first_chunk" CG iterations launched before the first convergence
  *                     read-back. 1 .. 12.
  *   "update_tile_m"   row tile of the mixed path's FP32 trailing-matrix
- *                     update (k_lu_update_f32 work items): 128 or 64.
+ *                     update (k_lu_update_f32 work items): 64 (default) or
+ *                     128. Both accumulate every element in the same order.
  *   "lanes"           batches that may run on the device at once (each on a
  *                     lane: two streams, a device arena, pinned staging);
  *                     also the worker count of a session created afterwards.

@@ -221,13 +221,14 @@ def test_update_tile_64_is_bitwise_the_128_tile():
     the same order as with 128-row tiles: the factors, and so the solutions,
     are bitwise the same."""
     probs = random_problems(14, [51, 200, 700, 1300, 2100])
-    assert F.get_tunable("update_tile_m") == 128
-    a = F.solve_batch(probs, precision=F.PREC_MIXED)
-    F.set_tunable("update_tile_m", 64)
+    old = F.get_tunable("update_tile_m")
     try:
+        F.set_tunable("update_tile_m", 128)
+        a = F.solve_batch(probs, precision=F.PREC_MIXED)
+        F.set_tunable("update_tile_m", 64)
         b = F.solve_batch(probs, precision=F.PREC_MIXED)
     finally:
-        F.set_tunable("update_tile_m", 128)
+        F.set_tunable("update_tile_m", old)
     for x, y in zip(a, b):
         assert x["status"] == F.OK and y["status"] == F.OK
         assert np.array_equal(x["x"], y["x"])
