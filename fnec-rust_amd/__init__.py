@@ -71,7 +71,7 @@ class Request(C.Structure):
                 ("j_a", C.c_void_p), ("j_b", C.c_void_p), ("j_sign", C.c_void_p), ("n_junc", C.c_size_t),
                 ("freq_hz", C.c_double), ("ground", Ground), ("feed_index", C.c_size_t),
                 ("v_feed", C64), ("rp_theta", C.c_void_p), ("rp_phi", C.c_void_p),
-                ("rp_npts", C.c_size_t), ("tag", C.c_uint64)]
+                ("rp_npts", C.c_size_t), ("tag", C.c_uint64), ("x_out", C.c_void_p), ("rp_out", C.c_void_p)]
 
 
 class Result(C.Structure):
@@ -410,9 +410,11 @@ def profile(enable: bool | None = None, reset: bool = False, only: str | None = 
     return out
 
 
-def make_request(p: dict, tag: int = 0, rp_theta=None, rp_phi=None) -> tuple:
+def make_request(p: dict, tag: int = 0, rp_theta=None, rp_phi=None, outputs: bool = False) -> tuple:
     """fnec_request for a Deck.problem dict (worker route); returns
-    (request, keepalive) — keep the second item alive while the request is used."""
+    (request, keepalive) — keep the second item alive while the request is used.
+    With outputs, the request carries buffers for the currents (keepalive "x")
+    and, with a grid, the RP rows (keepalive "rows") that a session fills."""
     keep = {k: np.ascontiguousarray(p[k]) for k in ("segs", "rhs", "cos", "wf", "wl", "ja", "jb", "js")}
     q = Request()
     q.segs = _p(keep["segs"])
@@ -437,6 +439,12 @@ def make_request(p: dict, tag: int = 0, rp_theta=None, rp_phi=None) -> tuple:
         q.rp_phi = _p(keep["ph"])
         q.rp_npts = len(keep["th"])
     q.tag = tag
+    if outputs:
+        keep["x"] = np.zeros(len(keep["segs"]) + max(len(keep["wf"]), 1), np.complex128)
+        q.x_out = _p(keep["x"])
+        if rp_theta is not None:
+            keep["rows"] = np.zeros((len(keep["th"]), 4))
+            q.rp_out = _p(keep["rows"])
     return q, keep
 
 

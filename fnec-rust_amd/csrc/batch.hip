@@ -105,6 +105,10 @@ int run_batch(const fnec_request *const *reqs, size_t count, int precision, fnec
             const fnec_request &r = *reqs[i];
             results[i].tag = r.tag;
             results[i].status = FNEC_OK;
+            if (!results[i].x)
+                results[i].x = r.x_out;
+            if (!results[i].rp)
+                results[i].rp = r.rp_out;
             pre[i] = problem_from_hallen(hps[i], r.segs, r.n, r.rhs, r.cos_vec, r.wire_first, r.wire_last,
                                          r.n_wires, r.j_a, r.j_b, r.j_sign, r.n_junc, r.freq_hz, &r.ground);
             if (!pre[i] && r.feed_index >= r.n) {

@@ -167,6 +167,12 @@ typedef struct {
     const double *rp_phi;
     size_t rp_npts;
     uint64_t tag;            /* caller's request id, echoed in the result */
+    /* Optional output buffers (NULL: none): the currents and constants
+     * (n + max(n_wires, 1)) and the RP rows (rp_npts). A session writes them
+     * before the request's result is polled, so they must stay valid until
+     * then; a batch call uses them where the result's x / rp are NULL. */
+    fnec_c64 *x_out;
+    fnec_rp_row *rp_out;
 } fnec_request;
 
 typedef struct {

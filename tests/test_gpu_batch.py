@@ -199,6 +199,37 @@ def test_session_overlaps_batches_and_results_match_alone(precision):
     assert light_start < heavy_end
 
 
+def test_session_returns_currents_and_pattern_rows():
+    """Requests that carry output buffers get the currents and the RP rows
+    written before their result is polled, bitwise those of a batch call."""
+    probs = problems(28, [51, 400, 97, 700, 64])
+    grids = [(TH2, PH2) if i % 2 == 0 else None for i in range(len(probs))]
+    batch = F.solve_batch(probs, precision=1, rp_grids=grids)
+    L = F.lib()
+    sess = L.fnec_hip_session_create(1 << 40, 1, 256)
+    assert sess
+    try:
+        reqs = [F.make_request(p, tag=i, rp_theta=g[0] if g else None, rp_phi=g[1] if g else None, outputs=True)
+                for i, (p, g) in enumerate(zip(probs, grids))]
+        for q, _ in reqs:
+            assert L.fnec_hip_session_submit(sess, C.byref(q)) == 0
+        out = (F.Result * 16)()
+        got = {}
+        deadline = time.monotonic() + 60.0
+        while len(got) < len(probs) and time.monotonic() < deadline:
+            k = L.fnec_hip_session_poll(sess, C.cast(out, C.c_void_p), 16, 200000)
+            for j in range(k):
+                got[out[j].tag] = out[j].status
+        assert len(got) == len(probs)
+    finally:
+        L.fnec_hip_session_destroy(sess)
+    for i, b in enumerate(batch):
+        assert got[i] == F.OK
+        assert np.array_equal(reqs[i][1]["x"], b["x"])
+        if grids[i] is not None:
+            assert np.array_equal(reqs[i][1]["rows"], b["rp"])
+
+
 def test_session_rejects_request_over_budget():
     probs = problems(24, [400])
     st, _ = run_session(probs, 1, 1 << 20, 8)
