@@ -261,8 +261,9 @@ def measured_traffic(kernel, precision):
     e = d.get("kernels", {}).get(kernel)
     if not e or e.get("precision") != ("mixed" if precision == F.PREC_MIXED else "fp64"):
         return None
-    if kernel == "lu_update" and e.get("symbol") != f"k_lu_update_f32<{F.get_tunable('update_tile_m')}, 64>":
-        return None  # measured with the other row tile
+    tm = F.get_tunable("update_tile_m")
+    if kernel == "lu_update" and e.get("symbol") != f"k_lu_update_f32<{tm}, 64, {'true' if tm == 64 else 'false'}>":
+        return None  # measured with the other tile and product form
     return {"bytes_per_launch": e["hbm_bytes_per_launch"], "source": "profiles/serving_stream_traffic.json"}
 
 

@@ -753,7 +753,12 @@ constexpr int kTile32N = 64;   // columns per FP32 update tile
 
 constexpr int kXcds = 8;
 
-template <int TM, int TN>
+// kGauss: three real products per complex multiply-add (Gauss):
+// p1 = Σ ar·br, p2 = Σ ai·bi, p3 = Σ (ar + ai)(br + bi), then
+// C -= (p1 - p2) + i (p3 - p1 - p2). Three MFMA chains instead of four; the
+// FP32 rounding differs from the four-product form at the 1e-6 relative level,
+// which the factor (a CG preconditioner) absorbs.
+template <int TM, int TN, bool kGauss = false>
 __global__ __launch_bounds__(256) void k_lu_update_f32(const LuDesc<float> *__restrict__ descs,
                                                        const int4 *__restrict__ work, int cnt, int kbeg, int nk,
                                                        int rbeg, int cbeg, int cend)
@@ -855,13 +860,17 @@ __global__ __launch_bounds__(256) void k_lu_update_f32(const LuDesc<float> *__re
                 ci[a][b] = f4{y.x, y.y, y.z, y.w};
             }
         }
-    typename MF::acc accr[AM][AN], acci[AM][AN];
+    // four-product form: accr = -Re, acci = -Im of the product sum; Gauss
+    // form: accr = p1, acci = p2, accs = p3
+    typename MF::acc accr[AM][AN], acci[AM][AN], accs[kGauss ? AM : 1][kGauss ? AN : 1];
 #pragma unroll
     for (int a = 0; a < AM; ++a)
 #pragma unroll
         for (int b = 0; b < AN; ++b) {
             accr[a][b] = MF::zero();
             acci[a][b] = MF::zero();
+            if constexpr (kGauss)
+                accs[a][b] = MF::zero();
         }
     store_ab();
     __syncthreads();
@@ -881,16 +890,34 @@ __global__ __launch_bounds__(256) void k_lu_update_f32(const LuDesc<float> *__re
                 yr[b] = sb_r[kk + lk][wc * WN + b * 16 + lrow];
                 yi[b] = sb_i[kk + lk][wc * WN + b * 16 + lrow];
             }
+            if constexpr (kGauss) {
+                float xs[AM], ys[AN];
 #pragma unroll
-            for (int a = 0; a < AM; ++a)
+                for (int a = 0; a < AM; ++a)
+                    xs[a] = xr[a] + xi[a];
 #pragma unroll
-                for (int b = 0; b < AN; ++b) {
-                    // -(xr + i xi)(yr + i yi)
-                    accr[a][b] = MF::fma(-xr[a], yr[b], accr[a][b]);
-                    accr[a][b] = MF::fma(xi[a], yi[b], accr[a][b]);
-                    acci[a][b] = MF::fma(-xr[a], yi[b], acci[a][b]);
-                    acci[a][b] = MF::fma(-xi[a], yr[b], acci[a][b]);
-                }
+                for (int b = 0; b < AN; ++b)
+                    ys[b] = yr[b] + yi[b];
+#pragma unroll
+                for (int a = 0; a < AM; ++a)
+#pragma unroll
+                    for (int b = 0; b < AN; ++b) {
+                        accr[a][b] = MF::fma(xr[a], yr[b], accr[a][b]);
+                        acci[a][b] = MF::fma(xi[a], yi[b], acci[a][b]);
+                        accs[a][b] = MF::fma(xs[a], ys[b], accs[a][b]);
+                    }
+            } else {
+#pragma unroll
+                for (int a = 0; a < AM; ++a)
+#pragma unroll
+                    for (int b = 0; b < AN; ++b) {
+                        // -(xr + i xi)(yr + i yi)
+                        accr[a][b] = MF::fma(-xr[a], yr[b], accr[a][b]);
+                        accr[a][b] = MF::fma(xi[a], yi[b], accr[a][b]);
+                        acci[a][b] = MF::fma(-xr[a], yi[b], acci[a][b]);
+                        acci[a][b] = MF::fma(-xi[a], yr[b], acci[a][b]);
+                    }
+            }
         }
         if (chunk + 1 < nk) {
             __syncthreads(); // this chunk's operands consumed
@@ -910,8 +937,13 @@ __global__ __launch_bounds__(256) void k_lu_update_f32(const LuDesc<float> *__re
             f4 vr, vi;
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
-                vr[q] = cr[a][b][q] + accr[a][b][q];
-                vi[q] = ci[a][b][q] + acci[a][b][q];
+                if constexpr (kGauss) {
+                    vr[q] = cr[a][b][q] - accr[a][b][q] + acci[a][b][q];
+                    vi[q] = ci[a][b][q] - accs[a][b][q] + accr[a][b][q] + acci[a][b][q];
+                } else {
+                    vr[q] = cr[a][b][q] + accr[a][b][q];
+                    vi[q] = ci[a][b][q] + acci[a][b][q];
+                }
             }
             // rows past m (at most 3, inside the padded column) keep 0:
             // their C and L21 entries are 0
@@ -1085,13 +1117,15 @@ __global__ __launch_bounds__(256) void k_lu_trsm64_f32(const LuDesc<float> *__re
         }
 }
 
-// tm: row tile, 128 or 64 (the work list's row-tile count follows it).
+// tm: row tile (the work list's row-tile count follows it): 64 with the
+// three-product (Gauss) form, or 128 with the four-product form, each the
+// faster form at its tile size.
 inline void lu_launch_update_f32(const LuDesc<float> *d, size_t off, size_t cnt, const int *ints,
                                  int kbeg, int nk, int rbeg, int cbeg, int cend, int tm, hipStream_t st)
 {
     const unsigned grid = (unsigned)((cnt + kXcds - 1) / kXcds * kXcds);
     if (tm == 64)
-        hipLaunchKernelGGL((k_lu_update_f32<64, kTile32N>), dim3(grid), dim3(256), 0, st, d,
+        hipLaunchKernelGGL((k_lu_update_f32<64, kTile32N, true>), dim3(grid), dim3(256), 0, st, d,
                            (const int4 *)(ints + off), (int)cnt, kbeg, nk, rbeg, cbeg, cend);
     else
         hipLaunchKernelGGL((k_lu_update_f32<kTile32, kTile32N>), dim3(grid), dim3(256), 0, st, d,

@@ -250,8 +250,9 @@ int fnec_hip_debug_mfma_probe_f32(const float *a16x4, const float *b4x16, float 
  *   "mix_first_chunk" CG iterations launched before the first convergence
  *                     read-back. 1 .. 12.
  *   "update_tile_m"   row tile of the mixed path's FP32 trailing-matrix
- *                     update (k_lu_update_f32 work items): 64 (default) or
- *                     128. Both accumulate every element in the same order.
+ *                     update (k_lu_update_f32 work items): 64 (default; three
+ *                     real MFMA products per complex multiply-add) or 128
+ *                     (four). The factors differ at FP32 rounding level.
  *   "lanes"           batches that may run on the device at once (each on a
  *                     lane: two streams, a device arena, pinned staging);
  *                     also the worker count of a session created afterwards.
