@@ -2230,7 +2230,25 @@ int rp_host_from(RpHost &h, const fnec_seg *segs, const fnec_c64 *cur, size_t n,
             o[3 + c] = segs[i].dir[c];
         }
         o[6] = segs[i].length;
-        o[7] = segs[i].radius;
+        // slot 7 (the far field has no use for the radius): 1 when the
+        // segment continues a uniform run, i.e. has the previous segment's
+        // length and direction and its midpoint is the previous midpoint plus
+        // length * dir to within rounding (a straight wire's equal segments);
+        // the far-field sum then advances the phase factor by a rotation
+        double cont = 0.0;
+        if (i > 0) {
+            const fnec_seg &a = segs[i - 1], &b = segs[i];
+            bool same = a.length == b.length;
+            double scale = b.length, dev = 0.0;
+            for (int c = 0; c < 3 && same; ++c) {
+                same = a.dir[c] == b.dir[c];
+                scale += std::fabs(a.mid[c]) + std::fabs(b.mid[c]);
+                dev = std::max(dev, std::fabs(b.mid[c] - a.mid[c] - b.length * b.dir[c]));
+            }
+            if (same && dev <= 64.0 * 2.220446049250313e-16 * scale)
+                cont = 1.0;
+        }
+        o[7] = cont;
     }
     if (cur) {
         h.cur.resize(2 * n);

@@ -624,7 +624,7 @@ __device__ void fresnel(double th_deg, double eps_r, double sigma, double k, dou
 }
 
 // Segments staged in LDS as {mid x,y,z, dir x,y,z, length, I re, I im}.
-constexpr int kRpStage = 9;
+constexpr int kRpStage = 10; // mid xyz, dir xyz, length, run flag, current re, im
 constexpr int kRpChunk = 256;
 
 struct Dir {
@@ -646,17 +646,34 @@ __device__ __forceinline__ Dir unit_vectors(double th_deg, double ph_deg)
 
 // Accumulate one LDS chunk of segments into (F_theta, F_phi) for direction u,
 // in segment index order (farfield.rs:277-288, :526-543, :368-385).
+// Far-field sum over one staged chunk of segments. The phase factor
+// e^{j k u.mid} of a segment that continues a uniform run (flag in slot 7:
+// midpoint = previous midpoint + length * dir) is the previous one times the
+// run's rotation e^{j k length u.dir}; it is evaluated directly at the start
+// of a run, at the start of the chunk and every kRpAnchor segments, so a
+// rotated factor is at most kRpAnchor - 1 products from a direct one.
+constexpr int kRpAnchor = 16;
+
 __device__ __forceinline__ void ff_chunk(const double *__restrict__ sg, int cnt, double k,
                                          int ground, const Dir &u, double gvr, double gvi,
                                          double ghr, double ghi, double &ftr, double &fti,
                                          double &fpr, double &fpi)
 {
+    double c = 1.0, s = 0.0, rc = 1.0, rs = 0.0;     // phase factor, rotation
+    double c2 = 1.0, s2 = 0.0, rc2 = 1.0, rs2 = 0.0; // the same for the image
     for (int m = 0; m < cnt; ++m) {
         const double *g = sg + m * kRpStage;
         const double mx = g[0], my = g[1], mz = g[2], dx = g[3], dy = g[4], dz = g[5];
-        const double len = g[6], ir = g[7], ii = g[8];
-        double s, c;
-        sincos_phase(k * (mx * u.rx + my * u.ry + mz * u.rz), s, c);
+        const double len = g[6], ir = g[8], ii = g[9];
+        const bool rotate = g[7] != 0.0 && (m % kRpAnchor) != 0;
+        if (rotate) {
+            const double nc = c * rc - s * rs, ns = s * rc + c * rs;
+            c = nc;
+            s = ns;
+        } else {
+            sincos_phase(k * (mx * u.rx + my * u.ry + mz * u.rz), s, c);
+            sincos_phase(k * len * (dx * u.rx + dy * u.ry + dz * u.rz), rs, rc);
+        }
         const double pr = len * c, pi = len * s;
         const double wr = ir * pr - ii * pi, wi = ir * pi + ii * pr;
         const double qt = dx * u.tx + dy * u.ty + dz * u.tz;
@@ -667,8 +684,14 @@ __device__ __forceinline__ void ff_chunk(const double *__restrict__ sg, int cnt,
         fpi += wi * qp;
         if (ground != 0) {
             // image: position (x, y, -z), direction (-dx, -dy, dz) (farfield.rs:64-82)
-            double s2, c2;
-            sincos_phase(k * (mx * u.rx + my * u.ry - mz * u.rz), s2, c2);
+            if (rotate) {
+                const double nc = c2 * rc2 - s2 * rs2, ns = s2 * rc2 + c2 * rs2;
+                c2 = nc;
+                s2 = ns;
+            } else {
+                sincos_phase(k * (mx * u.rx + my * u.ry - mz * u.rz), s2, c2);
+                sincos_phase(k * len * (dx * u.rx + dy * u.ry - dz * u.rz), rs2, rc2);
+            }
             const double p2r = len * c2, p2i = len * s2;
             const double vr = ir * p2r - ii * p2i, vi = ir * p2i + ii * p2r;
             const double it = -dx * u.tx - dy * u.ty + dz * u.tz;
@@ -696,10 +719,10 @@ __device__ __forceinline__ void stage_chunk(const RpProb &R, double *sg, int bas
         const int m = e / kRpStage, c = e % kRpStage;
         const int gm = base + m;
         double v;
-        if (c < 7)
+        if (c < 8)
             v = R.seg[(size_t)gm * kSegStride + c];
         else
-            v = R.cur[2 * (size_t)gm + (c - 7)];
+            v = R.cur[2 * (size_t)gm + (c - 8)];
         sg[e] = v;
     }
 }
