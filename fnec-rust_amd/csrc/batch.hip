@@ -350,6 +350,7 @@ void session_worker(fnec_session *s)
         if (!msg.empty())
             s->err = msg;
         for (size_t i = 0; i < batch.size(); ++i) {
+            res[i].tag = batch[i]->req.tag; // also when the batch never ran
             if (st)
                 res[i].status = st;
             res[i].t_submit_us = batch[i]->t_submit;

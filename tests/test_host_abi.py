@@ -166,3 +166,57 @@ def test_request_footprint_monotone():
     m = F.lib().fnec_hip_request_footprint(2000, 4, 0, 0, 1)
     p = F.lib().fnec_hip_request_footprint(2000, 4, 0, 0, 0)
     assert 24 * 2000 * 2000 < m < p
+
+
+def test_session_machinery_returns_every_request_once():
+    """The session's worker threads, admission and polling, host side: every
+    submitted request comes back exactly once, with its tag, one status for
+    its whole batch, and ordered timestamps; batches respect max_batch; a
+    request over the budget is rejected at submit; destroy returns with
+    requests still queued. Without a GPU every batch ends UNAVAILABLE (the
+    caller's CPU-fallback signal); with one, OK."""
+    import time
+
+    L = F.lib()
+    rng = np.random.default_rng(3)
+    probs = []
+    for n in (51, 80, 64, 120, 51, 97, 70):
+        d = F.Deck(W.random_request_deck(rng, n, 14.0))
+        probs.append(d.problem("worker", d.frequencies("worker")[0]))
+    sess = L.fnec_hip_session_create(1 << 40, F.PREC_MIXED, 3)
+    assert sess
+    try:
+        keep = [F.make_request(p, tag=100 + i) for i, p in enumerate(probs)]
+        for q, _ in keep:
+            assert L.fnec_hip_session_submit(sess, C.byref(q)) == 0
+        out = (F.Result * 16)()
+        got = {}
+        deadline = time.monotonic() + 60.0
+        while len(got) < len(probs) and time.monotonic() < deadline:
+            k = L.fnec_hip_session_poll(sess, C.cast(out, C.c_void_p), 16, 100000)
+            assert k >= 0
+            for j in range(k):
+                r = out[j]
+                assert r.tag not in got
+                got[r.tag] = (r.status, r.batch, r.t_submit_us, r.t_start_us, r.t_done_us)
+        assert sorted(got) == [100 + i for i in range(len(probs))]
+        assert L.fnec_hip_session_pending(sess) == 0
+        statuses = {v[0] for v in got.values()}
+        assert len(statuses) == 1 and statuses <= {F.OK, F.UNAVAILABLE}
+        per_batch = {}
+        for v in got.values():
+            per_batch[v[1]] = per_batch.get(v[1], 0) + 1
+            assert v[2] <= v[3] <= v[4]
+        assert max(per_batch.values()) <= 3
+        # over the budget: rejected at submit, nothing queued
+        small = L.fnec_hip_session_create(1 << 10, F.PREC_MIXED, 8)
+        try:
+            assert L.fnec_hip_session_submit(small, C.byref(keep[0][0])) == F.RESOURCE_EXHAUSTED
+            assert L.fnec_hip_session_pending(small) == 0
+        finally:
+            L.fnec_hip_session_destroy(small)
+        # destroy with requests queued or running returns
+        for q, _ in keep:
+            assert L.fnec_hip_session_submit(sess, C.byref(q)) == 0
+    finally:
+        L.fnec_hip_session_destroy(sess)
