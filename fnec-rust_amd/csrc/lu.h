@@ -1073,33 +1073,36 @@ __global__ __launch_bounds__(256) void k_lu_trsm64_f32(const LuDesc<float> *__re
     __syncthreads();
     const int lane = t & 63, wave = t >> 6, wr = wave >> 1, wc = wave & 1;
     const int lrow = lane & 15, lk = lane >> 4;
-    typename MF::acc accr[2][2], acci[2][2];
+    // three-product (Gauss) form, as in k_lu_update_f32: p1 = Σ xr yr,
+    // p2 = Σ xi yi, p3 = Σ (xr + xi)(yr + yi); product (p1 - p2) + i (p3 - p1 - p2)
+    typename MF::acc p1[2][2], p2[2][2], p3[2][2];
 #pragma unroll
     for (int a = 0; a < 2; ++a)
 #pragma unroll
         for (int b = 0; b < 2; ++b) {
-            accr[a][b] = MF::zero();
-            acci[a][b] = MF::zero();
+            p1[a][b] = MF::zero();
+            p2[a][b] = MF::zero();
+            p3[a][b] = MF::zero();
         }
 #pragma unroll 4
     for (int kk = 0; kk < NB; kk += 4) {
-        float xr[2], xi[2], yr[2], yi[2];
+        float xr[2], xi[2], xs[2], yr[2], yi[2], ys[2];
 #pragma unroll
         for (int a = 0; a < 2; ++a) {
             xr[a] = sa_r[kk + lk][wr * 32 + a * 16 + lrow];
             xi[a] = sa_i[kk + lk][wr * 32 + a * 16 + lrow];
             yr[a] = sb_r[kk + lk][wc * 32 + a * 16 + lrow];
             yi[a] = sb_i[kk + lk][wc * 32 + a * 16 + lrow];
+            xs[a] = xr[a] + xi[a];
+            ys[a] = yr[a] + yi[a];
         }
 #pragma unroll
         for (int a = 0; a < 2; ++a)
 #pragma unroll
             for (int b = 0; b < 2; ++b) {
-                // (xr + i xi)(yr + i yi)
-                accr[a][b] = MF::fma(xr[a], yr[b], accr[a][b]);
-                accr[a][b] = MF::fma(-xi[a], yi[b], accr[a][b]);
-                acci[a][b] = MF::fma(xr[a], yi[b], acci[a][b]);
-                acci[a][b] = MF::fma(xi[a], yr[b], acci[a][b]);
+                p1[a][b] = MF::fma(xr[a], yr[b], p1[a][b]);
+                p2[a][b] = MF::fma(xi[a], yi[b], p2[a][b]);
+                p3[a][b] = MF::fma(xs[a], ys[b], p3[a][b]);
             }
     }
     // the whole block was staged before any write: the product goes back in place
@@ -1112,8 +1115,14 @@ __global__ __launch_bounds__(256) void k_lu_trsm64_f32(const LuDesc<float> *__re
             if (c >= nc)
                 continue;
             const size_t o = (size_t)(c0 + c) * lda + P + r;
-            *(float4 *)(D.ar + o) = make_float4(accr[a][b][0], accr[a][b][1], accr[a][b][2], accr[a][b][3]);
-            *(float4 *)(D.ai + o) = make_float4(acci[a][b][0], acci[a][b][1], acci[a][b][2], acci[a][b][3]);
+            f4 vr, vi;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                vr[q] = p1[a][b][q] - p2[a][b][q];
+                vi[q] = p3[a][b][q] - p1[a][b][q] - p2[a][b][q];
+            }
+            *(float4 *)(D.ar + o) = make_float4(vr[0], vr[1], vr[2], vr[3]);
+            *(float4 *)(D.ai + o) = make_float4(vi[0], vi[1], vi[2], vi[3]);
         }
 }
 
