@@ -155,6 +155,60 @@ def serve(items, window, steps, warmup, prof_windows, precision, budget, max_bat
     return elapsed, np.array(lat), results, dom, prof_all, prof_timed, batches
 
 
+def serve_poisson(items, times, warmup_items, precision, budget, max_batch, prof_class):
+    """Open loop: request i is submitted at its arrival time times[i] (s from
+    the start), whatever is outstanding; latency runs from that arrival time
+    to the poll that returns the result. ``warmup_items`` requests are served
+    first (closed loop) and not timed. Events are recorded around
+    ``prof_class`` only."""
+    L = F.lib()
+    sess = L.fnec_hip_session_create(budget, precision, max_batch)
+    if not sess:
+        raise F.FnecError(F.INVALID_ARG, "session create failed")
+    out = (F.Result * 4096)()
+    results = {}
+
+    def drain(k_expect, t_arr, lat, bt, wait_us):
+        k = L.fnec_hip_session_poll(sess, C.cast(out, C.c_void_p), 4096, wait_us)
+        if k < 0:
+            F._check(-k)
+        now = time.perf_counter()
+        for i in range(k):
+            r = out[i]
+            if r.status:
+                raise F.FnecError(r.status, f"request {r.tag} failed")
+            results[r.tag] = complex(r.z_in.re, r.z_in.im)
+            if t_arr is not None:
+                lat.append(now - t_arr[r.tag])
+                b = bt.setdefault(r.batch, [r.t_start_us, r.t_done_us, 0])
+                b[2] += 1
+        return k
+
+    left = 0
+    for it in warmup_items:
+        F._check(L.fnec_hip_session_submit(sess, C.byref(it["req"])))
+        left += 1
+    while left:
+        left -= drain(left, None, None, None, 2000)
+    F.profile(enable=True, reset=True, only=prof_class)
+    lat, bt, t_arr = [], {}, {}
+    nxt = done = 0
+    t0 = time.perf_counter()
+    while done < len(items):
+        now = time.perf_counter() - t0
+        while nxt < len(items) and times[nxt] <= now:
+            it = items[nxt]
+            t_arr[it["req"].tag] = t0 + times[nxt]
+            F._check(L.fnec_hip_session_submit(sess, C.byref(it["req"])))
+            nxt += 1
+        wait = 2000 if nxt >= len(items) else max(0, min(2000, int((times[nxt] - now) * 1e6)))
+        done += drain(None, t_arr, lat, bt, wait)
+    elapsed = time.perf_counter() - t0
+    prof = F.profile(enable=False, only="")
+    L.fnec_hip_session_destroy(sess)
+    return elapsed, np.array(lat), results, prof, bt
+
+
 def batch_overlap(batches):
     """Batches formed in the timed region, their mean size, and the share of
     the span from the first batch start to the last batch end during which
@@ -293,6 +347,10 @@ def main():
     ap.add_argument("--plain-sample", type=int, default=256)
     ap.add_argument("--max-batch", type=int, default=256)
     ap.add_argument("--lanes", type=int, default=2, help="batches on the device at once")
+    ap.add_argument("--arrival", choices=["windows", "poisson"], default="windows",
+                    help="windows: closed loop at saturation (the throughput figure); poisson: open loop, "
+                         "each request submitted at its arrival time (latency at a given load)")
+    ap.add_argument("--rate", type=float, default=2000.0, help="requests/s of the poisson arrival mode")
     ap.add_argument("--profile-class", default=None,
                     help="kernel class timed in the timed windows (default: the dominant one)")
     ap.add_argument("--no-autotune", action="store_true",
@@ -332,11 +390,27 @@ def main():
                  "start_window_ms": round(at["start_seconds"] * 1e3, 2), "trials": len(at["trials"]),
                  "tuning_s": round(time.perf_counter() - t_at, 2)}
         print(f"[bench] autotune: {tuned}", file=sys.stderr, flush=True)
-    elapsed, lat, zs_served, dom, prof_all, prof, batches = serve(
-        items, args.window, args.steps, args.warmup, args.profile_windows, precision,
-        int(args.budget_gib * 2 ** 30), args.max_batch, depth, args.profile_class)
-    first = (args.warmup + args.profile_windows) * args.window
-    timed = args.steps * args.window
+    if args.arrival == "poisson":
+        # open loop: the stream's Poisson arrival times, rescaled to the rate
+        first = args.warmup * args.window
+        timed_items = items[first:first + args.steps * args.window]
+        t_first = stream[first]["t_arrival"]
+        scale = 1000.0 / args.rate  # the stream is generated at 1000 requests/s
+        times = [(stream[first + i]["t_arrival"] - t_first) * scale for i in range(len(timed_items))]
+        dom = args.profile_class or "lu_update"
+        elapsed, lat, zs_served, prof, batches = serve_poisson(
+            timed_items, times, items[:first], precision, int(args.budget_gib * 2 ** 30), args.max_batch, dom)
+        prof_all = prof
+        timed = len(timed_items)
+        arrival = f"Poisson at {args.rate:g} requests/s, open loop (each request submitted at its arrival time)"
+    else:
+        elapsed, lat, zs_served, dom, prof_all, prof, batches = serve(
+            items, args.window, args.steps, args.warmup, args.profile_windows, precision,
+            int(args.budget_gib * 2 ** 30), args.max_batch, depth, args.profile_class)
+        first = (args.warmup + args.profile_windows) * args.window
+        timed = args.steps * args.window
+        arrival = (f"windows of {args.window} requests; a window is submitted whenever fewer than {depth} "
+                   f"are outstanding")
     value = timed / elapsed
     print(f"[bench] served {timed} requests in {elapsed:.2f}s", file=sys.stderr, flush=True)
 
@@ -378,7 +452,8 @@ def main():
         "avg_launch_us": round(avg_us, 2),
         "launches": int(launches),
         "algorithmic_bytes_per_launch": round(nbytes / launches) if nbytes else None,
-        "kernels_source": f"{args.profile_windows} untimed windows of {args.window} with events on every class",
+        "kernels_source": (f"{args.profile_windows} untimed windows of {args.window} with events on every class"
+                           if args.arrival == "windows" else f"timed region, events on {dom} only"),
         "kernels": {k: {"launches": int(v[0]), "total_ms": round(v[1] / 1e3, 3),
                         "tflops": round(v[2] / (v[1] * 1e-6) / 1e12, 3) if v[1] and v[2] else None,
                         "gbs": round(v[3] / (v[1] * 1e-6) / 1e9, 1) if v[1] and v[3] else None,
@@ -405,7 +480,8 @@ def main():
         "data": "synthetic",
         "config": {
             "workload": "mixed-N serving stream (N log-U[51,4096], 30% 2x2 deg RP, 10% PEC), "
-                        f"{timed} timed requests in windows of {args.window}",
+                        f"{timed} timed requests, " + (f"Poisson arrivals at {args.rate:g}/s" if args.arrival == "poisson"
+                                                       else f"in windows of {args.window}"),
             "precision": args.precision,
             "max_batch": args.max_batch,
             "tunables": tuned["knobs"] if tuned else {k: F.get_tunable(k) for k in AT.SPACE},
@@ -413,8 +489,7 @@ def main():
         "autotune": tuned,
         "latency_ms": {"p50": round(float(np.percentile(lat, 50)) * 1e3, 3),
                        "p99": round(float(np.percentile(lat, 99)) * 1e3, 3),
-                       "arrival": f"windows of {args.window} requests; a window is submitted "
-                                  f"whenever fewer than {depth} are outstanding"},
+                       "arrival": arrival},
         "serving": dict(lanes=args.lanes, **(batch_overlap(batches) or {})),
         "plain_fp64_gpu": {"value": round(plain_rate, 3), "unit": "solves/s",
                            "sample": len(sample_items)},
