@@ -351,6 +351,8 @@ def main():
                     help="windows: closed loop at saturation (the throughput figure); poisson: open loop, "
                          "each request submitted at its arrival time (latency at a given load)")
     ap.add_argument("--rate", type=float, default=2000.0, help="requests/s of the poisson arrival mode")
+    ap.add_argument("--batch-split-n", type=int, default=None,
+                    help="serving batches take one N class, split at this size (0: one class)")
     ap.add_argument("--profile-class", default=None,
                     help="kernel class timed in the timed windows (default: the dominant one)")
     ap.add_argument("--no-autotune", action="store_true",
@@ -372,6 +374,8 @@ def main():
         sys.exit(2)
     F.init()
     F.set_tunable("lanes", args.lanes)
+    if args.batch_split_n is not None:
+        F.set_tunable("batch_split_n", args.batch_split_n)
     depth = args.depth or args.lanes
     precision = F.PREC_MIXED if args.precision == "mixed" else F.PREC_FP64
     total = (args.warmup + args.profile_windows + args.steps) * args.window
@@ -490,7 +494,8 @@ def main():
         "latency_ms": {"p50": round(float(np.percentile(lat, 50)) * 1e3, 3),
                        "p99": round(float(np.percentile(lat, 99)) * 1e3, 3),
                        "arrival": arrival},
-        "serving": dict(lanes=args.lanes, **(batch_overlap(batches) or {})),
+        "serving": dict(lanes=args.lanes, batch_split_n=F.get_tunable("batch_split_n"),
+                        **(batch_overlap(batches) or {})),
         "plain_fp64_gpu": {"value": round(plain_rate, 3), "unit": "solves/s",
                            "sample": len(sample_items)},
         "speedup_vs_plain_fp64": round(value / plain_rate, 3),

@@ -312,13 +312,24 @@ void session_worker(fnec_session *s)
             return;
         std::vector<OwnedRequest *> batch;
         size_t bytes = 0;
-        while (!s->queue.empty() && (int)batch.size() < s->max_batch) {
-            OwnedRequest *o = s->queue.front();
+        // size classes (tunable batch_split_n): while the queue holds less
+        // than a full batch (latency regime), the batch takes only requests
+        // of the oldest request's class, so that small solves do not wait for
+        // a large one's factorisation (FIFO within the class); with a full
+        // batch queued (throughput regime) it is formed FIFO across classes
+        const long split = (int)s->queue.size() < s->max_batch ? get_tunable("batch_split_n") : 0;
+        const bool big = split > 0 && s->queue.front()->req.n > (size_t)split;
+        for (auto it = s->queue.begin(); it != s->queue.end() && (int)batch.size() < s->max_batch;) {
+            OwnedRequest *o = *it;
+            if (split > 0 && (o->req.n > (size_t)split) != big) {
+                ++it;
+                continue;
+            }
             if (s->in_flight_bytes + bytes + o->footprint > s->budget)
                 break;
             bytes += o->footprint;
             batch.push_back(o);
-            s->queue.pop_front();
+            it = s->queue.erase(it);
         }
         if (batch.empty())
             continue; // another worker took the requests, or they do not fit yet

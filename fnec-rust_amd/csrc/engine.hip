@@ -746,6 +746,7 @@ static int g_mix_first_chunk = kMixFirstChunk;
 // so a problem's solve time shrinks towards the chain latency of its blocks
 // instead of one workgroup streaming the whole factor.
 static int g_update_tile_m = 64; // row tile of the FP32 trailing update (64 or 128)
+static int g_batch_split_n = 1024; // serving batches: N classes split at this size (0: one class)
 static int g_gang_n = 512;
 static int g_gang_max = 4;
 static int gang_size(int n)
@@ -2334,6 +2335,10 @@ int set_tunable(const char *name, long value)
         g_gang_max = (int)value;
         return FNEC_OK;
     }
+    if (k == "batch_split_n" && value >= 0 && value <= 1 << 16) {
+        g_batch_split_n = (int)value;
+        return FNEC_OK;
+    }
     if (k == "update_tile_m" && (value == 64 || value == 128)) {
         g_update_tile_m = (int)value;
         return FNEC_OK;
@@ -2363,6 +2368,8 @@ long get_tunable(const char *name)
         return g_lanes_on;
     if (k == "update_tile_m")
         return g_update_tile_m;
+    if (k == "batch_split_n")
+        return g_batch_split_n;
     return -1;
 }
 

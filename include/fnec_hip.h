@@ -199,8 +199,9 @@ int fnec_hip_solve_batch(const fnec_request *reqs, size_t count, int precision, 
  * 2) take queued requests as batches and solve them; each batch runs on a lane
  * of its own (streams, device arena), so a batch formed from requests that
  * arrived while another is on the device runs beside it and their kernels
- * overlap. Requests are admitted in arrival order while the device footprint
- * of all batches in flight stays within budget_bytes; a batch holds Z, the LU
+ * overlap. Requests are admitted in arrival order (within a size class, see
+ * the tunable "batch_split_n") while the device footprint of all batches in
+ * flight stays within budget_bytes; a batch holds Z, the LU
  * factors and the currents on the device until it completes. A batch closes
  * when no request has been submitted for 500 us, when it holds max_batch
  * requests, or when its oldest request has waited 20 ms. submit copies the
@@ -253,6 +254,13 @@ int fnec_hip_debug_mfma_probe_f32(const float *a16x4, const float *b4x16, float 
  *                     update (k_lu_update_f32 work items): 64 (default; three
  *                     real MFMA products per complex multiply-add) or 128
  *                     (four). The factors differ at FP32 rounding level.
+ *   "batch_split_n"   serving sessions: while the queue holds less than
+ *                     max_batch requests, a batch takes only requests of its
+ *                     oldest request's size class (N <= value or N > value),
+ *                     so that small solves do not wait for a large one's
+ *                     factorisation; with a full batch queued, batches are
+ *                     formed across classes. 0 .. 65536 (default 1024; 0: one
+ *                     class).
  *   "lanes"           batches that may run on the device at once (each on a
  *                     lane: two streams, a device arena, pinned staging);
  *                     also the worker count of a session created afterwards.
