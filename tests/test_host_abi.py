@@ -220,3 +220,40 @@ def test_session_machinery_returns_every_request_once():
             assert L.fnec_hip_session_submit(sess, C.byref(q)) == 0
     finally:
         L.fnec_hip_session_destroy(sess)
+
+
+def test_session_batches_keep_to_one_size_class():
+    """While less than max_batch requests are queued, a batch holds requests
+    of one size class only (tunable batch_split_n), FIFO within the class."""
+    import time
+
+    L = F.lib()
+    rng = np.random.default_rng(4)
+    sizes = (51, 140, 64, 160, 90, 130, 70)
+    probs = []
+    for n in sizes:
+        d = F.Deck(W.random_request_deck(rng, n, 14.0))
+        probs.append(d.problem("worker", d.frequencies("worker")[0]))
+    old = F.get_tunable("batch_split_n")
+    F.set_tunable("batch_split_n", 100)
+    sess = L.fnec_hip_session_create(1 << 40, F.PREC_MIXED, 64)
+    assert sess
+    try:
+        keep = [F.make_request(p, tag=i) for i, p in enumerate(probs)]
+        for q, _ in keep:
+            assert L.fnec_hip_session_submit(sess, C.byref(q)) == 0
+        out = (F.Result * 16)()
+        got = {}
+        deadline = time.monotonic() + 60.0
+        while len(got) < len(probs) and time.monotonic() < deadline:
+            k = L.fnec_hip_session_poll(sess, C.cast(out, C.c_void_p), 16, 100000)
+            for j in range(k):
+                got[out[j].tag] = out[j].batch
+        assert len(got) == len(probs)
+    finally:
+        L.fnec_hip_session_destroy(sess)
+        F.set_tunable("batch_split_n", old)
+    classes = {}
+    for tag, b in got.items():
+        classes.setdefault(b, set()).add(sizes[tag] > 100)
+    assert all(len(c) == 1 for c in classes.values())
