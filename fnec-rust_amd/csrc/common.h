@@ -17,6 +17,7 @@ constexpr double kSingular = 1e-30;                // linear.rs:1354
 constexpr int kTile = 64;      // Gram / trailing-update output tile (complex elements)
 constexpr int kGramK = 16;     // Gram K chunk
 constexpr int kPanel = 32;     // LU panel width
+constexpr int kBsThreads = 1024; // k_backsub: one workgroup per problem, many row-update loads in flight
 constexpr int kSegStride = 8;  // doubles per segment: mid xyz, dir xyz, length, radius
 
 // One Hallén problem inside a batch. All pointers are device pointers into

@@ -1817,7 +1817,7 @@ static int plan_launch(Plan &pl, char *base, bool with_solve)
     HIPCHK(hipEventRecord(LN().ev[3], st));
     {
         hipEvent_t e = prof_begin(PC_BACKSUB);
-        hipLaunchKernelGGL(k_backsub, dim3((unsigned)pl.all_cnt), dim3(256), 0, st, dp,
+        hipLaunchKernelGGL(k_backsub, dim3((unsigned)pl.all_cnt), dim3(kBsThreads), 0, st, dp,
                            wl_ptr(base, pl, pl.all_off));
         prof_end(PC_BACKSUB, e, pl.backsub_flops, 0);
     }

@@ -417,7 +417,7 @@ __global__ __launch_bounds__(256) void k_gram_fix(const Prob *__restrict__ probs
 // Blocks of 64 rows from the bottom; one workgroup per problem.
 constexpr int kBsBlock = 64;
 
-__global__ __launch_bounds__(256) void k_backsub(const Prob *__restrict__ probs,
+__global__ __launch_bounds__(kBsThreads) void k_backsub(const Prob *__restrict__ probs,
                                                  const int *__restrict__ plist)
 {
     const Prob &P = probs[plist[blockIdx.x]];
@@ -425,38 +425,53 @@ __global__ __launch_bounds__(256) void k_backsub(const Prob *__restrict__ probs,
         return;
     const int s = P.s, lda = P.lda, t = threadIdx.x;
     double *x = P.x;
-    for (int i = t; i < s; i += 256) {
+    for (int i = t; i < s; i += kBsThreads) {
         x[2 * i] = P.ar[(size_t)s * lda + i];
         x[2 * i + 1] = P.ai[(size_t)s * lda + i];
     }
     __syncthreads();
     __shared__ double xb_r[kBsBlock], xb_i[kBsBlock];
+    // the block's triangle, [column i - r0][row m - r0]
+    __shared__ double ut_r[kBsBlock][kBsBlock + 1], ut_i[kBsBlock][kBsBlock + 1];
     for (int r1 = s; r1 > 0; r1 -= kBsBlock) {
-        const int r0 = max(0, r1 - kBsBlock);
+        const int r0 = max(0, r1 - kBsBlock), nb = r1 - r0;
+        // stage the block's columns (rows r0 .. r1) and its part of x in LDS
+        for (int e = t; e < nb * nb; e += kBsThreads) {
+            const int c = e / nb, m = e % nb;
+            ut_r[c][m] = P.ar[(size_t)(r0 + c) * lda + r0 + m];
+            ut_i[c][m] = P.ai[(size_t)(r0 + c) * lda + r0 + m];
+        }
+        for (int m = t; m < nb; m += kBsThreads) {
+            xb_r[m] = x[2 * (r0 + m)];
+            xb_i[m] = x[2 * (r0 + m) + 1];
+        }
+        __syncthreads();
         // triangle [r0, r1): serial in i, parallel over the rows above it
-        for (int i = r1 - 1; i >= r0; --i) {
+        for (int i = nb - 1; i >= 0; --i) {
             if (t == 0) {
-                const double dr = P.ar[(size_t)i * lda + i], di = P.ai[(size_t)i * lda + i];
+                const double dr = ut_r[i][i], di = ut_i[i][i];
                 const double nsq = dr * dr + di * di;
-                const double br = x[2 * i], bi = x[2 * i + 1];
-                const double vr = (br * dr + bi * di) / nsq, vi = (bi * dr - br * di) / nsq;
-                x[2 * i] = vr;
-                x[2 * i + 1] = vi;
-                xb_r[i - r0] = vr;
-                xb_i[i - r0] = vi;
+                const double br = xb_r[i], bi = xb_i[i];
+                xb_r[i] = (br * dr + bi * di) / nsq;
+                xb_i[i] = (bi * dr - br * di) / nsq;
             }
             __syncthreads();
-            const double vr = xb_r[i - r0], vi = xb_i[i - r0];
-            for (int m = r0 + t; m < i; m += 256) {
-                const double ur = P.ar[(size_t)i * lda + m], ui = P.ai[(size_t)i * lda + m];
-                x[2 * m] -= ur * vr - ui * vi;
-                x[2 * m + 1] -= ur * vi + ui * vr;
+            const double vr = xb_r[i], vi = xb_i[i];
+            if (t < i) {
+                const double ur = ut_r[i][t], ui = ut_i[i][t];
+                xb_r[t] -= ur * vr - ui * vi;
+                xb_i[t] -= ur * vi + ui * vr;
             }
             __syncthreads();
         }
+        for (int m = t; m < nb; m += kBsThreads) {
+            x[2 * (r0 + m)] = xb_r[m];
+            x[2 * (r0 + m) + 1] = xb_i[m];
+        }
         // rows above the block
-        for (int m = t; m < r0; m += 256) {
+        for (int m = t; m < r0; m += kBsThreads) {
             double sr = 0.0, si = 0.0;
+#pragma unroll 16
             for (int i = r0; i < r1; ++i) {
                 const double ur = P.ar[(size_t)i * lda + m], ui = P.ai[(size_t)i * lda + m];
                 const double vr = xb_r[i - r0], vi = xb_i[i - r0];
