@@ -351,6 +351,8 @@ def main():
                     help="windows: closed loop at saturation (the throughput figure); poisson: open loop, "
                          "each request submitted at its arrival time (latency at a given load)")
     ap.add_argument("--rate", type=float, default=2000.0, help="requests/s of the poisson arrival mode")
+    ap.add_argument("--mix-tol-exp", type=int, default=None,
+                    help="CG stopping rule of the mixed path: relative residual 10^-exp (default 14)")
     ap.add_argument("--batch-split-n", type=int, default=None,
                     help="serving batches take one N class, split at this size (0: one class)")
     ap.add_argument("--profile-class", default=None,
@@ -374,6 +376,8 @@ def main():
         sys.exit(2)
     F.init()
     F.set_tunable("lanes", args.lanes)
+    if args.mix_tol_exp is not None:
+        F.set_tunable("mix_tol_exp", args.mix_tol_exp)
     if args.batch_split_n is not None:
         F.set_tunable("batch_split_n", args.batch_split_n)
     depth = args.depth or args.lanes

@@ -746,6 +746,7 @@ static int g_mix_first_chunk = kMixFirstChunk;
 // so a problem's solve time shrinks towards the chain latency of its blocks
 // instead of one workgroup streaming the whole factor.
 static int g_update_tile_m = 64; // row tile of the FP32 trailing update (64 or 128)
+static int g_mix_tol_exp = 14; // CG stops at |h - (M^H M + lambda) x| <= 10^-exp |h|
 static int g_batch_split_n = 1024; // serving batches: N classes split at this size (0: one class)
 static int g_gang_n = 512;
 static int g_gang_max = 4;
@@ -757,7 +758,6 @@ static int gang_size(int n)
     const int nb = (n + 63) / 64;
     return (int)std::max<long>(1, std::min<long>(q, std::min(g_gang_max, nb)));
 }
-constexpr double kMixTol = 1e-14;    // |h - (M^H M + lambda) x| <= tol |h|
 
 static size_t take(size_t &cur, size_t bytes)
 {
@@ -1464,7 +1464,7 @@ static int plan_upload(Plan &pl, const std::vector<const HostProblem *> &hps, ch
             P.dinv = (float *)(base + m.dinv);
             P.nchunk = m.nchunk;
             P.maxit = kMixMaxIt;
-            P.tol = kMixTol;
+            P.tol = std::pow(10.0, -(double)g_mix_tol_exp);
             LuDesc<float> D;
             std::memset(&D, 0, sizeof(D));
             D.ar = P.fr;
@@ -2335,6 +2335,10 @@ int set_tunable(const char *name, long value)
         g_gang_max = (int)value;
         return FNEC_OK;
     }
+    if (k == "mix_tol_exp" && value >= 8 && value <= 15) {
+        g_mix_tol_exp = (int)value;
+        return FNEC_OK;
+    }
     if (k == "batch_split_n" && value >= 0 && value <= 1 << 16) {
         g_batch_split_n = (int)value;
         return FNEC_OK;
@@ -2370,6 +2374,8 @@ long get_tunable(const char *name)
         return g_update_tile_m;
     if (k == "batch_split_n")
         return g_batch_split_n;
+    if (k == "mix_tol_exp")
+        return g_mix_tol_exp;
     return -1;
 }
 

@@ -254,6 +254,10 @@ int fnec_hip_debug_mfma_probe_f32(const float *a16x4, const float *b4x16, float 
  *                     update (k_lu_update_f32 work items): 64 (default; three
  *                     real MFMA products per complex multiply-add) or 128
  *                     (four). The factors differ at FP32 rounding level.
+ *   "mix_tol_exp"     mixed path: the CG stops when the normal-equation
+ *                     residual is at most 10^-value of its right-hand side.
+ *                     8 .. 15 (default 14). Not tuned automatically: it trades
+ *                     accuracy, not speed alone.
  *   "batch_split_n"   serving sessions: while the queue holds less than
  *                     max_batch requests, a batch takes only requests of its
  *                     oldest request's size class (N <= value or N > value),
