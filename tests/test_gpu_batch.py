@@ -234,3 +234,56 @@ def test_session_rejects_request_over_budget():
     probs = problems(24, [400])
     st, _ = run_session(probs, 1, 1 << 20, 8)
     assert st == F.RESOURCE_EXHAUSTED
+
+
+def test_direct_batches_beside_a_session_and_one_lane():
+    """Engine lanes under contention: direct batch calls from another thread
+    while a session serves (each call waits for a free lane), and a session
+    created with one lane. Every result is bitwise that of the request alone."""
+    import threading
+
+    probs = problems(29, [51, 300, 77, 900, 150, 64, 500, 120, 700, 90])
+    alone = F.solve_batch(probs, precision=1)
+    L = F.lib()
+
+    def serve(precision):
+        sess = L.fnec_hip_session_create(1 << 40, precision, 4)
+        assert sess
+        try:
+            keep = [F.make_request(p, tag=i) for i, p in enumerate(probs)]
+            for q, _ in keep:
+                assert L.fnec_hip_session_submit(sess, C.byref(q)) == 0
+            out = (F.Result * 16)()
+            got = {}
+            deadline = time.monotonic() + 120.0
+            while len(got) < len(probs) and time.monotonic() < deadline:
+                k = L.fnec_hip_session_poll(sess, C.cast(out, C.c_void_p), 16, 200000)
+                for j in range(k):
+                    got[out[j].tag] = (out[j].status, complex(out[j].z_in.re, out[j].z_in.im))
+            return got
+        finally:
+            L.fnec_hip_session_destroy(sess)
+
+    side = []
+
+    def direct():
+        for _ in range(3):
+            side.append(F.solve_batch(probs[::2], precision=1))
+
+    th = threading.Thread(target=direct)
+    th.start()
+    got = serve(1)
+    th.join()
+    for i, a in enumerate(alone):
+        assert got[i] == (F.OK, a["z_in"])
+    for res in side:
+        for p, a, b in zip(probs[::2], alone[::2], res):
+            assert b["status"] == F.OK and np.array_equal(a["x"], b["x"])
+    old = F.get_tunable("lanes")
+    F.set_tunable("lanes", 1)
+    try:
+        got = serve(1)
+    finally:
+        F.set_tunable("lanes", old)
+    for i, a in enumerate(alone):
+        assert got[i] == (F.OK, a["z_in"])
