@@ -639,7 +639,7 @@ __device__ void fresnel(double th_deg, double eps_r, double sigma, double k, dou
 }
 
 // Segments staged in LDS as {mid x,y,z, dir x,y,z, length, I re, I im}.
-constexpr int kRpStage = 10; // mid xyz, dir xyz, length, run flag, current re, im
+constexpr int kRpStage = 10; // mid xyz, dir xyz, length, run flag, current x length (re, im)
 constexpr int kRpChunk = 256;
 
 struct Dir {
@@ -666,7 +666,9 @@ __device__ __forceinline__ Dir unit_vectors(double th_deg, double ph_deg)
 // midpoint = previous midpoint + length * dir) is the previous one times the
 // run's rotation e^{j k length u.dir}; it is evaluated directly at the start
 // of a run, at the start of the chunk and every kRpAnchor segments, so a
-// rotated factor is at most kRpAnchor - 1 products from a direct one.
+// rotated factor is at most kRpAnchor - 1 products from a direct one. Along a
+// run the direction and length are the same, so the projections of dir on
+// the theta and phi unit vectors are computed at those evaluation points only.
 constexpr int kRpAnchor = 16;
 
 __device__ __forceinline__ void ff_chunk(const double *__restrict__ sg, int cnt, double k,
@@ -676,41 +678,41 @@ __device__ __forceinline__ void ff_chunk(const double *__restrict__ sg, int cnt,
 {
     double c = 1.0, s = 0.0, rc = 1.0, rs = 0.0;     // phase factor, rotation
     double c2 = 1.0, s2 = 0.0, rc2 = 1.0, rs2 = 0.0; // the same for the image
+    double qt = 0.0, qp = 0.0, it = 0.0, ip = 0.0;   // dir . theta, dir . phi (and the image's)
     for (int m = 0; m < cnt; ++m) {
         const double *g = sg + m * kRpStage;
-        const double mx = g[0], my = g[1], mz = g[2], dx = g[3], dy = g[4], dz = g[5];
-        const double len = g[6], ir = g[8], ii = g[9];
+        const double ir = g[8], ii = g[9]; // current x length
         const bool rotate = g[7] != 0.0 && (m % kRpAnchor) != 0;
         if (rotate) {
             const double nc = c * rc - s * rs, ns = s * rc + c * rs;
             c = nc;
             s = ns;
         } else {
+            const double mx = g[0], my = g[1], mz = g[2], dx = g[3], dy = g[4], dz = g[5], len = g[6];
             sincos_phase(k * (mx * u.rx + my * u.ry + mz * u.rz), s, c);
             sincos_phase(k * len * (dx * u.rx + dy * u.ry + dz * u.rz), rs, rc);
+            qt = dx * u.tx + dy * u.ty + dz * u.tz;
+            qp = dx * u.px + dy * u.py;
+            if (ground != 0) {
+                // image: position (x, y, -z), direction (-dx, -dy, dz) (farfield.rs:64-82)
+                sincos_phase(k * (mx * u.rx + my * u.ry - mz * u.rz), s2, c2);
+                sincos_phase(k * len * (dx * u.rx + dy * u.ry - dz * u.rz), rs2, rc2);
+                it = -dx * u.tx - dy * u.ty + dz * u.tz;
+                ip = -dx * u.px - dy * u.py;
+            }
         }
-        const double pr = len * c, pi = len * s;
-        const double wr = ir * pr - ii * pi, wi = ir * pi + ii * pr;
-        const double qt = dx * u.tx + dy * u.ty + dz * u.tz;
-        const double qp = dx * u.px + dy * u.py;
+        const double wr = ir * c - ii * s, wi = ir * s + ii * c;
         ftr += wr * qt;
         fti += wi * qt;
         fpr += wr * qp;
         fpi += wi * qp;
         if (ground != 0) {
-            // image: position (x, y, -z), direction (-dx, -dy, dz) (farfield.rs:64-82)
             if (rotate) {
                 const double nc = c2 * rc2 - s2 * rs2, ns = s2 * rc2 + c2 * rs2;
                 c2 = nc;
                 s2 = ns;
-            } else {
-                sincos_phase(k * (mx * u.rx + my * u.ry - mz * u.rz), s2, c2);
-                sincos_phase(k * len * (dx * u.rx + dy * u.ry - dz * u.rz), rs2, rc2);
             }
-            const double p2r = len * c2, p2i = len * s2;
-            const double vr = ir * p2r - ii * p2i, vi = ir * p2i + ii * p2r;
-            const double it = -dx * u.tx - dy * u.ty + dz * u.tz;
-            const double ip = -dx * u.px - dy * u.py;
+            const double vr = ir * c2 - ii * s2, vi = ir * s2 + ii * c2;
             const double atr = vr * it, ati = vi * it, apr = vr * ip, api = vi * ip;
             if (ground == 1) {
                 ftr += atr;
@@ -737,7 +739,7 @@ __device__ __forceinline__ void stage_chunk(const RpProb &R, double *sg, int bas
         if (c < 8)
             v = R.seg[(size_t)gm * kSegStride + c];
         else
-            v = R.cur[2 * (size_t)gm + (c - 8)];
+            v = R.cur[2 * (size_t)gm + (c - 8)] * R.seg[(size_t)gm * kSegStride + 6];
         sg[e] = v;
     }
 }
