@@ -665,67 +665,84 @@ __device__ __forceinline__ Dir unit_vectors(double th_deg, double ph_deg)
 // e^{j k u.mid} of a segment that continues a uniform run (flag in slot 7:
 // midpoint = previous midpoint + length * dir) is the previous one times the
 // run's rotation e^{j k length u.dir}; it is evaluated directly at the start
-// of a run, at the start of the chunk and every kRpAnchor segments, so a
+// of a run, at the start of a chain and every kRpAnchor segments of it, so a
 // rotated factor is at most kRpAnchor - 1 products from a direct one. Along a
 // run the direction and length are the same, so the projections of dir on
 // the theta and phi unit vectors are computed at those evaluation points only.
+// The chunk is walked as two interleaved chains (its first and second half),
+// so that each lane has two independent rotation recurrences in flight.
 constexpr int kRpAnchor = 16;
+
+struct FfChain {
+    double c = 1.0, s = 0.0, rc = 1.0, rs = 0.0;     // phase factor, rotation
+    double c2 = 1.0, s2 = 0.0, rc2 = 1.0, rs2 = 0.0; // the same for the image
+    double qt = 0.0, qp = 0.0, it = 0.0, ip = 0.0;   // dir . theta, dir . phi (and the image's)
+};
+
+__device__ __forceinline__ void ff_term(const double *__restrict__ g, bool anchor, double k, int ground,
+                                        const Dir &u, double gvr, double gvi, double ghr, double ghi,
+                                        FfChain &q, double &ftr, double &fti, double &fpr, double &fpi)
+{
+    const double ir = g[8], ii = g[9]; // current x length
+    const bool rotate = g[7] != 0.0 && !anchor;
+    if (rotate) {
+        const double nc = q.c * q.rc - q.s * q.rs, ns = q.s * q.rc + q.c * q.rs;
+        q.c = nc;
+        q.s = ns;
+    } else {
+        const double mx = g[0], my = g[1], mz = g[2], dx = g[3], dy = g[4], dz = g[5], len = g[6];
+        sincos_phase(k * (mx * u.rx + my * u.ry + mz * u.rz), q.s, q.c);
+        sincos_phase(k * len * (dx * u.rx + dy * u.ry + dz * u.rz), q.rs, q.rc);
+        q.qt = dx * u.tx + dy * u.ty + dz * u.tz;
+        q.qp = dx * u.px + dy * u.py;
+        if (ground != 0) {
+            // image: position (x, y, -z), direction (-dx, -dy, dz) (farfield.rs:64-82)
+            sincos_phase(k * (mx * u.rx + my * u.ry - mz * u.rz), q.s2, q.c2);
+            sincos_phase(k * len * (dx * u.rx + dy * u.ry - dz * u.rz), q.rs2, q.rc2);
+            q.it = -dx * u.tx - dy * u.ty + dz * u.tz;
+            q.ip = -dx * u.px - dy * u.py;
+        }
+    }
+    const double wr = ir * q.c - ii * q.s, wi = ir * q.s + ii * q.c;
+    ftr += wr * q.qt;
+    fti += wi * q.qt;
+    fpr += wr * q.qp;
+    fpi += wi * q.qp;
+    if (ground != 0) {
+        if (rotate) {
+            const double nc = q.c2 * q.rc2 - q.s2 * q.rs2, ns = q.s2 * q.rc2 + q.c2 * q.rs2;
+            q.c2 = nc;
+            q.s2 = ns;
+        }
+        const double vr = ir * q.c2 - ii * q.s2, vi = ir * q.s2 + ii * q.c2;
+        const double atr = vr * q.it, ati = vi * q.it, apr = vr * q.ip, api = vi * q.ip;
+        if (ground == 1) {
+            ftr += atr;
+            fti += ati;
+            fpr += apr;
+            fpi += api;
+        } else {
+            ftr += gvr * atr - gvi * ati;
+            fti += gvr * ati + gvi * atr;
+            fpr -= ghr * apr - ghi * api;
+            fpi -= ghr * api + ghi * apr;
+        }
+    }
+}
 
 __device__ __forceinline__ void ff_chunk(const double *__restrict__ sg, int cnt, double k,
                                          int ground, const Dir &u, double gvr, double gvi,
                                          double ghr, double ghi, double &ftr, double &fti,
                                          double &fpr, double &fpi)
 {
-    double c = 1.0, s = 0.0, rc = 1.0, rs = 0.0;     // phase factor, rotation
-    double c2 = 1.0, s2 = 0.0, rc2 = 1.0, rs2 = 0.0; // the same for the image
-    double qt = 0.0, qp = 0.0, it = 0.0, ip = 0.0;   // dir . theta, dir . phi (and the image's)
-    for (int m = 0; m < cnt; ++m) {
-        const double *g = sg + m * kRpStage;
-        const double ir = g[8], ii = g[9]; // current x length
-        const bool rotate = g[7] != 0.0 && (m % kRpAnchor) != 0;
-        if (rotate) {
-            const double nc = c * rc - s * rs, ns = s * rc + c * rs;
-            c = nc;
-            s = ns;
-        } else {
-            const double mx = g[0], my = g[1], mz = g[2], dx = g[3], dy = g[4], dz = g[5], len = g[6];
-            sincos_phase(k * (mx * u.rx + my * u.ry + mz * u.rz), s, c);
-            sincos_phase(k * len * (dx * u.rx + dy * u.ry + dz * u.rz), rs, rc);
-            qt = dx * u.tx + dy * u.ty + dz * u.tz;
-            qp = dx * u.px + dy * u.py;
-            if (ground != 0) {
-                // image: position (x, y, -z), direction (-dx, -dy, dz) (farfield.rs:64-82)
-                sincos_phase(k * (mx * u.rx + my * u.ry - mz * u.rz), s2, c2);
-                sincos_phase(k * len * (dx * u.rx + dy * u.ry - dz * u.rz), rs2, rc2);
-                it = -dx * u.tx - dy * u.ty + dz * u.tz;
-                ip = -dx * u.px - dy * u.py;
-            }
-        }
-        const double wr = ir * c - ii * s, wi = ir * s + ii * c;
-        ftr += wr * qt;
-        fti += wi * qt;
-        fpr += wr * qp;
-        fpi += wi * qp;
-        if (ground != 0) {
-            if (rotate) {
-                const double nc = c2 * rc2 - s2 * rs2, ns = s2 * rc2 + c2 * rs2;
-                c2 = nc;
-                s2 = ns;
-            }
-            const double vr = ir * c2 - ii * s2, vi = ir * s2 + ii * c2;
-            const double atr = vr * it, ati = vi * it, apr = vr * ip, api = vi * ip;
-            if (ground == 1) {
-                ftr += atr;
-                fti += ati;
-                fpr += apr;
-                fpi += api;
-            } else {
-                ftr += gvr * atr - gvi * ati;
-                fti += gvr * ati + gvi * atr;
-                fpr -= ghr * apr - ghi * api;
-                fpi -= ghr * api + ghi * apr;
-            }
-        }
+    const int half = (cnt + 1) / 2; // chain A: [0, half), chain B: [half, cnt)
+    FfChain qa, qb;
+    for (int mm = 0; mm < half; ++mm) {
+        const bool anchor = (mm % kRpAnchor) == 0;
+        ff_term(sg + mm * kRpStage, anchor, k, ground, u, gvr, gvi, ghr, ghi, qa, ftr, fti, fpr, fpi);
+        if (half + mm < cnt)
+            ff_term(sg + (half + mm) * kRpStage, anchor, k, ground, u, gvr, gvi, ghr, ghi, qb, ftr, fti, fpr,
+                    fpi);
     }
 }
 
