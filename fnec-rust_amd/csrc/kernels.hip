@@ -802,7 +802,7 @@ __global__ __launch_bounds__(kRpThreads) void k_rp_points(const RpProb *__restri
 // theta_i = i*tmax/(nt-1) (nt = 181 free / 91 ground), phi = 0..359 degrees,
 // trapezoid half weights at both theta ends, sin(theta) == 0 skipped
 // (farfield.rs:422-464). One partial sum per block, fixed reduction order.
-__global__ __launch_bounds__(kRpThreads) void k_rp_norm(const RpProb *__restrict__ probs,
+__global__ __launch_bounds__(kRpThreads) __attribute__((amdgpu_waves_per_eu(3))) void k_rp_norm(const RpProb *__restrict__ probs,
                                                         const int2 *__restrict__ work)
 {
     const int2 wk = work[blockIdx.x];
