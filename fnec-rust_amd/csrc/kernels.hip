@@ -50,8 +50,8 @@ __device__ __forceinline__ void quad8(double ox, double oy, double oz, double sx
                                       double a2, double k, double &re, double &im)
 {
     double sr = 0.0, si = 0.0;
-    // four points per iteration: four independent dependency chains per lane
-#pragma unroll 4
+    // two points per iteration: two independent dependency chains per lane
+#pragma unroll 2
     for (int m = 0; m < 8; ++m) {
         double sc = c_gl8n[m] * h;
         double px = ox - (sx + sc * dx);
