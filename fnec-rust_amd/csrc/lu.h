@@ -183,8 +183,10 @@ __global__ __launch_bounds__(kPanelThreads) void k_lu_panel(const LuDesc<T> *__r
 // factorisation is the partial-pivoting one and is committed; otherwise the
 // panel is left untouched for k_lu_panel.
 
-// One wavefront per problem: LU of the jb x jb diagonal block in LDS, one
-// lane per row (jb <= 64), so no workgroup barrier is needed.
+// One wavefront per problem: LU of the jb x jb diagonal block, one lane per
+// row. A lane keeps its row in registers (the loops over the block's columns
+// are unrolled, so every register index is static); at step j the pivot row
+// is broadcast through LDS and each lane below it eliminates in registers.
 template <typename T>
 __global__ __launch_bounds__(64) void k_panel_diag(const LuDesc<T> *__restrict__ descs,
                                                    const int *__restrict__ plist, int p)
@@ -195,44 +197,63 @@ __global__ __launch_bounds__(64) void k_panel_diag(const LuDesc<T> *__restrict__
         return;
     const int jb = min(kPanel, s - p);
     const int lda = D.lda, lane = threadIdx.x;
-    __shared__ T dr[kPanel][kPanel + 1], di[kPanel][kPanel + 1];
-    for (int e = lane; e < jb * jb; e += 64) {
-        const int r = e % jb, c = e / jb;
-        dr[r][c] = D.ar[(size_t)(p + c) * lda + p + r];
-        di[r][c] = D.ai[(size_t)(p + c) * lda + p + r];
-    }
-    __syncthreads();
-    T cmax_mine = 0; // lane j keeps column j's maximum
-    for (int j = 0; j < jb; ++j) {
-        const T ur = dr[j][j], ui = di[j][j];
-        const T nsq = ur * ur + ui * ui;
-        const T ir = ur / nsq, ii = -ui / nsq;
-        // |a_rj| below the diagonal at step j (max is order independent)
-        T v = (lane > j && lane < jb) ? cmod(dr[lane][j], di[lane][j]) : (T)0;
-        for (int off = 32; off > 0; off >>= 1)
-            v = fmax(v, __shfl_xor(v, off, 64));
-        if (lane == j)
-            cmax_mine = v;
-        if (lane > j && lane < jb) {
-            const T xr = dr[lane][j], xi = di[lane][j];
-            const T lr = xr * ir - xi * ii, li = xr * ii + xi * ir;
-            dr[lane][j] = lr;
-            di[lane][j] = li;
-            for (int c = j + 1; c < jb; ++c) {
-                const T vr = dr[j][c], vi = di[j][c];
-                dr[lane][c] -= lr * vr - li * vi;
-                di[lane][c] -= lr * vi + li * vr;
-            }
+    const bool mine = lane < jb;
+    __shared__ T rowr[kPanel], rowi[kPanel];
+    T ar[kPanel], ai[kPanel];
+#pragma unroll
+    for (int c = 0; c < kPanel; ++c) {
+        ar[c] = ai[c] = (T)0;
+        if (mine && c < jb) {
+            ar[c] = D.ar[(size_t)(p + c) * lda + p + lane];
+            ai[c] = D.ai[(size_t)(p + c) * lda + p + lane];
         }
-        __syncthreads();
     }
-    for (int e = lane; e < jb * jb; e += 64) {
-        const int r = e % jb, c = e / jb;
-        D.pr[(size_t)c * lda + p + r] = dr[r][c];
-        D.pi[(size_t)c * lda + p + r] = di[r][c];
+    T cmax_mine = 0, dmag = 0; // lane j: column j's maximum below the diagonal, |u_jj|
+#pragma unroll
+    for (int j = 0; j < kPanel; ++j) {
+        if (j < jb) { // uniform
+            if (lane == j) {
+#pragma unroll
+                for (int c = j; c < kPanel; ++c) {
+                    rowr[c] = ar[c];
+                    rowi[c] = ai[c];
+                }
+            }
+            __syncthreads();
+            const T ur = rowr[j], ui = rowi[j];
+            const T nsq = ur * ur + ui * ui;
+            const T ir = ur / nsq, ii = -ui / nsq;
+            // |a_rj| below the diagonal at step j (max is order independent)
+            T v = (lane > j && mine) ? cmod(ar[j], ai[j]) : (T)0;
+            for (int off = 32; off > 0; off >>= 1)
+                v = fmax(v, __shfl_xor(v, off, 64));
+            if (lane == j) {
+                cmax_mine = v;
+                dmag = cmod(ur, ui);
+            }
+            if (lane > j && mine) {
+                const T xr = ar[j], xi = ai[j];
+                const T lr = xr * ir - xi * ii, li = xr * ii + xi * ir;
+                ar[j] = lr;
+                ai[j] = li;
+#pragma unroll
+                for (int c = j + 1; c < kPanel; ++c) {
+                    const T vr = rowr[c], vi = rowi[c];
+                    ar[c] -= lr * vr - li * vi;
+                    ai[c] -= lr * vi + li * vr;
+                }
+            }
+            __syncthreads(); // row j is consumed before row j + 1 is published
+        }
     }
-    if (lane < jb) {
-        D.diagmag[lane] = (double)cmod(dr[lane][lane], di[lane][lane]);
+    if (mine) {
+#pragma unroll
+        for (int c = 0; c < kPanel; ++c)
+            if (c < jb) {
+                D.pr[(size_t)c * lda + p + lane] = ar[c];
+                D.pi[(size_t)c * lda + p + lane] = ai[c];
+            }
+        D.diagmag[lane] = (double)dmag;
         D.colmax[lane] = (unsigned long long)__double_as_longlong((double)cmax_mine);
     }
 }
@@ -336,48 +357,72 @@ __global__ __launch_bounds__(64) void k_panel_diag_f32(const LuDesc<float> *__re
         return;
     const int jb = min(NB, s - p);
     const int lda = D.lda, lane = threadIdx.x;
-    __shared__ float dr[NB][NB + 1], di[NB][NB + 1];
-    __shared__ float xr[NB][NB + 1], xi[NB][NB + 1];
-    for (int e = lane; e < NB * NB; e += 64) {
-        const int r = e % NB, c = e / NB;
-        const bool in = r < jb && c < jb;
-        dr[r][c] = in ? D.ar[(size_t)(p + c) * lda + p + r] : (r == c ? 1.f : 0.f);
-        di[r][c] = in ? D.ai[(size_t)(p + c) * lda + p + r] : 0.f;
-        xr[r][c] = r == c ? 1.f : 0.f;
-        xi[r][c] = 0.f;
+    const bool mine = lane < NB;
+    __shared__ float rowr[NB], rowi[NB];
+    __shared__ float ur_s[NB][NB + 1], ui_s[NB][NB + 1]; // U, for U_D^{-1}
+    // lane r keeps row r in registers; a partial block is completed with the
+    // identity so that the factorisation and the inverse are defined on NB x NB
+    float ar[NB], ai[NB];
+#pragma unroll
+    for (int c = 0; c < NB; ++c) {
+        const bool in = lane < jb && c < jb;
+        ar[c] = in ? D.ar[(size_t)(p + c) * lda + p + lane] : (lane == c ? 1.f : 0.f);
+        ai[c] = in ? D.ai[(size_t)(p + c) * lda + p + lane] : 0.f;
     }
-    __syncthreads();
-    float lmax = 0.f; // lane j: max |l_ij| of column j within the block
+    float lmax = 0.f; // lane j: max |l_ij|^2 of column j within the block
+#pragma unroll
     for (int j = 0; j < NB; ++j) {
-        const float ur = dr[j][j], ui = di[j][j];
+        if (lane == j) {
+#pragma unroll
+            for (int c = j; c < NB; ++c) {
+                rowr[c] = ar[c];
+                rowi[c] = ai[c];
+            }
+        }
+        __syncthreads();
+        const float ur = rowr[j], ui = rowi[j];
         const float nsq = ur * ur + ui * ui;
         const float ir = ur / nsq, ii = -ui / nsq;
         float m = 0.f;
-        if (lane > j && lane < NB) {
-            const float x = dr[lane][j], y = di[lane][j];
+        if (lane > j && mine) {
+            const float x = ar[j], y = ai[j];
             const float lr = x * ir - y * ii, li = x * ii + y * ir;
-            dr[lane][j] = lr;
-            di[lane][j] = li;
+            ar[j] = lr;
+            ai[j] = li;
             m = lr * lr + li * li;
-#pragma unroll 4
+#pragma unroll
             for (int c = j + 1; c < NB; ++c) {
-                const float vr = dr[j][c], vi = di[j][c];
-                dr[lane][c] -= lr * vr - li * vi;
-                di[lane][c] -= lr * vi + li * vr;
+                const float vr = rowr[c], vi = rowi[c];
+                ar[c] -= lr * vr - li * vi;
+                ai[c] -= lr * vi + li * vr;
             }
         }
         for (int off = 32; off > 0; off >>= 1)
             m = fmaxf(m, __shfl_xor(m, off, 64));
         if (lane == j)
             lmax = m;
-        __syncthreads();
+        __syncthreads(); // row j is consumed before row j + 1 is published
     }
-    // U_D^{-1}: row i of X is final once scaled by 1/u_ii; then every row
-    // k < i takes -= u_ki X[i][:]
+    if (mine) {
+#pragma unroll
+        for (int c = 0; c < NB; ++c) {
+            ur_s[lane][c] = ar[c];
+            ui_s[lane][c] = ai[c];
+        }
+    }
+    __syncthreads();
+    // U_D^{-1} in LDS: row i of X is final once scaled by 1/u_ii; then every
+    // row k < i takes -= u_ki X[i][:]
+    __shared__ float xr[NB][NB + 1], xi[NB][NB + 1];
+    for (int e = lane; e < NB * NB; e += 64) {
+        xr[e / NB][e % NB] = (e / NB == e % NB) ? 1.f : 0.f;
+        xi[e / NB][e % NB] = 0.f;
+    }
+    __syncthreads();
     for (int i = NB - 1; i >= 0; --i) {
-        const float ur = dr[i][i], ui = di[i][i];
-        const float nsq = ur * ur + ui * ui;
-        const float ir = ur / nsq, ii = -ui / nsq;
+        const float dr = ur_s[i][i], di = ui_s[i][i];
+        const float nsq = dr * dr + di * di;
+        const float ir = dr / nsq, ii = -di / nsq;
         if (lane >= i && lane < NB) {
             const float x = xr[i][lane], y = xi[i][lane];
             xr[i][lane] = x * ir - y * ii;
@@ -387,16 +432,19 @@ __global__ __launch_bounds__(64) void k_panel_diag_f32(const LuDesc<float> *__re
         const int w = NB - i;
         for (int e = lane; e < i * w; e += 64) {
             const int k = e / w, c = i + e % w;
-            const float a = dr[k][i], b = di[k][i], x = xr[i][c], y = xi[i][c];
+            const float a = ur_s[k][i], b = ui_s[k][i], x = xr[i][c], y = xi[i][c];
             xr[k][c] -= a * x - b * y;
             xi[k][c] -= a * y + b * x;
         }
         __syncthreads();
     }
-    for (int e = lane; e < jb * jb; e += 64) {
-        const int r = e % jb, c = e / jb;
-        D.pr[(size_t)c * lda + p + r] = dr[r][c];
-        D.pi[(size_t)c * lda + p + r] = di[r][c];
+    if (mine) {
+#pragma unroll
+        for (int c = 0; c < NB; ++c)
+            if (c < jb && lane < jb) {
+                D.pr[(size_t)c * lda + p + lane] = ar[c];
+                D.pi[(size_t)c * lda + p + lane] = ai[c];
+            }
     }
     float *ui_r = D.linv + 2 * 64 * 64, *ui_i = ui_r + NB * NB;
     for (int e = lane; e < NB * NB; e += 64) {
@@ -404,7 +452,8 @@ __global__ __launch_bounds__(64) void k_panel_diag_f32(const LuDesc<float> *__re
         ui_i[e] = xi[e / NB][e % NB];
     }
     if (lane < jb) {
-        D.diagmag[lane] = (double)__builtin_sqrtf(dr[lane][lane] * dr[lane][lane] + di[lane][lane] * di[lane][lane]);
+        const float dr = ur_s[lane][lane], di = ui_s[lane][lane];
+        D.diagmag[lane] = (double)__builtin_sqrtf(dr * dr + di * di);
         D.colmax[lane] = (unsigned long long)__double_as_longlong((double)__builtin_sqrtf(lmax));
     }
 }
