@@ -238,6 +238,25 @@ def test_rp_multi_wire_matches_oracle(oracle, seed, n, junctioned, ground):
     assert np.max(np.abs(rows[:, 3] - r["rp"][:, 3])) < 1e-9
 
 
+@pytest.mark.parametrize("nseg,ground", [(700, ""), (600, "GN 1")])
+def test_rp_long_straight_wire_matches_oracle(oracle, nseg, ground):
+    """One straight wire of several hundred equal segments: a single uniform
+    run that spans staging chunks of 256 segments and both halves of each
+    chunk, so the phase factor is rotated over long stretches and re-anchored
+    at chunk, chain and 16-segment boundaries."""
+    z0 = 45.0 if ground else 0.0  # over ground: the wire spans z = 5 .. 85 m
+    txt = (f"CM long wire\nGW 1 {nseg} 0 0 {z0 - 40.0} 0.3 0.2 {z0 + 40.0} 0.002\n"
+           + (ground + "\n" if ground else "")
+           + f"EX 0 1 {nseg // 2} 0 1 0\nFR 0 1 0 0 7.0 0\nRP 0 37 73 0.0 0.0 5.0 5.0\nEN\n")
+    r = oracle.run_deck(txt, "cli")
+    p = F.Deck(txt).problem("cli", r["freq_hz"])
+    assert len(p["segs"]) == nseg
+    rows, total = F.rp(p["segs"], r["currents"], r["freq_hz"], r["rp_theta"], r["rp_phi"],
+                       p["ground_struct"])
+    assert abs(total - r["rp_total"]) / r["rp_total"] < 1e-12
+    pattern_close(rows, r["rp"])
+
+
 def test_rp_fresnel_matches_oracle(oracle):
     txt = deck_text("dipole-gn0-fresnel-51seg").replace("EN", "RP 0 19 37 0 0 10 10\nEN")
     r = oracle.run_deck(txt, "cli")
