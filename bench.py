@@ -371,7 +371,7 @@ def main():
         copy_probe()
         return
     if args.gpus != 1:
-        print("bench.py: this build measures one GPU; multi-GPU runs are not part of it "
+        print("bench.py measures one GPU; multi-GPU runs are not supported "
               "(requests shard across GPUs with no device communication)", file=sys.stderr)
         sys.exit(2)
     F.init()
