@@ -190,8 +190,11 @@ typedef struct {
     fnec_solve_stats stats;
 } fnec_result;
 
-/* Solve every request in one batched submission (persistent kernels over a
- * work list, one stream). results[i] corresponds to reqs[i]. */
+/* Solve every request in one batched submission: each kernel launch walks a
+ * work list of (problem, tile) items over the whole batch, on one engine lane
+ * (its solve stream plus the LU's look-ahead stream). results[i] corresponds
+ * to reqs[i]; a result's x / rp (or, where NULL, the request's x_out / rp_out)
+ * receive the currents and the pattern rows. */
 int fnec_hip_solve_batch(const fnec_request *reqs, size_t count, int precision, fnec_result *results);
 
 /* ---- serving session (continuous batching, nec_worker serving mode) ----
