@@ -212,8 +212,10 @@ int fnec_hip_solve_batch(const fnec_request *reqs, size_t count, int precision, 
  * max), waiting up to wait_us for the first one while requests are queued or
  * running; pending counts queued, running and unpolled requests. A result's
  * status is that of its request; after a failed one, fnec_hip_last_error()
- * on the polling thread gives the batch's message. destroy waits for the
- * batches on the device and drops queued requests. */
+ * on the polling thread gives the batch's message. submit, poll and pending
+ * may be called from several threads at once; destroy waits for the batches
+ * on the device, drops queued requests, and must not overlap any other call
+ * on the same session. */
 typedef struct fnec_session fnec_session;
 fnec_session *fnec_hip_session_create(size_t budget_bytes, int precision, int max_batch);
 int fnec_hip_session_submit(fnec_session *s, const fnec_request *req);
