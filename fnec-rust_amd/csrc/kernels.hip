@@ -49,15 +49,17 @@ __device__ __forceinline__ void quad8(double ox, double oy, double oz, double sx
                                       double sz, double dx, double dy, double dz, double h,
                                       double a2, double k, double &re, double &im)
 {
+    // R^2 at the point sc along the unit direction: |d - sc dir|^2 + a^2 =
+    // (|d|^2 + a^2) + sc (sc - 2 d.dir), d = observation - source centre
+    const double ex = ox - sx, ey = oy - sy, ez = oz - sz;
+    const double A = ex * ex + ey * ey + ez * ez + a2;
+    const double B = -2.0 * (ex * dx + ey * dy + ez * dz);
     double sr = 0.0, si = 0.0;
     // two points per iteration: two independent dependency chains per lane
 #pragma unroll 2
     for (int m = 0; m < 8; ++m) {
-        double sc = c_gl8n[m] * h;
-        double px = ox - (sx + sc * dx);
-        double py = oy - (sy + sc * dy);
-        double pz = oz - (sz + sc * dz);
-        const double r2 = px * px + py * py + pz * pz + a2;
+        const double sc = c_gl8n[m] * h;
+        const double r2 = fma(sc, sc + B, A);
         const double ir = rsqrt(r2), r = r2 * ir; // one reciprocal square root for R and 1/R
         double s, c;
         sincos_phase(k * r, s, c);
