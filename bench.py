@@ -42,7 +42,10 @@ METRIC = "solves/s + p99 latency, mixed-N queue, vs plain FP64 GPU at rel Î”Zinâ
 PEAK_FP64_TFLOPS = 78.6
 PEAK_FP32_TFLOPS = 157.3
 PEAK_HBM_GBS = 8000.0
-HBM_BOUND = ("zsolve", "matvec", "backsub")
+# lu_update_k128 runs the FP32 algorithm on the f16 matrix cores (three f16
+# products per FP32 product): about 192 f16 flops per HBM byte at K = 128,
+# under the f16 ridge of 2500 TFLOP/s / 8 TB/s = 312, so HBM bounds it
+HBM_BOUND = ("zsolve", "matvec", "backsub", "lu_update_k128")
 
 RP_THETA = np.repeat(np.arange(0.0, 181.0, 2.0)[None, :], 180, axis=0).ravel()  # phi outer, theta inner
 RP_PHI = np.repeat(np.arange(0.0, 360.0, 2.0), 91)
@@ -318,6 +321,8 @@ def measured_traffic(kernel, precision):
     tm = F.get_tunable("update_tile_m")
     if kernel == "lu_update" and e.get("symbol") != f"k_lu_update_f32<{tm}, 64, {'true' if tm == 64 else 'false'}>":
         return None  # measured with the other tile and product form
+    if kernel == "lu_update_k128" and e.get("symbol") != "k_lu_update_f16x2":
+        return None
     return {"bytes_per_launch": e["hbm_bytes_per_launch"], "source": "profiles/serving_stream_traffic.json"}
 
 
@@ -357,6 +362,8 @@ def main():
                     help="serving batches take one N class, split at this size (0: one class)")
     ap.add_argument("--profile-class", default=None,
                     help="kernel class timed in the timed windows (default: the dominant one)")
+    ap.add_argument("--update-k128", type=int, choices=[0, 1], default=None,
+                    help="K = 128 trailing-update passes of the FP32 LU (default 1; with --no-autotune)")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the engine's default knobs (no autotune on a window of the stream)")
     ap.add_argument("--depth", type=int, default=None,
@@ -380,6 +387,8 @@ def main():
         F.set_tunable("mix_tol_exp", args.mix_tol_exp)
     if args.batch_split_n is not None:
         F.set_tunable("batch_split_n", args.batch_split_n)
+    if args.update_k128 is not None:
+        F.set_tunable("update_k128", args.update_k128)
     depth = args.depth or args.lanes
     precision = F.PREC_MIXED if args.precision == "mixed" else F.PREC_FP64
     total = (args.warmup + args.profile_windows + args.steps) * args.window
@@ -405,7 +414,8 @@ def main():
         t_first = stream[first]["t_arrival"]
         scale = 1000.0 / args.rate  # the stream is generated at 1000 requests/s
         times = [(stream[first + i]["t_arrival"] - t_first) * scale for i in range(len(timed_items))]
-        dom = args.profile_class or "lu_update"
+        k128 = F.get_tunable("update_k128") == 1 and F.get_tunable("update_tile_m") == 64
+        dom = args.profile_class or ("lu_update_k128" if k128 else "lu_update")
         elapsed, lat, zs_served, prof, batches = serve_poisson(
             timed_items, times, items[:first], precision, int(args.budget_gib * 2 ** 30), args.max_batch, dom)
         prof_all = prof

@@ -382,7 +382,7 @@ def solve_problem(p: dict, precision: int = PREC_FP64, with_stats: bool = False)
 
 
 PROFILE_CLASSES = ("fill", "gram", "lu_panel", "lu_trsm", "lu_update", "backsub", "rp", "zsolve", "matvec",
-                   "pcg_ctl", "fsetup", "lu_update_narrow",
+                   "pcg_ctl", "fsetup", "lu_update_narrow", "lu_update_k128",
                    # host-side phases (steady clock): problem and plan building, staging
                    # copies, far-field set-up, result processing, time blocked on the stream,
                    # kernel enqueueing up to the first wait

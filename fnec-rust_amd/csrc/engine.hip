@@ -222,12 +222,12 @@ struct ProfClass {
 // Classes from PC_HOST_PREP on are host-side phases timed with a steady clock
 // (launches = occurrences, no flops or bytes).
 enum { PC_FILL, PC_GRAM, PC_LU_PANEL, PC_LU_TRSM, PC_LU_UPDATE, PC_BACKSUB, PC_RP, PC_ZSOLVE,
-       PC_MV, PC_PCG, PC_FSETUP, PC_LU_UPDN, PC_HOST_PREP, PC_HOST_STAGE, PC_HOST_RP, PC_HOST_POST,
-       PC_HOST_WAIT, PC_HOST_ENQ, PC_N };
+       PC_MV, PC_PCG, PC_FSETUP, PC_LU_UPDN, PC_LU_UPD128, PC_HOST_PREP, PC_HOST_STAGE, PC_HOST_RP,
+       PC_HOST_POST, PC_HOST_WAIT, PC_HOST_ENQ, PC_N };
 static const char *kProfNames[PC_N] = {"fill", "gram", "lu_panel", "lu_trsm", "lu_update",
                                        "backsub", "rp", "zsolve", "matvec", "pcg_ctl", "fsetup",
-                                       "lu_update_narrow", "host_prep", "host_stage", "host_rp",
-                                       "host_post", "host_wait", "host_enqueue"};
+                                       "lu_update_narrow", "lu_update_k128", "host_prep", "host_stage",
+                                       "host_rp", "host_post", "host_wait", "host_enqueue"};
 static bool g_prof = false;
 static ProfClass g_prof_acc[PC_N];
 static std::vector<hipEvent_t> g_prof_pool;
@@ -707,7 +707,14 @@ struct Plan {
         size_t ot_off, ot_cnt;     // interchanges left of P, and of + L11 solve for columns [P + 64, n)
         size_t on_off, on_cnt;     // outer update (K = 64) of the next step's columns [P + 64, P + 128)
         size_t of_off, of_cnt;     // outer update of the columns [P + 128, n)
-        double f_panel, f_trsm, f_ua, b_ua, f_on, b_on, f_of, b_of;
+        // K = 128 passes: a deferring step leaves its update of [P + 128, n)
+        // to the next step; that combining step updates its own U12 rows
+        // [P, P + 64) with the deferred factors (thin list, K = 64) between
+        // its interchanges and its L11^{-1} product, then runs on and of with
+        // K = 128 from P - 64
+        bool defer, combine;
+        size_t th_off, th_cnt;
+        double f_panel, f_trsm, f_ua, b_ua, f_on, b_on, f_of, b_of, f_th, b_th;
     };
     std::vector<Outer> outer32;
     std::vector<LuDesc<float>> lud32;
@@ -746,6 +753,7 @@ static int g_mix_first_chunk = kMixFirstChunk;
 // so a problem's solve time shrinks towards the chain latency of its blocks
 // instead of one workgroup streaming the whole factor.
 static int g_update_tile_m = 64; // row tile of the FP32 trailing update (64 or 128)
+static int g_update_k128 = 1;    // K = 128 trailing-update passes (with 64-row tiles)
 static int g_mix_tol_exp = 14; // CG stops at |h - (M^H M + lambda) x| <= 10^-exp |h|
 static int g_batch_split_n = 1024; // serving batches: N classes split at this size (0: one class)
 static int g_gang_n = 512;
@@ -908,7 +916,7 @@ static void lu_plan(std::vector<int> &ints, const std::vector<int> &ms, const st
 // two 32-column panels, so that the trailing update runs with K = 64 and reads
 // and writes the trailing matrix once per 64 columns.
 static void lu_plan_outer(std::vector<int> &ints, const std::vector<int> &ms, std::vector<Plan::Outer> &out,
-                          std::vector<int4> &gensegs, size_t &gen_items, int tm)
+                          std::vector<int4> &gensegs, size_t &gen_items, int tm, bool k128)
 {
     auto align4 = [&]() {
         while (ints.size() % 4)
@@ -958,9 +966,14 @@ static void lu_plan_outer(std::vector<int> &ints, const std::vector<int> &ms, st
         }
         st.commit_cnt = (ints.size() - st.commit_off) / 2;
     };
+    const int nsteps = (smax + 2 * kPanel - 1) / (2 * kPanel);
     for (int P = 0; P < smax; P += 2 * kPanel) {
         Plan::Outer o{};
         o.P = P;
+        const int k = P / (2 * kPanel);
+        // K = 128 pairs: even steps defer (when a next step exists), odd steps combine
+        o.combine = k128 && (k & 1);
+        o.defer = k128 && !(k & 1) && k + 1 < nsteps;
         o.a.p = P;
         o.b.p = P + kPanel;
         o.has_b = false;
@@ -1028,7 +1041,7 @@ static void lu_plan_outer(std::vector<int> &ints, const std::vector<int> &ms, st
         o.of_off = gen_items;
         for (size_t b = 0; b < np; ++b) {
             const int s = ms[b];
-            if (s <= P + 4 * kPanel)
+            if (s <= P + 4 * kPanel || o.defer)
                 continue;
             const int ny = (s - P - 2 * kPanel + tm - 1) / tm;
             const int nz = (s - P - 4 * kPanel + kTile32N - 1) / kTile32N;
@@ -1036,8 +1049,21 @@ static void lu_plan_outer(std::vector<int> &ints, const std::vector<int> &ms, st
             gen_items += (size_t)ny * nz;
         }
         o.of_cnt = gen_items - o.of_off;
+        // combining step: rows [P, P + 64) of the columns [P + 64, n), in
+        // 64-row tiles whatever tm
+        o.th_off = gen_items;
+        for (size_t b = 0; b < np; ++b) {
+            const int s = ms[b];
+            if (!o.combine || s <= P + 2 * kPanel)
+                continue;
+            const int nz = (s - P - 2 * kPanel + kTile32N - 1) / kTile32N;
+            gensegs.push_back(make_int4((int)gen_items, (int)b, 1, nz));
+            gen_items += (size_t)nz;
+        }
+        o.th_cnt = gen_items - o.th_off;
         // algorithmic work of the launches (8 flops per complex multiply-add;
         // 8 bytes per FP32 complex element)
+        const double kd = o.combine ? 128.0 : 64.0; // depth of the on / of passes
         for (size_t b = 0; b < np; ++b) {
             const double s = ms[b];
             if (s <= P)
@@ -1050,10 +1076,17 @@ static void lu_plan_outer(std::vector<int> &ints, const std::vector<int> &ms, st
             const double rem = std::max(0.0, s - P - 2.0 * kPanel);
             const double cn = std::min(64.0, rem), cf = std::max(0.0, rem - 64.0);
             o.f_trsm += 8.0 * cols_a * ja * ja / 2.0 + 8.0 * rem * 64.0 * 64.0 / 2.0;
-            o.f_on += 8.0 * rem * cn * 64.0;
-            o.b_on += 8.0 * (2.0 * rem * cn + rem * 64.0 + 64.0 * cn);
-            o.f_of += 8.0 * rem * cf * 64.0;
-            o.b_of += 8.0 * (2.0 * rem * cf + rem * 64.0 + 64.0 * cf);
+            o.f_on += 8.0 * rem * cn * kd;
+            o.b_on += 8.0 * (2.0 * rem * cn + rem * kd + kd * cn);
+            if (!o.defer) {
+                o.f_of += 8.0 * rem * cf * kd;
+                o.b_of += 8.0 * (2.0 * rem * cf + rem * kd + kd * cf);
+            }
+            if (o.combine && s > P + 2.0 * kPanel) {
+                const double ct = s - P - 2.0 * kPanel; // rows [P, P + 64) x these columns, K = 64
+                o.f_th += 8.0 * 64.0 * ct * 64.0;
+                o.b_th += 8.0 * (2.0 * 64.0 * ct + 64.0 * 64.0 + 64.0 * ct);
+            }
         }
         out.push_back(o);
     }
@@ -1196,7 +1229,8 @@ static void plan_build(Plan &pl, const std::vector<const HostProblem *> &hps, bo
         for (size_t b = 0; b < np; ++b)
             ms[b] = hps[b]->n;
         pl.upd_tm = g_update_tile_m;
-        lu_plan_outer(pl.ints, ms, pl.outer32, pl.gensegs, pl.gen_items, pl.upd_tm);
+        lu_plan_outer(pl.ints, ms, pl.outer32, pl.gensegs, pl.gen_items, pl.upd_tm,
+                      g_update_k128 != 0 && pl.upd_tm == 64);
         // CG solves of problems above solve1_max_n (flag-chained k_trsv0/1):
         // items (problem, -1, phase, block), phase 0 blocks in forward order,
         // then phase 1 blocks in backward order, interleaved across problems
@@ -1621,16 +1655,36 @@ static int mixed_launch(Plan &pl, char *base)
             HIPCHK(hipEventRecord(ev_chain[0], side));
         }
         HIPCHK(hipStreamWaitEvent(st, ev_chain[k], 0));
-        if (o.ot_cnt) {
+        if (o.ot_cnt && o.combine) {
+            // interchanges, then the deferred step's factors on this step's
+            // U12 rows, then the L11^{-1} product
             hipEvent_t e = prof_begin(PC_LU_TRSM);
-            lu_launch_trsm64_f32(lud, o.ot_off, o.ot_cnt, ints, o.P, st);
+            lu_launch_trsm64_f32(lud, o.ot_off, o.ot_cnt, ints, o.P, kTrsmSwap, st);
+            prof_end(PC_LU_TRSM, e, 0, 0);
+            if (o.th_cnt) {
+                e = prof_begin(PC_LU_UPDN);
+                lu_launch_update_f32(lud, 4 * o.th_off, o.th_cnt, gen_ptr(base, pl), o.P - 2 * kPanel, 2, o.P,
+                                     o.P + 2 * kPanel, 0x7fffffff, 64, st);
+                prof_end(PC_LU_UPDN, e, o.f_th, o.b_th);
+            }
+            e = prof_begin(PC_LU_TRSM);
+            lu_launch_trsm64_f32(lud, o.ot_off, o.ot_cnt, ints, o.P, kTrsmSolve, st);
+            prof_end(PC_LU_TRSM, e, o.f_trsm, 0);
+        } else if (o.ot_cnt) {
+            hipEvent_t e = prof_begin(PC_LU_TRSM);
+            lu_launch_trsm64_f32(lud, o.ot_off, o.ot_cnt, ints, o.P, kTrsmSwap | kTrsmSolve, st);
             prof_end(PC_LU_TRSM, e, o.f_trsm, 0);
         }
         if (o.on_cnt) {
-            hipEvent_t e = prof_begin(PC_LU_UPDATE);
-            lu_launch_update_f32(lud, 4 * o.on_off, o.on_cnt, gen_ptr(base, pl), o.P, 2, o.P + 2 * kPanel,
-                                 o.P + 2 * kPanel, o.P + 4 * kPanel, pl.upd_tm, st);
-            prof_end(PC_LU_UPDATE, e, o.f_on, o.b_on);
+            const int pc = o.combine ? PC_LU_UPD128 : PC_LU_UPDATE;
+            hipEvent_t e = prof_begin(pc);
+            if (o.combine)
+                lu_launch_update_f16x2(lud, 4 * o.on_off, o.on_cnt, gen_ptr(base, pl), o.P - 2 * kPanel,
+                                       o.P + 2 * kPanel, o.P + 2 * kPanel, o.P + 4 * kPanel, st);
+            else
+                lu_launch_update_f32(lud, 4 * o.on_off, o.on_cnt, gen_ptr(base, pl), o.P, 2, o.P + 2 * kPanel,
+                                     o.P + 2 * kPanel, o.P + 4 * kPanel, pl.upd_tm, st);
+            prof_end(pc, e, o.f_on, o.b_on);
         }
         if (k + 1 < nsteps) {
             HIPCHK(hipEventRecord(ev_near[k], st));
@@ -1639,10 +1693,15 @@ static int mixed_launch(Plan &pl, char *base)
             HIPCHK(hipEventRecord(ev_chain[k + 1], side));
         }
         if (o.of_cnt) {
-            hipEvent_t e = prof_begin(PC_LU_UPDATE);
-            lu_launch_update_f32(lud, 4 * o.of_off, o.of_cnt, gen_ptr(base, pl), o.P, 2, o.P + 2 * kPanel,
-                                 o.P + 4 * kPanel, 0x7fffffff, pl.upd_tm, st);
-            prof_end(PC_LU_UPDATE, e, o.f_of, o.b_of);
+            const int pc = o.combine ? PC_LU_UPD128 : PC_LU_UPDATE;
+            hipEvent_t e = prof_begin(pc);
+            if (o.combine)
+                lu_launch_update_f16x2(lud, 4 * o.of_off, o.of_cnt, gen_ptr(base, pl), o.P - 2 * kPanel,
+                                       o.P + 2 * kPanel, o.P + 4 * kPanel, 0x7fffffff, st);
+            else
+                lu_launch_update_f32(lud, 4 * o.of_off, o.of_cnt, gen_ptr(base, pl), o.P, 2, o.P + 2 * kPanel,
+                                     o.P + 4 * kPanel, 0x7fffffff, pl.upd_tm, st);
+            prof_end(pc, e, o.f_of, o.b_of);
         }
     }
     HIPCHK(hipGetLastError());
@@ -2347,6 +2406,10 @@ int set_tunable(const char *name, long value)
         g_update_tile_m = (int)value;
         return FNEC_OK;
     }
+    if (k == "update_k128" && (value == 0 || value == 1)) {
+        g_update_k128 = (int)value;
+        return FNEC_OK;
+    }
     if (k == "lanes" && value >= 1 && value <= kMaxLanes) {
         std::lock_guard<std::mutex> lk(g_lane_mu);
         g_lanes_on = (int)value;
@@ -2372,6 +2435,8 @@ long get_tunable(const char *name)
         return g_lanes_on;
     if (k == "update_tile_m")
         return g_update_tile_m;
+    if (k == "update_k128")
+        return g_update_k128;
     if (k == "batch_split_n")
         return g_batch_split_n;
     if (k == "mix_tol_exp")

@@ -25,11 +25,18 @@
 //                with C, K = 32 nk columns of L21 per launch.
 //   k_l11_inv    FP32 outer step: L11^{-1} of the 64 x 64 diagonal block.
 //   k_lu_trsm64_f32  FP32 outer step: interchanges of two 32-column panels
-//                and U12 = L11^{-1} A12 on MFMA for the columns right of both.
+//                and U12 = L11^{-1} A12 on MFMA for the columns right of both
+//                (together, or as two launches).
+//   k_lu_update_f16x2  the K = 128 trailing update of two outer steps on the
+//                f16 matrix cores (operands split into hi + lo f16 pairs).
 // The FP32 factorisation (mixed path) is two-level: each 64-column step
 // factors two 32-column panels (the second after a narrow K = 32 update of
 // its columns only), then updates the trailing matrix once with K = 64, so
-// the trailing matrix is read and written once per 64 columns.
+// the trailing matrix is read and written once per 64 columns. With K = 128
+// passes, every other step updates only the next step's 64 columns, and the
+// step after it first brings its own U12 rows up to date (a thin K = 64
+// update between its interchanges and its L11^{-1} product), then updates
+// the rest with both steps' factors at once: once per 128 columns.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -1001,6 +1008,198 @@ __global__ __launch_bounds__(256) void k_lu_update_f32(const LuDesc<float> *__re
         }
 }
 
+typedef _Float16 h8 __attribute__((ext_vector_type(8)));
+
+// x = hi + lo 2^-11 with hi and lo in f16: hi is x rounded to f16 (0 below
+// the f16 normal range, so that no hi is subnormal), lo the remainder scaled
+// into the f16 range. x - hi and the scaling are exact in FP32, so
+// |x - hi - lo 2^-11| <= 2^-22 |x| + 2^-26 for |x| < 65504.
+__device__ __forceinline__ void split_f16x2(float x, _Float16 &hi, _Float16 &lo)
+{
+    const bool normal = fabsf(x) >= 0x1p-14f;
+    const _Float16 h = normal ? (_Float16)x : (_Float16)0.f;
+    hi = h;
+    lo = (_Float16)((x - (float)h) * 2048.f);
+}
+
+// FP32 trailing update on the f16 matrix cores, for K = 128 passes: A22 -=
+// L21 U12 over 64 x 64 tiles, with the work list, bounds and launch shape of
+// k_lu_update_f32<64, 64>. Every FP32 operand is split by split_f16x2, and a
+// real product a b is taken as hi_a hi_b + 2^-11 (hi_a lo_b + lo_a hi_b) with
+// FP32 accumulation: relative error about 2^-21 per product (the dropped
+// lo lo term and the rounding of lo), where FP32 products have 2^-24. The
+// factor is a CG preconditioner, so this changes the iteration's rate, not
+// the solution it converges to. Per 16 x 16 tile and 32-deep K chunk: 12
+// v_mfma_f32_16x16x32_f16 (hi hi and two cross terms for each of the four
+// real products of the complex product), where the FP32 forms issue 24 or
+// 32 v_mfma_f32_16x16x4f32 of the same issue cost each. A K = 64 pass is
+// bound by the read and write of C and gains little from this; a K = 128
+// pass, with half the C traffic per flop, runs about 1.5x the FP32 K = 64
+// form's rate.
+//
+// Loader thread t takes row (t & 63) of L21 at the 8 columns
+// k0 + 8 (t >> 6) .. + 7 (each load instruction reads 64 consecutive rows of
+// one column) and column (t & 63) of U12 at rows k0 + 8 (t >> 6) .. + 7 (two
+// float4 per plane). It splits the values and writes each plane as one
+// 16-byte run of LDS laid out [plane][row][k]. That run is an MFMA operand
+// as it stands: lane l supplies 8 consecutive k of row l % 16, with the k
+// group l / 16 (any fixed assignment of k to lane group and element is the
+// same dot product, as long as L21 and U12 use the same one, as here).
+// C is read in the epilogue, which keeps the kernel within 168 registers:
+// three workgroups per CU.
+constexpr int kH2Ld = 40; // halves per LDS row: 32 k and 8 of padding
+
+__global__ __launch_bounds__(256, 3) void k_lu_update_f16x2(const LuDesc<float> *__restrict__ descs,
+                                                            const int4 *__restrict__ work, int cnt, int kbeg,
+                                                            int nk, int rbeg, int cbeg, int cend)
+{
+    constexpr int TM = 64, TN = 64;
+    const int per = (cnt + kXcds - 1) / kXcds;
+    const int item = (blockIdx.x % kXcds) * per + blockIdx.x / kXcds;
+    if (item >= cnt)
+        return;
+    const int4 wk = work[item];
+    const LuDesc<float> &D = descs[wk.x];
+    const int s = D.m, nc = min(D.ncols, cend);
+    if (D.info[0] != 0 || rbeg >= s || kbeg + kPanel * nk > s)
+        return;
+    const int lda = D.lda;
+    const int r0 = rbeg + wk.y * TM, c0 = cbeg + wk.z * TN;
+    // planes: hi and lo of the real part, hi and lo of the imaginary part
+    __shared__ __align__(16) _Float16 sa[4][TM][kH2Ld];
+    __shared__ __align__(16) _Float16 sb[4][TN][kH2Ld];
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const int wr = wave >> 1, wc = wave & 1;
+    const int lrow = lane & 15, lk = lane >> 4;
+    const int ld_r = t & 63, ld_g = t >> 6;
+    float *ar = D.ar, *ai = D.ai;
+    float xa_r[8], xa_i[8], xb_r[8], xb_i[8];
+    auto load_ab = [&](int k0) {
+        const int gr = r0 + ld_r;
+        const int kc = k0 + 8 * ld_g;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            xa_r[q] = xa_i[q] = 0.f;
+            if (gr < s) {
+                const size_t o = (size_t)(kc + q) * lda + gr;
+                xa_r[q] = ar[o];
+                xa_i[q] = ai[o];
+            }
+        }
+        const int gc = c0 + ld_r;
+        float4 b0r = make_float4(0.f, 0.f, 0.f, 0.f), b1r = b0r, b0i = b0r, b1i = b0r;
+        if (gc < nc) {
+            const size_t o = (size_t)gc * lda + kc;
+            b0r = *(const float4 *)(ar + o);
+            b1r = *(const float4 *)(ar + o + 4);
+            b0i = *(const float4 *)(ai + o);
+            b1i = *(const float4 *)(ai + o + 4);
+        }
+        xb_r[0] = b0r.x, xb_r[1] = b0r.y, xb_r[2] = b0r.z, xb_r[3] = b0r.w;
+        xb_r[4] = b1r.x, xb_r[5] = b1r.y, xb_r[6] = b1r.z, xb_r[7] = b1r.w;
+        xb_i[0] = b0i.x, xb_i[1] = b0i.y, xb_i[2] = b0i.z, xb_i[3] = b0i.w;
+        xb_i[4] = b1i.x, xb_i[5] = b1i.y, xb_i[6] = b1i.z, xb_i[7] = b1i.w;
+    };
+    auto store_ab = [&]() {
+        h8 hr, lr, hi, li;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            _Float16 a, b;
+            split_f16x2(xa_r[q], a, b);
+            hr[q] = a, lr[q] = b;
+            split_f16x2(xa_i[q], a, b);
+            hi[q] = a, li[q] = b;
+        }
+        *(h8 *)&sa[0][ld_r][8 * ld_g] = hr;
+        *(h8 *)&sa[1][ld_r][8 * ld_g] = lr;
+        *(h8 *)&sa[2][ld_r][8 * ld_g] = hi;
+        *(h8 *)&sa[3][ld_r][8 * ld_g] = li;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            _Float16 a, b;
+            split_f16x2(xb_r[q], a, b);
+            hr[q] = a, lr[q] = b;
+            split_f16x2(xb_i[q], a, b);
+            hi[q] = a, li[q] = b;
+        }
+        *(h8 *)&sb[0][ld_r][8 * ld_g] = hr;
+        *(h8 *)&sb[1][ld_r][8 * ld_g] = lr;
+        *(h8 *)&sb[2][ld_r][8 * ld_g] = hi;
+        *(h8 *)&sb[3][ld_r][8 * ld_g] = li;
+    };
+    // re_m, im_m: hi hi terms; re_x, im_x: cross terms, in units of 2^-11
+    f4 re_m[2][2], re_x[2][2], im_m[2][2], im_x[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+            re_m[a][b] = re_x[a][b] = im_m[a][b] = im_x[a][b] = f4{0.f, 0.f, 0.f, 0.f};
+    load_ab(kbeg);
+    store_ab();
+    __syncthreads();
+    for (int chunk = 0; chunk < nk; ++chunk) {
+        if (chunk + 1 < nk)
+            load_ab(kbeg + kPanel * (chunk + 1)); // in flight during this chunk's products
+#pragma unroll
+        for (int a = 0; a < 2; ++a) {
+            h8 fa[4], na[2];
+#pragma unroll
+            for (int p = 0; p < 4; ++p)
+                fa[p] = *(const h8 *)&sa[p][wr * 32 + a * 16 + lrow][8 * lk];
+            na[0] = -fa[2]; // -hi, -lo of the imaginary part
+            na[1] = -fa[3];
+#pragma unroll
+            for (int b = 0; b < 2; ++b) {
+                h8 fb[4];
+#pragma unroll
+                for (int p = 0; p < 4; ++p)
+                    fb[p] = *(const h8 *)&sb[p][wc * 32 + b * 16 + lrow][8 * lk];
+                // Re: ar br - ai bi
+                re_m[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(fa[0], fb[0], re_m[a][b], 0, 0, 0);
+                re_m[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(na[0], fb[2], re_m[a][b], 0, 0, 0);
+                re_x[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(fa[0], fb[1], re_x[a][b], 0, 0, 0);
+                re_x[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(fa[1], fb[0], re_x[a][b], 0, 0, 0);
+                re_x[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(na[0], fb[3], re_x[a][b], 0, 0, 0);
+                re_x[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(na[1], fb[2], re_x[a][b], 0, 0, 0);
+                // Im: ar bi + ai br
+                im_m[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(fa[0], fb[2], im_m[a][b], 0, 0, 0);
+                im_m[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(fa[2], fb[0], im_m[a][b], 0, 0, 0);
+                im_x[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(fa[0], fb[3], im_x[a][b], 0, 0, 0);
+                im_x[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(fa[1], fb[2], im_x[a][b], 0, 0, 0);
+                im_x[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(fa[2], fb[1], im_x[a][b], 0, 0, 0);
+                im_x[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(fa[3], fb[0], im_x[a][b], 0, 0, 0);
+            }
+        }
+        if (chunk + 1 < nk) {
+            __syncthreads(); // this chunk's operands consumed
+            store_ab();
+            __syncthreads();
+        }
+    }
+    constexpr float kLo = 0x1p-11f;
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+            const int gi = r0 + wr * 32 + a * 16 + 4 * lk; // rows gi .. gi + 3
+            const int gj = c0 + wc * 32 + b * 16 + lrow;
+            if (gj >= nc || gi >= s)
+                continue;
+            const size_t o = (size_t)gj * lda + gi;
+            const float4 x = *(const float4 *)(ar + o), y = *(const float4 *)(ai + o);
+            const f4 cr = f4{x.x, x.y, x.z, x.w}, ci = f4{y.x, y.y, y.z, y.w};
+            f4 vr, vi;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                vr[q] = cr[q] - fmaf(re_x[a][b][q], kLo, re_m[a][b][q]);
+                vi[q] = ci[q] - fmaf(im_x[a][b][q], kLo, im_m[a][b][q]);
+            }
+            // rows past m (at most 3, inside the padded column) keep 0
+            *(float4 *)(ar + o) = make_float4(vr[0], vr[1], vr[2], vr[3]);
+            *(float4 *)(ai + o) = make_float4(vi[0], vi[1], vi[2], vi[3]);
+        }
+}
+
 // L11^{-1} of a 64-column outer step of the FP32 LU (L11 unit lower at
 // (P, P), after both panels' interchanges) into D.linv, column-major, real
 // then imaginary plane. Right-looking on X = I in LDS: at step j every
@@ -1050,8 +1249,12 @@ __global__ __launch_bounds__(256) void k_l11_inv(const LuDesc<float> *__restrict
 // the factors read P Z = L U.
 constexpr int kTrsm64Cols = 64;
 
+// k_lu_trsm64_f32 modes: the interchanges (left items, and the columns of
+// right items), the L11^{-1} product (right items), or both in one launch.
+constexpr int kTrsmSwap = 1, kTrsmSolve = 2;
+
 __global__ __launch_bounds__(256) void k_lu_trsm64_f32(const LuDesc<float> *__restrict__ descs,
-                                                       const int2 *__restrict__ work, int P)
+                                                       const int2 *__restrict__ work, int P, int mode)
 {
     typedef Mfma<float> MF;
     constexpr int NB = 2 * kPanel, LPA = NB + 16, LPB = NB + 4; // two workgroups per CU
@@ -1062,6 +1265,8 @@ __global__ __launch_bounds__(256) void k_lu_trsm64_f32(const LuDesc<float> *__re
         return;
     const bool left = wk.y < 0;
     if (!left && P + NB >= s)
+        return;
+    if (left && !(mode & kTrsmSwap))
         return;
     const int lda = D.lda, t = threadIdx.x;
     const int jbt = min(NB, s - P); // interchanges of this step
@@ -1092,8 +1297,10 @@ __global__ __launch_bounds__(256) void k_lu_trsm64_f32(const LuDesc<float> *__re
     }
     const int c0 = P + NB + wk.y * kTrsm64Cols;
     const int nc = min(kTrsm64Cols, D.ncols - c0);
-    if (any_swap && t < nc)
+    if ((mode & kTrsmSwap) && any_swap && t < nc)
         swap_rows(D.ar + (size_t)(c0 + t) * lda, D.ai + (size_t)(c0 + t) * lda);
+    if (!(mode & kTrsmSolve))
+        return;
     __syncthreads(); // the interchanged rows are visible to the workgroup
     // stage L11^{-1} ([k][row] from its column-major copy) and the A12 block
     // (rows P .. P + 63 of the columns c0 ..), one float4 of 4 rows per access
@@ -1208,10 +1415,20 @@ inline void lu_launch_trsm_f32(const LuDesc<float> *d, size_t off, size_t cnt, c
 }
 
 inline void lu_launch_trsm64_f32(const LuDesc<float> *d, size_t off, size_t cnt, const int *ints, int P,
-                                 hipStream_t st)
+                                 int mode, hipStream_t st)
 {
     hipLaunchKernelGGL(k_lu_trsm64_f32, dim3((unsigned)cnt), dim3(256), 0, st, d,
-                       (const int2 *)(ints + off), P);
+                       (const int2 *)(ints + off), P, mode);
+}
+
+// K = 128 trailing update (two outer steps' L21 and U12, columns / rows
+// [kbeg, kbeg + 128)) on the f16 matrix cores, 64 x 64 tiles.
+inline void lu_launch_update_f16x2(const LuDesc<float> *d, size_t off, size_t cnt, const int *ints, int kbeg,
+                                   int rbeg, int cbeg, int cend, hipStream_t st)
+{
+    const unsigned grid = (unsigned)((cnt + kXcds - 1) / kXcds * kXcds);
+    hipLaunchKernelGGL(k_lu_update_f16x2, dim3(grid), dim3(256), 0, st, d, (const int4 *)(ints + off), (int)cnt,
+                       kbeg, 4, rbeg, cbeg, cend);
 }
 
 inline void lu_launch_l11_inv(const LuDesc<float> *d, size_t off, size_t cnt, const int *ints, int P,

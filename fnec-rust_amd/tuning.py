@@ -17,6 +17,7 @@ import numpy as np
 
 SPACE = {
     "update_tile_m": (128, 64),
+    "update_k128": (0, 1),
     "gang_n": (0, 384, 512, 768, 1024, 1536),
     "gang_max": (2, 4, 8),
     "solve1_max_n": (1024, 2048, 4096, 8192),

@@ -259,6 +259,13 @@ int fnec_hip_debug_mfma_probe_f32(const float *a16x4, const float *b4x16, float 
  *                     update (k_lu_update_f32 work items): 64 (default; three
  *                     real MFMA products per complex multiply-add) or 128
  *                     (four). The factors differ at FP32 rounding level.
+ *   "update_k128"     1 (default): with update_tile_m 64, every other 64-column
+ *                     step leaves its trailing update to the next, which
+ *                     applies both with one K = 128 pass on the f16 matrix
+ *                     cores (each FP32 operand split into two f16 parts;
+ *                     product error about 2^-21 relative), reading and writing
+ *                     the trailing matrix once per 128 columns. 0: one K = 64
+ *                     FP32 pass per step.
  *   "mix_tol_exp"     mixed path: the CG stops when the normal-equation
  *                     residual is at most 10^-value of its right-hand side.
  *                     8 .. 15 (default 14). Not tuned automatically: it trades

@@ -32,6 +32,7 @@ def test_search_rejects_failing_start():
 def test_default_space_matches_engine_ranges():
     # values the engine accepts (fnec_hip_set_tunable ranges)
     assert set(A.SPACE["update_tile_m"]) <= {64, 128}
+    assert set(A.SPACE["update_k128"]) <= {0, 1}
     assert all(0 <= v <= 65536 for v in A.SPACE["gang_n"])
     assert all(1 <= v <= 64 for v in A.SPACE["gang_max"])
     assert all(0 <= v <= 8192 for v in A.SPACE["solve1_max_n"])

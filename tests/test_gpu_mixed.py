@@ -217,26 +217,76 @@ def test_mixed_with_row_interchanges():
 
 
 def test_update_tile_forms_meet_the_contract():
-    """The FP32 trailing update in both forms: 64-row tiles with three real
-    products per complex multiply-add (the default), 128-row tiles with four.
-    The factors differ at FP32 rounding level; both solves meet the mixed
-    path's gates against the plain FP64 solve, and each is deterministic."""
+    """The FP32 trailing update in its forms: 64-row tiles with three real
+    products per complex multiply-add, with K = 128 passes on the f16 matrix
+    cores for every other step (the default) or one K = 64 pass per step,
+    and 128-row tiles with four products (K = 64 passes). The factors differ
+    at rounding level; every solve meets the mixed path's gates against the
+    plain FP64 solve, and each form is deterministic."""
     probs = random_problems(14, [51, 200, 700, 1300, 2100])
     ref = F.solve_batch(probs, precision=F.PREC_FP64)
-    old = F.get_tunable("update_tile_m")
+    old = F.get_tunable("update_tile_m"), F.get_tunable("update_k128")
     try:
-        for tm in (64, 128):
+        for tm, k128 in ((64, 1), (64, 0), (128, 1)):
             F.set_tunable("update_tile_m", tm)
+            F.set_tunable("update_k128", k128)
             a = F.solve_batch(probs, precision=F.PREC_MIXED)
             b = F.solve_batch(probs, precision=F.PREC_MIXED)
             for p, r, x, y in zip(probs, ref, a, b):
                 assert x["status"] == F.OK
                 assert np.array_equal(x["x"], y["x"])
-                assert abs(x["z_in"] - r["z_in"]) <= 1e-6 * abs(r["z_in"]), (tm, len(p["segs"]))
+                assert abs(x["z_in"] - r["z_in"]) <= 1e-6 * abs(r["z_in"]), (tm, k128, len(p["segs"]))
                 assert rel(x["x"], r["x"]) <= 1e-7
                 assert x["stats"]["backward_error"] <= 1e-12
     finally:
-        F.set_tunable("update_tile_m", old)
+        F.set_tunable("update_tile_m", old[0])
+        F.set_tunable("update_k128", old[1])
+
+
+def interchange_deck(n):
+    """Two parallel wires of n segments half a radius long: kappa(Z) ~ 3e3,
+    so the FP32 LU of Z interchanges rows."""
+    a = 0.01
+    ln = n * a * 0.5
+    return (f"GW 1 {n} 0 0 {-ln / 2} 0 0 {ln / 2} {a}\nGW 2 {n} 0.05 0 {-ln / 2} 0.05 0 {ln / 2} {a}\n"
+            f"EX 0 1 {n // 2} 0 1 0\nFR 0 1 0 0 30 0\nEN\n")
+
+
+def test_k128_deferred_update_with_row_interchanges():
+    """Deferred K = 128 updates under row interchanges. With N = 400 (7 outer
+    steps: the last one alone) and N = 460 (8: the last one combining), the
+    interchanges of a combining step move rows that still lack the deferred
+    step's update into its U12 block and out of it. A factor that mishandled
+    them would cost the CG its rate or its convergence. Both forms meet the
+    gates against plain FP64 with a few iterations, and a batch of both
+    equals the single solves bitwise."""
+    probs = []
+    for n in (200, 230):
+        d = F.Deck(interchange_deck(n))
+        probs.append(d.problem("worker", d.frequencies("worker")[0]))
+    ref = [F.solve_problem(p, precision=F.PREC_FP64) for p in probs]
+    old = F.get_tunable("update_k128")
+    try:
+        its = {}
+        for k128 in (1, 0):
+            F.set_tunable("update_k128", k128)
+            batch = F.solve_batch(probs, precision=F.PREC_MIXED)
+            for p, x64, b in zip(probs, ref, batch):
+                n = len(p["segs"])
+                xm, st = F.solve_problem(p, precision=F.PREC_MIXED, with_stats=True)
+                assert np.array_equal(xm, b["x"]), (k128, n)
+                assert st["n_swaps"] > 0 and st["exact_panels"] > 0, (k128, n)
+                fi = int(p["feed_index"])
+                assert abs(x64[fi] - xm[fi]) / abs(x64[fi]) <= 1e-6, (k128, n)
+                assert rel(xm, x64) <= 1e-6, (k128, n)
+                assert st["backward_error"] <= 1e-12, (k128, n)
+                its[(k128, n)] = st["refine_iters"]
+                print(f"K=128 {k128}, N {n}: {st['n_swaps']} interchanges over {st['exact_panels']} panels, "
+                      f"{st['refine_iters']} CG iterations")
+        for n in (400, 460):
+            assert its[(1, n)] <= its[(0, n)] + 1 and its[(1, n)] <= 8, its
+    finally:
+        F.set_tunable("update_k128", old)
 
 
 def test_autotune_keeps_results_within_contract():
