@@ -915,6 +915,11 @@ static void lu_plan(std::vector<int> &ints, const std::vector<int> &ms, const st
 // Two-level plan of the FP32 LU (mixed path): 64-column outer steps made of
 // two 32-column panels, so that the trailing update runs with K = 64 and reads
 // and writes the trailing matrix once per 64 columns.
+// Trailing-update items run in groups of this many row tiles, column tiles
+// outer within a group: the group's L21 rows and each U12 column tile are
+// read from HBM about once per group instead of once per row tile.
+constexpr int kUpdGroupRows = 8;
+
 static void lu_plan_outer(std::vector<int> &ints, const std::vector<int> &ms, std::vector<Plan::Outer> &out,
                           std::vector<int4> &gensegs, size_t &gen_items, int tm, bool k128)
 {
@@ -1045,7 +1050,7 @@ static void lu_plan_outer(std::vector<int> &ints, const std::vector<int> &ms, st
                 continue;
             const int ny = (s - P - 2 * kPanel + tm - 1) / tm;
             const int nz = (s - P - 4 * kPanel + kTile32N - 1) / kTile32N;
-            gensegs.push_back(make_int4((int)gen_items, (int)b, ny, nz));
+            gensegs.push_back(make_int4((int)gen_items, (int)b, ny, nz | kUpdGroupRows << 24));
             gen_items += (size_t)ny * nz;
         }
         o.of_cnt = gen_items - o.of_off;

@@ -90,14 +90,26 @@ __device__ __forceinline__ void self_term(double h, double a, double k, double &
     im = si * h;
 }
 
-// Regular work lists from segment records (first item, problem, ny, nz):
-// items (problem, y, z, 0) in y-major order. One workgroup per segment.
+// Regular work lists from segment records (first item, problem, ny,
+// nz | gy << 24): items (problem, y, z, 0). gy = 0: y-major order. gy > 0:
+// groups of gy rows, each group in z-major order (z outer, y inner), so that
+// workgroups running together share their column operands as well as their
+// row operands in L2. One workgroup per segment.
 __global__ __launch_bounds__(256) void k_gen_items(const int4 *__restrict__ segs, int4 *__restrict__ out)
 {
     const int4 sg = segs[blockIdx.x];
-    const int total = sg.z * sg.w;
-    for (int e = threadIdx.x; e < total; e += 256)
-        out[(size_t)sg.x + e] = make_int4(sg.y, e / sg.w, e % sg.w, 0);
+    const int ny = sg.z, nz = sg.w & 0xffffff, gy = sg.w >> 24;
+    const int total = ny * nz;
+    for (int e = threadIdx.x; e < total; e += 256) {
+        int y = e / nz, z = e % nz;
+        if (gy > 0) {
+            const int g = e / (gy * nz), i = e - g * gy * nz;
+            const int rows = min(gy, ny - g * gy); // rows in this group
+            y = g * gy + i % rows;
+            z = i / rows;
+        }
+        out[(size_t)sg.x + e] = make_int4(sg.y, y, z, 0);
+    }
 }
 
 constexpr int kFillRows = 64;
