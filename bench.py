@@ -321,6 +321,8 @@ def measured_traffic(kernel, precision):
     tm = F.get_tunable("update_tile_m")
     if kernel == "lu_update" and e.get("symbol") != f"k_lu_update_f32<{tm}, 64, {'true' if tm == 64 else 'false'}>":
         return None  # measured with the other tile and product form
+    if kernel == "lu_update" and tm == 64 and F.get_tunable("update_k128") == 1:
+        return None  # measured with one K = 64 pass per step
     if kernel == "lu_update_k128" and e.get("symbol") != "k_lu_update_f16x2":
         return None
     return {"bytes_per_launch": e["hbm_bytes_per_launch"], "source": "profiles/serving_stream_traffic.json"}
