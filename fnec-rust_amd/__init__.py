@@ -478,8 +478,9 @@ def solve_batch(problems: list, precision: int = PREC_FP64, rp_grids: list | Non
 
 
 def autotune(problems: list, precision: int = PREC_MIXED, **kw) -> dict:
-    """Pick the engine's knobs (update tile, CG gang shape, solve size limit,
-    CG read-back chunk) by timing batched solves of `problems`; see
+    """Pick the engine's knobs (update tile and K = 128 passes, CG gang shape,
+    solve size limit, CG read-back chunk) by timing batched solves of
+    `problems`; see
     tuning.autotune for the options."""
     from .tuning import autotune as _autotune
 

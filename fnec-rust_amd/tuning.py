@@ -25,10 +25,12 @@ SPACE = {
 }
 
 
-def search(space: dict, start: dict, measure, rounds: int = 2):
+def search(space: dict, start: dict, measure, rounds: int = 2, min_gain: float = 0.0):
     """Coordinate search. measure(knobs) -> (seconds, ok); a candidate with
-    ok False is never chosen. Returns (best knobs, best seconds, trials) where
-    trials lists (knobs, seconds, ok) in the order measured."""
+    ok False is never chosen, and a candidate replaces the incumbent only when
+    it is faster by more than the fraction min_gain (timing noise), so that
+    knobs do not drift on noise. Returns (best knobs, best seconds, trials)
+    where trials lists (knobs, seconds, ok) in the order measured."""
     best = dict(start)
     t_best, ok = measure(best)
     if not ok:
@@ -47,7 +49,7 @@ def search(space: dict, start: dict, measure, rounds: int = 2):
                 t, ok = measure(cand)
                 seen[key] = t if ok else float("inf")
                 trials.append((dict(cand), t, ok))
-                if ok and t < t_best:
+                if ok and t < t_best * (1.0 - min_gain):
                     best, t_best, improved = cand, t, True
         if not improved:
             break
@@ -55,13 +57,15 @@ def search(space: dict, start: dict, measure, rounds: int = 2):
 
 
 def autotune(problems: list, precision: int | None = None, space: dict | None = None, repeats: int = 2,
-             rounds: int = 2, tol: float = 1e-6, apply: bool = True) -> dict:
+             rounds: int = 2, tol: float = 1e-6, apply: bool = True, min_gain: float = 0.02) -> dict:
     """Tune the engine's knobs on `problems` (Deck.problem dicts, or prebuilt
     fnec_request structures, solved as one batch). Each candidate is timed as
     the best of `repeats` solves (the C call only: the requests are built
     once). A candidate is rejected when any
     problem's Z_in moves by more than `tol` (relative) from the starting
-    setting's, or any status is not OK. With apply, the best knobs are left
+    setting's, or any status is not OK, and must beat the incumbent by more
+    than min_gain (2% by default: one batch's timing varies by about 1%
+    run to run). With apply, the best knobs are left
     set; otherwise the starting ones are restored. Returns {"knobs", "seconds",
     "start_seconds", "trials"}."""
     import importlib
@@ -111,7 +115,7 @@ def autotune(problems: list, precision: int | None = None, space: dict | None = 
         return best_t, ok_all
 
     try:
-        best, t_best, trials = search(space, start, measure, rounds)
+        best, t_best, trials = search(space, start, measure, rounds, min_gain)
     except Exception:
         for k, v in start.items():
             F.set_tunable(k, v)

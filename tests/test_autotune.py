@@ -17,6 +17,16 @@ def test_search_finds_separable_optimum():
     assert len(keys) == len(set(keys))
 
 
+def test_search_keeps_the_incumbent_within_the_noise_margin():
+    space = {"a": (1, 2, 3)}
+    cost = {1: 1.0, 2: 0.99, 3: 0.9}
+    best, t, _ = A.search(space, {"a": 1}, lambda k: (cost[k["a"]], True), min_gain=0.02)
+    assert best == {"a": 3} and t == pytest.approx(0.9)
+    cost[3] = 0.985  # every candidate within 2% of the start: no move
+    best, t, _ = A.search(space, {"a": 1}, lambda k: (cost[k["a"]], True), min_gain=0.02)
+    assert best == {"a": 1} and t == pytest.approx(1.0)
+
+
 def test_search_never_picks_a_rejected_candidate():
     space = {"a": (1, 2, 3)}
     best, t, trials = A.search(space, {"a": 1}, lambda k: (1.0 / k["a"], k["a"] != 3))
