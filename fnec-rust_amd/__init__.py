@@ -31,7 +31,7 @@ EXPORTS = [
     "fnec_hip_solve_hallen", "fnec_hip_solve_hallen_paths", "fnec_hip_rp",
     "fnec_hip_solve_batch", "fnec_hip_session_create", "fnec_hip_session_submit",
     "fnec_hip_session_poll", "fnec_hip_session_pending", "fnec_hip_session_destroy",
-    "fnec_hip_request_footprint", "fnec_hip_debug_normal_eqs", "fnec_hip_debug_mfma_probe",
+    "fnec_hip_request_footprint", "fnec_hip_debug_normal_eqs", "fnec_hip_debug_lu_f32", "fnec_hip_debug_mfma_probe",
     "fnec_hip_debug_mfma_probe_f32", "fnec_hip_set_tunable", "fnec_hip_get_tunable",
 ]
 
@@ -119,6 +119,7 @@ def lib():
         L.fnec_hip_request_footprint.argtypes = [sz, sz, sz, sz, i32]
         L.fnec_hip_debug_normal_eqs.argtypes = [vp, sz, vp, vp, vp, vp, sz, vp, vp, vp, sz, dbl,
                                                 C.POINTER(Ground), vp]
+        L.fnec_hip_debug_lu_f32.argtypes = [vp, sz, vp, vp, vp, vp, sz, vp, vp, vp, sz, dbl, vp, vp, vp]
         L.fnec_hip_debug_mfma_probe.argtypes = [vp, vp, vp]
         L.fnec_hip_debug_mfma_probe_f32.argtypes = [vp, vp, vp]
         L.fnec_hip_set_tunable.argtypes = [C.c_char_p, C.c_long]
@@ -234,6 +235,22 @@ def debug_normal_eqs(segs, rhs, cosv, wire_first=(), wire_last=(), ja=(), jb=(),
                                            len(wf), _p(a), _p(b), _p(sg), len(a), float(freq_hz),
                                            C.byref(g or ground()), _p(aug)))
     return aug
+
+
+def debug_lu_f32(p: dict):
+    """The mixed path's FP32 LU of Z for a Deck.problem dict: (packed factors
+    L\\U as complex64 n x n, ipiv int32 n) with P Z32 = L U."""
+    segs = np.ascontiguousarray(p["segs"], SEG)
+    n = len(segs)
+    wf, wl, a, b = _sz(p["wf"]), _sz(p["wl"]), _sz(p["ja"]), _sz(p["jb"])
+    sg = np.ascontiguousarray(np.asarray(p["js"], np.float64))
+    lu = np.zeros((n, n), np.complex64)
+    ipiv = np.zeros(n, np.int32)
+    _check(lib().fnec_hip_debug_lu_f32(_p(segs), n, _p(np.ascontiguousarray(p["rhs"], np.complex128)),
+                                       _p(np.ascontiguousarray(p["cos"], np.float64)), _p(wf), _p(wl), len(wf),
+                                       _p(a), _p(b), _p(sg), len(a), float(p["freq_hz"]),
+                                       C.byref(p["ground_struct"]), _p(lu), _p(ipiv)))
+    return lu, ipiv
 
 
 def debug_mfma_probe(a16x4: np.ndarray, b4x16: np.ndarray) -> np.ndarray:

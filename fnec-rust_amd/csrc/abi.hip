@@ -176,6 +176,23 @@ int fnec_hip_debug_normal_eqs(const fnec_seg *segs, size_t n, const fnec_c64 *rh
     return normal_eqs_only(hp, aug_out);
 }
 
+int fnec_hip_debug_lu_f32(const fnec_seg *segs, size_t n, const fnec_c64 *rhs, const double *cos_vec,
+                          const size_t *wire_first, const size_t *wire_last, size_t n_wires, const size_t *j_a,
+                          const size_t *j_b, const double *j_sign, size_t n_junc, double freq_hz,
+                          const fnec_ground *ground, float *lu_out, int32_t *ipiv_out)
+{
+    std::lock_guard<std::mutex> lk(global_mutex());
+    int st = ensure_ready();
+    if (st)
+        return st;
+    HostProblem hp;
+    st = problem_from_hallen(hp, segs, n, rhs, cos_vec, wire_first, wire_last, n_wires, j_a, j_b,
+                             j_sign, n_junc, freq_hz, ground);
+    if (st)
+        return st;
+    return lu_f32_only(hp, lu_out, ipiv_out);
+}
+
 int fnec_hip_debug_mfma_probe(const double *a16x4, const double *b4x16, double *lanes_out)
 {
     std::lock_guard<std::mutex> lk(global_mutex());

@@ -112,6 +112,9 @@ int solve_problems(const std::vector<const HostProblem *> &hps, std::vector<fnec
                    const std::function<void(const std::vector<double *> &)> &overlap = {});
 int fill_only(const HostProblem &hp, fnec_c64 *z_out);
 int normal_eqs_only(const HostProblem &hp, fnec_c64 *aug_out);
+// Fill and the mixed path's FP32 LU of Z only: the packed factors (n x n
+// row-major, re and im interleaved) and the interchange of each row.
+int lu_f32_only(const HostProblem &hp, float *lu_out, int *ipiv_out);
 int rp_host_from(RpHost &h, const fnec_seg *segs, const fnec_c64 *cur, size_t n, double freq_hz,
                  const fnec_ground *ground, const double *theta, const double *phi, size_t npts);
 // Far-field jobs in one batch: rp_stage uploads the inputs on the copy

@@ -692,6 +692,7 @@ struct Plan {
     std::vector<LuDesc<double>> lud;
     // mixed-precision path
     bool mixed = false;
+    bool lu_only = false; // mixed path: stop after the FP32 factorisation
     struct MixOff {
         size_t fr, fpr, rowcol, fh, lk, ym, ls, vr, vz, vp, vq, vt, vv, part, colsq, pcg, pstate;
         size_t perm, tws, tflags, dinv, gx, linv;
@@ -1711,6 +1712,8 @@ static int mixed_launch(Plan &pl, char *base)
     }
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(LN().ev[3], st));
+    if (pl.lu_only)
+        return FNEC_OK;
     const int4 *vsol = (const int4 *)(ints + pl.vsol_off);
     unsigned long long *tickets = (unsigned long long *)(base + pl.off_tickets);
     int ntrsv = 0; // launches of k_trsv so far: epoch ntrsv + 1, ticket counter ntrsv
@@ -2098,6 +2101,52 @@ int normal_eqs_only(const HostProblem &hp, fnec_c64 *aug_out)
             aug_out[(size_t)i * (S + 1) + j].re = ar[(size_t)j * lda + i];
             aug_out[(size_t)i * (S + 1) + j].im = ai[(size_t)j * lda + i];
         }
+    return FNEC_OK;
+}
+
+int lu_f32_only(const HostProblem &hp, float *lu_out, int *ipiv_out)
+{
+    LaneScope lane_scope;
+    if (lane_scope.status())
+        return lane_scope.status();
+    if (hp.w < 1 || hp.w > kMixMaxW || hp.ncons > kMixMaxCons) {
+        set_error("mixed precision accepts 1.." + std::to_string(kMixMaxW) + " homogeneous columns and at most " +
+                  std::to_string(kMixMaxCons) + " constraint rows");
+        return FNEC_INVALID_ARG;
+    }
+    Plan pl;
+    pl.mixed = true;
+    pl.lu_only = true;
+    std::vector<const HostProblem *> v{&hp};
+    plan_build(pl, v, true);
+    char *base = nullptr;
+    int st = arena_reserve(pl.bytes, &base);
+    if (st)
+        return st;
+    if ((st = plan_upload(pl, v, base, true)))
+        return st;
+    if ((st = plan_launch(pl, base, true)))
+        return st;
+    hipStream_t s = LN().stream;
+    const int n = hp.n, ldz = roundup(n, 64);
+    const size_t plane = (size_t)ldz * n;
+    std::vector<float> f(2 * plane);
+    std::vector<int> info(4), ipiv(n);
+    HIPCHK(hipMemcpyAsync(f.data(), base + pl.mix[0].fr, 4 * f.size(), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(ipiv.data(), base + pl.off_ipiv[0], 4 * (size_t)n, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(info.data(), base + pl.off_info[0], 4 * info.size(), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    if (info[0]) {
+        set_error("FP32 factorisation met a singular pivot");
+        return FNEC_SINGULAR;
+    }
+    for (int i = 0; i < n; ++i) {
+        ipiv_out[i] = ipiv[i];
+        for (int j = 0; j < n; ++j) {
+            lu_out[2 * ((size_t)i * n + j)] = f[(size_t)j * ldz + i];
+            lu_out[2 * ((size_t)i * n + j) + 1] = f[plane + (size_t)j * ldz + i];
+        }
+    }
     return FNEC_OK;
 }
 

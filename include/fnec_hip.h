@@ -235,6 +235,15 @@ int fnec_hip_debug_normal_eqs(const fnec_seg *segs, size_t n, const fnec_c64 *rh
                               const size_t *wire_last, size_t n_wires, const size_t *j_a,
                               const size_t *j_b, const double *j_sign, size_t n_junc,
                               double freq_hz, const fnec_ground *ground, fnec_c64 *aug_out);
+/* The mixed path's FP32 LU of Z for one solve_hallen problem, with the
+ * current tunables: the packed factors L\U (unit L below the diagonal; n x n
+ * row-major, real and imaginary parts interleaved) and ipiv (row i was
+ * interchanged with row ipiv[i], in order of i), so that P Z32 = L U for the
+ * FP32 copy Z32 of the fill. */
+int fnec_hip_debug_lu_f32(const fnec_seg *segs, size_t n, const fnec_c64 *rhs, const double *cos_vec,
+                          const size_t *wire_first, const size_t *wire_last, size_t n_wires,
+                          const size_t *j_a, const size_t *j_b, const double *j_sign, size_t n_junc,
+                          double freq_hz, const fnec_ground *ground, float *lu_out, int32_t *ipiv_out);
 int fnec_hip_debug_mfma_probe(const double *a16x4, const double *b4x16, double *lanes_out);
 /* The same probe for the FP32 MFMA (v_mfma_f32_16x16x4f32) used by the mixed path. */
 int fnec_hip_debug_mfma_probe_f32(const float *a16x4, const float *b4x16, float *lanes_out);

@@ -308,3 +308,47 @@ def test_autotune_keeps_results_within_contract():
     finally:
         for k, v in start.items():
             F.set_tunable(k, v)
+
+
+def lu_residual(p, lu, ipiv):
+    """||P Z32 - L U||_F / ||Z32||_F for packed factors, with P from the
+    interchanges in order and Z32 the FP32 rounding of the fill."""
+    z = F.fill_z(p["segs"], p["freq_hz"], p["ground_struct"]).astype(np.complex64).astype(np.complex128)
+    n = len(z)
+    perm = np.arange(n)
+    for i, r in enumerate(ipiv):
+        perm[i], perm[r] = perm[r], perm[i]
+    f = lu.astype(np.complex128)
+    low = np.tril(f, -1) + np.eye(n)
+    return np.linalg.norm(z[perm] - low @ np.triu(f)) / np.linalg.norm(z)
+
+
+def test_fp32_factor_reconstructs_z():
+    """The mixed path's FP32 factorisation itself, independent of the CG it
+    preconditions: P Z32 = L U to FP32 accuracy for each update form, K = 128
+    passes on the f16 matrix cores (default) and one K = 64 FP32 pass per
+    step. Decks whose factorisation takes interchanges (N = 400, an odd step
+    count; N = 460, even) and a random geometry without (N = 700)."""
+    probs = []
+    for n in (200, 230):
+        d = F.Deck(interchange_deck(n))
+        probs.append(d.problem("worker", d.frequencies("worker")[0]))
+    probs += random_problems(21, [700])
+    old = F.get_tunable("update_k128")
+    try:
+        for i, p in enumerate(probs):
+            n = len(p["segs"])
+            res = {}
+            for k128 in (1, 0):
+                F.set_tunable("update_k128", k128)
+                lu, ipiv = F.debug_lu_f32(p)
+                swaps = int((ipiv != np.arange(n)).sum())
+                res[k128] = lu_residual(p, lu, ipiv)
+                print(f"N {n}, K=128 {k128}: {swaps} interchanges, relative residual {res[k128]:.2e}")
+                # about 1e-7 measured; a missed or doubled update would leave O(1e-2)
+                assert res[k128] <= 1e-6, (n, k128, res[k128])
+                if i < 2:
+                    assert swaps > 0, n
+            assert res[1] <= 4.0 * res[0] + 1e-7, (n, res)
+    finally:
+        F.set_tunable("update_k128", old)
