@@ -1257,7 +1257,8 @@ __global__ __launch_bounds__(256) void k_lu_trsm64_f32(const LuDesc<float> *__re
                                                        const int2 *__restrict__ work, int P, int mode)
 {
     typedef Mfma<float> MF;
-    constexpr int NB = 2 * kPanel, LPA = NB + 16, LPB = NB + 4; // two workgroups per CU
+    // K in two chunks of 32 staged in turn: 37 KB of LDS, four workgroups per CU
+    constexpr int NB = 2 * kPanel, KC = NB / 2, LPA = NB + 16, LPB = NB + 4;
     const int2 wk = work[blockIdx.x];
     const LuDesc<float> &D = descs[wk.x];
     const int s = D.m;
@@ -1270,8 +1271,8 @@ __global__ __launch_bounds__(256) void k_lu_trsm64_f32(const LuDesc<float> *__re
         return;
     const int lda = D.lda, t = threadIdx.x;
     const int jbt = min(NB, s - P); // interchanges of this step
-    __shared__ float sa_r[NB][LPA], sa_i[NB][LPA]; // L11^{-1}: [k][row]
-    __shared__ float sb_r[NB][LPB], sb_i[NB][LPB]; // A12 block: [k][column]
+    __shared__ float sa_r[KC][LPA], sa_i[KC][LPA]; // L11^{-1}, one K chunk: [k][row]
+    __shared__ float sb_r[KC][LPB], sb_i[KC][LPB]; // A12 block, one K chunk: [k][column]
     __shared__ int piv_s[NB];
     if (t < NB)
         piv_s[t] = t < jbt ? D.ipiv[P + t] : P + t;
@@ -1302,31 +1303,34 @@ __global__ __launch_bounds__(256) void k_lu_trsm64_f32(const LuDesc<float> *__re
     if (!(mode & kTrsmSolve))
         return;
     __syncthreads(); // the interchanged rows are visible to the workgroup
-    // stage L11^{-1} ([k][row] from its column-major copy) and the A12 block
-    // (rows P .. P + 63 of the columns c0 ..), one float4 of 4 rows per access
+    // stage K chunk h of L11^{-1} ([k][row] from its column-major copy) and
+    // of the A12 block (rows P + 32 h .. + 31 of the columns c0 ..), one
+    // float4 of 4 rows per access
+    auto stage = [&](int h) {
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        const int e = t + 256 * i;       // 1024 float4 per plane
-        const int k = e >> 4, r4 = (e & 15) * 4;
-        *(float4 *)&sa_r[k][r4] = *(const float4 *)(D.linv + k * NB + r4);
-        *(float4 *)&sa_i[k][r4] = *(const float4 *)(D.linv + NB * NB + k * NB + r4);
-        const int c = e >> 4;            // column of the block, rows r4 .. r4 + 3
-        float4 br = make_float4(0.f, 0.f, 0.f, 0.f), bi = br;
-        if (c < nc) {
-            const size_t o = (size_t)(c0 + c) * lda + P + r4;
-            br = *(const float4 *)(D.ar + o);
-            bi = *(const float4 *)(D.ai + o);
+        for (int i = 0; i < 2; ++i) {
+            const int e = t + 256 * i;       // 512 float4 per plane and chunk
+            const int k = e >> 4, r4 = (e & 15) * 4;
+            const int kg = KC * h + k;
+            *(float4 *)&sa_r[k][r4] = *(const float4 *)(D.linv + kg * NB + r4);
+            *(float4 *)&sa_i[k][r4] = *(const float4 *)(D.linv + NB * NB + kg * NB + r4);
+            const int c = e >> 3, q4 = (e & 7) * 4; // column of the block, chunk rows q4 .. q4 + 3
+            float4 br = make_float4(0.f, 0.f, 0.f, 0.f), bi = br;
+            if (c < nc) {
+                const size_t o = (size_t)(c0 + c) * lda + P + KC * h + q4;
+                br = *(const float4 *)(D.ar + o);
+                bi = *(const float4 *)(D.ai + o);
+            }
+            sb_r[q4][c] = br.x;
+            sb_r[q4 + 1][c] = br.y;
+            sb_r[q4 + 2][c] = br.z;
+            sb_r[q4 + 3][c] = br.w;
+            sb_i[q4][c] = bi.x;
+            sb_i[q4 + 1][c] = bi.y;
+            sb_i[q4 + 2][c] = bi.z;
+            sb_i[q4 + 3][c] = bi.w;
         }
-        sb_r[r4][c] = br.x;
-        sb_r[r4 + 1][c] = br.y;
-        sb_r[r4 + 2][c] = br.z;
-        sb_r[r4 + 3][c] = br.w;
-        sb_i[r4][c] = bi.x;
-        sb_i[r4 + 1][c] = bi.y;
-        sb_i[r4 + 2][c] = bi.z;
-        sb_i[r4 + 3][c] = bi.w;
-    }
-    __syncthreads();
+    };
     const int lane = t & 63, wave = t >> 6, wr = wave >> 1, wc = wave & 1;
     const int lrow = lane & 15, lk = lane >> 4;
     // three-product (Gauss) form, as in k_lu_update_f32: p1 = Σ xr yr,
@@ -1340,8 +1344,13 @@ __global__ __launch_bounds__(256) void k_lu_trsm64_f32(const LuDesc<float> *__re
             p2[a][b] = MF::zero();
             p3[a][b] = MF::zero();
         }
+    for (int h = 0; h < 2; ++h) {
+    if (h)
+        __syncthreads(); // chunk 0 consumed
+    stage(h);
+    __syncthreads();
 #pragma unroll 4
-    for (int kk = 0; kk < NB; kk += 4) {
+    for (int kk = 0; kk < KC; kk += 4) {
         float xr[2], xi[2], xs[2], yr[2], yi[2], ys[2];
 #pragma unroll
         for (int a = 0; a < 2; ++a) {
@@ -1361,7 +1370,9 @@ __global__ __launch_bounds__(256) void k_lu_trsm64_f32(const LuDesc<float> *__re
                 p3[a][b] = MF::fma(xs[a], ys[b], p3[a][b]);
             }
     }
-    // the whole block was staged before any write: the product goes back in place
+    }
+    // both chunks of the block were read before any write: the product goes
+    // back in place
 #pragma unroll
     for (int a = 0; a < 2; ++a)
 #pragma unroll
