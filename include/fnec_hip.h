@@ -112,10 +112,11 @@ int fnec_hip_fill_z(const fnec_seg *segs, size_t n, double freq_hz, const fnec_g
  * n_wires == 0 means one range (0, n-1), as in the reference.
  * x_out: n + max(n_wires,1) values (currents, then one constant per wire).
  * precision: FNEC_PREC_FP64 (plain: FP64 Gram + LU of the normal equations,
- * the reference's algorithm) or FNEC_PREC_MIXED (FP32 MFMA LU of Z used to
- * precondition FP64 conjugate gradients on the same regularised normal
- * equations, converged to a relative residual of 1e-14; stats report the
- * iterations and the normwise backward error). stats may be NULL. */
+ * the reference's algorithm) or FNEC_PREC_MIXED (an FP32 LU of Z on the
+ * matrix cores, its trailing updates on f16 hi + lo operand pairs by default,
+ * used to precondition FP64 conjugate gradients on the same regularised
+ * normal equations, converged to a relative residual of 1e-14; stats report
+ * the iterations and the normwise backward error). stats may be NULL. */
 int fnec_hip_solve_hallen(const fnec_seg *segs, size_t n, const fnec_c64 *rhs, const double *cos_vec,
                           const size_t *wire_first, const size_t *wire_last, size_t n_wires,
                           const size_t *j_a, const size_t *j_b, const double *j_sign, size_t n_junc,
