@@ -1032,8 +1032,9 @@ __device__ __forceinline__ void split_f16x2(float x, _Float16 &hi, _Float16 &lo)
 // the solution it converges to. Operands must stay below 65504 in modulus
 // (the f16 range of hi): L21 is bounded by 1 through partial pivoting and
 // U12 by the pivot growth times max |Z|, whose Hallén entries are O(10). A
-// larger operand would turn the factor non-finite, and the CG then reports
-// a breakdown (FNEC_INTERNAL), never a wrong result. Per 16 x 16 tile and 32-deep K chunk: 12
+// larger operand would turn the factor non-finite, which the solve reports
+// as a failure status (a singular pivot or a CG breakdown), never as a
+// wrong result. Per 16 x 16 tile and 32-deep K chunk: 12
 // v_mfma_f32_16x16x32_f16 (hi hi and two cross terms for each of the four
 // real products of the complex product), where the FP32 forms issue 24 or
 // 32 v_mfma_f32_16x16x4f32 of the same issue cost each. A K = 64 pass is
