@@ -60,7 +60,13 @@ __device__ __forceinline__ void quad8(double ox, double oy, double oz, double sx
     for (int m = 0; m < 8; ++m) {
         const double sc = c_gl8n[m] * h;
         const double r2 = fma(sc, sc + B, A);
-        const double ir = rsqrt(r2), r = r2 * ir; // one reciprocal square root for R and 1/R
+        // one reciprocal square root for R and 1/R: the hardware estimate and
+        // one Newton step (r2 >= a^2 > 0 is finite and normal, so rsqrt()'s
+        // special-case handling is not needed; the step squares the
+        // estimate's relative error)
+        double ir = __builtin_amdgcn_rsq(r2);
+        ir = fma(0.5 * ir, fma(-r2 * ir, ir, 1.0), ir);
+        const double r = r2 * ir;
         double s, c;
         sincos_phase(k * r, s, c);
         double wr = c_gl8w[m] * ir;
