@@ -20,7 +20,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libfnec_hip.so")
 
 OK, SINGULAR, DIM_MISMATCH, UNAVAILABLE, RESOURCE_EXHAUSTED, INTERNAL, INVALID_ARG = range(7)
-PREC_FP64, PREC_MIXED = 0, 1
+PREC_FP64, PREC_MIXED, PREC_FP64_MFMA = 0, 1, 2
 GROUND = {"free": 0, "pec": 1, "finite": 2}
 
 SEG = np.dtype([("mid", np.float64, 3), ("dir", np.float64, 3), ("length", np.float64),
@@ -33,6 +33,7 @@ EXPORTS = [
     "fnec_hip_session_poll", "fnec_hip_session_pending", "fnec_hip_session_destroy",
     "fnec_hip_request_footprint", "fnec_hip_debug_normal_eqs", "fnec_hip_debug_lu_f32", "fnec_hip_debug_mfma_probe",
     "fnec_hip_debug_mfma_probe_f32", "fnec_hip_set_tunable", "fnec_hip_get_tunable",
+    "fnec_hip_debug_solve_z", "fnec_hip_debug_solve_square", "fnec_hip_debug_refmath",
 ]
 
 
@@ -121,6 +122,10 @@ def lib():
                                                 C.POINTER(Ground), vp]
         L.fnec_hip_debug_lu_f32.argtypes = [vp, sz, vp, vp, vp, vp, sz, vp, vp, vp, sz, dbl, vp, vp, vp]
         L.fnec_hip_debug_mfma_probe.argtypes = [vp, vp, vp]
+        L.fnec_hip_debug_solve_z.argtypes = [vp, sz, vp, vp, vp, vp, sz, vp, vp, vp, sz, vp,
+                                             C.POINTER(SolveStats)]
+        L.fnec_hip_debug_solve_square.argtypes = [vp, vp, sz, vp, vp, C.POINTER(SolveStats)]
+        L.fnec_hip_debug_refmath.argtypes = [i32, vp, vp, sz, vp]
         L.fnec_hip_debug_mfma_probe_f32.argtypes = [vp, vp, vp]
         L.fnec_hip_set_tunable.argtypes = [C.c_char_p, C.c_long]
         L.fnec_hip_get_tunable.argtypes = [C.c_char_p]
@@ -235,6 +240,46 @@ def debug_normal_eqs(segs, rhs, cosv, wire_first=(), wire_last=(), ja=(), jb=(),
                                            len(wf), _p(a), _p(b), _p(sg), len(a), float(freq_hz),
                                            C.byref(g or ground()), _p(aug)))
     return aug
+
+
+def debug_solve_z(z, rhs, cosv, wire_first=(), wire_last=(), ja=(), jb=(), js=()):
+    """The plain path (normal equations, elimination, back substitution) on a
+    caller-supplied Z (n x n): (x, stats dict)."""
+    z = np.ascontiguousarray(z, np.complex128)
+    n = z.shape[0]
+    wf, wl, a, b = _sz(wire_first), _sz(wire_last), _sz(ja), _sz(jb)
+    sg = np.ascontiguousarray(np.asarray(js, np.float64))
+    x = np.zeros(n + max(len(wf), 1), np.complex128)
+    st = SolveStats()
+    _check(lib().fnec_hip_debug_solve_z(_p(z), n, _p(np.ascontiguousarray(rhs, np.complex128)),
+                                        _p(np.ascontiguousarray(cosv, np.float64)), _p(wf), _p(wl), len(wf),
+                                        _p(a), _p(b), _p(sg), len(a), _p(x), C.byref(st)))
+    return x, st.as_dict()
+
+
+def debug_solve_square(a, b):
+    """The plain path's elimination + back substitution of a x = b:
+    (status, x, ipiv, stats dict); status SINGULAR leaves x unset."""
+    a = np.ascontiguousarray(a, np.complex128)
+    b = np.ascontiguousarray(b, np.complex128)
+    n = a.shape[0]
+    x = np.zeros(n, np.complex128)
+    ipiv = np.zeros(n, np.int32)
+    st = SolveStats()
+    rc = lib().fnec_hip_debug_solve_square(_p(a), _p(b), n, _p(x), _p(ipiv), C.byref(st))
+    if rc not in (OK, SINGULAR):
+        _check(rc)
+    return rc, x, ipiv, st.as_dict()
+
+
+def debug_refmath(fn: int, x, y=None) -> np.ndarray:
+    """The plain path's device sin (0), cos (1), hypot (2), sqrt (3), x / y (4),
+    and the sin (5) and cos (6) of sincos."""
+    x = np.ascontiguousarray(x, np.float64)
+    yy = None if y is None else np.ascontiguousarray(y, np.float64)
+    out = np.zeros_like(x)
+    _check(lib().fnec_hip_debug_refmath(int(fn), _p(x), _p(yy), x.size, _p(out)))
+    return out
 
 
 def debug_lu_f32(p: dict):

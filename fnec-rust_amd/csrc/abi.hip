@@ -16,7 +16,7 @@ namespace {
 
 int check_precision(int precision)
 {
-    if (precision != FNEC_PREC_FP64 && precision != FNEC_PREC_MIXED) {
+    if (precision != FNEC_PREC_FP64 && precision != FNEC_PREC_MIXED && precision != FNEC_PREC_FP64_MFMA) {
         set_error("unknown precision");
         return FNEC_INVALID_ARG;
     }
@@ -174,6 +174,52 @@ int fnec_hip_debug_normal_eqs(const fnec_seg *segs, size_t n, const fnec_c64 *rh
     if (st)
         return st;
     return normal_eqs_only(hp, aug_out);
+}
+
+int fnec_hip_debug_solve_z(const fnec_c64 *z, size_t n, const fnec_c64 *rhs, const double *cos_vec,
+                           const size_t *wire_first, const size_t *wire_last, size_t n_wires,
+                           const size_t *j_a, const size_t *j_b, const double *j_sign, size_t n_junc,
+                           fnec_c64 *x_out, fnec_solve_stats *stats)
+{
+    std::lock_guard<std::mutex> lk(global_mutex());
+    int st = ensure_ready();
+    if (st)
+        return st;
+    if (!z || !x_out || n == 0) {
+        set_error("null or empty input");
+        return FNEC_INVALID_ARG;
+    }
+    // the segments only size the problem: the fill does not run
+    std::vector<fnec_seg> segs(n, fnec_seg{{0.0, 0.0, 0.0}, {0.0, 0.0, 1.0}, 1.0, 1e-3});
+    HostProblem hp;
+    st = problem_from_hallen(hp, segs.data(), n, rhs, cos_vec, wire_first, wire_last, n_wires, j_a, j_b,
+                             j_sign, n_junc, 1.0, nullptr);
+    if (st)
+        return st;
+    return solve_given_z(hp, z, x_out, stats);
+}
+
+int fnec_hip_debug_solve_square(const fnec_c64 *a, const fnec_c64 *b, size_t n, fnec_c64 *x_out,
+                                int32_t *ipiv_out, fnec_solve_stats *stats)
+{
+    std::lock_guard<std::mutex> lk(global_mutex());
+    int st = ensure_ready();
+    if (st)
+        return st;
+    if (!a || !b || !x_out || n == 0 || n > (1u << 16)) {
+        set_error("null input or order out of range (1..65536)");
+        return FNEC_INVALID_ARG;
+    }
+    return solve_square_exact(a, b, (int)n, x_out, ipiv_out, stats);
+}
+
+int fnec_hip_debug_refmath(int fn, const double *x, const double *y, size_t n, double *out)
+{
+    std::lock_guard<std::mutex> lk(global_mutex());
+    int st = ensure_ready();
+    if (st)
+        return st;
+    return refmath_probe(fn, x, y, out, n);
 }
 
 int fnec_hip_debug_lu_f32(const fnec_seg *segs, size_t n, const fnec_c64 *rhs, const double *cos_vec,

@@ -402,6 +402,10 @@ int fnec_hip_solve_batch(const fnec_request *reqs, size_t count, int precision, 
         set_error("null request or result array");
         return FNEC_INVALID_ARG;
     }
+    if (precision != FNEC_PREC_FP64 && precision != FNEC_PREC_MIXED && precision != FNEC_PREC_FP64_MFMA) {
+        set_error("unknown precision");
+        return FNEC_INVALID_ARG;
+    }
     std::vector<const fnec_request *> rp(count);
     for (size_t i = 0; i < count; ++i)
         rp[i] = &reqs[i];
@@ -410,7 +414,7 @@ int fnec_hip_solve_batch(const fnec_request *reqs, size_t count, int precision, 
 
 fnec_session *fnec_hip_session_create(size_t budget_bytes, int precision, int max_batch)
 {
-    if (precision != FNEC_PREC_FP64 && precision != FNEC_PREC_MIXED) {
+    if (precision != FNEC_PREC_FP64 && precision != FNEC_PREC_MIXED && precision != FNEC_PREC_FP64_MFMA) {
         set_error("unknown precision");
         return nullptr;
     }

@@ -38,6 +38,7 @@ struct Prob {
     double gam_re;   // scalar reflection coefficient for ground == 2
     double gam_im;
     const double *seg;     // n * kSegStride
+    const double *selfln;  // n: 2 ln((h + R_end) / a) of each segment's self term (host libm)
     double *zr, *zi;       // Z, ldz x n
     double *ar, *ai;       // [G | M^H y], lda x (s + 1)
     const double *cosv;    // n

@@ -16,6 +16,7 @@ struct HostProblem {
     int ground = 0;
     double k = 0.0, freq = 0.0, gam_re = 0.0, gam_im = 0.0;
     std::vector<double> seg;   // n * 8
+    std::vector<double> selfln; // n: 2 ln((h + R_end) / a), the self term's analytic part
     std::vector<double> cosv;  // n
     std::vector<double> rhs;   // 2n
     std::vector<int> runs;     // 3 per run: row begin, row end, column
@@ -112,6 +113,10 @@ int solve_problems(const std::vector<const HostProblem *> &hps, std::vector<fnec
                    const std::function<void(const std::vector<double *> &)> &overlap = {});
 int fill_only(const HostProblem &hp, fnec_c64 *z_out);
 int normal_eqs_only(const HostProblem &hp, fnec_c64 *aug_out);
+int solve_given_z(const HostProblem &hp, const fnec_c64 *z, fnec_c64 *x_out, fnec_solve_stats *stats);
+int solve_square_exact(const fnec_c64 *a, const fnec_c64 *b, int n, fnec_c64 *x_out, int *ipiv_out,
+                       fnec_solve_stats *stats);
+int refmath_probe(int fn, const double *x, const double *y, double *out, size_t n);
 // Fill and the mixed path's FP32 LU of Z only: the packed factors (n x n
 // row-major, re and im interleaved) and the interchange of each row.
 int lu_f32_only(const HostProblem &hp, float *lu_out, int *ipiv_out);

@@ -54,6 +54,18 @@ __global__ void k_mix_step(const Prob *, const int *);
 __global__ void k_mix_flags(const Prob *, unsigned, int *);
 __global__ void k_mfma_probe32(const float *, const float *, float *);
 __global__ void k_mfma_probe(const double *, const double *, double *);
+// exact.hip (plain path in the reference's operation order)
+__global__ void k_fill_exact(const Prob *, const int4 *);
+__global__ void k_gx_tile(const Prob *, const int4 *);
+__global__ void k_gx_cols(const Prob *, const int2 *);
+__global__ void k_gx_fix(const Prob *, const int *);
+__global__ void k_ge_diag(const Prob *, const int *, int);
+__global__ void k_ge_rows(const Prob *, const int2 *, int);
+__global__ void k_ge_exact(const Prob *, const int *, int);
+__global__ void k_ge_u12(const Prob *, const int2 *, int);
+__global__ void k_ge_update(const Prob *, const int4 *, int);
+__global__ void k_backsub_exact(const Prob *, const int *);
+__global__ void k_refmath_probe(int, const double *, const double *, double *, int);
 
 // ---------------------------------------------------------------------------
 // context
@@ -461,6 +473,7 @@ static inline int roundup(int v, int m) { return (v + m - 1) / m * m; }
 // matrix.rs:285-290.
 static void csqrt_nc(double re, double im, double &sr, double &si)
 {
+#pragma clang fp contract(off)
     if (im == 0.0) {
         if (!std::signbit(re)) {
             sr = std::sqrt(re);
@@ -485,6 +498,7 @@ static void csqrt_nc(double re, double im, double &sr, double &si)
 
 static void fresnel_scalar(double freq, double eps_r, double sigma, double &gr, double &gi)
 {
+#pragma clang fp contract(off)
     double omega = 2.0 * kPi * freq;
     double er = std::isnan(eps_r) ? 1e-6 : std::max(eps_r, 1e-6);
     double sg = std::isnan(sigma) ? 0.0 : std::max(sigma, 0.0);
@@ -494,6 +508,21 @@ static void fresnel_scalar(double freq, double eps_r, double sigma, double &gr, 
     double ns = br * br + bi * bi;
     gr = (ar * br + ai * bi) / ns;
     gi = (ai * br - ar * bi) / ns;
+}
+
+// The self term's analytic part 2 ln((h + R_end) / a), R_end = sqrt(h^2 + a^2),
+// h = length / 2 (matrix.rs:302-316), one per segment with the host libm, in
+// the reference's operation order.
+static void self_log_terms(HostProblem &hp)
+{
+#pragma clang fp contract(off)
+    const size_t n = hp.seg.size() / kSegStride;
+    hp.selfln.resize(n);
+    for (size_t i = 0; i < n; ++i) {
+        const double half = hp.seg[i * kSegStride + 6] * 0.5, a = hp.seg[i * kSegStride + 7];
+        const double r_end = std::sqrt(half * half + a * a);
+        hp.selfln[i] = 2.0 * std::log((half + r_end) / a);
+    }
 }
 
 int build_problem(HostProblem &hp, const fnec_seg *segs, size_t n, const fnec_c64 *rhs,
@@ -543,6 +572,7 @@ int build_problem(HostProblem &hp, const fnec_seg *segs, size_t n, const fnec_c6
         o[6] = segs[i].length;
         o[7] = segs[i].radius;
     }
+    self_log_terms(hp);
     hp.cosv.assign(cos_vec, cos_vec + n);
     hp.rhs.resize(2 * n);
     for (size_t i = 0; i < n; ++i) {
@@ -677,6 +707,12 @@ struct Plan {
     size_t gen_items = 0, off_gen = 0, gensegs_off = 0;
     size_t gram_off = 0, gram_cnt = 0;
     size_t cols_off = 0, cols_cnt = 0;
+    // plain path in the reference's operation order (FNEC_PREC_FP64); the
+    // FP64 MFMA path (FNEC_PREC_FP64_MFMA) leaves it false
+    bool exact = false;
+    bool given_z = false;  // Z uploaded by the caller (debug entry): no fill launch
+    bool given_a = false;  // [A | b] uploaded by the caller: elimination only
+    size_t xcols_off = 0, xcols_cnt = 0;
     size_t all_off = 0, all_cnt = 0;
     size_t res_off = 0, res_cnt = 0;
     std::vector<LuStep> steps;
@@ -687,7 +723,7 @@ struct Plan {
     size_t bytes = 0;
     size_t a_begin = 0, a_end = 0; // byte range of the A planes (zeroed per run)
     std::vector<size_t> off_seg, off_z, off_a, off_cos, off_rhs, off_runs, off_cons, off_sign,
-        off_ipiv, off_info, off_x, off_stats, off_scr, off_panel, off_pmeta;
+        off_ipiv, off_info, off_x, off_stats, off_scr, off_panel, off_pmeta, off_selfln;
     size_t off_probs = 0, off_ints = 0, off_lud = 0;
     std::vector<LuDesc<double>> lud;
     // mixed-precision path
@@ -758,6 +794,7 @@ static int g_update_k128 = 1;    // K = 128 trailing-update passes (with 64-row 
 static int g_mix_tol_exp = 14; // CG stops at |h - (M^H M + lambda) x| <= 10^-exp |h|
 static int g_batch_split_n = 1024; // serving batches: N classes split at this size (0: one class)
 static int g_gang_n = 512;
+static int g_exact_backsub = 0; // plain path: back substitution in the reference's order
 static int g_gang_max = 4;
 static int gang_size(int n)
 {
@@ -1118,6 +1155,7 @@ static void plan_build(Plan &pl, const std::vector<const HostProblem *> &hps, bo
     pl.off_scr.resize(np);
     pl.off_panel.resize(np);
     pl.off_pmeta.resize(np);
+    pl.off_selfln.resize(np);
     pl.mix.clear();
     // A planes first, contiguous, so one zero-fill covers them
     pl.a_begin = 0;
@@ -1181,6 +1219,7 @@ static void plan_build(Plan &pl, const std::vector<const HostProblem *> &hps, bo
     for (size_t b = 0; b < np; ++b) {
         const HostProblem &h = *hps[b];
         pl.off_seg[b] = take(cur, (size_t)h.n * 8 * kSegStride);
+        pl.off_selfln[b] = take(cur, 8 * (size_t)h.n);
         pl.off_cos[b] = take(cur, 8 * (size_t)h.n);
         pl.off_rhs[b] = take(cur, 16 * (size_t)h.n);
         pl.off_runs[b] = take(cur, 4 * h.runs.size() + 4);
@@ -1379,6 +1418,12 @@ static void plan_build(Plan &pl, const std::vector<const HostProblem *> &hps, bo
                 push2((int)b, y);
         pl.cols_cnt = (pl.ints.size() - pl.cols_off) / 2;
         align4();
+        pl.xcols_off = pl.ints.size();
+        for (size_t b = 0; b < np; ++b)
+            for (int y = 0; y < (hps[b]->n + 255) / 256; ++y)
+                push2((int)b, y);
+        pl.xcols_cnt = (pl.ints.size() - pl.xcols_off) / 2;
+        align4();
         pl.all_off = pl.ints.size();
         for (size_t b = 0; b < np; ++b)
             pl.ints.push_back((int)b);
@@ -1454,6 +1499,7 @@ static int plan_upload(Plan &pl, const std::vector<const HostProblem *> &hps, ch
         P.gam_re = h.gam_re;
         P.gam_im = h.gam_im;
         P.seg = (const double *)(base + pl.off_seg[b]);
+        P.selfln = (const double *)(base + pl.off_selfln[b]);
         P.zr = (double *)(base + pl.off_z[b]);
         P.zi = P.zr + (size_t)P.ldz * h.n;
         if (with_solve && !pl.mixed) {
@@ -1556,6 +1602,7 @@ static int plan_upload(Plan &pl, const std::vector<const HostProblem *> &hps, ch
     for (size_t b = 0; b < np; ++b) {
         const HostProblem &h = *hps[b];
         put(pl.off_seg[b], h.seg.data(), 8 * h.seg.size());
+        put(pl.off_selfln[b], h.selfln.data(), 8 * h.selfln.size());
         put(pl.off_cos[b], h.cosv.data(), 8 * h.cosv.size());
         put(pl.off_rhs[b], h.rhs.data(), 8 * h.rhs.size());
         put(pl.off_runs[b], h.runs.data(), 4 * h.runs.size());
@@ -1825,16 +1872,84 @@ static int mixed_launch(Plan &pl, char *base)
 }
 
 // Launch the whole solve on the context stream; records phase events.
+// Plain path in the reference's operation order: normal equations (unless
+// the caller supplied [A | b]), elimination in 32-column panels, back
+// substitution. Events 2 and 3 bracket the elimination.
+static int exact_launch(Plan &pl, char *base)
+{
+    hipStream_t st = LN().stream;
+    const Prob *dp = (const Prob *)(base + pl.off_probs);
+    const int *all = wl_ptr(base, pl, pl.all_off);
+    if (!pl.given_a) {
+        hipEvent_t e = prof_begin(PC_GRAM);
+        if (pl.gram_cnt)
+            hipLaunchKernelGGL(k_gx_tile, dim3((unsigned)pl.gram_cnt), dim3(256), 0, st, dp,
+                               (const int4 *)wl_ptr(base, pl, pl.gram_off));
+        if (pl.xcols_cnt)
+            hipLaunchKernelGGL(k_gx_cols, dim3((unsigned)pl.xcols_cnt), dim3(256), 0, st, dp,
+                               (const int2 *)wl_ptr(base, pl, pl.xcols_off));
+        hipLaunchKernelGGL(k_gx_fix, dim3((unsigned)pl.all_cnt), dim3(64), 0, st, dp, all);
+        prof_end(PC_GRAM, e, pl.gram_flops, 0);
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(LN().ev[2], st));
+    const int *ints = wl_ptr(base, pl, 0);
+    for (size_t si = 0; si < pl.steps.size(); ++si) {
+        const LuStep &s = pl.steps[si];
+        if (s.panel_cnt) {
+            hipEvent_t e = prof_begin(PC_LU_PANEL);
+            hipLaunchKernelGGL(k_ge_diag, dim3((unsigned)s.panel_cnt), dim3(64), 0, st, dp,
+                               ints + s.panel_off, s.p);
+            if (s.rows_cnt)
+                hipLaunchKernelGGL(k_ge_rows, dim3((unsigned)s.rows_cnt), dim3(256), 0, st, dp,
+                                   (const int2 *)(ints + s.rows_off), s.p);
+            hipLaunchKernelGGL(k_ge_exact, dim3((unsigned)s.panel_cnt), dim3(1024), 0, st, dp,
+                               ints + s.panel_off, s.p);
+            prof_end(PC_LU_PANEL, e, pl.step_panel_flops[si], 0);
+        }
+        if (s.trsm_cnt) {
+            hipEvent_t e = prof_begin(PC_LU_TRSM);
+            hipLaunchKernelGGL(k_ge_u12, dim3((unsigned)s.trsm_cnt), dim3(64), 0, st, dp,
+                               (const int2 *)(ints + s.trsm_off), s.p);
+            prof_end(PC_LU_TRSM, e, pl.step_trsm_flops[si], 0);
+        }
+        if (s.upd_cnt) {
+            hipEvent_t e = prof_begin(PC_LU_UPDATE);
+            hipLaunchKernelGGL(k_ge_update, dim3((unsigned)s.upd_cnt), dim3(256), 0, st, dp,
+                               (const int4 *)(ints + s.upd_off), s.p);
+            prof_end(PC_LU_UPDATE, e, pl.step_upd_flops[si], 0);
+        }
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(LN().ev[3], st));
+    {
+        hipEvent_t e = prof_begin(PC_BACKSUB);
+        if (g_exact_backsub)
+            hipLaunchKernelGGL(k_backsub_exact, dim3((unsigned)pl.all_cnt), dim3(256), 0, st, dp, all);
+        else
+            hipLaunchKernelGGL(k_backsub, dim3((unsigned)pl.all_cnt), dim3(kBsThreads), 0, st, dp, all);
+        prof_end(PC_BACKSUB, e, pl.backsub_flops, 0);
+    }
+    if (!pl.given_a) {
+        hipLaunchKernelGGL(k_residual_rows, dim3((unsigned)pl.res_cnt), dim3(256), 0, st, dp,
+                           (const int2 *)wl_ptr(base, pl, pl.res_off));
+        hipLaunchKernelGGL(k_residual_finish, dim3((unsigned)pl.all_cnt), dim3(64), 0, st, dp, all);
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(LN().ev[4], st));
+    return FNEC_OK;
+}
+
 static int plan_launch(Plan &pl, char *base, bool with_solve)
 {
     hipStream_t st = LN().stream;
     const Prob *dp = (const Prob *)(base + pl.off_probs);
     HIPCHK(hipEventRecord(LN().ev[0], st));
     gen_launch(pl, base, st);
-    if (pl.fill_cnt) {
+    if (pl.fill_cnt && !pl.given_z && !pl.given_a) {
         hipEvent_t e = prof_begin(PC_FILL);
-        hipLaunchKernelGGL(k_fill_z, dim3((unsigned)pl.fill_cnt), dim3(256), 0, st, dp,
-                           (const int4 *)gen_ptr(base, pl) + pl.fill_off);
+        hipLaunchKernelGGL(pl.exact ? k_fill_exact : k_fill_z, dim3((unsigned)pl.fill_cnt), dim3(256), 0,
+                           st, dp, (const int4 *)gen_ptr(base, pl) + pl.fill_off);
         prof_end(PC_FILL, e, pl.fill_flops, 0);
     }
     HIPCHK(hipGetLastError());
@@ -1843,6 +1958,8 @@ static int plan_launch(Plan &pl, char *base, bool with_solve)
         return FNEC_OK;
     if (pl.mixed)
         return mixed_launch(pl, base);
+    if (pl.exact)
+        return exact_launch(pl, base);
     {
         size_t count = (pl.a_end - pl.a_begin) / 8;
         if (count)
@@ -1925,6 +2042,7 @@ int solve_problems(const std::vector<const HostProblem *> &hps, std::vector<fnec
             }
     Plan pl;
     pl.mixed = mixed;
+    pl.exact = precision == FNEC_PREC_FP64;
     // the work lists keep their capacity across batches
     std::vector<int> &ints_cache = LN().plan_ints;
     pl.ints.swap(ints_cache);
@@ -2040,6 +2158,7 @@ int fill_only(const HostProblem &hp, fnec_c64 *z_out)
     if (lane_scope.status())
         return lane_scope.status();
     Plan pl;
+    pl.exact = true;
     std::vector<const HostProblem *> v{&hp};
     plan_build(pl, v, false);
     char *base = nullptr;
@@ -2069,6 +2188,7 @@ int normal_eqs_only(const HostProblem &hp, fnec_c64 *aug_out)
     if (lane_scope.status())
         return lane_scope.status();
     Plan pl;
+    pl.exact = true;
     std::vector<const HostProblem *> v{&hp};
     plan_build(pl, v, true);
     char *base = nullptr;
@@ -2080,15 +2200,13 @@ int normal_eqs_only(const HostProblem &hp, fnec_c64 *aug_out)
     hipStream_t s = LN().stream;
     const Prob *dp = (const Prob *)(base + pl.off_probs);
     gen_launch(pl, base, s);
-    hipLaunchKernelGGL(k_fill_z, dim3((unsigned)pl.fill_cnt), dim3(256), 0, s, dp,
+    hipLaunchKernelGGL(k_fill_exact, dim3((unsigned)pl.fill_cnt), dim3(256), 0, s, dp,
                        (const int4 *)gen_ptr(base, pl) + pl.fill_off);
-    size_t count = (pl.a_end - pl.a_begin) / 8;
-    hipLaunchKernelGGL(k_zero, dim3(1024), dim3(256), 0, s, (double *)(base + pl.a_begin), count);
-    hipLaunchKernelGGL(k_gram, dim3((unsigned)pl.gram_cnt), dim3(256), 0, s, dp,
+    hipLaunchKernelGGL(k_gx_tile, dim3((unsigned)pl.gram_cnt), dim3(256), 0, s, dp,
                        (const int4 *)wl_ptr(base, pl, pl.gram_off));
-    hipLaunchKernelGGL(k_gram_cols, dim3((unsigned)pl.cols_cnt), dim3(256), 0, s, dp,
-                       (const int2 *)wl_ptr(base, pl, pl.cols_off));
-    hipLaunchKernelGGL(k_gram_fix, dim3((unsigned)pl.all_cnt), dim3(256), 0, s, dp,
+    hipLaunchKernelGGL(k_gx_cols, dim3((unsigned)pl.xcols_cnt), dim3(256), 0, s, dp,
+                       (const int2 *)wl_ptr(base, pl, pl.xcols_off));
+    hipLaunchKernelGGL(k_gx_fix, dim3((unsigned)pl.all_cnt), dim3(64), 0, s, dp,
                        wl_ptr(base, pl, pl.all_off));
     HIPCHK(hipGetLastError());
     const int S = hp.s, lda = roundup(S, 64);
@@ -2101,6 +2219,123 @@ int normal_eqs_only(const HostProblem &hp, fnec_c64 *aug_out)
             aug_out[(size_t)i * (S + 1) + j].re = ar[(size_t)j * lda + i];
             aug_out[(size_t)i * (S + 1) + j].im = ai[(size_t)j * lda + i];
         }
+    return FNEC_OK;
+}
+
+// Plain path on a caller-supplied Z (n x n row-major): normal equations,
+// elimination and back substitution of the problem hp, without the fill.
+int solve_given_z(const HostProblem &hp, const fnec_c64 *z, fnec_c64 *x_out, fnec_solve_stats *stats)
+{
+    LaneScope lane_scope;
+    if (lane_scope.status())
+        return lane_scope.status();
+    Plan pl;
+    pl.exact = true;
+    pl.given_z = true;
+    std::vector<const HostProblem *> v{&hp};
+    plan_build(pl, v, true);
+    char *base = nullptr;
+    int st = arena_reserve(pl.bytes, &base);
+    if (st)
+        return st;
+    if ((st = plan_upload(pl, v, base, true)))
+        return st;
+    const int n = hp.n, ldz = roundup(n, 64);
+    std::vector<double> zp(2 * (size_t)ldz * n, 0.0);
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j) {
+            zp[(size_t)j * ldz + i] = z[(size_t)i * n + j].re;
+            zp[(size_t)ldz * n + (size_t)j * ldz + i] = z[(size_t)i * n + j].im;
+        }
+    hipStream_t s = LN().stream;
+    HIPCHK(hipMemcpyAsync(base + pl.off_z[0], zp.data(), 8 * zp.size(), hipMemcpyHostToDevice, s));
+    if ((st = plan_launch(pl, base, true)))
+        return st;
+    std::vector<char> back(pl.io_end - pl.io_begin);
+    HIPCHK(hipMemcpyAsync(back.data(), base + pl.io_begin, back.size(), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    const int *info = (const int *)(back.data() + (pl.off_info[0] - pl.io_begin));
+    const double *sv = (const double *)(back.data() + (pl.off_stats[0] - pl.io_begin));
+    if (stats) {
+        *stats = fnec_solve_stats{};
+        stats->abs_res = sv[0];
+        stats->rel_res = sv[1];
+        stats->diag_spread = sv[2];
+        stats->n_swaps = info[1];
+        stats->exact_panels = info[2];
+        stats->rows = hp.r;
+        stats->unknowns = hp.s;
+    }
+    if (info[0])
+        return FNEC_SINGULAR;
+    std::memcpy(x_out, back.data() + (pl.off_x[0] - pl.io_begin), 16 * (size_t)hp.s);
+    return FNEC_OK;
+}
+
+// Plain elimination (partial pivoting in the reference's order) and back
+// substitution of a caller-supplied n x n system a x = b (a row-major).
+// ipiv_out[c] is the row interchanged with row c at elimination step c.
+int solve_square_exact(const fnec_c64 *a, const fnec_c64 *b, int n, fnec_c64 *x_out, int *ipiv_out,
+                       fnec_solve_stats *stats)
+{
+    LaneScope lane_scope;
+    if (lane_scope.status())
+        return lane_scope.status();
+    HostProblem hp;
+    hp.n = n;
+    hp.s = n;
+    hp.r = n;
+    hp.w = 0;
+    hp.seg.assign((size_t)n * kSegStride, 1.0);
+    hp.selfln.assign(n, 0.0);
+    hp.cosv.assign(n, 0.0);
+    hp.rhs.assign(2 * (size_t)n, 0.0);
+    hp.runs = {0, n, 0};
+    hp.k = 1.0;
+    Plan pl;
+    pl.exact = true;
+    pl.given_a = true;
+    std::vector<const HostProblem *> v{&hp};
+    plan_build(pl, v, true);
+    char *base = nullptr;
+    int st = arena_reserve(pl.bytes, &base);
+    if (st)
+        return st;
+    if ((st = plan_upload(pl, v, base, true)))
+        return st;
+    const int lda = roundup(n, 64);
+    const size_t plane = (size_t)lda * (n + 1);
+    std::vector<double> ap(2 * plane, 0.0);
+    for (int i = 0; i < n; ++i) {
+        for (int j = 0; j < n; ++j) {
+            ap[(size_t)j * lda + i] = a[(size_t)i * n + j].re;
+            ap[plane + (size_t)j * lda + i] = a[(size_t)i * n + j].im;
+        }
+        ap[(size_t)n * lda + i] = b[i].re;
+        ap[plane + (size_t)n * lda + i] = b[i].im;
+    }
+    hipStream_t s = LN().stream;
+    HIPCHK(hipMemcpyAsync(base + pl.off_a[0], ap.data(), 8 * ap.size(), hipMemcpyHostToDevice, s));
+    if ((st = plan_launch(pl, base, true)))
+        return st;
+    std::vector<char> back(pl.io_end - pl.io_begin);
+    HIPCHK(hipMemcpyAsync(back.data(), base + pl.io_begin, back.size(), hipMemcpyDeviceToHost, s));
+    std::vector<int> piv(n);
+    HIPCHK(hipMemcpyAsync(piv.data(), base + pl.off_ipiv[0], 4 * (size_t)n, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    const int *info = (const int *)(back.data() + (pl.off_info[0] - pl.io_begin));
+    if (stats) {
+        *stats = fnec_solve_stats{};
+        stats->n_swaps = info[1];
+        stats->exact_panels = info[2];
+        stats->rows = n;
+        stats->unknowns = n;
+    }
+    if (ipiv_out)
+        std::memcpy(ipiv_out, piv.data(), 4 * (size_t)n);
+    if (info[0])
+        return FNEC_SINGULAR;
+    std::memcpy(x_out, back.data() + (pl.off_x[0] - pl.io_begin), 16 * (size_t)n);
     return FNEC_OK;
 }
 
@@ -2389,6 +2624,34 @@ int rp_host_from(RpHost &h, const fnec_seg *segs, const fnec_c64 *cur, size_t n,
     return FNEC_OK;
 }
 
+int refmath_probe(int fn, const double *x, const double *y, double *out, size_t n)
+{
+    LaneScope lane_scope;
+    if (lane_scope.status())
+        return lane_scope.status();
+    if (fn < 0 || fn > 6 || !x || !out || ((fn == 2 || fn == 4) && !y) || n > ((size_t)1 << 26)) {
+        set_error("refmath probe: bad arguments");
+        return FNEC_INVALID_ARG;
+    }
+    char *base = nullptr;
+    const size_t bytes = 8 * n;
+    int st = arena_reserve(3 * bytes + 1024, &base);
+    if (st || n == 0)
+        return st;
+    hipStream_t s = LN().stream;
+    double *dx = (double *)base, *dy = (double *)(base + ((bytes + 255) & ~(size_t)255)),
+           *dout = (double *)(base + 2 * ((bytes + 255) & ~(size_t)255));
+    HIPCHK(hipMemcpyAsync(dx, x, bytes, hipMemcpyHostToDevice, s));
+    if (y)
+        HIPCHK(hipMemcpyAsync(dy, y, bytes, hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_refmath_probe, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, fn, dx,
+                       y ? dy : dx, dout, (int)n);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(out, dout, bytes, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    return FNEC_OK;
+}
+
 int mfma_probe(const double *a, const double *b, double *out)
 {
     LaneScope lane_scope;
@@ -2464,6 +2727,10 @@ int set_tunable(const char *name, long value)
         g_update_k128 = (int)value;
         return FNEC_OK;
     }
+    if (k == "exact_backsub" && (value == 0 || value == 1)) {
+        g_exact_backsub = (int)value;
+        return FNEC_OK;
+    }
     if (k == "lanes" && value >= 1 && value <= kMaxLanes) {
         std::lock_guard<std::mutex> lk(g_lane_mu);
         g_lanes_on = (int)value;
@@ -2495,6 +2762,8 @@ long get_tunable(const char *name)
         return g_batch_split_n;
     if (k == "mix_tol_exp")
         return g_mix_tol_exp;
+    if (k == "exact_backsub")
+        return g_exact_backsub;
     return -1;
 }
 

@@ -39,7 +39,16 @@ enum {
     FNEC_INVALID_ARG = 6,
 };
 
-enum { FNEC_PREC_FP64 = 0, FNEC_PREC_MIXED = 1 };
+/* FNEC_PREC_FP64       plain: the reference's algorithm and operation order in
+ *                      FP64, bit for bit (see fnec_hip_solve_hallen).
+ * FNEC_PREC_MIXED      FP32 LU of Z on the matrix cores preconditioning FP64
+ *                      conjugate gradients on the same normal equations.
+ * FNEC_PREC_FP64_MFMA  the same FP64 normal equations formed and factored on
+ *                      the FP64 matrix cores (blocked, FMA accumulation): faster
+ *                      than FNEC_PREC_FP64, but its summation order differs
+ *                      from the reference's, so on ill-conditioned normal
+ *                      equations its results move by about cond(G) * 1e-16. */
+enum { FNEC_PREC_FP64 = 0, FNEC_PREC_MIXED = 1, FNEC_PREC_FP64_MFMA = 2 };
 
 enum { FNEC_GROUND_FREE = 0, FNEC_GROUND_PEC = 1, FNEC_GROUND_FINITE = 2 };
 
@@ -111,8 +120,14 @@ int fnec_hip_fill_z(const fnec_seg *segs, size_t n, double freq_hz, const fnec_g
  * LU with partial pivoting. Z never leaves the device.
  * n_wires == 0 means one range (0, n-1), as in the reference.
  * x_out: n + max(n_wires,1) values (currents, then one constant per wire).
- * precision: FNEC_PREC_FP64 (plain: FP64 Gram + LU of the normal equations,
- * the reference's algorithm) or FNEC_PREC_MIXED (an FP32 LU of Z on the
+ * precision: FNEC_PREC_FP64 (plain: the fill, the normal equations and their
+ * Gaussian elimination in the reference's operation order, with the host
+ * libm's sin / cos / hypot reproduced on the device, so Z, the normal matrix
+ * and its factors equal the reference CPU path's bit for bit; the back
+ * substitution runs blocked, about 1e-14 relative from the reference's
+ * ascending-j sums, or in their order with the tunable "exact_backsub"),
+ * FNEC_PREC_FP64_MFMA (the same system on the FP64 matrix cores) or
+ * FNEC_PREC_MIXED (an FP32 LU of Z on the
  * matrix cores, its trailing updates on f16 hi + lo operand pairs by default,
  * used to precondition FP64 conjugate gradients on the same regularised
  * normal equations, converged to a relative residual of 1e-14; stats report
@@ -246,6 +261,25 @@ int fnec_hip_debug_lu_f32(const fnec_seg *segs, size_t n, const fnec_c64 *rhs, c
                           const size_t *j_a, const size_t *j_b, const double *j_sign, size_t n_junc,
                           double freq_hz, const fnec_ground *ground, float *lu_out, int32_t *ipiv_out);
 int fnec_hip_debug_mfma_probe(const double *a16x4, const double *b4x16, double *lanes_out);
+/* The plain path (FNEC_PREC_FP64) of fnec_hip_solve_hallen on a caller-supplied
+ * Z (n x n row-major, z[i*n+j] = A[i][j]) instead of the device fill: normal
+ * equations, elimination, back substitution and residual. Parity hook: on the
+ * oracle's Z it isolates the solve stages from the fill. */
+int fnec_hip_debug_solve_z(const fnec_c64 *z, size_t n, const fnec_c64 *rhs, const double *cos_vec,
+                           const size_t *wire_first, const size_t *wire_last, size_t n_wires,
+                           const size_t *j_a, const size_t *j_b, const double *j_sign, size_t n_junc,
+                           fnec_c64 *x_out, fnec_solve_stats *stats);
+/* The plain path's elimination and back substitution on a caller-supplied
+ * square system a x = b (a: n x n row-major), as solve_square_in_place does
+ * it (linear.rs:1321-1383): ipiv_out[c] (may be NULL) is the row interchanged
+ * with row c at elimination step c; FNEC_SINGULAR when a pivot's modulus is
+ * below 1e-30. stats (may be NULL) gets n_swaps and exact_panels. */
+int fnec_hip_debug_solve_square(const fnec_c64 *a, const fnec_c64 *b, size_t n, fnec_c64 *x_out,
+                                int32_t *ipiv_out, fnec_solve_stats *stats);
+/* The plain path's device elementary functions over an array: fn 0 sin(x),
+ * 1 cos(x), 2 hypot(x, y), 3 sqrt(x), 4 x / y, 5 and 6 the sine and cosine
+ * of sincos(x) (y may be NULL except for 2 and 4). */
+int fnec_hip_debug_refmath(int fn, const double *x, const double *y, size_t n, double *out);
 /* The same probe for the FP32 MFMA (v_mfma_f32_16x16x4f32) used by the mixed path. */
 int fnec_hip_debug_mfma_probe_f32(const float *a16x4, const float *b4x16, float *lanes_out);
 
@@ -287,6 +321,10 @@ int fnec_hip_debug_mfma_probe_f32(const float *a16x4, const float *b4x16, float 
  *                     factorisation; with a full batch queued, batches are
  *                     formed across classes. 0 .. 65536 (default 1024; 0: one
  *                     class).
+ *   "exact_backsub"   plain path: 1 runs the back substitution in the
+ *                     reference's order (ascending j in every row sum; a
+ *                     serial chain of S^2/2 subtractions), so x equals the
+ *                     reference's bit for bit; 0 (default) runs it blocked.
  *   "lanes"           batches that may run on the device at once (each on a
  *                     lane: two streams, a device arena, pinned staging);
  *                     also the worker count of a session created afterwards.

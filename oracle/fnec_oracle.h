@@ -102,6 +102,8 @@ cx or_fill_elem(const or_seg *obs, const or_seg *src, double k, int is_self);
 
 /* ---- linear (linear.rs) ---- */
 int or_solve_square(const cx *a, const cx *b, int n, cx *x);
+/* the same, recording the pivot row chosen at each elimination column */
+int or_solve_square_piv(const cx *a, const cx *b, int n, cx *x, int *piv);
 int or_solve_hallen(const cx *z, int n, const cx *rhs, const double *cosv, const int *wf,
                     const int *wl, int nw, const int *ja, const int *jb, const double *js, int nj,
                     cx *x /* n + max(nw,1) */);
@@ -127,6 +129,9 @@ double or_radiation_efficiency(const or_seg *s, int n, const cx *cur, double fre
 
 /* ---- solve_session.rs helpers ---- */
 double or_diag_spread(const cx *z, int n);
+/* host libm over arrays: fn 0 sin(x), 1 cos(x), 2 hypot(x, y), 5 and 6 the sin
+ * and cos of sincos(x) (libm.c) */
+void or_libm_eval(int fn, const double *x, const double *y, size_t n, double *out);
 void or_residual_hallen(const cx *z, int n, const cx *cur, const cx *chom, int nchom,
                         const double *cosv, const cx *rhs, const int *wf, const int *wl, int nw,
                         double *abs_res, double *rel_res);

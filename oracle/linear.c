@@ -32,8 +32,9 @@ static void rows_free(cx **m, int r)
     free(m);
 }
 
-/* Elimination on [A | b] held as row vectors; consumes `aug`. */
-static int solve_aug(cx **aug, int n, cx *x)
+/* Elimination on [A | b] held as row vectors; consumes `aug`. piv (may be
+ * NULL) receives the pivot row chosen at each column. */
+static int solve_aug_piv(cx **aug, int n, cx *x, int *pivots)
 {
     int col, row, k, i, j;
     for (col = 0; col < n; col++) {
@@ -49,6 +50,8 @@ static int solve_aug(cx **aug, int n, cx *x)
                 piv = row;
             }
         }
+        if (pivots)
+            pivots[col] = piv;
         if (cx_norm(aug[piv][col]) < 1e-30)
             return OR_SINGULAR;
         if (piv != col) {
@@ -75,12 +78,14 @@ static int solve_aug(cx **aug, int n, cx *x)
     return OR_OK;
 }
 
+static int solve_aug(cx **aug, int n, cx *x) { return solve_aug_piv(aug, n, x, NULL); }
+
 /* Iterator::max_by with partial_cmp(...).unwrap_or(Equal): the fold keeps
  * the later element unless the earlier one is strictly greater. The loop
  * above implements "replace when !(v < best)". For NaN, partial_cmp returns
  * None -> Equal -> later element kept, which `!(v < best)` also does. */
 
-int or_solve_square(const cx *a, const cx *b, int n, cx *x)
+int or_solve_square_piv(const cx *a, const cx *b, int n, cx *x, int *piv)
 {
     cx **aug = rows_alloc(n, n + 1);
     int i, st;
@@ -88,10 +93,12 @@ int or_solve_square(const cx *a, const cx *b, int n, cx *x)
         memcpy(aug[i], a + (size_t)i * (size_t)n, sizeof(cx) * (size_t)n);
         aug[i][n] = b[i];
     }
-    st = solve_aug(aug, n, x);
+    st = solve_aug_piv(aug, n, x, piv);
     rows_free(aug, n);
     return st;
 }
+
+int or_solve_square(const cx *a, const cx *b, int n, cx *x) { return or_solve_square_piv(a, b, n, x, NULL); }
 
 /* Build M and y of the augmented Hallén system (linear.rs:688-762). */
 static int build_m(const cx *z, int n, const double *cosv, const cx *rhs, const int *wf,

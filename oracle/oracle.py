@@ -97,6 +97,8 @@ def lib():
         L.or_fill_elem.argtypes = [dp, dp, C.c_double, C.c_int]
         L.or_fill_elem.restype = CX
         L.or_solve_square.argtypes = [dp, dp, C.c_int, dp]
+        L.or_solve_square_piv.argtypes = [dp, dp, C.c_int, dp, dp]
+        L.or_libm_eval.argtypes = [C.c_int, dp, dp, C.c_size_t, dp]
         L.or_solve_hallen.argtypes = [dp, C.c_int, dp, dp, dp, dp, C.c_int, dp, dp, dp, C.c_int, dp]
         L.or_hallen_normal_eqs.argtypes = [dp, C.c_int, dp, dp, dp, dp, C.c_int, dp, dp, dp, C.c_int, dp, dp, dp]
         L.or_radiation_pattern.argtypes = [dp, C.c_int, dp, C.c_double, Ground, dp, dp, C.c_int, dp]
@@ -173,6 +175,26 @@ def fill_z(segs: np.ndarray, freq_hz: float, g: Ground | None = None) -> np.ndar
     z = np.zeros((n, n), np.complex128)
     lib().or_fill_z(_p(segs), n, float(freq_hz), g or ground(), z.ctypes.data_as(C.c_void_p))
     return z
+
+
+def solve_square_piv(a: np.ndarray, b: np.ndarray) -> tuple[int, np.ndarray, np.ndarray]:
+    """solve_square_in_place with the pivot row chosen at each column."""
+    a = np.ascontiguousarray(a, np.complex128)
+    b = np.ascontiguousarray(b, np.complex128)
+    x = np.zeros(len(b), np.complex128)
+    piv = np.full(len(b), -1, np.int32)
+    st = lib().or_solve_square_piv(_p(a.ravel()), _p(b), len(b), _p(x), _p(piv))
+    return st, x, piv
+
+
+def libm(fn: int, x: np.ndarray, y: np.ndarray | None = None) -> np.ndarray:
+    """The host libm over an array: sin (0), cos (1), hypot(x, y) (2), and the
+    sin (5) and cos (6) of sincos(x)."""
+    x = np.ascontiguousarray(x, np.float64)
+    yy = None if y is None else np.ascontiguousarray(y, np.float64)
+    out = np.zeros_like(x)
+    lib().or_libm_eval(int(fn), _p(x), _p(yy), x.size, _p(out))
+    return out
 
 
 def solve_square(a: np.ndarray, b: np.ndarray) -> tuple[int, np.ndarray]:

@@ -3,10 +3,10 @@ and many independent geometries of uneven N, each solved in one batched call
 in FP64 and in mixed precision.
 
 - FP64 points against the oracle (the C restatement of the reference CPU
-  path), judged like the single-deck parity tests: against the exact
-  (extended-precision) solution of the same system, the GPU's error is at
-  most max(1e-9, 2x the reference's), and GPU vs reference differ by at most
-  max(1e-9, 1.5x the sum of both);
+  path): currents and Z_in within 1e-12, unrelaxed (the contract is 1e-9),
+  including the resonance of the 4a sweep (13.68 and 14.05 MHz), where the
+  normal equations are ill-conditioned and a change of summation order
+  alone moves the answer by 1e-7;
 - every mixed point within relative dZin 1e-6 of the FP64 point, or, where
   the FP64 point is the less accurate one, of the oracle, with normwise
   backward error <= 1e-12.
@@ -38,23 +38,24 @@ def check_against_oracle(oracle, txt, p, x, f):
     xe = exact_solution(oracle, inp, z)
     e_ref = rel(r["currents"], xe[:n])
     e_gpu = rel(x[:n], xe[:n])
-    assert e_gpu < max(1e-9, 2.0 * e_ref), (f, e_gpu, e_ref)
-    bar = max(1e-9, 1.5 * (e_gpu + e_ref))
-    assert rel(x[:n], r["currents"]) < bar, f
+    assert rel(x[:n], r["currents"]) < 1e-12, (f, rel(x[:n], r["currents"]))
     fi = int(p["feed_index"])
     zin = p["v_feed"] / x[fi]
-    assert abs(zin - r["feed_z"][0]) / abs(r["feed_z"][0]) < bar, f
+    assert abs(zin - r["feed_z"][0]) / abs(r["feed_z"][0]) < 1e-12, f
     return e_gpu, e_ref
 
 
 def check_mixed_vs_fp64(oracle, texts, probs, b64, bm):
-    """Mixed Z_in within 1e-6 of the FP64 GPU point. Where the FP64 point is
-    itself further than that from the exact solution (near a resonance of an
-    array the least-squares residual is large and the normal equations are
-    ill-conditioned: at 14.05 MHz in the 4a sweep the FP64 GPU Z_in is 1.5e-6
-    from exact, the reference's 1.3e-7, the mixed one 6e-10), the mixed point
-    is judged against the oracle (the FP64 reference path, 1e-6) and the exact
-    solution (at least as close as the FP64 GPU point)."""
+    """Mixed Z_in within 1e-6 of the FP64 point (the plain path, which rounds
+    as the reference does). Near a resonance of an array the least-squares
+    residual is large and the normal equations are ill-conditioned: there the
+    reference's FP64 elimination itself is up to 1.3e-6 from the exact
+    (extended-precision) solution (13.92 MHz in the 4a sweep), while the mixed
+    path, which never forms the normal matrix, stays within 1e-9 of it. A
+    point more than 1e-6 from the FP64 point is therefore judged against the
+    exact solution: the mixed Z_in within 1e-6 of it and no further from it
+    than the FP64 point, and the distance to the FP64 point explained by the
+    FP64 point's own error."""
     worst = 0.0
     for t, p, a, b in zip(texts, probs, b64, bm):
         assert a["status"] == F.OK and b["status"] == F.OK
@@ -66,7 +67,6 @@ def check_mixed_vs_fp64(oracle, texts, probs, b64, bm):
         f = p["freq_hz"]
         r = oracle.run_deck(t, "worker", 0, f)
         dz_ref = abs(b["z_in"] - r["feed_z"][0]) / abs(r["feed_z"][0])
-        assert dz_ref <= 1e-6, (len(p["segs"]), f, dz, dz_ref)
         inp = oracle.deck_inputs(t, f, "worker")
         kind, er, sg = inp["ground"]
         z = oracle.fill_z(inp["segs"], f, oracle.Ground(kind if kind != 3 else 0, er, sg))
@@ -75,10 +75,11 @@ def check_mixed_vs_fp64(oracle, texts, probs, b64, bm):
         ze = p["v_feed"] / xe[fi]
         e_mixed = abs(b["z_in"] - ze) / abs(ze)
         e_fp64 = abs(a["z_in"] - ze) / abs(ze)
-        print(f"{f / 1e6:.3f} MHz (N {len(p['segs'])}): mixed vs FP64 GPU {dz:.1e}, vs reference "
-              f"{dz_ref:.1e}; Z_in error vs exact: mixed {e_mixed:.1e}, FP64 GPU {e_fp64:.1e}")
-        assert e_mixed <= max(e_fp64, 1e-9)
-        worst = max(worst, dz_ref)
+        e_ref = abs(r["feed_z"][0] - ze) / abs(ze)
+        print(f"{f / 1e6:.3f} MHz (N {len(p['segs'])}): mixed vs FP64 {dz:.1e}, vs reference {dz_ref:.1e}; "
+              f"Z_in error vs exact: mixed {e_mixed:.1e}, FP64 {e_fp64:.1e}, reference {e_ref:.1e}")
+        assert e_mixed <= 1e-6 and e_mixed <= max(e_fp64, 1e-9)
+        assert dz <= (e_fp64 + e_mixed) * (1 + 1e-6)
     return worst
 
 
@@ -92,9 +93,10 @@ def test_config4a_frequency_sweep_batch(oracle):
     assert len(probs) == 256 and all(len(p["segs"]) == 1024 for p in probs)
     b64 = F.solve_batch(probs, precision=F.PREC_FP64)
     bm = F.solve_batch(probs, precision=F.PREC_MIXED)
-    for i in (128,):  # the oracle takes ~30 s per 1024-segment point
+    for i in (128, 168, 171):  # 168 and 171: the array's resonance, ill-conditioned
         e_gpu, e_ref = check_against_oracle(oracle, txt, probs[i], b64[i]["x"], fs[i])
-        print(f"{fs[i] / 1e6:.2f} MHz: GPU err {e_gpu:.1e}, reference err {e_ref:.1e}")
+        print(f"{fs[i] / 1e6:.2f} MHz: GPU vs reference < 1e-12; error vs exact: GPU {e_gpu:.1e}, "
+              f"reference {e_ref:.1e}")
     worst = check_mixed_vs_fp64(oracle, [txt] * len(probs), probs, b64, bm)
     print(f"mixed vs FP64 over the sweep: max rel dZin {worst:.2e}")
     # a batch point is the single solve of that point, bitwise

@@ -1,13 +1,13 @@
 """GPU parity of the HIP path against the oracle, through the C ABI.
 
 Tolerances (relative unless stated):
-- Z fill elements: 1e-12 (FP64 quadrature; sincos/sqrt/div rounding only).
-- Gram [G | M^H y]: 1e-12 normwise (summation order differs from the oracle).
-- Currents / Z_in: 1e-9 (the plain-FP64 accuracy contract). The attainable floor is
-  kappa(G) * eps for any change of summation order: kappa(G) ~ 5e6 for the
-  51-segment dipole and ~2e8 at N = 2048, i.e. 1e-10 .. 5e-10; decks whose own
-  reference output carries ~1e-8 rounding noise (two mirror-symmetric
-  sources) are gated at 1e-7 and say so.
+- Z fill elements and [G | M^H y]: 1e-12 here; test_gpu_exact.py holds the
+  plain path to bitwise equality with the oracle.
+- Currents / Z_in of the plain path: 1e-12 against the oracle, unrelaxed (the
+  contract is 1e-9). The plain path forms and eliminates the normal equations
+  in the reference's operation order, so only its blocked back substitution
+  differs (about 1e-15); the distance to the exact (extended-precision)
+  solution is printed for information, reference and GPU alike.
 - Pattern gains: 1e-9 on the linear directivity u * 4 pi / total, with the
   -999.99 sentinel status identical.
 - Plain FP64 path: bitwise run-to-run and alone-vs-batched.
@@ -132,14 +132,11 @@ def exact_solution(oracle, inp, z):
 
 @pytest.mark.parametrize("name,sem", SOLVE_CASES)
 def test_solve_matches_oracle(oracle, name, sem):
-    """Accuracy parity with the reference, judged against the exact
-    (extended-precision) solution of the same system:
-    - the GPU's own error is at most max(1e-9, 2x the reference's error);
-    - GPU vs reference differ by at most max(1e-9, 1.5x the sum of both).
-    For the 51-segment dipole decks the reference's own error is ~1e-10, so
-    this is the 1e-9 bar; for ill-conditioned decks (yagi-5elm kappa(G) =
-    2.9e8, multi-source 4.6e8, dipole-loaded 3.7e10) the reference itself is
-    5e-9 .. 2e-8 away from the exact answer and the bar follows it."""
+    """Currents and Z_in within 1e-12 of the reference, unrelaxed, on every
+    deck, also where the normal equations are ill-conditioned (yagi-5elm
+    kappa(G) = 2.9e8, multi-source 4.6e8, dipole-loaded 3.7e10: there the
+    reference itself is 5e-9 .. 2e-8 from the exact answer, and the GPU sits
+    at the same distance, because it rounds as the reference does)."""
     txt = deck_text(name)
     r = oracle.run_deck(txt, sem)
     n = r["n"]
@@ -151,12 +148,12 @@ def test_solve_matches_oracle(oracle, name, sem):
     x, st = F.solve_problem(p, with_stats=True)
     e_ref = rel(r["currents"], xe[:n])
     e_gpu = rel(x[:n], xe[:n])
-    assert e_gpu < max(1e-9, 2.0 * e_ref), (e_gpu, e_ref)
-    bar = max(1e-9, 1.5 * (e_gpu + e_ref))
-    assert rel(x[:n], r["currents"]) < bar
+    d = rel(x[:n], r["currents"])
+    print(f"{name} ({sem}): GPU vs reference {d:.1e}; error vs exact: GPU {e_gpu:.1e}, reference {e_ref:.1e}")
+    assert d < 1e-12
     for k, idx in enumerate(r["feed_idx"]):
         zin_g = r["feed_v"][k] / x[idx]
-        assert abs(zin_g - r["feed_z"][k]) / abs(r["feed_z"][k]) < bar
+        assert abs(zin_g - r["feed_z"][k]) / abs(r["feed_z"][k]) < 1e-12
     assert abs(st["abs_res"] - r["abs_res"]) <= 1e-3 * abs(r["abs_res"]) + 1e-14
     assert st["diag_spread"] == pytest.approx(r["diag_spread"], rel=1e-13)
 
@@ -329,9 +326,9 @@ def test_plain_fp64_is_bitwise_deterministic():
 
 def test_config2_full_size_matches_oracle_fixture():
     """N = 2048 over PEC (16 wires x 128 segments) against the oracle's full
-    run and the extended-precision solution stored beside it. The reference's
-    own currents are 1.4e-8 from the exact answer at this size (kappa(G) ~
-    2e8), so the same accuracy-parity bars as the corpus decks apply."""
+    run: currents, Z_in and gains within 1e-12 / 1e-9 of the reference,
+    unrelaxed. The reference's own currents are 1.4e-8 from the exact answer
+    at this size (kappa(G) ~ 2e8); the GPU's are the same distance away."""
     fx = np.load(os.path.join(GOLDEN, "oracle", "config2.npz"))
     d = F.Deck(W.config2_deck())
     p = d.problem("cli", 14.2e6)
@@ -341,28 +338,24 @@ def test_config2_full_size_matches_oracle_fixture():
     xe = fx["x_exact"]
     e_ref = rel(fx["x"][:n], xe[:n])
     e_gpu = rel(x[:n], xe[:n])
-    assert e_gpu < max(1e-9, 2.0 * e_ref), (e_gpu, e_ref)
-    bar = max(1e-9, 1.5 * (e_gpu + e_ref))
-    assert rel(x[:n], fx["x"][:n]) < bar
+    d = rel(x[:n], fx["x"][:n])
+    print(f"config 2: GPU vs reference {d:.1e}; error vs exact: GPU {e_gpu:.1e}, reference {e_ref:.1e}")
+    assert d < 1e-12
+    bar = 1e-12
     idx = int(fx["feed_idx"][0])
     zin = p["feed_v"][0] / x[idx]
-    z_exact = p["feed_v"][0] / xe[idx]
     z_ref = fx["feed_z"][0]
-    e_zref = abs(z_ref - z_exact) / abs(z_exact)
-    e_zgpu = abs(zin - z_exact) / abs(z_exact)
-    assert e_zgpu < max(1e-9, 2.0 * e_zref)
-    assert abs(zin - z_ref) / abs(z_ref) < max(1e-9, 1.5 * (e_zgpu + e_zref))
+    assert abs(zin - z_ref) / abs(z_ref) < 1e-12
     # RP kernel alone at full size (65,341 points + 32,400 normalisation
     # directions, PEC image): the oracle's own currents in, 1e-9 bar.
     rows_k, total_k = F.rp(p["segs"], fx["x"][:n], 14.2e6, p["rp_theta"], p["rp_phi"],
                            p["ground_struct"])
     assert abs(total_k - float(fx["rp_total"])) / float(fx["rp_total"]) < 1e-12
     pattern_close(rows_k, fx["rp"], tol=1e-9)
-    # end to end: near-null directions amplify the currents' ~1e-8 rounding by
-    # the far-field cancellation ratio, so gains are held to 1e-6 there
+    # end to end: the GPU's own currents into the GPU far field
     rows, total = F.rp(p["segs"], x[:n], 14.2e6, p["rp_theta"], p["rp_phi"], p["ground_struct"])
     assert abs(total - float(fx["rp_total"])) / float(fx["rp_total"]) < bar
-    pattern_close(rows, fx["rp"], tol=1e-6)
+    pattern_close(rows, fx["rp"], tol=1e-9)
 
 
 def test_invalid_arguments_are_rejected():
