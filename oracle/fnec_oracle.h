@@ -101,6 +101,9 @@ void or_fill_z(const or_seg *s, int n, double freq_hz, or_ground g, cx *z /* n*n
 cx or_fill_elem(const or_seg *obs, const or_seg *src, double k, int is_self);
 
 /* ---- linear (linear.rs) ---- */
+/* Threads for the fill, Gram and elimination row loops (default 1, as the
+ * reference runs; results do not depend on it). Fixture generation only. */
+void or_set_threads(int n);
 int or_solve_square(const cx *a, const cx *b, int n, cx *x);
 /* the same, recording the pivot row chosen at each elimination column */
 int or_solve_square_piv(const cx *a, const cx *b, int n, cx *x, int *piv);

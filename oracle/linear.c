@@ -15,6 +15,13 @@
 
 #include "oracle_internal.h"
 
+/* Threads for the row loops below (fixture generation only; 1 by default,
+ * as the reference solve runs on one thread). Each parallel iteration owns
+ * whole elements or rows and keeps their operation order, so results do not
+ * depend on the thread count. */
+int or_threads = 1;
+void or_set_threads(int n) { or_threads = n > 0 ? n : 1; }
+
 static cx **rows_alloc(int r, int c)
 {
     cx **m = (cx **)malloc(sizeof(cx *) * (size_t)(r > 0 ? r : 1));
@@ -36,7 +43,7 @@ static void rows_free(cx **m, int r)
  * NULL) receives the pivot row chosen at each column. */
 static int solve_aug_piv(cx **aug, int n, cx *x, int *pivots)
 {
-    int col, row, k, i, j;
+    int col, row, i, j;
     for (col = 0; col < n; col++) {
         int piv = col;
         double best = cx_norm(aug[col][col]);
@@ -60,12 +67,14 @@ static int solve_aug_piv(cx **aug, int n, cx *x, int *pivots)
             aug[piv] = t;
         }
         diag = aug[col][col];
+#pragma omp parallel for num_threads(or_threads) schedule(static) if (or_threads > 1 && n - col > 256)
         for (row = col + 1; row < n; row++) {
             cx factor = cx_div(aug[row][col], diag);
             cx *ar = aug[row], *ac = aug[col];
-            for (k = col; k <= n; k++) {
-                cx sub = cx_mul(factor, ac[k]);
-                ar[k] = cx_sub(ar[k], sub);
+            int kk;
+            for (kk = col; kk <= n; kk++) {
+                cx sub = cx_mul(factor, ac[kk]);
+                ar[kk] = cx_sub(ar[kk], sub);
             }
         }
     }
@@ -166,9 +175,11 @@ static int build_m(const cx *z, int n, const double *cosv, const cx *rhs, const 
 static cx **normal_eqs(cx **m, const cx *y, int rows, int cols)
 {
     cx **ata = rows_alloc(cols, cols + 1);
-    int i, j, r;
+    int i;
+#pragma omp parallel for num_threads(or_threads) schedule(dynamic, 4) if (or_threads > 1)
     for (i = 0; i < cols; i++) {
         cx sum;
+        int j, r;
         for (j = 0; j < cols; j++) {
             sum = cx_make(0.0, 0.0);
             for (r = 0; r < rows; r++)

@@ -99,11 +99,13 @@ static cx fresnel_reflection_scalar(double freq_hz, double eps_r, double sigma)
 void or_fill_z(const or_seg *s, int n, double freq_hz, or_ground g, cx *z)
 {
     double k = 2.0 * OR_PI * freq_hz / OR_C0;
-    int i, j;
+    int i;
     cx gamma = cx_make(0.0, 0.0);
     if (g.kind == OR_GROUND_FINITE)
         gamma = fresnel_reflection_scalar(freq_hz, g.eps_r, g.sigma);
+#pragma omp parallel for num_threads(or_threads) schedule(dynamic, 4) if (or_threads > 1)
     for (i = 0; i < n; i++) {
+        int j;
         for (j = 0; j < n; j++) {
             cx direct = or_fill_elem(&s[i], &s[j], k, i == j);
             cx img = cx_make(0.0, 0.0);

@@ -97,6 +97,7 @@ def lib():
         L.or_fill_elem.argtypes = [dp, dp, C.c_double, C.c_int]
         L.or_fill_elem.restype = CX
         L.or_solve_square.argtypes = [dp, dp, C.c_int, dp]
+        L.or_set_threads.argtypes = [C.c_int]
         L.or_solve_square_piv.argtypes = [dp, dp, C.c_int, dp, dp]
         L.or_libm_eval.argtypes = [C.c_int, dp, dp, C.c_size_t, dp]
         L.or_solve_hallen.argtypes = [dp, C.c_int, dp, dp, dp, dp, C.c_int, dp, dp, dp, C.c_int, dp]
@@ -105,6 +106,10 @@ def lib():
         L.or_radiation_pattern.restype = C.c_double
         L.or_power_integral.argtypes = [dp, C.c_int, dp, C.c_double, Ground]
         L.or_power_integral.restype = C.c_double
+        L.or_far_field.argtypes = [dp, C.c_int, dp, C.c_double, Ground, C.c_double, C.c_double,
+                                   C.POINTER(CX), C.POINTER(CX)]
+        L.or_radiation_efficiency.argtypes = [dp, C.c_int, dp, C.c_double, Ground, C.c_double]
+        L.or_radiation_efficiency.restype = C.c_double
         L.or_fmt_fixed.argtypes = [C.c_char_p, C.c_int, C.c_double, C.c_int]
         L.or_fmt_exp.argtypes = [C.c_char_p, C.c_int, C.c_double, C.c_int]
         L.or_axial_ratio.argtypes = [CX, CX]
@@ -175,6 +180,12 @@ def fill_z(segs: np.ndarray, freq_hz: float, g: Ground | None = None) -> np.ndar
     z = np.zeros((n, n), np.complex128)
     lib().or_fill_z(_p(segs), n, float(freq_hz), g or ground(), z.ctypes.data_as(C.c_void_p))
     return z
+
+
+def set_threads(n: int) -> None:
+    """Threads for the fill, Gram and elimination row loops (fixtures only;
+    results do not depend on the count)."""
+    lib().or_set_threads(int(n))
 
 
 def solve_square_piv(a: np.ndarray, b: np.ndarray) -> tuple[int, np.ndarray, np.ndarray]:
@@ -253,6 +264,23 @@ def power_integral(segs, cur, k, g: Ground | None = None) -> float:
     segs = np.ascontiguousarray(segs, dtype=SEG_DTYPE)
     cur = np.ascontiguousarray(cur, np.complex128)
     return lib().or_power_integral(_p(segs), len(segs), _p(cur), float(k), g or ground())
+
+
+def far_field(segs, cur, k, th_deg, ph_deg, g: Ground | None = None) -> tuple[complex, complex]:
+    """(F_theta, F_phi) of far_field_components(_with_image / _fresnel)."""
+    segs = np.ascontiguousarray(segs, SEG_DTYPE)
+    cur = np.ascontiguousarray(cur, np.complex128)
+    ft, fp = CX(), CX()
+    lib().or_far_field(_p(segs), len(segs), _p(cur), float(k), g or ground(), float(th_deg),
+                       float(ph_deg), C.byref(ft), C.byref(fp))
+    return complex(ft.re, ft.im), complex(fp.re, fp.im)
+
+
+def radiation_efficiency(segs, cur, freq_hz, p_in, g: Ground | None = None) -> float:
+    segs = np.ascontiguousarray(segs, SEG_DTYPE)
+    cur = np.ascontiguousarray(cur, np.complex128)
+    return lib().or_radiation_efficiency(_p(segs), len(segs), _p(cur), float(freq_hz), g or ground(),
+                                         float(p_in))
 
 
 def fmt_fixed(v: float, prec: int) -> str:

@@ -42,4 +42,6 @@ static inline int ex_is_current(const or_card *c) { return ex_type(c) == 4; }
 
 int or_find_seg(const or_seg *s, int n, uint32_t tag, uint32_t idx);
 
+extern int or_threads; /* row-loop threads (linear.c) */
+
 #endif

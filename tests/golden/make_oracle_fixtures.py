@@ -1,7 +1,7 @@
 """Writes tests/golden/oracle/*.npz: full-size results of the oracle (the C
 restatement of the reference CPU path) that GPU parity tests compare against
 when the oracle itself would take minutes. Run from the repository root:
-    python tests/golden/make_oracle_fixtures.py [config2] [config4a] ...
+    python tests/golden/make_oracle_fixtures.py [config2] [config3] [config4a_f0] ...
 """
 import importlib
 import os
@@ -40,10 +40,12 @@ def exact_solution(deck, freq_hz, semantics):
     return x.astype(np.complex128)
 
 
-def run(name, deck, freq_hz=0.0, semantics="cli"):
+def run(name, deck, freq_hz=0.0, semantics="cli", threads=1):
+    O.set_threads(threads)  # row loops only: the results do not depend on it
     t0 = time.time()
     r = O.run_deck(deck, semantics, 0, freq_hz)
     dt = time.time() - t0
+    O.set_threads(1)
     kw = dict(x=r["x"], feed_z=r["feed_z"], feed_idx=np.array(r["feed_idx"]), abs_res=r["abs_res"],
               rel_res=r["rel_res"], diag_spread=r["diag_spread"], seconds=dt, freq_hz=r["freq_hz"],
               x_exact=exact_solution(deck, r["freq_hz"], semantics))
@@ -68,5 +70,8 @@ if __name__ == "__main__":
               np.linalg.norm(d["x"][:n] - d["x_exact"][:n]) / np.linalg.norm(d["x_exact"][:n]))
     if "config2" in which:
         run("config2", W.config2_deck())
+    if "config3" in which:
+        # N = 8192, worker semantics (one KCL row), the 5-degree RP grid
+        run("config3", W.config3_deck(), 0.0, "worker", threads=os.cpu_count() or 1)
     if "config4a_f0" in which:
         run("config4a_f0", W.config4a_deck(), W.config4a_frequencies()[0])

@@ -10,6 +10,7 @@ Checks, in order of strength:
   normal equations, an independently assembled Gram matrix, and the closed
   form of the Hertzian-dipole power integral.
 """
+import cmath
 import json
 import math
 
@@ -420,3 +421,111 @@ def test_residual_and_diagonal_spread(oracle):
     assert abs(np.linalg.norm(res) / np.linalg.norm(y[:n]) - r["rel_res"]) <= 1e-6 * r["rel_res"] + 1e-18
     d = np.abs(np.diag(z))
     assert abs(d.max() / d.min() - r["diag_spread"]) <= 1e-12 * r["diag_spread"]
+
+
+# ---- Fresnel far field (farfield.rs:333-388) and radiation efficiency
+# (farfield.rs:308-322): pinned by limits and closed forms ----
+
+C0 = 299792458.0
+EPS0_FF = 8.8541878128e-12
+ETA0 = 4.0 * math.pi * 1e-7 * C0
+
+
+def _element(oracle, mid, direction, length=0.01):
+    """One short segment (a Hertzian element) at mid along direction."""
+    p1 = np.asarray(mid, float) - 0.5 * length * np.asarray(direction, float)
+    p2 = np.asarray(mid, float) + 0.5 * length * np.asarray(direction, float)
+    return oracle.make_segments([(p1, p2, 1, 1e-4)])
+
+
+def _gammas_from_fields(oracle, eps_r, sigma, th, k=0.3, h=3.0):
+    """Reflection coefficients recovered from single-element fields: a
+    z-directed element gives Gamma_v through F_theta at phi = 0, an x-directed
+    one Gamma_h through F_phi at phi = 90 deg (its direct and image terms in
+    closed form, unit current)."""
+    g = oracle.ground("finite", eps_r, sigma)
+    L = 0.01
+    ct, st = math.cos(math.radians(th)), math.sin(math.radians(th))
+    w = L * complex(math.cos(k * h * ct), math.sin(k * h * ct))    # direct, e^{jk r.r_n}
+    iw = L * complex(math.cos(-k * h * ct), math.sin(-k * h * ct))  # image at -h
+    ez = _element(oracle, (0, 0, h), (0, 0, 1), L)
+    ft, _ = oracle.far_field(ez, [1.0 + 0j], k, th, 0.0, g)
+    gv = (ft / (-st) - w) / iw       # F_theta = -sin(th) (w + Gamma_v iw)
+    ex = _element(oracle, (0, 0, h), (1, 0, 0), L)
+    _, fp = oracle.far_field(ex, [1.0 + 0j], k, th, 90.0, g)
+    gh = -(fp + w) / iw              # F_phi = -w - Gamma_h iw
+    return gv, gh
+
+
+def test_fresnel_normal_incidence(oracle):
+    """At theta = 0: Gamma_h = -Gamma_v = (1 - sqrt(eps_c)) / (1 + sqrt(eps_c))."""
+    k = 0.3
+    for eps_r, sigma in ((13.0, 0.005), (5.0, 0.001), (80.0, 4.0)):
+        eps_c = complex(eps_r, -sigma / (k * C0 * EPS0_FF))
+        want = (1 - cmath.sqrt(eps_c)) / (1 + cmath.sqrt(eps_c))
+        gv, gh = _gammas_from_fields(oracle, eps_r, sigma, 1e-6, k)
+        assert abs(gh - want) < 1e-9 * abs(want) + 1e-12
+        assert abs(gv + want) < 1e-9 * abs(want) + 1e-12
+
+
+def test_fresnel_grazing_limit(oracle):
+    """As theta -> 90 deg both coefficients tend to -1."""
+    for eps_r, sigma in ((13.0, 0.005), (3.0, 0.0)):
+        gv, gh = _gammas_from_fields(oracle, eps_r, sigma, 89.9999)
+        assert abs(gv + 1) < 1e-4 and abs(gh + 1) < 1e-4
+
+
+def test_fresnel_matches_textbook_coefficients(oracle):
+    """Away from the limits: Gamma_v = (eps_c cos - s) / (eps_c cos + s) and
+    Gamma_h = (cos - s) / (cos + s), s = sqrt(eps_c - sin^2), for an incidence
+    angle measured from the zenith."""
+    k = 0.3
+    for eps_r, sigma in ((13.0, 0.005), (4.0, 0.02)):
+        eps_c = complex(eps_r, -sigma / (k * C0 * EPS0_FF))
+        for th in (10.0, 45.0, 70.0, 85.0):
+            c = math.cos(math.radians(th))
+            sq = cmath.sqrt(eps_c - math.sin(math.radians(th)) ** 2)
+            gv_w = (eps_c * c - sq) / (eps_c * c + sq)
+            gh_w = (c - sq) / (c + sq)
+            gv, gh = _gammas_from_fields(oracle, eps_r, sigma, th, k)
+            assert abs(gv - gv_w) < 1e-9 and abs(gh - gh_w) < 1e-9, th
+
+
+def test_fresnel_conductor_limit_is_the_pec_image(oracle):
+    """sigma -> infinity reproduces the PEC far field (Gamma_v -> 1,
+    Gamma_h -> -1 with the image's sign conventions)."""
+    segs = oracle.make_segments([((0, 0, 4.0), (1.2, 0.7, 9.0), 9, 1e-3), ((2, -1, 6), (2, 3, 6), 7, 1e-3)])
+    rng = np.random.default_rng(2)
+    cur = rng.standard_normal(len(segs)) + 1j * rng.standard_normal(len(segs))
+    k = 2 * math.pi * 14.2e6 / C0
+    for th, ph in ((0.0, 0.0), (30.0, 40.0), (60.0, 200.0), (85.0, 300.0)):
+        fpec = oracle.far_field(segs, cur, k, th, ph, oracle.ground("pec"))
+        ffre = oracle.far_field(segs, cur, k, th, ph, oracle.ground("finite", 13.0, 1e14))
+        for a, b in zip(ffre, fpec):
+            assert abs(a - b) <= 1e-6 * (abs(fpec[0]) + abs(fpec[1]))
+
+
+def test_radiation_efficiency_hertzian_closed_form(oracle):
+    """(k^2 eta0 / 32 pi^2) times the trapezoid power integral of a Hertzian
+    element is eta0 k^2 (I L)^2 / 12 pi; the efficiency against twice that
+    input power is 1/2 (to the integration grid's accuracy)."""
+    f = 10e6
+    k = 2 * math.pi * f / C0
+    L = 0.01
+    el = _element(oracle, (0, 0, 0), (0, 0, 1), L)
+    p_rad = ETA0 * k * k * L * L / (12 * math.pi)
+    e = oracle.radiation_efficiency(el, [1.0 + 0j], f, 2.0 * p_rad)
+    assert e == pytest.approx(0.5, rel=1e-4)
+    assert oracle.radiation_efficiency(el, [1.0 + 0j], f, 0.0) == 1.0
+
+
+def test_radiation_efficiency_of_a_lossless_dipole(oracle):
+    """The 51-segment half-wave dipole in free space radiates what it takes
+    in: P_rad / P_in within 1% of 1 (P_in = Re(V conj(I)) / 2)."""
+    r = oracle.run_deck(deck("dipole-freesp-rp-51seg"), "cli")
+    i_feed = r["x"][r["feed_idx"][0]]
+    p_in = 0.5 * (r["feed_v"][0] * i_feed.conjugate()).real
+    e = oracle.radiation_efficiency(r["segs"], r["currents"], r["freq_hz"], 0.5 * p_in)
+    assert e == 1.0  # clamped: P_rad > P_in / 2
+    e = oracle.radiation_efficiency(r["segs"], r["currents"], r["freq_hz"], p_in * 1.02)
+    assert 0.97 < e < 1.0
