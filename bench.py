@@ -7,14 +7,23 @@ grid; 10% over PEC ground), built by fnec-rust_amd/workloads.py. A "step" is
 one window of requests submitted to the serving session and drained;
 ``--steps K`` windows are timed after ``--warmup W`` untimed ones.
 
-Printed (one JSON line): served solves/s (value), p50/p99 latency, the plain
-FP64 one-request-at-a-time GPU path on a sample of the same stream (the
-reference point of the speedup) with the speedup over it and the max relative
-Z_in difference between the served and the plain path, the roofline of the
-dominant kernel from HIP events recorded on its stream during the timed
-region (events around that kernel class only; the per-class breakdown comes
-from untimed windows profiled before it), and a CPU baseline (the oracle, timed on this host on a bounded
-sample and extrapolated over the stream's N distribution).
+Printed (one JSON line):
+- served solves/s (value) and the closed-loop window latency p50/p99;
+- open-loop latency: p50/p99 with the stream's own Poisson arrival times at a
+  stated load (a separate slice of the stream);
+- the plain FP64 path (the reference's algorithm and rounding) one request at
+  a time over exactly the requests of the timed region (the reference point
+  of the speedup), the speedup over it, and, for context, the FP64
+  matrix-core path on the first 256 of them;
+- accuracy: the max relative Z_in difference of the served results from the
+  plain path over every timed request, and from the oracle (the C
+  restatement of the reference CPU path) over the timed requests the CPU
+  baseline solves;
+- the roofline of the dominant kernel from HIP events recorded on its stream
+  during the timed region (events around that kernel class only; the
+  per-class breakdown comes from untimed windows profiled before it);
+- a CPU baseline: the oracle on this host's cores on a bounded sample of the
+  timed requests, extrapolated over the stream's N distribution.
 """
 from __future__ import annotations
 
@@ -231,15 +240,15 @@ def batch_overlap(batches):
             "two_or_more_on_device_frac": round(busy2 / span, 3) if span > 0 else 0.0}
 
 
-def plain_fp64(items, sample):
+def plain_fp64(items, precision=F.PREC_FP64):
     """One request at a time through fnec_hip_solve_hallen (+ fnec_hip_rp)."""
     zs = {}
     t0 = time.perf_counter()
-    for it in items[:sample]:
+    for it in items:
         k = it["keep"]
         r = it["req"]
         x = F.solve_hallen(k["segs"], k["rhs"], k["cos"], k["wf"], k["wl"], k["ja"], k["jb"], k["js"],
-                           r.freq_hz, r.ground, F.PREC_FP64)
+                           r.freq_hz, r.ground, precision)
         n = len(k["segs"])
         zs[r.tag] = complex(r.v_feed.re, r.v_feed.im) / x[r.feed_index]
         if it["meta"]["rp"]:
@@ -247,13 +256,27 @@ def plain_fp64(items, sample):
     return time.perf_counter() - t0, zs
 
 
-def cpu_baseline(stream, budget_s, threads):
-    """Oracle (C restatement of the reference CPU path) on a bounded sample."""
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or platform.machine()
+
+
+def cpu_baseline(stream, timed, budget_s, threads):
+    """Oracle (C restatement of the reference CPU path, one thread per solve
+    as the reference) on a bounded sample of the timed requests with N <= 600,
+    spread over ``threads`` host threads. Returns the baseline record and the
+    oracle's Z_in of every sampled request (by tag)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O  # noqa: E402
     O.build()
     rng = np.random.default_rng(7)
-    small = [r for r in stream if r["n"] <= 600]
+    small = [r for r in timed if r["n"] <= 600]
     order = rng.permutation(len(small))
 
     def one(r):
@@ -263,14 +286,14 @@ def cpu_baseline(stream, budget_s, threads):
             kind = 1 if r["pec"] else 0
             O.radiation_pattern(res["segs"], res["currents"], r["freq_hz"], RP_THETA, RP_PHI,
                                 O.Ground(kind, 0.0, 0.0))
-        return r["n"], r["rp"], time.perf_counter() - t
+        return r["n"], r["rp"], time.perf_counter() - t, r["id"], complex(res["feed_z"][0])
 
     samples = []
     t0 = time.perf_counter()
     with cf.ThreadPoolExecutor(max_workers=threads) as ex:
         pending = set()
         it = iter(order)
-        for _ in range(threads):
+        for _ in range(min(threads, len(small))):
             pending.add(ex.submit(one, small[next(it)]))
         while pending:
             done, pending = cf.wait(pending, return_when=cf.FIRST_COMPLETED)
@@ -297,12 +320,12 @@ def cpu_baseline(stream, budget_s, threads):
         "unit": "solves/s",
         "cores": threads,
         "kind": "port",
-        "sample": (f"{len(samples)} requests with N<=600 from the same stream solved by the oracle "
-                   f"(worker semantics + RP) on {threads} threads in {wall:.1f} s; per-request cost "
-                   f"fitted as a*N^3 + b*N*[rp] and averaged over all {len(stream)} requests of the "
-                   f"stream (extrapolated)"),
-        "cpu": platform.processor() or platform.machine(),
-    }
+        "sample": (f"{len(samples)} of the timed requests with N<=600 solved by the oracle (worker semantics + RP), "
+                   f"one thread per solve, {threads} solves at a time, in {wall:.1f} s; per-request cost fitted "
+                   f"as a*N^3 + b*N*[rp] on N {int(ns.min())}..{int(ns.max())} and extrapolated over all "
+                   f"{len(stream)} requests of the stream (N up to {int(N.max())})"),
+        "cpu": cpu_model(),
+    }, {s[3]: s[4] for s in samples}
 
 
 def measured_traffic(kernel, precision):
@@ -351,7 +374,11 @@ def main():
     ap.add_argument("--profile-windows", type=int, default=2,
                     help="untimed windows with events on every kernel class, before the timed ones")
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="mixed")
-    ap.add_argument("--plain-sample", type=int, default=256)
+    ap.add_argument("--plain-sample", type=int, default=0,
+                    help="timed requests the plain FP64 path solves (0: all of them)")
+    ap.add_argument("--latency-rate", type=float, default=2000.0,
+                    help="load of the open-loop latency phase (requests/s; 0 skips it)")
+    ap.add_argument("--latency-requests", type=int, default=2048)
     ap.add_argument("--max-batch", type=int, default=256)
     ap.add_argument("--lanes", type=int, default=2, help="batches on the device at once")
     ap.add_argument("--arrival", choices=["windows", "poisson"], default="windows",
@@ -394,8 +421,9 @@ def main():
     depth = args.depth or args.lanes
     precision = F.PREC_MIXED if args.precision == "mixed" else F.PREC_FP64
     total = (args.warmup + args.profile_windows + args.steps) * args.window
+    n_lat = args.latency_requests if args.arrival == "windows" and args.latency_rate > 0 else 0
     t_gen = time.perf_counter()
-    stream = W.config5_stream(count=total)
+    stream = W.config5_stream(count=total + n_lat)
     items = build_requests(stream)
     t_gen = time.perf_counter() - t_gen
     print(f"[bench] built {len(items)} requests in {t_gen:.1f}s; serving {args.steps} x {args.window}",
@@ -434,12 +462,36 @@ def main():
     value = timed / elapsed
     print(f"[bench] served {timed} requests in {elapsed:.2f}s", file=sys.stderr, flush=True)
 
-    sample_items = items[first:first + args.plain_sample]
-    t_plain, zs_plain = plain_fp64(sample_items, args.plain_sample)
+    # open-loop latency at a stated load, on a slice of the stream after the
+    # closed-loop requests (served through a fresh session)
+    open_loop = None
+    if n_lat:
+        lat_items = items[total:total + n_lat]
+        t_first = stream[total]["t_arrival"]
+        scale = 1000.0 / args.latency_rate  # the stream is generated at 1000 requests/s
+        times = [(stream[total + i]["t_arrival"] - t_first) * scale for i in range(n_lat)]
+        el_ol, lat_ol, _, _, bt_ol = serve_poisson(lat_items, times, items[:args.window], precision,
+                                                   int(args.budget_gib * 2 ** 30), args.max_batch, dom)
+        open_loop = {"offered_per_s": args.latency_rate, "served_per_s": round(n_lat / el_ol, 1),
+                     "requests": n_lat, "p50": round(float(np.percentile(lat_ol, 50)) * 1e3, 3),
+                     "p99": round(float(np.percentile(lat_ol, 99)) * 1e3, 3),
+                     "arrival": "the stream's Poisson arrival times rescaled to the load, open loop; "
+                                "latency from arrival to the poll that returns the result",
+                     **(batch_overlap(bt_ol) or {})}
+        print(f"[bench] open loop at {args.latency_rate:g}/s: p50 {open_loop['p50']:.1f} ms, "
+              f"p99 {open_loop['p99']:.1f} ms", file=sys.stderr, flush=True)
+
+    timed_items = items[first:first + timed]
+    sample_items = timed_items[:args.plain_sample] if args.plain_sample else timed_items
+    t_plain, zs_plain = plain_fp64(sample_items)
     plain_rate = len(sample_items) / t_plain
-    served_sample_time = sum(1.0 for _ in sample_items) / value
     dz = max(abs(zs_served[k] - zs_plain[k]) / abs(zs_plain[k]) for k in zs_plain)
     print(f"[bench] plain FP64 one-at-a-time: {plain_rate:.1f} solves/s on {len(sample_items)}",
+          file=sys.stderr, flush=True)
+    mfma_items = timed_items[:256]
+    t_mfma, zs_mfma = plain_fp64(mfma_items, F.PREC_FP64_MFMA)
+    mfma_rate = len(mfma_items) / t_mfma
+    print(f"[bench] FP64 matrix-core path one-at-a-time: {mfma_rate:.1f} solves/s on {len(mfma_items)}",
           file=sys.stderr, flush=True)
 
     # dominant kernel class by time in the profiled pass; its events in the
@@ -480,10 +532,20 @@ def main():
                         "bytes_per_launch": round(v[3] / v[0]) if v[3] else None}
                     for k, v in prof_all.items() if v[0] > 0},
     }
-    cpu = None
+    cpu, z_oracle = None, {}
     if not args.no_cpu_baseline:
-        threads = min(16, os.cpu_count() or 1)
-        cpu = cpu_baseline(stream, args.cpu_budget_s, threads)
+        # the host's cores as this process sees them, at most the 16 host
+        # cores a one-GPU share of the GPU machine provides
+        threads = min(16, len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count() or 1)
+        cpu, z_oracle = cpu_baseline(stream, [stream[first + i] for i in range(timed)], args.cpu_budget_s,
+                                     threads)
+    acc = {"vs_plain_fp64": {"max_rel_dzin": dz, "requests": len(zs_plain)}}
+    if z_oracle:
+        acc["vs_oracle"] = {
+            "max_rel_dzin_served": max(abs(zs_served[k] - z) / abs(z) for k, z in z_oracle.items()),
+            "max_rel_dzin_plain_fp64": max(abs(zs_plain[k] - z) / abs(z) for k, z in z_oracle.items()
+                                           if k in zs_plain),
+            "requests": len(z_oracle)}
 
     line = {
         "metric": METRIC,
@@ -512,9 +574,17 @@ def main():
                        "arrival": arrival},
         "serving": dict(lanes=args.lanes, batch_split_n=F.get_tunable("batch_split_n"),
                         **(batch_overlap(batches) or {})),
+        "open_loop_latency_ms": open_loop,
         "plain_fp64_gpu": {"value": round(plain_rate, 3), "unit": "solves/s",
-                           "sample": len(sample_items)},
+                           "requests": len(sample_items),
+                           "what": "fnec_hip_solve_hallen(FNEC_PREC_FP64) + fnec_hip_rp, one request at a "
+                                   "time, over the timed requests: the reference's algorithm and rounding"},
         "speedup_vs_plain_fp64": round(value / plain_rate, 3),
+        "fp64_matrix_core_gpu": {"value": round(mfma_rate, 3), "unit": "solves/s", "requests": len(mfma_items),
+                                 "speedup": round(value / mfma_rate, 3),
+                                 "max_rel_dzin_vs_plain_fp64": max(abs(zs_mfma[k] - zs_plain[k]) / abs(zs_plain[k])
+                                                                   for k in zs_mfma)},
+        "accuracy": acc,
         "max_rel_dzin_vs_plain_fp64": dz,
         "roofline": roofline,
         "cpu_baseline": cpu,
