@@ -34,6 +34,7 @@ EXPORTS = [
     "fnec_hip_request_footprint", "fnec_hip_debug_normal_eqs", "fnec_hip_debug_lu_f32", "fnec_hip_debug_mfma_probe",
     "fnec_hip_debug_mfma_probe_f32", "fnec_hip_set_tunable", "fnec_hip_get_tunable",
     "fnec_hip_debug_solve_z", "fnec_hip_debug_solve_square", "fnec_hip_debug_refmath",
+    "fnec_hip_rp_noalloc",
 ]
 
 
@@ -109,6 +110,7 @@ def lib():
         L.fnec_hip_solve_hallen_paths.argtypes = [vp, sz, vp, vp, vp, vp, sz, dbl, C.POINTER(Ground),
                                                   i32, vp, C.POINTER(SolveStats)]
         L.fnec_hip_rp.argtypes = [vp, vp, sz, dbl, C.POINTER(Ground), vp, vp, sz, vp, C.POINTER(dbl)]
+        L.fnec_hip_rp_noalloc.argtypes = [vp, vp, sz, dbl, C.POINTER(Ground), vp, vp, sz, vp, C.POINTER(dbl)]
         L.fnec_hip_solve_batch.argtypes = [vp, sz, i32, vp]
         L.fnec_hip_session_create.restype = vp
         L.fnec_hip_session_create.argtypes = [sz, i32, i32]

@@ -158,6 +158,31 @@ int fnec_hip_rp(const fnec_seg *segs, const fnec_c64 *currents, size_t n, double
     return FNEC_OK;
 }
 
+int fnec_hip_rp_noalloc(const fnec_seg *segs, const fnec_c64 *currents, size_t n, double freq_hz,
+                        const fnec_ground *ground, const double *theta_deg, const double *phi_deg,
+                        size_t npts, fnec_rp_row *out, double *total_out)
+{
+    std::lock_guard<std::mutex> lk(global_mutex());
+    int st = ensure_ready();
+    if (st)
+        return st;
+    if (npts && !out) {
+        set_error("null output");
+        return FNEC_INVALID_ARG;
+    }
+    RpHost job;
+    st = rp_host_from(job, segs, currents, n, freq_hz, ground, theta_deg, phi_deg, npts);
+    if (st)
+        return st;
+    double total = 0.0;
+    st = rp_run_noalloc(job, out, total);
+    if (st)
+        return st;
+    if (total_out)
+        *total_out = total;
+    return FNEC_OK;
+}
+
 int fnec_hip_debug_normal_eqs(const fnec_seg *segs, size_t n, const fnec_c64 *rhs,
                               const double *cos_vec, const size_t *wire_first,
                               const size_t *wire_last, size_t n_wires, const size_t *j_a,

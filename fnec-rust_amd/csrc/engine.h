@@ -41,6 +41,8 @@ struct RpHost {
 struct RpStaged {
     size_t nj = 0;
     char *base = nullptr;
+    bool fixed = false;                // base is the device scratch array
+    std::vector<char> hstage, hback;   // pageable staging when fixed
     std::vector<size_t> o_tot, o_out, npts;
     size_t out_begin = 0, rows_begin = 0, out_end = 0;
     size_t pts_off = 0, pts_cnt = 0, norm_off = 0, norm_cnt = 0, all_off = 0;
@@ -125,7 +127,14 @@ int rp_host_from(RpHost &h, const fnec_seg *segs, const fnec_c64 *cur, size_t n,
 // Far-field jobs in one batch: rp_stage uploads the inputs on the copy
 // stream; rp_finish runs the kernels after the solve stream's work and reads
 // back the totals, and the rows of the jobs whose outs[j] is not null.
-int rp_stage(const std::vector<RpHost> &jobs, RpStaged &st);
+// With fixed_base, the job lives at that device address (fixed_cap bytes;
+// FNEC_RESOURCE_EXHAUSTED beyond) instead of the lane's arena, and stages
+// through pageable host memory.
+int rp_stage(const std::vector<RpHost> &jobs, RpStaged &st, char *fixed_base = nullptr,
+             size_t fixed_cap = 0);
+// Device scratch of fnec_hip_rp_noalloc.
+constexpr size_t kNoAllocScratchBytes = (size_t)32 << 20;
+int rp_run_noalloc(const RpHost &job, fnec_rp_row *out, double &total);
 int rp_finish(RpStaged &st, const std::vector<fnec_rp_row *> &outs, std::vector<double> &totals);
 int rp_run(const std::vector<RpHost> &jobs, std::vector<fnec_rp_row *> &outs,
            std::vector<double> &totals);

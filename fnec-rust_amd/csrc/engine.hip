@@ -90,7 +90,18 @@ struct Context {
     bool ready = false;
     int status = FNEC_UNAVAILABLE;
     int device = -1;
+    // device address of `scratch` on each device, from hipGetSymbolAddress
+    // with that device current (the library drives g_ctx.device only)
+    std::vector<char *> scratch_dev;
 };
+
+// Working memory of fnec_hip_rp_noalloc: its far-field job (inputs, fields,
+// partial sums, totals, rows, work list) lives here instead of an arena the
+// library allocates, so that entry point calls no hipMalloc. Each device
+// holds its own copy, at its own address (Context::scratch_dev), until the
+// process ends; every such call on a device uses the same bytes, and the
+// process-wide lock of the C ABI keeps calls from overlapping.
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
 
 static Context g_ctx;
 
@@ -394,7 +405,23 @@ static int ctx_init()
         g_ctx.status = FNEC_INTERNAL;
         return g_ctx.status;
     }
-    // lanes create their streams when first bound
+    g_ctx.scratch_dev.assign(count, nullptr);
+    void *sp = nullptr;
+    const hipError_t se = hipGetSymbolAddress(&sp, HIP_SYMBOL(scratch));
+    if (se != hipSuccess || !sp) {
+        set_error(std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(se));
+        g_ctx.status = FNEC_INTERNAL;
+        return g_ctx.status;
+    }
+    g_ctx.scratch_dev[g_ctx.device] = (char *)sp;
+    // the lanes in use create their streams and events here, so that later
+    // calls (fnec_hip_rp_noalloc among them) create nothing; a lane enabled
+    // later by the tunable creates its own when first bound
+    for (int i = 0; i < g_lanes_on; ++i)
+        if (lane_init(g_lanes[i]) != FNEC_OK) {
+            g_ctx.status = FNEC_INTERNAL;
+            return g_ctx.status;
+        }
     g_ctx.status = FNEC_OK;
     return g_ctx.status;
 }
@@ -2389,7 +2416,7 @@ int lu_f32_only(const HostProblem &hp, float *lu_out, int *ipiv_out)
 // far field
 // ---------------------------------------------------------------------------
 
-int rp_stage(const std::vector<RpHost> &jobs, RpStaged &S)
+int rp_stage(const std::vector<RpHost> &jobs, RpStaged &S, char *fixed_base, size_t fixed_cap)
 {
     LaneScope lane_scope;
     if (lane_scope.status())
@@ -2398,6 +2425,7 @@ int rp_stage(const std::vector<RpHost> &jobs, RpStaged &S)
     const size_t nj = jobs.size();
     S = RpStaged{};
     S.nj = nj;
+    S.fixed = fixed_base != nullptr;
     size_t cur = 0;
     std::vector<size_t> o_seg(nj), o_cur(nj), o_th(nj), o_ph(nj), o_field(nj), o_part(nj);
     S.o_tot.resize(nj);
@@ -2449,12 +2477,25 @@ int rp_stage(const std::vector<RpHost> &jobs, RpStaged &S)
         cur += 32 * jobs[j].npts;
     }
     S.out_end = cur = (cur + 255) & ~(size_t)255;
-    int st = grow_arena(LN().rp_arena, LN().rp_arena_bytes, cur + 256, &S.base);
-    if (st)
-        return st;
+    if (S.fixed) {
+        if (cur + 256 > fixed_cap) {
+            set_error("far-field job needs " + std::to_string(cur + 256) + " bytes of device scratch; " +
+                      std::to_string(fixed_cap) + " are available");
+            return FNEC_RESOURCE_EXHAUSTED;
+        }
+        S.base = fixed_base;
+    } else {
+        int st = grow_arena(LN().rp_arena, LN().rp_arena_bytes, cur + 256, &S.base);
+        if (st)
+            return st;
+    }
     char *base = S.base;
     std::vector<RpProb> probs(nj);
-    char *hs = pinned(LN().h_rp, LN().h_rp_bytes, S.out_begin - in_begin);
+    // staging in pinned memory the lane grows on demand; with a fixed base,
+    // in pageable host memory (no allocation by the HIP runtime)
+    if (S.fixed)
+        S.hstage.resize(S.out_begin - in_begin);
+    char *hs = S.fixed ? S.hstage.data() : pinned(LN().h_rp, LN().h_rp_bytes, S.out_begin - in_begin);
     if (!hs) {
         set_error("pinned host memory exhausted");
         return FNEC_RESOURCE_EXHAUSTED;
@@ -2527,7 +2568,9 @@ int rp_finish(RpStaged &S, const std::vector<fnec_rp_row *> &outs, std::vector<d
     for (size_t j = 0; j < nj; ++j)
         rows = rows || (outs.size() > j && outs[j] && S.npts[j]);
     const size_t end = rows ? S.out_end : S.rows_begin;
-    char *hb = pinned(LN().h_back, LN().h_back_bytes, end - S.out_begin);
+    if (S.fixed)
+        S.hback.resize(end - S.out_begin);
+    char *hb = S.fixed ? S.hback.data() : pinned(LN().h_back, LN().h_back_bytes, end - S.out_begin);
     if (!hb) {
         set_error("pinned host memory exhausted");
         return FNEC_RESOURCE_EXHAUSTED;
@@ -2557,6 +2600,33 @@ int rp_run(const std::vector<RpHost> &jobs, std::vector<fnec_rp_row *> &outs, st
     if (st)
         return st;
     return rp_finish(S, outs, totals);
+}
+
+// One far-field job in the device scratch array: no device or pinned memory
+// is allocated.
+int rp_run_noalloc(const RpHost &job, fnec_rp_row *out, double &total)
+{
+    LaneScope lane_scope;
+    if (lane_scope.status())
+        return lane_scope.status();
+    char *sp = g_ctx.device >= 0 && (size_t)g_ctx.device < g_ctx.scratch_dev.size()
+                   ? g_ctx.scratch_dev[g_ctx.device] : nullptr;
+    if (!sp) {
+        set_error("device scratch not set up");
+        return FNEC_INTERNAL;
+    }
+    RpStaged S;
+    std::vector<RpHost> jobs{job};
+    int st = rp_stage(jobs, S, sp, kNoAllocScratchBytes);
+    if (st)
+        return st;
+    std::vector<fnec_rp_row *> outs{out};
+    std::vector<double> totals;
+    st = rp_finish(S, outs, totals);
+    if (st)
+        return st;
+    total = totals[0];
+    return FNEC_OK;
 }
 
 int rp_host_from(RpHost &h, const fnec_seg *segs, const fnec_c64 *cur, size_t n, double freq_hz,

@@ -160,6 +160,21 @@ int fnec_hip_rp(const fnec_seg *segs, const fnec_c64 *currents, size_t n, double
                 const fnec_ground *ground, const double *theta_deg, const double *phi_deg,
                 size_t npts, fnec_rp_row *out, double *total_out);
 
+/* ---- RP far field without device allocation ----
+ * No counterpart in the reference. For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops): the same
+ * computation and results as fnec_hip_rp, with every device buffer of the
+ * job (inputs, fields, partial sums of the normalisation integral, totals,
+ * rows, work list) in a fixed 32 MiB array each device holds from library
+ * initialisation until the process ends. It calls no hipMalloc, hipFree or
+ * pinned-memory allocation; inputs and outputs are copied through pageable
+ * host memory. A job that needs more than 32 MiB (80 B per segment and 80 B
+ * per pattern point, plus about 4 KB) returns FNEC_RESOURCE_EXHAUSTED with
+ * the sizes in fnec_hip_last_error(). Calls share the array: the
+ * process-wide lock of this ABI serialises them. */
+int fnec_hip_rp_noalloc(const fnec_seg *segs, const fnec_c64 *currents, size_t n, double freq_hz,
+                        const fnec_ground *ground, const double *theta_deg, const double *phi_deg,
+                        size_t npts, fnec_rp_row *out, double *total_out);
 /* ---- batched solves (sweeps and many geometries in one submission) ----
  * One request = one Hallén solve (+ optional RP grid). Arrays are borrowed for
  * the duration of the call. */
