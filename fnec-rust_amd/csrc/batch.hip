@@ -189,6 +189,12 @@ int run_batch(const fnec_request *const *reqs, size_t count, int precision, fnec
         rp_dev_x = dev_x;
         rp_thread = std::thread(rp_setup);
     };
+    // a failed call leaves every admitted request with its status, not OK
+    auto fail_admitted = [&](int code) {
+        for (size_t q = 0; q < ok.size(); ++q)
+            results[idx[q]].status = code;
+        return code;
+    };
     std::vector<fnec_solve_stats> stats;
     std::vector<int> status;
     std::vector<double *> dev_x;
@@ -196,10 +202,10 @@ int run_batch(const fnec_request *const *reqs, size_t count, int precision, fnec
     if (rp_thread.joinable())
         rp_thread.join();
     if (st)
-        return st;
+        return fail_admitted(st);
     if (rp_st) {
         set_error(rp_err);
-        return rp_st;
+        return fail_admitted(rp_st);
     }
     {
         HostTimer ht(kHostPost);
@@ -247,7 +253,7 @@ int run_batch(const fnec_request *const *reqs, size_t count, int precision, fnec
         std::vector<double> totals;
         st = rp_finish(staged, outs, totals);
         if (st)
-            return st;
+            return fail_admitted(st);
         for (size_t j = 0; j < jobs.size(); ++j)
             if (!status[job_q[j]])
                 results[idx[job_q[j]]].rp_total = totals[j];

@@ -225,7 +225,9 @@ typedef struct {
  * work list of (problem, tile) items over the whole batch, on one engine lane
  * (its solve stream plus the LU's look-ahead stream). results[i] corresponds
  * to reqs[i]; a result's x / rp (or, where NULL, the request's x_out / rp_out)
- * receive the currents and the pattern rows. */
+ * receive the currents and the pattern rows. When the call returns non-zero,
+ * every request it admitted carries that status (its outputs are not valid);
+ * requests refused before admission carry their own. */
 int fnec_hip_solve_batch(const fnec_request *reqs, size_t count, int precision, fnec_result *results);
 
 /* ---- serving session (continuous batching, nec_worker serving mode) ----
