@@ -51,9 +51,10 @@ METRIC = "solves/s + p99 latency, mixed-N queue, vs plain FP64 GPU at rel Î”Zinâ
 PEAK_FP64_TFLOPS = 78.6
 PEAK_FP32_TFLOPS = 157.3
 PEAK_HBM_GBS = 8000.0
-# lu_update_k128 runs the FP32 algorithm on the f16 matrix cores (three f16
-# products per FP32 product): about 192 f16 flops per HBM byte at K = 128,
-# under the f16 ridge of 2500 TFLOP/s / 8 TB/s = 312, so HBM bounds it
+PEAK_F16_TFLOPS = 2500.0
+# lu_update_k128 runs the FP32 algorithm on the f16 matrix cores, three f16
+# products per FP32 product: its algorithmic peak is 2500 / 3 = 833 TFLOP/s,
+# a ridge of 104 flops per HBM byte, above its 64 at K = 128, so HBM bounds it
 HBM_BOUND = ("zsolve", "matvec", "backsub", "lu_update_k128")
 
 RP_THETA = np.repeat(np.arange(0.0, 181.0, 2.0)[None, :], 180, axis=0).ravel()  # phi outer, theta inner
@@ -346,8 +347,9 @@ def measured_traffic(kernel, precision):
         return None  # measured with the other tile and product form
     if kernel == "lu_update" and tm == 64 and F.get_tunable("update_k128") == 1:
         return None  # measured with one K = 64 pass per step
-    if kernel == "lu_update_k128" and e.get("symbol") != "k_lu_update_f16x2":
-        return None
+    if kernel == "lu_update_k128" and e.get("symbol") != (
+            "k_lu_update_f16x2s" if F.get_tunable("update_ring") == 1 else "k_lu_update_f16x2d"):
+        return None  # measured with the other form of the K = 128 pass
     return {"bytes_per_launch": e["hbm_bytes_per_launch"], "source": "profiles/serving_stream_traffic.json"}
 
 
@@ -393,6 +395,9 @@ def main():
                     help="kernel class timed in the timed windows (default: the dominant one)")
     ap.add_argument("--update-k128", type=int, choices=[0, 1], default=None,
                     help="K = 128 trailing-update passes of the FP32 LU (default 1; with --no-autotune)")
+    ap.add_argument("--update-ring", type=int, choices=[0, 1], default=None,
+                    help="K = 128 passes stream their operands through an LDS ring (1) or load them into "
+                         "registers (0, the default)")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the engine's default knobs (no autotune on a window of the stream)")
     ap.add_argument("--depth", type=int, default=None,
@@ -418,6 +423,8 @@ def main():
         F.set_tunable("batch_split_n", args.batch_split_n)
     if args.update_k128 is not None:
         F.set_tunable("update_k128", args.update_k128)
+    if args.update_ring is not None:
+        F.set_tunable("update_ring", args.update_ring)
     depth = args.depth or args.lanes
     precision = F.PREC_MIXED if args.precision == "mixed" else F.PREC_FP64
     total = (args.warmup + args.profile_windows + args.steps) * args.window
@@ -502,6 +509,8 @@ def main():
     avg_us = tot_us / launches
     mfma = dom in ("gram", "lu_update")
     peak_f = PEAK_FP32_TFLOPS if precision == F.PREC_MIXED and dom == "lu_update" else PEAK_FP64_TFLOPS
+    if dom == "lu_update_k128":
+        peak_f = PEAK_F16_TFLOPS / 3.0
     ridge = peak_f * 1e12 / (PEAK_HBM_GBS * 1e9)
     if dom in HBM_BOUND or (nbytes and flops / nbytes < ridge):
         achieved = nbytes / launches / (avg_us * 1e-6) / 1e9
@@ -582,8 +591,8 @@ def main():
         "speedup_vs_plain_fp64": round(value / plain_rate, 3),
         "fp64_matrix_core_gpu": {"value": round(mfma_rate, 3), "unit": "solves/s", "requests": len(mfma_items),
                                  "speedup": round(value / mfma_rate, 3),
-                                 "max_rel_dzin_vs_plain_fp64": max(abs(zs_mfma[k] - zs_plain[k]) / abs(zs_plain[k])
-                                                                   for k in zs_mfma)},
+                                 "max_rel_dzin_vs_plain_fp64": max((abs(zs_mfma[k] - zs_plain[k]) / abs(zs_plain[k])
+                                                                    for k in zs_mfma if k in zs_plain), default=None)},
         "accuracy": acc,
         "max_rel_dzin_vs_plain_fp64": dz,
         "roofline": roofline,

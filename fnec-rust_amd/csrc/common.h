@@ -144,6 +144,8 @@ template <typename T> struct LuDesc {
     int *fast_ok;     // [0]: the fast path factored the current panel
     int pad1;
     T *linv;          // FP32 two-level LU: L11^{-1} of the current 64-column step, 2 x 64 x 64
+    void *h16;        // FP32 LU, K = 128 passes: the combining step's L21 and U12 split into
+                      // f16 hi / lo planes (lu.h, k_split_f16x2), 2 x 16 x lda x 32 halves
 };
 
 // Work lists of one panel step (offsets into the batch's int list).

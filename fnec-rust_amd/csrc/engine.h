@@ -53,7 +53,7 @@ struct RpStaged {
 // Host-phase timing for the per-class profile (a no-op while profiling is
 // off): problem and plan building, staging copies, far-field preparation,
 // result processing, time blocked on the stream, and kernel enqueueing.
-enum { kHostPrep = 13, kHostStage, kHostRp, kHostPost, kHostWait, kHostEnq };
+enum { kHostPrep = 14, kHostStage, kHostRp, kHostPost, kHostWait, kHostEnq };
 class HostTimer {
   public:
     explicit HostTimer(int cls);

@@ -245,11 +245,11 @@ struct ProfClass {
 // Classes from PC_HOST_PREP on are host-side phases timed with a steady clock
 // (launches = occurrences, no flops or bytes).
 enum { PC_FILL, PC_GRAM, PC_LU_PANEL, PC_LU_TRSM, PC_LU_UPDATE, PC_BACKSUB, PC_RP, PC_ZSOLVE,
-       PC_MV, PC_PCG, PC_FSETUP, PC_LU_UPDN, PC_LU_UPD128, PC_HOST_PREP, PC_HOST_STAGE, PC_HOST_RP,
+       PC_MV, PC_PCG, PC_FSETUP, PC_LU_UPDN, PC_LU_UPD128, PC_LU_SPLIT, PC_HOST_PREP, PC_HOST_STAGE, PC_HOST_RP,
        PC_HOST_POST, PC_HOST_WAIT, PC_HOST_ENQ, PC_N };
 static const char *kProfNames[PC_N] = {"fill", "gram", "lu_panel", "lu_trsm", "lu_update",
                                        "backsub", "rp", "zsolve", "matvec", "pcg_ctl", "fsetup",
-                                       "lu_update_narrow", "lu_update_k128", "host_prep", "host_stage",
+                                       "lu_update_narrow", "lu_update_k128", "lu_split", "host_prep", "host_stage",
                                        "host_rp", "host_post", "host_wait", "host_enqueue"};
 static bool g_prof = false;
 static ProfClass g_prof_acc[PC_N];
@@ -758,7 +758,7 @@ struct Plan {
     bool lu_only = false; // mixed path: stop after the FP32 factorisation
     struct MixOff {
         size_t fr, fpr, rowcol, fh, lk, ym, ls, vr, vz, vp, vq, vt, vv, part, colsq, pcg, pstate;
-        size_t perm, tws, tflags, dinv, gx, linv;
+        size_t perm, tws, tflags, dinv, gx, linv, h16;
         int nchunk;
     };
     std::vector<MixOff> mix;
@@ -778,6 +778,7 @@ struct Plan {
         // K = 128 from P - 64
         bool defer, combine;
         size_t th_off, th_cnt;
+        size_t sp_off, sp_cnt;     // combining step: (problem, 64-block, chunk * 2 + kind) of k_split_f16x2
         double f_panel, f_trsm, f_ua, b_ua, f_on, b_on, f_of, b_of, f_th, b_th;
     };
     std::vector<Outer> outer32;
@@ -818,6 +819,11 @@ static int g_mix_first_chunk = kMixFirstChunk;
 // instead of one workgroup streaming the whole factor.
 static int g_update_tile_m = 64; // row tile of the FP32 trailing update (64 or 128)
 static int g_update_k128 = 1;    // K = 128 trailing-update passes (with 64-row tiles)
+// K = 128 passes through the LDS ring (k_lu_update_f16x2s, fastest alone) or
+// with register-direct operands (k_lu_update_f16x2d, the default: the ring's
+// persistent grid holds every CU's LDS, which delays the other stream's
+// panel kernels that overlap the update in a serving window)
+static int g_update_ring = 0;
 static int g_mix_tol_exp = 14; // CG stops at |h - (M^H M + lambda) x| <= 10^-exp |h|
 static int g_batch_split_n = 1024; // serving batches: N classes split at this size (0: one class)
 static int g_gang_n = 512;
@@ -856,7 +862,8 @@ size_t problem_bytes(int n, int s, int ncons, int nruns, int precision)
         b += 4 * nn + 16 * nn * (nc + 1) + 8 * (nc + 1) * ((nn + 63) / 64);  // perm, solves
         b += ((nn + 63) / 64) * 2 * 64 * 64 * 8 + (2 * 64 * 64 + 2 * 32 * 32) * 4; // block inverses
         b += 8 * 2 * 256 * 16 * ((nn + 63) / 64);                    // gang-published blocks
-        b += 23 * 256;                                               // alignment of 23 buffers
+        b += 2048 * ldz;                                             // split K = 128 operands
+        b += 24 * 256;                                               // alignment of 24 buffers
     } else {
         b += 2 * 8 * lda * (ss + 1) + 256;                           // [G | M^H y]
         b += 2 * 8 * lda * kPanel + 256;                             // panel workspace
@@ -1096,6 +1103,19 @@ static void lu_plan_outer(std::vector<int> &ints, const std::vector<int> &ms, st
                 push2((int)b, y);
         }
         o.ot_cnt = (ints.size() - o.ot_off) / 2;
+        // combining step: L21 rows and U12 columns [P + 64, n) in blocks of
+        // 64, four 32-deep chunks, two kinds
+        align4();
+        o.sp_off = ints.size();
+        for (size_t b = 0; b < np; ++b) {
+            const int s = ms[b];
+            if (!o.combine || s <= P + 2 * kPanel)
+                continue;
+            for (int y = 0; y < (s - P - 2 * kPanel + 63) / 64; ++y)
+                for (int z = 0; z < 8; ++z)
+                    push4((int)b, y, z, 0);
+        }
+        o.sp_cnt = (ints.size() - o.sp_off) / 4;
         // outer update items (problem, row tile, column tile), generated on the
         // device: the next step's 64 columns, then the rest
         o.on_off = gen_items;
@@ -1224,6 +1244,7 @@ static void plan_build(Plan &pl, const std::vector<const HostProblem *> &hps, bo
             m.perm = take(cur, 4 * n);
             m.dinv = take(cur, ((n + 63) / 64) * 2 * 64 * 64 * 8);
             m.linv = take(cur, (2 * 64 * 64 + 2 * 32 * 32) * 4); // L11^{-1}, then the panel's U_D^{-1}
+            m.h16 = take(cur, 2048 * ldz);                          // split L21 / U12 of a combining step
             m.tws = take(cur, 16 * n * (nc + 1));
             pl.mix.push_back(m);
         }
@@ -1593,6 +1614,7 @@ static int plan_upload(Plan &pl, const std::vector<const HostProblem *> &hps, ch
             D.diagmag = P.diagmag;
             D.fast_ok = P.fast_ok;
             D.linv = (float *)(base + m.linv);
+            D.h16 = base + m.h16;
             pl.lud32.push_back(D);
 
         } else if (with_solve) {
@@ -1750,6 +1772,11 @@ static int mixed_launch(Plan &pl, char *base)
             e = prof_begin(PC_LU_TRSM);
             lu_launch_trsm64_f32(lud, o.ot_off, o.ot_cnt, ints, o.P, kTrsmSolve, st);
             prof_end(PC_LU_TRSM, e, o.f_trsm, 0);
+            if (o.sp_cnt) {
+                e = prof_begin(PC_LU_SPLIT);
+                lu_launch_split_f16x2(lud, o.sp_off, o.sp_cnt, ints, o.P - 2 * kPanel, o.P + 2 * kPanel, st);
+                prof_end(PC_LU_SPLIT, e, 0, 0);
+            }
         } else if (o.ot_cnt) {
             hipEvent_t e = prof_begin(PC_LU_TRSM);
             lu_launch_trsm64_f32(lud, o.ot_off, o.ot_cnt, ints, o.P, kTrsmSwap | kTrsmSolve, st);
@@ -1760,7 +1787,7 @@ static int mixed_launch(Plan &pl, char *base)
             hipEvent_t e = prof_begin(pc);
             if (o.combine)
                 lu_launch_update_f16x2(lud, 4 * o.on_off, o.on_cnt, gen_ptr(base, pl), o.P - 2 * kPanel,
-                                       o.P + 2 * kPanel, o.P + 2 * kPanel, o.P + 4 * kPanel, st);
+                                       o.P + 2 * kPanel, o.P + 2 * kPanel, o.P + 4 * kPanel, g_update_ring != 0, st);
             else
                 lu_launch_update_f32(lud, 4 * o.on_off, o.on_cnt, gen_ptr(base, pl), o.P, 2, o.P + 2 * kPanel,
                                      o.P + 2 * kPanel, o.P + 4 * kPanel, pl.upd_tm, st);
@@ -1777,7 +1804,7 @@ static int mixed_launch(Plan &pl, char *base)
             hipEvent_t e = prof_begin(pc);
             if (o.combine)
                 lu_launch_update_f16x2(lud, 4 * o.of_off, o.of_cnt, gen_ptr(base, pl), o.P - 2 * kPanel,
-                                       o.P + 2 * kPanel, o.P + 4 * kPanel, 0x7fffffff, st);
+                                       o.P + 2 * kPanel, o.P + 4 * kPanel, 0x7fffffff, g_update_ring != 0, st);
             else
                 lu_launch_update_f32(lud, 4 * o.of_off, o.of_cnt, gen_ptr(base, pl), o.P, 2, o.P + 2 * kPanel,
                                      o.P + 4 * kPanel, 0x7fffffff, pl.upd_tm, st);
@@ -2793,6 +2820,10 @@ int set_tunable(const char *name, long value)
         g_update_tile_m = (int)value;
         return FNEC_OK;
     }
+    if (k == "update_ring" && (value == 0 || value == 1)) {
+        g_update_ring = (int)value;
+        return FNEC_OK;
+    }
     if (k == "update_k128" && (value == 0 || value == 1)) {
         g_update_k128 = (int)value;
         return FNEC_OK;
@@ -2826,6 +2857,8 @@ long get_tunable(const char *name)
         return g_lanes_on;
     if (k == "update_tile_m")
         return g_update_tile_m;
+    if (k == "update_ring")
+        return g_update_ring;
     if (k == "update_k128")
         return g_update_k128;
     if (k == "batch_split_n")

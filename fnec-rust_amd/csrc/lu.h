@@ -27,8 +27,11 @@
 //   k_lu_trsm64_f32  FP32 outer step: interchanges of two 32-column panels
 //                and U12 = L11^{-1} A12 on MFMA for the columns right of both
 //                (together, or as two launches).
-//   k_lu_update_f16x2  the K = 128 trailing update of two outer steps on the
-//                f16 matrix cores (operands split into hi + lo f16 pairs).
+//   k_split_f16x2  the K = 128 passes' L21 and U12 split into hi + lo f16
+//                pairs, once per combining step.
+//   k_lu_update_f16x2s / k_lu_update_f16x2d  the K = 128 trailing update of
+//                two outer steps on the f16 matrix cores (operands streamed
+//                through an LDS ring by DMA / loaded into registers).
 // The FP32 factorisation (mixed path) is two-level: each 64-column step
 // factors two 32-column panels (the second after a narrow K = 32 update of
 // its columns only), then updates the trailing matrix once with K = 64, so
@@ -1022,43 +1025,114 @@ __device__ __forceinline__ void split_f16x2(float x, _Float16 &hi, _Float16 &lo)
     lo = (_Float16)((x - (float)h) * 2048.f);
 }
 
+// The K = 128 passes' operands, split once per combining step: L21 (rows
+// [rbeg, ldz) of the columns [kbeg, kbeg + 128)) and U12 (rows [kbeg, kbeg +
+// 128) of the columns [rbeg, ldz)) into f16 hi / lo planes of the real and
+// imaginary parts. Halves h16[kind][chunk * 4 + plane][g][index][8] (kind 0:
+// L21 by row; 1: U12 by column; chunk: 32 k; g: 8 k of the chunk), so that
+// the 8 k one MFMA lane supplies are 16 contiguous bytes and 16 consecutive
+// rows of one k group are 256. Rows and columns at or past the order are
+// written as 0. Work item (problem, block of 64 rows / columns, chunk * 2 +
+// kind); 8 k of one index per thread: for L21 thread t takes row (t & 63) and
+// group (t >> 6) (a load instruction reads 64 consecutive rows of a column),
+// for U12 column (t >> 2) and group (t & 3) (four lanes read a column's 32
+// contiguous k).
+__global__ __launch_bounds__(256) void k_split_f16x2(const LuDesc<float> *__restrict__ descs,
+                                                     const int4 *__restrict__ work, int kbeg, int rbeg)
+{
+    const int4 wk = work[blockIdx.x];
+    const LuDesc<float> &D = descs[wk.x];
+    if (D.info[0] != 0)
+        return;
+    const int s = D.m, lda = D.lda;
+    const int c = wk.z >> 1, kind = wk.z & 1, t = threadIdx.x;
+    const int li = kind == 0 ? (t & 63) : (t >> 2), g = kind == 0 ? (t >> 6) : (t & 3);
+    const int idx = rbeg + wk.y * 64 + li, k0 = kbeg + 32 * c + 8 * g;
+    const float *ar = D.ar, *ai = D.ai;
+    float xr[8], xi[8];
+    if (kind == 0) {
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            xr[q] = xi[q] = 0.f;
+            if (idx < s) {
+                const size_t o = (size_t)(k0 + q) * lda + idx;
+                xr[q] = ar[o];
+                xi[q] = ai[o];
+            }
+        }
+    } else {
+        float4 b0r = make_float4(0.f, 0.f, 0.f, 0.f), b1r = b0r, b0i = b0r, b1i = b0r;
+        if (idx < min(D.ncols, s)) {
+            const size_t o = (size_t)idx * lda + k0;
+            b0r = *(const float4 *)(ar + o);
+            b1r = *(const float4 *)(ar + o + 4);
+            b0i = *(const float4 *)(ai + o);
+            b1i = *(const float4 *)(ai + o + 4);
+        }
+        xr[0] = b0r.x, xr[1] = b0r.y, xr[2] = b0r.z, xr[3] = b0r.w;
+        xr[4] = b1r.x, xr[5] = b1r.y, xr[6] = b1r.z, xr[7] = b1r.w;
+        xi[0] = b0i.x, xi[1] = b0i.y, xi[2] = b0i.z, xi[3] = b0i.w;
+        xi[4] = b1i.x, xi[5] = b1i.y, xi[6] = b1i.z, xi[7] = b1i.w;
+    }
+    h8 pl[4];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        _Float16 a, b;
+        split_f16x2(xr[q], a, b);
+        pl[0][q] = a, pl[1][q] = b;
+        split_f16x2(xi[q], a, b);
+        pl[2][q] = a, pl[3][q] = b;
+    }
+    _Float16 *h = (_Float16 *)D.h16 + (size_t)kind * 16 * lda * 32;
+#pragma unroll
+    for (int p = 0; p < 4; ++p)
+        *(h8 *)(h + (((size_t)(c * 4 + p) * 4 + g) * lda + idx) * 8) = pl[p];
+}
+
 // FP32 trailing update on the f16 matrix cores, for K = 128 passes: A22 -=
 // L21 U12 over 64 x 64 tiles, with the work list, bounds and launch shape of
-// k_lu_update_f32<64, 64>. Every FP32 operand is split by split_f16x2, and a
-// real product a b is taken as hi_a hi_b + 2^-11 (hi_a lo_b + lo_a hi_b) with
-// FP32 accumulation: relative error about 2^-21 per product (the dropped
-// lo lo term and the rounding of lo), where FP32 products have 2^-24. The
-// factor is a CG preconditioner, so this changes the iteration's rate, not
-// the solution it converges to. Operands must stay below 65504 in modulus
-// (the f16 range of hi): L21 is bounded by 1 through partial pivoting and
-// U12 by the pivot growth times max |Z|, whose Hallén entries are O(10). A
-// larger operand would turn the factor non-finite, which the solve reports
-// as a failure status (a singular pivot or a CG breakdown), never as a
-// wrong result. Per 16 x 16 tile and 32-deep K chunk: 12
-// v_mfma_f32_16x16x32_f16 (hi hi and two cross terms for each of the four
-// real products of the complex product), where the FP32 forms issue 24 or
-// 32 v_mfma_f32_16x16x4f32 of the same issue cost each. A K = 64 pass is
-// bound by the read and write of C and gains little from this; a K = 128
-// pass, with half the C traffic per flop, runs about 1.5x the FP32 K = 64
-// form's rate.
+// k_lu_update_f32<64, 64>. Every FP32 operand is split by split_f16x2 (once
+// per combining step, k_split_f16x2), and a real product a b is taken as
+// hi_a hi_b + 2^-11 (hi_a lo_b + lo_a hi_b) with FP32 accumulation: relative
+// error about 2^-21 per product (the dropped lo lo term and the rounding of
+// lo), where FP32 products have 2^-24. The factor is a CG preconditioner, so
+// this changes the iteration's rate, not the solution it converges to.
+// Operands must stay below 65504 in modulus (the f16 range of hi): L21 is
+// bounded by 1 through partial pivoting and U12 by the pivot growth times
+// max |Z|, whose Hallén entries are O(10). A larger operand would turn the
+// factor non-finite, which the solve reports as a failure status (a singular
+// pivot or a CG breakdown), never as a wrong result. Per 16 x 16 tile and
+// 32-deep K chunk: 12 v_mfma_f32_16x16x32_f16 (hi hi and two cross terms for
+// each of the four real products of the complex product), where the FP32
+// forms issue 24 or 32 v_mfma_f32_16x16x4f32 of the same issue cost each.
 //
-// Loader thread t takes row (t & 63) of L21 at the 8 columns
-// k0 + 8 (t >> 6) .. + 7 (each load instruction reads 64 consecutive rows of
-// one column) and column (t & 63) of U12 at rows k0 + 8 (t >> 6) .. + 7 (two
-// float4 per plane). It splits the values and writes each plane as one
-// 16-byte run of LDS laid out [plane][row][k]. That run is an MFMA operand
-// as it stands: lane l supplies 8 consecutive k of row l % 16, with the k
-// group l / 16 (any fixed assignment of k to lane group and element is the
-// same dot product, as long as L21 and U12 use the same one, as here).
-// C is read in the epilogue, which keeps the kernel within 168 registers:
-// three workgroups per CU.
-constexpr int kH2Ld = 40; // halves per LDS row: 32 k and 8 of padding
+// Two forms. k_lu_update_f16x2d: each wavefront loads its MFMA operands
+// straight from the split layout into registers: lane l of a 16-row operand
+// reads the 16 bytes of row l % 16 in k group l / 16, so one load
+// instruction reads four runs of 256 contiguous bytes. A wavefront owns a
+// 32 x 32 block of the tile (2 x 2 MFMA tiles); chunk c + 1's operands are
+// loaded while chunk c's products issue (double buffer in registers), and C
+// is loaded during the last chunk. k_lu_update_f16x2s streams the operands
+// through LDS (below). Both issue the same products in the same
+// order per accumulator, so they give the same bits.
 
-__global__ __launch_bounds__(256, 3) void k_lu_update_f16x2(const LuDesc<float> *__restrict__ descs,
-                                                            const int4 *__restrict__ work, int cnt, int kbeg,
-                                                            int nk, int rbeg, int cbeg, int cend)
+// A pointer into global memory (address space 1), for kernels whose
+// pointers arrive through descriptors and would otherwise be generic (flat).
+template <typename T> using gptr = __attribute__((address_space(1))) T *;
+
+__device__ __forceinline__ h8 neg_h8(h8 x)
 {
-    constexpr int TM = 64, TN = 64;
+    typedef unsigned u4v __attribute__((ext_vector_type(4)));
+    u4v u = __builtin_bit_cast(u4v, x);
+    u ^= 0x80008000u; // the sign bits of the eight halves
+    return __builtin_bit_cast(h8, u);
+}
+
+__global__ __launch_bounds__(256, 2) void k_lu_update_f16x2d(const LuDesc<float> *__restrict__ descs,
+                                                             const int4 *__restrict__ work, int cnt, int kbeg,
+                                                             int rbeg, int cbeg, int cend)
+{
+    constexpr int TM = 64, TN = 64, NK = 4;
     const int per = (cnt + kXcds - 1) / kXcds;
     const int item = (blockIdx.x % kXcds) * per + blockIdx.x / kXcds;
     if (item >= cnt)
@@ -1066,107 +1140,75 @@ __global__ __launch_bounds__(256, 3) void k_lu_update_f16x2(const LuDesc<float> 
     const int4 wk = work[item];
     const LuDesc<float> &D = descs[wk.x];
     const int s = D.m, nc = min(D.ncols, cend);
-    if (D.info[0] != 0 || rbeg >= s || kbeg + kPanel * nk > s)
+    if (D.info[0] != 0 || rbeg >= s || kbeg + kPanel * NK > s)
         return;
     const int lda = D.lda;
     const int r0 = rbeg + wk.y * TM, c0 = cbeg + wk.z * TN;
-    // planes: hi and lo of the real part, hi and lo of the imaginary part
-    __shared__ __align__(16) _Float16 sa[4][TM][kH2Ld];
-    __shared__ __align__(16) _Float16 sb[4][TN][kH2Ld];
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const int wr = wave >> 1, wc = wave & 1;
     const int lrow = lane & 15, lk = lane >> 4;
-    const int ld_r = t & 63, ld_g = t >> 6;
-    float *ar = D.ar, *ai = D.ai;
-    float xa_r[8], xa_i[8], xb_r[8], xb_i[8];
-    auto load_ab = [&](int k0) {
-        const int gr = r0 + ld_r;
-        const int kc = k0 + 8 * ld_g;
+    const size_t ps = (size_t)lda * 32; // halves per plane of a chunk
+    // global-address-space views: loads then count on vmcnt alone, so a wait
+    // for one buffer leaves the other's loads in flight
+    const gptr<_Float16> h = (gptr<_Float16>)D.h16;
+    const gptr<_Float16> pa = h + ((size_t)lk * lda + r0 + wr * 32 + lrow) * 8;
+    const gptr<_Float16> pb = h + 16 * ps + ((size_t)lk * lda + c0 + wc * 32 + lrow) * 8;
+    h8 A[2][2][4], B[2][2][4]; // [buffer][16-row block][plane]
+    auto load = [&](int buf, int chunk) {
 #pragma unroll
-        for (int q = 0; q < 8; ++q) {
-            xa_r[q] = xa_i[q] = 0.f;
-            if (gr < s) {
-                const size_t o = (size_t)(kc + q) * lda + gr;
-                xa_r[q] = ar[o];
-                xa_i[q] = ai[o];
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int p = 0; p < 4; ++p) {
+                A[buf][a][p] = *(gptr<h8>)(pa + (size_t)(chunk * 4 + p) * ps + a * 16 * 8);
+                B[buf][a][p] = *(gptr<h8>)(pb + (size_t)(chunk * 4 + p) * ps + a * 16 * 8);
             }
-        }
-        const int gc = c0 + ld_r;
-        float4 b0r = make_float4(0.f, 0.f, 0.f, 0.f), b1r = b0r, b0i = b0r, b1i = b0r;
-        if (gc < nc) {
-            const size_t o = (size_t)gc * lda + kc;
-            b0r = *(const float4 *)(ar + o);
-            b1r = *(const float4 *)(ar + o + 4);
-            b0i = *(const float4 *)(ai + o);
-            b1i = *(const float4 *)(ai + o + 4);
-        }
-        xb_r[0] = b0r.x, xb_r[1] = b0r.y, xb_r[2] = b0r.z, xb_r[3] = b0r.w;
-        xb_r[4] = b1r.x, xb_r[5] = b1r.y, xb_r[6] = b1r.z, xb_r[7] = b1r.w;
-        xb_i[0] = b0i.x, xb_i[1] = b0i.y, xb_i[2] = b0i.z, xb_i[3] = b0i.w;
-        xb_i[4] = b1i.x, xb_i[5] = b1i.y, xb_i[6] = b1i.z, xb_i[7] = b1i.w;
     };
-    auto store_ab = [&]() {
-        h8 hr, lr, hi, li;
-#pragma unroll
-        for (int q = 0; q < 8; ++q) {
-            _Float16 a, b;
-            split_f16x2(xa_r[q], a, b);
-            hr[q] = a, lr[q] = b;
-            split_f16x2(xa_i[q], a, b);
-            hi[q] = a, li[q] = b;
-        }
-        *(h8 *)&sa[0][ld_r][8 * ld_g] = hr;
-        *(h8 *)&sa[1][ld_r][8 * ld_g] = lr;
-        *(h8 *)&sa[2][ld_r][8 * ld_g] = hi;
-        *(h8 *)&sa[3][ld_r][8 * ld_g] = li;
-#pragma unroll
-        for (int q = 0; q < 8; ++q) {
-            _Float16 a, b;
-            split_f16x2(xb_r[q], a, b);
-            hr[q] = a, lr[q] = b;
-            split_f16x2(xb_i[q], a, b);
-            hi[q] = a, li[q] = b;
-        }
-        *(h8 *)&sb[0][ld_r][8 * ld_g] = hr;
-        *(h8 *)&sb[1][ld_r][8 * ld_g] = lr;
-        *(h8 *)&sb[2][ld_r][8 * ld_g] = hi;
-        *(h8 *)&sb[3][ld_r][8 * ld_g] = li;
-    };
-    // re_m, im_m: hi hi terms; re_x, im_x: cross terms, in units of 2^-11
     f4 re_m[2][2], re_x[2][2], im_m[2][2], im_x[2][2];
 #pragma unroll
     for (int a = 0; a < 2; ++a)
 #pragma unroll
         for (int b = 0; b < 2; ++b)
             re_m[a][b] = re_x[a][b] = im_m[a][b] = im_x[a][b] = f4{0.f, 0.f, 0.f, 0.f};
-    load_ab(kbeg);
-    store_ab();
-    __syncthreads();
-    for (int chunk = 0; chunk < nk; ++chunk) {
-        if (chunk + 1 < nk)
-            load_ab(kbeg + kPanel * (chunk + 1)); // in flight during this chunk's products
+    const gptr<float> ar = (gptr<float>)D.ar, ai = (gptr<float>)D.ai;
+    f4 cr[2][2], ci[2][2];
+    load(0, 0);
+#pragma unroll
+    for (int chunk = 0; chunk < NK; ++chunk) {
+        const int cur = chunk & 1;
+        if (chunk + 1 < NK) {
+            load(cur ^ 1, chunk + 1);
+        } else {
+#pragma unroll
+            for (int a = 0; a < 2; ++a)
+#pragma unroll
+                for (int b = 0; b < 2; ++b) {
+                    // unconditional (a branch here would make the compiler
+                    // wait for every load in flight): rows below the order
+                    // stay inside the padded column, and a column past the
+                    // last carried one reads the last, its result unused
+                    const int gi = r0 + wr * 32 + a * 16 + 4 * lk;
+                    const int gj = min(c0 + wc * 32 + b * 16 + lrow, nc - 1);
+                    const size_t o = (size_t)gj * lda + gi;
+                    cr[a][b] = *(gptr<f4>)(ar + o);
+                    ci[a][b] = *(gptr<f4>)(ai + o);
+                }
+        }
+        // keep this chunk's products after the next chunk's loads, so that
+        // the waits for this chunk's operands leave those loads in flight
+        __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
         for (int a = 0; a < 2; ++a) {
-            h8 fa[4], na[2];
-#pragma unroll
-            for (int p = 0; p < 4; ++p)
-                fa[p] = *(const h8 *)&sa[p][wr * 32 + a * 16 + lrow][8 * lk];
-            na[0] = -fa[2]; // -hi, -lo of the imaginary part
-            na[1] = -fa[3];
+            const h8 *fa = A[cur][a];
+            const h8 na0 = neg_h8(fa[2]), na1 = neg_h8(fa[3]);
 #pragma unroll
             for (int b = 0; b < 2; ++b) {
-                h8 fb[4];
-#pragma unroll
-                for (int p = 0; p < 4; ++p)
-                    fb[p] = *(const h8 *)&sb[p][wc * 32 + b * 16 + lrow][8 * lk];
-                // Re: ar br - ai bi
+                const h8 *fb = B[cur][b];
                 re_m[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(fa[0], fb[0], re_m[a][b], 0, 0, 0);
-                re_m[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(na[0], fb[2], re_m[a][b], 0, 0, 0);
+                re_m[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(na0, fb[2], re_m[a][b], 0, 0, 0);
                 re_x[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(fa[0], fb[1], re_x[a][b], 0, 0, 0);
                 re_x[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(fa[1], fb[0], re_x[a][b], 0, 0, 0);
-                re_x[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(na[0], fb[3], re_x[a][b], 0, 0, 0);
-                re_x[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(na[1], fb[2], re_x[a][b], 0, 0, 0);
-                // Im: ar bi + ai br
+                re_x[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(na0, fb[3], re_x[a][b], 0, 0, 0);
+                re_x[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(na1, fb[2], re_x[a][b], 0, 0, 0);
                 im_m[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(fa[0], fb[2], im_m[a][b], 0, 0, 0);
                 im_m[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(fa[2], fb[0], im_m[a][b], 0, 0, 0);
                 im_x[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(fa[0], fb[3], im_x[a][b], 0, 0, 0);
@@ -1175,11 +1217,7 @@ __global__ __launch_bounds__(256, 3) void k_lu_update_f16x2(const LuDesc<float> 
                 im_x[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(fa[3], fb[0], im_x[a][b], 0, 0, 0);
             }
         }
-        if (chunk + 1 < nk) {
-            __syncthreads(); // this chunk's operands consumed
-            store_ab();
-            __syncthreads();
-        }
+        __builtin_amdgcn_sched_barrier(0);
     }
     constexpr float kLo = 0x1p-11f;
 #pragma unroll
@@ -1191,18 +1229,260 @@ __global__ __launch_bounds__(256, 3) void k_lu_update_f16x2(const LuDesc<float> 
             if (gj >= nc || gi >= s)
                 continue;
             const size_t o = (size_t)gj * lda + gi;
-            const float4 x = *(const float4 *)(ar + o), y = *(const float4 *)(ai + o);
-            const f4 cr = f4{x.x, x.y, x.z, x.w}, ci = f4{y.x, y.y, y.z, y.w};
             f4 vr, vi;
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
-                vr[q] = cr[q] - fmaf(re_x[a][b][q], kLo, re_m[a][b][q]);
-                vi[q] = ci[q] - fmaf(im_x[a][b][q], kLo, im_m[a][b][q]);
+                vr[q] = cr[a][b][q] - fmaf(re_x[a][b][q], kLo, re_m[a][b][q]);
+                vi[q] = ci[a][b][q] - fmaf(im_x[a][b][q], kLo, im_m[a][b][q]);
             }
-            // rows past m (at most 3, inside the padded column) keep 0
-            *(float4 *)(ar + o) = make_float4(vr[0], vr[1], vr[2], vr[3]);
-            *(float4 *)(ai + o) = make_float4(vi[0], vi[1], vi[2], vi[3]);
+            *(gptr<f4>)(ar + o) = vr;
+            *(gptr<f4>)(ai + o) = vi;
         }
+}
+
+// The update streamed through a five-stage LDS ring by the gfx950 LDS DMA
+// loads (global_load_lds_dwordx4): a persistent grid, one workgroup per CU,
+// each taking every kUpdRingSlots-th tile of its XCD's contiguous share of
+// the list. The 32 KiB stage of a chunk holds 32 MFMA fragments (L21 row
+// blocks and U12 column blocks of 16, four planes each), each written by one
+// DMA instruction in lane order (lane l: row l % 16, k group l / 16), so a
+// wavefront reads a fragment with one conflict-free ds_read_b128 at lane x
+// 16. Chunks stream across tiles: while chunk q is multiplied, chunks q + 1
+// .. q + 4 (the next tile's during this one) are in flight, and C is loaded
+// into registers a tile ahead, so the start-up latency of a tile is hidden
+// behind the products of the one before. Before stage q is read, each
+// wavefront waits for its own DMA of chunk q (the loads issued later stay in
+// flight: loads complete in order) and the workgroup meets at a barrier,
+// which also frees stage q - 1 for chunk q + 4. The products and their order
+// per accumulator are those of k_lu_update_f16x2d.
+constexpr int kUpdRingSlots = 32;  // workgroups per XCD: one on each of its CUs
+// 5 x 32 KiB: the whole LDS of a CU. The DMA runs kUpdRingStages - 1 = 4
+// chunks (one tile) ahead, which the loop's chunk and count bookkeeping
+// assumes: chunk c of the next tile is issued at chunk c of this one.
+constexpr int kUpdRingStages = 5;
+
+// The ring's stage hand-over: this wavefront's DMA of the chunk about to be
+// read has landed (at most N of its vector memory operations outstanding,
+// which complete in order), its LDS reads of the previous chunk have
+// returned, then the workgroup's barrier. Nothing is scheduled across it.
+template <int N> __device__ __forceinline__ void ring_handover()
+{
+    __builtin_amdgcn_sched_barrier(0);
+    // s_waitcnt vmcnt(N) lgkmcnt(0), expcnt left alone (gfx9 encoding)
+    __builtin_amdgcn_s_waitcnt((N & 15) | ((N >> 4) << 14) | (7 << 4));
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_sched_barrier(0);
+}
+
+__global__ __launch_bounds__(256, 1) void k_lu_update_f16x2s(const LuDesc<float> *__restrict__ descs,
+                                                             const int4 *__restrict__ work, int cnt, int kbeg,
+                                                             int rbeg, int cbeg, int cend)
+{
+    constexpr int TM = 64, TN = 64;
+    __shared__ __align__(16) h8 ring[kUpdRingStages][32][64];
+    const int per = (cnt + kXcds - 1) / kXcds;
+    const int lo = (blockIdx.x % kXcds) * per, hi = min(cnt, lo + per);
+    const int slot = blockIdx.x / kXcds, nslot = gridDim.x / kXcds;
+    // the wavefront index as a scalar: the DMA's fragment choice and LDS
+    // address then stay in scalar registers
+    const int t = threadIdx.x, lane = t & 63, wave = __builtin_amdgcn_readfirstlane(t >> 6);
+    const int wr = wave >> 1, wc = wave & 1;
+    const int lrow = lane & 15, lk = lane >> 4;
+    typedef const __attribute__((address_space(4))) LuDesc<float> CDesc;
+    typedef const __attribute__((address_space(4))) int CInt;
+    struct Tile {
+        bool ok;
+        int it, s, nc, lda, r0, c0;
+        float *ar, *ai;
+        const _Float16 *h;
+    };
+    // the first tile the update computes among items it, it + nslot, ...
+    // (a problem that is singular or too small for this pass is skipped);
+    // ok = false when there is none. Scalar loads through the constant
+    // address space, which leave the vector loads' counter alone.
+    auto next_tile = [&](int it) {
+        Tile T;
+        T.ok = false;
+        for (; it < hi; it += nslot) {
+            CInt *wk = (CInt *)work + 4 * (size_t)it;
+            CDesc &D = ((CDesc *)descs)[wk[0]];
+            T.s = D.m;
+            if (*(CInt *)D.info != 0 || rbeg >= T.s || kbeg + 4 * kPanel > T.s)
+                continue;
+            T.ok = true;
+            T.it = it;
+            T.nc = min(D.ncols, cend);
+            T.lda = D.lda;
+            T.r0 = rbeg + wk[1] * TM;
+            T.c0 = cbeg + wk[2] * TN;
+            T.ar = D.ar;
+            T.ai = D.ai;
+            T.h = (const _Float16 *)D.h16;
+            break;
+        }
+        return T;
+    };
+    // this wavefront's 8 of the 32 fragments of chunk c of tile X into stage S
+    auto dma = [&](const Tile &X, int chunk, h8 (*S)[64]) {
+        const unsigned ps = (unsigned)X.lda * 64; // bytes per (chunk, plane)
+        const gptr<const char> hb = (gptr<const char>)X.h;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const int f = wave * 8 + j, kind = f >> 4, blk = (f >> 2) & 3, p = f & 3;
+            const int idx = (kind ? X.c0 : X.r0) + blk * 16 + lrow;
+            const unsigned off =
+                kind * 16 * ps + (unsigned)(chunk * 4 + p) * ps + ((unsigned)(lk * X.lda + idx)) * 16;
+            __builtin_amdgcn_global_load_lds((gptr<const void>)(hb + off),
+                                             (__attribute__((address_space(3))) void *)&S[f][0], 16, 0, 0);
+        }
+    };
+    f4 re_m[2][2], re_x[2][2], im_m[2][2], im_x[2][2];
+    // A chunk's products from stage S. The fragments are read by inline
+    // ds_read_b128 blocks that wait for their own results: the compiler
+    // then has no LDS read to order after the DMA, which it would otherwise
+    // guard with a wait for every vector memory operation in flight (it
+    // cannot tell the DMA of this stage from the later ones once stores are
+    // outstanding); the hand-over already covers this stage's DMA.
+    auto products = [&](h8 (*S)[64]) {
+        const unsigned base = (unsigned)(size_t)&S[0][lane]; // LDS byte address
+        const unsigned va = base + (wr * 2) * 4 * 1024, vb = base + (16 + wc * 2 * 4) * 1024;
+        h8 fa[4], fb[2][4];
+        asm volatile("ds_read_b128 %0, %12\n\t"
+                     "ds_read_b128 %1, %12 offset:1024\n\t"
+                     "ds_read_b128 %2, %12 offset:2048\n\t"
+                     "ds_read_b128 %3, %12 offset:3072\n\t"
+                     "ds_read_b128 %4, %13\n\t"
+                     "ds_read_b128 %5, %13 offset:1024\n\t"
+                     "ds_read_b128 %6, %13 offset:2048\n\t"
+                     "ds_read_b128 %7, %13 offset:3072\n\t"
+                     "ds_read_b128 %8, %13 offset:4096\n\t"
+                     "ds_read_b128 %9, %13 offset:5120\n\t"
+                     "ds_read_b128 %10, %13 offset:6144\n\t"
+                     "ds_read_b128 %11, %13 offset:7168\n\t"
+                     "s_waitcnt lgkmcnt(0)"
+                     : "=&v"(fa[0]), "=&v"(fa[1]), "=&v"(fa[2]), "=&v"(fa[3]), "=&v"(fb[0][0]), "=&v"(fb[0][1]),
+                       "=&v"(fb[0][2]), "=&v"(fb[0][3]), "=&v"(fb[1][0]), "=&v"(fb[1][1]), "=&v"(fb[1][2]),
+                       "=&v"(fb[1][3])
+                     : "v"(va), "v"(vb));
+#pragma unroll
+        for (int a = 0; a < 2; ++a) {
+            if (a == 1)
+                asm volatile("ds_read_b128 %0, %4 offset:4096\n\t"
+                             "ds_read_b128 %1, %4 offset:5120\n\t"
+                             "ds_read_b128 %2, %4 offset:6144\n\t"
+                             "ds_read_b128 %3, %4 offset:7168\n\t"
+                             "s_waitcnt lgkmcnt(0)"
+                             : "=&v"(fa[0]), "=&v"(fa[1]), "=&v"(fa[2]), "=&v"(fa[3])
+                             : "v"(va));
+            const h8 na0 = neg_h8(fa[2]), na1 = neg_h8(fa[3]);
+#pragma unroll
+            for (int b = 0; b < 2; ++b) {
+                const h8 *g = fb[b];
+                re_m[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(fa[0], g[0], re_m[a][b], 0, 0, 0);
+                re_m[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(na0, g[2], re_m[a][b], 0, 0, 0);
+                re_x[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(fa[0], g[1], re_x[a][b], 0, 0, 0);
+                re_x[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(fa[1], g[0], re_x[a][b], 0, 0, 0);
+                re_x[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(na0, g[3], re_x[a][b], 0, 0, 0);
+                re_x[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(na1, g[2], re_x[a][b], 0, 0, 0);
+                im_m[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(fa[0], g[2], im_m[a][b], 0, 0, 0);
+                im_m[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(fa[2], g[0], im_m[a][b], 0, 0, 0);
+                im_x[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(fa[0], g[3], im_x[a][b], 0, 0, 0);
+                im_x[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(fa[1], g[2], im_x[a][b], 0, 0, 0);
+                im_x[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(fa[2], g[1], im_x[a][b], 0, 0, 0);
+                im_x[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(fa[3], g[0], im_x[a][b], 0, 0, 0);
+            }
+        }
+    };
+    // C of tile X (unconditional: rows below the order stay inside the
+    // padded column, and a column past the last carried one reads the last,
+    // its result unused)
+    auto load_c = [&](const Tile &X, f4 (&r)[2][2], f4 (&m)[2][2]) {
+        // uniform plane bases plus 32-bit byte offsets (a plane is at most
+        // ldz x n FP32): one VGPR per load and no 64-bit temporaries
+        const gptr<const char> br = (gptr<const char>)X.ar, bi = (gptr<const char>)X.ai;
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int b = 0; b < 2; ++b) {
+                const int gi = X.r0 + wr * 32 + a * 16 + 4 * lk;
+                const int gj = min(X.c0 + wc * 32 + b * 16 + lrow, X.nc - 1);
+                const unsigned o = ((unsigned)gj * X.lda + gi) * 4;
+                r[a][b] = *(gptr<const f4>)(br + o);
+                m[a][b] = *(gptr<const f4>)(bi + o);
+            }
+    };
+    // one tile: its C is in (cr, ci), loaded a tile ahead; the next tile's C
+    // goes to (cn, cin). Returns the next tile.
+    // one tile: its C is in (cr, ci), loaded a tile ahead; the next tile's C
+    // goes to (cn, cin). Chunk q of the stream (four per tile) sits in stage
+    // q mod kUpdRingStages; sq is the current chunk's stage. Returns the
+    // next tile.
+    int sq = 0;
+    auto run_tile = [&](const Tile &T, f4 (&cr)[2][2], f4 (&ci)[2][2], f4 (&cn)[2][2], f4 (&cin)[2][2]) {
+        const Tile N = next_tile(T.it + nslot);
+        const Tile Nd = N.ok ? N : T; // with no next tile, a harmless reload keeps the counts fixed
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int b = 0; b < 2; ++b)
+                re_m[a][b] = re_x[a][b] = im_m[a][b] = im_x[a][b] = f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            // Issued after this chunk's DMA: the next three chunks and one
+            // tile's C (the DMA runs four chunks ahead and C is issued once
+            // per tile): 32 loads of this wavefront. Outstanding stores
+            // only add to the count.
+            ring_handover<32>();
+            const int sd = sq == 0 ? kUpdRingStages - 1 : sq - 1; // stage of chunk q + 4, freed by the barrier
+            dma(Nd, c, ring[sd]);
+            if (c == 0)
+                load_c(Nd, cn, cin); // the next tile's C, a tile ahead
+            __builtin_amdgcn_sched_barrier(0); // the loads issue before the products
+            products(ring[sq]);
+            sq = sq + 1 == kUpdRingStages ? 0 : sq + 1;
+        }
+        constexpr float kLo = 0x1p-11f;
+        const gptr<float> ar = (gptr<float>)T.ar, ai = (gptr<float>)T.ai;
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int b = 0; b < 2; ++b) {
+                const int gi = T.r0 + wr * 32 + a * 16 + 4 * lk; // rows gi .. gi + 3
+                const int gj = T.c0 + wc * 32 + b * 16 + lrow;
+                if (gj >= T.nc || gi >= T.s)
+                    continue;
+                const size_t o = (size_t)gj * T.lda + gi;
+                f4 vr, vi;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    vr[q] = cr[a][b][q] - fmaf(re_x[a][b][q], kLo, re_m[a][b][q]);
+                    vi[q] = ci[a][b][q] - fmaf(im_x[a][b][q], kLo, im_m[a][b][q]);
+                }
+                *(gptr<f4>)(ar + o) = vr;
+                *(gptr<f4>)(ai + o) = vi;
+            }
+        return N;
+    };
+    // two register sets for C, alternating between tiles (a copy from one
+    // to the other would wait for the loads in flight)
+    f4 c0r[2][2], c0i[2][2], c1r[2][2], c1i[2][2];
+    Tile T = next_tile(lo + slot);
+    if (!T.ok)
+        return;
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+        dma(T, c, ring[c]);
+    load_c(T, c0r, c0i);
+    while (true) {
+        T = run_tile(T, c0r, c0i, c1r, c1i);
+        if (!T.ok)
+            break;
+        T = run_tile(T, c1r, c1i, c0r, c0i);
+        if (!T.ok)
+            break;
+    }
+    // every DMA landed before the workgroup ends (the last tile's reloads)
+    __builtin_amdgcn_s_waitcnt(0);
 }
 
 // L11^{-1} of a 64-column outer step of the FP32 LU (L11 unit lower at
@@ -1439,12 +1719,27 @@ inline void lu_launch_trsm64_f32(const LuDesc<float> *d, size_t off, size_t cnt,
 
 // K = 128 trailing update (two outer steps' L21 and U12, columns / rows
 // [kbeg, kbeg + 128)) on the f16 matrix cores, 64 x 64 tiles.
+// K = 128 trailing update (two outer steps' L21 and U12, columns / rows
+// [kbeg, kbeg + 128), split by k_split_f16x2) on the f16 matrix cores, 64 x
+// 64 tiles: the LDS ring (persistent, one workgroup per CU) or the
+// register-direct form (one workgroup per tile).
 inline void lu_launch_update_f16x2(const LuDesc<float> *d, size_t off, size_t cnt, const int *ints, int kbeg,
-                                   int rbeg, int cbeg, int cend, hipStream_t st)
+                                   int rbeg, int cbeg, int cend, bool ring, hipStream_t st)
 {
     const unsigned grid = (unsigned)((cnt + kXcds - 1) / kXcds * kXcds);
-    hipLaunchKernelGGL(k_lu_update_f16x2, dim3(grid), dim3(256), 0, st, d, (const int4 *)(ints + off), (int)cnt,
-                       kbeg, 4, rbeg, cbeg, cend);
+    if (ring)
+        hipLaunchKernelGGL(k_lu_update_f16x2s, dim3(std::min<unsigned>(grid, kXcds * kUpdRingSlots)), dim3(256), 0,
+                           st, d, (const int4 *)(ints + off), (int)cnt, kbeg, rbeg, cbeg, cend);
+    else
+        hipLaunchKernelGGL(k_lu_update_f16x2d, dim3(grid), dim3(256), 0, st, d, (const int4 *)(ints + off),
+                           (int)cnt, kbeg, rbeg, cbeg, cend);
+}
+
+inline void lu_launch_split_f16x2(const LuDesc<float> *d, size_t off, size_t cnt, const int *ints, int kbeg,
+                                  int rbeg, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_split_f16x2, dim3((unsigned)cnt), dim3(256), 0, st, d, (const int4 *)(ints + off), kbeg,
+                       rbeg);
 }
 
 inline void lu_launch_l11_inv(const LuDesc<float> *d, size_t off, size_t cnt, const int *ints, int P,

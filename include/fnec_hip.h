@@ -327,6 +327,12 @@ int fnec_hip_debug_mfma_probe_f32(const float *a16x4, const float *b4x16, float 
  *                     product error about 2^-21 relative), reading and writing
  *                     the trailing matrix once per 128 columns. 0: one K = 64
  *                     FP32 pass per step.
+ *   "update_ring"     K = 128 passes: 1 streams the split operands through
+ *                     an LDS ring by DMA on a persistent grid (one workgroup
+ *                     per CU, all 160 KiB of its LDS; fastest alone); 0
+ *                     (default) loads them into registers, one workgroup per
+ *                     tile, which leaves room for kernels of other streams
+ *                     (the next step's panels) to run beside it. Same bits.
  *   "mix_tol_exp"     mixed path: the CG stops when the normal-equation
  *                     residual is at most 10^-value of its right-hand side.
  *                     8 .. 15 (default 14). Not tuned automatically: it trades
