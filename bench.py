@@ -62,39 +62,16 @@ RP_PHI = np.repeat(np.arange(0.0, 360.0, 2.0), 91)
 
 
 def build_requests(stream):
-    """Host side: decks -> Hallén inputs (worker semantics) -> fnec_request."""
+    """Host side: decks -> Hallén inputs (worker semantics) -> fnec_request,
+    each with buffers for its currents and, with a grid, its pattern rows, so
+    that a served request returns what a worker reply carries."""
     items = []
     for r in stream:
         d = F.Deck(r["deck"])
         p = d.problem("worker", r["freq_hz"])
         d.close()
-        keep = {
-            "segs": np.ascontiguousarray(p["segs"]), "rhs": np.ascontiguousarray(p["rhs"]),
-            "cos": np.ascontiguousarray(p["cos"]), "wf": np.ascontiguousarray(p["wf"]),
-            "wl": np.ascontiguousarray(p["wl"]), "ja": np.ascontiguousarray(p["ja"]),
-            "jb": np.ascontiguousarray(p["jb"]), "js": np.ascontiguousarray(p["js"]),
-        }
-        q = F.Request()
-        q.segs = F._p(keep["segs"])
-        q.n = len(keep["segs"])
-        q.rhs = F._p(keep["rhs"])
-        q.cos_vec = F._p(keep["cos"])
-        q.wire_first = F._p(keep["wf"])
-        q.wire_last = F._p(keep["wl"])
-        q.n_wires = len(keep["wf"])
-        q.j_a = F._p(keep["ja"])
-        q.j_b = F._p(keep["jb"])
-        q.j_sign = F._p(keep["js"])
-        q.n_junc = len(keep["ja"])
-        q.freq_hz = r["freq_hz"]
-        q.ground = p["ground_struct"]
-        q.feed_index = int(p["feed_index"])
-        q.v_feed = F.C64(p["v_feed"].real, p["v_feed"].imag)
-        if r["rp"]:
-            q.rp_theta = F._p(RP_THETA)
-            q.rp_phi = F._p(RP_PHI)
-            q.rp_npts = len(RP_THETA)
-        q.tag = r["id"]
+        q, keep = F.make_request(p, tag=r["id"], rp_theta=RP_THETA if r["rp"] else None,
+                                 rp_phi=RP_PHI if r["rp"] else None, outputs=True)
         items.append({"req": q, "keep": keep, "meta": r, "p": p})
     return items
 
@@ -241,9 +218,14 @@ def batch_overlap(batches):
             "two_or_more_on_device_frac": round(busy2 / span, 3) if span > 0 else 0.0}
 
 
-def plain_fp64(items, precision=F.PREC_FP64):
-    """One request at a time through fnec_hip_solve_hallen (+ fnec_hip_rp)."""
+def plain_fp64(items, precision=F.PREC_FP64, outputs=None):
+    """One request at a time through fnec_hip_solve_hallen (+ fnec_hip_rp).
+    With a dict ``outputs``, the served currents and pattern rows of each item
+    (its request's output buffers) are compared with this path's, outside the
+    timed loop's arithmetic: worst relative current difference (2-norm) and
+    worst linear-gain difference relative to the pattern's maximum."""
     zs = {}
+    t_cmp = 0.0
     t0 = time.perf_counter()
     for it in items:
         k = it["keep"]
@@ -252,9 +234,21 @@ def plain_fp64(items, precision=F.PREC_FP64):
                            r.freq_hz, r.ground, precision)
         n = len(k["segs"])
         zs[r.tag] = complex(r.v_feed.re, r.v_feed.im) / x[r.feed_index]
+        rows = None
         if it["meta"]["rp"]:
-            F.rp(k["segs"], x[:n], r.freq_hz, RP_THETA, RP_PHI, r.ground)
-    return time.perf_counter() - t0, zs
+            rows, _ = F.rp(k["segs"], x[:n], r.freq_hz, RP_THETA, RP_PHI, r.ground)
+        if outputs is not None:
+            tc = time.perf_counter()
+            dx = float(np.linalg.norm(k["x"][:n] - x[:n]) / np.linalg.norm(x[:n]))
+            outputs["max_rel_dx"] = max(outputs.get("max_rel_dx", 0.0), dx)
+            if rows is not None:
+                live = rows[:, 0] > -999.0
+                ga, gb = 10.0 ** (k["rows"][live, 0] / 10.0), 10.0 ** (rows[live, 0] / 10.0)
+                dg = float(np.max(np.abs(ga - gb)) / np.max(gb))
+                outputs["max_gain_diff"] = max(outputs.get("max_gain_diff", 0.0), dg)
+                outputs["patterns"] = outputs.get("patterns", 0) + 1
+            t_cmp += time.perf_counter() - tc
+    return time.perf_counter() - t0 - t_cmp, zs
 
 
 def cpu_model() -> str:
@@ -490,7 +484,8 @@ def main():
 
     timed_items = items[first:first + timed]
     sample_items = timed_items[:args.plain_sample] if args.plain_sample else timed_items
-    t_plain, zs_plain = plain_fp64(sample_items)
+    served_out = {}
+    t_plain, zs_plain = plain_fp64(sample_items, outputs=served_out)
     plain_rate = len(sample_items) / t_plain
     dz = max(abs(zs_served[k] - zs_plain[k]) / abs(zs_plain[k]) for k in zs_plain)
     print(f"[bench] plain FP64 one-at-a-time: {plain_rate:.1f} solves/s on {len(sample_items)}",
@@ -548,7 +543,10 @@ def main():
         threads = min(16, len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count() or 1)
         cpu, z_oracle = cpu_baseline(stream, [stream[first + i] for i in range(timed)], args.cpu_budget_s,
                                      threads)
-    acc = {"vs_plain_fp64": {"max_rel_dzin": dz, "requests": len(zs_plain)}}
+    acc = {"vs_plain_fp64": {"max_rel_dzin": dz, "requests": len(zs_plain),
+                             "max_rel_dcurrents": served_out.get("max_rel_dx"),
+                             "max_gain_diff": served_out.get("max_gain_diff"),
+                             "patterns": served_out.get("patterns", 0)}}
     if z_oracle:
         acc["vs_oracle"] = {
             "max_rel_dzin_served": max(abs(zs_served[k] - z) / abs(z) for k, z in z_oracle.items()),

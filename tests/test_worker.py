@@ -1,0 +1,117 @@
+"""CPU tests of fnec_hip_worker's protocol handling (no device needed): the
+nec_worker stdio loop and task validation, following the reference's own
+worker tests (crates/nec_worker/src/worker.rs:145-270, solve.rs:16-80):
+
+- VSWR: matched load 1, open / short circuit and negative resistance
+  infinite (written as null, as serde_json writes non-finite numbers), a
+  100-ohm load 2, a reactive load finite and above 1;
+- the loop: empty input and a lone shutdown give no output, blank lines are
+  skipped, lines after {"cmd":"shutdown"} (surrounding blanks allowed) are
+  not read, end of input ends the loop;
+- task errors: malformed JSON and missing fields (task_id "unknown"), bad
+  base64 and non-UTF-8 decks (parse_error with the task's id), unsupported
+  basis and loaded decks (unsupported_config), an empty or garbage deck
+  (geometry error), a deck without a type-0 EX (no feedpoint).
+The GPU test test_gpu_worker.py serves a request stream through it.
+"""
+import base64
+import json
+import os
+import subprocess
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+BIN = os.path.join(ROOT, "fnec-rust_amd", "bin", "fnec_hip_worker")
+
+DIPOLE = "CM t\nGW 1 11 0 0 -5 0 0 5 0.001\nGE 0\nEX 0 1 6 0 1 0\nFR 0 1 0 0 14.175 0\nEN\n"
+
+
+@pytest.fixture(scope="module", autouse=True)
+def built():
+    if not os.path.exists(BIN):
+        pytest.skip("fnec_hip_worker is not built (run __graft_entry__.build())")
+
+
+def run(text, *args):
+    p = subprocess.run([BIN, *args], input=text, capture_output=True, text=True, timeout=60)
+    assert p.returncode == 0, p.stderr
+    return [json.loads(line) for line in p.stdout.splitlines()]
+
+
+def task(tid, deck="", basis="hallen", freq=14.175e6, b64=None, **extra):
+    d = {"task_id": tid, "deck_hash": "0" * 64,
+         "deck_b64": b64 if b64 is not None else base64.b64encode(deck.encode()).decode(),
+         "solver_config": {"basis": basis, "ground_model": "none"}, "frequency_hz": freq}
+    d.update(extra)
+    return json.dumps(d)
+
+
+def vswr(re, im):
+    p = subprocess.run([BIN, "--vswr", str(re), str(im)], capture_output=True, text=True, timeout=30)
+    return json.loads(p.stdout)
+
+
+def test_vswr_cases():
+    assert abs(vswr(50.0, 0.0) - 1.0) < 1e-9
+    assert vswr(1e100, 0.0) is None
+    assert vswr(0.0, 0.0) is None
+    assert abs(vswr(100.0, 0.0) - 2.0) < 1e-9
+    v = vswr(50.0, 50.0)
+    assert v is not None and v > 1.0
+    assert vswr(-10.0, 0.0) is None
+
+
+def test_loop_empty_input_and_shutdown():
+    assert run("") == []
+    assert run('{"cmd":"shutdown"}\n') == []
+    assert run('\n\n{"cmd":"shutdown"}\n') == []
+    # nothing after the shutdown line is read
+    assert run('  {"cmd":"shutdown"}  \nnot json\n') == []
+    # another command is a task, and a malformed one
+    out = run('{"cmd":"other"}\n')
+    assert len(out) == 1 and out[0]["status"] == "error" and out[0]["task_id"] == "unknown"
+
+
+def test_malformed_and_incomplete_messages():
+    out = run('not json at all\n{"task_id":"t1"}\n[1,2]\n{"task_id":"t1","deck_hash":"x","deck_b64":"",'
+              '"solver_config":{},"frequency_hz":"14e6"}\n')
+    assert len(out) == 4
+    for r in out:
+        assert r["status"] == "error" and r["error_code"] == "parse_error" and r["task_id"] == "unknown"
+        assert r["error_message"].startswith("failed to deserialize task")
+    assert "missing field `deck_hash`" in out[1]["error_message"]
+
+
+def test_bad_base64_and_utf8():
+    bad = [task("b1", b64="!!!invalid!!!"), task("b2", b64="QUJD="), task("b3", b64="QUI"),
+           task("b4", b64="QUJ="),  # non-zero trailing bits
+           task("u1", b64=base64.b64encode(b"GW \xff\xfe").decode())]
+    out = run("\n".join(bad) + "\n")
+    assert [r["task_id"] for r in out] == ["b1", "b2", "b3", "b4", "u1"]
+    for r in out[:4]:
+        assert r["error_code"] == "parse_error" and r["error_message"].startswith("base64 decode failed")
+    assert out[4]["error_code"] == "parse_error" and out[4]["error_message"].startswith("deck is not valid UTF-8")
+    assert all(r["frequency_hz"] == 14.175e6 for r in out)
+
+
+def test_deck_level_errors():
+    nofeed = "CM test\nGW 0 1 0 0 0 0 0 1 0.001\nGE 0\nFR 0 1 0 0 14.175 0\nEN\n"
+    loaded = DIPOLE.replace("GE 0\n", "GE 0\nLD 0 1 6 6 50 0 0\n")
+    out = run("\n".join([task("basis", DIPOLE, basis="pulse"), task("empty", ""), task("garbage", "NOT A NEC DECK"),
+                         task("nofeed", nofeed), task("loaded", loaded),
+                         json.dumps({**json.loads(task("prec", DIPOLE)),
+                                     "solver_config": {"basis": "hallen", "precision": "fp16"}})]) + "\n")
+    got = {r["task_id"]: r for r in out}
+    assert got["basis"]["error_code"] == "unsupported_config"
+    assert got["basis"]["error_message"] == "basis 'pulse' not supported in worker; only 'hallen' is implemented"
+    assert got["empty"]["error_code"] == "parse_error" and got["empty"]["error_message"].startswith("geometry error")
+    assert got["garbage"]["error_code"] == "parse_error"
+    assert got["nofeed"]["error_message"] == "no EX type-0 card found in deck"
+    assert got["loaded"]["error_code"] == "unsupported_config"
+    assert got["prec"]["error_code"] == "unsupported_config"
+
+
+def test_bad_rp_extension():
+    out = run(task("rp", DIPOLE, rp={"theta_deg": [0.0, 90.0], "phi_deg": [0.0]}) + "\n")
+    assert out[0]["task_id"] == "rp" and out[0]["error_code"] == "parse_error"
