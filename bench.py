@@ -76,6 +76,17 @@ def build_requests(stream):
     return items
 
 
+# host clock readings (ns) at the start and end of the closed-loop timed
+# region, so that a kernel trace of the same run can be cut to that region
+TIMED_REGION = {}
+
+
+def host_clocks():
+    return {"monotonic": time.clock_gettime_ns(time.CLOCK_MONOTONIC),
+            "boottime": time.clock_gettime_ns(time.CLOCK_BOOTTIME),
+            "monotonic_raw": time.clock_gettime_ns(time.CLOCK_MONOTONIC_RAW)}
+
+
 def serve(items, window, steps, warmup, prof_windows, precision, budget, max_batch, depth, timed_class=None):
     """Warmup windows, then ``prof_windows`` untimed windows with events on
     every kernel class (the per-class breakdown; it names the dominant class),
@@ -137,9 +148,11 @@ def serve(items, window, steps, warmup, prof_windows, precision, budget, max_bat
     if timed_class:
         dom = timed_class
     F.profile(enable=True, reset=True, only=dom)
+    clocks0 = host_clocks()
     t0 = time.perf_counter()
     lat, batches = run_phase(warmup + prof_windows, steps)
     elapsed = time.perf_counter() - t0
+    TIMED_REGION.update(start=clocks0, end=host_clocks())
     prof_timed = F.profile(enable=False, only="")
     L.fnec_hip_session_destroy(sess)
     return elapsed, np.array(lat), results, dom, prof_all, prof_timed, batches
@@ -333,7 +346,8 @@ def measured_traffic(kernel, precision):
             d = json.load(f)
     except (OSError, ValueError):
         return None
-    e = d.get("kernels", {}).get(kernel)
+    ring = kernel == "lu_update_k128" and F.get_tunable("update_ring") == 1
+    e = d.get("kernels", {}).get(kernel + ("_ring" if ring else ""))
     if not e or e.get("precision") != ("mixed" if precision == F.PREC_MIXED else "fp64"):
         return None
     tm = F.get_tunable("update_tile_m")
@@ -341,9 +355,8 @@ def measured_traffic(kernel, precision):
         return None  # measured with the other tile and product form
     if kernel == "lu_update" and tm == 64 and F.get_tunable("update_k128") == 1:
         return None  # measured with one K = 64 pass per step
-    if kernel == "lu_update_k128" and e.get("symbol") != (
-            "k_lu_update_f16x2s" if F.get_tunable("update_ring") == 1 else "k_lu_update_f16x2d"):
-        return None  # measured with the other form of the K = 128 pass
+    if kernel == "lu_update_k128" and e.get("symbol") != ("k_lu_update_f16x2s" if ring else "k_lu_update_f16x2d"):
+        return None  # measured with another form of the K = 128 pass
     return {"bytes_per_launch": e["hbm_bytes_per_launch"], "source": "profiles/serving_stream_traffic.json"}
 
 
@@ -530,6 +543,7 @@ def main():
         "algorithmic_bytes_per_launch": round(nbytes / launches) if nbytes else None,
         "kernels_source": (f"{args.profile_windows} untimed windows of {args.window} with events on every class"
                            if args.arrival == "windows" else f"timed region, events on {dom} only"),
+        "timed_region_clock_ns": TIMED_REGION or None,
         "kernels": {k: {"launches": int(v[0]), "total_ms": round(v[1] / 1e3, 3),
                         "tflops": round(v[2] / (v[1] * 1e-6) / 1e12, 3) if v[1] and v[2] else None,
                         "gbs": round(v[3] / (v[1] * 1e-6) / 1e9, 1) if v[1] and v[3] else None,
