@@ -170,6 +170,7 @@ def serve_poisson(items, times, warmup_items, precision, budget, max_batch, prof
         raise F.FnecError(F.INVALID_ARG, "session create failed")
     out = (F.Result * 4096)()
     results = {}
+    lat_by_tag = {}
 
     def drain(k_expect, t_arr, lat, bt, wait_us):
         k = L.fnec_hip_session_poll(sess, C.cast(out, C.c_void_p), 4096, wait_us)
@@ -183,6 +184,7 @@ def serve_poisson(items, times, warmup_items, precision, budget, max_batch, prof
             results[r.tag] = complex(r.z_in.re, r.z_in.im)
             if t_arr is not None:
                 lat.append(now - t_arr[r.tag])
+                lat_by_tag[r.tag] = now - t_arr[r.tag]
                 b = bt.setdefault(r.batch, [r.t_start_us, r.t_done_us, 0])
                 b[2] += 1
         return k
@@ -209,7 +211,13 @@ def serve_poisson(items, times, warmup_items, precision, budget, max_batch, prof
     elapsed = time.perf_counter() - t0
     prof = F.profile(enable=False, only="")
     L.fnec_hip_session_destroy(sess)
+    LAT_BY_TAG.clear()
+    LAT_BY_TAG.update(lat_by_tag)
     return elapsed, np.array(lat), results, prof, bt
+
+
+# latency (s) of each request of the last open-loop phase, by tag
+LAT_BY_TAG = {}
 
 
 def batch_overlap(batches):
@@ -492,6 +500,13 @@ def main():
                      "arrival": "the stream's Poisson arrival times rescaled to the load, open loop; "
                                 "latency from arrival to the poll that returns the result",
                      **(batch_overlap(bt_ol) or {})}
+        split = F.get_tunable("batch_split_n")
+        for name, sel in (("n_small", lambda n: n <= split), ("n_large", lambda n: n > split)):
+            ls = [LAT_BY_TAG[it["req"].tag] for it in lat_items if sel(it["meta"]["n"]) and it["req"].tag in LAT_BY_TAG]
+            if ls:
+                open_loop[name] = {"n_max" if name == "n_small" else "n_min": split if name == "n_small" else split + 1,
+                                   "requests": len(ls), "p50": round(float(np.percentile(ls, 50)) * 1e3, 3),
+                                   "p99": round(float(np.percentile(ls, 99)) * 1e3, 3)}
         print(f"[bench] open loop at {args.latency_rate:g}/s: p50 {open_loop['p50']:.1f} ms, "
               f"p99 {open_loop['p99']:.1f} ms", file=sys.stderr, flush=True)
 

@@ -67,13 +67,21 @@ class C64(C.Structure):
     _fields_ = [("re", C.c_double), ("im", C.c_double)]
 
 
+class Stamp(C.Structure):
+    _fields_ = [("row", C.c_size_t), ("col", C.c_size_t), ("delta", C64)]
+
+
+STAMP = np.dtype([("row", np.uint64), ("col", np.uint64), ("re", np.float64), ("im", np.float64)])
+
+
 class Request(C.Structure):
     _fields_ = [("segs", C.c_void_p), ("n", C.c_size_t), ("rhs", C.c_void_p), ("cos_vec", C.c_void_p),
                 ("wire_first", C.c_void_p), ("wire_last", C.c_void_p), ("n_wires", C.c_size_t),
                 ("j_a", C.c_void_p), ("j_b", C.c_void_p), ("j_sign", C.c_void_p), ("n_junc", C.c_size_t),
                 ("freq_hz", C.c_double), ("ground", Ground), ("feed_index", C.c_size_t),
                 ("v_feed", C64), ("rp_theta", C.c_void_p), ("rp_phi", C.c_void_p),
-                ("rp_npts", C.c_size_t), ("tag", C.c_uint64), ("x_out", C.c_void_p), ("rp_out", C.c_void_p)]
+                ("rp_npts", C.c_size_t), ("tag", C.c_uint64), ("x_out", C.c_void_p), ("rp_out", C.c_void_p),
+                ("stamps", C.c_void_p), ("n_stamps", C.c_size_t)]
 
 
 class Result(C.Structure):
@@ -349,6 +357,7 @@ def _host_sigs(L):
     L.fnec_deck_segment_count.restype = C.c_size_t
     L.fnec_deck_segment_count.argtypes = [C.c_void_p]
     L.fnec_deck_build.argtypes = [C.c_void_p, C.c_int, C.c_double, C.POINTER(DeckProblem)]
+    L.fnec_deck_solve.argtypes = [C.POINTER(DeckProblem), C.c_int, C.c_void_p, C.c_void_p]
     L.fnec_deck_report.argtypes = [C.c_void_p, C.c_size_t, C.c_int, C.c_char_p, C.c_size_t,
                                    C.POINTER(C.c_size_t)]
     L._host_ready = True
@@ -424,6 +433,7 @@ class Deck:
             "feed_seg": _arr(pr.feed_seg, pr.n_feeds, np.uint32),
             "seg_tag": _arr(pr.seg_tag, n, np.uint32),
             "seg_index": _arr(pr.seg_index, n, np.uint32),
+            "stamps": _arr(r.stamps, r.n_stamps, STAMP),
         }
         return out
 
@@ -437,12 +447,26 @@ class Deck:
 
 
 def solve_problem(p: dict, precision: int = PREC_FP64, with_stats: bool = False):
-    """Solve the dict returned by Deck.problem on the GPU (either route)."""
-    if p["route"] == 1:
-        return solve_hallen_paths(p["segs"], p["rhs"], p["cos"], p["path_of"], p["free_ends"],
-                                  p["freq_hz"], p["ground_struct"], precision, with_stats)
-    return solve_hallen(p["segs"], p["rhs"], p["cos"], p["wf"], p["wl"], p["ja"], p["jb"], p["js"],
-                        p["freq_hz"], p["ground_struct"], precision, with_stats)
+    """Solve the dict returned by Deck.problem on the GPU (either route, with
+    its LD/TL/NT stamps) through fnec_deck_solve."""
+    L = _host_sigs(lib())
+    q, keep = make_request(p)
+    pr = DeckProblem()
+    pr.route = int(p["route"])
+    pr.req = q
+    n = len(keep["segs"])
+    if pr.route == 1:
+        keep["po"] = _sz(p["path_of"])
+        keep["fe"] = _sz(p["free_ends"])
+        pr.path_of = _p(keep["po"])
+        pr.free_ends = _p(keep["fe"])
+        pr.n_free = len(keep["fe"])
+        x = np.zeros(n + (int(keep["po"].max()) + 1 if n else 0), np.complex128)
+    else:
+        x = np.zeros(n + max(len(keep["wf"]), 1), np.complex128)
+    st = SolveStats()
+    _check(L.fnec_deck_solve(C.byref(pr), int(precision), _p(x), C.byref(st)))
+    return (x, st.as_dict()) if with_stats else x
 
 
 PROFILE_CLASSES = ("fill", "gram", "lu_panel", "lu_trsm", "lu_update", "backsub", "rp", "zsolve", "matvec",
@@ -496,6 +520,10 @@ def make_request(p: dict, tag: int = 0, rp_theta=None, rp_phi=None, outputs: boo
     q.ground = p["ground_struct"]
     q.feed_index = int(p["feed_index"])
     q.v_feed = C64(p["v_feed"].real, p["v_feed"].imag)
+    if len(p.get("stamps", ())):
+        keep["stamps"] = np.ascontiguousarray(p["stamps"], STAMP)
+        q.stamps = _p(keep["stamps"])
+        q.n_stamps = len(keep["stamps"])
     if rp_theta is not None:
         keep["th"] = np.ascontiguousarray(rp_theta, np.float64)
         keep["ph"] = np.ascontiguousarray(rp_phi, np.float64)

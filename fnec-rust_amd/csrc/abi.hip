@@ -8,6 +8,7 @@
 #include <vector>
 
 #include "../../include/fnec_hip.h"
+#include "../../include/fnec_host.h"
 #include "engine.h"
 
 using namespace fnec;
@@ -127,6 +128,34 @@ int fnec_hip_solve_hallen_paths(const fnec_seg *segs, size_t n, const fnec_c64 *
     }
     HostProblem hp;
     st = problem_from_paths(hp, segs, n, rhs, cos_vec, path_of, free_ends, n_free, freq_hz, ground);
+    if (st)
+        return st;
+    return solve_one(hp, precision, x_out, stats);
+}
+
+int fnec_deck_solve(const fnec_deck_problem *pr, int precision, fnec_c64 *x_out,
+                    fnec_solve_stats *stats)
+{
+    std::lock_guard<std::mutex> lk(global_mutex());
+    int st = ensure_ready();
+    if (st)
+        return st;
+    if ((st = check_precision(precision)))
+        return st;
+    if (!pr || !x_out) {
+        set_error("null argument");
+        return FNEC_INVALID_ARG;
+    }
+    const fnec_request &r = pr->req;
+    HostProblem hp;
+    if (pr->route == 1)
+        st = problem_from_paths(hp, r.segs, r.n, r.rhs, r.cos_vec, pr->path_of, pr->free_ends,
+                                pr->n_free, r.freq_hz, &r.ground);
+    else
+        st = problem_from_hallen(hp, r.segs, r.n, r.rhs, r.cos_vec, r.wire_first, r.wire_last,
+                                 r.n_wires, r.j_a, r.j_b, r.j_sign, r.n_junc, r.freq_hz, &r.ground);
+    if (!st)
+        st = problem_add_stamps(hp, r.stamps, r.n_stamps, r.n);
     if (st)
         return st;
     return solve_one(hp, precision, x_out, stats);

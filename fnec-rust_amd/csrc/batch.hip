@@ -38,6 +38,7 @@ struct OwnedRequest {
     std::vector<double> cosv;
     std::vector<size_t> wf, wl, ja, jb;
     std::vector<double> js, theta, phi;
+    std::vector<fnec_stamp> stamps;
     fnec_request req;
     double t_submit = 0.0;
     size_t footprint = 0;
@@ -69,6 +70,9 @@ struct OwnedRequest {
             theta.assign(r.rp_theta, r.rp_theta + r.rp_npts);
             phi.assign(r.rp_phi, r.rp_phi + r.rp_npts);
         }
+        stamps.clear();
+        if (r.n_stamps && r.stamps)
+            stamps.assign(r.stamps, r.stamps + r.n_stamps);
         req = r;
         req.segs = segs.data();
         req.rhs = rhs.data();
@@ -81,6 +85,8 @@ struct OwnedRequest {
         req.rp_theta = theta.empty() ? nullptr : theta.data();
         req.rp_phi = phi.empty() ? nullptr : phi.data();
         req.rp_npts = theta.size();
+        req.stamps = stamps.empty() ? nullptr : stamps.data();
+        req.n_stamps = stamps.size();
     }
 };
 
@@ -115,6 +121,8 @@ int run_batch(const fnec_request *const *reqs, size_t count, int precision, fnec
                 set_error("feed index outside the segment list");
                 pre[i] = FNEC_INVALID_ARG;
             }
+            if (!pre[i])
+                pre[i] = problem_add_stamps(hps[i], r.stamps, r.n_stamps, r.n);
         }
     }
     std::vector<const HostProblem *> ok;
@@ -436,7 +444,7 @@ fnec_session *fnec_hip_session_create(size_t budget_bytes, int precision, int ma
 
 int fnec_hip_session_submit(fnec_session *s, const fnec_request *req)
 {
-    if (!s || !req || !req->segs || !req->rhs || !req->cos_vec || req->n == 0) {
+    if (!s || !req || !req->segs || !req->rhs || !req->cos_vec || req->n == 0 || (req->n_stamps && !req->stamps)) {
         set_error("invalid request");
         return FNEC_INVALID_ARG;
     }

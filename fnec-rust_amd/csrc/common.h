@@ -39,6 +39,9 @@ struct Prob {
     double gam_im;
     const double *seg;     // n * kSegStride
     const double *selfln;  // n: 2 ln((h + R_end) / a) of each segment's self term (host libm)
+    const double *stamps;  // nstamps * 4: row, col, re, im of each Z stamp, in order
+    int nstamps;
+    int pad2;
     double *zr, *zi;       // Z, ldz x n
     double *ar, *ai;       // [G | M^H y], lda x (s + 1)
     const double *cosv;    // n

@@ -23,6 +23,7 @@ struct HostProblem {
     std::vector<int> rowcol;   // n: homogeneous column of each row
     std::vector<int> cons;     // 2 per constraint row
     std::vector<double> cons_sign;
+    std::vector<double> stamps; // 4 per Z stamp, applied in order: row, col, re, im
 };
 
 // Host-side description of one far-field job. The grid is borrowed from
@@ -105,6 +106,9 @@ int problem_from_hallen(HostProblem &hp, const fnec_seg *segs, size_t n, const f
 int problem_from_paths(HostProblem &hp, const fnec_seg *segs, size_t n, const fnec_c64 *rhs,
                        const double *cos_vec, const size_t *path_of, const size_t *free_ends,
                        size_t nfree, double freq_hz, const fnec_ground *ground);
+// Impedance stamps (row, column, delta) added to Z after the fill, before the
+// solve; rows and columns are checked against the n segments.
+int problem_add_stamps(HostProblem &hp, const fnec_stamp *stamps, size_t n_stamps, size_t n);
 
 // overlap (optional) is called once, before any of the solve's kernels is
 // queued, with the device addresses of the solution vectors; it may start host

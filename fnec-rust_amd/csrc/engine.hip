@@ -24,6 +24,7 @@ namespace fnec {
 
 // kernels.hip
 __global__ void k_fill_z(const Prob *, const int4 *);
+__global__ void k_stamp_z(const Prob *, const int *);
 __global__ void k_gram(const Prob *, const int4 *);
 __global__ void k_gram_cols(const Prob *, const int2 *);
 __global__ void k_gram_fix(const Prob *, const int *);
@@ -569,6 +570,7 @@ int build_problem(HostProblem &hp, const fnec_seg *segs, size_t n, const fnec_c6
         set_error("frequency must be positive");
         return FNEC_INVALID_ARG;
     }
+    hp.stamps.clear();
     hp.n = (int)n;
     hp.w = ncols;
     hp.s = (int)n + ncols;
@@ -717,6 +719,39 @@ int problem_from_paths(HostProblem &hp, const fnec_seg *segs, size_t n, const fn
     return build_problem(hp, segs, n, rhs, cos_vec, row_col, (int)np, cons, sign, freq_hz, ground);
 }
 
+int problem_add_stamps(HostProblem &hp, const fnec_stamp *stamps, size_t n_stamps, size_t n)
+{
+    hp.stamps.clear();
+    if (!n_stamps)
+        return FNEC_OK;
+    if (!stamps) {
+        set_error("null stamp list");
+        return FNEC_INVALID_ARG;
+    }
+    // grouped by entry, in call order within an entry (k_stamp_z adds each
+    // group in order; distinct entries are independent)
+    std::vector<size_t> order(n_stamps);
+    for (size_t q = 0; q < n_stamps; ++q) {
+        if (stamps[q].row >= n || stamps[q].col >= n) {
+            set_error("stamp outside the segment list");
+            return FNEC_INVALID_ARG;
+        }
+        order[q] = q;
+    }
+    std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) {
+        return stamps[a].col != stamps[b].col ? stamps[a].col < stamps[b].col : stamps[a].row < stamps[b].row;
+    });
+    hp.stamps.reserve(4 * n_stamps);
+    for (size_t q : order) {
+        const fnec_stamp &t = stamps[q];
+        hp.stamps.push_back((double)t.row);
+        hp.stamps.push_back((double)t.col);
+        hp.stamps.push_back(t.delta.re);
+        hp.stamps.push_back(t.delta.im);
+    }
+    return FNEC_OK;
+}
+
 // ---------------------------------------------------------------------------
 // batch planning
 // ---------------------------------------------------------------------------
@@ -727,6 +762,7 @@ struct Plan {
     // work lists, concatenated into one int buffer
     std::vector<int> ints;
     size_t fill_off = 0, fill_cnt = 0; // fill items in the generated region (item units)
+    size_t stamp_off = 0, stamp_cnt = 0; // problems with Z stamps (k_stamp_z)
     // Regular work lists (problem, y, z) generated on the device by
     // k_gen_items from one segment (first item, problem, ny, nz) per problem
     // and launch: the host builds O(problems) records instead of O(tiles) items.
@@ -750,7 +786,7 @@ struct Plan {
     size_t bytes = 0;
     size_t a_begin = 0, a_end = 0; // byte range of the A planes (zeroed per run)
     std::vector<size_t> off_seg, off_z, off_a, off_cos, off_rhs, off_runs, off_cons, off_sign,
-        off_ipiv, off_info, off_x, off_stats, off_scr, off_panel, off_pmeta, off_selfln;
+        off_ipiv, off_info, off_x, off_stats, off_scr, off_panel, off_pmeta, off_selfln, off_stamps;
     size_t off_probs = 0, off_ints = 0, off_lud = 0;
     std::vector<LuDesc<double>> lud;
     // mixed-precision path
@@ -1203,6 +1239,7 @@ static void plan_build(Plan &pl, const std::vector<const HostProblem *> &hps, bo
     pl.off_panel.resize(np);
     pl.off_pmeta.resize(np);
     pl.off_selfln.resize(np);
+    pl.off_stamps.resize(np);
     pl.mix.clear();
     // A planes first, contiguous, so one zero-fill covers them
     pl.a_begin = 0;
@@ -1268,6 +1305,7 @@ static void plan_build(Plan &pl, const std::vector<const HostProblem *> &hps, bo
         const HostProblem &h = *hps[b];
         pl.off_seg[b] = take(cur, (size_t)h.n * 8 * kSegStride);
         pl.off_selfln[b] = take(cur, 8 * (size_t)h.n);
+        pl.off_stamps[b] = take(cur, 8 * h.stamps.size() + 8);
         pl.off_cos[b] = take(cur, 8 * (size_t)h.n);
         pl.off_rhs[b] = take(cur, 16 * (size_t)h.n);
         pl.off_runs[b] = take(cur, 4 * h.runs.size() + 4);
@@ -1303,6 +1341,12 @@ static void plan_build(Plan &pl, const std::vector<const HostProblem *> &hps, bo
         pl.gen_items += (size_t)rt * ct;
     }
     pl.fill_cnt = pl.gen_items - pl.fill_off;
+    align4();
+    pl.stamp_off = pl.ints.size();
+    for (size_t b = 0; b < np; ++b)
+        if (!hps[b]->stamps.empty())
+            pl.ints.push_back((int)b);
+    pl.stamp_cnt = pl.ints.size() - pl.stamp_off;
     pl.fill_flops = pl.gram_flops = pl.backsub_flops = 0;
     for (size_t b = 0; b < np; ++b) {
         const double n = hps[b]->n, s = hps[b]->s;
@@ -1548,6 +1592,8 @@ static int plan_upload(Plan &pl, const std::vector<const HostProblem *> &hps, ch
         P.gam_im = h.gam_im;
         P.seg = (const double *)(base + pl.off_seg[b]);
         P.selfln = (const double *)(base + pl.off_selfln[b]);
+        P.stamps = (const double *)(base + pl.off_stamps[b]);
+        P.nstamps = (int)(h.stamps.size() / 4);
         P.zr = (double *)(base + pl.off_z[b]);
         P.zi = P.zr + (size_t)P.ldz * h.n;
         if (with_solve && !pl.mixed) {
@@ -1652,6 +1698,7 @@ static int plan_upload(Plan &pl, const std::vector<const HostProblem *> &hps, ch
         const HostProblem &h = *hps[b];
         put(pl.off_seg[b], h.seg.data(), 8 * h.seg.size());
         put(pl.off_selfln[b], h.selfln.data(), 8 * h.selfln.size());
+        put(pl.off_stamps[b], h.stamps.data(), 8 * h.stamps.size());
         put(pl.off_cos[b], h.cosv.data(), 8 * h.cosv.size());
         put(pl.off_rhs[b], h.rhs.data(), 8 * h.rhs.size());
         put(pl.off_runs[b], h.runs.data(), 4 * h.runs.size());
@@ -2005,6 +2052,9 @@ static int plan_launch(Plan &pl, char *base, bool with_solve)
         hipLaunchKernelGGL(pl.exact ? k_fill_exact : k_fill_z, dim3((unsigned)pl.fill_cnt), dim3(256), 0,
                            st, dp, (const int4 *)gen_ptr(base, pl) + pl.fill_off);
         prof_end(PC_FILL, e, pl.fill_flops, 0);
+        if (pl.stamp_cnt)
+            hipLaunchKernelGGL(k_stamp_z, dim3((unsigned)pl.stamp_cnt), dim3(256), 0, st, dp,
+                               wl_ptr(base, pl, pl.stamp_off));
     }
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(LN().ev[1], st));

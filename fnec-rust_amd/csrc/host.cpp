@@ -31,6 +31,7 @@ struct Card {
     std::string mn;
     int nf = 0;
     double f[16] = {0};
+    std::vector<std::string> raw; // NT: the fields as written (card.rs NtCard::raw_fields)
 };
 
 uint32_t as_u32(double v)
@@ -122,6 +123,7 @@ struct fnec_deck {
     std::vector<double> cosv, js, theta, phi;
     std::vector<size_t> wf, wl, ja, jb, path_of, free_ends, feed_idx;
     std::vector<uint32_t> feed_tag, feed_seg, seg_tag, seg_index;
+    std::vector<fnec_stamp> stamps;
 };
 
 namespace {
@@ -661,12 +663,233 @@ std::string fmt_exp(double v, int prec)
 bool deck_unsupported(const std::vector<Card> &cards)
 {
     for (const Card &c : cards) {
-        if (card_is(c, "LD") || card_is(c, "TL") || card_is(c, "NT"))
-            return true;
         if (card_is(c, "EX") && (ex_plane(c) || ex_current(c)))
             return true;
     }
     return false;
+}
+
+// ---- LD / TL / NT stamps on Z (loads.rs:34-223, tl.rs:47-244,
+// network.rs:55-140): complex arithmetic in num-complex 0.4's definitions
+// (naive division, norm = hypot), as the reference computes them.
+struct Cz {
+    double re, im;
+};
+Cz cz(double r, double i) { return Cz{r, i}; }
+Cz operator+(Cz a, Cz b) { return Cz{a.re + b.re, a.im + b.im}; }
+Cz operator-(Cz a, Cz b) { return Cz{a.re - b.re, a.im - b.im}; }
+Cz operator*(Cz a, Cz b) { return Cz{a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re}; }
+Cz operator*(Cz a, double s) { return Cz{a.re * s, a.im * s}; }
+Cz operator/(Cz a, double s) { return Cz{a.re / s, a.im / s}; }
+Cz operator/(Cz a, Cz b)
+{
+    const double ns = b.re * b.re + b.im * b.im;
+    return Cz{(a.re * b.re + a.im * b.im) / ns, (a.im * b.re - a.re * b.im) / ns};
+}
+double cnorm(Cz a) { return std::hypot(a.re, a.im); }
+
+constexpr double kTwoPiH = 2.0 * 3.14159265358979323846;
+constexpr double kMu0H = 4.0 * 3.14159265358979323846 * 1e-7;
+constexpr double kC0H = 299792458.0;
+
+// I0(z) / I1(z) by the power series (|z| <= 15) or the asymptotic expansion
+// (loads.rs:34-59)
+Cz bessel_i0_over_i1(Cz z)
+{
+    if (cnorm(z) > 15.0) {
+        const Cz inv = cz(1.0, 0.0) / z;
+        return cz(1.0, 0.0) + inv * 0.5 + inv * inv * 0.375;
+    }
+    const Cz half = z * 0.5, half_sq = half * half;
+    Cz i0 = cz(1.0, 0.0), t0 = cz(1.0, 0.0), i1 = half, t1 = half;
+    for (int k = 1; k < 300; ++k) {
+        const double kf = (double)k;
+        t0 = t0 * (half_sq / (kf * kf));
+        i0 = i0 + t0;
+        t1 = t1 * (half_sq / (kf * (kf + 1.0)));
+        i1 = i1 + t1;
+        if (cnorm(t0) <= 1e-17 * cnorm(i0) && cnorm(t1) <= 1e-17 * cnorm(i1))
+            break;
+    }
+    return i0 / i1;
+}
+
+// the round wire's internal (skin-effect) impedance (loads.rs:73-80)
+Cz wire_skin_impedance(double sigma, double a, double dl, double omega)
+{
+    if (omega <= 0.0)
+        return cz(dl / (sigma * 3.14159265358979323846 * a * a), 0.0);
+    const double delta = std::sqrt(2.0 / (omega * kMu0H * sigma));
+    const Cz gamma = cz(1.0 / delta, 1.0 / delta);
+    return gamma / (kTwoPiH * a * sigma) * bessel_i0_over_i1(gamma * a) * dl;
+}
+
+// (tag, segment) -> segment index; segment 0 is the tag's centre, the lower
+// one of an even count (tl.rs:186-244)
+int stamp_seg(const std::vector<Seg> &s, uint32_t tag, uint32_t seg)
+{
+    if (seg != 0) {
+        for (size_t i = 0; i < s.size(); ++i)
+            if (s[i].tag == tag && s[i].idx == seg)
+                return (int)i;
+        return -1;
+    }
+    std::vector<size_t> of_tag;
+    for (size_t i = 0; i < s.size(); ++i)
+        if (s[i].tag == tag)
+            of_tag.push_back(i);
+    if (of_tag.empty())
+        return -1;
+    size_t pick;
+    if (of_tag.size() % 2 == 1) {
+        pick = of_tag.size() / 2;
+    } else {
+        const uint32_t lower = (uint32_t)(of_tag.size() / 2);
+        pick = of_tag.size() / 2 >= 1 ? of_tag.size() / 2 - 1 : 0;
+        for (size_t k = 0; k < of_tag.size(); ++k)
+            if (s[of_tag[k]].idx == lower) {
+                pick = k;
+                break;
+            }
+    }
+    return (int)of_tag[pick];
+}
+
+// str::parse::<u32>: optional '+', decimal digits, no overflow
+bool parse_u32_strict(const std::string &t, uint32_t &out)
+{
+    size_t p = t.size() && t[0] == '+' ? 1 : 0;
+    if (p >= t.size())
+        return false;
+    unsigned long long v = 0;
+    for (; p < t.size(); ++p) {
+        if (t[p] < '0' || t[p] > '9')
+            return false;
+        v = v * 10 + (unsigned)(t[p] - '0');
+        if (v > 4294967295ull)
+            return false;
+    }
+    out = (uint32_t)v;
+    return true;
+}
+
+// The stamps in the order the reference adds them to Z: the LD loads (one
+// diagonal stamp per loaded segment, its loads summed in card order), then
+// the TL stamps, then (with_nt: the CLI route) the NT stamps. Cards the
+// reference warns about and skips are skipped.
+void build_stamps(const std::vector<Card> &cards, const std::vector<Seg> &s, double freq, bool with_nt,
+                  std::vector<fnec_stamp> &out)
+{
+    out.clear();
+    const size_t n = s.size();
+    const double omega = kTwoPiH * freq;
+    std::vector<Cz> loads(n, cz(0.0, 0.0));
+    for (const Card &c : cards) {
+        if (!card_is(c, "LD"))
+            continue;
+        const int32_t type = as_i32(c.f[0]);
+        const uint32_t tag = as_u32(c.f[1]), first = as_u32(c.f[2]);
+        uint32_t last = as_u32(c.f[3]);
+        const double f1 = c.nf > 4 ? c.f[4] : 0.0, f2 = c.nf > 5 ? c.f[5] : 0.0, f3 = c.nf > 6 ? c.f[6] : 0.0;
+        if (last == 0)
+            last = first;
+        for (size_t i = 0; i < n; ++i) {
+            if (tag != 0 && s[i].tag != tag)
+                continue;
+            if (first != 0 && (s[i].idx < first || s[i].idx > last))
+                continue;
+            Cz z;
+            if (type == 0) {
+                z = cz(f1, omega * f2 - (f3 > 0.0 ? 1.0 / (omega * f3) : 0.0));
+            } else if (type == 1) {
+                Cz y = cz(0.0, 0.0);
+                if (f1 > 0.0)
+                    y = y + cz(1.0 / f1, 0.0);
+                if (f2 > 0.0)
+                    y = y + cz(0.0, -1.0 / (omega * f2));
+                if (f3 > 0.0)
+                    y = y + cz(0.0, omega * f3);
+                z = cnorm(y) < 1e-30 ? cz(0.0, 0.0) : cz(1.0, 0.0) / y;
+            } else if (type == 2) {
+                z = cz(f1, omega * f2);
+            } else if (type == 3) {
+                z = cz(f1, -(f3 > 0.0 ? 1.0 / (omega * f3) : 0.0));
+            } else if (type == 4) {
+                z = cz(f1, f2);
+            } else if (type == 5 && f1 > 0.0) {
+                z = wire_skin_impedance(f1, s[i].radius, s[i].length, omega);
+            } else {
+                continue; // sigma <= 0 or an unsupported type: ignored with a warning
+            }
+            loads[i] = loads[i] + z;
+        }
+    }
+    for (size_t i = 0; i < n; ++i)
+        if (loads[i].re != 0.0 || loads[i].im != 0.0)
+            out.push_back(fnec_stamp{i, i, fnec_c64{loads[i].re, loads[i].im}});
+    auto two_port = [&](int a, int b, Cz zd, Cz zo) {
+        out.push_back(fnec_stamp{(size_t)a, (size_t)a, fnec_c64{zd.re, zd.im}});
+        out.push_back(fnec_stamp{(size_t)b, (size_t)b, fnec_c64{zd.re, zd.im}});
+        out.push_back(fnec_stamp{(size_t)a, (size_t)b, fnec_c64{zo.re, zo.im}});
+        out.push_back(fnec_stamp{(size_t)b, (size_t)a, fnec_c64{zo.re, zo.im}});
+    };
+    const double k = kTwoPiH * freq / kC0H;
+    for (const Card &c : cards) {
+        if (!card_is(c, "TL"))
+            continue;
+        const int a = stamp_seg(s, as_u32(c.f[0]), as_u32(c.f[1])), b = stamp_seg(s, as_u32(c.f[2]), as_u32(c.f[3]));
+        const uint32_t type = as_u32(c.f[5]);
+        const double z0 = c.f[6], len = c.f[7], f3 = c.nf > 8 ? c.f[8] : 1.0;
+        if (a < 0 || b < 0 || a == b || z0 <= 0.0 || len <= 0.0)
+            continue;
+        if (type != 0) {
+            // lossy: gamma l = F3 ln10 / 20 + j k len; Z0 coth, Z0 csch with
+            // num-complex sinh = (sinh a cos b, cosh a sin b), cosh = (cosh a cos b, sinh a sin b)
+            const double bl = k * len, al = f3 * 2.302585092994046 / 20.0;
+            const Cz sh = cz(std::sinh(al) * std::cos(bl), std::cosh(al) * std::sin(bl));
+            const Cz ch = cz(std::cosh(al) * std::cos(bl), std::sinh(al) * std::sin(bl));
+            if (cnorm(sh) < 1e-9)
+                continue;
+            const Cz z0c = cz(z0, 0.0);
+            two_port(a, b, z0c * ch / sh, z0c / sh);
+            continue;
+        }
+        const double vf = (f3 >= 0.0 && f3 <= 1.0 && f3 > 0.0) ? f3 : 1.0;
+        const double theta = k * len / vf, st = std::sin(theta);
+        if (std::fabs(st) < 1e-9)
+            continue;
+        const double cot = std::cos(theta) / st, csc = 1.0 / st;
+        two_port(a, b, cz(0.0, -z0 * cot), cz(0.0, -z0 * csc));
+    }
+    if (!with_nt)
+        return;
+    for (const Card &c : cards) {
+        if (!card_is(c, "NT") || c.raw.size() < 10)
+            continue;
+        uint32_t t1, s1, t2, s2;
+        if (!parse_u32_strict(c.raw[0], t1) || !parse_u32_strict(c.raw[1], s1) || !parse_u32_strict(c.raw[2], t2) ||
+            !parse_u32_strict(c.raw[3], s2))
+            continue;
+        double y[6];
+        bool ok = true;
+        for (int q = 0; q < 6; ++q)
+            ok = parse_f64(c.raw[4 + q], y[q]) && ok;
+        if (!ok)
+            continue;
+        const int a = stamp_seg(s, t1, s1), b = stamp_seg(s, t2, s2);
+        if (a < 0 || b < 0 || a == b)
+            continue;
+        const Cz y11 = cz(y[0], y[1]), y12 = cz(y[2], y[3]), y21 = y12, y22 = cz(y[4], y[5]);
+        const Cz det = y11 * y22 - y12 * y21;
+        if (cnorm(det) < 1e-30)
+            continue;
+        out.push_back(fnec_stamp{(size_t)a, (size_t)a, [&] { Cz z = y22 / det; return fnec_c64{z.re, z.im}; }()});
+        out.push_back(fnec_stamp{(size_t)b, (size_t)b, [&] { Cz z = y11 / det; return fnec_c64{z.re, z.im}; }()});
+        out.push_back(fnec_stamp{(size_t)a, (size_t)b,
+                                 [&] { Cz z = cz(-y12.re, -y12.im) / det; return fnec_c64{z.re, z.im}; }()});
+        out.push_back(fnec_stamp{(size_t)b, (size_t)a,
+                                 [&] { Cz z = cz(-y21.re, -y21.im) / det; return fnec_c64{z.re, z.im}; }()});
+    }
 }
 
 } // namespace
@@ -739,6 +962,8 @@ fnec_deck *fnec_deck_parse(const char *text, char *err, size_t errlen)
         Card c;
         c.mn = mn;
         c.nf = nf;
+        if (mn == "NT")
+            c.raw.assign(toks.begin() + 1, toks.end());
         for (int q = 0; q < nparse && q < 16; ++q)
             if (!parse_f64(toks[1 + q], c.f[q])) {
                 if (err && errlen)
@@ -824,7 +1049,7 @@ int fnec_deck_build(fnec_deck *d, int semantics, double freq_hz, fnec_deck_probl
     if (fnec_deck_segment_count(d) == 0)
         return FNEC_INVALID_ARG;
     if (deck_unsupported(d->cards)) {
-        fnec::set_error("LD/TL/NT, plane-wave and current-source decks stay on the CPU path");
+        fnec::set_error("plane-wave and current-source excitations stay on the CPU path");
         return FNEC_UNAVAILABLE;
     }
     const std::vector<Seg> &s = d->segs;
@@ -947,6 +1172,11 @@ int fnec_deck_build(fnec_deck *d, int semantics, double freq_hz, fnec_deck_probl
     r.rp_theta = d->theta.empty() ? nullptr : d->theta.data();
     r.rp_phi = d->phi.empty() ? nullptr : d->phi.data();
     r.rp_npts = d->theta.size();
+    // LD and TL stamps for every caller, NT for the CLI (solve_session.rs:1285-1309;
+    // nec_worker solve.rs:179-185; fnec_py lib.rs:58-63)
+    build_stamps(d->cards, s, freq_hz, semantics == FNEC_SEM_CLI, d->stamps);
+    r.stamps = d->stamps.empty() ? nullptr : d->stamps.data();
+    r.n_stamps = d->stamps.size();
     out->route = route;
     out->path_of = d->path_of.empty() ? nullptr : d->path_of.data();
     out->free_ends = d->free_ends.empty() ? nullptr : d->free_ends.data();
@@ -977,15 +1207,16 @@ int fnec_deck_report(fnec_deck *d, size_t freq_index, int precision, char *outbu
         return st;
     const fnec_request &r = pr.req;
     const size_t n = r.n;
-    std::vector<fnec_c64> x(n + std::max<size_t>(pr.route ? (pr.n_free / 2) : r.n_wires, 1));
+    // route 1 writes one constant per conductor path (max path_of + 1)
+    size_t ncons = r.n_wires;
+    if (pr.route == 1) {
+        ncons = 0;
+        for (size_t i = 0; i < n; ++i)
+            ncons = std::max(ncons, pr.path_of[i] + 1);
+    }
+    std::vector<fnec_c64> x(n + std::max<size_t>(ncons, 1));
     fnec_solve_stats stats;
-    if (pr.route == 1)
-        st = fnec_hip_solve_hallen_paths(r.segs, n, r.rhs, r.cos_vec, pr.path_of, pr.free_ends,
-                                         pr.n_free, f, &r.ground, precision, x.data(), &stats);
-    else
-        st = fnec_hip_solve_hallen(r.segs, n, r.rhs, r.cos_vec, r.wire_first, r.wire_last, r.n_wires,
-                                   r.j_a, r.j_b, r.j_sign, r.n_junc, f, &r.ground, precision,
-                                   x.data(), &stats);
+    st = fnec_deck_solve(&pr, precision, x.data(), &stats);
     if (st)
         return st;
     std::string o;
@@ -1017,6 +1248,20 @@ int fnec_deck_report(fnec_deck *d, size_t freq_index, int precision, char *outbu
                 o += std::to_string(as_u32(c.f[0])) + " " + std::to_string(ex_tag(c)) + " " +
                      std::to_string(ex_seg(c)) + " " + std::to_string(as_u32(c.f[3])) + " " +
                      fmt_fixed(ex_vr(c), 6) + " " + fmt_fixed(ex_vi(c), 6) + "\n";
+    }
+    // LD cards as parsed (solve_session.rs:966-982, nec_report lib.rs:335-344)
+    int nld = 0;
+    for (const Card &c : d->cards)
+        if (card_is(c, "LD"))
+            ++nld;
+    if (nld) {
+        o += "\nLOADS\nN_LOADS " + std::to_string(nld) + "\nTYPE TAG SEG_FIRST SEG_LAST F1 F2 F3\n";
+        for (const Card &c : d->cards)
+            if (card_is(c, "LD"))
+                o += std::to_string(as_i32(c.f[0])) + " " + std::to_string(as_u32(c.f[1])) + " " +
+                     std::to_string(as_u32(c.f[2])) + " " + std::to_string(as_u32(c.f[3])) + " " +
+                     fmt_fixed(c.nf > 4 ? c.f[4] : 0.0, 6) + " " + fmt_fixed(c.nf > 5 ? c.f[5] : 0.0, 6) +
+                     " " + fmt_fixed(c.nf > 6 ? c.f[6] : 0.0, 6) + "\n";
     }
     o += "\nCURRENTS\nTAG SEG I_RE I_IM I_MAG I_PHASE\n";
     for (size_t i = 0; i < n; ++i) {

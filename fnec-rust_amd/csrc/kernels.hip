@@ -207,6 +207,40 @@ __global__ __launch_bounds__(256) void k_fill_z(const Prob *__restrict__ probs,
     }
 }
 
+// The LD / TL / NT stamps of one problem, Z[row][col] += delta in the order
+// given (ZMatrix::add_to_diagonal / add_to_entry, matrix.rs:131-147; applied
+// after assembly, solve_session.rs:1285-1309): one thread walks the list, so
+// stamps on the same entry add in order. The mixed path's FP32 copy of an
+// entry is rounded again from the stamped FP64 value, as the fill rounds it.
+__global__ __launch_bounds__(256) void k_stamp_z(const Prob *__restrict__ probs,
+                                                 const int *__restrict__ plist)
+{
+    // The stamps arrive grouped by matrix entry (their order kept within an
+    // entry): the thread at the head of a group adds that group's deltas in
+    // order, so each entry sees the sequence of additions of the CPU path.
+    const Prob &P = probs[plist[blockIdx.x]];
+    for (int q = threadIdx.x; q < P.nstamps; q += blockDim.x) {
+        const double *st = P.stamps + 4 * (size_t)q;
+        if (q > 0 && st[-4] == st[0] && st[-3] == st[1])
+            continue;
+        const size_t o = (size_t)st[1] * P.ldz + (size_t)st[0]; // column-major: col * ldz + row
+        double re = P.zr[o], im = P.zi[o];
+        for (int k = q; k < P.nstamps; ++k) {
+            const double *sk = P.stamps + 4 * (size_t)k;
+            if (sk[0] != st[0] || sk[1] != st[1])
+                break;
+            re += sk[2];
+            im += sk[3];
+        }
+        P.zr[o] = re;
+        P.zi[o] = im;
+        if (P.fr) {
+            P.fr[o] = (float)re;
+            P.fi[o] = (float)im;
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Gram G = Z^H Z (complex HERK on FP64 MFMA)
 // ---------------------------------------------------------------------------

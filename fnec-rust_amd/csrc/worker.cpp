@@ -30,9 +30,10 @@
 //   solver_config.precision  "fp64" (the reference's algorithm and rounding;
 //                               the default when exec is "cpu"), "mixed" (the
 //                               default when exec is "gpu") or "fp64_mfma".
-// exec_used is always "gpu": the worker has no CPU solver. Decks the device
-// path does not cover (LD / TL / NT cards, plane-wave or current-source EX)
-// are answered with error_code "unsupported_config".
+// exec_used is always "gpu": the worker has no CPU solver. LD and TL cards
+// are stamped into Z on the device (NT cards are ignored, as the reference's
+// worker ignores them); decks the device path does not cover (plane-wave or
+// current-source EX) are answered with error_code "unsupported_config".
 #include <algorithm>
 #include <atomic>
 #include <charconv>

@@ -175,6 +175,16 @@ int fnec_hip_rp(const fnec_seg *segs, const fnec_c64 *currents, size_t n, double
 int fnec_hip_rp_noalloc(const fnec_seg *segs, const fnec_c64 *currents, size_t n, double freq_hz,
                         const fnec_ground *ground, const double *theta_deg, const double *phi_deg,
                         size_t npts, fnec_rp_row *out, double *total_out);
+/* One Z-matrix stamp, Z[row][col] += delta (ZMatrix::add_to_entry /
+ * add_to_diagonal, matrix.rs:131-147; row = observation segment, col =
+ * source segment): the LD loads (loads.rs:103-206, one diagonal stamp per
+ * loaded segment holding the sum of its loads), the TL two-port stamps
+ * (tl.rs:47-184) and the NT network stamps (network.rs:55-140). */
+typedef struct {
+    size_t row, col;
+    fnec_c64 delta;
+} fnec_stamp;
+
 /* ---- batched solves (sweeps and many geometries in one submission) ----
  * One request = one Hallén solve (+ optional RP grid). Arrays are borrowed for
  * the duration of the call. */
@@ -204,6 +214,13 @@ typedef struct {
      * then; a batch call uses them where the result's x / rp are NULL. */
     fnec_c64 *x_out;
     fnec_rp_row *rp_out;
+    /* Optional Z stamps (NULL / 0: none), added in this order to the filled Z
+     * before anything reads it, as the reference adds the LD, TL and NT
+     * stamps after assembling Z (solve_session.rs:1285-1309, nec_worker
+     * solve.rs:179-185): the solve, the residual and the diagonal spread all
+     * see the stamped matrix. row and col must be below n. */
+    const fnec_stamp *stamps;
+    size_t n_stamps;
 } fnec_request;
 
 typedef struct {

@@ -13,9 +13,11 @@
  *   solve_frequency_point (Hallén route)    apps/nec-cli/src/solve_session.rs:1186-1775
  *   worker solve                            crates/nec_worker/src/solve.rs:150-289
  *   render_text_report                      crates/nec_report/src/lib.rs:307-482
- * Unsupported cards for this path (LD, TL, NT, plane-wave and current-source
- * EX) make fnec_deck_build return FNEC_UNAVAILABLE, the signal on which the
- * reference's callers keep the CPU path.
+ *   LD / TL / NT impedance stamps           crates/nec_solver/src/loads.rs,
+ *                                           tl.rs, network.rs
+ * Plane-wave and current-source EX make fnec_deck_build return
+ * FNEC_UNAVAILABLE, the signal on which the reference's callers keep the CPU
+ * path.
  */
 #ifndef FNEC_HOST_H
 #define FNEC_HOST_H
@@ -64,6 +66,14 @@ typedef struct {
 } fnec_deck_problem;
 
 int fnec_deck_build(fnec_deck *d, int semantics, double freq_hz, fnec_deck_problem *out);
+
+/* Solve one built problem on the GPU: route 0 or 1 as fnec_hip_solve_hallen or
+ * fnec_hip_solve_hallen_paths, with the request's LD/TL/NT stamps added to Z
+ * after the fill (solve_session.rs:1285-1309). x_out holds n currents followed
+ * by the route's homogeneous constants: route 0 max(n_wires, 1), route 1 one
+ * per conductor path (max path_of + 1). */
+int fnec_deck_solve(const fnec_deck_problem *pr, int precision, fnec_c64 *x_out,
+                    fnec_solve_stats *stats);
 
 /* CLI pipeline for one frequency of the deck on the GPU: solve, feedpoint
  * rows, currents, RP with normalisation, finite-ground efficiency, and the

@@ -17,7 +17,7 @@
 
 #include "oracle_internal.h"
 
-static int parse_f64_strict(const char *s, double *out)
+int or_parse_f64_strict(const char *s, double *out)
 {
     /* Rust's str::parse::<f64>: optional sign, decimal digits with optional
      * fraction and exponent, or inf/infinity/nan. No hex, no whitespace. */
@@ -156,8 +156,11 @@ or_deck *or_deck_parse(const char *text, char *err, int errlen)
             memcpy(c.mn, mn, 2);
             c.mn[2] = 0;
             c.nf = nf;
+            if (!strcmp(mn, "NT"))
+                for (i = 0; i < nf && i < OR_MAX_RAW; i++)
+                    snprintf(c.raw[i], sizeof(c.raw[i]), "%s", toks[1 + i]);
             for (i = 0; i < nparse && i < OR_MAX_FIELDS; i++) {
-                if (parse_f64_strict(toks[1 + i], &c.f[i])) {
+                if (or_parse_f64_strict(toks[1 + i], &c.f[i])) {
                     if (err)
                         snprintf(err, errlen, "line %d: %s field %d: cannot parse '%s'", lineno,
                                  mn, i + 1, toks[1 + i]);

@@ -100,6 +100,21 @@ int or_build_hallen_rhs_paths(const or_deck *d, const or_seg *s, int n, double f
 void or_fill_z(const or_seg *s, int n, double freq_hz, or_ground g, cx *z /* n*n row-major */);
 cx or_fill_elem(const or_seg *obs, const or_seg *src, double k, int is_self);
 
+/* ---- LD / TL / NT stamps (loads.rs:103-223, tl.rs:47-244, network.rs:55-140) ---- */
+typedef struct {
+    int row, col;
+    cx delta;
+} or_stamp;
+void or_build_loads(const or_deck *d, const or_seg *s, int n, double freq_hz, cx *loads);
+int or_build_stamps(const or_deck *d, const or_seg *s, int n, double freq_hz, int with_nt, or_stamp *out,
+                    int max);
+void or_apply_stamps(cx *z /* n*n row-major */, int n, const or_stamp *st, int ns);
+/* The stamps a deck's text gets at freq_hz under the caller semantics (CLI:
+ * loads, TL, NT; worker and Python: loads, TL). Returns the count, or a
+ * negative status when the deck does not parse or build. */
+int or_deck_stamps(const char *text, int semantics, double freq_hz, int *rows, int *cols, double *re,
+                   double *im, int max);
+
 /* ---- linear (linear.rs) ---- */
 /* Threads for the fill, Gram and elimination row loops (default 1, as the
  * reference runs; results do not depend on it). Fixture generation only. */

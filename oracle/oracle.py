@@ -402,3 +402,21 @@ def deck_inputs(text: str, freq_hz: float, semantics: str = "worker") -> dict:
         return out
     finally:
         L.or_deck_free(d)
+
+
+def stamps(text: str, freq_hz: float, semantics: str = "worker") -> list:
+    """The LD / TL (and for the CLI NT) stamps on Z of a deck at freq_hz, in
+    application order: [(row, col, delta)] (or_deck_stamps)."""
+    L = lib()
+    sem = {"cli": 0, "worker": 1, "py": 2}[semantics]
+    L.or_deck_stamps.argtypes = [C.c_char_p, C.c_int, C.c_double, C.c_void_p, C.c_void_p, C.c_void_p,
+                                 C.c_void_p, C.c_int]
+    cnt = L.or_deck_stamps(text.encode(), sem, float(freq_hz), None, None, None, None, 0)
+    if cnt < 0:
+        raise RuntimeError(f"deck error {-cnt}")
+    rows = np.zeros(max(cnt, 1), np.int32)
+    cols = np.zeros(max(cnt, 1), np.int32)
+    re = np.zeros(max(cnt, 1))
+    im = np.zeros(max(cnt, 1))
+    L.or_deck_stamps(text.encode(), sem, float(freq_hz), _p(rows), _p(cols), _p(re), _p(im), cnt)
+    return [(int(rows[i]), int(cols[i]), complex(re[i], im[i])) for i in range(cnt)]

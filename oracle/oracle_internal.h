@@ -7,10 +7,12 @@
 
 #define OR_MAX_FIELDS 16
 
+#define OR_MAX_RAW 12
 typedef struct {
     char mn[4];
     int nf;
     double f[OR_MAX_FIELDS];
+    char raw[OR_MAX_RAW][40]; /* NT: the raw field text (card.rs NtCard::raw_fields) */
 } or_card;
 
 struct or_deck {
@@ -19,6 +21,7 @@ struct or_deck {
 };
 
 uint32_t or_as_u32(double v);
+int or_parse_f64_strict(const char *s, double *out);
 int32_t or_as_i32(double v);
 void or_deck_rp_flags(const or_deck *d, int *normalize, int *avg_gain);
 

@@ -198,7 +198,22 @@ static void report_v1(const or_deck *d, or_result *res, const char *mode, const 
             sb_add(&b, t);
         }
     }
-    (void)nld;
+    /* LD cards as parsed (solve_session.rs:966-982, nec_report lib.rs:335-344) */
+    if (nld) {
+        snprintf(t, sizeof(t), "\nLOADS\nN_LOADS %d\nTYPE TAG SEG_FIRST SEG_LAST F1 F2 F3\n", nld);
+        sb_add(&b, t);
+        for (i = 0; i < d->n; i++) {
+            const or_card *c = &d->cards[i];
+            if (strcmp(c->mn, "LD"))
+                continue;
+            or_fmt_fixed(f1, 64, c->nf > 4 ? c->f[4] : 0.0, 6);
+            or_fmt_fixed(f2, 64, c->nf > 5 ? c->f[5] : 0.0, 6);
+            or_fmt_fixed(f3, 64, c->nf > 6 ? c->f[6] : 0.0, 6);
+            snprintf(t, sizeof(t), "%d %u %u %u %s %s %s\n", (int)or_as_i32(c->f[0]),
+                     or_as_u32(c->f[1]), or_as_u32(c->f[2]), or_as_u32(c->f[3]), f1, f2, f3);
+            sb_add(&b, t);
+        }
+    }
     sb_add(&b, "\nCURRENTS\nTAG SEG I_RE I_IM I_MAG I_PHASE\n");
     for (i = 0; i < res->n; i++) {
         cx c = res->x[i];
@@ -273,11 +288,6 @@ int or_run_deck(const char *text, int semantics, int freq_index, double freq_ove
         fail(res, OR_PARSE, err);
         return res->status;
     }
-    if (or_deck_has_card(d, "LD") || or_deck_has_card(d, "TL") || or_deck_has_card(d, "NT")) {
-        fail(res, OR_UNSUPPORTED, "LD/TL/NT stamps are outside the Hallén delta-gap path");
-        or_deck_free(d);
-        return res->status;
-    }
     for (i = 0; i < d->n; i++)
         if (!strcmp(d->cards[i].mn, "EX") && (ex_is_plane_wave(&d->cards[i]) || ex_is_current(&d->cards[i]))) {
             fail(res, OR_UNSUPPORTED, "plane-wave / current-source excitation");
@@ -321,6 +331,17 @@ int or_run_deck(const char *text, int semantics, int freq_index, double freq_ove
         }
     }
     or_fill_z(res->segs, n, res->freq_hz, res->ground, z);
+    {
+        /* LD, TL (and for the CLI NT) stamps on Z (solve_session.rs:1285-1309,
+         * nec_worker solve.rs:179-185, fnec_py lib.rs:58-63) */
+        const int ns = or_build_stamps(d, res->segs, n, res->freq_hz, semantics == OR_SEM_CLI, NULL, 0);
+        if (ns > 0) {
+            or_stamp *st = (or_stamp *)malloc(sizeof(or_stamp) * (size_t)ns);
+            or_build_stamps(d, res->segs, n, res->freq_hz, semantics == OR_SEM_CLI, st, ns);
+            or_apply_stamps(z, n, st, ns);
+            free(st);
+        }
+    }
     res->diag_spread = or_diag_spread(z, n);
 
     if (semantics == OR_SEM_CLI) {
@@ -535,4 +556,36 @@ void or_result_free(or_result *res)
     free(res->rp);
     free(res->report);
     memset(res, 0, sizeof(*res));
+}
+
+int or_deck_stamps(const char *text, int semantics, double freq_hz, int *rows, int *cols, double *re,
+                   double *im, int max)
+{
+    char err[256];
+    or_deck *d = or_deck_parse(text, err, sizeof(err));
+    or_seg *segs;
+    or_stamp *st;
+    int n, ns, i;
+    if (!d)
+        return -OR_PARSE;
+    segs = (or_seg *)malloc(sizeof(or_seg) * (size_t)(1 << 20));
+    n = or_build_geometry(d, segs, 1 << 20, err, sizeof(err));
+    if (n < 0) {
+        free(segs);
+        or_deck_free(d);
+        return n;
+    }
+    ns = or_build_stamps(d, segs, n, freq_hz, semantics == OR_SEM_CLI, NULL, 0);
+    st = (or_stamp *)malloc(sizeof(or_stamp) * (size_t)(ns > 0 ? ns : 1));
+    or_build_stamps(d, segs, n, freq_hz, semantics == OR_SEM_CLI, st, ns);
+    for (i = 0; i < ns && i < max; i++) {
+        rows[i] = st[i].row;
+        cols[i] = st[i].col;
+        re[i] = st[i].delta.re;
+        im[i] = st[i].delta.im;
+    }
+    free(st);
+    free(segs);
+    or_deck_free(d);
+    return ns;
 }

@@ -276,7 +276,8 @@ def print_unit(tok: str) -> float:
 # bars of test_solve_matches_oracle instead.
 @pytest.mark.parametrize("name", ["dipole-freesp-rp-large-grid", "dipole-ground-rp-51seg",
                                   "dipole-gn0-fresnel-51seg", "frequency-sweep-dipole",
-                                  "yagi-5elm-51seg"])
+                                  "yagi-5elm-51seg", "dipole-ld-series-rl-51seg",
+                                  "tl-two-dipoles-linked", "dipole-nt-tl-equiv-freesp-51seg"])
 def test_cli_report_matches_oracle(oracle, name):
     """Report contract v1 from the GPU pipeline vs the oracle: identical
     layout and tokens, except numbers that differ by one unit in their last
@@ -294,11 +295,11 @@ def test_cli_report_matches_oracle(oracle, name):
             t = b.split()
             if len(t) == 6 and section == "CURRENTS" and t[0].isdigit():
                 imax = max(imax, float(t[4]))
-            if b in ("CURRENTS", "FEEDPOINTS", "RADIATION_PATTERN", "SOURCES", "NORMALIZED_PATTERN"):
+            if b in ("CURRENTS", "FEEDPOINTS", "RADIATION_PATTERN", "SOURCES", "LOADS", "NORMALIZED_PATTERN"):
                 section = b
         section = ""
         for a, b in zip(got, want):
-            if b in ("CURRENTS", "FEEDPOINTS", "RADIATION_PATTERN", "SOURCES", "NORMALIZED_PATTERN"):
+            if b in ("CURRENTS", "FEEDPOINTS", "RADIATION_PATTERN", "SOURCES", "LOADS", "NORMALIZED_PATTERN"):
                 section = b
             if a == b:
                 continue

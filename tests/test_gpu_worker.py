@@ -1,11 +1,13 @@
 """GPU test of fnec_hip_worker --serve on a request stream: 120 tasks of the
 mixed-N serving stream (N <= 2048, parallel and junctioned wire sets, some
-over PEC ground), written to the worker's stdin in one go with malformed and
-unsupported tasks among them; the worker answers each as its batch finishes,
-out of order, and exits at end of input after answering every task.
+over PEC ground), written to the worker's stdin in one go with malformed,
+unsupported and loaded (LD / TL) tasks among them; the worker answers each as
+its batch finishes, out of order, and exits at end of input after answering
+every task.
 
 - every task_id is answered exactly once; the bad tasks with their error
-  codes;
+  codes; the loaded task within 1e-6 of the plain FP64 solve of the same
+  stamped problem;
 - mixed-precision tasks (exec "gpu"): Z_in and, where asked for, the currents
   within 1e-6 of the plain FP64 solve of the same Hallén inputs, and the
   pattern rows within 1e-6 (linear gain) of the plain path's pattern;
@@ -75,15 +77,34 @@ def test_worker_serves_a_stream():
             lines.append(json.dumps({"task_id": "bad64", "deck_hash": "x", "deck_b64": "!!",
                                      "solver_config": {}, "frequency_hz": 1.0}))
         if k == 50:
-            loaded = r["deck"].replace("EX ", "LD 0 1 1 1 50 0 0\nEX ", 1)
+            loaded = r["deck"].replace("EX ", "LD 0 1 1 1 50 1e-7 0\nLD 5 2 0 0 5.8e7 0 0\n"
+                                                 "TL 1 1 2 1 1 0 50 0.7 1.0\nEX ", 1)
             assert "LD " in loaded
-            lines.append(msg("loaded", loaded, r["freq_hz"]))
+            lines.append(msg("loaded", loaded, r["freq_hz"], return_currents=True))
+            plane = r["deck"].replace("EX ", "EX 1 1 1 0 0 0 0\nEX ", 1)
+            lines.append(msg("plane", plane, r["freq_hz"]))
+            extra_decks = {"loaded": (loaded, r["freq_hz"])}
     out = serve(lines)
     ids = [o["task_id"] for o in out]
-    assert sorted(ids) == sorted(list(meta) + ["bad64", "loaded"])
+    assert sorted(ids) == sorted(list(meta) + ["bad64", "loaded", "plane"])
     got = {o["task_id"]: o for o in out}
     assert got["bad64"]["error_code"] == "parse_error"
-    assert got["loaded"]["error_code"] == "unsupported_config"
+    assert got["plane"]["error_code"] == "unsupported_config"
+    deck, freq = extra_decks["loaded"]
+    d = F.Deck(deck)
+    p = d.problem("worker", freq)
+    d.close()
+    assert len(p["stamps"]) > 2
+    x = F.solve_problem(p, precision=F.PREC_FP64)
+    z = p["v_feed"] / x[int(p["feed_index"])]
+    o = got["loaded"]
+    assert o["status"] == "ok", o
+    zw = complex(o["impedance"]["re_ohm"], o["impedance"]["im_ohm"])
+    xw = np.array([complex(a, b) for a, b in o["currents"]])
+    n = len(p["segs"])
+    print(f"loaded task: |dZ|/|Z| {abs(zw - z) / abs(z):.2e}")
+    assert abs(zw - z) <= 1e-6 * abs(z)
+    assert np.linalg.norm(xw[:n] - x[:n]) <= 1e-6 * np.linalg.norm(x[:n])
     order = [i for i in ids if i in meta]
     print(f"replies out of submission order: {sum(a != b for a, b in zip(order, list(meta)))} of {len(order)}")
     worst_z = worst_x = worst_g = 0.0

@@ -140,9 +140,10 @@ def test_config3_has_one_kcl_row():
 
 
 def test_unsupported_cards_signal_cpu_fallback():
-    txt = deck_text("dipole-freesp-rp-51seg").replace("EN", "LD 0 1 26 26 50 0 0\nEN")
-    with pytest.raises(F.FnecUnavailable):
-        F.Deck(txt).problem("cli", 14.2e6)
+    for ex in ("EX 1 1 1 0 0 0 0", "EX 4 1 26 0 1 0"):
+        txt = deck_text("dipole-freesp-rp-51seg").replace("EN", ex + "\nEN")
+        with pytest.raises(F.FnecUnavailable):
+            F.Deck(txt).problem("cli", 14.2e6)
 
 
 def test_parse_errors():

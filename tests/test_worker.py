@@ -10,7 +10,7 @@ worker tests (crates/nec_worker/src/worker.rs:145-270, solve.rs:16-80):
   not read, end of input ends the loop;
 - task errors: malformed JSON and missing fields (task_id "unknown"), bad
   base64 and non-UTF-8 decks (parse_error with the task's id), unsupported
-  basis and loaded decks (unsupported_config), an empty or garbage deck
+  basis and plane-wave decks (unsupported_config), an empty or garbage deck
   (geometry error), a deck without a type-0 EX (no feedpoint).
 The GPU test test_gpu_worker.py serves a request stream through it.
 """
@@ -97,9 +97,9 @@ def test_bad_base64_and_utf8():
 
 def test_deck_level_errors():
     nofeed = "CM test\nGW 0 1 0 0 0 0 0 1 0.001\nGE 0\nFR 0 1 0 0 14.175 0\nEN\n"
-    loaded = DIPOLE.replace("GE 0\n", "GE 0\nLD 0 1 6 6 50 0 0\n")
+    plane = DIPOLE.replace("EX 0 1 6 0 1 0\n", "EX 0 1 6 0 1 0\nEX 1 1 1 0 0 0 0\n")
     out = run("\n".join([task("basis", DIPOLE, basis="pulse"), task("empty", ""), task("garbage", "NOT A NEC DECK"),
-                         task("nofeed", nofeed), task("loaded", loaded),
+                         task("nofeed", nofeed), task("plane", plane),
                          json.dumps({**json.loads(task("prec", DIPOLE)),
                                      "solver_config": {"basis": "hallen", "precision": "fp16"}})]) + "\n")
     got = {r["task_id"]: r for r in out}
@@ -108,7 +108,7 @@ def test_deck_level_errors():
     assert got["empty"]["error_code"] == "parse_error" and got["empty"]["error_message"].startswith("geometry error")
     assert got["garbage"]["error_code"] == "parse_error"
     assert got["nofeed"]["error_message"] == "no EX type-0 card found in deck"
-    assert got["loaded"]["error_code"] == "unsupported_config"
+    assert got["plane"]["error_code"] == "unsupported_config"
     assert got["prec"]["error_code"] == "unsupported_config"
 
 
