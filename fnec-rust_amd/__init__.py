@@ -34,7 +34,7 @@ EXPORTS = [
     "fnec_hip_request_footprint", "fnec_hip_debug_normal_eqs", "fnec_hip_debug_lu_f32", "fnec_hip_debug_mfma_probe",
     "fnec_hip_debug_mfma_probe_f32", "fnec_hip_set_tunable", "fnec_hip_get_tunable",
     "fnec_hip_debug_solve_z", "fnec_hip_debug_solve_square", "fnec_hip_debug_refmath",
-    "fnec_hip_rp_noalloc",
+    "fnec_hip_rp_noalloc", "fnec_hip_debug_fill_fast",
 ]
 
 
@@ -113,6 +113,7 @@ def lib():
         vp, sz, dbl, i32 = C.c_void_p, C.c_size_t, C.c_double, C.c_int
         L.fnec_hip_last_error.restype = C.c_char_p
         L.fnec_hip_fill_z.argtypes = [vp, sz, dbl, C.POINTER(Ground), vp]
+        L.fnec_hip_debug_fill_fast.argtypes = [vp, sz, dbl, C.POINTER(Ground), vp]
         L.fnec_hip_solve_hallen.argtypes = [vp, sz, vp, vp, vp, vp, sz, vp, vp, vp, sz, dbl,
                                             C.POINTER(Ground), i32, vp, C.POINTER(SolveStats)]
         L.fnec_hip_solve_hallen_paths.argtypes = [vp, sz, vp, vp, vp, vp, sz, dbl, C.POINTER(Ground),
@@ -184,11 +185,14 @@ def device_count() -> int:
     return lib().fnec_hip_device_count()
 
 
-def fill_z(segs: np.ndarray, freq_hz: float, g: Ground | None = None) -> np.ndarray:
+def fill_z(segs: np.ndarray, freq_hz: float, g: Ground | None = None, fast: bool = False) -> np.ndarray:
+    """Z (row-major): the plain path's fill, or with fast the mixed and
+    FP64-MFMA paths' (fnec_hip_debug_fill_fast)."""
     segs = np.ascontiguousarray(segs, SEG)
     n = len(segs)
     z = np.zeros((n, n), np.complex128)
-    _check(lib().fnec_hip_fill_z(_p(segs), n, float(freq_hz), C.byref(g or ground()), _p(z)))
+    fn = lib().fnec_hip_debug_fill_fast if fast else lib().fnec_hip_fill_z
+    _check(fn(_p(segs), n, float(freq_hz), C.byref(g or ground()), _p(z)))
     return z
 
 

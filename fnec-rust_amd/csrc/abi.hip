@@ -83,7 +83,28 @@ int fnec_hip_fill_z(const fnec_seg *segs, size_t n, double freq_hz, const fnec_g
     st = build_problem(hp, segs, n, rhs.data(), cosv.data(), row_col, 1, {}, {}, freq_hz, ground);
     if (st)
         return st;
-    return fill_only(hp, z_out);
+    return fill_only(hp, z_out, true);
+}
+
+int fnec_hip_debug_fill_fast(const fnec_seg *segs, size_t n, double freq_hz, const fnec_ground *ground,
+                             fnec_c64 *z_out)
+{
+    std::lock_guard<std::mutex> lk(global_mutex());
+    int st = ensure_ready();
+    if (st)
+        return st;
+    if (!z_out) {
+        set_error("null output");
+        return FNEC_INVALID_ARG;
+    }
+    std::vector<fnec_c64> rhs(n, fnec_c64{0.0, 0.0});
+    std::vector<double> cosv(n, 0.0);
+    HostProblem hp;
+    std::vector<int> row_col(n, 0);
+    st = build_problem(hp, segs, n, rhs.data(), cosv.data(), row_col, 1, {}, {}, freq_hz, ground);
+    if (st)
+        return st;
+    return fill_only(hp, z_out, false);
 }
 
 int fnec_hip_solve_hallen(const fnec_seg *segs, size_t n, const fnec_c64 *rhs, const double *cos_vec,

@@ -33,7 +33,7 @@ struct Prob {
     int ldz;    // leading dimension of Z
     int lda;    // leading dimension of the augmented normal matrix
     int ground; // 0 free, 1 PEC, 2 scalar-Gamma image
-    int pad0;
+    int fill_centred; // k_fill_z: phases taken relative to the source centre's (1) or whole (0)
     double k;        // fill wavenumber 2*pi*f/c0 (matrix.rs:194)
     double gam_re;   // scalar reflection coefficient for ground == 2
     double gam_im;
@@ -124,6 +124,39 @@ __device__ __forceinline__ void sincos_phase(double x, double &s, double &c)
         s = -s;
     if ((q + 1) & 2)
         c = -c;
+}
+
+// sin and cos of a small phase by their Taylor polynomials, sin to d^kSinDeg
+// and cos to d^(kSinDeg + 1), no range reduction. The fill uses them for
+//   kSinDeg  9: |d| <= 0.121, truncation below 1.9e-18 (sin) and 2e-20 (cos)
+//   kSinDeg 11: |d| <= 0.250, below 2.4e-18 and 5e-20
+//   kSinDeg 13: |d| <= 0.490, below 1.8e-17 and 6e-19.
+template <int kSinDeg> __device__ __forceinline__ void sincos_small(double d, double &s, double &c)
+{
+    static_assert(kSinDeg == 9 || kSinDeg == 11 || kSinDeg == 13, "degree 9, 11 or 13");
+    const double d2 = d * d;
+    double ps, pc;
+    if constexpr (kSinDeg == 13) {
+        ps = fma(d2, 1.6059043836821613e-10, -2.505210838544172e-08); // 1/13!, -1/11!
+        ps = fma(d2, ps, 2.7557319223985893e-06);
+        pc = fma(d2, -1.1470745597729725e-11, 2.08767569878681e-09); // -1/14!, 1/12!
+        pc = fma(d2, pc, -2.755731922398589e-07);
+    } else if constexpr (kSinDeg == 11) {
+        ps = fma(d2, -2.505210838544172e-08, 2.7557319223985893e-06); // -1/11!, 1/9!
+        pc = fma(d2, 2.08767569878681e-09, -2.755731922398589e-07);   // 1/12!, -1/10!
+    } else {
+        ps = 2.7557319223985893e-06; // 1/9!
+        pc = -2.755731922398589e-07; // -1/10!
+    }
+    ps = fma(d2, ps, -0.0001984126984126984); // -1/7!
+    ps = fma(d2, ps, 0.008333333333333333);
+    ps = fma(d2, ps, -0.16666666666666666);
+    s = fma(d * d2, ps, d);
+    pc = fma(d2, pc, 2.48015873015873e-05); // 1/8!
+    pc = fma(d2, pc, -0.001388888888888889);
+    pc = fma(d2, pc, 0.041666666666666664);
+    pc = fma(d2, pc, -0.5);
+    c = fma(d2, pc, 1.0);
 }
 
 constexpr int kMixMaxCons = 256; // constraint rows the mixed path accepts

@@ -24,6 +24,7 @@ struct HostProblem {
     std::vector<int> cons;     // 2 per constraint row
     std::vector<double> cons_sign;
     std::vector<double> stamps; // 4 per Z stamp, applied in order: row, col, re, im
+    double phase_max = 0.0;     // bound on every fill phase k R (direct and image)
 };
 
 // Host-side description of one far-field job. The grid is borrowed from
@@ -117,7 +118,8 @@ int solve_problems(const std::vector<const HostProblem *> &hps, std::vector<fnec
                    std::vector<fnec_solve_stats> &stats, std::vector<int> &status,
                    std::vector<double *> *device_x, int precision,
                    const std::function<void(const std::vector<double *> &)> &overlap = {});
-int fill_only(const HostProblem &hp, fnec_c64 *z_out);
+// Z alone: the plain path's fill (exact) or the fast paths' (k_fill_z).
+int fill_only(const HostProblem &hp, fnec_c64 *z_out, bool exact);
 int normal_eqs_only(const HostProblem &hp, fnec_c64 *aug_out);
 int solve_given_z(const HostProblem &hp, const fnec_c64 *z, fnec_c64 *x_out, fnec_solve_stats *stats);
 int solve_square_exact(const fnec_c64 *a, const fnec_c64 *b, int n, fnec_c64 *x_out, int *ipiv_out,

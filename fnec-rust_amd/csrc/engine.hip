@@ -24,6 +24,7 @@ namespace fnec {
 
 // kernels.hip
 __global__ void k_fill_z(const Prob *, const int4 *);
+__global__ void k_fill_z_large(const Prob *, const int4 *);
 __global__ void k_stamp_z(const Prob *, const int *);
 __global__ void k_gram(const Prob *, const int4 *);
 __global__ void k_gram_cols(const Prob *, const int2 *);
@@ -601,6 +602,22 @@ int build_problem(HostProblem &hp, const fnec_seg *segs, size_t n, const fnec_c6
         o[6] = segs[i].length;
         o[7] = segs[i].radius;
     }
+    // Any two quadrature points p, q (or p and q's image) are at most
+    // |p| + |q| apart, and R adds at most the radius: k (2 max|p| + max a)
+    // bounds every phase of the fill (NaN inputs give NaN: the large form).
+    {
+        double pmax = 0.0, amax = 0.0;
+        for (size_t i = 0; i < n; ++i) {
+            const double *o = &hp.seg[i * kSegStride];
+            const double m = std::sqrt(o[0] * o[0] + o[1] * o[1] + o[2] * o[2]);
+            const double d = std::sqrt(o[3] * o[3] + o[4] * o[4] + o[5] * o[5]);
+            pmax = std::max(pmax, m + 0.5 * std::fabs(o[6]) * d);
+            amax = std::max(amax, std::fabs(o[7]));
+            if (!std::isfinite(m + d + o[6] + o[7]))
+                pmax = NAN;
+        }
+        hp.phase_max = hp.k * (2.0 * pmax + amax);
+    }
     self_log_terms(hp);
     hp.cosv.assign(cos_vec, cos_vec + n);
     hp.rhs.resize(2 * n);
@@ -773,6 +790,7 @@ struct Plan {
     // plain path in the reference's operation order (FNEC_PREC_FP64); the
     // FP64 MFMA path (FNEC_PREC_FP64_MFMA) leaves it false
     bool exact = false;
+    bool fill_large = false; // some fill phase may reach 9e4: k_fill_z_large
     bool given_z = false;  // Z uploaded by the caller (debug entry): no fill launch
     bool given_a = false;  // [A | b] uploaded by the caller: elimination only
     size_t xcols_off = 0, xcols_cnt = 0;
@@ -860,6 +878,7 @@ static int g_update_k128 = 1;    // K = 128 trailing-update passes (with 64-row 
 // persistent grid holds every CU's LDS, which delays the other stream's
 // panel kernels that overlap the update in a serving window)
 static int g_update_ring = 0;
+static int g_fill_centred = 1;
 static int g_mix_tol_exp = 14; // CG stops at |h - (M^H M + lambda) x| <= 10^-exp |h|
 static int g_batch_split_n = 1024; // serving batches: N classes split at this size (0: one class)
 static int g_gang_n = 512;
@@ -1240,6 +1259,10 @@ static void plan_build(Plan &pl, const std::vector<const HostProblem *> &hps, bo
     pl.off_pmeta.resize(np);
     pl.off_selfln.resize(np);
     pl.off_stamps.resize(np);
+    pl.fill_large = false;
+    for (const HostProblem *h : hps)
+        if (!(h->phase_max < 9e4))
+            pl.fill_large = true;
     pl.mix.clear();
     // A planes first, contiguous, so one zero-fill covers them
     pl.a_begin = 0;
@@ -1587,6 +1610,7 @@ static int plan_upload(Plan &pl, const std::vector<const HostProblem *> &hps, ch
         P.ldz = roundup(h.n, 64);
         P.lda = roundup(h.s, 64);
         P.ground = h.ground;
+        P.fill_centred = g_fill_centred;
         P.k = h.k;
         P.gam_re = h.gam_re;
         P.gam_im = h.gam_im;
@@ -2049,7 +2073,8 @@ static int plan_launch(Plan &pl, char *base, bool with_solve)
     gen_launch(pl, base, st);
     if (pl.fill_cnt && !pl.given_z && !pl.given_a) {
         hipEvent_t e = prof_begin(PC_FILL);
-        hipLaunchKernelGGL(pl.exact ? k_fill_exact : k_fill_z, dim3((unsigned)pl.fill_cnt), dim3(256), 0,
+        hipLaunchKernelGGL(pl.exact ? k_fill_exact : pl.fill_large ? k_fill_z_large : k_fill_z,
+                           dim3((unsigned)pl.fill_cnt), dim3(256), 0,
                            st, dp, (const int4 *)gen_ptr(base, pl) + pl.fill_off);
         prof_end(PC_FILL, e, pl.fill_flops, 0);
         if (pl.stamp_cnt)
@@ -2256,13 +2281,13 @@ int solve_problems(const std::vector<const HostProblem *> &hps, std::vector<fnec
     return FNEC_OK;
 }
 
-int fill_only(const HostProblem &hp, fnec_c64 *z_out)
+int fill_only(const HostProblem &hp, fnec_c64 *z_out, bool exact)
 {
     LaneScope lane_scope;
     if (lane_scope.status())
         return lane_scope.status();
     Plan pl;
-    pl.exact = true;
+    pl.exact = exact;
     std::vector<const HostProblem *> v{&hp};
     plan_build(pl, v, false);
     char *base = nullptr;
@@ -2882,6 +2907,10 @@ int set_tunable(const char *name, long value)
         g_exact_backsub = (int)value;
         return FNEC_OK;
     }
+    if (k == "fill_centred" && (value == 0 || value == 1)) {
+        g_fill_centred = (int)value;
+        return FNEC_OK;
+    }
     if (k == "lanes" && value >= 1 && value <= kMaxLanes) {
         std::lock_guard<std::mutex> lk(g_lane_mu);
         g_lanes_on = (int)value;
@@ -2917,6 +2946,8 @@ long get_tunable(const char *name)
         return g_mix_tol_exp;
     if (k == "exact_backsub")
         return g_exact_backsub;
+    if (k == "fill_centred")
+        return g_fill_centred;
     return -1;
 }
 

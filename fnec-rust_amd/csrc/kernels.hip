@@ -13,6 +13,7 @@
 #include <math.h>
 
 #include "common.h"
+#include "phase_table.h"
 
 namespace fnec {
 
@@ -43,44 +44,65 @@ __constant__ double c_gl8w[8] = {0.101228536290376, 0.222381034453374, 0.3137066
 // Z fill
 // ---------------------------------------------------------------------------
 
-// Off-diagonal element integral  half * sum_m w_m exp(-jkR_m)/R_m  over the
-// source segment (src mid s, dir d, half length h, radius a) seen from obs.
-__device__ __forceinline__ void quad8(double ox, double oy, double oz, double sx, double sy,
-                                      double sz, double dx, double dy, double dz, double h,
-                                      double a2, double k, double &re, double &im)
+// (sin, cos)(j pi/32), j = 0 .. 63, for sincos_tab
+__constant__ double2 c_phase_tab[FNEC_PHASE_TABLE_LEN] = {FNEC_PHASE_TABLE_INIT};
+
+// Copies the table into a kernel's LDS (the caller synchronises).
+__device__ __forceinline__ void load_phase_tab(double2 *tab)
 {
-    // R^2 at the point sc along the unit direction: |d - sc dir|^2 + a^2 =
-    // (|d|^2 + a^2) + sc (sc - 2 d.dir), d = observation - source centre
-    const double ex = ox - sx, ey = oy - sy, ez = oz - sz;
-    const double A = ex * ex + ey * ey + ez * ez + a2;
-    const double B = -2.0 * (ex * dx + ey * dy + ez * dz);
-    double sr = 0.0, si = 0.0;
-    // two points per iteration: two independent dependency chains per lane
-#pragma unroll 2
-    for (int m = 0; m < 8; ++m) {
-        const double sc = c_gl8n[m] * h;
-        const double r2 = fma(sc, sc + B, A);
-        // one reciprocal square root for R and 1/R: the hardware estimate
-        // y and one third-order correction y += y e (1/2 + 3/8 e), e = 1 -
-        // r2 y^2, which takes the estimate's relative error d to about
-        // (5/16) d^3, below double rounding (r2 >= a^2 > 0 is finite and
-        // normal, so rsqrt()'s special-case handling is not needed)
-        double ir = __builtin_amdgcn_rsq(r2);
-        const double e = fma(-r2 * ir, ir, 1.0);
-        ir = fma(ir * e, fma(0.375, e, 0.5), ir);
-        const double r = r2 * ir;
-        double s, c;
-        sincos_phase(k * r, s, c);
-        double wr = c_gl8w[m] * ir;
-        sr += wr * c;
-        si -= wr * s;
+    for (int e = threadIdx.x; e < FNEC_PHASE_TABLE_LEN; e += blockDim.x)
+        tab[e] = c_phase_tab[e];
+}
+
+// sin and cos of a phase by Tang's table method: x = n pi/32 + r with
+// n = rint(32 x / pi) and r from a three-part Cody-Waite reduction (the parts
+// of sincos_phase's pi/2 split divided by 16, so n P1 and n P2 stay exact for
+// |n| < 2^20), |r| <= pi/64; the Taylor polynomials of sin r (to r^9) and
+// cos r - 1 (to r^8), truncated below 1e-22 and 3e-20; then
+//   sin x = S + (S (cos r - 1) + C sin r),  cos x = C + (C (cos r - 1) - S sin r)
+// with (S, C) = (sin, cos)(n pi/32) from the table (n mod 64). About 1 ulp.
+// kLarge: arguments beyond 1e5 take the library routine; without it the
+// caller guarantees |x| < 1e5.
+template <bool kLarge>
+__device__ __forceinline__ void sincos_tab(double x, const double2 *tab, double &s, double &c)
+{
+    if (kLarge && !(fabs(x) < 1e5)) {
+        sincos(x, &s, &c);
+        return;
     }
-    re = sr * h;
-    im = si * h;
+    const double n = rint(x * 10.185916357881302); // 32 / pi
+    double r = fma(-n, 0x1.921fb544p-4, x);
+    r = fma(-n, 0x1.0b4611a6p-38, r);
+    r = fma(-n, 0x1.3198a2e037073p-73, r);
+    const double r2 = r * r;
+    double ps = fma(r2, 2.7557319223985893e-06, -0.0001984126984126984); // 1/9!, -1/7!
+    ps = fma(r2, ps, 0.008333333333333333);
+    ps = fma(r2, ps, -0.16666666666666666);
+    const double sr = fma(r * r2, ps, r);
+    double pc = fma(r2, 2.48015873015873e-05, -0.001388888888888889); // 1/8!, -1/6!
+    pc = fma(r2, pc, 0.041666666666666664);
+    pc = fma(r2, pc, -0.5);
+    const double cm1 = r2 * pc;
+    const double2 t = tab[(int)n & 63]; // two's complement: n mod 64 also for n < 0
+    s = fma(t.y, sr, fma(t.x, cm1, t.x));
+    c = fma(-t.x, sr, fma(t.y, cm1, t.y));
+}
+
+// 1/sqrt(r2) for r2 >= a^2 > 0 (finite and normal, so rsqrt()'s special-case
+// handling is not needed): the hardware estimate y and one third-order
+// correction y += y e (1/2 + 3/8 e), e = 1 - r2 y^2, which takes the
+// estimate's relative error d to about (5/16) d^3, below double rounding.
+__device__ __forceinline__ double rsqrt_fill(double r2)
+{
+    double ir = __builtin_amdgcn_rsq(r2);
+    const double e = fma(-r2 * ir, ir, 1.0);
+    return fma(ir * e, fma(0.375, e, 0.5), ir);
 }
 
 // Self element: 4-point GL on G - 1/R plus 2 ln((h + R_end)/a) (matrix.rs:302-316).
-__device__ __forceinline__ void self_term(double h, double a, double k, double &re, double &im)
+template <bool kLarge>
+__device__ __forceinline__ void self_term(double h, double a, double k, const double2 *tab, double &re,
+                                          double &im)
 {
     double sr = 0.0, si = 0.0;
 #pragma unroll 1
@@ -88,7 +110,7 @@ __device__ __forceinline__ void self_term(double h, double a, double k, double &
         double l = c_gl4n[m] * h;
         double r = sqrt(l * l + a * a);
         double s, c;
-        sincos_phase(k * r, s, c);
+        sincos_tab<kLarge>(k * r, tab, s, c);
         double inv = 1.0 / r;
         sr += c_gl4w[m] * (c * inv - inv);
         si -= c_gl4w[m] * (s * inv);
@@ -97,6 +119,98 @@ __device__ __forceinline__ void self_term(double h, double a, double k, double &
     re = sr * h + 2.0 * log((h + rend) / a);
     im = si * h;
 }
+
+// Off-diagonal element integral  half * sum_m w_m exp(-jkR_m)/R_m  over the
+// source segment (src mid s, dir d, half length h, radius a) seen from obs;
+// sc[m] = node_m h.
+// kSinDeg 0: every point's phase k R_m reduced (sincos_tab).
+// kSinDeg 9, 11, 13, the centred form: exp(-jkR_m) = exp(-jkR_0) exp(-jd_m)
+// with R_0 the distance to the source centre and d_m = k R_m - k R_0. Since
+// |R_m - R_0| <= |sc_m| <= 0.9603 h, |d_m| <= 0.9603 k h needs no range
+// reduction (sincos_small of that degree), and the full phase's sin and cos
+// are taken once per element.
+template <int kSinDeg, bool kLarge>
+__device__ __forceinline__ void quad8(double ox, double oy, double oz, double sx, double sy,
+                                      double sz, double dx, double dy, double dz, double h,
+                                      double a2, double k, const double *sc8, const double2 *tab,
+                                      double &re, double &im)
+{
+    // R^2 at the point sc along the unit direction: |d - sc dir|^2 + a^2 =
+    // (|d|^2 + a^2) + sc (sc - 2 d.dir), d = observation - source centre
+    const double ex = ox - sx, ey = oy - sy, ez = oz - sz;
+    const double A = ex * ex + ey * ey + ez * ez + a2;
+    const double B = -2.0 * (ex * dx + ey * dy + ez * dz);
+    double sr = 0.0, si = 0.0;
+    if constexpr (kSinDeg > 0) {
+        const double ph0 = k * (A * rsqrt_fill(A));
+#pragma unroll 4
+        for (int m = 0; m < 8; ++m) {
+            const double sc = sc8[m];
+            const double r2 = fma(sc, sc + B, A);
+            const double ir = rsqrt_fill(r2);
+            double s, c;
+            sincos_small<kSinDeg>(fma(k, r2 * ir, -ph0), s, c);
+            const double wr = c_gl8w[m] * ir;
+            sr = fma(wr, c, sr);
+            si = fma(-wr, s, si);
+        }
+        double s0, c0;
+        sincos_tab<kLarge>(ph0, tab, s0, c0);
+        re = fma(sr, c0, si * s0) * h;
+        im = fma(si, c0, -(sr * s0)) * h;
+    } else {
+        // two points per iteration: two independent dependency chains per lane
+#pragma unroll 2
+        for (int m = 0; m < 8; ++m) {
+            const double sc = sc8[m];
+            const double r2 = fma(sc, sc + B, A);
+            const double ir = rsqrt_fill(r2);
+            double s, c;
+            sincos_tab<kLarge>(k * (r2 * ir), tab, s, c);
+            const double wr = c_gl8w[m] * ir;
+            sr += wr * c;
+            si -= wr * s;
+        }
+        re = sr * h;
+        im = si * h;
+    }
+}
+
+// The direct term of an element (self: the self term) and, with ground, the
+// image term, each scaled by its cos alpha; the image of a finite ground by
+// the scalar Gamma.
+template <int kSinDeg, bool kLarge>
+__device__ __forceinline__ void element(const Prob &P, double ox, double oy, double oz, double odx,
+                                        double ody, double odz, const double *sg, const double *sc8,
+                                        const double2 *tab, bool self, double &re, double &im)
+{
+    const double sx = sg[0], sy = sg[1], sz = sg[2], dx = sg[3], dy = sg[4], dz = sg[5];
+    const double h = sg[6] * 0.5, a2 = sg[7] * sg[7], k = P.k;
+    if (self)
+        self_term<kLarge>(h, sg[7], k, tab, re, im);
+    else
+        quad8<kSinDeg, kLarge>(ox, oy, oz, sx, sy, sz, dx, dy, dz, h, a2, k, sc8, tab, re, im);
+    const double ca = odx * dx + ody * dy + odz * dz;
+    re *= ca;
+    im *= ca;
+    if (P.ground != 0) {
+        // image: z -> -z, direction (-dx, -dy, +dz) (matrix.rs:265-277)
+        double ir, ii;
+        quad8<kSinDeg, kLarge>(ox, oy, oz, sx, sy, -sz, -dx, -dy, dz, h, a2, k, sc8, tab, ir, ii);
+        const double ica = odx * (-dx) + ody * (-dy) + odz * dz;
+        ir *= ica;
+        ii *= ica;
+        if (P.ground == 2) {
+            const double gr = P.gam_re * ir - P.gam_im * ii;
+            const double gi = P.gam_re * ii + P.gam_im * ir;
+            ir = gr;
+            ii = gi;
+        }
+        re += ir;
+        im += ii;
+    }
+}
+
 
 // Regular work lists from segment records (first item, problem, ny,
 // nz | gy << 24): items (problem, y, z, 0). gy = 0: y-major order. gy > 0:
@@ -125,16 +239,25 @@ constexpr int kFillCols = 16;
 
 // Work item: (problem, row tile, column tile). Each thread owns one
 // observation row and four source columns; a wavefront writes 64 consecutive
-// rows of one column, so the column-major stores are fully coalesced.
-__global__ __launch_bounds__(256) void k_fill_z(const Prob *__restrict__ probs,
-                                                const int4 *__restrict__ work)
+// rows of one column, so the column-major stores are fully coalesced. The
+// column's k h (uniform in a wavefront) picks the element form: the centred
+// form with the shortest polynomial its phase offsets allow (k h <= 0.126,
+// 0.26, 0.51), else every point's phase reduced. kLarge keeps the library
+// sin / cos for phases beyond 1e5 (the host bounds every phase by
+// k (2 max|p| + max a) and launches the kLarge form only when that reaches
+// 9e4).
+template <bool kLarge> __device__ __forceinline__ void fill_z_body(const Prob *__restrict__ probs,
+                                                                  const int4 *__restrict__ work)
 {
     const int4 wk = work[blockIdx.x];
     const Prob &P = probs[wk.x];
     const int i0 = wk.y * kFillRows, j0 = wk.z * kFillCols;
     __shared__ double so[kFillRows][6];
     __shared__ double ss[kFillCols][8];
+    __shared__ double sn[kFillCols][8]; // node_m h of each column
+    __shared__ double2 tab[FNEC_PHASE_TABLE_LEN];
     const int t = threadIdx.x;
+    load_phase_tab(tab);
     for (int e = t; e < kFillRows * 6; e += 256) {
         int row = e / 6, c = e % 6;
         int i = i0 + row;
@@ -144,6 +267,7 @@ __global__ __launch_bounds__(256) void k_fill_z(const Prob *__restrict__ probs,
         int col = e / 8, c = e % 8;
         int j = j0 + col;
         ss[col][c] = j < P.n ? P.seg[(size_t)j * kSegStride + c] : 1.0;
+        sn[col][c] = c_gl8n[c] * (j < P.n ? P.seg[(size_t)j * kSegStride + 6] * 0.5 : 0.5);
     }
     __syncthreads();
     const int row = t & 63;
@@ -163,40 +287,24 @@ __global__ __launch_bounds__(256) void k_fill_z(const Prob *__restrict__ probs,
     }
     const double ox = so[row][0], oy = so[row][1], oz = so[row][2];
     const double odx = so[row][3], ody = so[row][4], odz = so[row][5];
-    const double k = P.k;
 #pragma unroll 1
     for (int q = 0; q < 4; ++q) {
         const int col = (t >> 6) * 4 + q;
         const int j = j0 + col;
         if (j >= P.n)
             break;
-        const double sx = ss[col][0], sy = ss[col][1], sz = ss[col][2];
-        const double dx = ss[col][3], dy = ss[col][4], dz = ss[col][5];
-        const double h = ss[col][6] * 0.5, a = ss[col][7];
+        const double *sg = ss[col];
+        const double kh = P.k * sg[6] * 0.5;
+        const bool self = i == j;
         double re, im;
-        if (i == j)
-            self_term(h, a, k, re, im);
+        if (!P.fill_centred || !(kh <= 0.51))
+            element<0, kLarge>(P, ox, oy, oz, odx, ody, odz, sg, sn[col], tab, self, re, im);
+        else if (kh <= 0.126)
+            element<9, kLarge>(P, ox, oy, oz, odx, ody, odz, sg, sn[col], tab, self, re, im);
+        else if (kh <= 0.26)
+            element<11, kLarge>(P, ox, oy, oz, odx, ody, odz, sg, sn[col], tab, self, re, im);
         else
-            quad8(ox, oy, oz, sx, sy, sz, dx, dy, dz, h, a * a, k, re, im);
-        const double ca = odx * dx + ody * dy + odz * dz;
-        re *= ca;
-        im *= ca;
-        if (P.ground != 0) {
-            // image: z -> -z, direction (-dx, -dy, +dz) (matrix.rs:265-277)
-            double ir, ii;
-            quad8(ox, oy, oz, sx, sy, -sz, -dx, -dy, dz, h, a * a, k, ir, ii);
-            const double ica = odx * (-dx) + ody * (-dy) + odz * dz;
-            ir *= ica;
-            ii *= ica;
-            if (P.ground == 2) {
-                const double gr = P.gam_re * ir - P.gam_im * ii;
-                const double gi = P.gam_re * ii + P.gam_im * ir;
-                ir = gr;
-                ii = gi;
-            }
-            re += ir;
-            im += ii;
-        }
+            element<13, kLarge>(P, ox, oy, oz, odx, ody, odz, sg, sn[col], tab, self, re, im);
         const size_t o = (size_t)j * P.ldz + i;
         P.zr[o] = re;
         P.zi[o] = im;
@@ -207,10 +315,20 @@ __global__ __launch_bounds__(256) void k_fill_z(const Prob *__restrict__ probs,
     }
 }
 
-// The LD / TL / NT stamps of one problem, Z[row][col] += delta in the order
-// given (ZMatrix::add_to_diagonal / add_to_entry, matrix.rs:131-147; applied
-// after assembly, solve_session.rs:1285-1309): one thread walks the list, so
-// stamps on the same entry add in order. The mixed path's FP32 copy of an
+__global__ __launch_bounds__(256) void k_fill_z(const Prob *__restrict__ probs, const int4 *__restrict__ work)
+{
+    fill_z_body<false>(probs, work);
+}
+
+__global__ __launch_bounds__(256) void k_fill_z_large(const Prob *__restrict__ probs,
+                                                      const int4 *__restrict__ work)
+{
+    fill_z_body<true>(probs, work);
+}
+
+// The LD / TL / NT stamps of one problem, Z[row][col] += delta
+// (ZMatrix::add_to_diagonal / add_to_entry, matrix.rs:131-147; applied after
+// assembly, solve_session.rs:1285-1309). The mixed path's FP32 copy of an
 // entry is rounded again from the stamped FP64 value, as the fill rounds it.
 __global__ __launch_bounds__(256) void k_stamp_z(const Prob *__restrict__ probs,
                                                  const int *__restrict__ plist)

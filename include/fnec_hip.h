@@ -295,6 +295,11 @@ int fnec_hip_debug_lu_f32(const fnec_seg *segs, size_t n, const fnec_c64 *rhs, c
                           const size_t *j_a, const size_t *j_b, const double *j_sign, size_t n_junc,
                           double freq_hz, const fnec_ground *ground, float *lu_out, int32_t *ipiv_out);
 int fnec_hip_debug_mfma_probe(const double *a16x4, const double *b4x16, double *lanes_out);
+/* The FP64 Z of the mixed and FP64-MFMA paths' fill (k_fill_z), on
+ * fnec_hip_fill_z's inputs and layout; fnec_hip_fill_z itself returns the
+ * plain path's fill, the reference's Z bit for bit. */
+int fnec_hip_debug_fill_fast(const fnec_seg *segs, size_t n, double freq_hz, const fnec_ground *ground,
+                             fnec_c64 *z_out);
 /* The plain path (FNEC_PREC_FP64) of fnec_hip_solve_hallen on a caller-supplied
  * Z (n x n row-major, z[i*n+j] = A[i][j]) instead of the device fill: normal
  * equations, elimination, back substitution and residual. Parity hook: on the
@@ -361,6 +366,12 @@ int fnec_hip_debug_mfma_probe_f32(const float *a16x4, const float *b4x16, float 
  *                     factorisation; with a full batch queued, batches are
  *                     formed across classes. 0 .. 65536 (default 1024; 0: one
  *                     class).
+ *   "fill_centred"    mixed and FP64-MFMA paths' fill: 1 (default) takes each
+ *                     Gauss point's phase relative to the source centre's, a
+ *                     short polynomial without range reduction, and the full
+ *                     phase's sine and cosine once per element (segments with
+ *                     k h above 0.51 keep the per-point form); 0 reduces every
+ *                     point's phase.
  *   "exact_backsub"   plain path: 1 runs the back substitution in the
  *                     reference's order (ascending j in every row sum; a
  *                     serial chain of S^2/2 subtractions), so x equals the

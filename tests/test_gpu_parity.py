@@ -77,6 +77,52 @@ def test_fill_matches_oracle(oracle, name):
     assert elem_rel(zg, zo) < 1e-12
 
 
+# segments of k h = 0.6 (above the centred form's 0.51: every point's phase
+# reduced), 0.07, 0.16 and 0.4 (the centred form's three polynomial degrees)
+# in one fill, in free space and over PEC ground
+LONG_SEGMENTS = ("CE\nGW 1 5 0 0 -10 0 0 10 0.002\nGW 2 41 1 0 -10 1 0 10 0.001\nGW 3 3 3 0 -2 3 0 6 0.003\n"
+                 "GW 4 10 5 0 -5 5 0 5.55 0.002\nGE\nEX 0 1 3 0 1 0\nFR 0 1 0 0 14.2 0\nEN\n")
+LONG_SEGMENTS_PEC = ("CE\nGW 1 5 0 0 1 0 0 21 0.002\nGW 2 41 1 0 1 1 0 21 0.001\nGW 3 3 3 0 2 3 0 10 0.003\n"
+                     "GW 4 10 5 0 1 5 0 11.55 0.002\nGE 1\nGN 1\nEX 0 1 3 0 1 0\nFR 0 1 0 0 14.2 0\nEN\n")
+# two dipoles 400 km apart: phases k R near 1.2e5 (the fill's large-phase
+# form with the library sin / cos); such a phase carries an ulp of 1.5e-11
+# rad, which bounds the elementwise agreement of the far entries
+FAR_APART = ("CE\nGW 1 11 0 0 -5 0 0 5 0.001\nGW 2 11 4e5 0 -5 4e5 0 5 0.001\n"
+             "GE\nEX 0 1 6 0 1 0\nFR 0 1 0 0 14.2 0\nEN\n")
+
+
+@pytest.mark.parametrize("centred", [1, 0])
+@pytest.mark.parametrize("name", FILL_DECKS + ["config4a", "long-segments", "long-segments-pec", "far-apart"])
+def test_fast_fill_matches_oracle(oracle, name, centred):
+    """The mixed and FP64-MFMA paths' fill (k_fill_z) against the oracle's Z:
+    every entry within 1e-12 of its own size (1e-10 at phases near 1.2e5),
+    and within 1e-14 of the largest entry, in both phase forms (fill_centred
+    1: per-element phase with small per-point offsets; 0: every point's phase
+    reduced)."""
+    if name == "config4a":
+        txt, f = W.config4a_deck(), W.config4a_frequencies()[171]
+    elif name.startswith("long-segments"):
+        txt, f = (LONG_SEGMENTS_PEC if name.endswith("pec") else LONG_SEGMENTS), 14.2e6
+    elif name == "far-apart":
+        txt, f = FAR_APART, 14.2e6
+    else:
+        txt = deck_text(name)
+        f = F.Deck(txt).frequencies("worker")[0]
+    p = F.Deck(txt).problem("worker", f)
+    o = oracle.deck_inputs(txt, f, "worker")
+    kind, er, sg = o["ground"]
+    zo = oracle.fill_z(o["segs"], f, oracle.Ground(kind if kind != 3 else 0, er, sg))
+    F.set_tunable("fill_centred", centred)
+    try:
+        zg = F.fill_z(p["segs"], f, p["ground_struct"], fast=True)
+    finally:
+        F.set_tunable("fill_centred", 1)
+    e = elem_rel(zg, zo)
+    m = float(np.max(np.abs(zg - zo)) / np.max(np.abs(zo)))
+    print(f"{name} fill_centred {centred}: elementwise {e:.1e}, against the largest entry {m:.1e}")
+    assert e < (1e-10 if name == "far-apart" else 1e-12) and m < 1e-14
+
+
 @pytest.mark.parametrize("name", ["dipole-freesp-rp-51seg", "dipole-ground-51seg", "dipole-loaded",
                                   "yagi-5elm-51seg", "multi-source"])
 def test_normal_equations_match_oracle(oracle, name):

@@ -76,3 +76,26 @@ def test_sincos_table_is_the_mpmath_double_double():
         s, c = mp.sin(xi), mp.cos(xi)
         assert vals[4 * i] == float(s) and vals[4 * i + 1] == float(s - float(s))
         assert vals[4 * i + 2] == float(c) and vals[4 * i + 3] == float(c - float(c))
+
+
+def test_phase_table_is_the_nearest_double():
+    """phase_table.h (sincos_tab of the fast fill): sin and cos of j pi/32 as
+    the nearest doubles, against 300-bit mpmath, with exact zeros and ones at
+    the quarter turns and the symmetries sin(j) = cos(16 - j) and
+    sin(j + 32) = -sin(j) holding exactly."""
+    mp = pytest.importorskip("mpmath")
+    mp.mp.prec = 300
+    text = open(os.path.join(ROOT, "fnec-rust_amd", "csrc", "phase_table.h")).read()
+    body = text[text.index("FNEC_PHASE_TABLE_INIT"):].split("\n")[1:]
+    vals = [float.fromhex(v.strip(" {}")) if v.strip(" {}") != "0.0" else 0.0
+            for line in body for v in line.replace("\\", "").split(",") if v.strip(" {}")]
+    assert len(vals) == 128
+    for j in range(64):
+        s, c = vals[2 * j], vals[2 * j + 1]
+        x = mp.pi * j / 32
+        assert abs(s - mp.sin(x)) <= abs(np.spacing(s)) / 2 + mp.mpf(2) ** -280
+        assert abs(c - mp.cos(x)) <= abs(np.spacing(c)) / 2 + mp.mpf(2) ** -280
+        assert s == -vals[2 * ((j + 32) % 64)] and c == -vals[2 * ((j + 32) % 64) + 1]
+        if j <= 16:
+            assert s == vals[2 * (16 - j) + 1]
+    assert vals[0:2] == [0.0, 1.0] and vals[32:34] == [1.0, 0.0]
