@@ -96,6 +96,17 @@ int or_build_hallen_rhs_paths(const or_deck *d, const or_seg *s, int n, double f
                               const int *path_of, const double *sign_of, const double *s_of,
                               cx *rhs, double *cosv);
 
+/* ---- plane-wave and current-source inputs (planewave.rs, excitation.rs:350-447) ---- */
+int or_planewave_hallen(const or_deck *d, const or_seg *s, int n, double freq_hz, cx *rhs, double *cosv,
+                        double *sinv);
+int or_planewave_hallen_paths(const or_deck *d, const or_seg *s, int n, double freq_hz, const int *path_of,
+                              const double *sign_of, const double *s_of, cx *rhs, double *cosv, double *sinv);
+int or_current_source_shape(const or_deck *d, const or_seg *s, int n, double freq_hz, uint32_t tag,
+                            uint32_t seg, cx *shape, double *cosv, int *src);
+int or_current_source_shape_paths(const or_deck *d, const or_seg *s, int n, double freq_hz, uint32_t tag,
+                                  uint32_t seg, const int *path_of, const double *sign_of, const double *s_of,
+                                  cx *shape, double *cosv, int *src);
+
 /* ---- matrix (matrix.rs:186-218, 297-331) ---- */
 void or_fill_z(const or_seg *s, int n, double freq_hz, or_ground g, cx *z /* n*n row-major */);
 cx or_fill_elem(const or_seg *obs, const or_seg *src, double k, int is_self);
@@ -127,6 +138,18 @@ int or_solve_hallen(const cx *z, int n, const cx *rhs, const double *cosv, const
                     cx *x /* n + max(nw,1) */);
 int or_solve_hallen_paths(const cx *z, int n, const cx *rhs, const double *cosv,
                           const int *path_of, const int *free_ends, int nfree, cx *x);
+/* linear.rs:910-1319: x holds the currents then the homogeneous constants
+ * (plane wave: cos, sin per wire or path) and, for a current source, V last. */
+int or_solve_hallen_planewave(const cx *z, int n, const cx *rhs, const double *cosv, const double *sinv,
+                              const int *wf, const int *wl, int nw, cx *x /* n + 2 max(nw,1) */);
+int or_solve_hallen_planewave_paths(const cx *z, int n, const cx *rhs, const double *cosv,
+                                    const double *sinv, const int *path_of, const int *free_ends,
+                                    int nfree, cx *x /* n + 2 P */);
+int or_solve_hallen_current_source(const cx *z, int n, const cx *shape, const double *cosv, int src,
+                                   cx i0, const int *wf, const int *wl, int nw, cx *x /* n + w + 1 */);
+int or_solve_hallen_current_source_paths(const cx *z, int n, const cx *shape, const double *cosv,
+                                         int src, cx i0, const int *path_of, const int *free_ends,
+                                         int nfree, cx *x /* n + P + 1 */);
 /* The augmented system pieces, for kernel-level parity tests. Returns S. */
 int or_hallen_normal_eqs(const cx *z, int n, const double *cosv, const cx *rhs, const int *wf,
                          const int *wl, int nw, const int *ja, const int *jb, const double *js,
@@ -176,11 +199,19 @@ typedef struct {
     double rp_total;
     double abs_res, rel_res, diag_spread;
     char *report;  /* report contract v1 text (CLI semantics) */
+    /* CLI: 0 delta gap, 1 plane wave (receive), 2 current source */
+    int variant;
+    cx port_v;     /* current source: the solved port voltage V */
+    int nrecv;     /* plane-wave receive sweep (NTHETA NPHI > 1) */
+    double *recv_theta, *recv_phi, *recv_db;
 } or_result;
 
 int or_run_deck(const char *text, int semantics, int freq_index, double freq_override_hz,
                 int with_report, or_result *res);
 void or_result_free(or_result *res);
+int or_deck_variant_inputs(const char *text, double freq_hz, int *n_out, int *route, cx *rhs, double *cosv,
+                           double *sinv, int *src, cx *i0, int *wf, int *wl, int *nw, int *path_of, int *free_ends,
+                           int *nfree);
 
 /* Report contract v1 number formats (nec_report/src/lib.rs:437-482). */
 void or_fmt_fixed(char *buf, int cap, double v, int prec);

@@ -260,3 +260,156 @@ int or_solve_hallen_paths(const cx *z, int n, const cx *rhs, const double *cosv,
     rows_free(ata, cols);
     return st;
 }
+
+/* Shared tail of the variant solvers below: the regularised normal equations
+ * of the dense M, y (linear.rs:964-984 and its copies) and the elimination. */
+static int solve_dense(cx **m, cx *y, int rows, int cols, cx *x)
+{
+    cx **ata = normal_eqs(m, y, rows, cols);
+    int st;
+    rows_free(m, rows);
+    free(y);
+    st = solve_aug(ata, cols, x);
+    rows_free(ata, cols);
+    return st;
+}
+
+/* solve_hallen_planewave (linear.rs:910-985): two homogeneous columns (cos,
+ * sin) per wire, I = 0 at both ends of each wire. x: n + 2 w (w = max(nw, 1)). */
+int or_solve_hallen_planewave(const cx *z, int n, const cx *rhs, const double *cosv, const double *sinv,
+                              const int *wf, const int *wl, int nw, cx *x)
+{
+    int fb0 = 0, lb0 = n - 1, w, rows, cols, r, c, wi;
+    const int *f = wf, *l = wl;
+    int *row_wire;
+    cx **m, *y;
+    if (nw == 0 || n == 0) {
+        f = &fb0;
+        l = &lb0;
+        nw = n > 0 ? 1 : 0;
+    }
+    w = nw;
+    rows = n + 2 * w;
+    cols = n + 2 * w;
+    m = rows_alloc(rows, cols);
+    y = (cx *)calloc((size_t)(rows > 0 ? rows : 1), sizeof(cx));
+    row_wire = (int *)calloc((size_t)(n > 0 ? n : 1), sizeof(int));
+    for (wi = 0; wi < w; wi++)
+        for (r = f[wi]; r <= l[wi]; r++)
+            row_wire[r] = wi;
+    for (r = 0; r < n; r++) {
+        for (c = 0; c < n; c++)
+            m[r][c] = z[(size_t)r * (size_t)n + (size_t)c];
+        m[r][n + 2 * row_wire[r]] = cx_make(-cosv[r], 0.0);
+        m[r][n + 2 * row_wire[r] + 1] = cx_make(-sinv[r], 0.0);
+        y[r] = rhs[r];
+    }
+    for (wi = 0; wi < w; wi++) {
+        m[n + 2 * wi][f[wi]] = cx_make(1.0, 0.0);
+        m[n + 2 * wi + 1][l[wi]] = cx_make(1.0, 0.0);
+    }
+    free(row_wire);
+    return solve_dense(m, y, rows, cols, x);
+}
+
+/* solve_hallen_planewave_paths (linear.rs:1017-1091): two homogeneous columns
+ * per conductor path, I = 0 at the listed free ends. x: n + 2 P. */
+int or_solve_hallen_planewave_paths(const cx *z, int n, const cx *rhs, const double *cosv,
+                                    const double *sinv, const int *path_of, const int *free_ends,
+                                    int nfree, cx *x)
+{
+    int np = 0, i, r, c, rows, cols;
+    cx **m, *y;
+    for (i = 0; i < n; i++)
+        if (path_of[i] + 1 > np)
+            np = path_of[i] + 1;
+    cols = n + 2 * np;
+    rows = n + nfree;
+    m = rows_alloc(rows, cols);
+    y = (cx *)calloc((size_t)(rows > 0 ? rows : 1), sizeof(cx));
+    for (r = 0; r < n; r++) {
+        for (c = 0; c < n; c++)
+            m[r][c] = z[(size_t)r * (size_t)n + (size_t)c];
+        m[r][n + 2 * path_of[r]] = cx_make(-cosv[r], 0.0);
+        m[r][n + 2 * path_of[r] + 1] = cx_make(-sinv[r], 0.0);
+        y[r] = rhs[r];
+    }
+    for (i = 0; i < nfree; i++)
+        m[n + i][free_ends[i]] = cx_make(1.0, 0.0);
+    return solve_dense(m, y, rows, cols, x);
+}
+
+/* solve_hallen_current_source (linear.rs:1124-1210): one homogeneous column
+ * per wire plus the port voltage V (column n + w), I = 0 at both ends of each
+ * wire and I[src] = i0, those rows weighted by 1e6. x: n + w + 1. */
+int or_solve_hallen_current_source(const cx *z, int n, const cx *shape, const double *cosv, int src,
+                                   cx i0, const int *wf, const int *wl, int nw, cx *x)
+{
+    int fb0 = 0, lb0 = n - 1, w, rows, cols, vcol, r, c, wi, crow;
+    const int *f = wf, *l = wl;
+    int *row_wire;
+    const cx cw = cx_make(1.0e6, 0.0);
+    cx **m, *y;
+    if (nw == 0 || n == 0) {
+        f = &fb0;
+        l = &lb0;
+        nw = n > 0 ? 1 : 0;
+    }
+    w = nw;
+    vcol = n + w;
+    cols = n + w + 1;
+    rows = n + 2 * w + 1;
+    m = rows_alloc(rows, cols);
+    y = (cx *)calloc((size_t)rows, sizeof(cx));
+    row_wire = (int *)calloc((size_t)(n > 0 ? n : 1), sizeof(int));
+    for (wi = 0; wi < w; wi++)
+        for (r = f[wi]; r <= l[wi]; r++)
+            row_wire[r] = wi;
+    for (r = 0; r < n; r++) {
+        for (c = 0; c < n; c++)
+            m[r][c] = z[(size_t)r * (size_t)n + (size_t)c];
+        m[r][n + row_wire[r]] = cx_make(-cosv[r], 0.0);
+        m[r][vcol] = cx_neg(shape[r]);
+    }
+    crow = n;
+    for (wi = 0; wi < w; wi++) {
+        m[crow++][f[wi]] = cw;
+        m[crow++][l[wi]] = cw;
+    }
+    m[crow][src] = cw;
+    y[crow] = cx_mul(cw, i0);
+    free(row_wire);
+    return solve_dense(m, y, rows, cols, x);
+}
+
+/* solve_hallen_current_source_paths (linear.rs:1241-1319): one homogeneous
+ * column per path plus V (column n + P), weighted I = 0 rows at the free ends
+ * and the weighted source row. x: n + P + 1. */
+int or_solve_hallen_current_source_paths(const cx *z, int n, const cx *shape, const double *cosv,
+                                         int src, cx i0, const int *path_of, const int *free_ends,
+                                         int nfree, cx *x)
+{
+    int np = 0, i, r, c, rows, cols, vcol, crow;
+    const cx cw = cx_make(1.0e6, 0.0);
+    cx **m, *y;
+    for (i = 0; i < n; i++)
+        if (path_of[i] + 1 > np)
+            np = path_of[i] + 1;
+    vcol = n + np;
+    cols = n + np + 1;
+    rows = n + nfree + 1;
+    m = rows_alloc(rows, cols);
+    y = (cx *)calloc((size_t)rows, sizeof(cx));
+    for (r = 0; r < n; r++) {
+        for (c = 0; c < n; c++)
+            m[r][c] = z[(size_t)r * (size_t)n + (size_t)c];
+        m[r][n + path_of[r]] = cx_make(-cosv[r], 0.0);
+        m[r][vcol] = cx_neg(shape[r]);
+    }
+    crow = n;
+    for (i = 0; i < nfree; i++)
+        m[crow++][free_ends[i]] = cw;
+    m[crow][src] = cw;
+    y[crow] = cx_mul(cw, i0);
+    return solve_dense(m, y, rows, cols, x);
+}

@@ -71,6 +71,12 @@ class Result(C.Structure):
         ("rel_res", C.c_double),
         ("diag_spread", C.c_double),
         ("report", C.c_char_p),
+        ("variant", C.c_int),
+        ("port_v", CX),
+        ("nrecv", C.c_int),
+        ("recv_theta", C.POINTER(C.c_double)),
+        ("recv_phi", C.POINTER(C.c_double)),
+        ("recv_db", C.POINTER(C.c_double)),
     ]
 
 
@@ -167,6 +173,11 @@ def run_deck(text: str, semantics: str = "cli", freq_index: int = 0, freq_hz: fl
             out["rp_theta"] = np.array(r.rp_theta[:npt])
             out["rp_phi"] = np.array(r.rp_phi[:npt])
             out["rp"] = np.array(r.rp[: 4 * npt]).reshape(npt, 4)
+        out["variant"] = ("delta-gap", "plane-wave", "current-source")[r.variant]
+        out["port_v"] = complex(r.port_v.re, r.port_v.im)
+        if r.nrecv:
+            k = r.nrecv
+            out["recv"] = np.stack([np.array(r.recv_theta[:k]), np.array(r.recv_phi[:k]), np.array(r.recv_db[:k])], 1)
         if report and r.report:
             out["report"] = r.report.decode()
         return out
@@ -402,6 +413,32 @@ def deck_inputs(text: str, freq_hz: float, semantics: str = "worker") -> dict:
         return out
     finally:
         L.or_deck_free(d)
+
+
+def variant_inputs(text: str, freq_hz: float) -> dict | None:
+    """The CLI's plane-wave or current-source inputs for a deck
+    (or_deck_variant_inputs): kind, route, rhs (the forcing or the unit
+    source shape), cos, sin, src, i0, and the wire ranges or the paths."""
+    L = lib()
+    n_max = 1 << 16
+    n, route, src, nw, nfree = (C.c_int() for _ in range(5))
+    i0 = CX()
+    rhs = np.zeros(n_max, np.complex128)
+    cosv, sinv = np.zeros(n_max), np.zeros(n_max)
+    wf, wl, po, fe = (np.zeros(n_max + 1, np.int32) for _ in range(4))
+    L.or_deck_variant_inputs.argtypes = [C.c_char_p, C.c_double] + [C.c_void_p] * 13
+    k = L.or_deck_variant_inputs(text.encode(), float(freq_hz), C.byref(n), C.byref(route), _p(rhs), _p(cosv),
+                                 _p(sinv), C.byref(src), C.byref(i0), _p(wf), _p(wl), C.byref(nw), _p(po), _p(fe),
+                                 C.byref(nfree))
+    if k < 0:
+        raise RuntimeError(f"oracle status {-k}")
+    if k == 0:
+        return None
+    m = n.value
+    return {"kind": ("plane-wave", "current-source")[k - 1], "route": route.value, "n": m,
+            "rhs": rhs[:m].copy(), "cos": cosv[:m].copy(), "sin": sinv[:m].copy(), "src": src.value,
+            "i0": complex(i0.re, i0.im), "wf": wf[:nw.value].copy(), "wl": wl[:nw.value].copy(),
+            "path_of": po[:m].copy(), "free_ends": fe[:nfree.value].copy()}
 
 
 def stamps(text: str, freq_hz: float, semantics: str = "worker") -> list:

@@ -240,6 +240,17 @@ static void report_v1(const or_deck *d, or_result *res, const char *mode, const 
             sb_add(&b, t);
         }
     }
+    if (res->nrecv > 0) {
+        snprintf(t, sizeof(t), "\nRECEIVE_PATTERN\nN_POINTS %d\nTHETA PHI RESPONSE_DB\n", res->nrecv);
+        sb_add(&b, t);
+        for (i = 0; i < res->nrecv; i++) {
+            or_fmt_fixed(f1, 64, res->recv_theta[i], 4);
+            or_fmt_fixed(f2, 64, res->recv_phi[i], 4);
+            or_fmt_fixed(f3, 64, res->recv_db[i], 4);
+            snprintf(t, sizeof(t), "%s %s %s\n", f1, f2, f3);
+            sb_add(&b, t);
+        }
+    }
     if (normalize && res->np > 0) {
         double peak = -INFINITY;
         for (i = 0; i < res->np; i++)
@@ -270,6 +281,182 @@ static void fail(or_result *res, int st, const char *msg)
     snprintf(res->err, sizeof(res->err), "%s", msg);
 }
 
+/* solve_plane_wave_hallen (solve_session.rs:262-301): conductor paths when one
+ * is not trivial, else the per-wire solve on the raw GW wires. x gets the
+ * currents and the homogeneous constants; *s the count. */
+static int plane_wave_solve(const or_deck *d, const or_seg *segs, int n, double freq, const cx *z, cx **x,
+                            int *s_out, int *nhom)
+{
+    int *path_of = (int *)malloc(sizeof(int) * (size_t)n), *fa = (int *)malloc(sizeof(int) * (size_t)(n + 1));
+    int *fbv = (int *)malloc(sizeof(int) * (size_t)(n + 1)), *wf = (int *)malloc(sizeof(int) * (size_t)(n + 1));
+    int *wl = (int *)malloc(sizeof(int) * (size_t)(n + 1));
+    double *sign_of = (double *)malloc(sizeof(double) * (size_t)n), *s_of = (double *)malloc(sizeof(double) * (size_t)n);
+    double *cosv = (double *)malloc(sizeof(double) * (size_t)n), *sinv = (double *)malloc(sizeof(double) * (size_t)n);
+    cx *rhs = (cx *)malloc(sizeof(cx) * (size_t)n);
+    int nontriv = 0, np, st, i;
+    np = or_conductor_paths(segs, n, path_of, sign_of, s_of, fa, fbv, n + 1, &nontriv);
+    if (np > 0 && nontriv) {
+        int *fe = (int *)malloc(sizeof(int) * (size_t)(2 * np));
+        for (i = 0; i < np; i++) {
+            fe[2 * i] = fa[i];
+            fe[2 * i + 1] = fbv[i];
+        }
+        st = or_planewave_hallen_paths(d, segs, n, freq, path_of, sign_of, s_of, rhs, cosv, sinv);
+        *nhom = 2 * np;
+        *x = (cx *)calloc((size_t)(n + *nhom), sizeof(cx));
+        if (!st)
+            st = or_solve_hallen_planewave_paths(z, n, rhs, cosv, sinv, path_of, fe, 2 * np, *x);
+        free(fe);
+    } else {
+        const int nw = or_wire_endpoints(segs, n, wf, wl, n + 1);
+        st = or_planewave_hallen(d, segs, n, freq, rhs, cosv, sinv);
+        *nhom = 2 * (nw > 0 ? nw : 1);
+        *x = (cx *)calloc((size_t)(n + *nhom), sizeof(cx));
+        if (!st)
+            st = or_solve_hallen_planewave(z, n, rhs, cosv, sinv, wf, wl, nw, *x);
+    }
+    *s_out = n + *nhom;
+    free(path_of);
+    free(fa);
+    free(fbv);
+    free(wf);
+    free(wl);
+    free(sign_of);
+    free(s_of);
+    free(cosv);
+    free(sinv);
+    free(rhs);
+    return st;
+}
+
+/* solve_current_source_hallen (solve_session.rs:392-448): the first EX type 4
+ * card forces i0 = F1 + j F2 on its segment. */
+static int current_source_solve(const or_deck *d, const or_seg *segs, int n, double freq, const cx *z, cx **x,
+                                int *s_out, int *nhom, cx *port_v)
+{
+    const or_card *cs = NULL;
+    int *path_of = (int *)malloc(sizeof(int) * (size_t)n), *fa = (int *)malloc(sizeof(int) * (size_t)(n + 1));
+    int *fbv = (int *)malloc(sizeof(int) * (size_t)(n + 1)), *wf = (int *)malloc(sizeof(int) * (size_t)(n + 1));
+    int *wl = (int *)malloc(sizeof(int) * (size_t)(n + 1));
+    double *sign_of = (double *)malloc(sizeof(double) * (size_t)n), *s_of = (double *)malloc(sizeof(double) * (size_t)n);
+    double *cosv = (double *)malloc(sizeof(double) * (size_t)n);
+    cx *shape = (cx *)malloc(sizeof(cx) * (size_t)n), i0;
+    int nontriv = 0, np, st = OR_OK, i, src = -1, nw;
+    for (i = 0; i < d->n && !cs; i++)
+        if (!strcmp(d->cards[i].mn, "EX") && ex_is_current(&d->cards[i]))
+            cs = &d->cards[i];
+    i0 = cx_make(ex_vr(cs), ex_vi(cs));
+    np = or_conductor_paths(segs, n, path_of, sign_of, s_of, fa, fbv, n + 1, &nontriv);
+    nw = or_wire_endpoints(segs, n, wf, wl, n + 1);
+    *x = NULL;
+    if (np > 0 && nontriv) {
+        int *fe = (int *)malloc(sizeof(int) * (size_t)(2 * np));
+        for (i = 0; i < np; i++) {
+            fe[2 * i] = fa[i];
+            fe[2 * i + 1] = fbv[i];
+        }
+        st = or_current_source_shape_paths(d, segs, n, freq, ex_tag(cs), ex_seg(cs), path_of, sign_of, s_of, shape,
+                                           cosv, &src);
+        *nhom = np;
+        *x = (cx *)calloc((size_t)(n + np + 1), sizeof(cx));
+        if (!st)
+            st = or_solve_hallen_current_source_paths(z, n, shape, cosv, src, i0, path_of, fe, 2 * np, *x);
+        free(fe);
+    } else {
+        if (np <= 0) {
+            int *ja = (int *)malloc(sizeof(int) * (size_t)(4 * n + 4)), *jb = (int *)malloc(sizeof(int) * (size_t)(4 * n + 4));
+            double *js = (double *)malloc(sizeof(double) * (size_t)(4 * n + 4));
+            if (or_detect_junctions(segs, wf, wl, nw, 1e-6, ja, jb, js, 4 * n + 4) > 0)
+                st = OR_UNSUPPORTED; /* degree-3+ junctions, closed loops */
+            free(ja);
+            free(jb);
+            free(js);
+        }
+        *nhom = nw > 0 ? nw : 1;
+        *x = (cx *)calloc((size_t)(n + *nhom + 1), sizeof(cx));
+        if (!st)
+            st = or_current_source_shape(d, segs, n, freq, ex_tag(cs), ex_seg(cs), shape, cosv, &src);
+        if (!st)
+            st = or_solve_hallen_current_source(z, n, shape, cosv, src, i0, wf, wl, nw, *x);
+    }
+    *s_out = n + *nhom + 1;
+    *port_v = *x ? (*x)[*s_out - 1] : cx_make(0.0, 0.0);
+    free(path_of);
+    free(fa);
+    free(fbv);
+    free(wf);
+    free(wl);
+    free(sign_of);
+    free(s_of);
+    free(cosv);
+    free(shape);
+    return st;
+}
+
+/* plane_wave_receive_sweep (solve_session.rs:313-371): with NTHETA = max(I2,
+ * 1), NPHI = max(I3, 1) and NTHETA NPHI > 1, each incidence (theta0 + it
+ * dtheta, phi0 + ip dphi; it outer) is solved as a single-incidence deck and
+ * its peak |I| recorded; the response is 20 log10(peak / max peak). */
+static int receive_sweep(const or_deck *d, const or_seg *segs, int n, double freq, const cx *z, or_result *res)
+{
+    const or_card *ex = NULL;
+    or_deck dd;
+    uint32_t nth, nph, it, ip;
+    int i, st = OR_OK, cnt = 0;
+    double *peak, max_peak = 0.0;
+    for (i = 0; i < d->n && !ex; i++)
+        if (!strcmp(d->cards[i].mn, "EX") && ex_is_plane_wave(&d->cards[i]))
+            ex = &d->cards[i];
+    nth = ex_tag(ex) > 0 ? ex_tag(ex) : 1;
+    nph = ex_seg(ex) > 0 ? ex_seg(ex) : 1;
+    if ((uint64_t)nth * nph <= 1)
+        return OR_OK;
+    res->recv_theta = (double *)malloc(sizeof(double) * (size_t)nth * nph);
+    res->recv_phi = (double *)malloc(sizeof(double) * (size_t)nth * nph);
+    res->recv_db = (double *)malloc(sizeof(double) * (size_t)nth * nph);
+    peak = (double *)malloc(sizeof(double) * (size_t)nth * nph);
+    dd.n = d->n;
+    dd.cap = d->n;
+    dd.cards = (or_card *)malloc(sizeof(or_card) * (size_t)(d->n > 0 ? d->n : 1));
+    for (it = 0; it < nth && !st; it++)
+        for (ip = 0; ip < nph && !st; ip++) {
+            const double theta = (ex->nf > 4 ? ex->f[4] : 0.0) + (double)it * (ex->nf > 7 ? ex->f[7] : 0.0);
+            const double phi = (ex->nf > 5 ? ex->f[5] : 0.0) + (double)ip * (ex->nf > 8 ? ex->f[8] : 0.0);
+            cx *x = NULL;
+            int s, nhom, m;
+            double pk = 0.0;
+            memcpy(dd.cards, d->cards, sizeof(or_card) * (size_t)d->n);
+            for (i = 0; i < dd.n; i++) {
+                or_card *c = &dd.cards[i];
+                if (strcmp(c->mn, "EX") || !ex_is_plane_wave(c))
+                    continue;
+                while (c->nf < 9)
+                    c->f[c->nf++] = 0.0;
+                c->f[1] = 1.0;
+                c->f[2] = 1.0;
+                c->f[4] = theta;
+                c->f[5] = phi;
+                c->f[7] = 0.0;
+                c->f[8] = 0.0;
+            }
+            st = plane_wave_solve(&dd, segs, n, freq, z, &x, &s, &nhom);
+            for (m = 0; m < n && !st; m++)
+                pk = fmax(pk, cx_norm(x[m]));
+            free(x);
+            res->recv_theta[cnt] = theta;
+            res->recv_phi[cnt] = phi;
+            peak[cnt++] = pk;
+        }
+    free(dd.cards);
+    for (i = 0; i < cnt; i++)
+        max_peak = fmax(max_peak, peak[i]);
+    for (i = 0; i < cnt; i++)
+        res->recv_db[i] = peak[i] > 0.0 && max_peak > 0.0 ? 20.0 * log10(peak[i] / max_peak) : -999.99;
+    res->nrecv = st ? 0 : cnt;
+    free(peak);
+    return st;
+}
+
 int or_run_deck(const char *text, int semantics, int freq_index, double freq_override_hz,
                 int with_report, or_result *res)
 {
@@ -290,9 +477,17 @@ int or_run_deck(const char *text, int semantics, int freq_index, double freq_ove
     }
     for (i = 0; i < d->n; i++)
         if (!strcmp(d->cards[i].mn, "EX") && (ex_is_plane_wave(&d->cards[i]) || ex_is_current(&d->cards[i]))) {
-            fail(res, OR_UNSUPPORTED, "plane-wave / current-source excitation");
-            or_deck_free(d);
-            return res->status;
+            /* the CLI solves them on their own Hallén systems; the worker and
+             * the Python binding do not */
+            if (semantics != OR_SEM_CLI) {
+                fail(res, OR_UNSUPPORTED, "plane-wave / current-source excitation");
+                or_deck_free(d);
+                return res->status;
+            }
+            if (ex_is_plane_wave(&d->cards[i]))
+                res->variant = 1;
+            else if (res->variant == 0)
+                res->variant = 2;
         }
     if (freq_override_hz > 0.0)
         res->freq_hz = freq_override_hz;
@@ -344,7 +539,21 @@ int or_run_deck(const char *text, int semantics, int freq_index, double freq_ove
     }
     res->diag_spread = or_diag_spread(z, n);
 
-    if (semantics == OR_SEM_CLI) {
+    if (res->variant) {
+        /* plane wave first, then current source (solve_session.rs:1327-1337) */
+        int nhom = 0;
+        res->status = res->variant == 1 ? plane_wave_solve(d, res->segs, n, res->freq_hz, z, &res->x, &res->s, &nhom)
+                                        : current_source_solve(d, res->segs, n, res->freq_hz, z, &res->x, &res->s,
+                                                               &nhom, &res->port_v);
+        res->nw = nhom;
+        if (!res->status && res->variant == 1)
+            res->status = receive_sweep(d, res->segs, n, res->freq_hz, z, res);
+        if (res->status) {
+            fail(res, res->status, res->status == OR_SINGULAR ? "impedance matrix is singular"
+                                                               : "EX: plane-wave / current-source setup failed");
+            goto out;
+        }
+    } else if (semantics == OR_SEM_CLI) {
         int *path_of = (int *)malloc(sizeof(int) * (size_t)n);
         double *sign_of = (double *)malloc(sizeof(double) * (size_t)n);
         double *s_of = (double *)malloc(sizeof(double) * (size_t)n);
@@ -461,6 +670,8 @@ int or_run_deck(const char *text, int semantics, int freq_index, double freq_ove
         cur = res->x[idx];
         if (semantics == OR_SEM_WORKER)
             vs = cx_make(ex_vr(c), ex_vi(c));
+        else if (res->variant == 2 && ex_is_current(c))
+            vs = res->port_v; /* the solved port voltage (solve_session.rs:893-897) */
         else
             vs = cx_scale(v[idx], res->segs[idx].length);
         zin = cx_norm(cur) > 1e-60 ? cx_div(vs, cur) : vs;
@@ -525,7 +736,9 @@ int or_run_deck(const char *text, int semantics, int freq_index, double freq_ove
             }
         }
         if (with_report)
-            report_v1(d, res, res->used_paths ? "hallen" : "hallen", have_avg ? &avg : NULL, normalize);
+            report_v1(d, res,
+                      res->variant == 1 ? "hallen-planewave" : res->variant == 2 ? "hallen-current-source" : "hallen",
+                      have_avg ? &avg : NULL, normalize);
     }
 out:
     free(z);
@@ -555,6 +768,9 @@ void or_result_free(or_result *res)
     free(res->rp_phi);
     free(res->rp);
     free(res->report);
+    free(res->recv_theta);
+    free(res->recv_phi);
+    free(res->recv_db);
     memset(res, 0, sizeof(*res));
 }
 
@@ -588,4 +804,73 @@ int or_deck_stamps(const char *text, int semantics, double freq_hz, int *rows, i
     free(segs);
     or_deck_free(d);
     return ns;
+}
+
+/* The inputs the CLI builds for a plane-wave or current-source deck at
+ * freq_hz (after the stamps): returns 1 (plane wave) or 2 (current source), 0
+ * for a delta-gap deck, or a negative status. route 1 fills path_of and
+ * free_ends (*nfree), route 0 the raw wire ranges wf / wl (*nw). Arrays hold
+ * n entries (n + 1 for wf, wl, free_ends). */
+int or_deck_variant_inputs(const char *text, double freq_hz, int *n_out, int *route, cx *rhs, double *cosv,
+                           double *sinv, int *src, cx *i0, int *wf, int *wl, int *nw, int *path_of, int *free_ends,
+                           int *nfree)
+{
+    char err[256];
+    or_deck *d = or_deck_parse(text, err, sizeof(err));
+    or_seg *segs;
+    int n, i, variant = 0, np, nontriv = 0, st;
+    int *fa, *fbv;
+    double *sign_of, *s_of;
+    const or_card *cs = NULL;
+    if (!d)
+        return -OR_PARSE;
+    for (i = 0; i < d->n; i++)
+        if (!strcmp(d->cards[i].mn, "EX")) {
+            if (ex_is_plane_wave(&d->cards[i]))
+                variant = 1;
+            else if (ex_is_current(&d->cards[i])) {
+                if (!variant)
+                    variant = 2;
+                if (!cs)
+                    cs = &d->cards[i];
+            }
+        }
+    segs = (or_seg *)malloc(sizeof(or_seg) * (size_t)(1 << 20));
+    n = or_build_geometry(d, segs, 1 << 20, err, sizeof(err));
+    if (n < 0 || !variant) {
+        free(segs);
+        or_deck_free(d);
+        return n < 0 ? n : 0;
+    }
+    *n_out = n;
+    fa = (int *)malloc(sizeof(int) * (size_t)(n + 1));
+    fbv = (int *)malloc(sizeof(int) * (size_t)(n + 1));
+    sign_of = (double *)malloc(sizeof(double) * (size_t)n);
+    s_of = (double *)malloc(sizeof(double) * (size_t)n);
+    np = or_conductor_paths(segs, n, path_of, sign_of, s_of, fa, fbv, n + 1, &nontriv);
+    *nw = or_wire_endpoints(segs, n, wf, wl, n + 1);
+    *route = np > 0 && nontriv;
+    *nfree = 0;
+    if (*route)
+        for (i = 0; i < np; i++) {
+            free_ends[2 * i] = fa[i];
+            free_ends[2 * i + 1] = fbv[i];
+            *nfree += 2;
+        }
+    if (variant == 1)
+        st = *route ? or_planewave_hallen_paths(d, segs, n, freq_hz, path_of, sign_of, s_of, rhs, cosv, sinv)
+                    : or_planewave_hallen(d, segs, n, freq_hz, rhs, cosv, sinv);
+    else {
+        *i0 = cx_make(ex_vr(cs), ex_vi(cs));
+        st = *route ? or_current_source_shape_paths(d, segs, n, freq_hz, ex_tag(cs), ex_seg(cs), path_of, sign_of,
+                                                    s_of, rhs, cosv, src)
+                    : or_current_source_shape(d, segs, n, freq_hz, ex_tag(cs), ex_seg(cs), rhs, cosv, src);
+    }
+    free(fa);
+    free(fbv);
+    free(sign_of);
+    free(s_of);
+    free(segs);
+    or_deck_free(d);
+    return st ? -st : variant;
 }
