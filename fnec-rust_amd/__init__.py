@@ -34,7 +34,8 @@ EXPORTS = [
     "fnec_hip_request_footprint", "fnec_hip_debug_normal_eqs", "fnec_hip_debug_lu_f32", "fnec_hip_debug_mfma_probe",
     "fnec_hip_debug_mfma_probe_f32", "fnec_hip_set_tunable", "fnec_hip_get_tunable",
     "fnec_hip_debug_solve_z", "fnec_hip_debug_solve_square", "fnec_hip_debug_refmath",
-    "fnec_hip_rp_noalloc", "fnec_hip_debug_fill_fast",
+    "fnec_hip_rp_noalloc", "fnec_hip_debug_fill_fast", "fnec_hip_solve_hallen_planewave",
+    "fnec_hip_solve_hallen_current_source",
 ]
 
 
@@ -118,6 +119,11 @@ def lib():
                                             C.POINTER(Ground), i32, vp, C.POINTER(SolveStats)]
         L.fnec_hip_solve_hallen_paths.argtypes = [vp, sz, vp, vp, vp, vp, sz, dbl, C.POINTER(Ground),
                                                   i32, vp, C.POINTER(SolveStats)]
+        L.fnec_hip_solve_hallen_planewave.argtypes = [vp, sz, vp, sz, vp, vp, vp, vp, sz, vp, vp, sz, vp, sz,
+                                                      dbl, C.POINTER(Ground), i32, vp, C.POINTER(SolveStats)]
+        L.fnec_hip_solve_hallen_current_source.argtypes = [vp, sz, vp, vp, sz, C64, vp, vp, sz, vp, vp, sz, vp,
+                                                           sz, dbl, C.POINTER(Ground), i32, vp,
+                                                           C.POINTER(SolveStats)]
         L.fnec_hip_rp.argtypes = [vp, vp, sz, dbl, C.POINTER(Ground), vp, vp, sz, vp, C.POINTER(dbl)]
         L.fnec_hip_rp_noalloc.argtypes = [vp, vp, sz, dbl, C.POINTER(Ground), vp, vp, sz, vp, C.POINTER(dbl)]
         L.fnec_hip_solve_batch.argtypes = [vp, sz, i32, vp]
@@ -225,6 +231,57 @@ def solve_hallen_paths(segs, rhs, cosv, path_of, free_ends, freq_hz, g: Ground |
                                              _p(np.ascontiguousarray(cosv, np.float64)), _p(po), _p(fe),
                                              len(fe), float(freq_hz), C.byref(g or ground()),
                                              int(precision), _p(x), C.byref(st)))
+    return (x, st.as_dict()) if with_stats else x
+
+
+def _stamp_arr(stamps):
+    if stamps is None or len(stamps) == 0:
+        return None, 0
+    a = np.ascontiguousarray(stamps, STAMP)
+    return a, len(a)
+
+
+def solve_hallen_planewave(segs, rhs, cosv, sinv, freq_hz, wire_first=(), wire_last=(), path_of=None,
+                           free_ends=(), g: Ground | None = None, precision: int = PREC_FP64,
+                           stamps=None, with_stats: bool = False):
+    """fnec_hip_solve_hallen_planewave: rhs is (n,) or (n_rhs, n); returns x of
+    shape (n_rhs, n + 2 G) (or (n + 2 G,) for one incidence)."""
+    segs = np.ascontiguousarray(segs, SEG)
+    n = len(segs)
+    r = np.ascontiguousarray(np.atleast_2d(rhs), np.complex128)
+    nrhs = r.shape[0]
+    wf, wl, fe = _sz(wire_first), _sz(wire_last), _sz(free_ends)
+    po = None if path_of is None else _sz(path_of)
+    groups = (int(po.max()) + 1 if n else 0) if po is not None else max(len(wf), 1)
+    x = np.zeros((nrhs, n + 2 * groups), np.complex128)
+    sa, ns = _stamp_arr(stamps)
+    st = SolveStats()
+    _check(lib().fnec_hip_solve_hallen_planewave(
+        _p(segs), n, _p(r), nrhs, _p(np.ascontiguousarray(cosv, np.float64)),
+        _p(np.ascontiguousarray(sinv, np.float64)), _p(wf), _p(wl), len(wf), _p(po) if po is not None else None,
+        _p(fe), len(fe), _p(sa) if sa is not None else None, ns, float(freq_hz), C.byref(g or ground()),
+        int(precision), _p(x), C.byref(st)))
+    x = x if np.ndim(rhs) == 2 else x[0]
+    return (x, st.as_dict()) if with_stats else x
+
+
+def solve_hallen_current_source(segs, shape, cosv, src, i0, freq_hz, wire_first=(), wire_last=(), path_of=None,
+                                free_ends=(), g: Ground | None = None, precision: int = PREC_FP64,
+                                stamps=None, with_stats: bool = False):
+    """fnec_hip_solve_hallen_current_source: x = currents, constants, port voltage."""
+    segs = np.ascontiguousarray(segs, SEG)
+    n = len(segs)
+    wf, wl, fe = _sz(wire_first), _sz(wire_last), _sz(free_ends)
+    po = None if path_of is None else _sz(path_of)
+    groups = (int(po.max()) + 1 if n else 0) if po is not None else max(len(wf), 1)
+    x = np.zeros(n + groups + 1, np.complex128)
+    sa, ns = _stamp_arr(stamps)
+    st = SolveStats()
+    _check(lib().fnec_hip_solve_hallen_current_source(
+        _p(segs), n, _p(np.ascontiguousarray(shape, np.complex128)), _p(np.ascontiguousarray(cosv, np.float64)),
+        int(src), C64(complex(i0).real, complex(i0).imag), _p(wf), _p(wl), len(wf),
+        _p(po) if po is not None else None, _p(fe), len(fe), _p(sa) if sa is not None else None, ns,
+        float(freq_hz), C.byref(g or ground()), int(precision), _p(x), C.byref(st)))
     return (x, st.as_dict()) if with_stats else x
 
 
@@ -347,7 +404,10 @@ class DeckProblem(C.Structure):
     _fields_ = [("route", C.c_int), ("req", Request), ("path_of", C.c_void_p), ("free_ends", C.c_void_p),
                 ("n_free", C.c_size_t), ("n_feeds", C.c_size_t), ("feed_idx", C.c_void_p),
                 ("feed_v", C.c_void_p), ("feed_tag", C.c_void_p), ("feed_seg", C.c_void_p),
-                ("seg_tag", C.c_void_p), ("seg_index", C.c_void_p)]
+                ("seg_tag", C.c_void_p), ("seg_index", C.c_void_p), ("variant", C.c_int),
+                ("sin_vec", C.c_void_p), ("n_rhs", C.c_size_t), ("rhs_all", C.c_void_p), ("n_recv", C.c_size_t),
+                ("recv_theta", C.c_void_p), ("recv_phi", C.c_void_p), ("src", C.c_size_t), ("i0", C64),
+                ("feed_port", C.c_void_p)]
 
 
 def _host_sigs(L):
@@ -438,7 +498,17 @@ class Deck:
             "seg_tag": _arr(pr.seg_tag, n, np.uint32),
             "seg_index": _arr(pr.seg_index, n, np.uint32),
             "stamps": _arr(r.stamps, r.n_stamps, STAMP),
+            "variant": pr.variant,
         }
+        if pr.variant == 1:
+            out["sin"] = _arr(pr.sin_vec, n, np.float64)
+            out["rhs_all"] = _arr(pr.rhs_all, n * pr.n_rhs, np.complex128).reshape(pr.n_rhs, n)
+            out["recv_theta"] = _arr(pr.recv_theta, pr.n_recv, np.float64)
+            out["recv_phi"] = _arr(pr.recv_phi, pr.n_recv, np.float64)
+        if pr.variant == 2:
+            out["src"] = pr.src
+            out["i0"] = complex(pr.i0.re, pr.i0.im)
+            out["feed_port"] = _arr(pr.feed_port, pr.n_feeds, np.uint8)
         return out
 
     def report(self, freq_index: int = 0, precision: int = PREC_FP64) -> str:
@@ -465,9 +535,23 @@ def solve_problem(p: dict, precision: int = PREC_FP64, with_stats: bool = False)
         pr.path_of = _p(keep["po"])
         pr.free_ends = _p(keep["fe"])
         pr.n_free = len(keep["fe"])
-        x = np.zeros(n + (int(keep["po"].max()) + 1 if n else 0), np.complex128)
+        groups = int(keep["po"].max()) + 1 if n else 0
     else:
-        x = np.zeros(n + max(len(keep["wf"]), 1), np.complex128)
+        groups = max(len(keep["wf"]), 1)
+    pr.variant = int(p.get("variant", 0))
+    shape = None
+    if pr.variant == 1:
+        keep["sin"] = np.ascontiguousarray(p["sin"], np.float64)
+        keep["rhs_all"] = np.ascontiguousarray(p["rhs_all"], np.complex128)
+        pr.sin_vec = _p(keep["sin"])
+        pr.rhs_all = _p(keep["rhs_all"])
+        pr.n_rhs = keep["rhs_all"].shape[0]
+        shape = (pr.n_rhs, n + 2 * groups)
+    elif pr.variant == 2:
+        pr.src = int(p["src"])
+        pr.i0 = C64(p["i0"].real, p["i0"].imag)
+        shape = (n + groups + 1,)
+    x = np.zeros(shape or (n + groups,), np.complex128)
     st = SolveStats()
     _check(L.fnec_deck_solve(C.byref(pr), int(precision), _p(x), C.byref(st)))
     return (x, st.as_dict()) if with_stats else x

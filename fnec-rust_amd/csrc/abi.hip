@@ -154,6 +154,62 @@ int fnec_hip_solve_hallen_paths(const fnec_seg *segs, size_t n, const fnec_c64 *
     return solve_one(hp, precision, x_out, stats);
 }
 
+int fnec_hip_solve_hallen_planewave(const fnec_seg *segs, size_t n, const fnec_c64 *rhs, size_t n_rhs,
+                                    const double *cos_vec, const double *sin_vec,
+                                    const size_t *wire_first, const size_t *wire_last, size_t n_wires,
+                                    const size_t *path_of, const size_t *free_ends, size_t n_free,
+                                    const fnec_stamp *stamps, size_t n_stamps, double freq_hz,
+                                    const fnec_ground *ground, int precision, fnec_c64 *x_out,
+                                    fnec_solve_stats *stats)
+{
+    std::lock_guard<std::mutex> lk(global_mutex());
+    int st = ensure_ready();
+    if (st)
+        return st;
+    if ((st = check_precision(precision)))
+        return st;
+    if (!x_out) {
+        set_error("null output");
+        return FNEC_INVALID_ARG;
+    }
+    HostProblem hp;
+    st = problem_from_planewave(hp, segs, n, rhs, n_rhs, cos_vec, sin_vec, wire_first, wire_last, n_wires,
+                                path_of, free_ends, n_free, freq_hz, ground);
+    if (!st)
+        st = problem_add_stamps(hp, stamps, n_stamps, n);
+    if (st)
+        return st;
+    return solve_one(hp, precision, x_out, stats);
+}
+
+int fnec_hip_solve_hallen_current_source(const fnec_seg *segs, size_t n, const fnec_c64 *shape,
+                                         const double *cos_vec, size_t src, fnec_c64 i0,
+                                         const size_t *wire_first, const size_t *wire_last,
+                                         size_t n_wires, const size_t *path_of, const size_t *free_ends,
+                                         size_t n_free, const fnec_stamp *stamps, size_t n_stamps,
+                                         double freq_hz, const fnec_ground *ground, int precision,
+                                         fnec_c64 *x_out, fnec_solve_stats *stats)
+{
+    std::lock_guard<std::mutex> lk(global_mutex());
+    int st = ensure_ready();
+    if (st)
+        return st;
+    if ((st = check_precision(precision)))
+        return st;
+    if (!x_out) {
+        set_error("null output");
+        return FNEC_INVALID_ARG;
+    }
+    HostProblem hp;
+    st = problem_from_current_source(hp, segs, n, shape, cos_vec, src, i0, wire_first, wire_last, n_wires,
+                                     path_of, free_ends, n_free, freq_hz, ground);
+    if (!st)
+        st = problem_add_stamps(hp, stamps, n_stamps, n);
+    if (st)
+        return st;
+    return solve_one(hp, precision, x_out, stats);
+}
+
 int fnec_deck_solve(const fnec_deck_problem *pr, int precision, fnec_c64 *x_out,
                     fnec_solve_stats *stats)
 {
@@ -169,7 +225,16 @@ int fnec_deck_solve(const fnec_deck_problem *pr, int precision, fnec_c64 *x_out,
     }
     const fnec_request &r = pr->req;
     HostProblem hp;
-    if (pr->route == 1)
+    const size_t *pof = pr->route == 1 ? pr->path_of : nullptr;
+    if (pr->variant == 1)
+        st = problem_from_planewave(hp, r.segs, r.n, pr->rhs_all, pr->n_rhs, r.cos_vec, pr->sin_vec,
+                                    r.wire_first, r.wire_last, r.n_wires, pof, pr->free_ends, pr->n_free,
+                                    r.freq_hz, &r.ground);
+    else if (pr->variant == 2)
+        st = problem_from_current_source(hp, r.segs, r.n, r.rhs, r.cos_vec, pr->src, pr->i0, r.wire_first,
+                                         r.wire_last, r.n_wires, pof, pr->free_ends, pr->n_free, r.freq_hz,
+                                         &r.ground);
+    else if (pr->route == 1)
         st = problem_from_paths(hp, r.segs, r.n, r.rhs, r.cos_vec, pr->path_of, pr->free_ends,
                                 pr->n_free, r.freq_hz, &r.ground);
     else

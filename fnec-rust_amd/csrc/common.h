@@ -59,6 +59,26 @@ struct Prob {
     double *diagmag;       // kPanel: |u_jj|
     int *fast_ok;          // [0]: the fast path factored the current panel
     int pad1;
+    // ---- plane-wave and current-source systems (plain path only) ----
+    // Right-hand sides: nrhs columns s .. s + nrhs - 1 of [G | M^H Y], rhs
+    // holds nrhs forcing vectors of n (column k at rhs + 2 n k), x nrhs
+    // solutions of s (solution k at x + 2 s k).
+    int nrhs;
+    int variant;           // 0: delta-gap system; 1: plane wave; 2: current source
+    // Entries of M beyond the -cos runs: nxruns records (row begin, row end,
+    // column, kind), kind 1: -sinv[r], kind 2: -(shape_re, shape_im)[r].
+    const int *xruns;
+    int nxruns;
+    int pad3;
+    const double *sinv;    // n
+    const double *shape;   // n complex, interleaved
+    // Per row r < n: the extra column holding its -cos, -sin and -shape
+    // entries (-1 for none), 3 ints per row.
+    const int *rowent;
+    // Constraint rows: the value at a (null: 1) and the right-hand side y
+    // (null: 0; complex, the same for every right-hand side column).
+    const double *cons_aval;
+    const double *cons_y;
     // ---- mixed-precision path (null on the plain FP64 path) ----
     float *fr, *fi;        // FP32 copy of Z written by the fill, then its L\U factors
     float *fpr, *fpi;      // FP32 panel workspace, ldz x kPanel

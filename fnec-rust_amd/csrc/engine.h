@@ -25,6 +25,19 @@ struct HostProblem {
     std::vector<double> cons_sign;
     std::vector<double> stamps; // 4 per Z stamp, applied in order: row, col, re, im
     double phase_max = 0.0;     // bound on every fill phase k R (direct and image)
+    // plane-wave (1) and current-source (2) systems, plain path only: rhs
+    // holds nrhs forcing vectors of n; M's -sin / -shape entries as xruns
+    // (row begin, row end, column, kind 1 or 2); rowent the extra columns of
+    // each row (3 per row: -cos, -sin, -shape, -1 for none); the constraint
+    // rows' value at a (default 1) and their y (default 0)
+    int variant = 0;
+    int nrhs = 1;
+    std::vector<double> sinv;     // n
+    std::vector<double> shape;    // 2n
+    std::vector<int> xruns;
+    std::vector<int> rowent;      // 3n
+    std::vector<double> cons_aval; // ncons
+    std::vector<double> cons_y;    // 2 ncons
 };
 
 // Host-side description of one far-field job. The grid is borrowed from
@@ -107,6 +120,25 @@ int problem_from_hallen(HostProblem &hp, const fnec_seg *segs, size_t n, const f
 int problem_from_paths(HostProblem &hp, const fnec_seg *segs, size_t n, const fnec_c64 *rhs,
                        const double *cos_vec, const size_t *path_of, const size_t *free_ends,
                        size_t nfree, double freq_hz, const fnec_ground *ground);
+// The plane-wave systems of solve_hallen_planewave (linear.rs:910-985; per
+// wire range, I = 0 at both ends of each) and solve_hallen_planewave_paths
+// (linear.rs:1017-1091; per conductor path when path_of is not null, I = 0
+// at the free ends): two homogeneous columns (-cos, -sin) per wire or path;
+// rhs holds nrhs forcing vectors of n (several incidences share the matrix).
+int problem_from_planewave(HostProblem &hp, const fnec_seg *segs, size_t n, const fnec_c64 *rhs,
+                           size_t nrhs, const double *cos_vec, const double *sin_vec,
+                           const size_t *wf, const size_t *wl, size_t nw, const size_t *path_of,
+                           const size_t *free_ends, size_t nfree, double freq_hz,
+                           const fnec_ground *ground);
+// The current-source systems of solve_hallen_current_source (linear.rs:
+// 1124-1210) and solve_hallen_current_source_paths (linear.rs:1241-1319): one
+// homogeneous column per wire or path, the port-voltage column -shape, the
+// I = 0 rows and the source row I[src] = i0, all weighted by 1e6.
+int problem_from_current_source(HostProblem &hp, const fnec_seg *segs, size_t n,
+                                const fnec_c64 *shape, const double *cos_vec, size_t src,
+                                fnec_c64 i0, const size_t *wf, const size_t *wl, size_t nw,
+                                const size_t *path_of, const size_t *free_ends, size_t nfree,
+                                double freq_hz, const fnec_ground *ground);
 // Impedance stamps (row, column, delta) added to Z after the fill, before the
 // solve; rows and columns are checked against the n segments.
 int problem_add_stamps(HostProblem &hp, const fnec_stamp *stamps, size_t n_stamps, size_t n);

@@ -29,7 +29,7 @@ __global__ void k_stamp_z(const Prob *, const int *);
 __global__ void k_gram(const Prob *, const int4 *);
 __global__ void k_gram_cols(const Prob *, const int2 *);
 __global__ void k_gram_fix(const Prob *, const int *);
-__global__ void k_backsub(const Prob *, const int *);
+__global__ void k_backsub(const Prob *, const int2 *);
 __global__ void k_residual_rows(const Prob *, const int2 *);
 __global__ void k_residual_finish(const Prob *, const int *);
 __global__ void k_rp_points(const RpProb *, const int2 *);
@@ -60,13 +60,15 @@ __global__ void k_mfma_probe(const double *, const double *, double *);
 __global__ void k_fill_exact(const Prob *, const int4 *);
 __global__ void k_gx_tile(const Prob *, const int4 *);
 __global__ void k_gx_cols(const Prob *, const int2 *);
+__global__ void k_gx_rhs(const Prob *, const int4 *);
+__global__ void k_gx_extra(const Prob *, const int2 *);
 __global__ void k_gx_fix(const Prob *, const int *);
 __global__ void k_ge_diag(const Prob *, const int *, int);
 __global__ void k_ge_rows(const Prob *, const int2 *, int);
 __global__ void k_ge_exact(const Prob *, const int *, int);
 __global__ void k_ge_u12(const Prob *, const int2 *, int);
 __global__ void k_ge_update(const Prob *, const int4 *, int);
-__global__ void k_backsub_exact(const Prob *, const int *);
+__global__ void k_backsub_exact(const Prob *, const int2 *);
 __global__ void k_refmath_probe(int, const double *, const double *, double *, int);
 
 // ---------------------------------------------------------------------------
@@ -572,6 +574,14 @@ int build_problem(HostProblem &hp, const fnec_seg *segs, size_t n, const fnec_c6
         return FNEC_INVALID_ARG;
     }
     hp.stamps.clear();
+    hp.variant = 0;
+    hp.nrhs = 1;
+    hp.sinv.clear();
+    hp.shape.clear();
+    hp.xruns.clear();
+    hp.rowent.clear();
+    hp.cons_aval.clear();
+    hp.cons_y.clear();
     hp.n = (int)n;
     hp.w = ncols;
     hp.s = (int)n + ncols;
@@ -736,6 +746,190 @@ int problem_from_paths(HostProblem &hp, const fnec_seg *segs, size_t n, const fn
     return build_problem(hp, segs, n, rhs, cos_vec, row_col, (int)np, cons, sign, freq_hz, ground);
 }
 
+// Shared by the plane-wave and current-source layouts: the homogeneous column
+// (-cos) of each row, from the wire ranges (later ranges win, uncovered rows
+// take range 0, as the reference's row_wire does) or from path_of; fills the
+// per-range or per-path count and the I = 0 rows (both ends of every range,
+// or the listed free ends).
+static int variant_rows(size_t n, const size_t *wf, const size_t *wl, size_t nw, const size_t *path_of,
+                        const size_t *free_ends, size_t nfree, std::vector<int> &row_col, int &groups,
+                        std::vector<int> &ends)
+{
+    row_col.assign(n, 0);
+    ends.clear();
+    if (path_of) {
+        if (nfree && !free_ends) {
+            set_error("null free-end list");
+            return FNEC_INVALID_ARG;
+        }
+        size_t np = 0;
+        for (size_t i = 0; i < n; ++i) {
+            if (path_of[i] >= n) {
+                set_error("path index out of range");
+                return FNEC_INVALID_ARG;
+            }
+            row_col[i] = (int)path_of[i];
+            np = std::max(np, path_of[i] + 1);
+        }
+        for (size_t i = 0; i < nfree; ++i) {
+            if (free_ends[i] >= n) {
+                set_error("free end outside the segment list");
+                return FNEC_INVALID_ARG;
+            }
+            ends.push_back((int)free_ends[i]);
+        }
+        groups = (int)np;
+        return FNEC_OK;
+    }
+    std::vector<size_t> ef, el;
+    if (nw == 0 || n == 0) {
+        ef.push_back(0);
+        el.push_back(n ? n - 1 : 0);
+    } else {
+        if (!wf || !wl) {
+            set_error("null wire range");
+            return FNEC_INVALID_ARG;
+        }
+        ef.assign(wf, wf + nw);
+        el.assign(wl, wl + nw);
+    }
+    for (size_t w = 0; w < ef.size(); ++w) {
+        if (ef[w] >= n || el[w] >= n) {
+            set_error("wire range outside the segment list");
+            return FNEC_INVALID_ARG;
+        }
+        for (size_t r = ef[w]; r <= el[w]; ++r)
+            row_col[r] = (int)w;
+        ends.push_back((int)ef[w]);
+        ends.push_back((int)el[w]);
+    }
+    groups = (int)ef.size();
+    return FNEC_OK;
+}
+
+// Runs (row begin, row end, column, kind) of equal column along the rows.
+static void xruns_from(const std::vector<int> &col, int kind, std::vector<int> &out)
+{
+    const size_t n = col.size();
+    for (size_t r = 0; r < n;) {
+        size_t e = r + 1;
+        while (e < n && col[e] == col[r])
+            ++e;
+        out.push_back((int)r);
+        out.push_back((int)e);
+        out.push_back(col[r]);
+        out.push_back(kind);
+        r = e;
+    }
+}
+
+int problem_from_planewave(HostProblem &hp, const fnec_seg *segs, size_t n, const fnec_c64 *rhs,
+                           size_t nrhs, const double *cos_vec, const double *sin_vec,
+                           const size_t *wf, const size_t *wl, size_t nw, const size_t *path_of,
+                           const size_t *free_ends, size_t nfree, double freq_hz,
+                           const fnec_ground *ground)
+{
+    if (!sin_vec || nrhs == 0 || nrhs > ((size_t)1 << 20)) {
+        set_error("plane wave: null sin vector or no right-hand side");
+        return FNEC_INVALID_ARG;
+    }
+    std::vector<int> grp, ends;
+    int groups = 0;
+    int st = variant_rows(n, wf, wl, nw, path_of, free_ends, nfree, grp, groups, ends);
+    if (st)
+        return st;
+    // columns n + 2 g (-cos) and n + 2 g + 1 (-sin) of group g
+    std::vector<int> ccol(n), scol(n);
+    for (size_t r = 0; r < n; ++r) {
+        ccol[r] = 2 * grp[r];
+        scol[r] = 2 * grp[r] + 1;
+    }
+    // I = 0 rows: per range, first then last end (linear.rs:946-949); per
+    // path, the free ends in order (linear.rs:1069-1071)
+    std::vector<int> cons;
+    std::vector<double> sign;
+    for (int e : ends) {
+        cons.push_back(e);
+        cons.push_back(-1);
+        sign.push_back(1.0);
+    }
+    st = build_problem(hp, segs, n, rhs, cos_vec, ccol, 2 * groups, cons, sign, freq_hz, ground);
+    if (st)
+        return st;
+    hp.variant = 1;
+    hp.nrhs = (int)nrhs;
+    hp.sinv.assign(sin_vec, sin_vec + n);
+    hp.rhs.resize(2 * n * nrhs);
+    for (size_t i = 0; i < n * nrhs; ++i) {
+        hp.rhs[2 * i] = rhs[i].re;
+        hp.rhs[2 * i + 1] = rhs[i].im;
+    }
+    xruns_from(scol, 1, hp.xruns);
+    hp.rowent.resize(3 * n);
+    for (size_t r = 0; r < n; ++r) {
+        hp.rowent[3 * r] = ccol[r];
+        hp.rowent[3 * r + 1] = scol[r];
+        hp.rowent[3 * r + 2] = -1;
+    }
+    return FNEC_OK;
+}
+
+int problem_from_current_source(HostProblem &hp, const fnec_seg *segs, size_t n,
+                                const fnec_c64 *shape, const double *cos_vec, size_t src,
+                                fnec_c64 i0, const size_t *wf, const size_t *wl, size_t nw,
+                                const size_t *path_of, const size_t *free_ends, size_t nfree,
+                                double freq_hz, const fnec_ground *ground)
+{
+#pragma clang fp contract(off)
+    if (!shape || src >= n) {
+        set_error("current source: null shape or source segment outside the segment list");
+        return FNEC_INVALID_ARG;
+    }
+    std::vector<int> grp, ends;
+    int groups = 0;
+    int st = variant_rows(n, wf, wl, nw, path_of, free_ends, nfree, grp, groups, ends);
+    if (st)
+        return st;
+    // constraint rows, each weighted by 1e6: the I = 0 ends, then the source
+    // row (linear.rs:1163-1172, 1296-1302)
+    const double cw = 1.0e6;
+    std::vector<int> cons;
+    std::vector<double> sign;
+    for (int e : ends) {
+        cons.push_back(e);
+        cons.push_back(-1);
+        sign.push_back(1.0);
+    }
+    cons.push_back((int)src);
+    cons.push_back(-1);
+    sign.push_back(1.0);
+    // the forcing of the segment rows is zero; -shape is the port-voltage
+    // column n + groups
+    std::vector<fnec_c64> zero(n, fnec_c64{0.0, 0.0});
+    st = build_problem(hp, segs, n, zero.data(), cos_vec, grp, groups + 1, cons, sign, freq_hz, ground);
+    if (st)
+        return st;
+    hp.variant = 2;
+    hp.shape.resize(2 * n);
+    for (size_t i = 0; i < n; ++i) {
+        hp.shape[2 * i] = shape[i].re;
+        hp.shape[2 * i + 1] = shape[i].im;
+    }
+    hp.xruns = {0, (int)n, groups, 2};
+    hp.rowent.resize(3 * n);
+    for (size_t r = 0; r < n; ++r) {
+        hp.rowent[3 * r] = grp[r];
+        hp.rowent[3 * r + 1] = -1;
+        hp.rowent[3 * r + 2] = groups;
+    }
+    hp.cons_aval.assign(hp.ncons, cw);
+    hp.cons_y.assign(2 * (size_t)hp.ncons, 0.0);
+    // y = (1e6, 0) * i0 in num-complex's product form
+    hp.cons_y[2 * (hp.ncons - 1)] = cw * i0.re - 0.0 * i0.im;
+    hp.cons_y[2 * (hp.ncons - 1) + 1] = cw * i0.im + 0.0 * i0.re;
+    return FNEC_OK;
+}
+
 int problem_add_stamps(HostProblem &hp, const fnec_stamp *stamps, size_t n_stamps, size_t n)
 {
     hp.stamps.clear();
@@ -794,6 +988,9 @@ struct Plan {
     bool given_z = false;  // Z uploaded by the caller (debug entry): no fill launch
     bool given_a = false;  // [A | b] uploaded by the caller: elimination only
     size_t xcols_off = 0, xcols_cnt = 0;
+    size_t xrhs_off = 0, xrhs_cnt = 0;   // k_gx_rhs tiles (problems with several right-hand sides)
+    size_t xext_off = 0, xext_cnt = 0;   // k_gx_extra task blocks
+    size_t bs_off = 0, bs_cnt = 0;       // back substitution items (problem, right-hand side)
     size_t all_off = 0, all_cnt = 0;
     size_t res_off = 0, res_cnt = 0;
     std::vector<LuStep> steps;
@@ -805,6 +1002,11 @@ struct Plan {
     size_t a_begin = 0, a_end = 0; // byte range of the A planes (zeroed per run)
     std::vector<size_t> off_seg, off_z, off_a, off_cos, off_rhs, off_runs, off_cons, off_sign,
         off_ipiv, off_info, off_x, off_stats, off_scr, off_panel, off_pmeta, off_selfln, off_stamps;
+    // plane-wave / current-source inputs: sinv, shape, xruns, rowent, cons_aval, cons_y
+    struct VarOff {
+        size_t sinv, shape, xruns, rowent, caval, cy;
+    };
+    std::vector<VarOff> var;
     size_t off_probs = 0, off_ints = 0, off_lud = 0;
     std::vector<LuDesc<double>> lud;
     // mixed-precision path
@@ -1259,6 +1461,7 @@ static void plan_build(Plan &pl, const std::vector<const HostProblem *> &hps, bo
     pl.off_pmeta.resize(np);
     pl.off_selfln.resize(np);
     pl.off_stamps.resize(np);
+    pl.var.assign(np, Plan::VarOff{0, 0, 0, 0, 0, 0});
     pl.fill_large = false;
     for (const HostProblem *h : hps)
         if (!(h->phase_max < 9e4))
@@ -1269,7 +1472,7 @@ static void plan_build(Plan &pl, const std::vector<const HostProblem *> &hps, bo
     for (size_t b = 0; b < np; ++b) {
         const HostProblem &h = *hps[b];
         size_t lda = (size_t)roundup(h.s, 64);
-        pl.off_a[b] = with_solve && !pl.mixed ? take(cur, 2 * 8 * lda * (h.s + 1)) : 0;
+        pl.off_a[b] = with_solve && !pl.mixed ? take(cur, 2 * 8 * lda * (h.s + h.nrhs)) : 0;
     }
     pl.a_end = cur;
     for (size_t b = 0; b < np; ++b) {
@@ -1330,10 +1533,19 @@ static void plan_build(Plan &pl, const std::vector<const HostProblem *> &hps, bo
         pl.off_selfln[b] = take(cur, 8 * (size_t)h.n);
         pl.off_stamps[b] = take(cur, 8 * h.stamps.size() + 8);
         pl.off_cos[b] = take(cur, 8 * (size_t)h.n);
-        pl.off_rhs[b] = take(cur, 16 * (size_t)h.n);
+        pl.off_rhs[b] = take(cur, 16 * (size_t)h.n * h.nrhs);
         pl.off_runs[b] = take(cur, 4 * h.runs.size() + 4);
         pl.off_cons[b] = take(cur, 4 * h.cons.size() + 4);
         pl.off_sign[b] = take(cur, 8 * h.cons_sign.size() + 8);
+        if (h.variant) {
+            Plan::VarOff &v = pl.var[b];
+            v.sinv = take(cur, 8 * h.sinv.size() + 8);
+            v.shape = take(cur, 8 * h.shape.size() + 8);
+            v.xruns = take(cur, 4 * h.xruns.size() + 4);
+            v.rowent = take(cur, 4 * h.rowent.size() + 4);
+            v.caval = take(cur, 8 * h.cons_aval.size() + 8);
+            v.cy = take(cur, 8 * h.cons_y.size() + 8);
+        }
         if (with_solve && pl.mixed)
             pl.mix[b].rowcol = take(cur, 4 * (size_t)h.n);
     }
@@ -1539,6 +1751,23 @@ static void plan_build(Plan &pl, const std::vector<const HostProblem *> &hps, bo
                 push2((int)b, y);
         pl.xcols_cnt = (pl.ints.size() - pl.xcols_off) / 2;
         align4();
+        pl.xrhs_off = pl.ints.size();
+        for (size_t b = 0; b < np; ++b)
+            if (hps[b]->nrhs > 1)
+                for (int y = 0; y < (hps[b]->n + 63) / 64; ++y)
+                    for (int z = 0; z < (hps[b]->nrhs + 63) / 64; ++z)
+                        push4((int)b, y, z, 0);
+        pl.xrhs_cnt = (pl.ints.size() - pl.xrhs_off) / 4;
+        align4();
+        pl.xext_off = pl.ints.size();
+        for (size_t b = 0; b < np; ++b) {
+            const long h = hps[b]->s - hps[b]->n;
+            const long tasks = h * (h + 1) / 2 + h * hps[b]->nrhs;
+            for (long y = 0; y < (tasks + 255) / 256; ++y)
+                push2((int)b, (int)y);
+        }
+        pl.xext_cnt = (pl.ints.size() - pl.xext_off) / 2;
+        align4();
         pl.all_off = pl.ints.size();
         for (size_t b = 0; b < np; ++b)
             pl.ints.push_back((int)b);
@@ -1546,10 +1775,16 @@ static void plan_build(Plan &pl, const std::vector<const HostProblem *> &hps, bo
         std::vector<int> ms(np), ncs(np);
         for (size_t b = 0; b < np; ++b) {
             ms[b] = hps[b]->s;
-            ncs[b] = hps[b]->s + 1;
+            ncs[b] = hps[b]->s + hps[b]->nrhs;
         }
         lu_plan(pl.ints, ms, ncs, pl.steps, pl.step_upd_flops, pl.step_panel_flops,
                 pl.step_trsm_flops);
+        align4();
+        pl.bs_off = pl.ints.size();
+        for (size_t b = 0; b < np; ++b)
+            for (int k = 0; k < hps[b]->nrhs; ++k)
+                push2((int)b, k);
+        pl.bs_cnt = (pl.ints.size() - pl.bs_off) / 2;
         align4();
         pl.res_off = pl.ints.size();
         for (size_t b = 0; b < np; ++b)
@@ -1582,7 +1817,7 @@ static void plan_build(Plan &pl, const std::vector<const HostProblem *> &hps, bo
             pl.mix[b].pcg = take16(64);
             pl.mix[b].pstate = take16(16);
         }
-        pl.off_x[b] = take16(16 * (size_t)hps[b]->s);
+        pl.off_x[b] = take16(16 * (size_t)hps[b]->s * hps[b]->nrhs);
     }
     pl.io_end = cur = (cur + 255) & ~(size_t)255;
     // generated work items (device only, written by k_gen_items)
@@ -1622,7 +1857,19 @@ static int plan_upload(Plan &pl, const std::vector<const HostProblem *> &hps, ch
         P.zi = P.zr + (size_t)P.ldz * h.n;
         if (with_solve && !pl.mixed) {
             P.ar = (double *)(base + pl.off_a[b]);
-            P.ai = P.ar + (size_t)P.lda * (h.s + 1);
+            P.ai = P.ar + (size_t)P.lda * (h.s + h.nrhs);
+        }
+        P.nrhs = h.nrhs;
+        P.variant = h.variant;
+        if (h.variant) {
+            const Plan::VarOff &v = pl.var[b];
+            P.sinv = (const double *)(base + v.sinv);
+            P.shape = (const double *)(base + v.shape);
+            P.xruns = (const int *)(base + v.xruns);
+            P.nxruns = (int)h.xruns.size() / 4;
+            P.rowent = (const int *)(base + v.rowent);
+            P.cons_aval = h.cons_aval.empty() ? nullptr : (const double *)(base + v.caval);
+            P.cons_y = h.cons_y.empty() ? nullptr : (const double *)(base + v.cy);
         }
         P.cosv = (const double *)(base + pl.off_cos[b]);
         P.rhs = (const double *)(base + pl.off_rhs[b]);
@@ -1728,6 +1975,15 @@ static int plan_upload(Plan &pl, const std::vector<const HostProblem *> &hps, ch
         put(pl.off_runs[b], h.runs.data(), 4 * h.runs.size());
         put(pl.off_cons[b], h.cons.data(), 4 * h.cons.size());
         put(pl.off_sign[b], h.cons_sign.data(), 8 * h.cons_sign.size());
+        if (h.variant) {
+            const Plan::VarOff &v = pl.var[b];
+            put(v.sinv, h.sinv.data(), 8 * h.sinv.size());
+            put(v.shape, h.shape.data(), 8 * h.shape.size());
+            put(v.xruns, h.xruns.data(), 4 * h.xruns.size());
+            put(v.rowent, h.rowent.data(), 4 * h.rowent.size());
+            put(v.caval, h.cons_aval.data(), 8 * h.cons_aval.size());
+            put(v.cy, h.cons_y.data(), 8 * h.cons_y.size());
+        }
         if (with_solve && pl.mixed)
             put(pl.mix[b].rowcol, h.rowcol.data(), 4 * h.rowcol.size());
     }
@@ -2000,6 +2256,29 @@ static int mixed_launch(Plan &pl, char *base)
 // Plain path in the reference's operation order: normal equations (unless
 // the caller supplied [A | b]), elimination in 32-column panels, back
 // substitution. Events 2 and 3 bracket the elimination.
+// [G | M^H Y] of the plain path in the reference's summation order: Z^H Z
+// tiles, the extra columns against Z and the single right-hand side, the
+// other right-hand sides, the extra-column block, then the constraint rows and
+// lambda (which k_gx_fix adds last).
+static void gx_launch(const Plan &pl, char *base, hipStream_t st)
+{
+    const Prob *dp = (const Prob *)(base + pl.off_probs);
+    if (pl.gram_cnt)
+        hipLaunchKernelGGL(k_gx_tile, dim3((unsigned)pl.gram_cnt), dim3(256), 0, st, dp,
+                           (const int4 *)wl_ptr(base, pl, pl.gram_off));
+    if (pl.xcols_cnt)
+        hipLaunchKernelGGL(k_gx_cols, dim3((unsigned)pl.xcols_cnt), dim3(256), 0, st, dp,
+                           (const int2 *)wl_ptr(base, pl, pl.xcols_off));
+    if (pl.xrhs_cnt)
+        hipLaunchKernelGGL(k_gx_rhs, dim3((unsigned)pl.xrhs_cnt), dim3(256), 0, st, dp,
+                           (const int4 *)wl_ptr(base, pl, pl.xrhs_off));
+    if (pl.xext_cnt)
+        hipLaunchKernelGGL(k_gx_extra, dim3((unsigned)pl.xext_cnt), dim3(256), 0, st, dp,
+                           (const int2 *)wl_ptr(base, pl, pl.xext_off));
+    hipLaunchKernelGGL(k_gx_fix, dim3((unsigned)pl.all_cnt), dim3(64), 0, st, dp,
+                       wl_ptr(base, pl, pl.all_off));
+}
+
 static int exact_launch(Plan &pl, char *base)
 {
     hipStream_t st = LN().stream;
@@ -2007,13 +2286,7 @@ static int exact_launch(Plan &pl, char *base)
     const int *all = wl_ptr(base, pl, pl.all_off);
     if (!pl.given_a) {
         hipEvent_t e = prof_begin(PC_GRAM);
-        if (pl.gram_cnt)
-            hipLaunchKernelGGL(k_gx_tile, dim3((unsigned)pl.gram_cnt), dim3(256), 0, st, dp,
-                               (const int4 *)wl_ptr(base, pl, pl.gram_off));
-        if (pl.xcols_cnt)
-            hipLaunchKernelGGL(k_gx_cols, dim3((unsigned)pl.xcols_cnt), dim3(256), 0, st, dp,
-                               (const int2 *)wl_ptr(base, pl, pl.xcols_off));
-        hipLaunchKernelGGL(k_gx_fix, dim3((unsigned)pl.all_cnt), dim3(64), 0, st, dp, all);
+        gx_launch(pl, base, st);
         prof_end(PC_GRAM, e, pl.gram_flops, 0);
     }
     HIPCHK(hipGetLastError());
@@ -2049,10 +2322,11 @@ static int exact_launch(Plan &pl, char *base)
     HIPCHK(hipEventRecord(LN().ev[3], st));
     {
         hipEvent_t e = prof_begin(PC_BACKSUB);
+        const int2 *bs = (const int2 *)wl_ptr(base, pl, pl.bs_off);
         if (g_exact_backsub)
-            hipLaunchKernelGGL(k_backsub_exact, dim3((unsigned)pl.all_cnt), dim3(256), 0, st, dp, all);
+            hipLaunchKernelGGL(k_backsub_exact, dim3((unsigned)pl.bs_cnt), dim3(256), 0, st, dp, bs);
         else
-            hipLaunchKernelGGL(k_backsub, dim3((unsigned)pl.all_cnt), dim3(kBsThreads), 0, st, dp, all);
+            hipLaunchKernelGGL(k_backsub, dim3((unsigned)pl.bs_cnt), dim3(kBsThreads), 0, st, dp, bs);
         prof_end(PC_BACKSUB, e, pl.backsub_flops, 0);
     }
     if (!pl.given_a) {
@@ -2130,8 +2404,8 @@ static int plan_launch(Plan &pl, char *base, bool with_solve)
     HIPCHK(hipEventRecord(LN().ev[3], st));
     {
         hipEvent_t e = prof_begin(PC_BACKSUB);
-        hipLaunchKernelGGL(k_backsub, dim3((unsigned)pl.all_cnt), dim3(kBsThreads), 0, st, dp,
-                           wl_ptr(base, pl, pl.all_off));
+        hipLaunchKernelGGL(k_backsub, dim3((unsigned)pl.bs_cnt), dim3(kBsThreads), 0, st, dp,
+                           (const int2 *)wl_ptr(base, pl, pl.bs_off));
         prof_end(PC_BACKSUB, e, pl.backsub_flops, 0);
     }
     hipLaunchKernelGGL(k_residual_rows, dim3((unsigned)pl.res_cnt), dim3(256), 0, st, dp,
@@ -2161,6 +2435,11 @@ int solve_problems(const std::vector<const HostProblem *> &hps, std::vector<fnec
     if (lane_scope.status())
         return lane_scope.status();
     const bool mixed = precision == FNEC_PREC_MIXED;
+    for (const HostProblem *h : hps)
+        if ((h->variant || h->nrhs != 1) && precision != FNEC_PREC_FP64) {
+            set_error("plane-wave and current-source systems are solved on the plain FP64 path");
+            return FNEC_UNAVAILABLE;
+        }
     if (mixed)
         for (const HostProblem *h : hps)
             if (h->w < 1 || h->w > kMixMaxW || h->ncons > kMixMaxCons) {
@@ -2274,7 +2553,7 @@ int solve_problems(const std::vector<const HostProblem *> &hps, std::vector<fnec
         s.t_solve_us = t_sol;
         s.t_total_us = t_fill + t_gram + t_lu + t_sol;
         if (x_out.size() > b && x_out[b] && status[b] == FNEC_OK)
-            std::memcpy(x_out[b], back(pl.off_x[b]), 16 * (size_t)hps[b]->s);
+            std::memcpy(x_out[b], back(pl.off_x[b]), 16 * (size_t)hps[b]->s * hps[b]->nrhs);
         if (device_x)
             device_x->push_back(dev_x[b]);
     }
@@ -2331,12 +2610,7 @@ int normal_eqs_only(const HostProblem &hp, fnec_c64 *aug_out)
     gen_launch(pl, base, s);
     hipLaunchKernelGGL(k_fill_exact, dim3((unsigned)pl.fill_cnt), dim3(256), 0, s, dp,
                        (const int4 *)gen_ptr(base, pl) + pl.fill_off);
-    hipLaunchKernelGGL(k_gx_tile, dim3((unsigned)pl.gram_cnt), dim3(256), 0, s, dp,
-                       (const int4 *)wl_ptr(base, pl, pl.gram_off));
-    hipLaunchKernelGGL(k_gx_cols, dim3((unsigned)pl.xcols_cnt), dim3(256), 0, s, dp,
-                       (const int2 *)wl_ptr(base, pl, pl.xcols_off));
-    hipLaunchKernelGGL(k_gx_fix, dim3((unsigned)pl.all_cnt), dim3(64), 0, s, dp,
-                       wl_ptr(base, pl, pl.all_off));
+    gx_launch(pl, base, s);
     HIPCHK(hipGetLastError());
     const int S = hp.s, lda = roundup(S, 64);
     std::vector<double> ar((size_t)lda * (S + 1)), ai((size_t)lda * (S + 1));

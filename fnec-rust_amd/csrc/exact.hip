@@ -284,10 +284,30 @@ __global__ __launch_bounds__(256) void k_gx_tile(const Prob *__restrict__ probs,
         }
 }
 
+// Entry of M at row r < n in an extra column of the given kind (linear.rs:
+// 728-731, 942-945, 1157-1161): -cos_r and -sin_r as (value, 0), and the
+// current source's -shape_r (num-complex negation of both parts).
+__device__ __forceinline__ void m_extra(const Prob &P, int kind, int r, double &vr, double &vi)
+{
+    if (kind == 0) {
+        vr = -P.cosv[r];
+        vi = 0.0;
+    } else if (kind == 1) {
+        vr = -P.sinv[r];
+        vi = 0.0;
+    } else {
+        vr = -P.shape[2 * r];
+        vi = -P.shape[2 * r + 1];
+    }
+}
+
 // Work item (problem, block of 256 columns): for each column j < n of Z,
-//   (M^H y)[j]   = sum_{r < n} conj(Z_rj) rhs_r
-//   G[n+w][j]    = sum_{r in wire w} conj(-cos_r) Z_rj   (and its conjugate
-//                  G[j][n+w]); the rows of wire w are its runs, ascending.
+//   (M^H y)[j]   = sum_{r < n} conj(Z_rj) rhs_r   (one right-hand side; with
+//                  several, k_gx_rhs forms them)
+//   G[n+e][j]    = sum_{r in column e} conj(m_{r,n+e}) Z_rj   (and its
+//                  conjugate G[j][n+e]) for every extra column e: the rows of
+//                  a -cos column are its runs, of a -sin or -shape column its
+//                  xruns, in ascending order.
 __global__ __launch_bounds__(256) void k_gx_cols(const Prob *__restrict__ probs,
                                                  const int2 *__restrict__ work)
 {
@@ -298,15 +318,17 @@ __global__ __launch_bounds__(256) void k_gx_cols(const Prob *__restrict__ probs,
     if (j >= n)
         return;
     const double *zr = P.zr + (size_t)j * P.ldz, *zi = P.zi + (size_t)j * P.ldz;
-    double yr = 0.0, yi = 0.0;
-    for (int r = 0; r < n; ++r) {
-        double pr, pi;
-        cjmul(zr[r], zi[r], P.rhs[2 * r], P.rhs[2 * r + 1], pr, pi);
-        yr = yr + pr;
-        yi = yi + pi;
+    if (P.nrhs == 1) {
+        double yr = 0.0, yi = 0.0;
+        for (int r = 0; r < n; ++r) {
+            double pr, pi;
+            cjmul(zr[r], zi[r], P.rhs[2 * r], P.rhs[2 * r + 1], pr, pi);
+            yr = yr + pr;
+            yi = yi + pi;
+        }
+        P.ar[(size_t)s * lda + j] = yr;
+        P.ai[(size_t)s * lda + j] = yi;
     }
-    P.ar[(size_t)s * lda + j] = yr;
-    P.ai[(size_t)s * lda + j] = yi;
     for (int w = 0; w < P.w; ++w) {
         double gr = 0.0, gi = 0.0;
         for (int q = 0; q < P.nruns; ++q) {
@@ -320,6 +342,18 @@ __global__ __launch_bounds__(256) void k_gx_cols(const Prob *__restrict__ probs,
                 gi = gi + pi;
             }
         }
+        for (int q = 0; q < P.nxruns; ++q) {
+            const int rb = P.xruns[4 * q], re = P.xruns[4 * q + 1], kind = P.xruns[4 * q + 3];
+            if (P.xruns[4 * q + 2] != w)
+                continue;
+            for (int r = rb; r < re; ++r) {
+                double vr, vi, pr, pi;
+                m_extra(P, kind, r, vr, vi);
+                cjmul(vr, vi, zr[r], zi[r], pr, pi);
+                gr = gr + pr;
+                gi = gi + pi;
+            }
+        }
         const int cw = n + w;
         P.ar[(size_t)j * lda + cw] = gr; // G[n+w][j]
         P.ai[(size_t)j * lda + cw] = gi;
@@ -328,10 +362,164 @@ __global__ __launch_bounds__(256) void k_gx_cols(const Prob *__restrict__ probs,
     }
 }
 
-// One thread per problem, after k_gx_tile and k_gx_cols: the homogeneous
-// block and (M^H y) of the homogeneous rows, then the constraint rows' terms
-// in row order (each continues its elements' sums), then lambda
-// (linear.rs:782-785).
+// Work item (problem, tj, tk): the 64 x 64 tile of (M^H Y)[j][k] =
+// sum_{r < n} conj(Z_rj) y_k[r] for columns j < n of Z and right-hand sides
+// k (several incidences of a plane wave), each a sequential sum over r in
+// ascending order as k_gx_cols forms the single one. Written to column s + k
+// of [G | M^H Y].
+__global__ __launch_bounds__(256) void k_gx_rhs(const Prob *__restrict__ probs,
+                                                const int4 *__restrict__ work)
+{
+    const int4 wk = work[blockIdx.x];
+    const Prob &P = probs[wk.x];
+    const int n = P.n, ldz = P.ldz, lda = P.lda, nrhs = P.nrhs;
+    const int j0 = wk.y * kXT, k0 = wk.z * kXT;
+    __shared__ double sar[kXK][kXT + 1], sai[kXK][kXT + 1];
+    __shared__ double sbr[kXK][kXT + 1], sbi[kXK][kXT + 1];
+    const int t = threadIdx.x, ty = t >> 4, tx = t & 15;
+    double gr[4][4], gi[4][4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+            gr[u][v] = gi[u][v] = 0.0;
+    for (int r0 = 0; r0 < n; r0 += kXK) {
+        const int cnt = min(kXK, n - r0);
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int e = t + 256 * q, c = e >> 4, rr = e & 15;
+            double ar = 0.0, ai = 0.0, br = 0.0, bi = 0.0;
+            if (rr < cnt) {
+                if (j0 + c < n) {
+                    ar = P.zr[(size_t)(j0 + c) * ldz + r0 + rr];
+                    ai = P.zi[(size_t)(j0 + c) * ldz + r0 + rr];
+                }
+                if (k0 + c < nrhs) {
+                    const size_t o = 2 * ((size_t)(k0 + c) * n + r0 + rr);
+                    br = P.rhs[o];
+                    bi = P.rhs[o + 1];
+                }
+            }
+            sar[rr][c] = ar;
+            sai[rr][c] = ai;
+            sbr[rr][c] = br;
+            sbi[rr][c] = bi;
+        }
+        __syncthreads();
+        for (int rr = 0; rr < cnt; ++rr) {
+            double a_r[4], a_i[4], b_r[4], b_i[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                a_r[u] = sar[rr][ty + 16 * u];
+                a_i[u] = sai[rr][ty + 16 * u];
+                b_r[u] = sbr[rr][tx + 16 * u];
+                b_i[u] = sbi[rr][tx + 16 * u];
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    double pr, pi;
+                    cjmul(a_r[u], a_i[u], b_r[v], b_i[v], pr, pi);
+                    gr[u][v] = gr[u][v] + pr;
+                    gi[u][v] = gi[u][v] + pi;
+                }
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int j = j0 + ty + 16 * u, k = k0 + tx + 16 * v;
+            if (j >= n || k >= nrhs)
+                continue;
+            P.ar[(size_t)(P.s + k) * lda + j] = gr[u][v];
+            P.ai[(size_t)(P.s + k) * lda + j] = gi[u][v];
+        }
+}
+
+// Work item (problem, block of 256 tasks) over the extra columns e, f in
+// [0, s - n): task p < h (h + 1) / 2 forms G[n+e][n+f], f <= e (and its
+// conjugate mirror), as the sum over the rows of column e, ascending, of
+// conj(m_{r,n+e}) m_{r,n+f} (rows where column f is zero skipped); the next
+// h * nrhs tasks form (M^H y_k)[n+e] over the same rows. Rows of a column are
+// its runs (-cos) or xruns (-sin, -shape); rowent gives each row's columns.
+__global__ __launch_bounds__(256) void k_gx_extra(const Prob *__restrict__ probs,
+                                                  const int2 *__restrict__ work)
+{
+    const int2 wk = work[blockIdx.x];
+    const Prob &P = probs[wk.x];
+    const int n = P.n, s = P.s, lda = P.lda, h = s - n;
+    const int npair = h * (h + 1) / 2;
+    const int task = wk.y * 256 + threadIdx.x;
+    if (task >= npair + h * P.nrhs)
+        return;
+    int e, f = -1, k = -1;
+    if (task < npair) {
+        e = (int)((sqrt(8.0 * task + 1.0) - 1.0) * 0.5);
+        while ((e + 1) * (e + 2) / 2 <= task)
+            ++e;
+        while (e * (e + 1) / 2 > task)
+            --e;
+        f = task - e * (e + 1) / 2;
+    } else {
+        e = (task - npair) / P.nrhs;
+        k = (task - npair) % P.nrhs;
+    }
+    const double *y = k >= 0 ? P.rhs + 2 * (size_t)k * n : nullptr;
+    double ar = 0.0, ai = 0.0;
+    auto row = [&](int kind, int r) {
+        double vr, vi, pr, pi;
+        m_extra(P, kind, r, vr, vi);
+        if (k >= 0) {
+            cjmul(vr, vi, y[2 * r], y[2 * r + 1], pr, pi);
+        } else {
+            double wr = vr, wi = vi;
+            if (f != e) {
+                // a delta-gap row has one extra entry (no rowent): column f is
+                // zero there
+                if (!P.rowent)
+                    return;
+                int fk = -1;
+                for (int q = 0; q < 3; ++q)
+                    if (P.rowent[3 * r + q] == f)
+                        fk = q;
+                if (fk < 0)
+                    return;
+                m_extra(P, fk, r, wr, wi);
+            }
+            cjmul(vr, vi, wr, wi, pr, pi);
+        }
+        ar = ar + pr;
+        ai = ai + pi;
+    };
+    for (int q = 0; q < P.nruns; ++q)
+        if (P.runs[3 * q + 2] == e)
+            for (int r = P.runs[3 * q]; r < P.runs[3 * q + 1]; ++r)
+                row(0, r);
+    for (int q = 0; q < P.nxruns; ++q)
+        if (P.xruns[4 * q + 2] == e)
+            for (int r = P.xruns[4 * q]; r < P.xruns[4 * q + 1]; ++r)
+                row(P.xruns[4 * q + 3], r);
+    if (k >= 0) {
+        P.ar[(size_t)(s + k) * lda + n + e] = ar;
+        P.ai[(size_t)(s + k) * lda + n + e] = ai;
+        return;
+    }
+    P.ar[(size_t)(n + f) * lda + n + e] = ar; // G[n+e][n+f]
+    P.ai[(size_t)(n + f) * lda + n + e] = ai;
+    if (f != e) {
+        P.ar[(size_t)(n + e) * lda + n + f] = ar;
+        P.ai[(size_t)(n + e) * lda + n + f] = -ai;
+    }
+}
+
+// One thread per problem, after the kernels above: the constraint rows'
+// terms in row order (each continues its elements' sums of G and, where the
+// row's y is not zero, of M^H y), then lambda (linear.rs:782-785). A row's
+// entries are (a, aval) and, for a junction, (b, sign); a zero y adds +-0,
+// which leaves every sum as it is, so such rows skip M^H y.
 __global__ __launch_bounds__(64) void k_gx_fix(const Prob *__restrict__ probs,
                                                const int *__restrict__ plist)
 {
@@ -339,39 +527,13 @@ __global__ __launch_bounds__(64) void k_gx_fix(const Prob *__restrict__ probs,
     if (threadIdx.x != 0)
         return;
     const int n = P.n, s = P.s, lda = P.lda;
-    for (int w = 0; w < P.w; ++w) {
-        double cr = 0.0, ci = 0.0, yr = 0.0, yi = 0.0;
-        for (int q = 0; q < P.nruns; ++q) {
-            const int rb = P.runs[3 * q], re = P.runs[3 * q + 1];
-            if (P.runs[3 * q + 2] != w)
-                continue;
-            for (int r = rb; r < re; ++r) {
-                const double c = -P.cosv[r];
-                double pr, pi;
-                cjmul(c, 0.0, c, 0.0, pr, pi);
-                cr = cr + pr;
-                ci = ci + pi;
-                cjmul(c, 0.0, P.rhs[2 * r], P.rhs[2 * r + 1], pr, pi);
-                yr = yr + pr;
-                yi = yi + pi;
-            }
-        }
-        const int cw = n + w;
-        for (int v = 0; v < P.w; ++v) {
-            P.ar[(size_t)(n + v) * lda + cw] = v == w ? cr : 0.0;
-            P.ai[(size_t)(n + v) * lda + cw] = v == w ? ci : 0.0;
-        }
-        P.ar[(size_t)s * lda + cw] = yr;
-        P.ai[(size_t)s * lda + cw] = yi;
-    }
-    // constraint rows n + c: entries (a, 1) and, for a junction, (b, sign);
-    // each product conj(m_rp) m_rq continues the sum of G[p][q], p >= q
+    (void)n;
     for (int c = 0; c < P.ncons; ++c) {
         int idx[2];
         double val[2];
         int ne = 1;
         idx[0] = P.cons[2 * c];
-        val[0] = 1.0;
+        val[0] = P.cons_aval ? P.cons_aval[c] : 1.0;
         const int b = P.cons[2 * c + 1];
         if (b >= 0) {
             if (b == idx[0]) {
@@ -397,6 +559,18 @@ __global__ __launch_bounds__(64) void k_gx_fix(const Prob *__restrict__ probs,
                     P.ai[(size_t)p * lda + q] = -P.ai[o];
                 }
             }
+        if (P.cons_y) {
+            const double yr = P.cons_y[2 * c], yi = P.cons_y[2 * c + 1];
+            if (yr != 0.0 || yi != 0.0)
+                for (int k = 0; k < P.nrhs; ++k)
+                    for (int e = 0; e < ne; ++e) {
+                        double pr, pi;
+                        cjmul(val[e], 0.0, yr, yi, pr, pi);
+                        const size_t o = (size_t)(s + k) * lda + idx[e];
+                        P.ar[o] = P.ar[o] + pr;
+                        P.ai[o] = P.ai[o] + pi;
+                    }
+        }
     }
     for (int i = 0; i < s; ++i) {
         const size_t o = (size_t)i * lda + i;
@@ -669,7 +843,7 @@ __global__ __launch_bounds__(64) void k_ge_u12(const Prob *__restrict__ probs,
         piv[t] = P.ipiv[p + t];
     __syncthreads();
     const int col = c1 + wk.y * 64 + t;
-    if (col > s)
+    if (col >= s + P.nrhs)
         return;
     double *cr = P.ar + (size_t)col * lda, *ci = P.ai + (size_t)col * lda;
     for (int j = 0; j < jb; ++j) {
@@ -742,7 +916,7 @@ __global__ __launch_bounds__(256) void k_ge_update(const Prob *__restrict__ prob
     for (int e = t; e < jb * kXT; e += 256) {
         const int m = e / jb, c = e % jb;
         double a = 0.0, b = 0.0;
-        if (k0 + m <= s) {
+        if (k0 + m < s + P.nrhs) {
             a = P.ar[(size_t)(k0 + m) * lda + p + c];
             b = P.ai[(size_t)(k0 + m) * lda + p + c];
         }
@@ -757,7 +931,7 @@ __global__ __launch_bounds__(256) void k_ge_update(const Prob *__restrict__ prob
         for (int v = 0; v < 4; ++v) {
             const int r = r0 + ty + 16 * u, k = k0 + tx + 16 * v;
             cr[u][v] = ci[u][v] = 0.0;
-            if (r < s && k <= s) {
+            if (r < s && k < s + P.nrhs) {
                 cr[u][v] = P.ar[(size_t)k * lda + r];
                 ci[u][v] = P.ai[(size_t)k * lda + r];
             }
@@ -786,7 +960,7 @@ __global__ __launch_bounds__(256) void k_ge_update(const Prob *__restrict__ prob
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
             const int r = r0 + ty + 16 * u, k = k0 + tx + 16 * v;
-            if (r < s && k <= s) {
+            if (r < s && k < s + P.nrhs) {
                 P.ar[(size_t)k * lda + r] = cr[u][v];
                 P.ai[(size_t)k * lda + r] = ci[u][v];
             }
@@ -800,20 +974,22 @@ __global__ __launch_bounds__(256) void k_ge_update(const Prob *__restrict__ prob
 constexpr int kXBsThreads = 256;
 constexpr int kXBsChunk = 1024;
 
-// One workgroup per problem. Row i: sum = b_i - a_i,i+1 x_i+1 - ... (ascending
+// One workgroup per (problem, right-hand side). Row i: sum = b_i - a_i,i+1 x_i+1 - ... (ascending
 // j, each subtraction rounded), x_i = sum / a_ii. The products of a chunk of
 // the row are formed in parallel; one lane runs the subtraction chain.
 __global__ __launch_bounds__(kXBsThreads) void k_backsub_exact(const Prob *__restrict__ probs,
-                                                               const int *__restrict__ plist)
+                                                               const int2 *__restrict__ work)
 {
-    const Prob &P = probs[plist[blockIdx.x]];
+    const int2 wk = work[blockIdx.x];
+    const Prob &P = probs[wk.x];
     if (P.info[0] != 0)
         return;
     const int s = P.s, lda = P.lda, t = threadIdx.x;
-    double *x = P.x;
+    double *x = P.x + 2 * (size_t)s * wk.y;
+    const size_t bcol = (size_t)(s + wk.y) * lda;
     __shared__ double pr[kXBsChunk], pi[kXBsChunk];
     for (int i = s - 1; i >= 0; --i) {
-        double sr = P.ar[(size_t)s * lda + i], si = P.ai[(size_t)s * lda + i];
+        double sr = P.ar[bcol + i], si = P.ai[bcol + i];
         for (int j0 = i + 1; j0 < s; j0 += kXBsChunk) {
             const int cnt = min(kXBsChunk, s - j0);
             __syncthreads();

@@ -4,6 +4,7 @@
 // reference's operation order so that segment data and right-hand sides are
 // the same doubles the reference's callers pass to nec_accel.
 #include <algorithm>
+#include <array>
 #include <cctype>
 #include <cmath>
 #include <cstdio>
@@ -124,6 +125,10 @@ struct fnec_deck {
     std::vector<size_t> wf, wl, ja, jb, path_of, free_ends, feed_idx;
     std::vector<uint32_t> feed_tag, feed_seg, seg_tag, seg_index;
     std::vector<fnec_stamp> stamps;
+    // plane-wave / current-source storage
+    std::vector<double> sinv, recv_theta, recv_phi;
+    std::vector<fnec_c64> rhs_all;
+    std::vector<unsigned char> feed_port;
 };
 
 namespace {
@@ -635,6 +640,163 @@ int hallen_rhs_paths(const std::vector<Card> &cards, const std::vector<Seg> &s, 
     return FNEC_OK;
 }
 
+// ---- plane-wave and current-source inputs (planewave.rs:54-347,
+// excitation.rs:350-447) ----
+//
+// Where the reference takes the sine and the cosine of one angle, its
+// optimised code makes one sincos call; these builders call sincos there, and
+// sin alone where the reference needs only the sine, so every value is the
+// same libm result the reference's build computes.
+
+double card_f(const Card &c, int i) { return c.nf > i ? c.f[i] : 0.0; }
+
+fnec_c64 cmul(fnec_c64 a, fnec_c64 b) { return fnec_c64{a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re}; }
+
+// IncidentPlaneWave::from_ex_card (planewave.rs:59-76) at incidence (theta,
+// phi) in degrees: the propagation unit vector r_hat and the complex
+// polarisation vector major + j sense AR minor (planewave.rs:79-118).
+struct Wave {
+    double r_hat[3];
+    fnec_c64 pol[3];
+};
+
+Wave plane_wave(const Card &ex, double theta_deg, double phi_deg)
+{
+    const int type = ex_type(ex);
+    const double ar = type == 1 ? 0.0 : card_f(ex, 9);
+    const double sense = type == 2 ? 1.0 : (type == 3 ? -1.0 : 0.0);
+    double st, ct, sp, cp, se, ce;
+    sincos(deg2rad(theta_deg), &st, &ct);
+    sincos(deg2rad(phi_deg), &sp, &cp);
+    sincos(deg2rad(card_f(ex, 6)), &se, &ce);
+    Wave w;
+    w.r_hat[0] = st * cp;
+    w.r_hat[1] = st * sp;
+    w.r_hat[2] = ct;
+    const double th[3] = {ct * cp, ct * sp, -st}, ph[3] = {-sp, cp, 0.0};
+    const double jm = sense * ar; // jminor = (0, sense AR)
+    for (int i = 0; i < 3; ++i) {
+        const double major = ce * th[i] + se * ph[i];
+        const double minor = -se * th[i] + ce * ph[i];
+        // (major, 0) + jminor * minor
+        w.pol[i] = fnec_c64{major + 0.0 * minor, 0.0 + jm * minor};
+    }
+    return w;
+}
+
+// (pol . d) E0, E0 = 1: the sum of pol[c] d[c] from zero, then scaled by E0.
+fnec_c64 coupling(const Wave &w, const double d[3])
+{
+    fnec_c64 acc{0.0, 0.0};
+    for (int c = 0; c < 3; ++c) {
+        acc.re = acc.re + w.pol[c].re * d[c];
+        acc.im = acc.im + w.pol[c].im * d[c];
+    }
+    return fnec_c64{acc.re * 1.0, acc.im * 1.0};
+}
+
+// build_planewave_hallen (planewave.rs:153-243) on the raw wire ranges, or
+// build_planewave_hallen_paths (planewave.rs:271-347) when P is given: the
+// forcing rhs(s_m) = -j (2 pi / eta0) sum_{p on m's wire or path} E_t(s_p)
+// sin(k |s_m - s_p|) dl_p and the cos / sin columns of k s_m.
+void planewave_rhs(const Wave &wv, const std::vector<Seg> &s, double freq,
+                   const std::vector<std::pair<size_t, size_t>> &wires, const Paths *P, fnec_c64 *rhs,
+                   double *cosv, double *sinv)
+{
+    const size_t n = s.size();
+    const double k = 2.0 * kPi * freq / kC0;
+    const double scale = 2.0 * kPi / kEta0;
+    std::vector<size_t> grp(n, 0);
+    std::vector<double> sc(n, 0.0), sg(n, 1.0);
+    std::vector<fnec_c64> et(n);
+    std::vector<const double *> wdir;
+    std::vector<std::array<double, 3>> wmid;
+    if (P) {
+        grp = P->path_of;
+        sc = P->s_of;
+        sg = P->sign_of;
+    } else {
+        for (size_t wi = 0; wi < wires.size(); ++wi) {
+            for (size_t m = wires[wi].first; m <= wires[wi].second && m < n; ++m)
+                grp[m] = wi;
+            const Seg &a = s[wires[wi].first], &b = s[wires[wi].second];
+            wdir.push_back(a.dir);
+            wmid.push_back({(a.mid[0] + b.mid[0]) / 2.0, (a.mid[1] + b.mid[1]) / 2.0,
+                            (a.mid[2] + b.mid[2]) / 2.0});
+        }
+    }
+    for (size_t i = 0; i < n; ++i) {
+        const double *dir = P ? s[i].dir : wdir[grp[i]];
+        if (!P) {
+            const double d[3] = {s[i].mid[0] - wmid[grp[i]][0], s[i].mid[1] - wmid[grp[i]][1],
+                                 s[i].mid[2] - wmid[grp[i]][2]};
+            sc[i] = d[0] * dir[0] + d[1] * dir[1] + d[2] * dir[2];
+        }
+        const double phase = k * (wv.r_hat[0] * s[i].mid[0] + wv.r_hat[1] * s[i].mid[1] +
+                                  wv.r_hat[2] * s[i].mid[2]);
+        double sp, cp;
+        sincos(phase, &sp, &cp);
+        fnec_c64 e = cmul(coupling(wv, dir), fnec_c64{1.0 * cp, 1.0 * sp});
+        if (P)
+            e = fnec_c64{e.re * sg[i], e.im * sg[i]};
+        et[i] = e;
+    }
+    for (size_t m = 0; m < n; ++m) {
+        fnec_c64 acc{0.0, 0.0};
+        for (size_t p = 0; p < n; ++p) {
+            if (grp[p] != grp[m])
+                continue;
+            const double kern = std::sin(k * std::fabs(sc[m] - sc[p]));
+            const double tr = et[p].re * kern * s[p].length, ti = et[p].im * kern * s[p].length;
+            acc.re = acc.re + tr;
+            acc.im = acc.im + ti;
+        }
+        fnec_c64 r = cmul(fnec_c64{0.0, -scale}, acc);
+        if (P)
+            r = fnec_c64{r.re * sg[m], r.im * sg[m]};
+        rhs[m] = r;
+        double sv, cv;
+        sincos(k * sc[m], &sv, &cv);
+        cosv[m] = P ? sg[m] * cv : cv;
+        sinv[m] = P ? sg[m] * sv : sv;
+    }
+}
+
+// build_current_source_shape(_paths) (excitation.rs:350-447): the delta-gap
+// forcing and cos column of the deck with its first EX card replaced by a
+// unit voltage source at (tag, seg) and every other EX card dropped
+// (excitation.rs:365-392).
+int current_source_shape(const std::vector<Card> &cards, const std::vector<Seg> &s, double freq,
+                         uint32_t tag, uint32_t seg, const Paths *P, std::vector<fnec_c64> &shape,
+                         std::vector<double> &cosv, int &src)
+{
+    src = find_seg(s, tag, seg);
+    if (src < 0)
+        return FNEC_INVALID_ARG;
+    std::vector<Card> u;
+    bool placed = false;
+    for (const Card &c : cards) {
+        if (card_is(c, "EX")) {
+            if (!placed) {
+                Card e;
+                e.mn = "EX";
+                e.nf = 6;
+                e.f[0] = 0.0;
+                e.f[1] = (double)tag;
+                e.f[2] = (double)seg;
+                e.f[3] = 0.0;
+                e.f[4] = 1.0;
+                e.f[5] = 0.0;
+                u.push_back(e);
+                placed = true;
+            }
+            continue;
+        }
+        u.push_back(c);
+    }
+    return P ? hallen_rhs_paths(u, s, freq, *P, shape, cosv) : hallen_rhs(u, s, freq, shape, cosv);
+}
+
 // ---- Rust number formatting (nec_report/src/lib.rs:437-482) ----
 
 std::string fmt_fixed(double v, int prec)
@@ -658,15 +820,6 @@ std::string fmt_exp(double v, int prec)
     int ex = std::atoi(e + 1);
     *e = 0;
     return std::string(b) + "e" + std::to_string(ex);
-}
-
-bool deck_unsupported(const std::vector<Card> &cards)
-{
-    for (const Card &c : cards) {
-        if (card_is(c, "EX") && (ex_plane(c) || ex_current(c)))
-            return true;
-    }
-    return false;
 }
 
 // ---- LD / TL / NT stamps on Z (loads.rs:34-223, tl.rs:47-244,
@@ -1048,8 +1201,16 @@ int fnec_deck_build(fnec_deck *d, int semantics, double freq_hz, fnec_deck_probl
     }
     if (fnec_deck_segment_count(d) == 0)
         return FNEC_INVALID_ARG;
-    if (deck_unsupported(d->cards)) {
-        fnec::set_error("plane-wave and current-source excitations stay on the CPU path");
+    // the first plane-wave card wins over current sources (solve_session.rs:1327-1337)
+    int variant = 0;
+    for (const Card &c : d->cards)
+        if (card_is(c, "EX") && ex_plane(c))
+            variant = 1;
+        else if (card_is(c, "EX") && ex_current(c) && !variant)
+            variant = 2;
+    if (variant && semantics != FNEC_SEM_CLI) {
+        fnec::set_error("plane-wave and current-source excitations stay on the CPU path of the worker and "
+                        "the Python binding");
         return FNEC_UNAVAILABLE;
     }
     const std::vector<Seg> &s = d->segs;
@@ -1078,8 +1239,77 @@ int fnec_deck_build(fnec_deck *d, int semantics, double freq_hz, fnec_deck_probl
     }
     d->wf.clear(); d->wl.clear(); d->ja.clear(); d->jb.clear(); d->js.clear();
     d->path_of.clear(); d->free_ends.clear();
-    int route = 0;
-    if (semantics == FNEC_SEM_CLI) {
+    d->sinv.clear(); d->rhs_all.clear(); d->recv_theta.clear(); d->recv_phi.clear();
+    int route = 0, src = -1;
+    fnec_c64 i0{0.0, 0.0};
+    if (variant) {
+        // solve_plane_wave_hallen / solve_current_source_hallen
+        // (solve_session.rs:262-301, 392-448): the conductor-path route when
+        // some path is not trivial, else the raw wire ranges
+        Paths P = conductor_paths(s);
+        const bool paths = P.ok && P.count > 0 && P.nontrivial;
+        auto wires = wire_endpoints(s);
+        if (paths) {
+            route = 1;
+            d->path_of = P.path_of;
+            d->free_ends = P.free_ends;
+        } else
+            for (auto &w : wires) {
+                d->wf.push_back(w.first);
+                d->wl.push_back(w.second);
+            }
+        if (variant == 1) {
+            const Card *ex = nullptr;
+            for (const Card &c : d->cards)
+                if (!ex && card_is(c, "EX") && ex_plane(c))
+                    ex = &c;
+            if (!paths && !detect_junctions(s, wires, 1e-6).empty()) {
+                fnec::set_error("EX: incident plane wave is supported on straight, non-junctioned wires; "
+                                "junctioned geometry is not yet supported");
+                return FNEC_INVALID_ARG;
+            }
+            // the card's own incidence, then the receive sweep's (NTHETA NPHI > 1)
+            std::vector<std::pair<double, double>> inc{{card_f(*ex, 4), card_f(*ex, 5)}};
+            const uint64_t nth = std::max<uint32_t>(ex_tag(*ex), 1), nph = std::max<uint32_t>(ex_seg(*ex), 1);
+            if (nth * nph > 1) {
+                if (nth * nph > 65536) {
+                    fnec::set_error("receive sweep: more than 65536 incidences");
+                    return FNEC_INVALID_ARG;
+                }
+                for (uint64_t it = 0; it < nth; ++it)
+                    for (uint64_t ip = 0; ip < nph; ++ip) {
+                        const double th = card_f(*ex, 4) + (double)it * card_f(*ex, 7);
+                        const double ph = card_f(*ex, 5) + (double)ip * card_f(*ex, 8);
+                        inc.push_back({th, ph});
+                        d->recv_theta.push_back(th);
+                        d->recv_phi.push_back(ph);
+                    }
+            }
+            d->rhs_all.resize(n * inc.size());
+            d->cosv.resize(n);
+            d->sinv.resize(n);
+            for (size_t q = 0; q < inc.size(); ++q)
+                planewave_rhs(plane_wave(*ex, inc[q].first, inc[q].second), s, freq_hz, wires,
+                              paths ? &P : nullptr, d->rhs_all.data() + q * n, d->cosv.data(), d->sinv.data());
+            d->rhs.assign(d->rhs_all.begin(), d->rhs_all.begin() + n);
+        } else {
+            const Card *cs = nullptr;
+            for (const Card &c : d->cards)
+                if (!cs && card_is(c, "EX") && ex_current(c))
+                    cs = &c;
+            if (!paths && !P.ok && !detect_junctions(s, wires, 1e-6).empty()) {
+                fnec::set_error("EX: current source is supported on straight or degree-2 junctioned wires; "
+                                "degree-3+ (T/Y) junctions and closed loops are not yet supported");
+                return FNEC_INVALID_ARG;
+            }
+            if (current_source_shape(d->cards, s, freq_hz, ex_tag(*cs), ex_seg(*cs), paths ? &P : nullptr, d->rhs,
+                                     d->cosv, src)) {
+                fnec::set_error("EX: current-source segment not found");
+                return FNEC_INVALID_ARG;
+            }
+            i0 = fnec_c64{ex_vr(*cs), ex_vi(*cs)};
+        }
+    } else if (semantics == FNEC_SEM_CLI) {
         Paths P = conductor_paths(s);
         if (P.ok && P.count > 0 && P.nontrivial) {
             route = 1;
@@ -1091,7 +1321,7 @@ int fnec_deck_build(fnec_deck *d, int semantics, double freq_hz, fnec_deck_probl
             d->free_ends = P.free_ends;
         }
     }
-    if (route == 0) {
+    if (route == 0 && !variant) {
         if (hallen_rhs(d->cards, s, freq_hz, d->rhs, d->cosv)) {
             fnec::set_error("EX: bad excitation");
             return FNEC_INVALID_ARG;
@@ -1115,7 +1345,7 @@ int fnec_deck_build(fnec_deck *d, int semantics, double freq_hz, fnec_deck_probl
         }
     }
     // feedpoints (solve_session.rs:877-944; nec_worker solve.rs:243-268; fnec_py lib.rs:84-92)
-    d->feed_idx.clear(); d->feed_v.clear(); d->feed_tag.clear(); d->feed_seg.clear();
+    d->feed_idx.clear(); d->feed_v.clear(); d->feed_tag.clear(); d->feed_seg.clear(); d->feed_port.clear();
     for (const Card &c : d->cards) {
         if (!card_is(c, "EX") || ex_plane(c))
             continue;
@@ -1133,6 +1363,7 @@ int fnec_deck_build(fnec_deck *d, int semantics, double freq_hz, fnec_deck_probl
         d->feed_v.push_back(v);
         d->feed_tag.push_back(s[idx].tag);
         d->feed_seg.push_back(s[idx].idx);
+        d->feed_port.push_back(variant == 2 && ex_current(c) ? 1 : 0);
         if (semantics != FNEC_SEM_CLI)
             break;
     }
@@ -1188,6 +1419,16 @@ int fnec_deck_build(fnec_deck *d, int semantics, double freq_hz, fnec_deck_probl
     out->feed_seg = d->feed_seg.empty() ? nullptr : d->feed_seg.data();
     out->seg_tag = d->seg_tag.data();
     out->seg_index = d->seg_index.data();
+    out->variant = variant;
+    out->sin_vec = d->sinv.empty() ? nullptr : d->sinv.data();
+    out->n_rhs = variant == 1 ? d->rhs_all.size() / n : 1;
+    out->rhs_all = variant == 1 ? d->rhs_all.data() : r.rhs;
+    out->n_recv = d->recv_theta.size();
+    out->recv_theta = d->recv_theta.empty() ? nullptr : d->recv_theta.data();
+    out->recv_phi = d->recv_phi.empty() ? nullptr : d->recv_phi.data();
+    out->src = src >= 0 ? (size_t)src : 0;
+    out->i0 = i0;
+    out->feed_port = d->feed_port.empty() ? nullptr : d->feed_port.data();
     return FNEC_OK;
 }
 
@@ -1214,18 +1455,25 @@ int fnec_deck_report(fnec_deck *d, size_t freq_index, int precision, char *outbu
         for (size_t i = 0; i < n; ++i)
             ncons = std::max(ncons, pr.path_of[i] + 1);
     }
-    std::vector<fnec_c64> x(n + std::max<size_t>(ncons, 1));
+    ncons = std::max<size_t>(ncons, 1);
+    // a plane wave carries cos and sin constants per range or path, a current
+    // source the port voltage after its constants
+    const size_t s_len = n + (pr.variant == 1 ? 2 * ncons : ncons + (pr.variant == 2 ? 1 : 0));
+    std::vector<fnec_c64> x(s_len * (pr.variant == 1 ? pr.n_rhs : 1));
     fnec_solve_stats stats;
     st = fnec_deck_solve(&pr, precision, x.data(), &stats);
     if (st)
         return st;
+    const fnec_c64 port_v = pr.variant == 2 ? x[s_len - 1] : fnec_c64{0.0, 0.0};
     std::string o;
     o += "FNEC FEEDPOINT REPORT\nFORMAT_VERSION 1\n";
     o += "FREQ_MHZ " + fmt_fixed(f / 1e6, 6) + "\n";
-    o += "SOLVER_MODE hallen\nPULSE_RHS Nec2\n\nFEEDPOINTS\nTAG SEG V_RE V_IM I_RE I_IM Z_RE Z_IM\n";
+    o += std::string("SOLVER_MODE ") +
+         (pr.variant == 1 ? "hallen-planewave" : pr.variant == 2 ? "hallen-current-source" : "hallen") +
+         "\nPULSE_RHS Nec2\n\nFEEDPOINTS\nTAG SEG V_RE V_IM I_RE I_IM Z_RE Z_IM\n";
     double p_in = 0.0;
     for (size_t i = 0; i < pr.n_feeds; ++i) {
-        fnec_c64 v = pr.feed_v[i], c = x[pr.feed_idx[i]], z;
+        fnec_c64 v = pr.feed_port && pr.feed_port[i] ? port_v : pr.feed_v[i], c = x[pr.feed_idx[i]], z;
         if (std::hypot(c.re, c.im) > 1e-60) {
             double ns = c.re * c.re + c.im * c.im;
             z.re = (v.re * c.re + v.im * c.im) / ns;
@@ -1270,6 +1518,25 @@ int fnec_deck_report(fnec_deck *d, size_t freq_index, int precision, char *outbu
              fmt_exp(c.re, 6) + " " + fmt_exp(c.im, 6) + " " + fmt_exp(std::hypot(c.re, c.im), 6) +
              " " + fmt_fixed(rad2deg(std::atan2(c.im, c.re)), 4) + "\n";
     }
+    // receive pattern (solve_session.rs:313-371): each sweep incidence's peak
+    // |I| over the segments, in dB below the largest (nec_report lib.rs)
+    std::string recv;
+    if (pr.variant == 1 && pr.n_recv) {
+        std::vector<double> peak(pr.n_recv, 0.0);
+        double max_peak = 0.0;
+        for (size_t q = 0; q < pr.n_recv; ++q) {
+            const fnec_c64 *xq = x.data() + (q + 1) * s_len;
+            for (size_t m = 0; m < n; ++m)
+                peak[q] = std::fmax(peak[q], std::hypot(xq[m].re, xq[m].im));
+            max_peak = std::fmax(max_peak, peak[q]);
+        }
+        recv = "\nRECEIVE_PATTERN\nN_POINTS " + std::to_string(pr.n_recv) + "\nTHETA PHI RESPONSE_DB\n";
+        for (size_t q = 0; q < pr.n_recv; ++q) {
+            const double db = peak[q] > 0.0 && max_peak > 0.0 ? 20.0 * std::log10(peak[q] / max_peak) : -999.99;
+            recv += fmt_fixed(pr.recv_theta[q], 4) + " " + fmt_fixed(pr.recv_phi[q], 4) + " " + fmt_fixed(db, 4) +
+                    "\n";
+        }
+    }
     const size_t np = r.rp_npts;
     bool normalize = false, avg = false;
     for (const Card &c : d->cards)
@@ -1309,6 +1576,7 @@ int fnec_deck_report(fnec_deck *d, size_t freq_index, int precision, char *outbu
             o += fmt_fixed(r.rp_theta[i], 4) + " " + fmt_fixed(r.rp_phi[i], 4) + " " +
                  fmt_fixed(rows[i].gain_total_dbi, 4) + " " + fmt_fixed(rows[i].gain_theta_dbi, 4) +
                  " " + fmt_fixed(rows[i].gain_phi_dbi, 4) + " " + fmt_fixed(rows[i].axial_ratio, 4) + "\n";
+        o += recv;
         if (normalize) {
             double peak = -INFINITY;
             for (auto &row : rows)
@@ -1333,7 +1601,8 @@ int fnec_deck_report(fnec_deck *d, size_t freq_index, int precision, char *outbu
             if (den > 0.0)
                 o += "\nAVERAGE_POWER_GAIN " + fmt_fixed(num / den, 4) + "\n";
         }
-    }
+    } else
+        o += recv;
     if (needed)
         *needed = o.size() + 1;
     if (outbuf && cap) {

@@ -6,7 +6,7 @@
 //   k_gram_cols     homogeneous columns, M^H y (linear.rs:735-780).
 //   k_gram_fix      W x W block, constraint rows, lambda (linear.rs:744-785).
 //   (the blocked LU kernels are templates in lu.h, instantiated by engine.hip)
-//   k_backsub       U x = b' (blocked, one workgroup per problem).
+//   k_backsub       U x = b' (blocked, one workgroup per problem and right-hand side).
 //   k_residual_*    ||Z I - C cos - rhs|| and diag spread (solve_session.rs:118-207).
 //   k_rp_*          far field + normalisation integral (farfield.rs:265-647).
 #include <hip/hip_runtime.h>
@@ -587,21 +587,24 @@ __global__ __launch_bounds__(256) void k_gram_fix(const Prob *__restrict__ probs
         P.ar[(size_t)i * P.lda + i] += kLambda;
 }
 
-// Back substitution U x = b' with b' the transformed augmented column.
-// Blocks of 64 rows from the bottom; one workgroup per problem.
+// Back substitution U x = b' with b' a transformed right-hand side column.
+// Blocks of 64 rows from the bottom; one workgroup per (problem, right-hand
+// side).
 constexpr int kBsBlock = 64;
 
 __global__ __launch_bounds__(kBsThreads) void k_backsub(const Prob *__restrict__ probs,
-                                                 const int *__restrict__ plist)
+                                                 const int2 *__restrict__ work)
 {
-    const Prob &P = probs[plist[blockIdx.x]];
+    const int2 wk = work[blockIdx.x];
+    const Prob &P = probs[wk.x];
     if (P.info[0] != 0)
         return;
     const int s = P.s, lda = P.lda, t = threadIdx.x;
-    double *x = P.x;
+    double *x = P.x + 2 * (size_t)s * wk.y;
+    const size_t bcol = (size_t)(s + wk.y) * lda;
     for (int i = t; i < s; i += kBsThreads) {
-        x[2 * i] = P.ar[(size_t)s * lda + i];
-        x[2 * i + 1] = P.ai[(size_t)s * lda + i];
+        x[2 * i] = P.ar[bcol + i];
+        x[2 * i + 1] = P.ai[bcol + i];
     }
     __syncthreads();
     __shared__ double xb_r[kBsBlock], xb_i[kBsBlock];
@@ -677,7 +680,12 @@ __global__ __launch_bounds__(256) void k_residual_rows(const Prob *__restrict__ 
     const int row = wk.y * kResRows + t;
     __shared__ double red[3][256];
     double r2 = 0.0, b2 = 0.0, dmag = 0.0;
-    if (row < P.n) {
+    // the residual is that of the delta-gap system; plane-wave and
+    // current-source solves report only the diagonal spread, as the
+    // reference does (solve_session.rs:1327-1337)
+    if (row < P.n && P.variant != 0) {
+        dmag = hypot(P.zr[(size_t)row * P.ldz + row], P.zi[(size_t)row * P.ldz + row]);
+    } else if (row < P.n) {
         double sr = 0.0, si = 0.0;
         for (int c = 0; c < P.n; ++c) {
             const double zr = P.zr[(size_t)c * P.ldz + row], zi = P.zi[(size_t)c * P.ldz + row];

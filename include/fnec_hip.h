@@ -185,6 +185,47 @@ typedef struct {
     fnec_c64 delta;
 } fnec_stamp;
 
+/* ---- plane-wave (receive) variant ----
+ * Reproduces nec_solver::solve_hallen_planewave (linear.rs:910-985; wire
+ * ranges as in fnec_hip_solve_hallen, n_wires == 0 meaning one range) or, when
+ * path_of is not NULL, solve_hallen_planewave_paths (linear.rs:1017-1091; the
+ * CLI's conductor paths, solve_session.rs:262-301): two homogeneous columns
+ * (-cos, -sin) per range or path, I = 0 rows at both ends of every range or at
+ * the listed free ends, no KCL rows; inputs as build_planewave_hallen(_paths)
+ * returns them (planewave.rs:153-347). n_rhs incidences share the matrix: rhs
+ * holds n_rhs forcing vectors of n (incidence k at rhs + k n) and one
+ * elimination of the normal matrix serves them all, as the receive sweep
+ * (solve_session.rs:313-371) solves every incidence on the same Z; each
+ * incidence's result is bitwise what a solve of it alone gives. x_out: n_rhs
+ * blocks of n + 2 G values (currents, then the cos and sin constants of each
+ * range or path in turn). stamps (may be NULL) are added to Z after the fill as
+ * fnec_request.stamps are. precision: FNEC_PREC_FP64 only (the plain path;
+ * FNEC_UNAVAILABLE otherwise). */
+int fnec_hip_solve_hallen_planewave(const fnec_seg *segs, size_t n, const fnec_c64 *rhs, size_t n_rhs,
+                                    const double *cos_vec, const double *sin_vec,
+                                    const size_t *wire_first, const size_t *wire_last, size_t n_wires,
+                                    const size_t *path_of, const size_t *free_ends, size_t n_free,
+                                    const fnec_stamp *stamps, size_t n_stamps, double freq_hz,
+                                    const fnec_ground *ground, int precision, fnec_c64 *x_out,
+                                    fnec_solve_stats *stats);
+
+/* ---- current-source variant ----
+ * Reproduces nec_solver::solve_hallen_current_source (linear.rs:1124-1210) or,
+ * when path_of is not NULL, solve_hallen_current_source_paths (linear.rs:
+ * 1241-1319): one homogeneous column (-cos) per range or path, the
+ * port-voltage column -shape, and rows weighted by 1e6: I = 0 at both ends of
+ * every range (or at the listed free ends), then I[src] = i0. shape and cos_vec
+ * are build_current_source_shape(_paths)'s (excitation.rs:350-447). x_out:
+ * n + G + 1 values: currents, the G constants, and the port voltage V
+ * (Z_in = V / i0, solve_session.rs:893-897). precision: FNEC_PREC_FP64 only. */
+int fnec_hip_solve_hallen_current_source(const fnec_seg *segs, size_t n, const fnec_c64 *shape,
+                                         const double *cos_vec, size_t src, fnec_c64 i0,
+                                         const size_t *wire_first, const size_t *wire_last,
+                                         size_t n_wires, const size_t *path_of, const size_t *free_ends,
+                                         size_t n_free, const fnec_stamp *stamps, size_t n_stamps,
+                                         double freq_hz, const fnec_ground *ground, int precision,
+                                         fnec_c64 *x_out, fnec_solve_stats *stats);
+
 /* ---- batched solves (sweeps and many geometries in one submission) ----
  * One request = one Hallén solve (+ optional RP grid). Arrays are borrowed for
  * the duration of the call. */

@@ -15,9 +15,9 @@
  *   render_text_report                      crates/nec_report/src/lib.rs:307-482
  *   LD / TL / NT impedance stamps           crates/nec_solver/src/loads.rs,
  *                                           tl.rs, network.rs
- * Plane-wave and current-source EX make fnec_deck_build return
- * FNEC_UNAVAILABLE, the signal on which the reference's callers keep the CPU
- * path.
+ * Plane-wave and current-source EX (solve_session.rs:262-448) are built and
+ * solved for the CLI; for the worker and the Python binding, which do not solve
+ * them, fnec_deck_build returns FNEC_UNAVAILABLE.
  */
 #ifndef FNEC_HOST_H
 #define FNEC_HOST_H
@@ -63,6 +63,29 @@ typedef struct {
     const uint32_t *feed_seg;
     const uint32_t *seg_tag;   /* per segment tag and 1-based index */
     const uint32_t *seg_index;
+    /* Plane-wave and current-source decks (CLI semantics only; the first
+     * plane-wave EX card wins over current sources, solve_session.rs:
+     * 1327-1337): variant 1 plane wave, 2 current source, 0 delta gap. Route
+     * 0 then uses the raw wire ranges (wire_endpoints_from_segs) with no KCL
+     * rows, route 1 the conductor paths. */
+    int variant;
+    /* variant 1: req.rhs / req.cos_vec / sin_vec are build_planewave_hallen(_paths)'s
+     * at the card's incidence; rhs_all holds n_rhs forcing vectors of n, the
+     * first that one, then one per incidence of the receive sweep
+     * (solve_session.rs:313-371) at (recv_theta, recv_phi), theta outer. */
+    const double *sin_vec;
+    size_t n_rhs;
+    const fnec_c64 *rhs_all;
+    size_t n_recv;
+    const double *recv_theta;
+    const double *recv_phi;
+    /* variant 2: req.rhs / req.cos_vec are build_current_source_shape(_paths)'s,
+     * src the source segment and i0 the forced current of the first EX 4
+     * card; feed_port[i] marks the feed rows (every EX 4 card) whose voltage
+     * is the solved port voltage (solve_session.rs:893-897). */
+    size_t src;
+    fnec_c64 i0;
+    const unsigned char *feed_port;
 } fnec_deck_problem;
 
 int fnec_deck_build(fnec_deck *d, int semantics, double freq_hz, fnec_deck_problem *out);
@@ -71,7 +94,10 @@ int fnec_deck_build(fnec_deck *d, int semantics, double freq_hz, fnec_deck_probl
  * fnec_hip_solve_hallen_paths, with the request's LD/TL/NT stamps added to Z
  * after the fill (solve_session.rs:1285-1309). x_out holds n currents followed
  * by the route's homogeneous constants: route 0 max(n_wires, 1), route 1 one
- * per conductor path (max path_of + 1). */
+ * per conductor path (max path_of + 1). Variant 1 solves as
+ * fnec_hip_solve_hallen_planewave (x_out: n_rhs blocks of n + 2 G values, G
+ * ranges or paths), variant 2 as fnec_hip_solve_hallen_current_source (n + G +
+ * 1 values, the port voltage last); both need FNEC_PREC_FP64. */
 int fnec_deck_solve(const fnec_deck_problem *pr, int precision, fnec_c64 *x_out,
                     fnec_solve_stats *stats);
 

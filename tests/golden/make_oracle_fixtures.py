@@ -72,6 +72,7 @@ if __name__ == "__main__":
         run("config2", W.config2_deck())
     if "config3" in which:
         # N = 8192, worker semantics (one KCL row), the 5-degree RP grid
-        run("config3", W.config3_deck(), 0.0, "worker", threads=os.cpu_count() or 1)
+        run("config3", W.config3_deck(), 0.0, "worker",
+            threads=int(os.environ.get("FIXTURE_THREADS", os.cpu_count() or 1)))
     if "config4a_f0" in which:
         run("config4a_f0", W.config4a_deck(), W.config4a_frequencies()[0])

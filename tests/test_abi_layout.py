@@ -17,12 +17,14 @@ STRUCTS = {
     "fnec_result": F.Result,
     "fnec_solve_stats": F.SolveStats,
     "fnec_ground": F.Ground,
+    "fnec_deck_problem": F.DeckProblem,
 }
 
 
 @pytest.mark.skipif(shutil.which("gcc") is None, reason="gcc not found")
 def test_ctypes_structs_match_the_header(tmp_path):
-    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "fnec_hip.h"', "int main(void) {"]
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "fnec_hip.h"', '#include "fnec_host.h"',
+             "int main(void) {"]
     for cname, py in STRUCTS.items():
         lines.append(f'printf("{cname} size %zu\\n", sizeof({cname}));')
         for fname, _ in py._fields_:

@@ -139,11 +139,90 @@ def test_config3_has_one_kcl_row():
     assert len(p["segs"]) == 8192 and len(p["ja"]) == 1 and p["js"][0] == -1.0
 
 
-def test_unsupported_cards_signal_cpu_fallback():
+def test_plane_wave_and_current_source_stay_on_the_cpu_for_worker_and_python():
     for ex in ("EX 1 1 1 0 0 0 0", "EX 4 1 26 0 1 0"):
         txt = deck_text("dipole-freesp-rp-51seg").replace("EN", ex + "\nEN")
-        with pytest.raises(F.FnecUnavailable):
-            F.Deck(txt).problem("cli", 14.2e6)
+        for sem in ("worker", "py"):
+            with pytest.raises(F.FnecUnavailable):
+                F.Deck(txt).problem(sem, 14.2e6)
+        assert F.Deck(txt).problem("cli", 14.2e6)["variant"] in (1, 2)
+
+
+def _vdeck(geom, ex, mhz=14.2, extra=""):
+    return f"CE\n{geom}\nGE\n{extra}{ex}\nFR 0 1 0 0 {mhz!r} 0\nEN\n"
+
+
+DIPOLE_G = "GW 1 51 0 0 -5.282 0 0 5.282 0.001"
+TWO_G = "GW 1 21 0 0 -2.5 0 0 2.5 0.001\nGW 2 21 1 0 -2.5 1 0 2.5 0.001"
+SPLIT_G = "GW 1 26 0 0 0 0 0 5.282 0.001\nGW 2 26 0 0 0 0 0 -5.282 0.001"
+VEE_G = "GW 1 10 0 0 5 0 3 1 0.001\nGW 2 10 0 0 5 0 -3 1 0.001"
+TILT_G = "GW 1 41 -3 -3 0 3 3 0 0.001"
+# plane waves (linear, right / left elliptic; per wire and on conductor
+# paths) and current sources (per wire, offset feed, paths, with a load)
+VARIANT_DECKS = [
+    _vdeck(DIPOLE_G, "EX 1 1 1 0 30 0 0"),
+    _vdeck(DIPOLE_G, "EX 1 1 1 0 90 0 0"),
+    _vdeck(TILT_G, "EX 2 1 1 0 45 20 10 0 0 0.5"),
+    _vdeck(TILT_G, "EX 3 1 1 0 45 0 0 0 0 0.5"),
+    _vdeck(TWO_G, "EX 1 1 1 0 90 90 0", 28.0),
+    _vdeck(SPLIT_G, "EX 1 1 1 0 55 0 0"),
+    _vdeck(VEE_G, "EX 1 1 1 0 60 30 15"),
+    _vdeck(DIPOLE_G, "EX 4 1 26 0 1 0"),
+    _vdeck(DIPOLE_G, "EX 4 1 15 0 2 -1"),
+    _vdeck(TWO_G, "EX 4 1 11 0 1 0", 28.0),
+    _vdeck(VEE_G, "EX 4 1 1 0 1 0"),
+    _vdeck(SPLIT_G, "EX 4 1 1 0 1 0.5"),
+    _vdeck(DIPOLE_G, "EX 4 1 26 0 1 0", 14.2, "LD 0 1 10 12 5 2e-6 1e-12\n"),
+]
+
+
+@pytest.mark.parametrize("txt", VARIANT_DECKS)
+def test_variant_inputs_equal_the_oracle_bitwise(oracle, txt):
+    """The host builders of build_planewave_hallen(_paths) and
+    build_current_source_shape(_paths) and the CLI's route choice
+    (solve_session.rs:262-448) give the oracle's doubles bit for bit."""
+    d = F.Deck(txt)
+    f = d.frequencies("cli")[0]
+    p = d.problem("cli", f)
+    v = oracle.variant_inputs(txt, f)
+    assert p["variant"] == {"plane-wave": 1, "current-source": 2}[v["kind"]]
+    assert p["route"] == v["route"]
+    for a, b in ((p["rhs"], v["rhs"]), (p["cos"], v["cos"])):
+        assert a.tobytes() == b.tobytes()
+    if p["variant"] == 1:
+        assert p["sin"].tobytes() == v["sin"].tobytes()
+        assert p["rhs_all"].shape[0] == 1 and p["rhs_all"][0].tobytes() == v["rhs"].tobytes()
+    else:
+        assert p["src"] == v["src"] and p["i0"] == v["i0"]
+        assert list(p["feed_port"]) == [1]
+    if p["route"] == 1:
+        assert list(p["path_of"]) == list(v["path_of"]) and list(p["free_ends"]) == list(v["free_ends"])
+    else:
+        assert list(p["wf"]) == list(v["wf"]) and list(p["wl"]) == list(v["wl"]) and len(p["ja"]) == 0
+
+
+def test_receive_sweep_inputs_equal_the_oracle_per_incidence(oracle):
+    """Each sweep incidence's forcing equals the oracle's for the deck with
+    the card set to that incidence (solve_session.rs:340-352), theta outer."""
+    txt = _vdeck(TILT_G, "EX 2 4 3 0 10 5 20 25 40 0.3")
+    p = F.Deck(txt).problem("cli", 14.2e6)
+    assert p["rhs_all"].shape == (13, 41)
+    th = [10.0 + it * 25.0 for it in range(4) for _ in range(3)]
+    ph = [5.0 + ip * 40.0 for _ in range(4) for ip in range(3)]
+    assert list(p["recv_theta"]) == th and list(p["recv_phi"]) == ph
+    for q in range(12):
+        one = _vdeck(TILT_G, f"EX 2 1 1 0 {th[q]!r} {ph[q]!r} 20 0 0 0.3")
+        assert p["rhs_all"][q + 1].tobytes() == oracle.variant_inputs(one, 14.2e6)["rhs"].tobytes()
+    assert p["rhs_all"][0].tobytes() == p["rhs"].tobytes()
+
+
+def test_variant_topology_errors():
+    # a degree-3 junction (no conductor paths): neither a plane wave nor a
+    # current source is solvable, as the oracle also refuses
+    tee = "GW 1 10 0 0 -1 0 0 0 0.001\nGW 2 10 0 0 0 0 0 1 0.001\nGW 3 10 0 0 0 0 1 0 0.001"
+    for ex in ("EX 1 1 1 0 30 0 0", "EX 4 1 5 0 1 0"):
+        with pytest.raises(F.FnecError):
+            F.Deck(_vdeck(tee, ex)).problem("cli", 14.2e6)
 
 
 def test_parse_errors():
