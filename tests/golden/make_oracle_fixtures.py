@@ -74,5 +74,19 @@ if __name__ == "__main__":
         # N = 8192, worker semantics (one KCL row), the 5-degree RP grid
         run("config3", W.config3_deck(), 0.0, "worker",
             threads=int(os.environ.get("FIXTURE_THREADS", os.cpu_count() or 1)))
+    if "config3-rp" in which:
+        # the oracle's pattern (gains and normalisation) on the deck's 5-degree
+        # grid from the oracle's currents, added to an existing config3 fixture
+        f = os.path.join(OUT, "config3.npz")
+        d = dict(np.load(f))
+        txt = W.config3_deck()
+        inp = O.deck_inputs(txt, float(d["freq_hz"]), "cli")
+        th = np.array([5.0 * it for ip in range(73) for it in range(37)])
+        ph = np.array([5.0 * ip for ip in range(73) for it in range(37)])
+        t0 = time.time()
+        rows, total = O.radiation_pattern(inp["segs"], d["x"][:8192], float(d["freq_hz"]), th, ph)
+        d.update(rp=rows, rp_theta=th, rp_phi=ph, rp_total=total)
+        np.savez_compressed(f, **d)
+        print("config3 rp", rows.shape, "seconds %.1f" % (time.time() - t0), flush=True)
     if "config4a_f0" in which:
         run("config4a_f0", W.config4a_deck(), W.config4a_frequencies()[0])

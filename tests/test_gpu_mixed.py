@@ -199,6 +199,41 @@ def test_config3_mixed_large_solve():
     assert st["backward_error"] <= 1e-12
 
 
+CONFIG3_FIXTURE = os.path.join(GOLDEN, "oracle", "config3.npz")
+
+
+@pytest.mark.skipif(not os.path.exists(CONFIG3_FIXTURE), reason="config 3 oracle fixture not generated")
+def test_config3_against_the_fp64_oracle():
+    """Configuration 3 against the FP64 oracle (tests/golden/oracle/config3.npz:
+    the oracle's solution, Z_in and 5-degree pattern, and the extended-precision
+    solution of the same system): the mixed path's Z_in, currents and gains,
+    and the plain path bit for bit with the reference-order back substitution."""
+    fx = np.load(CONFIG3_FIXTURE)
+    n = 8192
+    d = F.Deck(W.config3_deck(rp=False))
+    p = d.problem("worker", 7e6)
+    fi = int(p["feed_index"])
+    xm, st = F.solve_problem(p, precision=F.PREC_MIXED, with_stats=True)
+    z_o = fx["feed_z"][0]
+    dz = abs(p["v_feed"] / xm[fi] - z_o) / abs(z_o)
+    di = rel(xm[:n], fx["x"][:n])
+    e_m, e_o = rel(xm[:n], fx["x_exact"][:n]), rel(fx["x"][:n], fx["x_exact"][:n])
+    rows, _ = F.rp(p["segs"], xm[:n], 7e6, fx["rp_theta"], fx["rp_phi"], p["ground_struct"])
+    live = fx["rp"][:, 0] > -999.0
+    assert np.array_equal(rows[:, 0] > -999.0, live)
+    gl, gl_o = 10.0 ** (rows[live, 0] / 10.0), 10.0 ** (fx["rp"][live, 0] / 10.0)
+    dg = float(np.max(np.abs(gl - gl_o)) / np.max(gl_o))
+    print(f"config 3 vs the FP64 oracle: rel dZin {dz:.2e}, rel dI {di:.2e}, max rel dG {dg:.2e}; "
+          f"vs exact: mixed {e_m:.2e}, oracle {e_o:.2e}; its {st['refine_iters']}", flush=True)
+    assert dz <= 1e-6 and di <= 1e-6 and dg <= 1e-6
+    F.set_tunable("exact_backsub", 1)
+    try:
+        x64 = F.solve_problem(p, precision=F.PREC_FP64)
+    finally:
+        F.set_tunable("exact_backsub", 0)
+    assert np.array_equal(x64.view(np.uint64), fx["x"].view(np.uint64))
+
+
 def test_mixed_with_row_interchanges():
     """Segments half a radius long: the FP32 LU of Z interchanges rows
     (kappa(Z) ~ 3e3), which exercises the factor permutation in the solves."""

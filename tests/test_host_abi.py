@@ -173,6 +173,10 @@ VARIANT_DECKS = [
     _vdeck(VEE_G, "EX 4 1 1 0 1 0"),
     _vdeck(SPLIT_G, "EX 4 1 1 0 1 0.5"),
     _vdeck(DIPOLE_G, "EX 4 1 26 0 1 0", 14.2, "LD 0 1 10 12 5 2e-6 1e-12\n"),
+    _vdeck(DIPOLE_G, "EX 1 1 1 0 60 0 0", 14.2, "LD 0 1 20 32 10 0 0\n"),
+    _vdeck("GW 1 41 0 -5 10 0 5 10 0.001\nGW 2 41 3 -5 10 3 5 10 0.001", "EX 1 1 1 0 40 30 90", 14.2,
+           "GN 1\n"),
+    _vdeck("GW 1 41 0 -5 10 0 5 10 0.001", "EX 4 1 21 0 1 0", 14.2, "GN 0 0 0 0 13 0.005\n"),
 ]
 
 
