@@ -696,19 +696,23 @@ fnec_c64 coupling(const Wave &w, const double d[3])
 }
 
 // build_planewave_hallen (planewave.rs:153-243) on the raw wire ranges, or
-// build_planewave_hallen_paths (planewave.rs:271-347) when P is given: the
-// forcing rhs(s_m) = -j (2 pi / eta0) sum_{p on m's wire or path} E_t(s_p)
-// sin(k |s_m - s_p|) dl_p and the cos / sin columns of k s_m.
-void planewave_rhs(const Wave &wv, const std::vector<Seg> &s, double freq,
+// build_planewave_hallen_paths (planewave.rs:271-347) when P is given: for
+// each incident wave q, the forcing rhs_q(s_m) = -j (2 pi / eta0) sum_{p on
+// m's wire or path} E_t,q(s_p) sin(k |s_m - s_p|) dl_p (into rhs + q n), and
+// the cos / sin columns of k s_m (the same for every wave). The kernel
+// sin(k |s_m - s_p|) does not depend on the wave: each row's values are
+// computed once and serve every wave (the receive sweep's incidences), in the
+// reference's operation order for each.
+void planewave_rhs(const std::vector<Wave> &waves, const std::vector<Seg> &s, double freq,
                    const std::vector<std::pair<size_t, size_t>> &wires, const Paths *P, fnec_c64 *rhs,
                    double *cosv, double *sinv)
 {
-    const size_t n = s.size();
+    const size_t n = s.size(), nq = waves.size();
     const double k = 2.0 * kPi * freq / kC0;
     const double scale = 2.0 * kPi / kEta0;
     std::vector<size_t> grp(n, 0);
     std::vector<double> sc(n, 0.0), sg(n, 1.0);
-    std::vector<fnec_c64> et(n);
+    std::vector<fnec_c64> et(n * nq);
     std::vector<const double *> wdir;
     std::vector<std::array<double, 3>> wmid;
     if (P) {
@@ -725,36 +729,51 @@ void planewave_rhs(const Wave &wv, const std::vector<Seg> &s, double freq,
                             (a.mid[2] + b.mid[2]) / 2.0});
         }
     }
+    size_t ngrp = 0;
     for (size_t i = 0; i < n; ++i) {
+        ngrp = std::max(ngrp, grp[i] + 1);
         const double *dir = P ? s[i].dir : wdir[grp[i]];
         if (!P) {
             const double d[3] = {s[i].mid[0] - wmid[grp[i]][0], s[i].mid[1] - wmid[grp[i]][1],
                                  s[i].mid[2] - wmid[grp[i]][2]};
             sc[i] = d[0] * dir[0] + d[1] * dir[1] + d[2] * dir[2];
         }
-        const double phase = k * (wv.r_hat[0] * s[i].mid[0] + wv.r_hat[1] * s[i].mid[1] +
-                                  wv.r_hat[2] * s[i].mid[2]);
-        double sp, cp;
-        sincos(phase, &sp, &cp);
-        fnec_c64 e = cmul(coupling(wv, dir), fnec_c64{1.0 * cp, 1.0 * sp});
-        if (P)
-            e = fnec_c64{e.re * sg[i], e.im * sg[i]};
-        et[i] = e;
-    }
-    for (size_t m = 0; m < n; ++m) {
-        fnec_c64 acc{0.0, 0.0};
-        for (size_t p = 0; p < n; ++p) {
-            if (grp[p] != grp[m])
-                continue;
-            const double kern = std::sin(k * std::fabs(sc[m] - sc[p]));
-            const double tr = et[p].re * kern * s[p].length, ti = et[p].im * kern * s[p].length;
-            acc.re = acc.re + tr;
-            acc.im = acc.im + ti;
+        for (size_t q = 0; q < nq; ++q) {
+            const Wave &wv = waves[q];
+            const double phase = k * (wv.r_hat[0] * s[i].mid[0] + wv.r_hat[1] * s[i].mid[1] +
+                                      wv.r_hat[2] * s[i].mid[2]);
+            double sp, cp;
+            sincos(phase, &sp, &cp);
+            fnec_c64 e = cmul(coupling(wv, dir), fnec_c64{1.0 * cp, 1.0 * sp});
+            if (P)
+                e = fnec_c64{e.re * sg[i], e.im * sg[i]};
+            et[q * n + i] = e;
         }
-        fnec_c64 r = cmul(fnec_c64{0.0, -scale}, acc);
-        if (P)
-            r = fnec_c64{r.re * sg[m], r.im * sg[m]};
-        rhs[m] = r;
+    }
+    // members of each wire or path in ascending order (the reference's p loop)
+    std::vector<std::vector<size_t>> members(ngrp);
+    for (size_t p = 0; p < n; ++p)
+        members[grp[p]].push_back(p);
+    std::vector<double> kern;
+    for (size_t m = 0; m < n; ++m) {
+        const std::vector<size_t> &mem = members[grp[m]];
+        kern.resize(mem.size());
+        for (size_t j = 0; j < mem.size(); ++j)
+            kern[j] = std::sin(k * std::fabs(sc[m] - sc[mem[j]]));
+        for (size_t q = 0; q < nq; ++q) {
+            const fnec_c64 *e = et.data() + q * n;
+            fnec_c64 acc{0.0, 0.0};
+            for (size_t j = 0; j < mem.size(); ++j) {
+                const size_t p = mem[j];
+                const double tr = e[p].re * kern[j] * s[p].length, ti = e[p].im * kern[j] * s[p].length;
+                acc.re = acc.re + tr;
+                acc.im = acc.im + ti;
+            }
+            fnec_c64 r = cmul(fnec_c64{0.0, -scale}, acc);
+            if (P)
+                r = fnec_c64{r.re * sg[m], r.im * sg[m]};
+            rhs[q * n + m] = r;
+        }
         double sv, cv;
         sincos(k * sc[m], &sv, &cv);
         cosv[m] = P ? sg[m] * cv : cv;
@@ -1288,9 +1307,11 @@ int fnec_deck_build(fnec_deck *d, int semantics, double freq_hz, fnec_deck_probl
             d->rhs_all.resize(n * inc.size());
             d->cosv.resize(n);
             d->sinv.resize(n);
-            for (size_t q = 0; q < inc.size(); ++q)
-                planewave_rhs(plane_wave(*ex, inc[q].first, inc[q].second), s, freq_hz, wires,
-                              paths ? &P : nullptr, d->rhs_all.data() + q * n, d->cosv.data(), d->sinv.data());
+            std::vector<Wave> waves;
+            for (auto &a : inc)
+                waves.push_back(plane_wave(*ex, a.first, a.second));
+            planewave_rhs(waves, s, freq_hz, wires, paths ? &P : nullptr, d->rhs_all.data(), d->cosv.data(),
+                          d->sinv.data());
             d->rhs.assign(d->rhs_all.begin(), d->rhs_all.begin() + n);
         } else {
             const Card *cs = nullptr;
