@@ -30,6 +30,15 @@
 //   solver_config.precision  "fp64" (the reference's algorithm and rounding;
 //                               the default when exec is "cpu"), "mixed" (the
 //                               default when exec is "gpu") or "fp64_mfma".
+//
+// Result cache (--cache N, off by default): results of solved tasks are kept,
+// up to N, and evicted oldest first (ResultCache, crates/nec_worker/src/
+// result_cache.rs:27-108). The key is the reference's cache_key
+// (result_cache.rs:11-22: SHA-256 of the deck text, basis, ground model and
+// the frequency's bits) extended by what changes this worker's reply: the
+// precision, return_currents and the rp angles. A task whose key is cached is
+// answered at once with the stored reply under its own task_id, marked
+// "cached": true. Only "ok" replies are stored.
 // exec_used is always "gpu": the worker has no CPU solver. LD and TL cards
 // are stamped into Z on the device (NT cards are ignored, as the reference's
 // worker ignores them); decks the device path does not cover (plane-wave or
@@ -43,8 +52,10 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <deque>
 #include <iostream>
 #include <map>
+#include <unordered_map>
 #include <memory>
 #include <mutex>
 #include <string>
@@ -497,6 +508,70 @@ double vswr(double re, double im, double z0)
 
 // ---- the worker ----
 
+// ---- SHA-256 (FIPS 180-4) and the result-cache key ----
+std::string sha256_hex(const std::string &msg)
+{
+    static const uint32_t K[64] = {
+        0x428a2f98, 0x71374491, 0xb5c0fbcf, 0xe9b5dba5, 0x3956c25b, 0x59f111f1, 0x923f82a4, 0xab1c5ed5,
+        0xd807aa98, 0x12835b01, 0x243185be, 0x550c7dc3, 0x72be5d74, 0x80deb1fe, 0x9bdc06a7, 0xc19bf174,
+        0xe49b69c1, 0xefbe4786, 0x0fc19dc6, 0x240ca1cc, 0x2de92c6f, 0x4a7484aa, 0x5cb0a9dc, 0x76f988da,
+        0x983e5152, 0xa831c66d, 0xb00327c8, 0xbf597fc7, 0xc6e00bf3, 0xd5a79147, 0x06ca6351, 0x14292967,
+        0x27b70a85, 0x2e1b2138, 0x4d2c6dfc, 0x53380d13, 0x650a7354, 0x766a0abb, 0x81c2c92e, 0x92722c85,
+        0xa2bfe8a1, 0xa81a664b, 0xc24b8b70, 0xc76c51a3, 0xd192e819, 0xd6990624, 0xf40e3585, 0x106aa070,
+        0x19a4c116, 0x1e376c08, 0x2748774c, 0x34b0bcb5, 0x391c0cb3, 0x4ed8aa4a, 0x5b9cca4f, 0x682e6ff3,
+        0x748f82ee, 0x78a5636f, 0x84c87814, 0x8cc70208, 0x90befffa, 0xa4506ceb, 0xbef9a3f7, 0xc67178f2};
+    uint32_t h[8] = {0x6a09e667, 0xbb67ae85, 0x3c6ef372, 0xa54ff53a, 0x510e527f, 0x9b05688c, 0x1f83d9ab, 0x5be0cd19};
+    std::string m = msg;
+    const uint64_t bits = (uint64_t)msg.size() * 8;
+    m.push_back((char)0x80);
+    while (m.size() % 64 != 56)
+        m.push_back('\0');
+    for (int i = 7; i >= 0; --i)
+        m.push_back((char)((bits >> (8 * i)) & 0xff));
+    auto rotr = [](uint32_t x, int n) { return (x >> n) | (x << (32 - n)); };
+    for (size_t blk = 0; blk < m.size(); blk += 64) {
+        uint32_t w[64];
+        for (int t = 0; t < 16; ++t)
+            w[t] = (uint32_t)(unsigned char)m[blk + 4 * t] << 24 | (uint32_t)(unsigned char)m[blk + 4 * t + 1] << 16 |
+                   (uint32_t)(unsigned char)m[blk + 4 * t + 2] << 8 | (uint32_t)(unsigned char)m[blk + 4 * t + 3];
+        for (int t = 16; t < 64; ++t) {
+            const uint32_t s0 = rotr(w[t - 15], 7) ^ rotr(w[t - 15], 18) ^ (w[t - 15] >> 3);
+            const uint32_t s1 = rotr(w[t - 2], 17) ^ rotr(w[t - 2], 19) ^ (w[t - 2] >> 10);
+            w[t] = w[t - 16] + s0 + w[t - 7] + s1;
+        }
+        uint32_t a = h[0], b = h[1], c = h[2], d = h[3], e = h[4], f = h[5], g = h[6], hh = h[7];
+        for (int t = 0; t < 64; ++t) {
+            const uint32_t t1 = hh + (rotr(e, 6) ^ rotr(e, 11) ^ rotr(e, 25)) + ((e & f) ^ (~e & g)) + K[t] + w[t];
+            const uint32_t t2 = (rotr(a, 2) ^ rotr(a, 13) ^ rotr(a, 22)) + ((a & b) ^ (a & c) ^ (b & c));
+            hh = g;
+            g = f;
+            f = e;
+            e = d + t1;
+            d = c;
+            c = b;
+            b = a;
+            a = t1 + t2;
+        }
+        h[0] += a, h[1] += b, h[2] += c, h[3] += d, h[4] += e, h[5] += f, h[6] += g, h[7] += hh;
+    }
+    char out[65];
+    for (int i = 0; i < 8; ++i)
+        std::snprintf(out + 8 * i, 9, "%08x", h[i]);
+    return std::string(out, 64);
+}
+
+// cache_key (result_cache.rs:11-22): SHA-256 of deck \0 basis \0
+// ground_model \0 the frequency's IEEE bits as 16 lower-case hex digits
+std::string cache_key(const std::string &deck, const std::string &basis, const std::string &ground, double freq)
+{
+    uint64_t b;
+    std::memcpy(&b, &freq, 8);
+    char fb[17];
+    std::snprintf(fb, sizeof fb, "%016llx", (unsigned long long)b);
+    return sha256_hex(deck + std::string(1, '\0') + basis + std::string(1, '\0') + ground + std::string(1, '\0') +
+                      fb);
+}
+
 std::mutex g_out_mu;
 void emit(const std::string &line)
 {
@@ -513,6 +588,7 @@ std::string error_line(const std::string &task_id, double freq, const char *code
 }
 
 struct Task {
+    std::string key; // result-cache key (empty: the cache is off)
     std::string task_id;
     double freq = 0.0;
     fnec_c64 v_feed{0.0, 0.0};
@@ -566,6 +642,7 @@ struct Options {
     double budget_gib = 64.0;
     int max_batch = 256;
     int max_queue = 4096;
+    size_t cache = 0; // result-cache capacity (0: off)
 };
 
 class Worker {
@@ -590,6 +667,21 @@ class Worker {
             if (deck)
                 fnec_deck_free(deck);
             return;
+        }
+        if (!task->key.empty()) {
+            std::string tail;
+            {
+                std::lock_guard<std::mutex> lk(cache_mu_);
+                auto it = cache_.find(task->key);
+                if (it != cache_.end())
+                    tail = it->second;
+            }
+            if (!tail.empty()) {
+                fnec_deck_free(deck);
+                emit("{\"status\":\"ok\",\"task_id\":" + jstr(task->task_id) + tail.substr(0, tail.size() - 1) +
+                     ",\"cached\":true}");
+                return;
+            }
         }
         submit(std::move(task), deck);
         if (opt_.serial)
@@ -631,7 +723,7 @@ class Worker {
             msg = "failed to deserialize task: invalid type";
             return "parse_error";
         }
-        std::string basis = "hallen", exec = "cpu", precision;
+        std::string basis = "hallen", exec = "cpu", precision, ground = "none";
         for (auto &kv : cfg->obj) {
             if (kv.first == "basis" || kv.first == "ground_model" || kv.first == "exec" || kv.first == "precision") {
                 if (kv.second.kind != JVal::Str) {
@@ -640,6 +732,8 @@ class Worker {
                 }
                 if (kv.first == "basis")
                     basis = kv.second.str;
+                else if (kv.first == "ground_model")
+                    ground = kv.second.str;
                 else if (kv.first == "exec")
                     exec = kv.second.str;
                 else if (kv.first == "precision")
@@ -692,6 +786,20 @@ class Worker {
                 t.theta.push_back(th->arr[k].num);
                 t.phi.push_back(ph->arr[k].num);
             }
+        }
+        if (opt_.cache) {
+            // the reference's key, then this worker's reply options
+            std::string ext = cache_key(bytes, basis, ground, t.freq) + "|" + precision_name(t.precision) + "|" +
+                              (t.want_currents ? "1" : "0");
+            for (size_t k = 0; k < t.theta.size(); ++k) {
+                char b[64];
+                uint64_t u, w;
+                std::memcpy(&u, &t.theta[k], 8);
+                std::memcpy(&w, &t.phi[k], 8);
+                std::snprintf(b, sizeof b, "|%016llx%016llx", (unsigned long long)u, (unsigned long long)w);
+                ext += b;
+            }
+            t.key = sha256_hex(ext);
         }
         // parse, geometry, Hallén inputs (solve.rs:162-197)
         std::string text(bytes);
@@ -830,9 +938,12 @@ class Worker {
                         t = std::move(it->second);
                         tasks_.erase(it);
                     }
-                    if (r.status == FNEC_OK)
-                        emit(ok_line(*t, r));
-                    else if (r.status == FNEC_SINGULAR)
+                    if (r.status == FNEC_OK) {
+                        const std::string line = ok_line(*t, r);
+                        if (!t->key.empty())
+                            cache_put(t->key, line.substr(line.find(",\"frequency_hz\"")));
+                        emit(line);
+                    } else if (r.status == FNEC_SINGULAR)
                         emit(error_line(t->task_id, t->freq, "singular_matrix", "impedance matrix is singular"));
                     else if (r.status == FNEC_RESOURCE_EXHAUSTED)
                         emit(error_line(t->task_id, t->freq, "resource_exhausted", "device memory exhausted"));
@@ -847,7 +958,28 @@ class Worker {
         }
     }
 
+    // FIFO result cache (ResultCache::insert, result_cache.rs:76-96): a key
+    // already present is replaced in place; at capacity the oldest goes first
+    void cache_put(const std::string &key, const std::string &tail)
+    {
+        std::lock_guard<std::mutex> lk(cache_mu_);
+        auto it = cache_.find(key);
+        if (it != cache_.end()) {
+            it->second = tail;
+            return;
+        }
+        while (cache_.size() >= opt_.cache && !cache_order_.empty()) {
+            cache_.erase(cache_order_.front());
+            cache_order_.pop_front();
+        }
+        cache_order_.push_back(key);
+        cache_[key] = tail;
+    }
+
     Options opt_;
+    std::mutex cache_mu_;
+    std::unordered_map<std::string, std::string> cache_; // key -> reply after its task_id
+    std::deque<std::string> cache_order_;
     fnec_request req_{};
     std::mutex mu_;
     std::condition_variable cv_done_;
@@ -883,7 +1015,9 @@ int usage()
 {
     std::fprintf(stderr,
                  "usage: fnec_hip_worker [--serve | --serial] [--budget-gib G] [--max-batch B] [--max-queue Q]\n"
-                 "       fnec_hip_worker --vswr RE IM [Z0]\n");
+                 "                       [--cache N]\n"
+                 "       fnec_hip_worker --vswr RE IM [Z0]\n"
+                 "       fnec_hip_worker --cache-key BASIS GROUND_MODEL FREQ_HZ < deck\n");
     return 2;
 }
 
@@ -904,6 +1038,13 @@ int main(int argc, char **argv)
             opt.max_batch = std::atoi(argv[++a]);
         } else if (s == "--max-queue" && a + 1 < argc) {
             opt.max_queue = std::max(1, std::atoi(argv[++a]));
+        } else if (s == "--cache" && a + 1 < argc) {
+            opt.cache = (size_t)std::max(0, std::atoi(argv[++a]));
+        } else if (s == "--cache-key" && a + 3 < argc) {
+            // the reference's cache_key of the deck text on stdin
+            std::string deck((std::istreambuf_iterator<char>(std::cin)), std::istreambuf_iterator<char>());
+            std::printf("%s\n", cache_key(deck, argv[a + 1], argv[a + 2], std::atof(argv[a + 3])).c_str());
+            return 0;
         } else if (s == "--vswr" && a + 2 < argc) {
             const double z0 = a + 3 < argc ? std::atof(argv[a + 3]) : 50.0;
             std::printf("%s\n", jnum(vswr(std::atof(argv[a + 1]), std::atof(argv[a + 2]), z0)).c_str());

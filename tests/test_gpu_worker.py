@@ -152,3 +152,25 @@ def test_worker_admission_and_serial_order():
     out = serve(lines, "--serial")
     assert [o["task_id"] for o in out] == ["s0", "s1", "s2"]
     assert all(o["status"] == "ok" for o in out)
+
+
+def test_worker_result_cache():
+    """--cache N: a repeated task (same deck, solver config, frequency and
+    reply options) is answered from the cache under its own task_id, equal to
+    the first reply otherwise; a different frequency or option is solved;
+    at capacity the oldest entry is evicted first (result_cache.rs:76-96)."""
+    stream = [r for r in W.config5_stream(count=200) if r["n"] <= 600][:3]
+    a, b, c = stream
+    lines = [msg("a1", a["deck"], a["freq_hz"]), msg("b1", b["deck"], b["freq_hz"]),
+             msg("a2", a["deck"], a["freq_hz"]),
+             msg("a3", a["deck"], a["freq_hz"], return_currents=True),
+             msg("c1", c["deck"], c["freq_hz"]), msg("a4", a["deck"], a["freq_hz"])]
+    out = serve(lines, "--serial", "--cache", "2")
+    assert [o["task_id"] for o in out] == ["a1", "b1", "a2", "a3", "c1", "a4"]
+    by = {o["task_id"]: o for o in out}
+    assert by["a2"].pop("cached") is True and "cached" not in by["a1"]
+    assert {k: v for k, v in by["a2"].items() if k != "task_id"} == \
+        {k: v for k, v in by["a1"].items() if k != "task_id"}
+    assert "cached" not in by["a3"] and "currents" in by["a3"]  # reply options are part of the key
+    # capacity 2: a1 (b1, a3 and c1 inserted after it) is evicted by the time of a4
+    assert "cached" not in by["a4"] and by["a4"]["impedance"] == by["a1"]["impedance"]

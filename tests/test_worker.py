@@ -115,3 +115,23 @@ def test_deck_level_errors():
 def test_bad_rp_extension():
     out = run(task("rp", DIPOLE, rp={"theta_deg": [0.0, 90.0], "phi_deg": [0.0]}) + "\n")
     assert out[0]["task_id"] == "rp" and out[0]["error_code"] == "parse_error"
+
+
+def test_cache_key_is_the_references():
+    """--cache-key: SHA-256 of deck \\0 basis \\0 ground_model \\0 the
+    frequency's IEEE bits as 16 hex digits (result_cache.rs:11-22)."""
+    import hashlib
+    import struct
+    deck = "CE\nGW 1 51 0 0 -5.282 0 0 5.282 0.001\nGE\nEX 0 1 26 0 1 0\nFR 0 1 0 0 14.2 0\nEN\n"
+    for basis, ground, f in (("hallen", "none", 14.2e6), ("hallen", "perfect", 7.05e6), ("sinusoidal", "none", 3e6)):
+        p = subprocess.run([BIN, "--cache-key", basis, ground, repr(f)], input=deck, capture_output=True, text=True,
+                           timeout=30)
+        bits = struct.unpack("<Q", struct.pack("<d", f))[0]
+        want = hashlib.sha256(deck.encode() + b"\0" + basis.encode() + b"\0" + ground.encode() + b"\0" +
+                              f"{bits:016x}".encode()).hexdigest()
+        assert p.returncode == 0 and p.stdout.strip() == want
+    # the sha256 of an empty message and of a multi-block one
+    for text in ("", "x" * 1000):
+        p = subprocess.run([BIN, "--cache-key", "", "", "0"], input=text, capture_output=True, text=True, timeout=30)
+        want = hashlib.sha256(text.encode() + b"\0\0\0" + b"0" * 16).hexdigest()
+        assert p.stdout.strip() == want
