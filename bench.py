@@ -413,6 +413,8 @@ def main():
     ap.add_argument("--update-ring", type=int, choices=[0, 1], default=None,
                     help="K = 128 passes stream their operands through an LDS ring (1) or load them into "
                          "registers (0, the default)")
+    ap.add_argument("--set", action="append", default=[], metavar="NAME=VALUE",
+                    help="set an engine tunable after autotuning (repeatable)")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the engine's default knobs (no autotune on a window of the stream)")
     ap.add_argument("--depth", type=int, default=None,
@@ -459,6 +461,9 @@ def main():
                  "start_window_ms": round(at["start_seconds"] * 1e3, 2), "trials": len(at["trials"]),
                  "tuning_s": round(time.perf_counter() - t_at, 2)}
         print(f"[bench] autotune: {tuned}", file=sys.stderr, flush=True)
+    for kv in args.set:
+        name, value = kv.split("=", 1)
+        F.set_tunable(name, int(value))
     if args.arrival == "poisson":
         # open loop: the stream's Poisson arrival times, rescaled to the rate
         first = args.warmup * args.window
