@@ -197,6 +197,12 @@ __global__ __launch_bounds__(256) void k_fill_exact(const Prob *__restrict__ pro
 constexpr int kXT = 64;  // output tile
 constexpr int kXK = 16;  // rows of M per LDS chunk
 
+// Imaginary part of the mirror element G[j][i] = conj(G[i][j]): the
+// reference sums G[j][i] on its own, and its products are those of G[i][j]
+// with the imaginary parts negated, so the sum is -Im G[i][j] bit for bit,
+// except that an exactly zero sum is +0 both ways (round to nearest).
+__device__ __forceinline__ double mirror_im(double x) { return x == 0.0 ? 0.0 : -x; }
+
 // conj(a) b as num-complex forms it: (ar br - (-ai) bi, ar bi + (-ai) br),
 // which is (ar br + ai bi, ar bi - ai br) bit for bit.
 __device__ __forceinline__ void cjmul(double ar, double ai, double br, double bi, double &pr, double &pi)
@@ -279,7 +285,7 @@ __global__ __launch_bounds__(256) void k_gx_tile(const Prob *__restrict__ probs,
             P.ai[(size_t)j * lda + i] = gi[u][v];
             if (i != j) {
                 P.ar[(size_t)i * lda + j] = gr[u][v];
-                P.ai[(size_t)i * lda + j] = -gi[u][v];
+                P.ai[(size_t)i * lda + j] = mirror_im(gi[u][v]);
             }
         }
 }
@@ -358,7 +364,7 @@ __global__ __launch_bounds__(256) void k_gx_cols(const Prob *__restrict__ probs,
         P.ar[(size_t)j * lda + cw] = gr; // G[n+w][j]
         P.ai[(size_t)j * lda + cw] = gi;
         P.ar[(size_t)cw * lda + j] = gr; // G[j][n+w]
-        P.ai[(size_t)cw * lda + j] = -gi;
+        P.ai[(size_t)cw * lda + j] = mirror_im(gi);
     }
 }
 
@@ -469,6 +475,14 @@ __global__ __launch_bounds__(256) void k_gx_extra(const Prob *__restrict__ probs
     }
     const double *y = k >= 0 ? P.rhs + 2 * (size_t)k * n : nullptr;
     double ar = 0.0, ai = 0.0;
+    if (k < 0 && f != e && !P.rowent) {
+        // delta-gap rows have one extra entry: distinct columns share no row
+        P.ar[(size_t)(n + f) * lda + n + e] = 0.0;
+        P.ai[(size_t)(n + f) * lda + n + e] = 0.0;
+        P.ar[(size_t)(n + e) * lda + n + f] = 0.0;
+        P.ai[(size_t)(n + e) * lda + n + f] = 0.0;
+        return;
+    }
     auto row = [&](int kind, int r) {
         double vr, vi, pr, pi;
         m_extra(P, kind, r, vr, vi);
@@ -511,7 +525,7 @@ __global__ __launch_bounds__(256) void k_gx_extra(const Prob *__restrict__ probs
     P.ai[(size_t)(n + f) * lda + n + e] = ai;
     if (f != e) {
         P.ar[(size_t)(n + e) * lda + n + f] = ar;
-        P.ai[(size_t)(n + e) * lda + n + f] = -ai;
+        P.ai[(size_t)(n + e) * lda + n + f] = mirror_im(ai);
     }
 }
 
@@ -556,7 +570,7 @@ __global__ __launch_bounds__(64) void k_gx_fix(const Prob *__restrict__ probs,
                 P.ai[o] = P.ai[o] + pi;
                 if (p != q) {
                     P.ar[(size_t)p * lda + q] = P.ar[o];
-                    P.ai[(size_t)p * lda + q] = -P.ai[o];
+                    P.ai[(size_t)p * lda + q] = mirror_im(P.ai[o]);
                 }
             }
         if (P.cons_y) {
