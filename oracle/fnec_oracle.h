@@ -130,6 +130,9 @@ int or_deck_stamps(const char *text, int semantics, double freq_hz, int *rows, i
 /* Threads for the fill, Gram and elimination row loops (default 1, as the
  * reference runs; results do not depend on it). Fixture generation only. */
 void or_set_threads(int n);
+/* Cache-blocked Gram and elimination loops (default 0): the same operations
+ * per element in the same order, bit for bit; for fixtures of large systems. */
+void or_set_blocked(int on);
 int or_solve_square(const cx *a, const cx *b, int n, cx *x);
 /* the same, recording the pivot row chosen at each elimination column */
 int or_solve_square_piv(const cx *a, const cx *b, int n, cx *x, int *piv);

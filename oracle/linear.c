@@ -22,6 +22,20 @@
 int or_threads = 1;
 void or_set_threads(int n) { or_threads = n > 0 ? n : 1; }
 
+/* Cache-blocked loop nests for the Gram matrix and the elimination (0 by
+ * default: the reference's loop order, which the CPU baseline times). Every
+ * element still receives the same operations in the same order, so the
+ * results are bit for bit those of the reference-order loops; only the order
+ * in which different elements are visited changes. For fixtures of large
+ * systems, whose reference-order column walks over M and full sweeps of the
+ * trailing matrix per column do not fit in any cache. */
+int or_blocked = 0;
+void or_set_blocked(int on) { or_blocked = on != 0; }
+
+#define OR_RB 128 /* rows of M per block (Gram) */
+#define OR_TB 256 /* Gram output tile */
+#define OR_PB 64  /* elimination panel width */
+
 static cx **rows_alloc(int r, int c)
 {
     cx **m = (cx **)malloc(sizeof(cx *) * (size_t)(r > 0 ? r : 1));
@@ -87,7 +101,104 @@ static int solve_aug_piv(cx **aug, int n, cx *x, int *pivots)
     return OR_OK;
 }
 
-static int solve_aug(cx **aug, int n, cx *x) { return solve_aug_piv(aug, n, x, NULL); }
+/* solve_aug_piv with the elimination in panels of OR_PB columns: inside a
+ * panel, column col is searched and its rows below are eliminated on the
+ * panel's columns only, with the factors kept; then every row below the
+ * panel's first row takes, on the columns right of the panel (the right-hand
+ * side included), the panel's updates a -= f u in column order. Row pointers
+ * are swapped whole, as the reference swaps its row vectors, so each element
+ * sees the reference's sequence of values. */
+static int solve_aug_piv_blocked(cx **aug, int n, cx *x, int *pivots)
+{
+    int p, i, j;
+    cx *fac = (cx *)malloc(sizeof(cx) * (size_t)(n > 0 ? n : 1) * OR_PB);
+    for (p = 0; p < n; p += OR_PB) {
+        const int pe = p + OR_PB < n ? p + OR_PB : n;
+        int col, row;
+        for (col = p; col < pe; col++) {
+            int piv = col;
+            double best = cx_norm(aug[col][col]);
+            cx diag;
+            for (row = col + 1; row < n; row++) {
+                double v = cx_norm(aug[row][col]);
+                if (!(v < best)) {
+                    best = v;
+                    piv = row;
+                }
+            }
+            if (pivots)
+                pivots[col] = piv;
+            if (cx_norm(aug[piv][col]) < 1e-30) {
+                free(fac);
+                return OR_SINGULAR;
+            }
+            if (piv != col) {
+                cx *t = aug[col];
+                aug[col] = aug[piv];
+                aug[piv] = t;
+                /* the factors of earlier panel columns move with their rows */
+                for (j = 0; j < col - p; j++) {
+                    cx f = fac[(size_t)col * OR_PB + j];
+                    fac[(size_t)col * OR_PB + j] = fac[(size_t)piv * OR_PB + j];
+                    fac[(size_t)piv * OR_PB + j] = f;
+                }
+            }
+            diag = aug[col][col];
+#pragma omp parallel for num_threads(or_threads) schedule(static) if (or_threads > 1 && n - col > 256)
+            for (row = col + 1; row < n; row++) {
+                cx factor = cx_div(aug[row][col], diag);
+                cx *ar = aug[row], *ac = aug[col];
+                int kk;
+                fac[(size_t)row * OR_PB + (col - p)] = factor;
+                for (kk = col; kk < pe; kk++) {
+                    cx sub = cx_mul(factor, ac[kk]);
+                    ar[kk] = cx_sub(ar[kk], sub);
+                }
+            }
+        }
+        /* columns right of the panel: row r takes the updates of the panel
+         * columns below it, in order; a pivot row is final before any row
+         * below it reads it (rows in ascending order within the panel) */
+        for (row = p + 1; row < pe; row++) {
+            const int ce = row < pe ? row : pe;
+            for (col = p; col < ce; col++) {
+                const cx factor = fac[(size_t)row * OR_PB + (col - p)];
+                cx *ar = aug[row], *ac = aug[col];
+                int kk;
+                for (kk = pe; kk <= n; kk++) {
+                    cx sub = cx_mul(factor, ac[kk]);
+                    ar[kk] = cx_sub(ar[kk], sub);
+                }
+            }
+        }
+#pragma omp parallel for num_threads(or_threads) schedule(dynamic, 16) if (or_threads > 1)
+        for (row = pe; row < n; row++) {
+            cx *ar = aug[row];
+            int c2, kk;
+            for (c2 = p; c2 < pe; c2++) {
+                const cx factor = fac[(size_t)row * OR_PB + (c2 - p)];
+                const cx *ac = aug[c2];
+                for (kk = pe; kk <= n; kk++) {
+                    cx sub = cx_mul(factor, ac[kk]);
+                    ar[kk] = cx_sub(ar[kk], sub);
+                }
+            }
+        }
+    }
+    free(fac);
+    for (i = n - 1; i >= 0; i--) {
+        cx sum = aug[i][n];
+        for (j = i + 1; j < n; j++)
+            sum = cx_sub(sum, cx_mul(aug[i][j], x[j]));
+        x[i] = cx_div(sum, aug[i][i]);
+    }
+    return OR_OK;
+}
+
+static int solve_aug(cx **aug, int n, cx *x)
+{
+    return or_blocked ? solve_aug_piv_blocked(aug, n, x, NULL) : solve_aug_piv(aug, n, x, NULL);
+}
 
 /* Iterator::max_by with partial_cmp(...).unwrap_or(Equal): the fold keeps
  * the later element unless the earlier one is strictly greater. The loop
@@ -102,7 +213,7 @@ int or_solve_square_piv(const cx *a, const cx *b, int n, cx *x, int *piv)
         memcpy(aug[i], a + (size_t)i * (size_t)n, sizeof(cx) * (size_t)n);
         aug[i][n] = b[i];
     }
-    st = solve_aug_piv(aug, n, x, piv);
+    st = or_blocked ? solve_aug_piv_blocked(aug, n, x, piv) : solve_aug_piv(aug, n, x, piv);
     rows_free(aug, n);
     return st;
 }
@@ -171,11 +282,58 @@ static int build_m(const cx *z, int n, const double *cosv, const cx *rhs, const 
     return OR_OK;
 }
 
+/* The Gram part of normal_eqs below, cache-blocked: output tiles of OR_TB x
+ * OR_TB accumulate the rows of M in blocks of OR_RB, blocks in ascending
+ * order, so every ata[i][j] is the same sequential sum over ascending r. */
+static void gram_blocked(cx **m, int rows, int cols, cx **ata)
+{
+    const int nt = (cols + OR_TB - 1) / OR_TB;
+    int r0, i;
+    for (i = 0; i < cols; i++) {
+        int j;
+        for (j = 0; j < cols; j++)
+            ata[i][j] = cx_make(0.0, 0.0);
+    }
+    for (r0 = 0; r0 < rows; r0 += OR_RB) {
+        const int r1 = r0 + OR_RB < rows ? r0 + OR_RB : rows;
+        long t;
+#pragma omp parallel for num_threads(or_threads) schedule(dynamic, 1) if (or_threads > 1)
+        for (t = 0; t < (long)nt * nt; t++) {
+            const int i0 = (int)(t / nt) * OR_TB, j0 = (int)(t % nt) * OR_TB;
+            const int i1 = i0 + OR_TB < cols ? i0 + OR_TB : cols, j1 = j0 + OR_TB < cols ? j0 + OR_TB : cols;
+            int r, ii, jj;
+            for (r = r0; r < r1; r++) {
+                const cx *mr = m[r];
+                for (ii = i0; ii < i1; ii++) {
+                    const cx a = cx_conj(mr[ii]);
+                    cx *out = ata[ii];
+                    for (jj = j0; jj < j1; jj++)
+                        out[jj] = cx_add(out[jj], cx_mul(a, mr[jj]));
+                }
+            }
+        }
+    }
+}
+
 /* Regularised normal equations (linear.rs:764-785). Returns aug rows (S x S+1). */
 static cx **normal_eqs(cx **m, const cx *y, int rows, int cols)
 {
     cx **ata = rows_alloc(cols, cols + 1);
     int i;
+    if (or_blocked) {
+        gram_blocked(m, rows, cols, ata);
+#pragma omp parallel for num_threads(or_threads) schedule(dynamic, 16) if (or_threads > 1)
+        for (i = 0; i < cols; i++) {
+            cx sum = cx_make(0.0, 0.0);
+            int r;
+            for (r = 0; r < rows; r++)
+                sum = cx_add(sum, cx_mul(cx_conj(m[r][i]), y[r]));
+            ata[i][cols] = sum;
+        }
+        for (i = 0; i < cols; i++)
+            ata[i][i] = cx_add(ata[i][i], cx_make(1e-8, 0.0));
+        return ata;
+    }
 #pragma omp parallel for num_threads(or_threads) schedule(dynamic, 4) if (or_threads > 1)
     for (i = 0; i < cols; i++) {
         cx sum;

@@ -104,6 +104,7 @@ def lib():
         L.or_fill_elem.restype = CX
         L.or_solve_square.argtypes = [dp, dp, C.c_int, dp]
         L.or_set_threads.argtypes = [C.c_int]
+        L.or_set_blocked.argtypes = [C.c_int]
         L.or_solve_square_piv.argtypes = [dp, dp, C.c_int, dp, dp]
         L.or_libm_eval.argtypes = [C.c_int, dp, dp, C.c_size_t, dp]
         L.or_solve_hallen.argtypes = [dp, C.c_int, dp, dp, dp, dp, C.c_int, dp, dp, dp, C.c_int, dp]
@@ -197,6 +198,12 @@ def set_threads(n: int) -> None:
     """Threads for the fill, Gram and elimination row loops (fixtures only;
     results do not depend on the count)."""
     lib().or_set_threads(int(n))
+
+
+def set_blocked(on: bool) -> None:
+    """Cache-blocked Gram and elimination loops (fixtures of large systems;
+    results bit for bit those of the reference-order loops)."""
+    lib().or_set_blocked(int(bool(on)))
 
 
 def solve_square_piv(a: np.ndarray, b: np.ndarray) -> tuple[int, np.ndarray, np.ndarray]:

@@ -41,11 +41,15 @@ def exact_solution(deck, freq_hz, semantics):
 
 
 def run(name, deck, freq_hz=0.0, semantics="cli", threads=1):
-    O.set_threads(threads)  # row loops only: the results do not depend on it
+    # row loops and cache-blocked loop nests: the results do not depend on
+    # either (each element takes the same operations in the same order)
+    O.set_threads(threads)
+    O.set_blocked(True)
     t0 = time.time()
     r = O.run_deck(deck, semantics, 0, freq_hz)
     dt = time.time() - t0
     O.set_threads(1)
+    O.set_blocked(False)
     kw = dict(x=r["x"], feed_z=r["feed_z"], feed_idx=np.array(r["feed_idx"]), abs_res=r["abs_res"],
               rel_res=r["rel_res"], diag_spread=r["diag_spread"], seconds=dt, freq_hz=r["freq_hz"],
               x_exact=exact_solution(deck, r["freq_hz"], semantics))

@@ -11,6 +11,7 @@ Checks, in order of strength:
   form of the Hertzian-dipole power integral.
 """
 import cmath
+import importlib
 import json
 import math
 
@@ -529,3 +530,29 @@ def test_radiation_efficiency_of_a_lossless_dipole(oracle):
     assert e == 1.0  # clamped: P_rad > P_in / 2
     e = oracle.radiation_efficiency(r["segs"], r["currents"], r["freq_hz"], p_in * 1.02)
     assert 0.97 < e < 1.0
+
+
+def test_blocked_loops_are_bitwise_the_reference_order(oracle):
+    """or_set_blocked (cache-blocked Gram and elimination, used for large
+    fixtures) gives the reference-order loops' results bit for bit: systems
+    that interchange in almost every column, panel edges, and a deck."""
+    rng = np.random.default_rng(1)
+    try:
+        for n in (5, 63, 64, 65, 130, 200):
+            a = rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))
+            a[:, :n // 3] *= 1e-3
+            b = rng.normal(size=n) + 1j * rng.normal(size=n)
+            oracle.set_blocked(False)
+            s0, x0, p0 = oracle.solve_square_piv(a, b)
+            oracle.set_blocked(True)
+            s1, x1, p1 = oracle.solve_square_piv(a, b)
+            assert s0 == s1 and np.array_equal(p0, p1) and np.array_equal(x0.view(np.uint64), x1.view(np.uint64))
+        W = importlib.import_module("fnec-rust_amd.workloads")
+        txt = W.config3_deck(nseg=12, nwires=12)
+        oracle.set_blocked(False)
+        r0 = oracle.run_deck(txt, "worker", 0, 7e6)
+        oracle.set_blocked(True)
+        r1 = oracle.run_deck(txt, "worker", 0, 7e6)
+        assert np.array_equal(r0["x"].view(np.uint64), r1["x"].view(np.uint64))
+    finally:
+        oracle.set_blocked(False)
