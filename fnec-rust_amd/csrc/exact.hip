@@ -529,20 +529,18 @@ __global__ __launch_bounds__(256) void k_gx_extra(const Prob *__restrict__ probs
     }
 }
 
-// One thread per problem, after the kernels above: the constraint rows'
-// terms in row order (each continues its elements' sums of G and, where the
-// row's y is not zero, of M^H y), then lambda (linear.rs:782-785). A row's
+// One workgroup per problem, after the kernels above: the constraint rows'
+// terms in row order by one lane (each continues its elements' sums of G and,
+// where the row's y is not zero, of M^H y), then lambda on the diagonal
+// (linear.rs:782-785) by every lane. A row's
 // entries are (a, aval) and, for a junction, (b, sign); a zero y adds +-0,
 // which leaves every sum as it is, so such rows skip M^H y.
 __global__ __launch_bounds__(64) void k_gx_fix(const Prob *__restrict__ probs,
                                                const int *__restrict__ plist)
 {
     const Prob &P = probs[plist[blockIdx.x]];
-    if (threadIdx.x != 0)
-        return;
-    const int n = P.n, s = P.s, lda = P.lda;
-    (void)n;
-    for (int c = 0; c < P.ncons; ++c) {
+    const int s = P.s, lda = P.lda;
+    for (int c = 0; c < P.ncons && threadIdx.x == 0; ++c) {
         int idx[2];
         double val[2];
         int ne = 1;
@@ -586,7 +584,8 @@ __global__ __launch_bounds__(64) void k_gx_fix(const Prob *__restrict__ probs,
                     }
         }
     }
-    for (int i = 0; i < s; ++i) {
+    __syncthreads();
+    for (int i = threadIdx.x; i < s; i += 64) {
         const size_t o = (size_t)i * lda + i;
         P.ar[o] = P.ar[o] + kLambda;
         P.ai[o] = P.ai[o] + 0.0;
