@@ -12,6 +12,7 @@
 #include <cstring>
 #include <map>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/fnec_hip.h"
@@ -703,6 +704,26 @@ fnec_c64 coupling(const Wave &w, const double d[3])
 // sin(k |s_m - s_p|) does not depend on the wave: each row's values are
 // computed once and serve every wave (the receive sweep's incidences), in the
 // reference's operation order for each.
+// f(begin, end) over contiguous ranges of [0, count) on up to 16 host
+// threads when the loop is large (work: its elementary operations); every item
+// is computed by one thread with the operations of the serial loop.
+template <class F> void parallel_ranges(size_t count, double work, F f)
+{
+    unsigned nt = std::thread::hardware_concurrency();
+    nt = std::min(nt ? nt : 1u, 16u);
+    if (work < 4e6 || nt < 2 || count < 2) {
+        f((size_t)0, count);
+        return;
+    }
+    const size_t per = (count + nt - 1) / nt;
+    std::vector<std::thread> th;
+    for (size_t b = per; b < count; b += per)
+        th.emplace_back(f, b, std::min(count, b + per));
+    f((size_t)0, std::min(count, per));
+    for (auto &t : th)
+        t.join();
+}
+
 void planewave_rhs(const std::vector<Wave> &waves, const std::vector<Seg> &s, double freq,
                    const std::vector<std::pair<size_t, size_t>> &wires, const Paths *P, fnec_c64 *rhs,
                    double *cosv, double *sinv)
@@ -730,55 +751,65 @@ void planewave_rhs(const std::vector<Wave> &waves, const std::vector<Seg> &s, do
         }
     }
     size_t ngrp = 0;
-    for (size_t i = 0; i < n; ++i) {
+    for (size_t i = 0; i < n; ++i)
         ngrp = std::max(ngrp, grp[i] + 1);
-        const double *dir = P ? s[i].dir : wdir[grp[i]];
-        if (!P) {
-            const double d[3] = {s[i].mid[0] - wmid[grp[i]][0], s[i].mid[1] - wmid[grp[i]][1],
-                                 s[i].mid[2] - wmid[grp[i]][2]};
-            sc[i] = d[0] * dir[0] + d[1] * dir[1] + d[2] * dir[2];
+    // the incident field along each segment, for every wave
+    parallel_ranges(n, 60.0 * n * nq, [&](size_t i0, size_t i1) {
+        for (size_t i = i0; i < i1; ++i) {
+            const double *dir = P ? s[i].dir : wdir[grp[i]];
+            if (!P) {
+                const double d[3] = {s[i].mid[0] - wmid[grp[i]][0], s[i].mid[1] - wmid[grp[i]][1],
+                                     s[i].mid[2] - wmid[grp[i]][2]};
+                sc[i] = d[0] * dir[0] + d[1] * dir[1] + d[2] * dir[2];
+            }
+            for (size_t q = 0; q < nq; ++q) {
+                const Wave &wv = waves[q];
+                const double phase = k * (wv.r_hat[0] * s[i].mid[0] + wv.r_hat[1] * s[i].mid[1] +
+                                          wv.r_hat[2] * s[i].mid[2]);
+                double sp, cp;
+                sincos(phase, &sp, &cp);
+                fnec_c64 e = cmul(coupling(wv, dir), fnec_c64{1.0 * cp, 1.0 * sp});
+                if (P)
+                    e = fnec_c64{e.re * sg[i], e.im * sg[i]};
+                et[q * n + i] = e;
+            }
         }
-        for (size_t q = 0; q < nq; ++q) {
-            const Wave &wv = waves[q];
-            const double phase = k * (wv.r_hat[0] * s[i].mid[0] + wv.r_hat[1] * s[i].mid[1] +
-                                      wv.r_hat[2] * s[i].mid[2]);
-            double sp, cp;
-            sincos(phase, &sp, &cp);
-            fnec_c64 e = cmul(coupling(wv, dir), fnec_c64{1.0 * cp, 1.0 * sp});
-            if (P)
-                e = fnec_c64{e.re * sg[i], e.im * sg[i]};
-            et[q * n + i] = e;
-        }
-    }
+    });
     // members of each wire or path in ascending order (the reference's p loop)
     std::vector<std::vector<size_t>> members(ngrp);
+    size_t summed = 0;
     for (size_t p = 0; p < n; ++p)
         members[grp[p]].push_back(p);
-    std::vector<double> kern;
-    for (size_t m = 0; m < n; ++m) {
-        const std::vector<size_t> &mem = members[grp[m]];
-        kern.resize(mem.size());
-        for (size_t j = 0; j < mem.size(); ++j)
-            kern[j] = std::sin(k * std::fabs(sc[m] - sc[mem[j]]));
-        for (size_t q = 0; q < nq; ++q) {
-            const fnec_c64 *e = et.data() + q * n;
-            fnec_c64 acc{0.0, 0.0};
-            for (size_t j = 0; j < mem.size(); ++j) {
-                const size_t p = mem[j];
-                const double tr = e[p].re * kern[j] * s[p].length, ti = e[p].im * kern[j] * s[p].length;
-                acc.re = acc.re + tr;
-                acc.im = acc.im + ti;
+    for (size_t m = 0; m < n; ++m)
+        summed += members[grp[m]].size();
+    // each row's sums over its wire or path, for every wave
+    parallel_ranges(n, 6.0 * (double)summed * nq, [&](size_t m0, size_t m1) {
+        std::vector<double> kern;
+        for (size_t m = m0; m < m1; ++m) {
+            const std::vector<size_t> &mem = members[grp[m]];
+            kern.resize(mem.size());
+            for (size_t j = 0; j < mem.size(); ++j)
+                kern[j] = std::sin(k * std::fabs(sc[m] - sc[mem[j]]));
+            for (size_t q = 0; q < nq; ++q) {
+                const fnec_c64 *e = et.data() + q * n;
+                fnec_c64 acc{0.0, 0.0};
+                for (size_t j = 0; j < mem.size(); ++j) {
+                    const size_t p = mem[j];
+                    const double tr = e[p].re * kern[j] * s[p].length, ti = e[p].im * kern[j] * s[p].length;
+                    acc.re = acc.re + tr;
+                    acc.im = acc.im + ti;
+                }
+                fnec_c64 r = cmul(fnec_c64{0.0, -scale}, acc);
+                if (P)
+                    r = fnec_c64{r.re * sg[m], r.im * sg[m]};
+                rhs[q * n + m] = r;
             }
-            fnec_c64 r = cmul(fnec_c64{0.0, -scale}, acc);
-            if (P)
-                r = fnec_c64{r.re * sg[m], r.im * sg[m]};
-            rhs[q * n + m] = r;
+            double sv, cv;
+            sincos(k * sc[m], &sv, &cv);
+            cosv[m] = P ? sg[m] * cv : cv;
+            sinv[m] = P ? sg[m] * sv : sv;
         }
-        double sv, cv;
-        sincos(k * sc[m], &sv, &cv);
-        cosv[m] = P ? sg[m] * cv : cv;
-        sinv[m] = P ? sg[m] * sv : sv;
-    }
+    });
 }
 
 // build_current_source_shape(_paths) (excitation.rs:350-447): the delta-gap

@@ -220,6 +220,23 @@ def test_receive_sweep_inputs_equal_the_oracle_per_incidence(oracle):
     assert p["rhs_all"][0].tobytes() == p["rhs"].tobytes()
 
 
+def test_large_receive_sweep_inputs_equal_the_oracle(oracle):
+    """A sweep large enough that the host forms its forcing on several threads
+    (4 wires x 101 segments, 9 x 17 incidences): sampled incidences, and the
+    cos / sin columns, equal the oracle's single-incidence inputs bit for bit."""
+    geom = "\n".join(f"GW {w + 1} 101 {1.7 * w!r} 0.3 -5 {1.7 * w + 0.2!r} -0.3 5 0.001" for w in range(4))
+    txt = _vdeck(geom, "EX 1 9 17 0 5 0 0 20 21")
+    p = F.Deck(txt).problem("cli", 14.2e6)
+    assert p["rhs_all"].shape == (154, 404)
+    th, ph = [float(t) for t in p["recv_theta"]], [float(f) for f in p["recv_phi"]]
+    for q in (0, 1, 16, 17, 76, 100, 151, 152):
+        one = _vdeck(geom, f"EX 1 1 1 0 {th[q]!r} {ph[q]!r} 0")
+        v = oracle.variant_inputs(one, 14.2e6)
+        assert p["rhs_all"][q + 1].tobytes() == v["rhs"].tobytes(), q
+        if q == 0:
+            assert p["cos"].tobytes() == v["cos"].tobytes() and p["sin"].tobytes() == v["sin"].tobytes()
+
+
 def test_variant_topology_errors():
     # a degree-3 junction (no conductor paths): neither a plane wave nor a
     # current source is solvable, as the oracle also refuses
