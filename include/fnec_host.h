@@ -88,6 +88,9 @@ typedef struct {
     const unsigned char *feed_port;
 } fnec_deck_problem;
 
+/* A large receive sweep (4M or more elementary operations in its forcing)
+ * is formed on up to 16 short-lived host threads, joined before the call
+ * returns; the values are those of the one-thread loop. */
 int fnec_deck_build(fnec_deck *d, int semantics, double freq_hz, fnec_deck_problem *out);
 
 /* Solve one built problem on the GPU: route 0 or 1 as fnec_hip_solve_hallen or
